@@ -1,0 +1,62 @@
+"""ctypes prototypes of the C ABI in include/ebiv_gpu.h (libebivgpu.so)."""
+from __future__ import annotations
+
+import ctypes as C
+
+OK = 0
+ERR_CORRUPT = 1
+ERR_UNSUPPORTED = 2
+ERR_NOKEYFRAME = 3
+ERR_DIMENSIONS = 4
+ERR_ARGUMENT = 16
+ERR_DEVICE = 17
+
+# Every symbol include/ebiv_gpu.h declares; tests check the library exports them.
+EXPORTS = (
+    "ebiv_codec_create", "ebiv_codec_decode", "ebiv_codec_decode_device", "ebiv_codec_flush",
+    "ebiv_codec_destroy", "ebiv_codec_last_error", "ebiv_codec_stats", "ebiv_frame_rgba",
+    "ebiv_device_alloc", "ebiv_device_free", "ebiv_memcpy_h2d", "ebiv_memcpy_d2h", "ebiv_device_count",
+)
+
+
+class Frame(C.Structure):
+    _fields_ = [("y", C.c_void_p), ("cb", C.c_void_p), ("cr", C.c_void_p),
+                ("y_stride", C.c_int32), ("c_stride", C.c_int32),
+                ("width", C.c_int32), ("height", C.c_int32),
+                ("cwidth", C.c_int32), ("cheight", C.c_int32),
+                ("timestamp_ns", C.c_int64), ("on_device", C.c_int32), ("ring_slot", C.c_int32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("frames", C.c_uint64), ("symbols", C.c_uint64), ("tiles", C.c_uint64)]
+
+
+def bind(L):
+    vp, sz = C.c_void_p, C.c_size_t
+    L.ebiv_codec_create.restype = C.c_int
+    L.ebiv_codec_create.argtypes = [C.c_int, C.c_int, C.POINTER(vp)]
+    L.ebiv_codec_decode.restype = C.c_int
+    L.ebiv_codec_decode.argtypes = [vp, C.c_char_p, sz, C.c_int64, C.POINTER(Frame)]
+    L.ebiv_codec_decode_device.restype = C.c_int
+    L.ebiv_codec_decode_device.argtypes = [vp, C.c_char_p, vp, sz, C.c_int64, C.POINTER(Frame)]
+    L.ebiv_codec_flush.restype = None
+    L.ebiv_codec_flush.argtypes = [vp]
+    L.ebiv_codec_destroy.restype = None
+    L.ebiv_codec_destroy.argtypes = [vp]
+    L.ebiv_codec_last_error.restype = C.c_char_p
+    L.ebiv_codec_last_error.argtypes = [vp]
+    L.ebiv_codec_stats.restype = C.c_int
+    L.ebiv_codec_stats.argtypes = [vp, C.POINTER(Stats)]
+    L.ebiv_frame_rgba.restype = C.c_int
+    L.ebiv_frame_rgba.argtypes = [vp, C.POINTER(Frame), vp, C.c_int]
+    L.ebiv_device_alloc.restype = C.c_int
+    L.ebiv_device_alloc.argtypes = [C.c_int, sz, C.POINTER(vp)]
+    L.ebiv_device_free.restype = C.c_int
+    L.ebiv_device_free.argtypes = [vp]
+    L.ebiv_memcpy_h2d.restype = C.c_int
+    L.ebiv_memcpy_h2d.argtypes = [vp, vp, sz]
+    L.ebiv_memcpy_d2h.restype = C.c_int
+    L.ebiv_memcpy_d2h.argtypes = [vp, vp, sz]
+    L.ebiv_device_count.restype = C.c_int
+    L.ebiv_device_count.argtypes = []
+    return L

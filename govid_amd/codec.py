@@ -1,0 +1,201 @@
+"""Host mirror of govid's Codec interface over the MI355X decoder.
+
+govid.Codec (govid.go:41-44) is ``Decode(Packet) (*Frame, error)`` plus
+``Flush()``; ebiv.NewCodec(WithFrameRing(n)) builds one (ebiv/codec.go:45-64).
+This module keeps those names and meanings: ``Codec.decode(Packet) -> Frame``
+raises the Go sentinel errors as exceptions, ``flush()`` resets only the ring
+position, and a returned Frame aliases a ring buffer that stays valid for
+``ring`` further decodes (ebiv/codec.go:15-22).
+
+Decoding always runs on the GPU through libebivgpu.so; there is no other path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import capi, native
+
+
+class EbivError(Exception):
+    """Base of the ebiv sentinel errors (ebiv/format.go:110-119)."""
+
+
+class ErrCorrupt(EbivError):
+    pass
+
+
+class ErrUnsupported(EbivError):
+    pass
+
+
+class ErrNoKeyframe(EbivError):
+    pass
+
+
+class ErrDimensions(EbivError):
+    pass
+
+
+class DeviceError(RuntimeError):
+    pass
+
+
+_ERRORS = {capi.ERR_CORRUPT: ErrCorrupt, capi.ERR_UNSUPPORTED: ErrUnsupported,
+           capi.ERR_NOKEYFRAME: ErrNoKeyframe, capi.ERR_DIMENSIONS: ErrDimensions}
+
+
+@dataclass
+class Packet:
+    """govid.Packet (govid.go:9-13); timestamp in nanoseconds."""
+    data: bytes
+    timestamp: int = 0
+    keyframe: bool = False
+
+
+class Frame:
+    """govid.Frame (govid.go:16-22) with YCbCr planes as numpy views.
+
+    Host frames view pinned ring memory; device frames carry HBM pointers and
+    are read back with ``planes()``.
+    """
+
+    def __init__(self, codec: "Codec", raw: capi.Frame):
+        self._codec = codec
+        self._raw = raw
+        self.width = raw.width
+        self.height = raw.height
+        self.timestamp = raw.timestamp_ns
+        self.on_device = bool(raw.on_device)
+
+    def _host_plane(self, ptr, stride, w, h):
+        buf = (C.c_uint8 * (stride * h)).from_address(ptr)
+        return np.ctypeslib.as_array(buf).reshape(h, stride)[:, :w]
+
+    def _device_plane(self, ptr, stride, w, h):
+        out = np.empty((h, stride), np.uint8)
+        rc = self._codec._L.ebiv_memcpy_d2h(out.ctypes.data, ptr, stride * h)
+        if rc:
+            raise DeviceError("plane download failed")
+        return out[:, :w]
+
+    @property
+    def Y(self):
+        r = self._raw
+        f = self._device_plane if self.on_device else self._host_plane
+        return f(r.y, r.y_stride, r.width, r.height)
+
+    @property
+    def Cb(self):
+        r = self._raw
+        f = self._device_plane if self.on_device else self._host_plane
+        return f(r.cb, r.c_stride, r.cwidth, r.cheight)
+
+    @property
+    def Cr(self):
+        r = self._raw
+        f = self._device_plane if self.on_device else self._host_plane
+        return f(r.cr, r.c_stride, r.cwidth, r.cheight)
+
+    def planes(self):
+        """Packed copies of (Y, Cb, Cr)."""
+        return self.Y.copy(), self.Cb.copy(), self.Cr.copy()
+
+    def rgba(self) -> np.ndarray:
+        """Frame.RGBA (frame.go:85-91), converted on the GPU, as an (h, w, 4) array."""
+        out = np.empty((self.height, self.width, 4), np.uint8)
+        rc = self._codec._L.ebiv_frame_rgba(self._codec._h, C.byref(self._raw), out.ctypes.data, 0)
+        if rc:
+            raise DeviceError(f"rgba conversion failed ({rc})")
+        return out
+
+    def rgba_device(self, dst_ptr: int) -> None:
+        """Writes RGBA into device memory at dst_ptr (width*height*4 bytes)."""
+        rc = self._codec._L.ebiv_frame_rgba(self._codec._h, C.byref(self._raw), dst_ptr, 1)
+        if rc:
+            raise DeviceError(f"rgba conversion failed ({rc})")
+
+
+class Codec:
+    """ebiv.Codec on the GPU: ``Codec(ring=3).decode(Packet) -> Frame``."""
+
+    def __init__(self, ring: int = 3, device: int = 0):
+        self._L = native.gpu_lib()
+        h = C.c_void_p()
+        rc = self._L.ebiv_codec_create(ring, device, C.byref(h))
+        if rc:
+            raise DeviceError(f"ebiv_codec_create failed ({rc})")
+        self._h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.ebiv_codec_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc == capi.OK:
+            return
+        msg = self._L.ebiv_codec_last_error(self._h).decode()
+        if rc in _ERRORS:
+            raise _ERRORS[rc](msg)
+        raise DeviceError(f"{msg} ({rc})")
+
+    def decode(self, pkt) -> Frame:
+        """Codec.Decode (ebiv/codec.go:70-111); pkt is a Packet or bytes."""
+        if isinstance(pkt, (bytes, bytearray)):
+            pkt = Packet(bytes(pkt))
+        fr = capi.Frame()
+        rc = self._L.ebiv_codec_decode(self._h, bytes(pkt.data), len(pkt.data), int(pkt.timestamp), C.byref(fr))
+        self._check(rc)
+        return Frame(self, fr)
+
+    def decode_device(self, host_data: bytes, dev_ptr: int, timestamp: int = 0) -> Frame:
+        """Decode of a packet already in HBM; the frame stays in HBM."""
+        fr = capi.Frame()
+        rc = self._L.ebiv_codec_decode_device(self._h, host_data, dev_ptr, len(host_data), int(timestamp), C.byref(fr))
+        self._check(rc)
+        return Frame(self, fr)
+
+    def flush(self):
+        """Codec.Flush (ebiv/codec.go:115-117)."""
+        self._L.ebiv_codec_flush(self._h)
+
+    def stats(self) -> dict:
+        s = capi.Stats()
+        self._L.ebiv_codec_stats(self._h, C.byref(s))
+        return {"frames": s.frames, "symbols": s.symbols, "tiles": s.tiles}
+
+
+class DeviceBuffer:
+    """A block of HBM holding bytes (packets kept resident for decode_device)."""
+
+    def __init__(self, data: bytes, device: int = 0):
+        self._L = native.gpu_lib()
+        p = C.c_void_p()
+        rc = self._L.ebiv_device_alloc(device, max(len(data), 1), C.byref(p))
+        if rc:
+            raise DeviceError("device allocation failed")
+        self.ptr = p.value
+        self.size = len(data)
+        if data:
+            src = C.create_string_buffer(bytes(data), len(data))
+            if self._L.ebiv_memcpy_h2d(self.ptr, src, len(data)):
+                raise DeviceError("upload failed")
+
+    def close(self):
+        if self.ptr:
+            self._L.ebiv_device_free(self.ptr)
+            self.ptr = 0
+
+    def __del__(self):
+        self.close()
+
+
+def device_count() -> int:
+    return int(native.gpu_lib().ebiv_device_count())

