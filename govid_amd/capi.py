@@ -16,6 +16,7 @@ EXPORTS = (
     "ebiv_codec_create", "ebiv_codec_decode", "ebiv_codec_decode_device", "ebiv_codec_flush",
     "ebiv_codec_destroy", "ebiv_codec_last_error", "ebiv_codec_stats", "ebiv_frame_rgba",
     "ebiv_device_alloc", "ebiv_device_free", "ebiv_memcpy_h2d", "ebiv_memcpy_d2h", "ebiv_device_count",
+    "ebiv_codec_set_timing", "ebiv_codec_kernel_times",
 )
 
 
@@ -25,6 +26,12 @@ class Frame(C.Structure):
                 ("width", C.c_int32), ("height", C.c_int32),
                 ("cwidth", C.c_int32), ("cheight", C.c_int32),
                 ("timestamp_ns", C.c_int64), ("on_device", C.c_int32), ("ring_slot", C.c_int32)]
+
+
+class KernelTimes(C.Structure):
+    _fields_ = [("parse_ms", C.c_double), ("inter_ms", C.c_double), ("intra_ms", C.c_double),
+                ("rgba_ms", C.c_double), ("parse_n", C.c_uint64), ("inter_n", C.c_uint64),
+                ("intra_n", C.c_uint64), ("rgba_n", C.c_uint64)]
 
 
 class Stats(C.Structure):
@@ -59,4 +66,8 @@ def bind(L):
     L.ebiv_memcpy_d2h.argtypes = [vp, vp, sz]
     L.ebiv_device_count.restype = C.c_int
     L.ebiv_device_count.argtypes = []
+    L.ebiv_codec_set_timing.restype = C.c_int
+    L.ebiv_codec_set_timing.argtypes = [vp, C.c_int]
+    L.ebiv_codec_kernel_times.restype = C.c_int
+    L.ebiv_codec_kernel_times.argtypes = [vp, C.POINTER(KernelTimes)]
     return L

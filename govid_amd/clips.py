@@ -74,15 +74,13 @@ def synth_pan(w: int, h: int, seed: int):
 # ---- colour conversion -------------------------------------------------------------------
 def rgb_to_ycbcr420(rgb: np.ndarray):
     """color.RGBToYCbCr fixed point per pixel, then 2x2-mean chroma."""
-    r = rgb[..., 0].astype(np.int64)
-    g = rgb[..., 1].astype(np.int64)
-    b = rgb[..., 2].astype(np.int64)
+    r = rgb[..., 0].astype(np.int32)
+    g = rgb[..., 1].astype(np.int32)
+    b = rgb[..., 2].astype(np.int32)
     yy = (19595 * r + 38470 * g + 7471 * b + (1 << 15)) >> 16
 
-    def fix(v):
-        out = v >> 16
-        bad = (v & 0xFF000000) != 0
-        return np.where(bad, np.where(v < 0, 0, 255), out)
+    def fix(v):  # uint32(v)&0xff000000 == 0 ? v>>16 : ^(v>>31)
+        return np.where(v < 0, 0, np.where(v >= (1 << 24), 255, v >> 16))
 
     cb = fix(-11056 * r - 21712 * g + 32768 * b + (257 << 15))
     cr = fix(32768 * r - 27440 * g - 5328 * b + (257 << 15))
@@ -98,25 +96,29 @@ def rgb_to_ycbcr420(rgb: np.ndarray):
 
 
 def _to_u8(x: np.ndarray) -> np.ndarray:
-    return np.clip(np.rint(x), 0, 255).astype(np.uint8)
+    return np.clip(np.rint(x), 0, 255, out=x if x.dtype == np.float32 else None).astype(np.uint8)
 
 
 # ---- gradient noise ------------------------------------------------------------------------
 def perlin(h: int, w: int, rng: np.random.Generator, cell: int = 64, octaves: int = 4) -> np.ndarray:
     """Fractal 2-D gradient noise in roughly [-1, 1], shape (h, w)."""
-    out = np.zeros((h, w), np.float64)
+    out = np.zeros((h, w), np.float32)
     amp, total = 1.0, 0.0
     for o in range(octaves):
         c = max(2, cell >> o)
         gh, gw = h // c + 2, w // c + 2
         ang = rng.uniform(0, 2 * np.pi, (gh, gw))
         gx, gy = np.cos(ang), np.sin(ang)
-        y, x = np.mgrid[0:h, 0:w].astype(np.float64) / c
-        y0, x0 = np.floor(y).astype(int), np.floor(x).astype(int)
-        fy, fx = y - y0, x - x0
+        ys = np.arange(h, dtype=np.float32) / c
+        xs = np.arange(w, dtype=np.float32) / c
+        y0 = np.floor(ys).astype(np.int32)[:, None]
+        x0 = np.floor(xs).astype(np.int32)[None, :]
+        fy = (ys[:, None] - y0).astype(np.float32)
+        fx = (xs[None, :] - x0).astype(np.float32)
+        gx32, gy32 = gx.astype(np.float32), gy.astype(np.float32)
 
         def dot(iy, ix, dy, dx):
-            return gx[iy, ix] * dx + gy[iy, ix] * dy
+            return gx32[iy, ix] * dx + gy32[iy, ix] * dy
 
         n00 = dot(y0, x0, fy, fx)
         n10 = dot(y0, x0 + 1, fy, fx - 1)
@@ -126,10 +128,10 @@ def perlin(h: int, w: int, rng: np.random.Generator, cell: int = 64, octaves: in
         sy = fy * fy * fy * (fy * (fy * 6 - 15) + 10)
         nx0 = n00 + sx * (n10 - n00)
         nx1 = n01 + sx * (n11 - n01)
-        out += amp * (nx0 + sy * (nx1 - nx0))
+        out += np.float32(amp) * (nx0 + sy * (nx1 - nx0))
         total += amp
         amp *= 0.5
-    return out / total * 1.4
+    return out * np.float32(1.4 / total)
 
 
 # ---- workloads ------------------------------------------------------------------------------
@@ -193,11 +195,11 @@ class ParallaxClip:
             base = self.rng.uniform(40, 215, 3)
             tint = self.rng.uniform(20, 60, 3)
             n = perlin(h, tw, self.rng, cell=96 >> i)
-            tex = base[None, None, :] + n[..., None] * tint[None, None, :] * 1.5
+            tex = (base[None, None, :] + n[..., None] * (tint * 1.5)[None, None, :]).astype(np.float32)
             mask = None
             if i > 0:
                 m = perlin(h, tw, self.rng, cell=128)
-                mask = (m > 0.15 * i).astype(np.float64)
+                mask = (m > 0.15 * i).astype(np.float32)
             self.layers.append((sp, tex, mask))
         self.sigma = noise_sigma
 
@@ -207,16 +209,16 @@ class ParallaxClip:
         for sp, tex, mask in self.layers:
             off = sp * t
             i0 = int(np.floor(off))
-            f = off - i0
-            layer = tex[:, i0:i0 + w] * (1 - f) + tex[:, i0 + 1:i0 + 1 + w] * f
+            f = np.float32(off - i0)
+            layer = tex[:, i0:i0 + w] if f == 0 else tex[:, i0:i0 + w] * (1 - f) + tex[:, i0 + 1:i0 + 1 + w] * f
             if mask is None:
-                img = layer
+                img = layer.copy()
             else:
-                m = mask[:, i0:i0 + w] * (1 - f) + mask[:, i0 + 1:i0 + 1 + w] * f
-                img = img * (1 - m[..., None]) + layer * m[..., None]
+                m = mask[:, i0:i0 + w] if f == 0 else mask[:, i0:i0 + w] * (1 - f) + mask[:, i0 + 1:i0 + 1 + w] * f
+                img += (layer - img) * m[..., None]
         if t % 24 in (12, 13):
-            img = img + 48.0
-        img = img + self.rng.normal(0, self.sigma, img.shape)
+            img += np.float32(48.0)
+        img += self.rng.standard_normal(img.shape, dtype=np.float32) * np.float32(self.sigma)
         return rgb_to_ycbcr420(_to_u8(img))
 
 

@@ -166,6 +166,15 @@ class Codec:
         """Codec.Flush (ebiv/codec.go:115-117)."""
         self._L.ebiv_codec_flush(self._h)
 
+    def set_timing(self, enabled: bool) -> None:
+        """Brackets every kernel launch with HIP events (measurement only)."""
+        self._L.ebiv_codec_set_timing(self._h, int(enabled))
+
+    def kernel_times(self) -> dict:
+        t = capi.KernelTimes()
+        self._L.ebiv_codec_kernel_times(self._h, C.byref(t))
+        return {f: getattr(t, f) for f, _ in capi.KernelTimes._fields_}
+
     def stats(self) -> dict:
         s = capi.Stats()
         self._L.ebiv_codec_stats(self._h, C.byref(s))

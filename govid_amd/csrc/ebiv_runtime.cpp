@@ -175,6 +175,11 @@ struct ebiv_codec {
   DevArray<uint32_t> d_rgba;
   std::string msg;
   ebiv_stats stats{0, 0, 0};
+  // Kernel timing: events bracket each launch while enabled.
+  bool timing = false;
+  hipEvent_t ev[2][4] = {};
+  int ev_used[2] = {0, 0};
+  ebiv_kernel_times ktimes{};
 
   int fail(int code, const std::string& m) {
     msg = m;
@@ -191,7 +196,26 @@ struct ebiv_codec {
       d_status[i].release(); h_status[i].release();
     }
     d_rgba.release();
+    for (auto& row : ev)
+      for (auto& e : row)
+        if (e) hipEventDestroy(e);
     if (stream) hipStreamDestroy(stream);
+  }
+
+  bool ensure_events() {
+    for (auto& row : ev)
+      for (auto& e : row)
+        if (!e && hipEventCreate(&e) != hipSuccess) return false;
+    return true;
+  }
+  // Adds the timed launches of slot `slot` (after the stream has synchronised).
+  void collect_times(int slot, bool inter) {
+    if (!timing || !ev_used[slot]) return;
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, ev[slot][0], ev[slot][1]) == hipSuccess) { ktimes.parse_ms += ms; ktimes.parse_n++; }
+    if (inter && hipEventElapsedTime(&ms, ev[slot][1], ev[slot][2]) == hipSuccess) { ktimes.inter_ms += ms; ktimes.inter_n++; }
+    if (hipEventElapsedTime(&ms, ev[slot][2], ev[slot][3]) == hipSuccess) { ktimes.intra_ms += ms; ktimes.intra_n++; }
+    ev_used[slot] = 0;
   }
 
   // configure, codec.go:129-153.
@@ -412,10 +436,18 @@ struct ebiv_codec {
              const Planes& g) {
     if (hipMemcpyAsync(d_desc[slot].d, h_desc[slot].h, job.desc_bytes, hipMemcpyHostToDevice, stream) != hipSuccess)
       return fail(EBIV_ERR_DEVICE, "descriptor upload failed");
+    const bool tm = timing && ensure_events();
+    if (tm) hipEventRecord(ev[slot][0], stream);
     launch_parse(stream, job.ntiles, job.max_tile_w, dpkt, d_desc[slot].d, d_recs[slot].d, d_coefs[slot].d, d_status[slot].d);
+    if (tm) hipEventRecord(ev[slot][1], stream);
     if (inter)
       launch_recon_inter(stream, mbc * mbr, d_desc[slot].d, d_recs[slot].d, d_coefs[slot].d, d_status[slot].d, rec, r, g);
+    if (tm) hipEventRecord(ev[slot][2], stream);
     launch_recon_intra(stream, job.ntiles, d_desc[slot].d, d_recs[slot].d, d_coefs[slot].d, d_status[slot].d, rec);
+    if (tm) {
+      hipEventRecord(ev[slot][3], stream);
+      ev_used[slot] = 1;
+    }
     if (hipMemcpyAsync(h_status[slot].h, d_status[slot].d, sizeof(TileStatus) * job.ntiles, hipMemcpyDeviceToHost,
                        stream) != hipSuccess)
       return fail(EBIV_ERR_DEVICE, "status download failed");
@@ -588,6 +620,8 @@ struct ebiv_codec {
       }
     }
     if (hipStreamSynchronize(stream) != hipSuccess) return fail(EBIV_ERR_DEVICE, "decode failed on the device");
+    collect_times(0, inter);
+    collect_times(1, true);
     rc = tile_result(0, job);
     if (rc) {
       if (key && ov) std::memcpy(prev, saved, sizeof prev);
@@ -661,6 +695,19 @@ int ebiv_codec_stats(const ebiv_codec* c, ebiv_stats* out) {
   return EBIV_OK;
 }
 
+int ebiv_codec_set_timing(ebiv_codec* c, int enabled) {
+  if (!c) return EBIV_ERR_ARGUMENT;
+  c->timing = enabled != 0;
+  c->ktimes = ebiv_kernel_times{};
+  return EBIV_OK;
+}
+
+int ebiv_codec_kernel_times(const ebiv_codec* c, ebiv_kernel_times* out) {
+  if (!c || !out) return EBIV_ERR_ARGUMENT;
+  *out = c->ktimes;
+  return EBIV_OK;
+}
+
 int ebiv_frame_rgba(ebiv_codec* c, const ebiv_frame* f, uint8_t* dst, int dst_on_device) {
   if (!c || !f || !dst || !c->ready) return EBIV_ERR_ARGUMENT;
   if (f->ring_slot < 0 || f->ring_slot >= c->ring_size || c->ring[f->ring_slot] < 0) return EBIV_ERR_ARGUMENT;
@@ -672,12 +719,19 @@ int ebiv_frame_rgba(ebiv_codec* c, const ebiv_frame* f, uint8_t* dst, int dst_on
     if (!c->d_rgba.reserve(px)) return EBIV_ERR_DEVICE;
     out = c->d_rgba.d;
   }
+  const bool tm = c->timing && c->ensure_events();
+  if (tm) hipEventRecord(c->ev[0][0], c->stream);
   launch_rgba(c->stream, b.p.y, b.p.pw, b.p.cb, b.p.cr, b.p.cw, c->W, c->H, out);
+  if (tm) hipEventRecord(c->ev[0][1], c->stream);
   if (hipGetLastError() != hipSuccess) return EBIV_ERR_DEVICE;
   if (!dst_on_device &&
       hipMemcpyAsync(dst, c->d_rgba.d, px * 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
     return EBIV_ERR_DEVICE;
   if (hipStreamSynchronize(c->stream) != hipSuccess) return EBIV_ERR_DEVICE;
+  if (tm) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, c->ev[0][0], c->ev[0][1]) == hipSuccess) { c->ktimes.rgba_ms += ms; c->ktimes.rgba_n++; }
+  }
   return EBIV_OK;
 }
 

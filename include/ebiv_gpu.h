@@ -83,6 +83,17 @@ const char *ebiv_codec_last_error(const ebiv_codec *c);
 
 int ebiv_codec_stats(const ebiv_codec *c, ebiv_stats *out);
 
+/* Per-kernel device time of this codec's decodes, measured with HIP events
+ * on the codec's stream while timing is enabled (no reference counterpart:
+ * measurement only).  Durations in milliseconds, summed over launches. */
+typedef struct {
+    double parse_ms, inter_ms, intra_ms, rgba_ms;
+    uint64_t parse_n, inter_n, intra_n, rgba_n;
+} ebiv_kernel_times;
+
+int ebiv_codec_set_timing(ebiv_codec *c, int enabled);
+int ebiv_codec_kernel_times(const ebiv_codec *c, ebiv_kernel_times *out);
+
 /* Frame.RGBA for a frame this codec returned — frame.go:8-67, bit-exact.
  * dst receives width*height*4 bytes; dst_on_device selects HBM or host. */
 int ebiv_frame_rgba(ebiv_codec *c, const ebiv_frame *f, uint8_t *dst, int dst_on_device);
