@@ -16,7 +16,7 @@ EXPORTS = (
     "ebiv_codec_create", "ebiv_codec_decode", "ebiv_codec_decode_device", "ebiv_codec_flush",
     "ebiv_codec_destroy", "ebiv_codec_last_error", "ebiv_codec_stats", "ebiv_frame_rgba",
     "ebiv_device_alloc", "ebiv_device_free", "ebiv_memcpy_h2d", "ebiv_memcpy_d2h", "ebiv_device_count",
-    "ebiv_codec_set_timing", "ebiv_codec_kernel_times",
+    "ebiv_codec_set_timing", "ebiv_codec_kernel_times", "ebiv_codec_debug_dump",
 )
 
 
@@ -68,6 +68,8 @@ def bind(L):
     L.ebiv_device_count.argtypes = []
     L.ebiv_codec_set_timing.restype = C.c_int
     L.ebiv_codec_set_timing.argtypes = [vp, C.c_int]
+    L.ebiv_codec_debug_dump.restype = C.c_int
+    L.ebiv_codec_debug_dump.argtypes = [vp, C.c_int, vp, sz, vp, sz, vp, sz]
     L.ebiv_codec_kernel_times.restype = C.c_int
     L.ebiv_codec_kernel_times.argtypes = [vp, C.POINTER(KernelTimes)]
     return L

@@ -175,6 +175,24 @@ class Codec:
         self._L.ebiv_codec_kernel_times(self._h, C.byref(t))
         return {f: getattr(t, f) for f, _ in capi.KernelTimes._fields_}
 
+    def debug_dump(self, slot: int = 0):
+        """Entropy-kernel output of the last coded decode: (records, coefs, status)
+        as numpy structured arrays (diagnostics)."""
+        rec_dt = np.dtype([("kind", "u1"), ("part", "u1"), ("cbp", "u1"), ("tx", "u1"), ("refsel", "u1"),
+                           ("cmode", "u1"), ("pad0", "u1"), ("pad1", "u1"), ("wmask", "<u4"),
+                           ("coef_off", "<u4"), ("coef_count", "<u4"), ("mv", "<i2", (4, 2)),
+                           ("lmode", "<u4", (2,)), ("pad2", "<u4")])
+        st_dt = np.dtype([("err", "<i4"), ("mbs_done", "<u4"), ("nsym", "<u4"), ("n_intra", "<u4")])
+        n = self._L.ebiv_codec_debug_dump(self._h, slot, None, 0, None, 0, None, 0)
+        if n < 0:
+            raise DeviceError("debug dump failed")
+        recs = np.zeros(n, rec_dt)
+        coefs = np.zeros(n * 384, np.uint32)
+        status = np.zeros(4096, st_dt)
+        self._L.ebiv_codec_debug_dump(self._h, slot, recs.ctypes.data, recs.nbytes, coefs.ctypes.data, coefs.size,
+                                      status.ctypes.data, status.nbytes)
+        return recs, coefs, status
+
     def stats(self) -> dict:
         s = capi.Stats()
         self._L.ebiv_codec_stats(self._h, C.byref(s))

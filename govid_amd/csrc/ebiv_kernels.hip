@@ -676,7 +676,7 @@ struct IntraScratch {
 // above-right), so the first available sample and every gap fill are known.
 // at(dx, dy) reads the sample at block-relative (dx, dy), dx/dy >= -1.
 template <typename At>
-__device__ void build_ref(int32_t* ref, int n, bool av_above, bool av_left, bool av_ar, At at, int i) {
+__device__ void build_ref_one(int32_t* ref, int n, bool av_above, bool av_left, bool av_ar, At at, int i) {
   if (i > 4 * n) return;
   int32_t v;
   if (!av_left && !av_above) {
@@ -701,6 +701,13 @@ __device__ void build_ref(int32_t* ref, int n, bool av_above, bool av_left, bool
     }
   }
   ref[i] = v;
+}
+
+// All 4n+1 entries with one wavefront (n = 16 needs 65, so lane 0 also does 64).
+template <typename At>
+__device__ void build_ref(int32_t* ref, int n, bool av_above, bool av_left, bool av_ar, At at, int lane) {
+  build_ref_one(ref, n, av_above, av_left, av_ar, at, lane);
+  build_ref_one(ref, n, av_above, av_left, av_ar, at, lane + 64);
 }
 
 // Prediction of pixel (r, c) of an n x n block from its reference array
@@ -880,7 +887,13 @@ __global__ __launch_bounds__(256) void ebiv_recon_intra(const uint8_t* __restric
       if (r.kind != 2) continue;
       intra_mb(scratch[wave], coefs, r, hdr, td, rec, mbx, mby, lane);
     }
+    // Macroblocks of the next step read what other wavefronts of this
+    // workgroup stored in this one.  The barrier orders the stores, but the
+    // CU's vector L1 may still hold lines fetched before them (another
+    // wavefront's neighbour loads touch the same 128-byte lines), so the
+    // acquire at agent scope invalidates L1 before the next step's loads.
     __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
 }
 

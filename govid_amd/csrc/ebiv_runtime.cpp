@@ -695,6 +695,22 @@ int ebiv_codec_stats(const ebiv_codec* c, ebiv_stats* out) {
   return EBIV_OK;
 }
 
+int ebiv_codec_debug_dump(ebiv_codec* c, int slot, void* recs, size_t recs_bytes, uint32_t* coefs, size_t coef_count,
+                          void* status, size_t status_bytes) {
+  if (!c || slot < 0 || slot > 1 || !c->ready) return -EBIV_ERR_ARGUMENT;
+  if (hipSetDevice(c->device) != hipSuccess) return -EBIV_ERR_DEVICE;
+  const size_t nmb = static_cast<size_t>(c->mbc) * c->mbr;
+  if (recs && hipMemcpy(recs, c->d_recs[slot].d, std::min(recs_bytes, nmb * sizeof(MBRec)), hipMemcpyDeviceToHost) != hipSuccess)
+    return -EBIV_ERR_DEVICE;
+  if (coefs && hipMemcpy(coefs, c->d_coefs[slot].d, std::min(coef_count, nmb * kCoefPerMB) * 4, hipMemcpyDeviceToHost) != hipSuccess)
+    return -EBIV_ERR_DEVICE;
+  if (status && c->d_status[slot].d &&
+      hipMemcpy(status, c->d_status[slot].d, std::min(status_bytes, c->d_status[slot].cap * sizeof(TileStatus)),
+                hipMemcpyDeviceToHost) != hipSuccess)
+    return -EBIV_ERR_DEVICE;
+  return static_cast<int>(nmb);
+}
+
 int ebiv_codec_set_timing(ebiv_codec* c, int enabled) {
   if (!c) return EBIV_ERR_ARGUMENT;
   c->timing = enabled != 0;

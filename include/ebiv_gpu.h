@@ -94,6 +94,14 @@ typedef struct {
 int ebiv_codec_set_timing(ebiv_codec *c, int enabled);
 int ebiv_codec_kernel_times(const ebiv_codec *c, ebiv_kernel_times *out);
 
+/* Diagnostics (no reference counterpart): copies the entropy kernel's
+ * output of the last coded decode to host memory — macroblock records
+ * (48 bytes each, frame raster order), the coefficient list, and the per-tile
+ * status (16 bytes each).  slot 0 is the frame's payload, 1 a golden override.
+ * Returns the number of macroblocks, or a negative status. */
+int ebiv_codec_debug_dump(ebiv_codec *c, int slot, void *recs, size_t recs_bytes, uint32_t *coefs,
+                          size_t coef_count, void *status, size_t status_bytes);
+
 /* Frame.RGBA for a frame this codec returned — frame.go:8-67, bit-exact.
  * dst receives width*height*4 bytes; dst_on_device selects HBM or host. */
 int ebiv_frame_rgba(ebiv_codec *c, const ebiv_frame *f, uint8_t *dst, int dst_on_device);

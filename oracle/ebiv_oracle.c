@@ -624,8 +624,18 @@ typedef struct {
     int inter;
     int16_t *mv;
     int mvstride;
+    orc_mbtrace *trace; /* this macroblock's trace record, or NULL */
+    int blk;            /* block id being parsed, for the trace */
     int32_t levels[256], coeffs[256], recon[256], pred[256];
 } tdec;
+
+static void trace_levels(tdec *t, int n)
+{
+    if (!t->trace) return;
+    for (int i = 0; i < n * n; i++)
+        if (t->levels[i] && t->trace->ncoef < 384)
+            t->trace->coef[t->trace->ncoef++] = (uint32_t)i | (uint32_t)t->blk << 8 | (uint32_t)t->levels[i] << 13;
+}
 
 static void dequant(const tdec *t, int n) /* quantizer.dequantize, quant.go:72-80 */
 {
@@ -640,6 +650,7 @@ static void recon_block(tdec *t, int plane_id, int tx, plane *p, int x0, int y0,
     if (mode < 0 || mode >= N_INTRA_MODES) { rfail(&t->d, "intra mode"); return; }
     memset(t->levels, 0, sizeof(int32_t) * n * n);
     decode_block(&t->d, plane_id, tx, t->levels, n);
+    trace_levels(t, n);
     if (t->d.err) return;
     dequant(t, n);
     inverse_dct(t->coeffs, t->recon, n);
@@ -653,11 +664,15 @@ static void decode_luma_mb(tdec *t, int mbx, int mby, tbounds b)
     int tx = rdec_decode(&t->d, CTX_TXSIZE);
     if (tx < 0 || tx >= 3) { rfail(&t->d, "tx size"); return; }
     int n = 4 << tx;
+    if (t->trace) t->trace->tx = tx;
     int tl = b.x0 * MBS, tt = b.y0 * MBS, tr = b.x1 * MBS, mbt = mby * MBS, mbr = (mbx + 1) * MBS;
+    int bi = 0;
     for (int by = 0; by < MBS; by += n)
         for (int bx = 0; bx < MBS; bx += n) {
             int x0 = mbx * MBS + bx, y0 = mby * MBS + by;
             int mode = rdec_decode(&t->d, CTX_LUMAMODE);
+            if (t->trace) t->trace->lmode[bi] = mode;
+            t->blk = bi++;
             int ar = above_right_avail(x0, y0, n, tt, tr, mbt, mbr);
             recon_block(t, 0, tx, &t->rec->y, x0, y0, n, mode, y0 > tt, x0 > tl, ar);
         }
@@ -668,7 +683,10 @@ static void decode_chroma_mb(tdec *t, int mbx, int mby, tbounds b)
 {
     int tl = b.x0 * CMBS, tt = b.y0 * CMBS, x0 = mbx * CMBS, y0 = mby * CMBS;
     int mode = rdec_decode(&t->d, CTX_CHROMAMODE);
+    if (t->trace) t->trace->cmode = mode;
+    t->blk = 16;
     recon_block(t, 1, 0, &t->rec->cb, x0, y0, CMBS, mode, y0 > tt, x0 > tl, 0);
+    t->blk = 17;
     recon_block(t, 1, 0, &t->rec->cr, x0, y0, CMBS, mode, y0 > tt, x0 > tl, 0);
 }
 
@@ -677,6 +695,7 @@ static void recon_residual(tdec *t, int plane_id, int tx, plane *p, int x0, int 
 {
     memset(t->levels, 0, sizeof(int32_t) * n * n);
     decode_block(&t->d, plane_id, tx, t->levels, n);
+    trace_levels(t, n);
     if (t->d.err) return;
     dequant(t, n);
     inverse_dct(t->coeffs, t->recon, n);
@@ -711,6 +730,7 @@ static void recon_inter_chroma(tdec *t, int cx, int cy, int part, const int16_t 
 static void recon_inter_mb(tdec *t, const fbuf *ref, int mbx, int mby, int part, const int16_t (*mvs)[2])
 {
     int cbp = rdec_decode(&t->d, CTX_CBP);
+    if (t->trace) t->trace->cbp = cbp;
     if (cbp < 0 || cbp >= 8 || t->d.err) { rfail(&t->d, "cbp"); return; }
     if (cbp & CBP_LUMA) {
         int32_t mcy[256];
@@ -722,8 +742,11 @@ static void recon_inter_mb(tdec *t, const fbuf *ref, int mbx, int mby, int part,
         int tx = rdec_decode(&t->d, CTX_TXSIZE);
         if (tx < 0 || tx >= 3) { rfail(&t->d, "tx size"); return; }
         int n = 4 << tx;
+        if (t->trace) t->trace->tx = tx;
+        int bi = 0;
         for (int by = 0; by < MBS; by += n)
             for (int bx = 0; bx < MBS; bx += n) {
+                t->blk = bi++;
                 for (int r = 0; r < n; r++)
                     for (int c = 0; c < n; c++) t->pred[r * n + c] = mcy[(by + r) * MBS + bx + c];
                 recon_residual(t, 0, tx, &t->rec->y, mbx * MBS + bx, mby * MBS + by, n);
@@ -738,7 +761,9 @@ static void recon_inter_mb(tdec *t, const fbuf *ref, int mbx, int mby, int part,
         }
     }
     int cx = mbx * CMBS, cy = mby * CMBS;
+    t->blk = 16;
     recon_inter_chroma(t, cx, cy, part, mvs, &ref->cb, &t->rec->cb, (cbp & CBP_CB) != 0);
+    t->blk = 17;
     recon_inter_chroma(t, cx, cy, part, mvs, &ref->cr, &t->rec->cr, (cbp & CBP_CR) != 0);
 }
 
@@ -748,9 +773,11 @@ static void decode_inter_mb(tdec *t, int mbx, int mby, tbounds b)
     int mode = rdec_decode(&t->d, CTX_MBMODE);
     int idx = mby * t->mvstride + mbx;
     int16_t pred[2];
+    if (t->trace) t->trace->kind = mode;
     switch (mode) {
     case MB_SKIP: {
         predict_mv(t->mv, t->mvstride, mbx, mby, b, pred);
+        if (t->trace) { t->trace->mv[0][0] = pred[0]; t->trace->mv[0][1] = pred[1]; }
         t->mv[2 * idx] = pred[0];
         t->mv[2 * idx + 1] = pred[1];
         /* reconMCMB, decode_inter.go:66-71 */
@@ -786,6 +813,11 @@ static void decode_inter_mb(tdec *t, int mbx, int mby, tbounds b)
             int dy = zigzag_dec(class_value(&t->d, CTX_MVY));
             mvs[i][1] = clamp_mv(py + dy);
         }
+        if (t->trace) {
+            t->trace->refsel = rs;
+            t->trace->part = part;
+            for (int i = 0; i < part_count[part]; i++) { t->trace->mv[i][0] = mvs[i][0]; t->trace->mv[i][1] = mvs[i][1]; }
+        }
         if (t->d.err) return;
         if (rs == REF_GOLDEN) { t->mv[2 * idx] = t->mv[2 * idx + 1] = 0; }
         else { t->mv[2 * idx] = mvs[0][0]; t->mv[2 * idx + 1] = mvs[0][1]; }
@@ -798,10 +830,12 @@ static void decode_inter_mb(tdec *t, int mbx, int mby, tbounds b)
 }
 
 /* decodeTile, decode_intra.go:121-135. */
-static void decode_tile(tdec *t, tbounds b)
+static void decode_tile(tdec *t, tbounds b, orc_mbtrace *trace, int mbc)
 {
     for (int mby = b.y0; mby < b.y1; mby++)
         for (int mbx = b.x0; mbx < b.x1; mbx++) {
+            t->trace = trace ? &trace[mby * mbc + mbx] : NULL;
+            if (t->trace) { memset(t->trace, 0, sizeof *t->trace); t->trace->kind = 2; }
             if (t->inter) decode_inter_mb(t, mbx, mby, b);
             else { decode_luma_mb(t, mbx, mby, b); decode_chroma_mb(t, mbx, mby, b); }
             if (t->d.err) return;
@@ -821,6 +855,9 @@ struct orc_codec {
     int threads;
     uint64_t nsym;
     char msg[256];
+    int trace_on;
+    orc_mbtrace *trace;
+    int ntrace;
 };
 
 static int fail_msg(orc_codec *c, int code, const char *m)
@@ -847,7 +884,15 @@ void orc_codec_free(orc_codec *c)
     }
     for (int i = 0; i < 3; i++) { free(c->bufs[i].y.data); free(c->bufs[i].cb.data); free(c->bufs[i].cr.data); }
     free(c->mv);
+    free(c->trace);
     free(c);
+}
+
+void orc_codec_enable_trace(orc_codec *c, int on) { c->trace_on = on; }
+const orc_mbtrace *orc_codec_trace(const orc_codec *c, int *count)
+{
+    *count = c->ntrace;
+    return c->trace;
 }
 
 void orc_codec_flush(orc_codec *c) { c->next = 0; }
@@ -907,6 +952,7 @@ typedef struct {
     int *errs;
     const char **msgs;
     uint64_t *nsym;
+    orc_mbtrace *trace;
     atomic_int next;
 } frame_job;
 
@@ -924,7 +970,7 @@ static void run_tile(frame_job *j, int i)
         return;
     }
     tbounds b = {j->tb[4 * i], j->tb[4 * i + 1], j->tb[4 * i + 2], j->tb[4 * i + 3]};
-    decode_tile(t, b);
+    decode_tile(t, b, j->trace, j->rec->mbc);
     j->errs[i] = t->d.err;
     j->msgs[i] = t->d.msg;
     j->nsym[i] = t->d.nsym;
@@ -1009,6 +1055,12 @@ static int decode_coded_into(orc_codec *c, fbuf *rec, fbuf *ref, fbuf *golden, c
     j.nsym = calloc((size_t)nt, sizeof(uint64_t));
     atomic_init(&j.next, 0);
     memset(c->mv, 0, sizeof(int16_t) * 2 * (size_t)mbc * mbr); /* resetMV */
+    j.trace = NULL;
+    if (c->trace_on && rec == c->work) {
+        if (!c->trace) { c->trace = calloc((size_t)mbc * mbr, sizeof(orc_mbtrace)); c->ntrace = mbc * mbr; }
+        for (int i = 0; i < mbc * mbr; i++) c->trace[i].kind = -1;
+        j.trace = c->trace;
+    }
     int workers = c->threads < nt ? c->threads : nt;
     if (workers <= 1) {
         for (int i = 0; i < nt; i++) run_tile(&j, i);

@@ -56,6 +56,24 @@ uint64_t orc_codec_symbols(const orc_codec *c);
 int orc_codec_golden(const orc_codec *c, const uint8_t **y, const uint8_t **cb,
                      const uint8_t **cr, int *ypw, int *yph);
 
+/* Per-macroblock trace of the last coded decode (the frame's own payload, not
+ * a golden override): what the grammar read, for comparing the GPU's parse.
+ * coef entries: bits 0-7 position, 8-12 block (0-15 luma in decode order,
+ * 16 Cb, 17 Cr), 13-31 signed level. */
+typedef struct {
+    int32_t kind;              /* 0 skip, 1 inter, 2 intra, -1 not decoded */
+    int32_t tx, cmode, refsel, part, cbp;
+    int32_t lmode[16];
+    int16_t mv[4][2];
+    int32_t ncoef;
+    uint32_t coef[384];
+} orc_mbtrace;
+
+/* Enables tracing (allocates one record per macroblock on the next key
+ * frame); orc_codec_trace returns the records and their count. */
+void orc_codec_enable_trace(orc_codec *c, int on);
+const orc_mbtrace *orc_codec_trace(const orc_codec *c, int *count);
+
 /* convertYCbCr420ToRGBA — frame.go:8-67 (Rect.Min = 0). */
 void orc_ycbcr_to_rgba(const uint8_t *y, int y_stride, const uint8_t *cb,
                        const uint8_t *cr, int c_stride, int w, int h, uint8_t *dst);

@@ -27,6 +27,12 @@ class _Frame(C.Structure):
                 ("cwidth", C.c_int), ("cheight", C.c_int)]
 
 
+class _MBTrace(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("tx", C.c_int32), ("cmode", C.c_int32), ("refsel", C.c_int32),
+                ("part", C.c_int32), ("cbp", C.c_int32), ("lmode", C.c_int32 * 16), ("mv", C.c_int16 * 8),
+                ("ncoef", C.c_int32), ("coef", C.c_uint32 * 384)]
+
+
 def build() -> str:
     """Compiles the oracle with its Makefile (gcc only)."""
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
@@ -54,6 +60,9 @@ def lib():
         L.orc_codec_golden.restype = C.c_int
         L.orc_codec_golden.argtypes = [vp, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                        C.POINTER(C.c_void_p), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.orc_codec_enable_trace.argtypes = [vp, C.c_int]
+        L.orc_codec_trace.restype = C.POINTER(_MBTrace)
+        L.orc_codec_trace.argtypes = [vp, C.POINTER(C.c_int)]
         L.orc_ycbcr_to_rgba.argtypes = [vp, C.c_int, vp, vp, C.c_int, C.c_int, C.c_int, vp]
         L.orc_inverse_dct.restype = C.c_int
         L.orc_inverse_dct.argtypes = [i32p, i32p, C.c_int]
@@ -127,6 +136,21 @@ class OracleCodec:
 
     def flush(self):
         self._L.orc_codec_flush(self._h)
+
+    def enable_trace(self, on: bool = True):
+        self._L.orc_codec_enable_trace(self._h, int(on))
+
+    def trace(self):
+        """Per-macroblock trace of the last coded decode (list of dicts)."""
+        n = C.c_int()
+        p = self._L.orc_codec_trace(self._h, C.byref(n))
+        out = []
+        for i in range(n.value):
+            t = p[i]
+            out.append({"kind": t.kind, "tx": t.tx, "cmode": t.cmode, "refsel": t.refsel, "part": t.part,
+                        "cbp": t.cbp, "lmode": list(t.lmode), "mv": list(t.mv),
+                        "coef": sorted(t.coef[k] for k in range(t.ncoef))})
+        return out
 
     def symbols(self) -> int:
         return int(self._L.orc_codec_symbols(self._h))

@@ -34,3 +34,27 @@ def test_small_clips(case):
     wl, w, h, n = c.pop("workload"), c.pop("w"), c.pop("h"), c.pop("frames")
     print(f"case {wl} {w}x{h} x{n} {c}", flush=True)
     _run(clip_kw(wl, w, h, n, threads=4, **c), check_rgba=True)
+
+
+def test_bench_clip_1080p_all_frames():
+    """The benchmark's 1080p 4x3-tile clip (120 frames, two key frames, the
+    flash frames) decodes bit-exactly, every frame."""
+    import os
+
+    from govid_amd.container import Reader
+
+    path = os.path.join(os.path.dirname(os.path.dirname(__file__)), "bench_data", "parallax_1080p.ebiv")
+    rd = Reader.open(path)
+
+    class Clip:
+        packets = rd.packets()
+
+    from govid_amd.codec import Codec
+
+    gpu = Codec(ring=3)
+    ref = O.OracleCodec(ring=3, threads=8)
+    for i, pkt in enumerate(Clip.packets):
+        got = gpu.decode(pkt).planes()
+        want = ref.decode(pkt)
+        assert_planes_equal(got, want, f"frame {i}")
+    gpu.close()

@@ -1,0 +1,99 @@
+"""Entropy-kernel parity: the GPU's macroblock records and coefficient lists
+against the oracle's per-macroblock trace of the same frame."""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import clip_kw
+
+pytestmark = pytest.mark.gpu
+
+PART_COUNT = [1, 2, 2, 4]
+
+
+def compare_frame(gpu, ref, label=""):
+    recs, coefs, status = gpu.debug_dump(0)
+    tr = ref.trace()
+    bad = []
+    for i, t in enumerate(tr):
+        if t["kind"] < 0:
+            continue
+        r = recs[i]
+        errs = []
+        if int(r["kind"]) != t["kind"]:
+            errs.append(f"kind {r['kind']} != {t['kind']}")
+        else:
+            k = t["kind"]
+            if k == 2:
+                n = 4 << t["tx"]
+                nb = (16 // n) ** 2
+                if int(r["tx"]) != t["tx"]:
+                    errs.append(f"tx {r['tx']} != {t['tx']}")
+                lm = [(int(r["lmode"][b // 8]) >> (4 * (b % 8))) & 15 for b in range(nb)]
+                if lm != t["lmode"][:nb]:
+                    errs.append(f"lmodes {lm} != {t['lmode'][:nb]}")
+                if int(r["cmode"]) != t["cmode"]:
+                    errs.append(f"cmode {r['cmode']} != {t['cmode']}")
+            elif k == 1:
+                np_ = PART_COUNT[t["part"]]
+                gm = r["mv"][:np_].flatten().tolist()
+                if int(r["part"]) != t["part"] or int(r["refsel"]) != t["refsel"] or gm != t["mv"][:2 * np_]:
+                    errs.append(f"part/ref/mv {r['part']},{r['refsel']},{gm} != {t['part']},{t['refsel']},{t['mv'][:2*np_]}")
+                if int(r["cbp"]) != t["cbp"]:
+                    errs.append(f"cbp {r['cbp']} != {t['cbp']}")
+            else:
+                if r["mv"][0].tolist() != t["mv"][:2]:
+                    errs.append(f"skip mv {r['mv'][0].tolist()} != {t['mv'][:2]}")
+            off, cnt = int(r["coef_off"]), int(r["coef_count"])
+            g = sorted(coefs[off:off + cnt].tolist())
+            if g != t["coef"]:
+                gs, ts = set(g), set(t["coef"])
+                dec = lambda e: (e & 255, (e >> 8) & 31, np.int32(np.uint32(e)) >> 13)
+                errs.append(f"coefs: {len(g)} vs {len(t['coef'])}; gpu-only {[dec(e) for e in sorted(gs - ts)][:6]} "
+                            f"oracle-only {[dec(e) for e in sorted(ts - gs)][:6]}")
+        if errs:
+            bad.append((i, errs))
+    return bad
+
+
+def run_clip(clip, frames=None, label=""):
+    from govid_amd.codec import Codec
+
+    gpu = Codec()
+    ref = O.OracleCodec(threads=8)
+    ref.enable_trace(True)
+    W = clip.width if hasattr(clip, "width") else None
+    for fi, pkt in enumerate(clip.packets[: frames or len(clip.packets)]):
+        gpu.decode(pkt)
+        ref.decode(pkt)
+        bad = compare_frame(gpu, ref)
+        if bad:
+            mbc = (clip.width + 15) // 16
+            lines = [f"MB ({i % mbc},{i // mbc}): {'; '.join(e)}" for i, e in bad[:8]]
+            pytest.fail(f"{label} frame {fi}: {len(bad)} macroblocks differ\n" + "\n".join(lines))
+
+
+@pytest.mark.parametrize("case", [
+    dict(workload="synthframe", w=64, h=48, frames=3, qp=20, gop=1),
+    dict(workload="rects", w=96, h=64, frames=6, qp=22, gop=6, tile_cols=2, tile_rows=2),
+    dict(workload="parallax", w=256, h=128, frames=6, qp=22, gop=6, tile_cols=2, tile_rows=1),
+])
+def test_parse_matches_oracle(case):
+    c = dict(case)
+    wl, w, h, n = c.pop("workload"), c.pop("w"), c.pop("h"), c.pop("frames")
+    print(f"case {wl} {w}x{h} x{n} {c}", flush=True)
+    run_clip(clip_kw(wl, w, h, n, threads=4, **c), label=wl)
+
+
+def test_parse_bench_clip_first_frames():
+    import os
+    from govid_amd.container import Reader
+
+    path = os.path.join(os.path.dirname(os.path.dirname(__file__)), "bench_data", "parallax_1080p.ebiv")
+    rd = Reader.open(path)
+
+    class C:
+        packets = rd.packets()[:3]
+        width = rd.header.width
+
+    run_clip(C, label="parallax_1080p")
