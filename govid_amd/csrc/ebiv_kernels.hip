@@ -80,71 +80,6 @@ struct ByteWin {
   }
 };
 
-struct Rans {
-  ByteWin win;
-  uint32_t pos;              // bytes left below the read point
-  uint32_t s0, s1, s2, s3;   // interleaved states; s0 is the next one used
-  int32_t err;
-  uint32_t nsym;
-  uint32_t used0, used1, used2;
-  const uint16_t* cum;       // LDS copy of the frame tables
-};
-
-__device__ __forceinline__ void rans_fail(Rans& d, int code) {
-  if (!d.err) d.err = kErrCorrupt | (code << 8);
-}
-
-// ransDecoder.decode (rans.go:270-304): returns the symbol of context ctx.
-__device__ int rans_decode(Rans& d, int ctx, int lane) {
-  if (d.err) return 0;
-  uint32_t ub = ctx < 32 ? d.used0 : (ctx < 64 ? d.used1 : d.used2);
-  if (!((ub >> (ctx & 31)) & 1u)) {
-    rans_fail(d, kTileUnused);
-    return 0;
-  }
-  uint32_t x = d.s0;
-  uint32_t slot = x & (kRansM - 1);
-  uint32_t sym, start, freq;
-  if (ctx == kCtxBypass) {  // fixed {2048, 2048} (rans.go:317-322)
-    sym = slot >> 11;
-    start = sym << 11;
-    freq = 2048;
-  } else {
-    // Lane-parallel slot2sym: lane s holds the start of symbol s; the symbol
-    // is the last start not above the slot.
-    uint32_t c = d.cum[ctx * kCumStride + (lane < 16 ? lane : 16)];
-    uint64_t m = __ballot(c <= slot);
-    sym = static_cast<uint32_t>(__popcll(m)) - 1u;
-    start = rd_lane(c, static_cast<int>(sym));
-    freq = rd_lane(c, static_cast<int>(sym) + 1) - start;
-  }
-  x = freq * (x >> kRansScaleBits) + slot - start;
-  while (x < kRansLower) {
-    if (d.pos == 0) {
-      rans_fail(d, kTileUnderrun);
-      return 0;
-    }
-    d.pos--;
-    x = (x << 8) | d.win.byte(d.pos, lane);
-  }
-  d.s0 = d.s1;
-  d.s1 = d.s2;
-  d.s2 = d.s3;
-  d.s3 = x;
-  d.nsym++;
-  return static_cast<int>(sym);
-}
-
-// getClassValue (mv.go:95-108).
-__device__ uint32_t class_value(Rans& d, int ctx, int lane) {
-  int cls = rans_decode(d, ctx, lane);
-  if (cls == 0) return 0;
-  if (cls == 1) return 1;
-  uint32_t v = 0;
-  for (int i = 0; i < cls - 1; i++) v = (v << 1) | static_cast<uint32_t>(rans_decode(d, kCtxBypass, lane));
-  return (1u << (cls - 1)) | v;
-}
-
 __device__ __forceinline__ int zigzag_dec(uint32_t u) { return static_cast<int>(u >> 1) ^ -static_cast<int>(u & 1u); }
 __device__ __forceinline__ int16_t clamp_mv(int v) { return static_cast<int16_t>(v < -kMVClamp ? -kMVClamp : (v > kMVClamp ? kMVClamp : v)); }
 __device__ __forceinline__ int dtoken_ctx(int plane, int tx, int band, int prev) {
@@ -153,102 +88,8 @@ __device__ __forceinline__ int dtoken_ctx(int plane, int tx, int band, int prev)
 }
 __device__ __forceinline__ int dband(int i) { return i == 0 ? 0 : (i <= 2 ? 1 : (i <= 8 ? 2 : (i <= 20 ? 3 : 4))); }
 
-// Buffered coefficient writer: lane k holds entry base+k until 64 are queued.
-struct CoefOut {
-  uint32_t* dst;    // tile region
-  uint32_t base;    // tile-relative index of the first buffered entry
-  uint32_t nbuf;
-  uint32_t buf;
-};
-__device__ __forceinline__ void coef_flush(CoefOut& o, int lane) {
-  if (static_cast<uint32_t>(lane) < o.nbuf) o.dst[o.base + lane] = o.buf;
-  o.base += o.nbuf;
-  o.nbuf = 0;
-}
-__device__ __forceinline__ void coef_emit(CoefOut& o, uint32_t e, int lane) {
-  if (static_cast<uint32_t>(lane) == o.nbuf) o.buf = e;
-  if (++o.nbuf == 64) coef_flush(o, lane);
-}
 __device__ __forceinline__ uint32_t coef_entry(int pos, int blk, int32_t level) {
   return static_cast<uint32_t>(pos) | (static_cast<uint32_t>(blk) << 8) | (static_cast<uint32_t>(level) << 13);
-}
-
-// decodeBlock (residual.go:79-126), emitting nonzero levels.  Returns the
-// number of entries emitted.
-__device__ uint32_t parse_block(Rans& d, CoefOut& o, int plane, int tx, int n, int blk, int lane) {
-  const uint8_t* scan = c_zz[sidx_of(n)];
-  int total = n * n;
-  int first = -1, last = -1, prev = 0, first_pos = 0;
-  int32_t sum_abs = 0, first_mag = 0;
-  uint32_t emitted = 0;
-  for (int i = 0; i < total; i++) {
-    int t = rans_decode(d, dtoken_ctx(plane, tx, dband(i), prev), lane);
-    if (d.err) return emitted;
-    if (t == kTEOB) break;
-    if (t == kTZero) {
-      prev = 0;
-      continue;
-    }
-    int32_t mag;
-    if (t == kTEscape) {
-      mag = 5 + static_cast<int32_t>(class_value(d, kCtxEscClass, lane));
-      prev = 2;
-    } else {
-      mag = t;
-      prev = mag == 1 ? 1 : 2;
-    }
-    sum_abs += mag;
-    if (first < 0) {
-      first = i;
-      first_mag = mag;
-      first_pos = scan[i];
-    } else {
-      if (rans_decode(d, kCtxSign, lane) == 1) mag = -mag;
-      coef_emit(o, coef_entry(scan[i], blk, mag), lane);
-      emitted++;
-    }
-    if (d.err) return emitted;
-    last = i;
-  }
-  if (first < 0) return emitted;
-  bool neg;
-  if (last - first >= kSDHMinSpan) neg = (sum_abs & 1) != 0;
-  else neg = rans_decode(d, kCtxSign, lane) == 1;
-  coef_emit(o, coef_entry(first_pos, blk, neg ? -first_mag : first_mag), lane);
-  return emitted + 1;
-}
-
-struct MBState {
-  uint32_t kind, part, cbp, tx, refsel, cmode, wmask, ncoef;
-  uint32_t lmode0, lmode1;
-  int16_t mv[4][2];
-};
-
-// decodeLumaMB (decode_intra.go:137-157): a block is written by Go iff no
-// error has occurred by the end of its residual parse.
-__device__ void parse_luma_intra(Rans& d, CoefOut& o, MBState& m, int lane) {
-  int tx = rans_decode(d, kCtxTxSize, lane);
-  m.tx = static_cast<uint32_t>(tx);
-  int n = 4 << tx, per = 16 / n, nb = per * per;
-  for (int b = 0; b < nb; b++) {
-    int mode = rans_decode(d, kCtxLumaMode, lane);
-    if (b < 8) m.lmode0 |= static_cast<uint32_t>(mode) << (4 * b);
-    else m.lmode1 |= static_cast<uint32_t>(mode) << (4 * (b - 8));
-    m.ncoef += parse_block(d, o, 0, tx, n, b, lane);
-    if (d.err) return;
-    m.wmask |= 1u << b;
-  }
-}
-
-// decodeChromaMB (decode_intra.go:159-167).
-__device__ void parse_chroma_intra(Rans& d, CoefOut& o, MBState& m, int lane) {
-  m.cmode = static_cast<uint32_t>(rans_decode(d, kCtxChromaMode, lane));
-  m.ncoef += parse_block(d, o, 1, 0, 8, 16, lane);
-  if (d.err) return;
-  m.wmask |= 1u << 16;
-  m.ncoef += parse_block(d, o, 1, 0, 8, 17, lane);
-  if (d.err) return;
-  m.wmask |= 1u << 17;
 }
 
 __device__ __forceinline__ int16_t median3(int16_t a, int16_t b, int16_t c) {
@@ -259,7 +100,7 @@ __device__ __forceinline__ int16_t median3(int16_t a, int16_t b, int16_t c) {
 }
 
 // predictMV (mv.go:21-52) over the LDS rows of the tile's MV grid.
-__device__ void predict_mv(const uint32_t* prow, const uint32_t* crow, int lx, int tw, bool top_row, int16_t& px, int16_t& py) {
+__device__ __forceinline__ uint32_t predict_mv(const uint32_t* prow, const uint32_t* crow, int lx, int tw, bool top_row) {
   bool aok = lx > 0, bok = !top_row, cok = false;
   uint32_t a = 0, b = 0, c = 0;
   if (aok) a = crow[lx - 1];
@@ -268,17 +109,42 @@ __device__ void predict_mv(const uint32_t* prow, const uint32_t* crow, int lx, i
     if (lx + 1 < tw) { c = prow[lx + 1]; cok = true; }
     else if (aok) { c = prow[lx - 1]; cok = true; }
   }
+  if (!aok && !bok && !cok) return 0;
+  if (aok && !bok && !cok) return a;
+  if (!aok && bok && !cok) return b;
   auto X = [](uint32_t v) { return static_cast<int16_t>(v & 0xFFFFu); };
   auto Y = [](uint32_t v) { return static_cast<int16_t>(v >> 16); };
-  if (!aok && !bok && !cok) { px = 0; py = 0; return; }
-  if (aok && !bok && !cok) { px = X(a); py = Y(a); return; }
-  if (!aok && bok && !cok) { px = X(b); py = Y(b); return; }
-  px = median3(X(a), X(b), X(c));
-  py = median3(Y(a), Y(b), Y(c));
+  int16_t px = median3(X(a), X(b), X(c));
+  int16_t py = median3(Y(a), Y(b), Y(c));
+  return static_cast<uint32_t>(static_cast<uint16_t>(px)) | (static_cast<uint32_t>(static_cast<uint16_t>(py)) << 16);
 }
 
-__device__ __forceinline__ uint32_t pack_mv(int16_t x, int16_t y) {
-  return static_cast<uint32_t>(static_cast<uint16_t>(x)) | (static_cast<uint32_t>(static_cast<uint16_t>(y)) << 16);
+__device__ __forceinline__ int16_t mv_comp(uint32_t v, int comp) {
+  return static_cast<int16_t>(comp ? (v >> 16) : (v & 0xFFFFu));
+}
+__device__ __forceinline__ uint32_t mv_set(uint32_t v, int comp, int16_t c) {
+  uint32_t u = static_cast<uint16_t>(c);
+  return comp ? ((v & 0xFFFFu) | (u << 16)) : ((v & 0xFFFF0000u) | u);
+}
+
+// Grammar states of the entropy kernel.  Every state up to P_FSIGN decodes
+// exactly one symbol in the context it names; the states after it are
+// bookkeeping steps that decode nothing.  The order of decodes is that of
+// decodeTile / decodeLumaMB / decodeChromaMB (decode_intra.go:121-167),
+// decodeInterMB / reconInterMB (decode_inter.go:8-118), getClassValue
+// (mv.go:95-108) and decodeBlock (residual.go:79-126).
+enum PState : int {
+  P_MBMODE, P_REF, P_PART, P_MVCLS, P_MVBITS, P_CBP, P_TX, P_LMODE, P_CMODE,
+  P_TOKEN, P_SIGN, P_ESCCLS, P_ESCBITS, P_FSIGN,
+  P_ENDBLK,    // a block's tokens are complete: resolve the hidden/explicit first sign
+  P_BLKDONE,   // a block is complete: next block, next plane, or end of macroblock
+  P_LUMADONE,  // inter luma complete: uncoded chroma is written, coded chroma parsed
+  P_ENDMB,     // macroblock complete: write its record, start the next one
+  P_DONE
+};
+
+__device__ __forceinline__ uint32_t mv_pick(uint32_t m0, uint32_t m1, uint32_t m2, uint32_t m3, int i) {
+  return i == 0 ? m0 : (i == 1 ? m1 : (i == 2 ? m2 : m3));
 }
 
 __global__ __launch_bounds__(64) void ebiv_parse(const uint8_t* __restrict__ pkt, const uint8_t* __restrict__ desc,
@@ -296,168 +162,335 @@ __global__ __launch_bounds__(64) void ebiv_parse(const uint8_t* __restrict__ pkt
   const uint32_t* gcum = reinterpret_cast<const uint32_t*>(hdr->cum);
   constexpr int kCumWords = 72 * kCumStride / 2;
   for (int i = lane; i < kCumWords; i += 64) smem[i] = gcum[i];
-  uint32_t* mvrow0 = smem + kCumWords;
+  uint32_t* prow = smem + kCumWords;
+  uint32_t* crow = prow + hdr->max_tile_w;
   const int tw = td.x1 - td.x0;
-  uint32_t* mvrow1 = mvrow0 + hdr->max_tile_w;
-  wave_sync();
   __syncthreads();
 
-  Rans d;
-  d.cum = cum;
-  d.used0 = hdr->used[0];
-  d.used1 = hdr->used[1];
-  d.used2 = hdr->used[2];
-  d.err = 0;
-  d.nsym = 0;
-  TileStatus st{0, 0, 0, 0};
   if (td.byte_len < 16) {  // ransDecoder.reset (rans.go:237-250)
-    if (lane == 0) {
-      st.err = kErrCorrupt | (kTileShort << 8);
-      status[t] = st;
-    }
+    if (lane == 0) status[t] = TileStatus{kErrCorrupt | (kTileShort << 8), 0, 0, 0};
     return;
   }
-  d.win.init(pkt + td.byte_off, td.byte_len, lane);
-  d.pos = td.byte_len;
-  uint32_t s[4];
-  for (int i = 3; i >= 0; i--) {  // pop32: big-endian from the top, state 3 first
-    uint32_t a = d.win.byte(--d.pos, lane);
-    uint32_t b = d.win.byte(--d.pos, lane);
-    uint32_t c = d.win.byte(--d.pos, lane);
-    uint32_t e = d.win.byte(--d.pos, lane);
-    s[i] = (a << 24) | (b << 16) | (c << 8) | e;
+  const uint32_t used0 = hdr->used[0], used1 = hdr->used[1], used2 = hdr->used[2];
+  ByteWin win;
+  win.init(pkt + td.byte_off, td.byte_len, lane);
+  uint32_t pos = td.byte_len;
+  uint32_t s0, s1, s2, s3;
+  {
+    uint32_t sv[4];
+#pragma unroll
+    for (int i = 3; i >= 0; i--) {  // pop32: big-endian from the top, state 3 first
+      uint32_t a = win.byte(--pos, lane);
+      uint32_t b = win.byte(--pos, lane);
+      uint32_t c = win.byte(--pos, lane);
+      uint32_t e = win.byte(--pos, lane);
+      sv[i] = (a << 24) | (b << 16) | (c << 8) | e;
+    }
+    s0 = sv[0]; s1 = sv[1]; s2 = sv[2]; s3 = sv[3];
   }
-  d.s0 = s[0]; d.s1 = s[1]; d.s2 = s[2]; d.s3 = s[3];
 
-  CoefOut co;
-  co.dst = coefs + td.coef_base;
-  co.base = 0;
-  co.nbuf = 0;
-  co.buf = 0;
+  // Coefficient output: lane k buffers entry cbase+k until 64 are queued.
+  uint32_t* const cdst = coefs + td.coef_base;
+  uint32_t cbase = 0, cn = 0, cbuf = 0;
+
   const int mbc = hdr->mbc;
-  const bool inter = hdr->inter != 0;
-  uint32_t* prow = mvrow0;
-  uint32_t* crow = mvrow1;
-  uint32_t done = 0, nintra = 0;
+  const bool inter_frame = hdr->inter != 0;
+  int32_t err = 0;
+  uint32_t nsym = 0, done = 0, nintra = 0;
+  int mbx = td.x0, mby = td.y0;
 
-  for (int mby = td.y0; mby < td.y1 && !d.err; mby++) {
-    for (int mbx = td.x0; mbx < td.x1; mbx++) {
-      MBState m;
-      m.kind = 2; m.part = 0; m.cbp = 0; m.tx = 0; m.refsel = 0; m.cmode = 0;
-      m.wmask = 0; m.ncoef = 0; m.lmode0 = 0; m.lmode1 = 0;
-      for (int i = 0; i < 4; i++) { m.mv[i][0] = 0; m.mv[i][1] = 0; }
-      uint32_t coef_start = co.base + co.nbuf;
-      const int lx = mbx - td.x0;
-      const bool top = mby == td.y0;
-      uint32_t mvcur = 0;
-      if (!inter) {
-        parse_luma_intra(d, co, m, lane);
-        parse_chroma_intra(d, co, m, lane);
-      } else {
-        int mode = rans_decode(d, kCtxMBMode, lane);  // an error reads as skip (0)
-        if (mode == kMBSkip) {
-          int16_t px, py;
-          predict_mv(prow, crow, lx, tw, top, px, py);
-          m.kind = 0;
-          m.mv[0][0] = px; m.mv[0][1] = py;
-          m.wmask = 0x3FFFFu;
-          mvcur = pack_mv(px, py);
-        } else if (mode == kMBIntra) {
-          m.kind = 2;
-          parse_luma_intra(d, co, m, lane);
-          parse_chroma_intra(d, co, m, lane);
-        } else {
-          m.kind = 1;
-          int rs = rans_decode(d, kCtxRef, lane);
-          int16_t px, py;
-          predict_mv(prow, crow, lx, tw, top, px, py);
-          if (rs == kRefGolden) { px = 0; py = 0; }
-          int part = rans_decode(d, kCtxPart, lane);
-          m.refsel = static_cast<uint32_t>(rs);
-          m.part = static_cast<uint32_t>(part);
-          const int np = c_part_count[part];
-          for (int i = 0; i < np; i++) {
-            int ppx = i > 0 ? m.mv[0][0] : px;
-            int ppy = i > 0 ? m.mv[0][1] : py;
-            int dx = zigzag_dec(class_value(d, kCtxMVClassX, lane));
-            m.mv[i][0] = clamp_mv(ppx + dx);
-            int dy = zigzag_dec(class_value(d, kCtxMVClassY, lane));
-            m.mv[i][1] = clamp_mv(ppy + dy);
-          }
-          if (!d.err) {
-            mvcur = rs == kRefGolden ? 0u : pack_mv(m.mv[0][0], m.mv[0][1]);
-            int cbp = rans_decode(d, kCtxCBP, lane);  // reconInterMB (decode_inter.go:73-118)
-            m.cbp = static_cast<uint32_t>(cbp);
-            if (!d.err) {
-              bool luma_ok = true;
-              if (cbp & kCBPLuma) {
-                int tx = rans_decode(d, kCtxTxSize, lane);
-                m.tx = static_cast<uint32_t>(tx);
-                int n = 4 << tx, per = 16 / n, nb = per * per;
-                for (int b = 0; b < nb; b++) {
-                  m.ncoef += parse_block(d, co, 0, tx, n, b, lane);
-                  if (d.err) { luma_ok = false; break; }
-                  m.wmask |= 1u << b;
-                }
-              } else {
-                m.wmask |= 0xFFFFu;
-              }
-              if (luma_ok) {
-                if (cbp & kCBPCb) {
-                  m.ncoef += parse_block(d, co, 1, 0, 8, 16, lane);
-                  if (!d.err) m.wmask |= 1u << 16;
-                } else {
-                  m.wmask |= 1u << 16;
-                }
-                if (cbp & kCBPCr) {
-                  if (!d.err) {
-                    m.ncoef += parse_block(d, co, 1, 0, 8, 17, lane);
-                    if (!d.err) m.wmask |= 1u << 17;
-                  }
-                } else {
-                  m.wmask |= 1u << 17;
-                }
-              }
-            }
-          }
+  // Macroblock state.
+  uint32_t kind = 2, part = 0, cbp = 0, tx = 0, refsel = 0, cmode = 0, wmask = 0, ncoef = 0;
+  uint32_t lmode0 = 0, lmode1 = 0, coef_start = 0, mvcur = 0, pred = 0;
+  uint32_t mv0 = 0, mv1 = 0, mv2 = 0, mv3 = 0;
+  int pi = 0, comp = 0, npart = 1;
+  uint32_t clsval = 0;
+  int cls_bits = 0, cls = 0;
+  bool intra_mb = !inter_frame;
+  // Block state.
+  int plane = 0, blk = 0, n = 4, nb = 16, total = 16, si = 0, prev = 0, first = -1, last = -1;
+  int first_pos = 0;
+  int32_t first_mag = 0, pend_mag = 0, sum_abs = 0;
+
+  int st = inter_frame ? P_MBMODE : P_TX;
+  while (st != P_DONE) {
+    // ---------------- bookkeeping states (no symbol) ----------------
+    if (st >= P_ENDBLK) {
+      if (st == P_ENDBLK) {
+        if (first >= 0 && last - first < kSDHMinSpan) {
+          st = P_FSIGN;
+          continue;
         }
+        if (first >= 0) {  // sign hidden in the parity of the level sum
+          const uint32_t e = coef_entry(first_pos, blk, (sum_abs & 1) ? -first_mag : first_mag);
+          if (static_cast<uint32_t>(lane) == cn) cbuf = e;
+          if (++cn == 64) { cdst[cbase + lane] = cbuf; cbase += 64; cn = 0; }
+          ncoef++;
+        }
+        st = P_BLKDONE;
       }
-      if (m.kind == 2) nintra++;
-      crow[lx] = mvcur;
-      // Macroblock record (written by lane 0).
+      if (st == P_BLKDONE) {
+        wmask |= 1u << blk;
+        si = 0; prev = 0; first = -1; last = -1; sum_abs = 0;
+        if (plane == 0) {
+          if (blk + 1 < nb) {
+            blk++;
+            st = intra_mb ? P_LMODE : P_TOKEN;
+          } else {
+            st = intra_mb ? P_CMODE : P_LUMADONE;
+          }
+        } else if (blk == 16 && (intra_mb || (cbp & kCBPCr))) {
+          blk = 17;
+          st = P_TOKEN;
+        } else {
+          st = P_ENDMB;
+        }
+        if (st != P_LUMADONE && st != P_ENDMB) continue;
+      }
+      if (st == P_LUMADONE) {
+        if (!(cbp & kCBPCb)) wmask |= 1u << 16;
+        if (!(cbp & kCBPCr)) wmask |= 1u << 17;
+        si = 0; prev = 0; first = -1; last = -1; sum_abs = 0;
+        plane = 1; n = 8; total = 64;
+        if (cbp & kCBPCb) { blk = 16; st = P_TOKEN; continue; }
+        if (cbp & kCBPCr) { blk = 17; st = P_TOKEN; continue; }
+        st = P_ENDMB;
+      }
+      // P_ENDMB: the macroblock record (decode_intra.go:121-135 loop body).
+      if (kind == 2) nintra++;
+      crow[mbx - td.x0] = mvcur;
       if (lane == 0) {
         MBRec r;
-        r.kind = static_cast<uint8_t>(m.kind);
-        r.part = static_cast<uint8_t>(m.part);
-        r.cbp = static_cast<uint8_t>(m.cbp);
-        r.tx = static_cast<uint8_t>(m.tx);
-        r.refsel = static_cast<uint8_t>(m.refsel);
-        r.cmode = static_cast<uint8_t>(m.cmode);
+        r.kind = static_cast<uint8_t>(kind);
+        r.part = static_cast<uint8_t>(part);
+        r.cbp = static_cast<uint8_t>(cbp);
+        r.tx = static_cast<uint8_t>(tx);
+        r.refsel = static_cast<uint8_t>(refsel);
+        r.cmode = static_cast<uint8_t>(cmode);
         r.pad0 = 0;
         r.pad1 = 0;
-        r.wmask = m.wmask;
+        r.wmask = wmask;
         r.coef_off = td.coef_base + coef_start;
-        r.coef_count = m.ncoef;
-        for (int i = 0; i < 4; i++) { r.mv[i][0] = m.mv[i][0]; r.mv[i][1] = m.mv[i][1]; }
-        r.lmode[0] = m.lmode0;
-        r.lmode[1] = m.lmode1;
+        r.coef_count = ncoef;
+        r.mv[0][0] = mv_comp(mv0, 0); r.mv[0][1] = mv_comp(mv0, 1);
+        r.mv[1][0] = mv_comp(mv1, 0); r.mv[1][1] = mv_comp(mv1, 1);
+        r.mv[2][0] = mv_comp(mv2, 0); r.mv[2][1] = mv_comp(mv2, 1);
+        r.mv[3][0] = mv_comp(mv3, 0); r.mv[3][1] = mv_comp(mv3, 1);
+        r.lmode[0] = lmode0;
+        r.lmode[1] = lmode1;
         r.pad2 = 0;
         recs[mby * mbc + mbx] = r;
       }
       done++;
-      if (d.err) break;
+      if (err) break;
+      if (++mbx == td.x1) {
+        mbx = td.x0;
+        if (++mby == td.y1) break;
+        wave_sync();
+        uint32_t* tmp = prow; prow = crow; crow = tmp;
+      }
+      kind = 2; part = 0; cbp = 0; tx = 0; refsel = 0; cmode = 0; wmask = 0; ncoef = 0;
+      lmode0 = 0; lmode1 = 0; mvcur = 0; mv0 = mv1 = mv2 = mv3 = 0;
+      coef_start = cbase + cn;
+      intra_mb = !inter_frame;
+      plane = 0; blk = 0; si = 0; prev = 0; first = -1; last = -1; sum_abs = 0;
+      st = inter_frame ? P_MBMODE : P_TX;
+      continue;
     }
-    wave_sync();
-    uint32_t* tmp = prow; prow = crow; crow = tmp;
+
+    // ---------------- one symbol: ransDecoder.decode (rans.go:270-304) ----------------
+    int ctx;
+    switch (st) {
+      case P_MBMODE: ctx = kCtxMBMode; break;
+      case P_REF: ctx = kCtxRef; break;
+      case P_PART: ctx = kCtxPart; break;
+      case P_MVCLS: ctx = comp ? kCtxMVClassY : kCtxMVClassX; break;
+      case P_CBP: ctx = kCtxCBP; break;
+      case P_TX: ctx = kCtxTxSize; break;
+      case P_LMODE: ctx = kCtxLumaMode; break;
+      case P_CMODE: ctx = kCtxChromaMode; break;
+      case P_TOKEN: ctx = dtoken_ctx(plane, static_cast<int>(tx), dband(si), prev); break;
+      case P_SIGN: case P_FSIGN: ctx = kCtxSign; break;
+      case P_ESCCLS: ctx = kCtxEscClass; break;
+      default: ctx = kCtxBypass; break;  // P_MVBITS, P_ESCBITS
+    }
+    int v = 0;
+    const uint32_t ub = ctx < 32 ? used0 : (ctx < 64 ? used1 : used2);
+    if (!((ub >> (ctx & 31)) & 1u)) {
+      err = kErrCorrupt | (kTileUnused << 8);
+    } else {
+      uint32_t x = s0;
+      const uint32_t slot = x & (kRansM - 1);
+      uint32_t sym, start, freq;
+      if (ctx == kCtxBypass) {  // fixed {2048, 2048} (rans.go:317-322)
+        sym = slot >> 11;
+        start = sym << 11;
+        freq = 2048;
+      } else {
+        // Lane-parallel slot2sym: lane s holds the start of symbol s; the
+        // symbol is the last start not above the slot.
+        const uint32_t c = cum[ctx * kCumStride + (lane < 16 ? lane : 16)];
+        const uint64_t m = __ballot(c <= slot);
+        sym = static_cast<uint32_t>(__popcll(m)) - 1u;
+        start = rd_lane(c, static_cast<int>(sym));
+        freq = rd_lane(c, static_cast<int>(sym) + 1) - start;
+      }
+      x = freq * (x >> kRansScaleBits) + slot - start;
+      while (x < kRansLower) {
+        if (pos == 0) {
+          err = kErrCorrupt | (kTileUnderrun << 8);
+          break;
+        }
+        --pos;
+        x = (x << 8) | win.byte(pos, lane);
+      }
+      s0 = s1; s1 = s2; s2 = s3; s3 = x;
+      nsym++;
+      v = static_cast<int>(sym);
+    }
+    if (err) {
+      // The failing macroblock keeps what Go wrote of it: a failed mode
+      // decode reads as skip (decode_inter.go:9-16); otherwise only the
+      // blocks and planes completed before the failure.
+      if (st == P_MBMODE) {
+        pred = predict_mv(prow, crow, mbx - td.x0, tw, mby == td.y0);
+        kind = 0; mv0 = pred; mvcur = pred; wmask = 0x3FFFFu;
+      }
+      st = P_ENDMB;
+      continue;
+    }
+
+    // ---------------- grammar transition ----------------
+    bool has_emit = false;
+    uint32_t entry = 0;
+    int32_t mag = 0;  // > 0: a magnitude was completed this step
+    int mvval = -1;   // >= 0: an MV class value was completed this step
+    switch (st) {
+      case P_MBMODE:
+        if (v == kMBSkip) {
+          pred = predict_mv(prow, crow, mbx - td.x0, tw, mby == td.y0);
+          kind = 0; mv0 = pred; mvcur = pred; wmask = 0x3FFFFu;
+          st = P_ENDMB;
+        } else if (v == kMBIntra) {
+          kind = 2; intra_mb = true; mvcur = 0;
+          st = P_TX;
+        } else {
+          kind = 1;
+          st = P_REF;
+        }
+        break;
+      case P_REF:
+        refsel = static_cast<uint32_t>(v);
+        pred = v == kRefGolden ? 0u : predict_mv(prow, crow, mbx - td.x0, tw, mby == td.y0);
+        st = P_PART;
+        break;
+      case P_PART:
+        part = static_cast<uint32_t>(v);
+        npart = c_part_count[v];
+        pi = 0; comp = 0;
+        st = P_MVCLS;
+        break;
+      case P_MVCLS:
+        if (v <= 1) mvval = v;
+        else { cls = v; cls_bits = v - 1; clsval = 0; st = P_MVBITS; }
+        break;
+      case P_MVBITS:
+        clsval = (clsval << 1) | static_cast<uint32_t>(v);
+        if (--cls_bits == 0) mvval = static_cast<int>((1u << (cls - 1)) | clsval);
+        break;
+      case P_CBP:
+        cbp = static_cast<uint32_t>(v);
+        if (cbp & kCBPLuma) {
+          st = P_TX;
+        } else {
+          wmask |= 0xFFFFu;
+          st = P_LUMADONE;
+        }
+        break;
+      case P_TX:
+        tx = static_cast<uint32_t>(v);
+        n = 4 << v; total = n * n; nb = (16 / n) * (16 / n);
+        plane = 0; blk = 0; si = 0; prev = 0; first = -1; last = -1; sum_abs = 0;
+        st = intra_mb ? P_LMODE : P_TOKEN;
+        break;
+      case P_LMODE:
+        if (blk < 8) lmode0 |= static_cast<uint32_t>(v) << (4 * blk);
+        else lmode1 |= static_cast<uint32_t>(v) << (4 * (blk - 8));
+        st = P_TOKEN;
+        break;
+      case P_CMODE:
+        cmode = static_cast<uint32_t>(v);
+        plane = 1; blk = 16; n = 8; total = 64;
+        si = 0; prev = 0; first = -1; last = -1; sum_abs = 0;
+        st = P_TOKEN;
+        break;
+      case P_TOKEN:
+        if (v == kTEOB) {
+          st = P_ENDBLK;
+        } else if (v == kTZero) {
+          prev = 0;
+          st = (++si == total) ? P_ENDBLK : P_TOKEN;
+        } else if (v == kTEscape) {
+          st = P_ESCCLS;
+        } else {
+          prev = v == 1 ? 1 : 2;
+          mag = v;
+        }
+        break;
+      case P_SIGN:
+        entry = coef_entry(c_zz[sidx_of(n)][si], blk, v ? -pend_mag : pend_mag);
+        has_emit = true;
+        last = si;
+        st = (++si == total) ? P_ENDBLK : P_TOKEN;
+        break;
+      case P_FSIGN:
+        entry = coef_entry(first_pos, blk, v ? -first_mag : first_mag);
+        has_emit = true;
+        st = P_BLKDONE;
+        break;
+      case P_ESCCLS:
+        if (v <= 1) { prev = 2; mag = 5 + v; }
+        else { cls = v; cls_bits = v - 1; clsval = 0; st = P_ESCBITS; }
+        break;
+      default:  // P_ESCBITS
+        clsval = (clsval << 1) | static_cast<uint32_t>(v);
+        if (--cls_bits == 0) { prev = 2; mag = 5 + static_cast<int32_t>((1u << (cls - 1)) | clsval); }
+        break;
+    }
+    if (mag > 0) {  // a coefficient magnitude is known (residual.go:99-117)
+      sum_abs += mag;
+      if (first < 0) {
+        first = si;
+        first_mag = mag;
+        first_pos = c_zz[sidx_of(n)][si];
+        last = si;
+        st = (++si == total) ? P_ENDBLK : P_TOKEN;
+      } else {
+        pend_mag = mag;
+        st = P_SIGN;
+      }
+    }
+    if (mvval >= 0) {  // an MV delta component is known (decode_inter.go:36-45)
+      const int d = zigzag_dec(static_cast<uint32_t>(mvval));
+      const uint32_t base = pi > 0 ? mv0 : pred;
+      const uint32_t nv = mv_set(mv_pick(mv0, mv1, mv2, mv3, pi), comp, clamp_mv(mv_comp(base, comp) + d));
+      if (pi == 0) mv0 = nv; else if (pi == 1) mv1 = nv; else if (pi == 2) mv2 = nv; else mv3 = nv;
+      comp ^= 1;
+      if (comp == 0 && ++pi == npart) {
+        mvcur = refsel == kRefGolden ? 0u : mv0;
+        st = P_CBP;
+      } else {
+        st = P_MVCLS;
+      }
+    }
+    if (has_emit) {
+      if (static_cast<uint32_t>(lane) == cn) cbuf = entry;
+      if (++cn == 64) { cdst[cbase + lane] = cbuf; cbase += 64; cn = 0; }
+      ncoef++;
+    }
   }
-  coef_flush(co, lane);
-  if (lane == 0) {
-    st.err = d.err;
-    st.mbs_done = done;
-    st.nsym = d.nsym;
-    st.n_intra = nintra;
-    status[t] = st;
-  }
+  if (static_cast<uint32_t>(lane) < cn) cdst[cbase + lane] = cbuf;
+  if (lane == 0) status[t] = TileStatus{err, done, nsym, nintra};
 }
 
 // ============================================================================
