@@ -17,6 +17,9 @@ EXPORTS = (
     "ebiv_codec_destroy", "ebiv_codec_last_error", "ebiv_codec_stats", "ebiv_frame_rgba",
     "ebiv_device_alloc", "ebiv_device_free", "ebiv_memcpy_h2d", "ebiv_memcpy_d2h", "ebiv_device_count",
     "ebiv_codec_set_timing", "ebiv_codec_kernel_times", "ebiv_codec_debug_dump",
+    "ebiv_batch_create", "ebiv_batch_destroy", "ebiv_batch_last_error", "ebiv_batch_open", "ebiv_batch_close",
+    "ebiv_batch_submit", "ebiv_batch_flush", "ebiv_batch_pending", "ebiv_batch_step", "ebiv_batch_stats",
+    "ebiv_batch_set_timing", "ebiv_batch_kernel_times", "ebiv_batch_frame_rgba", "ebiv_session_bytes",
 )
 
 
@@ -26,6 +29,11 @@ class Frame(C.Structure):
                 ("width", C.c_int32), ("height", C.c_int32),
                 ("cwidth", C.c_int32), ("cheight", C.c_int32),
                 ("timestamp_ns", C.c_int64), ("on_device", C.c_int32), ("ring_slot", C.c_int32)]
+
+
+class Completion(C.Structure):
+    _fields_ = [("session", C.c_int32), ("status", C.c_int32), ("tag", C.c_uint64), ("frame", Frame),
+                ("message", C.c_char * 96)]
 
 
 class KernelTimes(C.Structure):
@@ -72,4 +80,32 @@ def bind(L):
     L.ebiv_codec_debug_dump.argtypes = [vp, C.c_int, vp, sz, vp, sz, vp, sz]
     L.ebiv_codec_kernel_times.restype = C.c_int
     L.ebiv_codec_kernel_times.argtypes = [vp, C.POINTER(KernelTimes)]
+    L.ebiv_batch_create.restype = C.c_int
+    L.ebiv_batch_create.argtypes = [C.c_int, C.POINTER(vp)]
+    L.ebiv_batch_destroy.restype = None
+    L.ebiv_batch_destroy.argtypes = [vp]
+    L.ebiv_batch_last_error.restype = C.c_char_p
+    L.ebiv_batch_last_error.argtypes = [vp]
+    L.ebiv_batch_open.restype = C.c_int
+    L.ebiv_batch_open.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int)]
+    L.ebiv_batch_close.restype = C.c_int
+    L.ebiv_batch_close.argtypes = [vp, C.c_int]
+    L.ebiv_batch_submit.restype = C.c_int
+    L.ebiv_batch_submit.argtypes = [vp, C.c_int, C.c_char_p, vp, sz, C.c_int64, C.c_uint64]
+    L.ebiv_batch_flush.restype = C.c_int
+    L.ebiv_batch_flush.argtypes = [vp, C.c_int]
+    L.ebiv_batch_pending.restype = C.c_int
+    L.ebiv_batch_pending.argtypes = [vp, C.c_int]
+    L.ebiv_batch_step.restype = C.c_int
+    L.ebiv_batch_step.argtypes = [vp, C.POINTER(Completion), C.c_int, C.POINTER(C.c_int)]
+    L.ebiv_batch_stats.restype = C.c_int
+    L.ebiv_batch_stats.argtypes = [vp, C.c_int, C.POINTER(Stats)]
+    L.ebiv_batch_set_timing.restype = C.c_int
+    L.ebiv_batch_set_timing.argtypes = [vp, C.c_int]
+    L.ebiv_batch_kernel_times.restype = C.c_int
+    L.ebiv_batch_kernel_times.argtypes = [vp, C.POINTER(KernelTimes)]
+    L.ebiv_batch_frame_rgba.restype = C.c_int
+    L.ebiv_batch_frame_rgba.argtypes = [vp, C.POINTER(Frame), vp, C.c_int]
+    L.ebiv_session_bytes.restype = sz
+    L.ebiv_session_bytes.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
     return L

@@ -62,7 +62,7 @@ class Frame:
     are read back with ``planes()``.
     """
 
-    def __init__(self, codec: "Codec", raw: capi.Frame):
+    def __init__(self, codec, raw: capi.Frame):
         self._codec = codec
         self._raw = raw
         self.width = raw.width
@@ -106,14 +106,14 @@ class Frame:
     def rgba(self) -> np.ndarray:
         """Frame.RGBA (frame.go:85-91), converted on the GPU, as an (h, w, 4) array."""
         out = np.empty((self.height, self.width, 4), np.uint8)
-        rc = self._codec._L.ebiv_frame_rgba(self._codec._h, C.byref(self._raw), out.ctypes.data, 0)
+        rc = self._codec._rgba(self._raw, out.ctypes.data, 0)
         if rc:
             raise DeviceError(f"rgba conversion failed ({rc})")
         return out
 
     def rgba_device(self, dst_ptr: int) -> None:
         """Writes RGBA into device memory at dst_ptr (width*height*4 bytes)."""
-        rc = self._codec._L.ebiv_frame_rgba(self._codec._h, C.byref(self._raw), dst_ptr, 1)
+        rc = self._codec._rgba(self._raw, dst_ptr, 1)
         if rc:
             raise DeviceError(f"rgba conversion failed ({rc})")
 
@@ -198,6 +198,9 @@ class Codec:
         self._L.ebiv_codec_stats(self._h, C.byref(s))
         return {"frames": s.frames, "symbols": s.symbols, "tiles": s.tiles}
 
+    def _rgba(self, raw, dst, on_device):
+        return self._L.ebiv_frame_rgba(self._h, C.byref(raw), dst, on_device)
+
 
 class DeviceBuffer:
     """A block of HBM holding bytes (packets kept resident for decode_device)."""
@@ -226,3 +229,110 @@ class DeviceBuffer:
 
 def device_count() -> int:
     return int(native.gpu_lib().ebiv_device_count())
+
+
+@dataclass
+class Completion:
+    """One decoded (or failed) packet of a batch step."""
+    session: int
+    tag: int
+    status: int
+    frame: Frame | None
+    error: EbivError | None
+
+
+class Batch:
+    """Many ebiv.Codec sessions decoded together on one GPU.
+
+    ``open()`` makes a session (one stream's codec state), ``submit()`` queues
+    its packets, ``step()`` decodes up to ``window`` queued packets of every
+    session in one entropy launch plus one reconstruction launch per frame
+    position, and returns a Completion per packet in submission order per
+    session.  Frames are HBM planes valid until ``ring`` further frames of
+    their session complete.
+    """
+
+    def __init__(self, device: int = 0):
+        self._L = native.gpu_lib()
+        h = C.c_void_p()
+        rc = self._L.ebiv_batch_create(device, C.byref(h))
+        if rc:
+            raise DeviceError(f"ebiv_batch_create failed ({rc})")
+        self._h = h
+        self.device = device
+        self._out = (capi.Completion * 0)()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.ebiv_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def open(self, ring: int = 3, window: int = 1) -> int:
+        sid = C.c_int()
+        rc = self._L.ebiv_batch_open(self._h, ring, window, C.byref(sid))
+        if rc:
+            raise DeviceError(f"ebiv_batch_open failed ({rc})")
+        return sid.value
+
+    def close_session(self, session: int) -> None:
+        if self._L.ebiv_batch_close(self._h, session):
+            raise DeviceError("unknown session")
+
+    def submit(self, session: int, pkt, dev_ptr: int | None = None, tag: int = 0) -> None:
+        if isinstance(pkt, (bytes, bytearray)):
+            pkt = Packet(bytes(pkt))
+        data = bytes(pkt.data)
+        rc = self._L.ebiv_batch_submit(self._h, session, data, dev_ptr, len(data), int(pkt.timestamp), tag)
+        if rc:
+            raise DeviceError(f"ebiv_batch_submit failed ({rc})")
+
+    def flush(self, session: int) -> None:
+        """Codec.Flush for one session; its queued packets are dropped."""
+        if self._L.ebiv_batch_flush(self._h, session):
+            raise DeviceError("unknown session")
+
+    def pending(self, session: int = 0) -> int:
+        return int(self._L.ebiv_batch_pending(self._h, session))
+
+    def step(self, max_out: int = 4096) -> list:
+        if len(self._out) < max_out:
+            self._out = (capi.Completion * max_out)()
+        n = C.c_int()
+        rc = self._L.ebiv_batch_step(self._h, self._out, max_out, C.byref(n))
+        if rc:
+            raise DeviceError(f"{self._L.ebiv_batch_last_error(self._h).decode()} ({rc})")
+        res = []
+        for i in range(n.value):
+            c = self._out[i]
+            if c.status == capi.OK:
+                fr = capi.Frame()
+                C.pointer(fr)[0] = c.frame
+                res.append(Completion(c.session, c.tag, 0, Frame(self, fr), None))
+            else:
+                err = _ERRORS.get(c.status, DeviceError)(c.message.decode(errors="replace"))
+                res.append(Completion(c.session, c.tag, c.status, None, err))
+        return res
+
+    def stats(self, session: int = 0) -> dict:
+        s = capi.Stats()
+        self._L.ebiv_batch_stats(self._h, session, C.byref(s))
+        return {"frames": s.frames, "symbols": s.symbols, "tiles": s.tiles}
+
+    def set_timing(self, enabled: bool) -> None:
+        self._L.ebiv_batch_set_timing(self._h, int(enabled))
+
+    def kernel_times(self) -> dict:
+        t = capi.KernelTimes()
+        self._L.ebiv_batch_kernel_times(self._h, C.byref(t))
+        return {f: getattr(t, f) for f, _ in capi.KernelTimes._fields_}
+
+    def _rgba(self, raw, dst, on_device):
+        return self._L.ebiv_batch_frame_rgba(self._h, C.byref(raw), dst, on_device)
+
+
+def session_bytes(width: int, height: int, ring: int = 3, window: int = 1) -> int:
+    """HBM one batch session holds at this geometry."""
+    return int(native.gpu_lib().ebiv_session_bytes(width, height, ring, window))

@@ -86,6 +86,24 @@ struct Planes {
   int cw, ch;
 };
 
+// One coded payload in a launch: where its packet, descriptor and
+// entropy-kernel outputs live, and the frame buffers its reconstruction uses.
+struct FrameJob {
+  const uint8_t* pkt;
+  const uint8_t* desc;
+  MBRec* recs;
+  uint32_t* coefs;
+  TileStatus* status;
+  uint32_t pad[2];
+  Planes rec, ref, gold;
+};
+
+// One workgroup's share of a launch: job index and a tile or macroblock index.
+struct WorkItem {
+  uint32_t job;
+  uint32_t idx;
+};
+
 // Error detail codes of TileStatus::err >> 8 (low byte is the Status).
 enum TileErr { kTileOK = 0, kTileShort = 1, kTileUnused = 2, kTileUnderrun = 3 };
 

@@ -147,14 +147,19 @@ __device__ __forceinline__ uint32_t mv_pick(uint32_t m0, uint32_t m1, uint32_t m
   return i == 0 ? m0 : (i == 1 ? m1 : (i == 2 ? m2 : m3));
 }
 
-__global__ __launch_bounds__(64) void ebiv_parse(const uint8_t* __restrict__ pkt, const uint8_t* __restrict__ desc,
-                                                 MBRec* __restrict__ recs, uint32_t* __restrict__ coefs,
-                                                 TileStatus* __restrict__ status) {
+__global__ __launch_bounds__(64) void ebiv_parse(const FrameJob* __restrict__ jobs, const WorkItem* __restrict__ items) {
   extern __shared__ uint32_t smem[];
+  const WorkItem it = items[blockIdx.x];
+  const FrameJob& job = jobs[it.job];
+  const uint8_t* const pkt = job.pkt;
+  const uint8_t* const desc = job.desc;
+  MBRec* const recs = job.recs;
+  uint32_t* const coefs = job.coefs;
+  TileStatus* const status = job.status;
   const FrameHdr* hdr = reinterpret_cast<const FrameHdr*>(desc);
   const TileDesc* tiles = reinterpret_cast<const TileDesc*>(desc + desc_tiles_off());
   const int lane = lane_id();
-  const int t = blockIdx.x;
+  const int t = static_cast<int>(it.idx);
   const TileDesc td = tiles[t];
 
   // LDS: cumulative tables, then two MV rows of the tile width.
@@ -613,16 +618,21 @@ __device__ __forceinline__ int tile_of(const uint8_t* desc, const FrameHdr* hdr,
 // ============================================================================
 // Inter reconstruction: one wavefront per macroblock
 // ============================================================================
-__global__ __launch_bounds__(256) void ebiv_recon_inter(const uint8_t* __restrict__ desc, const MBRec* __restrict__ recs,
-                                                        const uint32_t* __restrict__ coefs,
-                                                        const TileStatus* __restrict__ status, Planes rec, Planes ref,
-                                                        Planes gold) {
+__global__ __launch_bounds__(256) void ebiv_recon_inter(const FrameJob* __restrict__ jobs,
+                                                        const WorkItem* __restrict__ items) {
   __shared__ WaveScratch scratch[4];
+  const WorkItem it = items[blockIdx.x];
+  const FrameJob& job = jobs[it.job];
+  const uint8_t* const desc = job.desc;
+  const MBRec* const recs = job.recs;
+  const uint32_t* const coefs = job.coefs;
+  const TileStatus* const status = job.status;
+  const Planes rec = job.rec, ref = job.ref, gold = job.gold;
   const FrameHdr* hdr = reinterpret_cast<const FrameHdr*>(desc);
   const TileDesc* tiles = reinterpret_cast<const TileDesc*>(desc + desc_tiles_off());
   const int lane = lane_id(), wave = threadIdx.x >> 6;
   const int mbc = hdr->mbc, nmb = mbc * hdr->mbr;
-  const int mb = blockIdx.x * 4 + wave;
+  const int mb = static_cast<int>(it.idx) + wave;
   if (mb >= nmb) return;
   const int mbx = mb % mbc, mby = mb / mbc;
   const int t = tile_of(desc, hdr, mbx, mby);
@@ -893,14 +903,20 @@ __device__ void intra_mb(IntraScratch& S, const uint32_t* __restrict__ coefs, co
 // One workgroup per tile.  Macroblock (x, y) depends on (x-1, y), (x-1, y-1),
 // (x, y-1) and (x+1, y-1), so step s = x + 2y (tile-relative) may run all its
 // macroblocks at once.
-__global__ __launch_bounds__(256) void ebiv_recon_intra(const uint8_t* __restrict__ desc, const MBRec* __restrict__ recs,
-                                                        const uint32_t* __restrict__ coefs,
-                                                        const TileStatus* __restrict__ status, Planes rec) {
+__global__ __launch_bounds__(256) void ebiv_recon_intra(const FrameJob* __restrict__ jobs,
+                                                        const WorkItem* __restrict__ items) {
   __shared__ IntraScratch scratch[4];
+  const WorkItem it = items[blockIdx.x];
+  const FrameJob& job = jobs[it.job];
+  const uint8_t* const desc = job.desc;
+  const MBRec* const recs = job.recs;
+  const uint32_t* const coefs = job.coefs;
+  const TileStatus* const status = job.status;
+  const Planes rec = job.rec;
   const FrameHdr* hdr = reinterpret_cast<const FrameHdr*>(desc);
   const TileDesc* tiles = reinterpret_cast<const TileDesc*>(desc + desc_tiles_off());
   const int lane = lane_id(), wave = threadIdx.x >> 6;
-  const int t = blockIdx.x;
+  const int t = static_cast<int>(it.idx);
   const TileDesc td = tiles[t];
   const TileStatus st = status[t];
   if (st.n_intra == 0) return;
@@ -982,20 +998,17 @@ hipError_t upload_tables(const int32_t dct[3][256], const uint8_t zz[3][256]) {
   return hipMemcpyToSymbol(HIP_SYMBOL(c_zz), zz, 3 * 256);
 }
 
-void launch_parse(hipStream_t s, int ntiles, int max_tile_w, const uint8_t* pkt, const uint8_t* desc, MBRec* recs,
-                  uint32_t* coefs, TileStatus* status) {
+void launch_parse(hipStream_t s, int nitems, int max_tile_w, const FrameJob* jobs, const WorkItem* items) {
   size_t lds = (72 * kCumStride / 2 + 2 * static_cast<size_t>(max_tile_w)) * sizeof(uint32_t);
-  hipLaunchKernelGGL(ebiv_parse, dim3(ntiles), dim3(64), lds, s, pkt, desc, recs, coefs, status);
+  hipLaunchKernelGGL(ebiv_parse, dim3(nitems), dim3(64), lds, s, jobs, items);
 }
 
-void launch_recon_inter(hipStream_t s, int nmb, const uint8_t* desc, const MBRec* recs, const uint32_t* coefs,
-                        const TileStatus* status, Planes rec, Planes ref, Planes gold) {
-  hipLaunchKernelGGL(ebiv_recon_inter, dim3((nmb + 3) / 4), dim3(256), 0, s, desc, recs, coefs, status, rec, ref, gold);
+void launch_recon_inter(hipStream_t s, int nitems, const FrameJob* jobs, const WorkItem* items) {
+  hipLaunchKernelGGL(ebiv_recon_inter, dim3(nitems), dim3(256), 0, s, jobs, items);
 }
 
-void launch_recon_intra(hipStream_t s, int ntiles, const uint8_t* desc, const MBRec* recs, const uint32_t* coefs,
-                        const TileStatus* status, Planes rec) {
-  hipLaunchKernelGGL(ebiv_recon_intra, dim3(ntiles), dim3(256), 0, s, desc, recs, coefs, status, rec);
+void launch_recon_intra(hipStream_t s, int nitems, const FrameJob* jobs, const WorkItem* items) {
+  hipLaunchKernelGGL(ebiv_recon_intra, dim3(nitems), dim3(256), 0, s, jobs, items);
 }
 
 void launch_rgba(hipStream_t s, const uint8_t* y, int ys, const uint8_t* cb, const uint8_t* cr, int cs, int w, int h,

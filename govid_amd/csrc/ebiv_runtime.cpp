@@ -5,21 +5,34 @@
 // the frame header (ebiv/format.go:376-409), the golden-override split
 // (ebiv/codec.go:162-173), the coded header, table and tile framing
 // (ebiv/frame.go:197-274, ebiv/rans.go:396-481) and the codec state machine
-// (ebiv/codec.go:70-153).  Everything that touches pixels or the rANS streams
-// runs on the device (ebiv_kernels.hip).
+// (ebiv/codec.go:70-207).  Everything that touches pixels or rANS streams runs
+// on the device (ebiv_kernels.hip).
 //
-// Frame buffers live in a device pool.  A decode writes a free buffer; on
-// success that buffer becomes both the ring slot's frame and the reference
-// for the next inter frame, so the per-frame visible copy (frame.go:67-77)
-// and the per-GOP golden copy (frame.go:42-46) of the reference are replaced
-// by pointer moves.  Go's observable state is kept exactly: ring slots are
-// consumed before decoding, failed decodes leave ref and the ring untouched,
-// and a failed golden override leaves golden partially rewritten.
+// Sessions and steps.  A session is one stream's codec state.  A step takes
+// queued packets of any number of sessions (up to a window of frames per
+// session), parses their headers and tables on the host in stream order,
+// then runs one entropy launch over every tile of every payload, and
+// reconstruction in rounds: round r reconstructs the r-th payload of every
+// session.  Frame buffers are assigned as if every payload decodes; the tile
+// statuses read back at the end of the step say whether that held, and where
+// it did not, the session's later payloads are reconstructed again against
+// the references Go would have used.  The plain Codec.Decode is a
+// one-session, one-frame step.
+//
+// Frame buffers live in a per-session pool.  A successful decode's buffer
+// becomes its ring slot's frame and the next reference, so the per-frame
+// visible copy (frame.go:67-77), the work/ref swap (codec.go:189) and the
+// per-GOP golden copy (codec.go:183) become pointer moves.  Go's observable
+// state is kept: ring slots are consumed before decoding, a failed decode
+// leaves ref and the ring alone, and a failed golden override leaves golden
+// partially rewritten.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <deque>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -31,12 +44,9 @@
 namespace ebiv {
 
 hipError_t upload_tables(const int32_t dct[3][256], const uint8_t zz[3][256]);
-void launch_parse(hipStream_t s, int ntiles, int max_tile_w, const uint8_t* pkt, const uint8_t* desc, MBRec* recs,
-                  uint32_t* coefs, TileStatus* status);
-void launch_recon_inter(hipStream_t s, int nmb, const uint8_t* desc, const MBRec* recs, const uint32_t* coefs,
-                        const TileStatus* status, Planes rec, Planes ref, Planes gold);
-void launch_recon_intra(hipStream_t s, int ntiles, const uint8_t* desc, const MBRec* recs, const uint32_t* coefs,
-                        const TileStatus* status, Planes rec);
+void launch_parse(hipStream_t s, int nitems, int max_tile_w, const FrameJob* jobs, const WorkItem* items);
+void launch_recon_inter(hipStream_t s, int nitems, const FrameJob* jobs, const WorkItem* items);
+void launch_recon_intra(hipStream_t s, int nitems, const FrameJob* jobs, const WorkItem* items);
 void launch_rgba(hipStream_t s, const uint8_t* y, int ys, const uint8_t* cb, const uint8_t* cr, int cs, int w, int h,
                  uint32_t* out);
 
@@ -77,22 +87,51 @@ const StaticTables& statics() {
 std::mutex g_init_mu;
 std::vector<bool> g_dev_ready;
 
-int ensure_device(int dev, std::string& msg) {
+int ensure_device(int dev) {
   std::lock_guard<std::mutex> lk(g_init_mu);
   int count = 0;
-  if (hipGetDeviceCount(&count) != hipSuccess || dev < 0 || dev >= count) {
-    msg = "no usable HIP device";
-    return EBIV_ERR_DEVICE;
-  }
+  if (hipGetDeviceCount(&count) != hipSuccess || dev < 0 || dev >= count) return EBIV_ERR_DEVICE;
   if (static_cast<int>(g_dev_ready.size()) < count) g_dev_ready.resize(count, false);
-  if (hipSetDevice(dev) != hipSuccess) { msg = "hipSetDevice failed"; return EBIV_ERR_DEVICE; }
+  if (hipSetDevice(dev) != hipSuccess) return EBIV_ERR_DEVICE;
   if (!g_dev_ready[dev]) {
     const StaticTables& t = statics();
-    if (upload_tables(t.dct, t.zz) != hipSuccess) { msg = "constant table upload failed"; return EBIV_ERR_DEVICE; }
+    if (upload_tables(t.dct, t.zz) != hipSuccess) return EBIV_ERR_DEVICE;
     g_dev_ready[dev] = true;
   }
   return EBIV_OK;
 }
+
+inline size_t align256(size_t v) { return (v + 255) & ~static_cast<size_t>(255); }
+
+template <typename T>
+struct DevArray {
+  T* d = nullptr;
+  size_t cap = 0;
+  bool reserve(size_t n) {
+    if (n <= cap) return true;
+    release();
+    if (hipMalloc(&d, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) { d = nullptr; return false; }
+    cap = n;
+    return true;
+  }
+  void release() { if (d) (void)hipFree(d); d = nullptr; cap = 0; }
+  ~DevArray() { release(); }
+};
+
+template <typename T>
+struct PinnedArray {
+  T* h = nullptr;
+  size_t cap = 0;
+  bool reserve(size_t n) {
+    if (n <= cap) return true;
+    release();
+    if (hipHostMalloc(&h, std::max<size_t>(n, 1) * sizeof(T), hipHostMallocDefault) != hipSuccess) { h = nullptr; return false; }
+    cap = n;
+    return true;
+  }
+  void release() { if (h) (void)hipHostFree(h); h = nullptr; cap = 0; }
+  ~PinnedArray() { release(); }
+};
 
 // prevFreqs slot: Go keeps the backing array across key-frame wipes
 // (rans.go:397-405, 473-481), so a re-grown slice shows its old values.
@@ -104,122 +143,88 @@ struct PrevFreq {
 
 struct DevBuf {
   uint8_t* base = nullptr;
+  size_t bytes = 0;
   Planes p{};
 };
 
-template <typename T>
-struct DevArray {
-  T* d = nullptr;
-  size_t cap = 0;
-  bool reserve(size_t n) {
-    if (n <= cap) return true;
-    if (d) hipFree(d);
-    d = nullptr;
-    cap = 0;
-    if (hipMalloc(&d, n * sizeof(T)) != hipSuccess) { d = nullptr; return false; }
-    cap = n;
-    return true;
-  }
-  void release() { if (d) hipFree(d); d = nullptr; cap = 0; }
-};
-
-template <typename T>
-struct PinnedArray {
-  T* h = nullptr;
-  size_t cap = 0;
-  bool reserve(size_t n) {
-    if (n <= cap) return true;
-    if (h) hipHostFree(h);
-    h = nullptr;
-    cap = 0;
-    if (hipHostMalloc(&h, n * sizeof(T), hipHostMallocDefault) != hipSuccess) { h = nullptr; return false; }
-    cap = n;
-    return true;
-  }
-  void release() { if (h) hipHostFree(h); h = nullptr; cap = 0; }
-};
-
-// One coded payload ready for the device: descriptor + launch geometry.
-struct CodedJob {
+// Device scratch of one coded payload in flight: its host descriptor, the
+// entropy kernel's macroblock records and coefficients, the tile statuses.
+struct PayloadSlot {
+  PinnedArray<uint8_t> h_desc;
+  size_t desc_bytes = 0;
+  size_t desc_off = 0;  // offset in the step's staging buffer
   int ntiles = 0;
   int max_tile_w = 0;
-  size_t desc_bytes = 0;
+  bool inter = false;
+  DevArray<MBRec> recs;
+  DevArray<uint32_t> coefs;
+  DevArray<TileStatus> d_status;
+  PinnedArray<TileStatus> h_status;
 };
 
 }  // namespace
-}  // namespace ebiv
 
-using namespace ebiv;
+// One frame of a session within a step.
+struct StepFrame {
+  int rc = EBIV_OK;
+  std::string msg;
+  bool device_work = false;  // a coded payload or a raw copy runs on the device
+  bool raw = false;
+  bool key = false, inter = false, has_ov = false;
+  int ring_slot = -1;
+  int main_slot = -1, ov_slot = -1;  // payload slots
+  int ov_rc = EBIV_OK;               // host-side failure of the override payload
+  std::string ov_msg;
+  std::vector<PrevFreq> saved_prev;  // table history before the override's tables
+  const uint8_t* host = nullptr;
+  const uint8_t* dev = nullptr;  // packet bytes on the device
+  size_t staged_off = 0;
+  bool staged = false;
+  size_t len = 0;
+  size_t body_off = 0;
+  int64_t ts = 0;
+  uint64_t tag = 0;
+  // buffer assignment
+  int target = -1, gtarget = -1, ref_in = -1, gold_in = -1;
+  int out_buf = -1;
+};
 
-struct ebiv_codec {
+struct Session {
+  int id = 0;
   int ring_size = 3;
-  int device = 0;
+  int window = 1;
   bool ready = false;
   int W = 0, H = 0, CW = 0, CH = 0, mbc = 0, mbr = 0;
   int ystride = 0, cstride = 0;  // host ring image strides (geometry.go:22-30)
   int next = 0;
   std::vector<DevBuf> pool;
   std::vector<int> ring;  // pool index per slot, -1 = never written
-  int ref = -1, golden = -1, work = -1;
+  int ref = -1, golden = -1;
   std::vector<uint8_t*> host_ring;
   PrevFreq prev[kNumContexts];
-  hipStream_t stream = nullptr;
-  DevArray<uint8_t> d_pkt;
-  PinnedArray<uint8_t> h_pkt;
-  DevArray<uint8_t> d_desc[2];
-  PinnedArray<uint8_t> h_desc[2];
-  DevArray<MBRec> d_recs[2];
-  DevArray<uint32_t> d_coefs[2];
-  DevArray<TileStatus> d_status[2];
-  PinnedArray<TileStatus> h_status[2];
-  DevArray<uint32_t> d_rgba;
+  std::vector<std::unique_ptr<PayloadSlot>> slots;
   std::string msg;
   ebiv_stats stats{0, 0, 0};
-  // Kernel timing: events bracket each launch while enabled.
-  bool timing = false;
-  hipEvent_t ev[2][4] = {};
-  int ev_used[2] = {0, 0};
-  ebiv_kernel_times ktimes{};
+  struct Queued {
+    std::vector<uint8_t> host;
+    const uint8_t* dev = nullptr;
+    int64_t ts = 0;
+    uint64_t tag = 0;
+  };
+  std::deque<Queued> queue;
+
+  ~Session() {
+    for (auto& b : pool) if (b.base) (void)hipFree(b.base);
+    for (auto* h : host_ring) if (h) (void)hipHostFree(h);
+  }
 
   int fail(int code, const std::string& m) {
     msg = m;
     return code;
   }
-  ~ebiv_codec() {
-    if (stream) hipStreamSynchronize(stream);
-    for (auto& b : pool) if (b.base) hipFree(b.base);
-    for (auto* h : host_ring) if (h) hipHostFree(h);
-    d_pkt.release();
-    h_pkt.release();
-    for (int i = 0; i < 2; i++) {
-      d_desc[i].release(); h_desc[i].release(); d_recs[i].release(); d_coefs[i].release();
-      d_status[i].release(); h_status[i].release();
-    }
-    d_rgba.release();
-    for (auto& row : ev)
-      for (auto& e : row)
-        if (e) hipEventDestroy(e);
-    if (stream) hipStreamDestroy(stream);
-  }
-
-  bool ensure_events() {
-    for (auto& row : ev)
-      for (auto& e : row)
-        if (!e && hipEventCreate(&e) != hipSuccess) return false;
-    return true;
-  }
-  // Adds the timed launches of slot `slot` (after the stream has synchronised).
-  void collect_times(int slot, bool inter) {
-    if (!timing || !ev_used[slot]) return;
-    float ms = 0;
-    if (hipEventElapsedTime(&ms, ev[slot][0], ev[slot][1]) == hipSuccess) { ktimes.parse_ms += ms; ktimes.parse_n++; }
-    if (inter && hipEventElapsedTime(&ms, ev[slot][1], ev[slot][2]) == hipSuccess) { ktimes.inter_ms += ms; ktimes.inter_n++; }
-    if (hipEventElapsedTime(&ms, ev[slot][2], ev[slot][3]) == hipSuccess) { ktimes.intra_ms += ms; ktimes.intra_n++; }
-    ev_used[slot] = 0;
-  }
 
   // configure, codec.go:129-153.
-  int configure(int w, int h) {
+  int configure(int w, int h, hipStream_t stream) {
     if (static_cast<long>(w) * h > kMaxDecodePixels)
       return fail(EBIV_ERR_UNSUPPORTED, "ebiv: unsupported stream feature: frame exceeds the decoder's pixel bound");
     if (ready) {
@@ -232,41 +237,43 @@ struct ebiv_codec {
     ystride = align_up(W, 64);
     cstride = align_up(CW, 64);
     const int pw = mbc * kMB, ph = mbr * kMB, cw = mbc * kCMB, ch = mbr * kCMB;
-    const size_t ysz = static_cast<size_t>(pw) * ph, csz = static_cast<size_t>(align_up(cw * ch, 256));
-    pool.assign(static_cast<size_t>(ring_size) + 4, DevBuf{});
+    const size_t ysz = static_cast<size_t>(pw) * ph, csz = align256(static_cast<size_t>(cw) * ch);
+    // Live at a step's start: ring images, ref and golden.  Written by a
+    // step: one buffer per frame of the window and a golden-override target.
+    const size_t npool = static_cast<size_t>(ring_size) + 2 + static_cast<size_t>(window) + 1;
+    pool.assign(npool, DevBuf{});
     for (auto& b : pool) {
-      if (hipMalloc(&b.base, ysz + 2 * csz) != hipSuccess) return fail(EBIV_ERR_DEVICE, "frame pool allocation failed");
-      if (hipMemsetAsync(b.base, 0, ysz + 2 * csz, stream) != hipSuccess) return fail(EBIV_ERR_DEVICE, "frame pool clear failed");
+      b.bytes = ysz + 2 * csz;
+      if (hipMalloc(&b.base, b.bytes) != hipSuccess) { b.base = nullptr; return fail(EBIV_ERR_DEVICE, "frame pool allocation failed"); }
+      if (hipMemsetAsync(b.base, 0, b.bytes, stream) != hipSuccess) return fail(EBIV_ERR_DEVICE, "frame pool clear failed");
       b.p = Planes{b.base, b.base + ysz, b.base + ysz + csz, pw, ph, cw, ch};
     }
     ring.assign(ring_size, -1);
-    ref = 0; golden = 1; work = 2;
-    host_ring.assign(ring_size, nullptr);
-    const size_t hsz = static_cast<size_t>(ystride) * H + 2 * static_cast<size_t>(cstride) * CH;
-    for (auto*& hr : host_ring) {
-      if (hipHostMalloc(reinterpret_cast<void**>(&hr), hsz, hipHostMallocDefault) != hipSuccess)
-        return fail(EBIV_ERR_DEVICE, "host ring allocation failed");
-      std::memset(hr, 0, hsz);
-    }
-    const size_t nmb = static_cast<size_t>(mbc) * mbr;
-    for (int i = 0; i < 2; i++) {
-      if (!d_recs[i].reserve(nmb) || !d_coefs[i].reserve(nmb * kCoefPerMB))
-        return fail(EBIV_ERR_DEVICE, "macroblock buffer allocation failed");
-    }
+    ref = 0;
+    golden = 1;
     if (hipStreamSynchronize(stream) != hipSuccess) return fail(EBIV_ERR_DEVICE, "frame pool clear failed");
     next = 0;
     ready = true;
     return EBIV_OK;
   }
 
-  bool referenced(int b, int extra = -1) const {
-    if (b == ref || b == golden || b == work || b == extra) return true;
-    for (int r : ring) if (r == b) return true;
-    return false;
+  int ensure_host_ring() {
+    if (!host_ring.empty()) return EBIV_OK;
+    host_ring.assign(ring_size, nullptr);
+    const size_t hsz = static_cast<size_t>(ystride) * H + 2 * static_cast<size_t>(cstride) * CH;
+    for (auto*& hr : host_ring) {
+      if (hipHostMalloc(reinterpret_cast<void**>(&hr), hsz, hipHostMallocDefault) != hipSuccess) {
+        hr = nullptr;
+        return fail(EBIV_ERR_DEVICE, "host ring allocation failed");
+      }
+      std::memset(hr, 0, hsz);
+    }
+    return EBIV_OK;
   }
-  int free_buffer(int extra = -1) const {
-    for (int i = 0; i < static_cast<int>(pool.size()); i++) if (!referenced(i, extra)) return i;
-    return -1;
+
+  PayloadSlot* slot(int i) {
+    while (static_cast<int>(slots.size()) <= i) slots.emplace_back(new PayloadSlot());
+    return slots[i].get();
   }
 
   // parseTablesInto (rans.go:396-471) into the frame header's cumulative
@@ -342,10 +349,9 @@ struct ebiv_codec {
   }
 
   // decodeState.decodeCoded up to the tile fan-out (decode_intra.go:55-64,
-  // frame.go:197-274): fills the host descriptor of slot `slot`.  body is a
+  // frame.go:197-274): fills payload slot `si`'s descriptor.  body is a
   // sub-range of the packet starting at pkt_base.
-  int prepare_coded(const uint8_t* pkt_base, const uint8_t* body, size_t len, bool inter, bool key, int slot,
-                    CodedJob& job) {
+  int prepare_coded(const uint8_t* pkt_base, const uint8_t* body, size_t len, bool inter, bool key, int si) {
     if (len < 1) return fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: coded header truncated");
     const int qp = body[0];
     size_t pos = 1;
@@ -365,10 +371,11 @@ struct ebiv_codec {
 
     const int gc = static_cast<int>(std::min<uint64_t>(cols, static_cast<uint64_t>(mbc)));
     const int gr = static_cast<int>(std::min<uint64_t>(rows, static_cast<uint64_t>(mbr)));
-    const int ntiles_grid = gc * gr;
-    const size_t dsize = desc_size(ntiles_grid, mbc, mbr);
-    if (!h_desc[slot].reserve(dsize) || !d_desc[slot].reserve(dsize)) return fail(EBIV_ERR_DEVICE, "descriptor allocation failed");
-    uint8_t* hd = h_desc[slot].h;
+    const int nt = gc * gr;
+    const size_t dsize = desc_size(nt, mbc, mbr);
+    PayloadSlot* ps = slot(si);
+    if (!ps->h_desc.reserve(dsize)) return fail(EBIV_ERR_DEVICE, "descriptor allocation failed");
+    uint8_t* hd = ps->h_desc.h;
     std::memset(hd, 0, dsize);
     FrameHdr& hdr = *reinterpret_cast<FrameHdr*>(hd);
     size_t consumed = 0;
@@ -388,23 +395,22 @@ struct ebiv_codec {
       tiles.emplace_back(static_cast<size_t>(body + pos - pkt_base), static_cast<size_t>(l));
       pos += static_cast<size_t>(l);
     }
-    if (static_cast<uint64_t>(ntiles_grid) != tiles.size()) return fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream");
+    if (static_cast<uint64_t>(nt) != tiles.size()) return fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream");
 
     hdr.mbc = mbc;
     hdr.mbr = mbr;
-    hdr.ntiles = ntiles_grid;
+    hdr.ntiles = nt;
     hdr.inter = inter ? 1 : 0;
-    const int q = qp;
-    hdr.qac = statics().qstep[q];
-    hdr.qdc = q >= 6 ? statics().qstep[q - 6] : statics().qstep[q];
+    hdr.qac = statics().qstep[qp];
+    hdr.qdc = qp >= 6 ? statics().qstep[qp - 6] : statics().qstep[qp];
     hdr.grid_cols = gc;
     hdr.grid_rows = gr;
     TileDesc* td = reinterpret_cast<TileDesc*>(hd + desc_tiles_off());
-    uint16_t* ct = reinterpret_cast<uint16_t*>(hd + desc_coltile_off(ntiles_grid));
-    uint16_t* rt = reinterpret_cast<uint16_t*>(hd + desc_rowtile_off(ntiles_grid, mbc));
+    uint16_t* ct = reinterpret_cast<uint16_t*>(hd + desc_coltile_off(nt));
+    uint16_t* rt = reinterpret_cast<uint16_t*>(hd + desc_rowtile_off(nt, mbc));
     uint32_t coef = 0;
     int maxw = 1;
-    for (int i = 0; i < ntiles_grid; i++) {  // tileGrid.bounds, frame.go:170-177
+    for (int i = 0; i < nt; i++) {  // tileGrid.bounds, frame.go:170-177
       const int tc = i % gc, tr = i / gc;
       TileDesc& t = td[i];
       t.byte_off = static_cast<uint32_t>(tiles[i].first);
@@ -423,47 +429,25 @@ struct ebiv_codec {
     for (int tr = 0; tr < gr; tr++)
       for (int y = mbr * tr / gr; y < mbr * (tr + 1) / gr; y++) rt[y] = static_cast<uint16_t>(tr);
     hdr.max_tile_w = maxw;
-    job.ntiles = ntiles_grid;
-    job.max_tile_w = maxw;
-    job.desc_bytes = dsize;
-    if (!d_status[slot].reserve(ntiles_grid) || !h_status[slot].reserve(ntiles_grid))
-      return fail(EBIV_ERR_DEVICE, "status allocation failed");
-    return EBIV_OK;
-  }
-
-  // Queues the device decode of a prepared payload into `rec`.
-  int launch(int slot, const CodedJob& job, const uint8_t* dpkt, bool inter, const Planes& rec, const Planes& r,
-             const Planes& g) {
-    if (hipMemcpyAsync(d_desc[slot].d, h_desc[slot].h, job.desc_bytes, hipMemcpyHostToDevice, stream) != hipSuccess)
-      return fail(EBIV_ERR_DEVICE, "descriptor upload failed");
-    const bool tm = timing && ensure_events();
-    if (tm) hipEventRecord(ev[slot][0], stream);
-    launch_parse(stream, job.ntiles, job.max_tile_w, dpkt, d_desc[slot].d, d_recs[slot].d, d_coefs[slot].d, d_status[slot].d);
-    if (tm) hipEventRecord(ev[slot][1], stream);
-    if (inter)
-      launch_recon_inter(stream, mbc * mbr, d_desc[slot].d, d_recs[slot].d, d_coefs[slot].d, d_status[slot].d, rec, r, g);
-    if (tm) hipEventRecord(ev[slot][2], stream);
-    launch_recon_intra(stream, job.ntiles, d_desc[slot].d, d_recs[slot].d, d_coefs[slot].d, d_status[slot].d, rec);
-    if (tm) {
-      hipEventRecord(ev[slot][3], stream);
-      ev_used[slot] = 1;
-    }
-    if (hipMemcpyAsync(h_status[slot].h, d_status[slot].d, sizeof(TileStatus) * job.ntiles, hipMemcpyDeviceToHost,
-                       stream) != hipSuccess)
-      return fail(EBIV_ERR_DEVICE, "status download failed");
-    if (hipGetLastError() != hipSuccess) return fail(EBIV_ERR_DEVICE, "kernel launch failed");
+    ps->ntiles = nt;
+    ps->max_tile_w = maxw;
+    ps->desc_bytes = dsize;
+    ps->inter = inter;
+    const size_t nmb = static_cast<size_t>(mbc) * mbr;
+    if (!ps->recs.reserve(nmb) || !ps->coefs.reserve(nmb * kCoefPerMB) || !ps->d_status.reserve(nt) ||
+        !ps->h_status.reserve(nt))
+      return fail(EBIV_ERR_DEVICE, "payload buffer allocation failed");
     return EBIV_OK;
   }
 
   // First failing tile in tile order (decode_intra.go:86-93).
-  int tile_result(int slot, const CodedJob& job) {
-    for (int i = 0; i < job.ntiles; i++) {
-      const TileStatus& s = h_status[slot].h[i];
-      stats.symbols += s.nsym;
+  int tile_result(const PayloadSlot* ps) {
+    for (int i = 0; i < ps->ntiles; i++) {
+      stats.symbols += ps->h_status.h[i].nsym;
       stats.tiles++;
     }
-    for (int i = 0; i < job.ntiles; i++) {
-      const TileStatus& s = h_status[slot].h[i];
+    for (int i = 0; i < ps->ntiles; i++) {
+      const TileStatus& s = ps->h_status.h[i];
       if (s.err) {
         switch (s.err >> 8) {
           case kTileShort: return fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: tile stream too short to initialize rANS");
@@ -475,102 +459,51 @@ struct ebiv_codec {
     return EBIV_OK;
   }
 
-  void fill_frame(int b, ebiv_frame* out, int64_t ts, bool on_device, int slot) {
-    const DevBuf& f = pool[b];
-    out->width = W; out->height = H; out->cwidth = CW; out->cheight = CH;
-    out->timestamp_ns = ts;
-    out->ring_slot = slot;
-    if (on_device) {
-      out->y = f.p.y; out->cb = f.p.cb; out->cr = f.p.cr;
-      out->y_stride = f.p.pw; out->c_stride = f.p.cw;
-      out->on_device = 1;
-    } else {
-      uint8_t* h = host_ring[slot];
-      out->y = h;
-      out->cb = h + static_cast<size_t>(ystride) * H;
-      out->cr = out->cb + static_cast<size_t>(cstride) * CH;
-      out->y_stride = ystride; out->c_stride = cstride;
-      out->on_device = 0;
-    }
+  // Host half of Codec.Decode up to the device work (codec.go:70-100,
+  // 162-173): header, configure, ring slot, raw check, override split and
+  // payload preparation.  Payload slots are numbered from *next_slot.
+  // Returns false when the frame must close the session's window for this
+  // step: the override's tables are parsed only if its key frame decodes.
+  bool prepare_frame(StepFrame& f, int* next_slot, hipStream_t stream) {
+    const bool more = prepare_frame_inner(f, next_slot, stream);
+    if (f.rc && f.msg.empty()) f.msg = msg;
+    return more;
   }
 
-  // storeImage into the host ring (frame.go:67-77): visible region only.
-  int download(int b, int slot) {
-    const DevBuf& f = pool[b];
-    uint8_t* h = host_ring[slot];
-    uint8_t* hcb = h + static_cast<size_t>(ystride) * H;
-    uint8_t* hcr = hcb + static_cast<size_t>(cstride) * CH;
-    if (hipMemcpy2DAsync(h, ystride, f.p.y, f.p.pw, W, H, hipMemcpyDeviceToHost, stream) != hipSuccess ||
-        hipMemcpy2DAsync(hcb, cstride, f.p.cb, f.p.cw, CW, CH, hipMemcpyDeviceToHost, stream) != hipSuccess ||
-        hipMemcpy2DAsync(hcr, cstride, f.p.cr, f.p.cw, CW, CH, hipMemcpyDeviceToHost, stream) != hipSuccess)
-      return fail(EBIV_ERR_DEVICE, "frame download failed");
-    if (hipStreamSynchronize(stream) != hipSuccess) return fail(EBIV_ERR_DEVICE, "frame download failed");
-    return EBIV_OK;
-  }
-
-  // Codec.Decode, codec.go:70-111.
-  int decode(const uint8_t* data, const uint8_t* dev_data, size_t len, int64_t ts, ebiv_frame* out) {
-    if (!data && len) return fail(EBIV_ERR_ARGUMENT, "null packet");
-    if (hipSetDevice(device) != hipSuccess) return fail(EBIV_ERR_DEVICE, "hipSetDevice failed");
-    // parseFrameHeader, format.go:376-409
-    if (len < 1) return fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: empty frame payload");
+  bool prepare_frame_inner(StepFrame& f, int* next_slot, hipStream_t stream) {
+    const uint8_t* data = f.host;
+    const size_t len = f.len;
+    if (len < 1) { f.rc = fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: empty frame payload"); return true; }
     const int type = data[0] & 3, coding = (data[0] >> 2) & 7;
     const bool gov = ((data[0] >> 5) & 1) != 0;
-    if (type != 0 && type != 1) return fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: frame type");
-    if (gov && type != 0) return fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: golden override on a non-key frame");
+    if (type != 0 && type != 1) { f.rc = fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: frame type"); return true; }
+    if (gov && type != 0) { f.rc = fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: golden override on a non-key frame"); return true; }
     size_t hlen = 1;
     if (type == 0) {
-      if (len < 6) return fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: key frame header truncated");
+      if (len < 6) { f.rc = fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: key frame header truncated"); return true; }
       const int w = data[1] | data[2] << 8, h = data[3] | data[4] << 8;
-      if (w == 0 || h == 0) return fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: key frame dimensions");
-      if ((data[5] >> 4) != 0) return fail(EBIV_ERR_UNSUPPORTED, "ebiv: unsupported stream feature: chroma format");
-      if ((data[5] & 15) != 0) return fail(EBIV_ERR_UNSUPPORTED, "ebiv: unsupported stream feature: bit depth code");
+      if (w == 0 || h == 0) { f.rc = fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: key frame dimensions"); return true; }
+      if ((data[5] >> 4) != 0) { f.rc = fail(EBIV_ERR_UNSUPPORTED, "ebiv: unsupported stream feature: chroma format"); return true; }
+      if ((data[5] & 15) != 0) { f.rc = fail(EBIV_ERR_UNSUPPORTED, "ebiv: unsupported stream feature: bit depth code"); return true; }
       hlen = 6;
-      const int rc = configure(w, h);
-      if (rc) return rc;
+      if ((f.rc = configure(w, h, stream))) return true;
     } else if (!ready) {
-      return fail(EBIV_ERR_NOKEYFRAME, "ebiv: inter frame before any key frame");
+      f.rc = fail(EBIV_ERR_NOKEYFRAME, "ebiv: inter frame before any key frame");
+      return true;
     }
-    const int slot = next;
+    f.ring_slot = next;
     if (++next == ring_size) next = 0;
-    const bool on_device = dev_data != nullptr;
-
-    // The packet on the device (the kernels read tile bytes from it).
-    const uint8_t* dpkt = dev_data;
-    if (!on_device && (coding == 1 || coding == 2 || coding == 0)) {
-      if (!d_pkt.reserve(len + 256) || !h_pkt.reserve(len + 256)) return fail(EBIV_ERR_DEVICE, "packet buffer allocation failed");
-      std::memcpy(h_pkt.h, data, len);
-      if (hipMemcpyAsync(d_pkt.d, h_pkt.h, len, hipMemcpyHostToDevice, stream) != hipSuccess)
-        return fail(EBIV_ERR_DEVICE, "packet upload failed");
-      dpkt = d_pkt.d;
-    }
-
+    f.key = type == 0;
     if (coding == 0) {  // decodeRaw, codec.go:195-207
       const size_t want = static_cast<size_t>(W) * H + 2 * static_cast<size_t>(CW) * CH;
-      if (len - hlen != want) return fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: raw frame size");
-      const int b = free_buffer();
-      const DevBuf& f = pool[b];
-      const uint8_t* src = dpkt + hlen;
-      if (hipMemcpy2DAsync(f.p.y, f.p.pw, src, W, W, H, hipMemcpyDeviceToDevice, stream) != hipSuccess ||
-          hipMemcpy2DAsync(f.p.cb, f.p.cw, src + static_cast<size_t>(W) * H, CW, CW, CH, hipMemcpyDeviceToDevice, stream) != hipSuccess ||
-          hipMemcpy2DAsync(f.p.cr, f.p.cw, src + static_cast<size_t>(W) * H + static_cast<size_t>(CW) * CH, CW, CW, CH,
-                           hipMemcpyDeviceToDevice, stream) != hipSuccess)
-        return fail(EBIV_ERR_DEVICE, "raw frame copy failed");
-      ring[slot] = b;
-      if (!on_device) {
-        int rc = download(b, slot);
-        if (rc) return rc;
-      } else if (hipStreamSynchronize(stream) != hipSuccess) {
-        return fail(EBIV_ERR_DEVICE, "raw frame copy failed");
-      }
-      stats.frames++;
-      fill_frame(b, out, ts, on_device, slot);
-      return EBIV_OK;
+      if (len - hlen != want) { f.rc = fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: raw frame size"); return true; }
+      f.raw = true;
+      f.device_work = true;
+      f.body_off = hlen;
+      return true;
     }
-    if (coding != 1 && coding != 2) return fail(EBIV_ERR_UNSUPPORTED, "ebiv: unsupported stream feature: coding mode");
-
-    // Codec.decodeCoded, codec.go:162-192.
-    const bool inter = coding == 2, key = type == 0;
+    if (coding != 1 && coding != 2) { f.rc = fail(EBIV_ERR_UNSUPPORTED, "ebiv: unsupported stream feature: coding mode"); return true; }
+    f.inter = coding == 2;
     const uint8_t* body = data + hlen;
     size_t blen = len - hlen;
     const uint8_t* ov = nullptr;
@@ -578,134 +511,569 @@ struct ebiv_codec {
     if (gov) {
       uint64_t kl = 0;
       const int k = uvarint(body, blen, &kl);
-      if (k <= 0 || kl > static_cast<uint64_t>(blen - static_cast<size_t>(k)))
-        return fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: golden override split");
+      if (k <= 0 || kl > static_cast<uint64_t>(blen - static_cast<size_t>(k))) {
+        f.rc = fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: golden override split");
+        return true;
+      }
       ov = body + k + kl;
       ovlen = blen - static_cast<size_t>(k) - static_cast<size_t>(kl);
       body += k;
       blen = static_cast<size_t>(kl);
     }
-    CodedJob job;
-    int rc = prepare_coded(data, body, blen, inter, key, 0, job);
-    if (rc) return rc;
-    const Planes& wp = pool[work].p;
-    rc = launch(0, job, dpkt, inter, wp, pool[ref].p, pool[golden].p);
-    if (rc) return rc;
-
-    // The override is decoded right behind the key frame; its table parse is
-    // only kept if the key frame itself decodes (Go parses it afterwards).
-    CodedJob ovjob;
-    int gtarget = -1;
-    PrevFreq saved[kNumContexts];
-    bool ov_prepared = false;
-    int ov_rc = EBIV_OK;
-    std::string ov_msg;
-    if (key && ov) {
-      std::memcpy(saved, prev, sizeof prev);
-      ov_rc = prepare_coded(data, ov, ovlen, true, false, 1, ovjob);
-      ov_msg = msg;
-      if (ov_rc == EBIV_OK) {
-        // Decode into a fresh buffer holding a copy of the old golden: if the
-        // key frame fails it is dropped (Go never reaches the override), and
-        // if the override fails it is exactly Go's partially rewritten golden.
-        gtarget = free_buffer();
-        const DevBuf& src = pool[golden];
-        const DevBuf& dst = pool[gtarget];
-        const size_t bytes = static_cast<size_t>(src.p.pw) * src.p.ph + 2 * static_cast<size_t>(align_up(src.p.cw * src.p.ch, 256));
-        if (hipMemcpyAsync(dst.base, src.base, bytes, hipMemcpyDeviceToDevice, stream) != hipSuccess)
-          return fail(EBIV_ERR_DEVICE, "golden copy failed");
-        rc = launch(1, ovjob, dpkt, true, pool[gtarget].p, wp, wp);
-        if (rc) return rc;
-        ov_prepared = true;
-      }
+    if ((f.rc = prepare_coded(data, body, blen, f.inter, f.key, *next_slot))) return true;
+    f.main_slot = (*next_slot)++;
+    f.device_work = true;
+    if (f.key && ov) {
+      f.has_ov = true;
+      f.saved_prev.assign(prev, prev + kNumContexts);
+      f.ov_rc = prepare_coded(data, ov, ovlen, true, false, *next_slot);
+      if (f.ov_rc) f.ov_msg = msg;
+      else f.ov_slot = (*next_slot)++;
+      return false;
     }
-    if (hipStreamSynchronize(stream) != hipSuccess) return fail(EBIV_ERR_DEVICE, "decode failed on the device");
-    collect_times(0, inter);
-    collect_times(1, true);
-    rc = tile_result(0, job);
-    if (rc) {
-      if (key && ov) std::memcpy(prev, saved, sizeof prev);
-      return rc;
-    }
-    if (key && ov) {
-      if (ov_rc) return fail(ov_rc, ov_msg);
-      if (ov_prepared) {
-        golden = gtarget;  // on failure golden keeps the partial rewrite
-        rc = tile_result(1, ovjob);
-        if (rc) return rc;
-      }
-    } else if (key) {
-      golden = work;  // golden.copyFrom(work) by sharing the buffer
-    }
-    ring[slot] = work;
-    ref = work;
-    work = -1;
-    work = free_buffer();
-    stats.frames++;
-    if (!on_device) {
-      rc = download(ref, slot);
-      if (rc) return rc;
-    }
-    fill_frame(ref, out, ts, on_device, slot);
-    return EBIV_OK;
+    return true;
   }
 };
+
+namespace {
+
+// Assigns a session's frames their target buffers and inputs by walking them
+// in stream order as Go's state machine does (codec.go:174-190).  ok_main /
+// ok_ov are each payload's outcome (all true on the speculative pass).
+// Targets are never buffers live at the step's start, nor each other.
+bool assign_buffers(const Session& s, std::vector<StepFrame>& fr, const std::vector<char>& ok_main,
+                    const std::vector<char>& ok_ov, std::vector<int>& taken) {
+  auto live = [&](int b) {
+    if (b == s.ref || b == s.golden) return true;
+    for (int r : s.ring) if (r == b) return true;
+    for (int t : taken) if (t == b) return true;
+    return false;
+  };
+  auto pick = [&]() {
+    for (int i = 0; i < static_cast<int>(s.pool.size()); i++)
+      if (!live(i)) { taken.push_back(i); return i; }
+    return -1;
+  };
+  int ref = s.ref, golden = s.golden;
+  for (size_t k = 0; k < fr.size(); k++) {
+    StepFrame& f = fr[k];
+    if (f.rc || !f.device_work) continue;
+    if (f.target < 0 && (f.target = pick()) < 0) return false;
+    if (f.raw) continue;  // a raw frame writes its ring image only
+    f.ref_in = ref;
+    f.gold_in = golden;
+    if (f.ov_slot >= 0 && f.gtarget < 0 && (f.gtarget = pick()) < 0) return false;
+    if (!ok_main[k]) continue;
+    if (f.key) {
+      if (f.has_ov) {
+        if (f.ov_rc) continue;
+        golden = f.gtarget;  // decoded, or partially rewritten on failure
+        if (!ok_ov[k]) continue;
+      } else {
+        golden = f.target;
+      }
+    }
+    ref = f.target;
+  }
+  return true;
+}
+
+}  // namespace
+
+struct Engine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<std::unique_ptr<Session>> sessions;
+  int next_id = 1;
+  PinnedArray<uint8_t> h_stage;  // descriptors and host packets of a step
+  DevArray<uint8_t> d_stage;
+  PinnedArray<FrameJob> h_jobs;
+  DevArray<FrameJob> d_jobs;
+  PinnedArray<WorkItem> h_items;
+  DevArray<WorkItem> d_items;
+  DevArray<uint32_t> d_rgba;
+  std::string msg;
+  bool timing = false;
+  std::vector<hipEvent_t> evs;
+  std::vector<int> marks;  // kind per event pair
+  ebiv_kernel_times ktimes{};
+
+  ~Engine() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    sessions.clear();
+    for (auto e : evs) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  int fail(int code, const std::string& m) {
+    msg = m;
+    return code;
+  }
+
+  Session* find(int id) {
+    for (auto& s : sessions) if (s->id == id) return s.get();
+    return nullptr;
+  }
+
+  // Brackets a launch of kind k (0 parse, 1 inter, 2 intra, 3 rgba) with
+  // events while timing is on; returns the pair index or -1.
+  int mark_begin(int kind) {
+    if (!timing) return -1;
+    const size_t i = marks.size();
+    while (evs.size() < 2 * i + 2) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return -1;
+      evs.push_back(e);
+    }
+    if (hipEventRecord(evs[2 * i], stream) != hipSuccess) return -1;
+    marks.push_back(kind);
+    return static_cast<int>(i);
+  }
+  void mark_end(int m) {
+    if (m >= 0) (void)hipEventRecord(evs[2 * static_cast<size_t>(m) + 1], stream);
+  }
+  void collect_marks() {
+    for (size_t i = 0; i < marks.size(); i++) {
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, evs[2 * i], evs[2 * i + 1]) != hipSuccess) continue;
+      switch (marks[i]) {
+        case 0: ktimes.parse_ms += ms; ktimes.parse_n++; break;
+        case 1: ktimes.inter_ms += ms; ktimes.inter_n++; break;
+        case 2: ktimes.intra_ms += ms; ktimes.intra_n++; break;
+        default: ktimes.rgba_ms += ms; ktimes.rgba_n++; break;
+      }
+    }
+    marks.clear();
+  }
+
+  int run_step(std::vector<Session*>& ss, std::vector<std::vector<StepFrame>>& frames);
+};
+
+namespace {
+
+struct JobRef {
+  size_t si, fi;
+  bool ov;
+};
+
+PayloadSlot* payload_of(std::vector<Session*>& ss, std::vector<std::vector<StepFrame>>& frames, const JobRef& r) {
+  const StepFrame& f = frames[r.si][r.fi];
+  return ss[r.si]->slot(r.ov ? f.ov_slot : f.main_slot);
+}
+
+}  // namespace
+
+int Engine::run_step(std::vector<Session*>& ss, std::vector<std::vector<StepFrame>>& frames) {
+  // ---- staging: host packets and descriptors, one upload
+  size_t stage = 0;
+  for (size_t i = 0; i < ss.size(); i++)
+    for (auto& f : frames[i]) {
+      if (f.rc || !f.device_work) continue;
+      if (!f.dev) {
+        f.staged_off = stage;
+        f.staged = true;
+        stage = align256(stage + f.len + 256);  // the entropy kernel reads 256-byte windows
+      }
+      for (int sl : {f.main_slot, f.ov_slot}) {
+        if (sl < 0) continue;
+        PayloadSlot* ps = ss[i]->slot(sl);
+        ps->desc_off = stage;
+        stage = align256(stage + ps->desc_bytes);
+      }
+    }
+  if (stage) {
+    if (!h_stage.reserve(stage) || !d_stage.reserve(stage)) return fail(EBIV_ERR_DEVICE, "staging allocation failed");
+    for (size_t i = 0; i < ss.size(); i++)
+      for (auto& f : frames[i]) {
+        if (f.rc || !f.device_work) continue;
+        if (f.staged) std::memcpy(h_stage.h + f.staged_off, f.host, f.len);
+        for (int sl : {f.main_slot, f.ov_slot}) {
+          if (sl < 0) continue;
+          PayloadSlot* ps = ss[i]->slot(sl);
+          std::memcpy(h_stage.h + ps->desc_off, ps->h_desc.h, ps->desc_bytes);
+        }
+      }
+    if (hipMemcpyAsync(d_stage.d, h_stage.h, stage, hipMemcpyHostToDevice, stream) != hipSuccess)
+      return fail(EBIV_ERR_DEVICE, "staging upload failed");
+    for (size_t i = 0; i < ss.size(); i++)
+      for (auto& f : frames[i])
+        if (f.staged) f.dev = d_stage.d + f.staged_off;
+  }
+
+  // ---- speculative buffer assignment: every payload taken to decode
+  std::vector<std::vector<int>> taken(ss.size());
+  for (size_t i = 0; i < ss.size(); i++) {
+    const std::vector<char> ok(frames[i].size(), 1);
+    if (!assign_buffers(*ss[i], frames[i], ok, ok, taken[i])) return fail(EBIV_ERR_DEVICE, "frame pool exhausted");
+  }
+
+  // ---- jobs: one per coded payload; a session's payloads in stream order
+  std::vector<JobRef> jrefs;
+  std::vector<std::vector<size_t>> rounds;  // job indices of each reconstruction round
+  for (size_t i = 0; i < ss.size(); i++) {
+    size_t u = 0;
+    for (size_t k = 0; k < frames[i].size(); k++) {
+      const StepFrame& f = frames[i][k];
+      if (f.rc || !f.device_work || f.raw) continue;
+      for (bool ov : {false, true}) {
+        if (ov && f.ov_slot < 0) continue;
+        if (rounds.size() <= u) rounds.emplace_back();
+        rounds[u++].push_back(jrefs.size());
+        jrefs.push_back({i, k, ov});
+      }
+    }
+  }
+  auto fill_job = [&](FrameJob& j, const JobRef& r) {
+    const Session& s = *ss[r.si];
+    const StepFrame& f = frames[r.si][r.fi];
+    PayloadSlot* ps = payload_of(ss, frames, r);
+    j.pkt = f.dev;
+    j.desc = d_stage.d + ps->desc_off;
+    j.recs = ps->recs.d;
+    j.coefs = ps->coefs.d;
+    j.status = ps->d_status.d;
+    j.pad[0] = j.pad[1] = 0;
+    if (r.ov) {  // the override predicts only from its key (codec.go:178-181)
+      j.rec = s.pool[f.gtarget].p;
+      j.ref = s.pool[f.target].p;
+      j.gold = s.pool[f.target].p;
+    } else {
+      j.rec = s.pool[f.target].p;
+      j.ref = s.pool[f.ref_in].p;
+      j.gold = s.pool[f.gold_in].p;
+    }
+  };
+  // Items: the entropy launch's tiles, then per round its inter macroblock
+  // groups and its intra tiles.
+  auto add_recon_items = [&](size_t j, bool inter_part, size_t& w) {
+    const JobRef& jr = jrefs[j];
+    const Session& s = *ss[jr.si];
+    const PayloadSlot* ps = payload_of(ss, frames, jr);
+    if (inter_part) {
+      if (!ps->inter) return;
+      const uint32_t nmb = static_cast<uint32_t>(s.mbc * s.mbr);
+      for (uint32_t m = 0; m < nmb; m += 4) h_items.h[w++] = WorkItem{static_cast<uint32_t>(j), m};
+    } else {
+      for (int t = 0; t < ps->ntiles; t++) h_items.h[w++] = WorkItem{static_cast<uint32_t>(j), static_cast<uint32_t>(t)};
+    }
+  };
+  size_t nitems = 0, nparse = 0;
+  int max_tw = 1;
+  for (const JobRef& jr : jrefs) {
+    const PayloadSlot* ps = payload_of(ss, frames, jr);
+    const Session& s = *ss[jr.si];
+    nparse += static_cast<size_t>(ps->ntiles);
+    nitems += 2 * static_cast<size_t>(ps->ntiles) + (ps->inter ? (static_cast<size_t>(s.mbc) * s.mbr + 3) / 4 : 0);
+    max_tw = std::max(max_tw, ps->max_tile_w);
+  }
+  const size_t njobs = jrefs.size();
+  if (!h_jobs.reserve(njobs + 1) || !d_jobs.reserve(njobs + 1) || !h_items.reserve(nitems + 1) ||
+      !d_items.reserve(nitems + 1))
+    return fail(EBIV_ERR_DEVICE, "job table allocation failed");
+  for (size_t j = 0; j < njobs; j++) fill_job(h_jobs.h[j], jrefs[j]);
+  size_t w = 0;
+  for (size_t j = 0; j < njobs; j++) add_recon_items(j, false, w);
+  std::vector<size_t> inter_off(rounds.size()), intra_off(rounds.size()), round_end(rounds.size());
+  for (size_t r = 0; r < rounds.size(); r++) {
+    inter_off[r] = w;
+    for (size_t j : rounds[r]) add_recon_items(j, true, w);
+    intra_off[r] = w;
+    for (size_t j : rounds[r]) add_recon_items(j, false, w);
+    round_end[r] = w;
+  }
+  if (njobs &&
+      (hipMemcpyAsync(d_jobs.d, h_jobs.h, sizeof(FrameJob) * njobs, hipMemcpyHostToDevice, stream) != hipSuccess ||
+       hipMemcpyAsync(d_items.d, h_items.h, sizeof(WorkItem) * w, hipMemcpyHostToDevice, stream) != hipSuccess))
+    return fail(EBIV_ERR_DEVICE, "job table upload failed");
+
+  // ---- raw frames: packed planes into their buffers (codec.go:195-207)
+  for (size_t i = 0; i < ss.size(); i++)
+    for (auto& f : frames[i]) {
+      if (f.rc || !f.raw) continue;
+      const Session& s = *ss[i];
+      const DevBuf& b = s.pool[f.target];
+      const uint8_t* src = f.dev + f.body_off;
+      const size_t ysz = static_cast<size_t>(s.W) * s.H, csz = static_cast<size_t>(s.CW) * s.CH;
+      if (hipMemcpy2DAsync(b.p.y, b.p.pw, src, s.W, s.W, s.H, hipMemcpyDeviceToDevice, stream) != hipSuccess ||
+          hipMemcpy2DAsync(b.p.cb, b.p.cw, src + ysz, s.CW, s.CW, s.CH, hipMemcpyDeviceToDevice, stream) != hipSuccess ||
+          hipMemcpy2DAsync(b.p.cr, b.p.cw, src + ysz + csz, s.CW, s.CW, s.CH, hipMemcpyDeviceToDevice, stream) != hipSuccess)
+        return fail(EBIV_ERR_DEVICE, "raw frame copy failed");
+    }
+
+  // ---- one entropy launch over every tile of every payload
+  if (nparse) {
+    const int m = mark_begin(0);
+    launch_parse(stream, static_cast<int>(nparse), max_tw, d_jobs.d, d_items.d);
+    mark_end(m);
+  }
+  // A golden override decodes over a copy of the golden it replaces, so a
+  // failing override leaves Go's partial rewrite (codec.go:178-181).
+  auto golden_copy = [&](const JobRef& jr) -> bool {
+    const Session& s = *ss[jr.si];
+    const StepFrame& f = frames[jr.si][jr.fi];
+    const DevBuf& src = s.pool[f.gold_in];
+    return hipMemcpyAsync(s.pool[f.gtarget].base, src.base, src.bytes, hipMemcpyDeviceToDevice, stream) == hipSuccess;
+  };
+  for (size_t r = 0; r < rounds.size(); r++) {
+    for (size_t j : rounds[r])
+      if (jrefs[j].ov && !golden_copy(jrefs[j])) return fail(EBIV_ERR_DEVICE, "golden copy failed");
+    if (intra_off[r] > inter_off[r]) {
+      const int m = mark_begin(1);
+      launch_recon_inter(stream, static_cast<int>(intra_off[r] - inter_off[r]), d_jobs.d, d_items.d + inter_off[r]);
+      mark_end(m);
+    }
+    if (round_end[r] > intra_off[r]) {
+      const int m = mark_begin(2);
+      launch_recon_intra(stream, static_cast<int>(round_end[r] - intra_off[r]), d_jobs.d, d_items.d + intra_off[r]);
+      mark_end(m);
+    }
+  }
+  for (const JobRef& jr : jrefs) {
+    PayloadSlot* ps = payload_of(ss, frames, jr);
+    if (hipMemcpyAsync(ps->h_status.h, ps->d_status.d, sizeof(TileStatus) * ps->ntiles, hipMemcpyDeviceToHost, stream) !=
+        hipSuccess)
+      return fail(EBIV_ERR_DEVICE, "status download failed");
+  }
+  if (hipGetLastError() != hipSuccess) return fail(EBIV_ERR_DEVICE, "kernel launch failed");
+  if (hipStreamSynchronize(stream) != hipSuccess) return fail(EBIV_ERR_DEVICE, "decode failed on the device");
+  collect_marks();
+
+  // ---- outcomes; where a payload failed, the session's later payloads are
+  // reconstructed again against the buffers Go would have used
+  std::vector<std::vector<char>> okm(ss.size()), oko(ss.size());
+  for (size_t i = 0; i < ss.size(); i++) {
+    okm[i].assign(frames[i].size(), 1);
+    oko[i].assign(frames[i].size(), 1);
+  }
+  std::vector<char> any_fail(ss.size(), 0);
+  for (const JobRef& jr : jrefs) {
+    Session& s = *ss[jr.si];
+    StepFrame& f = frames[jr.si][jr.fi];
+    if (s.tile_result(payload_of(ss, frames, jr)) == EBIV_OK) continue;
+    any_fail[jr.si] = 1;
+    if (jr.ov) {
+      oko[jr.si][jr.fi] = 0;
+      f.ov_msg = s.msg;
+    } else {
+      okm[jr.si][jr.fi] = 0;
+      f.msg = s.msg;
+    }
+  }
+  for (size_t i = 0; i < ss.size(); i++) {
+    if (!any_fail[i]) continue;
+    std::vector<std::pair<int, int>> before;
+    for (auto& f : frames[i]) before.push_back({f.ref_in, f.gold_in});
+    if (!assign_buffers(*ss[i], frames[i], okm[i], oko[i], taken[i])) return fail(EBIV_ERR_DEVICE, "frame pool exhausted");
+    bool dirty = false;
+    for (size_t k = 0; k < frames[i].size(); k++) {
+      StepFrame& f = frames[i][k];
+      if (f.rc || !f.device_work || f.raw) continue;
+      dirty = dirty || before[k].first != f.ref_in || before[k].second != f.gold_in;
+      if (!dirty) continue;
+      for (bool ov : {false, true}) {
+        if (ov && f.ov_slot < 0) continue;
+        const JobRef jr{i, k, ov};
+        const PayloadSlot* ps = payload_of(ss, frames, jr);
+        const Session& s = *ss[i];
+        fill_job(h_jobs.h[0], jr);
+        size_t n = 0;
+        if (ps->inter)
+          for (uint32_t m = 0; m < static_cast<uint32_t>(s.mbc * s.mbr); m += 4) h_items.h[n++] = WorkItem{0, m};
+        const size_t ninter = n;
+        for (int t = 0; t < ps->ntiles; t++) h_items.h[n++] = WorkItem{0, static_cast<uint32_t>(t)};
+        if (ov && !golden_copy(jr)) return fail(EBIV_ERR_DEVICE, "golden copy failed");
+        if (hipMemcpyAsync(d_jobs.d, h_jobs.h, sizeof(FrameJob), hipMemcpyHostToDevice, stream) != hipSuccess ||
+            hipMemcpyAsync(d_items.d, h_items.h, sizeof(WorkItem) * n, hipMemcpyHostToDevice, stream) != hipSuccess)
+          return fail(EBIV_ERR_DEVICE, "job table upload failed");
+        if (ninter) launch_recon_inter(stream, static_cast<int>(ninter), d_jobs.d, d_items.d);
+        launch_recon_intra(stream, ps->ntiles, d_jobs.d, d_items.d + ninter);
+        if (hipGetLastError() != hipSuccess) return fail(EBIV_ERR_DEVICE, "kernel launch failed");
+        // the pinned tables are rewritten by the next payload
+        if (hipStreamSynchronize(stream) != hipSuccess) return fail(EBIV_ERR_DEVICE, "decode failed on the device");
+      }
+    }
+  }
+
+  // ---- Go's state transitions in stream order (codec.go:84-110, 174-190)
+  for (size_t i = 0; i < ss.size(); i++) {
+    Session& s = *ss[i];
+    for (size_t k = 0; k < frames[i].size(); k++) {
+      StepFrame& f = frames[i][k];
+      if (f.rc) continue;  // failed on the host: nothing ran on the device
+      if (f.raw) {
+        s.ring[f.ring_slot] = f.target;
+        f.out_buf = f.target;
+        s.stats.frames++;
+        continue;
+      }
+      if (!okm[i][k]) {
+        f.rc = EBIV_ERR_CORRUPT;
+        // Go never reached the override, so never parsed its tables.
+        if (f.has_ov) std::copy(f.saved_prev.begin(), f.saved_prev.end(), s.prev);
+        continue;
+      }
+      if (f.has_ov) {
+        if (f.ov_rc) { f.rc = f.ov_rc; f.msg = f.ov_msg; continue; }
+        s.golden = f.gtarget;
+        if (!oko[i][k]) { f.rc = EBIV_ERR_CORRUPT; f.msg = f.ov_msg; continue; }
+      } else if (f.key) {
+        s.golden = f.target;
+      }
+      s.ring[f.ring_slot] = f.target;
+      s.ref = f.target;
+      f.out_buf = f.target;
+      s.stats.frames++;
+    }
+  }
+  return EBIV_OK;
+}
+
+}  // namespace ebiv
+
+using namespace ebiv;
+
+struct ebiv_codec {
+  Engine eng;
+  Session* s = nullptr;
+};
+
+struct ebiv_batch {
+  Engine eng;
+};
+
+namespace {
+
+void fill_frame(const Session& s, int buf, int slot, int64_t ts, bool on_device, ebiv_frame* out) {
+  const DevBuf& f = s.pool[buf];
+  out->width = s.W; out->height = s.H; out->cwidth = s.CW; out->cheight = s.CH;
+  out->timestamp_ns = ts;
+  out->ring_slot = slot;
+  if (on_device) {
+    out->y = f.p.y; out->cb = f.p.cb; out->cr = f.p.cr;
+    out->y_stride = f.p.pw; out->c_stride = f.p.cw;
+    out->on_device = 1;
+  } else {
+    uint8_t* h = s.host_ring[slot];
+    out->y = h;
+    out->cb = h + static_cast<size_t>(s.ystride) * s.H;
+    out->cr = out->cb + static_cast<size_t>(s.cstride) * s.CH;
+    out->y_stride = s.ystride; out->c_stride = s.cstride;
+    out->on_device = 0;
+  }
+}
+
+// storeImage into the host ring (frame.go:67-77): visible region only.
+int download(Engine& e, Session& s, int buf, int slot) {
+  if (int rc = s.ensure_host_ring()) return rc;
+  const DevBuf& f = s.pool[buf];
+  uint8_t* h = s.host_ring[slot];
+  uint8_t* hcb = h + static_cast<size_t>(s.ystride) * s.H;
+  uint8_t* hcr = hcb + static_cast<size_t>(s.cstride) * s.CH;
+  if (hipMemcpy2DAsync(h, s.ystride, f.p.y, f.p.pw, s.W, s.H, hipMemcpyDeviceToHost, e.stream) != hipSuccess ||
+      hipMemcpy2DAsync(hcb, s.cstride, f.p.cb, f.p.cw, s.CW, s.CH, hipMemcpyDeviceToHost, e.stream) != hipSuccess ||
+      hipMemcpy2DAsync(hcr, s.cstride, f.p.cr, f.p.cw, s.CW, s.CH, hipMemcpyDeviceToHost, e.stream) != hipSuccess ||
+      hipStreamSynchronize(e.stream) != hipSuccess)
+    return s.fail(EBIV_ERR_DEVICE, "frame download failed");
+  return EBIV_OK;
+}
+
+int engine_init(Engine& e, int device) {
+  if (int rc = ensure_device(device)) return rc;
+  e.device = device;
+  if (hipStreamCreateWithFlags(&e.stream, hipStreamNonBlocking) != hipSuccess) {
+    e.stream = nullptr;
+    return EBIV_ERR_DEVICE;
+  }
+  return EBIV_OK;
+}
+
+// Codec.Decode (codec.go:70-111) as a one-frame step.
+int decode_one(ebiv_codec* c, const uint8_t* data, const uint8_t* dev, size_t len, int64_t ts, ebiv_frame* out) {
+  Session& s = *c->s;
+  Engine& e = c->eng;
+  if (!data && len) return s.fail(EBIV_ERR_ARGUMENT, "null packet");
+  if (hipSetDevice(e.device) != hipSuccess) return s.fail(EBIV_ERR_DEVICE, "hipSetDevice failed");
+  std::vector<Session*> ss{&s};
+  std::vector<std::vector<StepFrame>> frames(1);
+  frames[0].emplace_back();
+  StepFrame& f = frames[0][0];
+  f.host = data;
+  f.dev = dev;
+  f.len = len;
+  f.ts = ts;
+  int next_slot = 0;
+  s.prepare_frame(f, &next_slot, e.stream);
+  if (f.rc) return s.fail(f.rc, f.msg);
+  if (int rc = e.run_step(ss, frames)) return s.fail(rc, e.msg);
+  if (f.rc) return s.fail(f.rc, f.msg);
+  if (!dev) {
+    if (int rc = download(e, s, f.out_buf, f.ring_slot)) return rc;
+  }
+  fill_frame(s, f.out_buf, f.ring_slot, ts, dev != nullptr, out);
+  return EBIV_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
 int ebiv_codec_create(int ring_size, int device, ebiv_codec** out) {
   if (!out) return EBIV_ERR_ARGUMENT;
   *out = nullptr;
-  std::string msg;
-  int rc = ensure_device(device, msg);
-  if (rc) return rc;
   auto* c = new ebiv_codec();
-  c->ring_size = ring_size >= 2 ? ring_size : 3;
-  c->device = device;
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (int rc = engine_init(c->eng, device)) {
     delete c;
-    return EBIV_ERR_DEVICE;
+    return rc;
   }
+  c->eng.sessions.emplace_back(new Session());
+  c->s = c->eng.sessions.back().get();
+  c->s->ring_size = ring_size >= 2 ? ring_size : 3;
+  c->s->window = 1;
   *out = c;
   return EBIV_OK;
 }
 
 int ebiv_codec_decode(ebiv_codec* c, const uint8_t* data, size_t len, int64_t ts, ebiv_frame* out) {
   if (!c || !out) return EBIV_ERR_ARGUMENT;
-  return c->decode(data, nullptr, len, ts, out);
+  return decode_one(c, data, nullptr, len, ts, out);
 }
 
 int ebiv_codec_decode_device(ebiv_codec* c, const uint8_t* host_data, const uint8_t* dev_data, size_t len, int64_t ts,
                              ebiv_frame* out) {
   if (!c || !out || (!dev_data && len)) return EBIV_ERR_ARGUMENT;
-  return c->decode(host_data, dev_data, len, ts, out);
+  return decode_one(c, host_data, dev_data, len, ts, out);
 }
 
 void ebiv_codec_flush(ebiv_codec* c) {
-  if (c) c->next = 0;
+  if (c) c->s->next = 0;
 }
 
 void ebiv_codec_destroy(ebiv_codec* c) { delete c; }
 
-const char* ebiv_codec_last_error(const ebiv_codec* c) { return c ? c->msg.c_str() : "null codec"; }
+const char* ebiv_codec_last_error(const ebiv_codec* c) { return c ? c->s->msg.c_str() : "null codec"; }
 
 int ebiv_codec_stats(const ebiv_codec* c, ebiv_stats* out) {
   if (!c || !out) return EBIV_ERR_ARGUMENT;
-  *out = c->stats;
+  *out = c->s->stats;
   return EBIV_OK;
 }
 
 int ebiv_codec_debug_dump(ebiv_codec* c, int slot, void* recs, size_t recs_bytes, uint32_t* coefs, size_t coef_count,
                           void* status, size_t status_bytes) {
-  if (!c || slot < 0 || slot > 1 || !c->ready) return -EBIV_ERR_ARGUMENT;
-  if (hipSetDevice(c->device) != hipSuccess) return -EBIV_ERR_DEVICE;
-  const size_t nmb = static_cast<size_t>(c->mbc) * c->mbr;
-  if (recs && hipMemcpy(recs, c->d_recs[slot].d, std::min(recs_bytes, nmb * sizeof(MBRec)), hipMemcpyDeviceToHost) != hipSuccess)
+  if (!c || slot < 0 || slot > 1 || !c->s->ready || static_cast<int>(c->s->slots.size()) <= slot)
+    return -EBIV_ERR_ARGUMENT;
+  Session& s = *c->s;
+  if (hipSetDevice(c->eng.device) != hipSuccess) return -EBIV_ERR_DEVICE;
+  const PayloadSlot* ps = s.slots[slot].get();
+  const size_t nmb = static_cast<size_t>(s.mbc) * s.mbr;
+  if (recs && ps->recs.d &&
+      hipMemcpy(recs, ps->recs.d, std::min(recs_bytes, nmb * sizeof(MBRec)), hipMemcpyDeviceToHost) != hipSuccess)
     return -EBIV_ERR_DEVICE;
-  if (coefs && hipMemcpy(coefs, c->d_coefs[slot].d, std::min(coef_count, nmb * kCoefPerMB) * 4, hipMemcpyDeviceToHost) != hipSuccess)
+  if (coefs && ps->coefs.d &&
+      hipMemcpy(coefs, ps->coefs.d, std::min(coef_count, nmb * kCoefPerMB) * 4, hipMemcpyDeviceToHost) != hipSuccess)
     return -EBIV_ERR_DEVICE;
-  if (status && c->d_status[slot].d &&
-      hipMemcpy(status, c->d_status[slot].d, std::min(status_bytes, c->d_status[slot].cap * sizeof(TileStatus)),
+  if (status && ps->d_status.d &&
+      hipMemcpy(status, ps->d_status.d, std::min(status_bytes, ps->d_status.cap * sizeof(TileStatus)),
                 hipMemcpyDeviceToHost) != hipSuccess)
     return -EBIV_ERR_DEVICE;
   return static_cast<int>(nmb);
@@ -713,49 +1081,232 @@ int ebiv_codec_debug_dump(ebiv_codec* c, int slot, void* recs, size_t recs_bytes
 
 int ebiv_codec_set_timing(ebiv_codec* c, int enabled) {
   if (!c) return EBIV_ERR_ARGUMENT;
-  c->timing = enabled != 0;
-  c->ktimes = ebiv_kernel_times{};
+  c->eng.timing = enabled != 0;
+  c->eng.ktimes = ebiv_kernel_times{};
   return EBIV_OK;
 }
 
 int ebiv_codec_kernel_times(const ebiv_codec* c, ebiv_kernel_times* out) {
   if (!c || !out) return EBIV_ERR_ARGUMENT;
-  *out = c->ktimes;
+  *out = c->eng.ktimes;
+  return EBIV_OK;
+}
+
+// The device buffer behind a frame this engine returned: a device frame by
+// its plane pointer, a host frame by the ring slot whose image it views.
+static const DevBuf* buffer_of(const Engine& e, const ebiv_frame* f) {
+  for (auto& s : e.sessions) {
+    if (!s->ready) continue;
+    if (f->on_device) {
+      for (auto& p : s->pool)
+        if (p.p.y == f->y) return &p;
+    } else if (f->ring_slot >= 0 && f->ring_slot < s->ring_size && !s->host_ring.empty() &&
+               s->host_ring[f->ring_slot] == f->y && s->ring[f->ring_slot] >= 0) {
+      return &s->pool[s->ring[f->ring_slot]];
+    }
+  }
+  return nullptr;
+}
+
+static int frame_rgba(Engine& e, const ebiv_frame* f, uint8_t* dst, int dst_on_device) {
+  if (!f || !dst || !f->y || f->width <= 0 || f->height <= 0) return EBIV_ERR_ARGUMENT;
+  if (hipSetDevice(e.device) != hipSuccess) return EBIV_ERR_DEVICE;
+  const DevBuf* b = buffer_of(e, f);
+  if (!b) return EBIV_ERR_ARGUMENT;
+  const size_t px = static_cast<size_t>(f->width) * f->height;
+  uint32_t* out = reinterpret_cast<uint32_t*>(dst);
+  if (!dst_on_device) {
+    if (!e.d_rgba.reserve(px)) return EBIV_ERR_DEVICE;
+    out = e.d_rgba.d;
+  }
+  const int m = e.mark_begin(3);
+  launch_rgba(e.stream, b->p.y, b->p.pw, b->p.cb, b->p.cr, b->p.cw, f->width, f->height, out);
+  e.mark_end(m);
+  if (hipGetLastError() != hipSuccess) return EBIV_ERR_DEVICE;
+  if (!dst_on_device && hipMemcpyAsync(dst, e.d_rgba.d, px * 4, hipMemcpyDeviceToHost, e.stream) != hipSuccess)
+    return EBIV_ERR_DEVICE;
+  if (hipStreamSynchronize(e.stream) != hipSuccess) return EBIV_ERR_DEVICE;
+  e.collect_marks();
   return EBIV_OK;
 }
 
 int ebiv_frame_rgba(ebiv_codec* c, const ebiv_frame* f, uint8_t* dst, int dst_on_device) {
-  if (!c || !f || !dst || !c->ready) return EBIV_ERR_ARGUMENT;
-  if (f->ring_slot < 0 || f->ring_slot >= c->ring_size || c->ring[f->ring_slot] < 0) return EBIV_ERR_ARGUMENT;
-  if (hipSetDevice(c->device) != hipSuccess) return EBIV_ERR_DEVICE;
-  const DevBuf& b = c->pool[c->ring[f->ring_slot]];
-  const size_t px = static_cast<size_t>(c->W) * c->H;
-  uint32_t* out = reinterpret_cast<uint32_t*>(dst);
-  if (!dst_on_device) {
-    if (!c->d_rgba.reserve(px)) return EBIV_ERR_DEVICE;
-    out = c->d_rgba.d;
+  if (!c) return EBIV_ERR_ARGUMENT;
+  return frame_rgba(c->eng, f, dst, dst_on_device);
+}
+
+// ---- batched engine ------------------------------------------------------------
+int ebiv_batch_create(int device, ebiv_batch** out) {
+  if (!out) return EBIV_ERR_ARGUMENT;
+  *out = nullptr;
+  auto* b = new ebiv_batch();
+  if (int rc = engine_init(b->eng, device)) {
+    delete b;
+    return rc;
   }
-  const bool tm = c->timing && c->ensure_events();
-  if (tm) hipEventRecord(c->ev[0][0], c->stream);
-  launch_rgba(c->stream, b.p.y, b.p.pw, b.p.cb, b.p.cr, b.p.cw, c->W, c->H, out);
-  if (tm) hipEventRecord(c->ev[0][1], c->stream);
-  if (hipGetLastError() != hipSuccess) return EBIV_ERR_DEVICE;
-  if (!dst_on_device &&
-      hipMemcpyAsync(dst, c->d_rgba.d, px * 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
-    return EBIV_ERR_DEVICE;
-  if (hipStreamSynchronize(c->stream) != hipSuccess) return EBIV_ERR_DEVICE;
-  if (tm) {
-    float ms = 0;
-    if (hipEventElapsedTime(&ms, c->ev[0][0], c->ev[0][1]) == hipSuccess) { c->ktimes.rgba_ms += ms; c->ktimes.rgba_n++; }
-  }
+  *out = b;
   return EBIV_OK;
+}
+
+void ebiv_batch_destroy(ebiv_batch* b) { delete b; }
+
+int ebiv_batch_open(ebiv_batch* b, int ring_size, int window, int* session) {
+  if (!b || !session || window < 1 || window > 64) return EBIV_ERR_ARGUMENT;
+  auto s = std::make_unique<Session>();
+  s->id = b->eng.next_id++;
+  s->window = window;
+  // every frame a step returns stays valid until the next step returns
+  s->ring_size = std::max(ring_size >= 2 ? ring_size : 3, window);
+  *session = s->id;
+  b->eng.sessions.push_back(std::move(s));
+  return EBIV_OK;
+}
+
+int ebiv_batch_close(ebiv_batch* b, int session) {
+  if (!b) return EBIV_ERR_ARGUMENT;
+  auto& v = b->eng.sessions;
+  for (size_t i = 0; i < v.size(); i++)
+    if (v[i]->id == session) {
+      (void)hipSetDevice(b->eng.device);
+      (void)hipStreamSynchronize(b->eng.stream);
+      v.erase(v.begin() + static_cast<long>(i));
+      return EBIV_OK;
+    }
+  return EBIV_ERR_ARGUMENT;
+}
+
+int ebiv_batch_submit(ebiv_batch* b, int session, const uint8_t* host_data, const uint8_t* dev_data, size_t len,
+                      int64_t ts, uint64_t tag) {
+  if (!b || (!host_data && len)) return EBIV_ERR_ARGUMENT;
+  Session* s = b->eng.find(session);
+  if (!s) return EBIV_ERR_ARGUMENT;
+  Session::Queued q;
+  q.host.assign(host_data, host_data + len);
+  q.dev = dev_data;
+  q.ts = ts;
+  q.tag = tag;
+  s->queue.push_back(std::move(q));
+  return EBIV_OK;
+}
+
+int ebiv_batch_flush(ebiv_batch* b, int session) {
+  Session* s = b ? b->eng.find(session) : nullptr;
+  if (!s) return EBIV_ERR_ARGUMENT;
+  s->queue.clear();
+  s->next = 0;  // Codec.Flush
+  return EBIV_OK;
+}
+
+int ebiv_batch_pending(const ebiv_batch* b, int session) {
+  if (!b) return 0;
+  int n = 0;
+  for (auto& s : b->eng.sessions)
+    if (session == 0 || s->id == session) n += static_cast<int>(s->queue.size());
+  return n;
+}
+
+int ebiv_batch_step(ebiv_batch* b, ebiv_completion* out, int max_out, int* n_out) {
+  if (!b || !n_out || max_out < 0 || (max_out > 0 && !out)) return EBIV_ERR_ARGUMENT;
+  *n_out = 0;
+  Engine& e = b->eng;
+  if (hipSetDevice(e.device) != hipSuccess) return e.fail(EBIV_ERR_DEVICE, "hipSetDevice failed");
+  std::vector<Session*> ss;
+  std::vector<std::vector<StepFrame>> frames;
+  std::vector<std::vector<Session::Queued>> held;
+  int budget = max_out;
+  for (auto& sp : e.sessions) {
+    Session& s = *sp;
+    if (s.queue.empty() || budget <= 0) continue;
+    ss.push_back(&s);
+    frames.emplace_back();
+    held.emplace_back();
+    const int window = std::min(s.window, budget);
+    frames.back().reserve(static_cast<size_t>(window));
+    held.back().reserve(static_cast<size_t>(window));
+    int next_slot = 0;
+    while (!s.queue.empty() && static_cast<int>(frames.back().size()) < window) {
+      held.back().push_back(std::move(s.queue.front()));
+      s.queue.pop_front();
+      const Session::Queued& q = held.back().back();
+      frames.back().emplace_back();
+      StepFrame& f = frames.back().back();
+      f.host = q.host.data();
+      f.dev = q.dev;
+      f.len = q.host.size();
+      f.ts = q.ts;
+      f.tag = q.tag;
+      budget--;
+      if (!s.prepare_frame(f, &next_slot, e.stream)) break;
+    }
+  }
+  if (ss.empty()) return EBIV_OK;
+  if (int rc = e.run_step(ss, frames)) return rc;
+  int n = 0;
+  for (size_t i = 0; i < ss.size(); i++)
+    for (auto& f : frames[i]) {
+      ebiv_completion& c = out[n++];
+      std::memset(&c, 0, sizeof c);
+      c.session = ss[i]->id;
+      c.status = f.rc;
+      c.tag = f.tag;
+      if (f.rc == EBIV_OK) {
+        fill_frame(*ss[i], f.out_buf, f.ring_slot, f.ts, true, &c.frame);
+      } else {
+        std::strncpy(c.message, f.msg.c_str(), sizeof c.message - 1);
+        ss[i]->msg = f.msg;
+      }
+    }
+  *n_out = n;
+  return EBIV_OK;
+}
+
+const char* ebiv_batch_last_error(const ebiv_batch* b) { return b ? b->eng.msg.c_str() : "null batch"; }
+
+int ebiv_batch_set_timing(ebiv_batch* b, int enabled) {
+  if (!b) return EBIV_ERR_ARGUMENT;
+  b->eng.timing = enabled != 0;
+  b->eng.ktimes = ebiv_kernel_times{};
+  return EBIV_OK;
+}
+
+int ebiv_batch_kernel_times(const ebiv_batch* b, ebiv_kernel_times* out) {
+  if (!b || !out) return EBIV_ERR_ARGUMENT;
+  *out = b->eng.ktimes;
+  return EBIV_OK;
+}
+
+int ebiv_batch_stats(const ebiv_batch* b, int session, ebiv_stats* out) {
+  if (!b || !out) return EBIV_ERR_ARGUMENT;
+  ebiv_stats t{0, 0, 0};
+  for (auto& s : b->eng.sessions)
+    if (session == 0 || s->id == session) {
+      t.frames += s->stats.frames;
+      t.symbols += s->stats.symbols;
+      t.tiles += s->stats.tiles;
+    }
+  *out = t;
+  return EBIV_OK;
+}
+
+int ebiv_batch_frame_rgba(ebiv_batch* b, const ebiv_frame* f, uint8_t* dst, int dst_on_device) {
+  if (!b) return EBIV_ERR_ARGUMENT;
+  return frame_rgba(b->eng, f, dst, dst_on_device);
+}
+
+size_t ebiv_session_bytes(int width, int height, int ring_size, int window) {
+  if (width <= 0 || height <= 0) return 0;
+  const size_t mbc = static_cast<size_t>((width + kMB - 1) / kMB), mbr = static_cast<size_t>((height + kMB - 1) / kMB);
+  const size_t ysz = mbc * kMB * mbr * kMB, csz = align256(mbc * kCMB * mbr * kCMB);
+  const size_t rs = static_cast<size_t>(std::max(ring_size >= 2 ? ring_size : 3, window));
+  const size_t pool = (rs + 2 + static_cast<size_t>(window) + 1) * (ysz + 2 * csz);
+  // payload slots: a window's payloads plus an override
+  const size_t payload = (static_cast<size_t>(window) + 1) * mbc * mbr * (sizeof(MBRec) + kCoefPerMB * 4);
+  return pool + payload;
 }
 
 int ebiv_device_alloc(int device, size_t bytes, void** out) {
   if (!out) return EBIV_ERR_ARGUMENT;
-  std::string msg;
-  int rc = ensure_device(device, msg);
-  if (rc) return rc;
+  if (int rc = ensure_device(device)) return rc;
   return hipMalloc(out, bytes + 256) == hipSuccess ? EBIV_OK : EBIV_ERR_DEVICE;
 }
 int ebiv_device_free(void* p) { return hipFree(p) == hipSuccess ? EBIV_OK : EBIV_ERR_DEVICE; }

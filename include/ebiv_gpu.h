@@ -106,6 +106,65 @@ int ebiv_codec_debug_dump(ebiv_codec *c, int slot, void *recs, size_t recs_bytes
  * dst receives width*height*4 bytes; dst_on_device selects HBM or host. */
 int ebiv_frame_rgba(ebiv_codec *c, const ebiv_frame *f, uint8_t *dst, int dst_on_device);
 
+/* ---- Batched decode -----------------------------------------------------------
+ *
+ * Many streams decoded together on one device.  govid drives one Codec per
+ * stream from a frame source (source.go:9-33: next, seek, release; the
+ * decode-ahead source of source.go:83-131 keeps a bounded queue of decoded
+ * frames); here one engine decodes every stream's queue.  Each session is one
+ * ebiv.Codec with the same state machine, errors and ring rules as
+ * ebiv_codec_*; a step decodes up to `window` queued packets of every session
+ * with one entropy launch over all their tiles and one reconstruction launch
+ * per frame position, and returns one completion per packet it consumed, in
+ * submission order per session.  Completed frames are HBM planes and stay
+ * valid until ring_size further frames of their session have completed.
+ * Not thread-safe; distinct engines may run on distinct threads. */
+typedef struct ebiv_batch ebiv_batch;
+
+typedef struct {
+    int32_t session;
+    int32_t status;     /* EBIV_OK or the decode's error (as ebiv_codec_decode) */
+    uint64_t tag;       /* the submit call's tag */
+    ebiv_frame frame;   /* valid when status == EBIV_OK */
+    char message[96];   /* error text when status != EBIV_OK */
+} ebiv_completion;
+
+int ebiv_batch_create(int device, ebiv_batch **out);
+void ebiv_batch_destroy(ebiv_batch *b);
+const char *ebiv_batch_last_error(const ebiv_batch *b);
+
+/* Opens a session (a NewCodec(WithFrameRing(ring_size))); window is the most
+ * frames of this session one step decodes (1..64).  ring_size is raised to
+ * window when smaller. */
+int ebiv_batch_open(ebiv_batch *b, int ring_size, int window, int *session);
+int ebiv_batch_close(ebiv_batch *b, int session);
+
+/* Queues a packet.  host_data is copied; dev_data, when not null, holds the
+ * same bytes in HBM (from ebiv_device_alloc) and must stay valid until the
+ * packet completes; when null the engine stages the packet itself. */
+int ebiv_batch_submit(ebiv_batch *b, int session, const uint8_t *host_data, const uint8_t *dev_data, size_t len,
+                      int64_t timestamp_ns, uint64_t tag);
+
+/* Codec.Flush for a session (codec.go:115-117), dropping its queued packets:
+ * the seek path (source.go:45-50 and 226-234 seek the demuxer, then flush). */
+int ebiv_batch_flush(ebiv_batch *b, int session);
+
+/* Queued packets of one session (or of all when session is 0). */
+int ebiv_batch_pending(const ebiv_batch *b, int session);
+
+/* Runs one step; writes at most max_out completions.  Returns a runtime
+ * status (decode errors are reported per completion). */
+int ebiv_batch_step(ebiv_batch *b, ebiv_completion *out, int max_out, int *n_out);
+
+int ebiv_batch_stats(const ebiv_batch *b, int session, ebiv_stats *out);
+int ebiv_batch_set_timing(ebiv_batch *b, int enabled);
+int ebiv_batch_kernel_times(const ebiv_batch *b, ebiv_kernel_times *out);
+int ebiv_batch_frame_rgba(ebiv_batch *b, const ebiv_frame *f, uint8_t *dst, int dst_on_device);
+
+/* HBM one session holds at this geometry: frame pool and payload buffers
+ * (for admission against a device memory budget). */
+size_t ebiv_session_bytes(int width, int height, int ring_size, int window);
+
 /* Device memory helpers for callers that keep packets resident in HBM. */
 int ebiv_device_alloc(int device, size_t bytes, void **out);
 int ebiv_device_free(void *p);

@@ -34,3 +34,52 @@ def assert_planes_equal(got, want, label=""):
         assert a.shape == b.shape, f"{label} {name}: shape {a.shape} != {b.shape}"
         wrong, mx, first = plane_diff(a, b)
         assert wrong == 0, f"{label} plane {name}: {wrong}/{a.size} samples wrong, max error {mx}, first at {first}"
+
+
+def _uvarint(b: bytes, pos: int):
+    x, s = 0, 0
+    while True:
+        c = b[pos]
+        pos += 1
+        x |= (c & 0x7F) << s
+        if c < 0x80:
+            return x, pos
+        s += 7
+
+
+def payload_spans(pkt: bytes):
+    """Byte ranges of a coded frame's payloads (format.go:376-409 header, the
+    golden-override split of codec.go:162-173): [(start, end)] for the frame's
+    own payload and, on a key frame with an override, the override's."""
+    typ, gov = pkt[0] & 3, (pkt[0] >> 5) & 1
+    pos = 6 if typ == 0 else 1
+    if not gov:
+        return [(pos, len(pkt))]
+    kl, p2 = _uvarint(pkt, pos)
+    return [(p2, p2 + kl), (p2 + kl, len(pkt))]
+
+
+def tile_spans(pkt: bytes, payload: int = 0):
+    """(offset, length) of each tile stream of one payload (frame.go:197-274:
+    qp, tile columns and rows, table block, then length-prefixed tiles)."""
+    start, _ = payload_spans(pkt)[payload]
+    pos = start + 1
+    cols, pos = _uvarint(pkt, pos)
+    rows, pos = _uvarint(pkt, pos)
+    tlen, pos = _uvarint(pkt, pos)
+    pos += tlen
+    spans = []
+    for _ in range(cols * rows):
+        ln, pos = _uvarint(pkt, pos)
+        spans.append((pos, ln))
+        pos += ln
+    return spans
+
+
+def zero_tile(pkt: bytes, tile: int = 0, payload: int = 0) -> bytes:
+    """The packet with one tile's bytes zeroed: the framing still parses, and
+    the tile's rANS states start at 0, so its decode underruns."""
+    off, ln = tile_spans(pkt, payload)[tile]
+    b = bytearray(pkt)
+    b[off:off + ln] = bytes(ln)
+    return bytes(b)

@@ -1,0 +1,182 @@
+"""Batched decode (many sessions, several frames per session per step) vs the
+oracle, completion by completion, through the C ABI.
+
+Every session must behave exactly as its own ebiv.Codec would
+(codec.go:70-207): the same frames, the same errors, the same state after an
+error — including a failing frame inside a step's window, whose later frames
+decode against the reference the failure left in place.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import assert_planes_equal, clip_kw, zero_tile
+
+pytestmark = pytest.mark.gpu
+
+ERR = {1: "ErrCorrupt", 2: "ErrUnsupported", 3: "ErrNoKeyframe", 4: "ErrDimensions"}
+
+
+def _oracle_outcomes(packets, ring, flush_at=()):
+    ref = O.OracleCodec(ring=ring)
+    out = []
+    for i, p in enumerate(packets):
+        if i in flush_at:
+            ref.flush()
+        try:
+            out.append((0, ref.decode(p)))
+        except O.DecodeError as e:
+            out.append((e.status, None))
+    return out
+
+
+def _run_batch(streams, max_out=4096, check_rgba=False):
+    """streams: list of (packets, ring, window).  Submits everything, steps
+    until drained, and checks each completion against the oracle."""
+    from govid_amd.codec import Batch
+
+    b = Batch()
+    want, sids = [], []
+    for k, (packets, ring, window) in enumerate(streams):
+        sid = b.open(ring=ring, window=window)
+        sids.append(sid)
+        want.append(_oracle_outcomes(packets, max(ring, window)))
+        for i, p in enumerate(packets):
+            b.submit(sid, p, tag=(k << 32) | i)
+    seen = [0] * len(streams)
+    steps = 0
+    while b.pending():
+        done = b.step(max_out)
+        steps += 1
+        assert done, "a step with pending packets completed nothing"
+        for c in done:
+            k, i = c.tag >> 32, c.tag & 0xFFFFFFFF
+            assert c.session == sids[k]
+            assert i == seen[k], f"session {k}: completion {i} out of order (expected {seen[k]})"
+            seen[k] += 1
+            st, planes = want[k][i]
+            print(f"  step {steps} session {k} frame {i}: status {c.status} (oracle {st})", flush=True)
+            assert c.status == st, f"session {k} frame {i}: status {c.status} != oracle {ERR.get(st, st)}"
+            if st == 0:
+                assert_planes_equal(c.frame.planes(), planes, f"session {k} frame {i}")
+                if check_rgba:
+                    assert np.array_equal(c.frame.rgba(), O.rgba(*planes)), f"session {k} frame {i} rgba"
+    assert seen == [len(s[0]) for s in streams]
+    b.close()
+    return steps
+
+
+def _clip(wl, w, h, n, **kw):
+    return clip_kw(wl, w, h, n, threads=4, **kw).packets
+
+
+def test_sessions_of_mixed_geometry_and_windows():
+    streams = [
+        (_clip("synthpan", 64, 48, 9, qp=22, gop=4), 3, 4),
+        (_clip("rects", 96, 64, 10, qp=22, gop=10, tile_cols=4, tile_rows=3), 3, 3),
+        (_clip("synthpan", 37, 21, 5, qp=22, gop=5), 3, 1),
+        (_clip("parallax", 64, 48, 9, qp=28, gop=9, altref=4), 4, 5),
+        (_clip("synthframe", 64, 48, 3, qp=20, gop=1), 2, 2),
+    ]
+    print("mixed sessions", flush=True)
+    _run_batch(streams, check_rgba=True)
+
+
+def test_window_larger_than_stream_and_max_out_limit():
+    pk = _clip("synthpan", 64, 48, 6, qp=22, gop=6)
+    print("window 8 over 6 frames", flush=True)
+    _run_batch([(pk, 3, 8)])
+    print("max_out 2", flush=True)
+    assert _run_batch([(pk, 3, 4), (pk, 3, 4)], max_out=2) >= 6
+
+
+@pytest.mark.parametrize("bad", [[2], [0], [2, 3], [4, 7]])
+def test_failures_inside_the_window(bad):
+    """A tile that underruns on the device: the frame fails, later frames in
+    the same step predict from the reference Go kept (codec.go:174-190)."""
+    pk = list(_clip("synthpan", 64, 48, 9, qp=22, gop=4))
+    for i in bad:
+        pk[i] = zero_tile(pk[i])
+    print(f"corrupt frames {bad}", flush=True)
+    _run_batch([(pk, 3, 5), (list(_clip("synthpan", 64, 48, 9, qp=22, gop=4)), 3, 5)])
+
+
+@pytest.mark.parametrize("payload", [0, 1])
+def test_golden_override_failure(payload):
+    """Key frame with an alt-ref override: a failing key never reaches the
+    override (its tables stay unparsed); a failing override leaves golden
+    partially rewritten (codec.go:176-181)."""
+    pk = list(_clip("parallax", 64, 48, 9, qp=28, gop=9, altref=4))
+    from helpers import payload_spans
+
+    assert len(payload_spans(pk[0])) == 2, "clip has no golden override"
+    pk[0] = zero_tile(pk[0], 0, payload)
+    print(f"corrupt payload {payload} of the key frame", flush=True)
+    _run_batch([(pk, 3, 4)])
+
+
+def test_host_side_errors_in_window():
+    """Errors found by the host (no key frame yet, truncated header, unknown
+    coding) keep their place in the completion order."""
+    pk = list(_clip("synthpan", 64, 48, 6, qp=22, gop=3))
+    stream = [pk[1], pk[0], pk[1][:1], pk[2], bytes([0x1C]) + pk[3][1:], pk[3], pk[4], pk[5]]
+    _run_batch([(stream, 3, 4)])
+
+
+def test_flush_and_seek_to_key():
+    """Seek: flush the session, then continue from a later key frame."""
+    from govid_amd.codec import Batch
+
+    pk = list(_clip("synthpan", 64, 48, 8, qp=22, gop=4))
+    b = Batch()
+    sid = b.open(ring=3, window=4)
+    ref = O.OracleCodec(ring=3)
+    for p in pk[:3]:
+        b.submit(sid, p)
+    got = [c.frame.planes() for c in b.step()]
+    for g, p in zip(got, pk[:3]):
+        assert_planes_equal(g, ref.decode(p))
+    b.submit(sid, pk[3])
+    b.flush(sid)
+    ref.flush()
+    assert b.pending(sid) == 0
+    for p in pk[4:]:
+        b.submit(sid, p)
+    done = b.step()
+    assert len(done) == 4
+    for c, p in zip(done, pk[4:]):
+        assert c.status == 0
+        assert_planes_equal(c.frame.planes(), ref.decode(p))
+    b.close()
+
+
+def test_bench_clip_two_sessions_window_8():
+    """The benchmark's 1080p clip through the batched engine, two sessions."""
+    import os
+
+    from govid_amd.container import Reader
+
+    path = os.path.join(os.path.dirname(os.path.dirname(__file__)), "bench_data", "parallax_1080p.ebiv")
+    pk = Reader.open(path).packets()[:40]
+    from govid_amd.codec import Batch
+
+    b = Batch()
+    s1, s2 = b.open(ring=8, window=8), b.open(ring=8, window=8)
+    for p in pk:
+        b.submit(s1, p)
+    for p in pk[:24]:
+        b.submit(s2, p)
+    ref1, ref2 = O.OracleCodec(ring=8, threads=8), O.OracleCodec(ring=8, threads=8)
+    n1 = n2 = 0
+    while b.pending():
+        for c in b.step():
+            assert c.status == 0, c.error
+            if c.session == s1:
+                assert_planes_equal(c.frame.planes(), ref1.decode(pk[n1]), f"s1 frame {n1}")
+                n1 += 1
+            else:
+                assert_planes_equal(c.frame.planes(), ref2.decode(pk[n2]), f"s2 frame {n2}")
+                n2 += 1
+        print(f"  {n1} + {n2} frames checked", flush=True)
+    assert (n1, n2) == (40, 24)
+    b.close()

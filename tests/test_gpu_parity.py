@@ -28,6 +28,7 @@ def _run(clip, ring=3, check_rgba=False):
     dict(workload="synthpan", w=64, h=48, frames=6, qp=22, gop=6),
     dict(workload="rects", w=64, h=48, frames=6, qp=22, gop=3, tile_cols=2, tile_rows=2),
     dict(workload="synthpan", w=37, h=21, frames=5, qp=22, gop=5),
+    dict(workload="parallax", w=64, h=48, frames=9, qp=28, gop=9, altref=4),
 ])
 def test_small_clips(case):
     c = dict(case)

@@ -356,6 +356,7 @@ LOSSY = [
     dict(workload="synthpan", w=31, h=47, frames=5, qp=22, gop=5, tile_cols=2, tile_rows=3),
     dict(workload="rects", w=96, h=64, frames=10, qp=22, gop=10, tile_cols=4, tile_rows=3),
     dict(workload="rects", w=64, h=48, frames=9, qp=28, gop=9, altref=4),
+    dict(workload="parallax", w=64, h=48, frames=9, qp=28, gop=9, altref=4),  # key frame carries an override
     dict(workload="parallax", w=128, h=64, frames=30, qp=22, gop=30, tile_cols=2, tile_rows=2),
     dict(workload="noisepan", w=64, h=64, frames=6, qp=12, gop=3),
 ]
