@@ -186,6 +186,7 @@ def main():
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-only", action="store_true", help="skip the plain pass (for profiling the batched one)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this decoder runs on one GPU; runs on several GPUs are not part of this "
@@ -201,10 +202,15 @@ def main():
     log(f"bench: {len(packets)} packets, {sum(map(len, packets))} bytes, {W}x{H}")
 
     want = oracle_crcs(packets)
+    dev = [DeviceBuffer(p) for p in packets]
+    if args.batched_only:
+        bt = run_batched(packets, dev, want, args.sessions, args.window, args.steps, args.warmup)
+        print(json.dumps({"mode": "batched", "frames_per_s": round(bt["frames"] / bt["elapsed"], 2),
+                          "frames_differing": bt["bad"], "sessions": args.sessions, "window": args.window}))
+        return
     bad = 0 if args.no_verify else verify_plain(packets, want)
     log(f"bench: plain verify -> {bad} frames differ from the oracle")
 
-    dev = [DeviceBuffer(p) for p in packets]
     pl = run_plain(packets, dev, args.steps, args.warmup)
     fps = pl["frames"] / pl["elapsed"]
     kt = pl["kt"]

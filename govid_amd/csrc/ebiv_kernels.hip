@@ -27,7 +27,8 @@ namespace ebiv {
 __constant__ int32_t c_dct[3][256];   // buildDCTMatrix for n = 4, 8, 16
 __constant__ uint8_t c_zz[3][256];    // zigzag scan (row-major positions)
 __constant__ int32_t c_taps[4][4] = {{0, 64, 0, 0}, {-6, 55, 18, -3}, {-8, 40, 40, -8}, {-3, 18, 55, -6}};
-__constant__ int32_t c_part_count[4] = {1, 2, 2, 4};
+// partCount (model.go:91-96) as nibbles: 16x16 -> 1, 16x8 -> 2, 8x16 -> 2, 8x8 -> 4.
+constexpr uint32_t kPartCountNibbles = 0x4221u;
 // angModes (predict.go:36-43): angle, invAngle, vertical; modes 0-3 unused.
 __constant__ int32_t c_ang[kNumIntraModes][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {32, 0, 1},
                                                  {13, 0, 1}, {-13, -630, 1}, {32, 0, 0}, {13, 0, 0}, {-13, -630, 0}};
@@ -46,17 +47,21 @@ __device__ __forceinline__ uint32_t rd_lane(uint32_t v, int l) { return __builti
 // ============================================================================
 
 // Descending byte reader over a tile stream.  Each lane holds one dword of a
-// 256-byte window, so reading a byte is a readlane; the next lower window is
-// loaded one window ahead.
+// 256-byte window, so reading a byte is a readlane.  The next lower window is
+// loaded when the reader crosses into it and waited for there: a load kept in
+// flight across the symbol loop would make every loop-carried register copy
+// wait for it, and in-order completion would also make it wait for every
+// coefficient store before it.
 struct ByteWin {
   uintptr_t tile;      // absolute address of tile byte 0
   uintptr_t lo, hi;    // dword-aligned readable range [lo, hi)
   uintptr_t wbase;     // 256-aligned base of the current window
-  uint32_t cur, nxt;
+  uint32_t cur;
 
   __device__ __forceinline__ uint32_t fetch(uintptr_t wb, int lane) const {
     uintptr_t a = wb + 4u * lane;
-    return (a >= lo && a < hi) ? *reinterpret_cast<const uint32_t*>(a) : 0u;
+    using GWord = __attribute__((address_space(1))) const uint32_t;  // packets live in device global memory
+    return (a >= lo && a < hi) ? *(GWord*)a : 0u;
   }
   __device__ void init(const uint8_t* t, uint32_t len, int lane) {
     tile = reinterpret_cast<uintptr_t>(t);
@@ -64,21 +69,25 @@ struct ByteWin {
     hi = (tile + len + 3) & ~uintptr_t(3);
     wbase = (tile + (len ? len - 1 : 0)) & ~uintptr_t(255);
     cur = fetch(wbase, lane);
-    nxt = fetch(wbase - 256, lane);
   }
   // Byte p of the tile; p only ever decreases between calls.
   __device__ __forceinline__ uint32_t byte(uint32_t p, int lane) {
     uintptr_t a = tile + p;
     if (a < wbase) {
-      cur = nxt;
       wbase -= 256;
-      nxt = fetch(wbase - 256, lane);
+      cur = fetch(wbase, lane);
     }
     uint32_t off = static_cast<uint32_t>(a - wbase);
     uint32_t w = rd_lane(cur, static_cast<int>(off >> 2));
     return (w >> ((off & 3u) * 8u)) & 0xFFu;
   }
 };
+
+// Zigzag scan position si of an n x n block from per-size register tables:
+// lane l of zzv holds positions 4l..4l+3 (scan.go:17-35).
+__device__ __forceinline__ int zz_pos(uint32_t zzv, int si) {
+  return static_cast<int>((rd_lane(zzv, si >> 2) >> ((si & 3) * 8)) & 0xFFu);
+}
 
 __device__ __forceinline__ int zigzag_dec(uint32_t u) { return static_cast<int>(u >> 1) ^ -static_cast<int>(u & 1u); }
 __device__ __forceinline__ int16_t clamp_mv(int v) { return static_cast<int16_t>(v < -kMVClamp ? -kMVClamp : (v > kMVClamp ? kMVClamp : v)); }
@@ -127,43 +136,276 @@ __device__ __forceinline__ uint32_t mv_set(uint32_t v, int comp, int16_t c) {
   return comp ? ((v & 0xFFFFu) | (u << 16)) : ((v & 0xFFFF0000u) | u);
 }
 
-// Grammar states of the entropy kernel.  Every state up to P_FSIGN decodes
-// exactly one symbol in the context it names; the states after it are
-// bookkeeping steps that decode nothing.  The order of decodes is that of
-// decodeTile / decodeLumaMB / decodeChromaMB (decode_intra.go:121-167),
-// decodeInterMB / reconInterMB (decode_inter.go:8-118), getClassValue
-// (mv.go:95-108) and decodeBlock (residual.go:79-126).
-enum PState : int {
-  P_MBMODE, P_REF, P_PART, P_MVCLS, P_MVBITS, P_CBP, P_TX, P_LMODE, P_CMODE,
-  P_TOKEN, P_SIGN, P_ESCCLS, P_ESCBITS, P_FSIGN,
-  P_ENDBLK,    // a block's tokens are complete: resolve the hidden/explicit first sign
-  P_BLKDONE,   // a block is complete: next block, next plane, or end of macroblock
-  P_LUMADONE,  // inter luma complete: uncoded chroma is written, coded chroma parsed
-  P_ENDMB,     // macroblock complete: write its record, start the next one
-  P_DONE
+// ----------------------------------------------------------------------------
+// Entropy decoding of one tile.  All state is wave-uniform; the lanes only
+// take part in the cumulative-table search (one lane per symbol), the byte
+// window and the coefficient staging buffer.
+// ----------------------------------------------------------------------------
+struct Rans {
+  uint32_t x0, x1, x2, x3;  // x0 is state[counter & 3] (rans.go:279-282)
+  uint32_t pos;             // tile bytes not yet consumed
+  uint32_t nsym;
+  int32_t err;
+  ByteWin win;
+
+  // Renormalisation and state rotation after a symbol (rans.go:286-303).
+  __device__ __forceinline__ bool advance(uint32_t x, int lane) {
+    while (x < kRansLower) {
+      if (pos == 0) {
+        err = kErrCorrupt | (kTileUnderrun << 8);
+        return false;
+      }
+      x = (x << 8) | win.byte(--pos, lane);
+    }
+    x0 = x1; x1 = x2; x2 = x3; x3 = x;
+    nsym++;
+    return true;
+  }
+  // ransDecoder.decode (rans.go:270-304).  c is the context's cumulative row:
+  // lane s holds the start of symbol s, lane n the total, other lanes 0xFFFF;
+  // an unused context is 0xFFFF on every lane.  Returns -1 on error.
+  __device__ __forceinline__ int decode(uint32_t c, int lane) {
+    const uint32_t x = x0;
+    const uint32_t slot = x & (kRansM - 1);
+    const int cnt = static_cast<int>(__popcll(__ballot(c <= slot)));
+    if (cnt == 0) {
+      err = kErrCorrupt | (kTileUnused << 8);
+      return -1;
+    }
+    const uint32_t start = rd_lane(c, cnt - 1);
+    const uint32_t freq = rd_lane(c, cnt) - start;
+    if (!advance(freq * (x >> kRansScaleBits) + slot - start, lane)) return -1;
+    return cnt - 1;
+  }
+  // The bypass context, fixed at {2048, 2048} (rans.go:312-322).
+  __device__ __forceinline__ int bit(int lane) {
+    const uint32_t x = x0;
+    const uint32_t slot = x & (kRansM - 1);
+    const uint32_t b = slot >> 11;
+    if (!advance(2048u * (x >> kRansScaleBits) + slot - (b << 11), lane)) return -1;
+    return static_cast<int>(b);
+  }
+  // getClassValue (mv.go:95-108): a class symbol, then class-1 raw bits under
+  // an implicit leading one.  Returns -1 on error.
+  __device__ __forceinline__ int class_value(uint32_t c, int lane) {
+    const int v = decode(c, lane);
+    if (v <= 1) return v;
+    int val = 1;
+    for (int i = 1; i < v; i++) {
+      const int b = bit(lane);
+      if (b < 0) return -1;
+      val = (val << 1) | b;
+    }
+    return val;
+  }
 };
 
-__device__ __forceinline__ uint32_t mv_pick(uint32_t m0, uint32_t m1, uint32_t m2, uint32_t m3, int i) {
-  return i == 0 ? m0 : (i == 1 ? m1 : (i == 2 ? m2 : m3));
+// Coefficient list output: lane k stages entry base+k until 64 are queued.
+struct CoefOut {
+  uint32_t* dst;
+  uint32_t base, cn, count;
+  uint32_t buf;
+  __device__ __forceinline__ void emit(uint32_t e, int lane) {
+    if (static_cast<uint32_t>(lane) == cn) buf = e;
+    count++;
+    if (++cn == 64) {
+      dst[base + lane] = buf;
+      base += 64;
+      cn = 0;
+    }
+  }
+};
+
+__device__ __forceinline__ uint32_t cum_row(const uint16_t* cum, int ctx, int lane16) {
+  return cum[ctx * kCumStride + lane16];
 }
 
-__global__ __launch_bounds__(64) void ebiv_parse(const FrameJob* __restrict__ jobs, const WorkItem* __restrict__ items) {
+// decodeBlock (residual.go:79-126) for one block: tokens in zigzag order under
+// (plane, size, band, previous class) contexts, escapes, explicit signs, and
+// the first sign hidden in the level-sum parity when the span allows.  Levels
+// go to the coefficient list; the first coefficient goes last, once its sign
+// is known.  Returns false on a decode error.
+__device__ __forceinline__ bool parse_block(Rans& R, CoefOut& out, const uint16_t* cum, int lane16, int ctxbase,
+                                            uint32_t zzv, int total, int blk, uint32_t csign, int lane) {
+  int si = 0, prev = 0, band = 0, bnext = 1, first = -1, last = -1, first_pos = 0;
+  int32_t sum_abs = 0, first_mag = 0;
+  for (;;) {
+    const int t = R.decode(cum_row(cum, ctxbase + band * kNumPrevClass + prev, lane16), lane);
+    if (t < 0) return false;
+    if (t == kTEOB) break;
+    if (t == kTZero) {
+      prev = 0;
+    } else {
+      int32_t mag;
+      if (t == kTEscape) {
+        const int e = R.class_value(cum_row(cum, kCtxEscClass, lane16), lane);
+        if (e < 0) return false;
+        mag = 5 + e;
+        prev = 2;
+      } else {
+        mag = t;
+        prev = t == kTOne ? 1 : 2;
+      }
+      sum_abs += mag;
+      if (first < 0) {
+        first = si;
+        first_mag = mag;
+        first_pos = zz_pos(zzv, si);
+      } else {
+        const int s = R.decode(csign, lane);
+        if (s < 0) return false;
+        out.emit(coef_entry(zz_pos(zzv, si), blk, s ? -mag : mag), lane);
+      }
+      last = si;
+    }
+    if (++si == total) break;
+    if (si == bnext) {  // bandOf thresholds 1, 3, 9, 21 (model.go)
+      band++;
+      bnext = band < 4 ? static_cast<int>((0x15090301u >> (8 * band)) & 0xFFu) : (1 << 20);
+    }
+  }
+  if (first >= 0) {
+    bool neg;
+    if (last - first >= kSDHMinSpan) {
+      neg = (sum_abs & 1) != 0;
+    } else {
+      const int s = R.decode(csign, lane);
+      if (s < 0) return false;
+      neg = s != 0;
+    }
+    out.emit(coef_entry(first_pos, blk, neg ? -first_mag : first_mag), lane);
+  }
+  return true;
+}
+
+// Grammar of one macroblock as decoded so far: what the record stores.
+struct MBState {
+  uint32_t kind, part, cbp, tx, refsel, cmode, wmask, lmode0, lmode1;
+  uint32_t mv0, mv1, mv2, mv3, mvcur;
+};
+
+// A macroblock after its mode, as one pass over its blocks so that the block
+// decoder is inlined once.
+//   intra (decodeLumaMB / decodeChromaMB, decode_intra.go:137-167): transform
+//     size, then per luma block its mode and residual, then the chroma mode
+//     and both chroma residuals;
+//   inter (decodeInterMB / reconInterMB, decode_inter.go:18-118): reference,
+//     partition, one MV per partition (the first against the predictor, the
+//     others against the first), the coded-block pattern, then the coded
+//     residuals (transform size first when luma is coded).
+// The write mask follows what Go writes before an error stops the tile: a
+// block once its residual decoded; uncoded inter luma once the pattern is
+// known; uncoded chroma planes once luma is complete (even when an earlier
+// coded chroma plane then fails); nothing when the failure comes before the
+// pattern.
+__device__ __forceinline__ bool parse_mb(Rans& R, CoefOut& out, MBState& m, const uint16_t* cum, int lane16,
+                                         uint32_t zz4, uint32_t zz8, uint32_t zz16, uint32_t csign, bool intra,
+                                         uint32_t pred_last, int lane) {
+  bool luma_coded = true, cb_coded = true, cr_coded = true;
+  if (!intra) {
+    const int refsel = R.decode(cum_row(cum, kCtxRef, lane16), lane);
+    if (refsel < 0) return false;
+    m.refsel = static_cast<uint32_t>(refsel);
+    const uint32_t pred = refsel == kRefGolden ? 0u : pred_last;
+    const int part = R.decode(cum_row(cum, kCtxPart, lane16), lane);
+    if (part < 0) return false;
+    m.part = static_cast<uint32_t>(part);
+    const int npart = static_cast<int>((kPartCountNibbles >> (4 * part)) & 15u);
+    for (int pi = 0; pi < npart; pi++) {
+      const uint32_t base = pi > 0 ? m.mv0 : pred;
+      uint32_t mv = 0;
+      for (int comp = 0; comp < 2; comp++) {
+        const int v = R.class_value(cum_row(cum, comp ? kCtxMVClassY : kCtxMVClassX, lane16), lane);
+        if (v < 0) return false;
+        mv |= static_cast<uint32_t>(static_cast<uint16_t>(clamp_mv(mv_comp(base, comp) + zigzag_dec(static_cast<uint32_t>(v)))))
+              << (16 * comp);
+      }
+      if (pi == 0) m.mv0 = mv;
+      else if (pi == 1) m.mv1 = mv;
+      else if (pi == 2) m.mv2 = mv;
+      else m.mv3 = mv;
+    }
+    m.mvcur = refsel == kRefGolden ? 0u : m.mv0;
+    const int cbp = R.decode(cum_row(cum, kCtxCBP, lane16), lane);
+    if (cbp < 0) return false;
+    m.cbp = static_cast<uint32_t>(cbp);
+    luma_coded = (cbp & kCBPLuma) != 0;
+    cb_coded = (cbp & kCBPCb) != 0;
+    cr_coded = (cbp & kCBPCr) != 0;
+    if (!luma_coded) m.wmask |= 0xFFFFu;
+  }
+  int nb = 0, total = 0, ctxbase = 0;
+  uint32_t zzv = zz4;
+  if (luma_coded) {
+    const int tx = R.decode(cum_row(cum, kCtxTxSize, lane16), lane);
+    if (tx < 0) return false;
+    m.tx = static_cast<uint32_t>(tx);
+    const int n = 4 << tx;
+    nb = (16 / n) * (16 / n);
+    total = n * n;
+    zzv = tx == 0 ? zz4 : (tx == 1 ? zz8 : zz16);
+    ctxbase = kCtxTokenBase + tx * kNumBands * kNumPrevClass;
+  }
+  for (int k = 0; k < nb + 2; k++) {
+    const bool chroma = k >= nb;
+    const int blk = chroma ? 16 + (k - nb) : k;
+    if (k == nb) {  // luma complete
+      if (intra) {
+        const int cmode = R.decode(cum_row(cum, kCtxChromaMode, lane16), lane);
+        if (cmode < 0) return false;
+        m.cmode = static_cast<uint32_t>(cmode);
+      } else {
+        if (!cb_coded) m.wmask |= 1u << 16;
+        if (!cr_coded) m.wmask |= 1u << 17;
+      }
+    }
+    if (chroma && !(blk == 16 ? cb_coded : cr_coded)) continue;
+    if (intra && !chroma) {
+      const int mode = R.decode(cum_row(cum, kCtxLumaMode, lane16), lane);
+      if (mode < 0) return false;
+      if (k < 8) m.lmode0 |= static_cast<uint32_t>(mode) << (4 * k);
+      else m.lmode1 |= static_cast<uint32_t>(mode) << (4 * (k - 8));
+    }
+    if (!parse_block(R, out, cum, lane16, chroma ? kCtxTokenBase + kLumaTokenCtx : ctxbase, chroma ? zz8 : zzv,
+                     chroma ? 64 : total, blk, csign, lane))
+      return false;
+    m.wmask |= 1u << blk;
+  }
+  return true;
+}
+
+// Pointers read from a job table are generic pointers; the job table is read
+// through the constant address space, and the compiler takes generic
+// pointers loaded from constant memory to be global ones, so accesses
+// through a job's buffers are global (not flat) and uniform loads can be
+// scalar.
+using ConstJob = __attribute__((address_space(4))) const FrameJob;
+using ConstItem = __attribute__((address_space(4))) const WorkItem;
+__device__ __forceinline__ ConstJob& job_of(const FrameJob* jobs, uint32_t i) {
+  return ((ConstJob*)jobs)[i];
+}
+__device__ __forceinline__ Planes planes_of(const __attribute__((address_space(4))) Planes& p) {
+  return Planes{p.y, p.cb, p.cr, p.pw, p.ph, p.cw, p.ch};
+}
+__device__ __forceinline__ WorkItem item_of(const WorkItem* items) {
+  const ConstItem& it = ((ConstItem*)items)[blockIdx.x];
+  return WorkItem{it.job, it.idx};
+}
+
+// decodeTile (decode_intra.go:121-135) for tile t: every macroblock in raster
+// order until the first error; one record per macroblock reached.
+__device__ __forceinline__ void parse_tile(const uint8_t* __restrict__ pkt, const uint8_t* __restrict__ desc,
+                                           MBRec* __restrict__ recs, uint32_t* __restrict__ coefs,
+                                           TileStatus* __restrict__ status, const int t) {
   extern __shared__ uint32_t smem[];
-  const WorkItem it = items[blockIdx.x];
-  const FrameJob& job = jobs[it.job];
-  const uint8_t* const pkt = job.pkt;
-  const uint8_t* const desc = job.desc;
-  MBRec* const recs = job.recs;
-  uint32_t* const coefs = job.coefs;
-  TileStatus* const status = job.status;
   const FrameHdr* hdr = reinterpret_cast<const FrameHdr*>(desc);
   const TileDesc* tiles = reinterpret_cast<const TileDesc*>(desc + desc_tiles_off());
   const int lane = lane_id();
-  const int t = static_cast<int>(it.idx);
+  const int lane16 = lane < 16 ? lane : 16;
   const TileDesc td = tiles[t];
 
   // LDS: cumulative tables, then two MV rows of the tile width.
-  uint16_t* cum = reinterpret_cast<uint16_t*>(smem);
+  const uint16_t* cum = reinterpret_cast<const uint16_t*>(smem);
   const uint32_t* gcum = reinterpret_cast<const uint32_t*>(hdr->cum);
   constexpr int kCumWords = 72 * kCumStride / 2;
   for (int i = lane; i < kCumWords; i += 64) smem[i] = gcum[i];
@@ -176,326 +418,103 @@ __global__ __launch_bounds__(64) void ebiv_parse(const FrameJob* __restrict__ jo
     if (lane == 0) status[t] = TileStatus{kErrCorrupt | (kTileShort << 8), 0, 0, 0};
     return;
   }
-  const uint32_t used0 = hdr->used[0], used1 = hdr->used[1], used2 = hdr->used[2];
-  ByteWin win;
-  win.init(pkt + td.byte_off, td.byte_len, lane);
-  uint32_t pos = td.byte_len;
-  uint32_t s0, s1, s2, s3;
+  Rans R;
+  R.win.init(pkt + td.byte_off, td.byte_len, lane);
+  R.pos = td.byte_len;
+  R.nsym = 0;
+  R.err = 0;
   {
     uint32_t sv[4];
 #pragma unroll
     for (int i = 3; i >= 0; i--) {  // pop32: big-endian from the top, state 3 first
-      uint32_t a = win.byte(--pos, lane);
-      uint32_t b = win.byte(--pos, lane);
-      uint32_t c = win.byte(--pos, lane);
-      uint32_t e = win.byte(--pos, lane);
+      uint32_t a = R.win.byte(--R.pos, lane);
+      uint32_t b = R.win.byte(--R.pos, lane);
+      uint32_t c = R.win.byte(--R.pos, lane);
+      uint32_t e = R.win.byte(--R.pos, lane);
       sv[i] = (a << 24) | (b << 16) | (c << 8) | e;
     }
-    s0 = sv[0]; s1 = sv[1]; s2 = sv[2]; s3 = sv[3];
+    R.x0 = sv[0]; R.x1 = sv[1]; R.x2 = sv[2]; R.x3 = sv[3];
   }
+  // Scan tables in registers: lane l holds positions 4l..4l+3 of each size.
+  uint32_t zz4 = 0, zz8 = 0, zz16 = 0;
+  {
+    const uint32_t* z = reinterpret_cast<const uint32_t*>(c_zz);
+    if (lane < 4) zz4 = z[lane];
+    if (lane < 16) zz8 = z[64 + lane];
+    zz16 = z[128 + lane];
+  }
+  const uint32_t csign = cum_row(cum, kCtxSign, lane16);
+  // Let every start-up load land here, so no load is outstanding in the
+  // macroblock loop: a wait there would also wait for every store before it.
+  __builtin_amdgcn_s_waitcnt(0);
 
-  // Coefficient output: lane k buffers entry cbase+k until 64 are queued.
-  uint32_t* const cdst = coefs + td.coef_base;
-  uint32_t cbase = 0, cn = 0, cbuf = 0;
-
+  CoefOut out;
+  out.dst = coefs + td.coef_base;
+  out.base = 0; out.cn = 0; out.count = 0; out.buf = 0;
   const int mbc = hdr->mbc;
   const bool inter_frame = hdr->inter != 0;
-  int32_t err = 0;
-  uint32_t nsym = 0, done = 0, nintra = 0;
-  int mbx = td.x0, mby = td.y0;
-
-  // Macroblock state.
-  uint32_t kind = 2, part = 0, cbp = 0, tx = 0, refsel = 0, cmode = 0, wmask = 0, ncoef = 0;
-  uint32_t lmode0 = 0, lmode1 = 0, coef_start = 0, mvcur = 0, pred = 0;
-  uint32_t mv0 = 0, mv1 = 0, mv2 = 0, mv3 = 0;
-  int pi = 0, comp = 0, npart = 1;
-  uint32_t clsval = 0;
-  int cls_bits = 0, cls = 0;
-  bool intra_mb = !inter_frame;
-  // Block state.
-  int plane = 0, blk = 0, n = 4, nb = 16, total = 16, si = 0, prev = 0, first = -1, last = -1;
-  int first_pos = 0;
-  int32_t first_mag = 0, pend_mag = 0, sum_abs = 0;
-
-  int st = inter_frame ? P_MBMODE : P_TX;
-  while (st != P_DONE) {
-    // ---------------- bookkeeping states (no symbol) ----------------
-    if (st >= P_ENDBLK) {
-      if (st == P_ENDBLK) {
-        if (first >= 0 && last - first < kSDHMinSpan) {
-          st = P_FSIGN;
-          continue;
+  uint32_t done = 0, nintra = 0;
+  bool ok = true;
+  for (int mby = td.y0; ok && mby < td.y1; mby++) {
+    for (int mbx = td.x0; mbx < td.x1; mbx++) {
+      MBState m{2, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+      const uint32_t coef_start = out.base + out.cn, count0 = out.count;
+      bool coded = true;
+      if (inter_frame) {
+        // A failed mode decode reads as skip, which Go then reconstructs.
+        const int mode = R.decode(cum_row(cum, kCtxMBMode, lane16), lane);
+        if (mode < 0 || mode == kMBSkip) {
+          const uint32_t pred = predict_mv(prow, crow, mbx - td.x0, tw, mby == td.y0);
+          m.kind = 0; m.mv0 = pred; m.mvcur = pred; m.wmask = 0x3FFFFu;
+          ok = mode >= 0;
+          coded = false;
+        } else if (mode == kMBInter) {
+          m.kind = 1;
         }
-        if (first >= 0) {  // sign hidden in the parity of the level sum
-          const uint32_t e = coef_entry(first_pos, blk, (sum_abs & 1) ? -first_mag : first_mag);
-          if (static_cast<uint32_t>(lane) == cn) cbuf = e;
-          if (++cn == 64) { cdst[cbase + lane] = cbuf; cbase += 64; cn = 0; }
-          ncoef++;
-        }
-        st = P_BLKDONE;
       }
-      if (st == P_BLKDONE) {
-        wmask |= 1u << blk;
-        si = 0; prev = 0; first = -1; last = -1; sum_abs = 0;
-        if (plane == 0) {
-          if (blk + 1 < nb) {
-            blk++;
-            st = intra_mb ? P_LMODE : P_TOKEN;
-          } else {
-            st = intra_mb ? P_CMODE : P_LUMADONE;
-          }
-        } else if (blk == 16 && (intra_mb || (cbp & kCBPCr))) {
-          blk = 17;
-          st = P_TOKEN;
-        } else {
-          st = P_ENDMB;
-        }
-        if (st != P_LUMADONE && st != P_ENDMB) continue;
+      if (coded) {
+        const bool intra = m.kind == 2;
+        const uint32_t pred = intra ? 0u : predict_mv(prow, crow, mbx - td.x0, tw, mby == td.y0);
+        ok = parse_mb(R, out, m, cum, lane16, zz4, zz8, zz16, csign, intra, pred, lane);
       }
-      if (st == P_LUMADONE) {
-        if (!(cbp & kCBPCb)) wmask |= 1u << 16;
-        if (!(cbp & kCBPCr)) wmask |= 1u << 17;
-        si = 0; prev = 0; first = -1; last = -1; sum_abs = 0;
-        plane = 1; n = 8; total = 64;
-        if (cbp & kCBPCb) { blk = 16; st = P_TOKEN; continue; }
-        if (cbp & kCBPCr) { blk = 17; st = P_TOKEN; continue; }
-        st = P_ENDMB;
-      }
-      // P_ENDMB: the macroblock record (decode_intra.go:121-135 loop body).
-      if (kind == 2) nintra++;
-      crow[mbx - td.x0] = mvcur;
+      if (m.kind == 2) nintra++;
       if (lane == 0) {
+        crow[mbx - td.x0] = m.mvcur;
         MBRec r;
-        r.kind = static_cast<uint8_t>(kind);
-        r.part = static_cast<uint8_t>(part);
-        r.cbp = static_cast<uint8_t>(cbp);
-        r.tx = static_cast<uint8_t>(tx);
-        r.refsel = static_cast<uint8_t>(refsel);
-        r.cmode = static_cast<uint8_t>(cmode);
+        r.kind = static_cast<uint8_t>(m.kind);
+        r.part = static_cast<uint8_t>(m.part);
+        r.cbp = static_cast<uint8_t>(m.cbp);
+        r.tx = static_cast<uint8_t>(m.tx);
+        r.refsel = static_cast<uint8_t>(m.refsel);
+        r.cmode = static_cast<uint8_t>(m.cmode);
         r.pad0 = 0;
         r.pad1 = 0;
-        r.wmask = wmask;
+        r.wmask = m.wmask;
         r.coef_off = td.coef_base + coef_start;
-        r.coef_count = ncoef;
-        r.mv[0][0] = mv_comp(mv0, 0); r.mv[0][1] = mv_comp(mv0, 1);
-        r.mv[1][0] = mv_comp(mv1, 0); r.mv[1][1] = mv_comp(mv1, 1);
-        r.mv[2][0] = mv_comp(mv2, 0); r.mv[2][1] = mv_comp(mv2, 1);
-        r.mv[3][0] = mv_comp(mv3, 0); r.mv[3][1] = mv_comp(mv3, 1);
-        r.lmode[0] = lmode0;
-        r.lmode[1] = lmode1;
+        r.coef_count = out.count - count0;
+        r.mv[0][0] = mv_comp(m.mv0, 0); r.mv[0][1] = mv_comp(m.mv0, 1);
+        r.mv[1][0] = mv_comp(m.mv1, 0); r.mv[1][1] = mv_comp(m.mv1, 1);
+        r.mv[2][0] = mv_comp(m.mv2, 0); r.mv[2][1] = mv_comp(m.mv2, 1);
+        r.mv[3][0] = mv_comp(m.mv3, 0); r.mv[3][1] = mv_comp(m.mv3, 1);
+        r.lmode[0] = m.lmode0;
+        r.lmode[1] = m.lmode1;
         r.pad2 = 0;
         recs[mby * mbc + mbx] = r;
       }
       done++;
-      if (err) break;
-      if (++mbx == td.x1) {
-        mbx = td.x0;
-        if (++mby == td.y1) break;
-        wave_sync();
-        uint32_t* tmp = prow; prow = crow; crow = tmp;
-      }
-      kind = 2; part = 0; cbp = 0; tx = 0; refsel = 0; cmode = 0; wmask = 0; ncoef = 0;
-      lmode0 = 0; lmode1 = 0; mvcur = 0; mv0 = mv1 = mv2 = mv3 = 0;
-      coef_start = cbase + cn;
-      intra_mb = !inter_frame;
-      plane = 0; blk = 0; si = 0; prev = 0; first = -1; last = -1; sum_abs = 0;
-      st = inter_frame ? P_MBMODE : P_TX;
-      continue;
+      if (!ok) break;
     }
-
-    // ---------------- one symbol: ransDecoder.decode (rans.go:270-304) ----------------
-    int ctx;
-    switch (st) {
-      case P_MBMODE: ctx = kCtxMBMode; break;
-      case P_REF: ctx = kCtxRef; break;
-      case P_PART: ctx = kCtxPart; break;
-      case P_MVCLS: ctx = comp ? kCtxMVClassY : kCtxMVClassX; break;
-      case P_CBP: ctx = kCtxCBP; break;
-      case P_TX: ctx = kCtxTxSize; break;
-      case P_LMODE: ctx = kCtxLumaMode; break;
-      case P_CMODE: ctx = kCtxChromaMode; break;
-      case P_TOKEN: ctx = dtoken_ctx(plane, static_cast<int>(tx), dband(si), prev); break;
-      case P_SIGN: case P_FSIGN: ctx = kCtxSign; break;
-      case P_ESCCLS: ctx = kCtxEscClass; break;
-      default: ctx = kCtxBypass; break;  // P_MVBITS, P_ESCBITS
-    }
-    int v = 0;
-    const uint32_t ub = ctx < 32 ? used0 : (ctx < 64 ? used1 : used2);
-    if (!((ub >> (ctx & 31)) & 1u)) {
-      err = kErrCorrupt | (kTileUnused << 8);
-    } else {
-      uint32_t x = s0;
-      const uint32_t slot = x & (kRansM - 1);
-      uint32_t sym, start, freq;
-      if (ctx == kCtxBypass) {  // fixed {2048, 2048} (rans.go:317-322)
-        sym = slot >> 11;
-        start = sym << 11;
-        freq = 2048;
-      } else {
-        // Lane-parallel slot2sym: lane s holds the start of symbol s; the
-        // symbol is the last start not above the slot.
-        const uint32_t c = cum[ctx * kCumStride + (lane < 16 ? lane : 16)];
-        const uint64_t m = __ballot(c <= slot);
-        sym = static_cast<uint32_t>(__popcll(m)) - 1u;
-        start = rd_lane(c, static_cast<int>(sym));
-        freq = rd_lane(c, static_cast<int>(sym) + 1) - start;
-      }
-      x = freq * (x >> kRansScaleBits) + slot - start;
-      while (x < kRansLower) {
-        if (pos == 0) {
-          err = kErrCorrupt | (kTileUnderrun << 8);
-          break;
-        }
-        --pos;
-        x = (x << 8) | win.byte(pos, lane);
-      }
-      s0 = s1; s1 = s2; s2 = s3; s3 = x;
-      nsym++;
-      v = static_cast<int>(sym);
-    }
-    if (err) {
-      // The failing macroblock keeps what Go wrote of it: a failed mode
-      // decode reads as skip (decode_inter.go:9-16); otherwise only the
-      // blocks and planes completed before the failure.
-      if (st == P_MBMODE) {
-        pred = predict_mv(prow, crow, mbx - td.x0, tw, mby == td.y0);
-        kind = 0; mv0 = pred; mvcur = pred; wmask = 0x3FFFFu;
-      }
-      st = P_ENDMB;
-      continue;
-    }
-
-    // ---------------- grammar transition ----------------
-    bool has_emit = false;
-    uint32_t entry = 0;
-    int32_t mag = 0;  // > 0: a magnitude was completed this step
-    int mvval = -1;   // >= 0: an MV class value was completed this step
-    switch (st) {
-      case P_MBMODE:
-        if (v == kMBSkip) {
-          pred = predict_mv(prow, crow, mbx - td.x0, tw, mby == td.y0);
-          kind = 0; mv0 = pred; mvcur = pred; wmask = 0x3FFFFu;
-          st = P_ENDMB;
-        } else if (v == kMBIntra) {
-          kind = 2; intra_mb = true; mvcur = 0;
-          st = P_TX;
-        } else {
-          kind = 1;
-          st = P_REF;
-        }
-        break;
-      case P_REF:
-        refsel = static_cast<uint32_t>(v);
-        pred = v == kRefGolden ? 0u : predict_mv(prow, crow, mbx - td.x0, tw, mby == td.y0);
-        st = P_PART;
-        break;
-      case P_PART:
-        part = static_cast<uint32_t>(v);
-        npart = c_part_count[v];
-        pi = 0; comp = 0;
-        st = P_MVCLS;
-        break;
-      case P_MVCLS:
-        if (v <= 1) mvval = v;
-        else { cls = v; cls_bits = v - 1; clsval = 0; st = P_MVBITS; }
-        break;
-      case P_MVBITS:
-        clsval = (clsval << 1) | static_cast<uint32_t>(v);
-        if (--cls_bits == 0) mvval = static_cast<int>((1u << (cls - 1)) | clsval);
-        break;
-      case P_CBP:
-        cbp = static_cast<uint32_t>(v);
-        if (cbp & kCBPLuma) {
-          st = P_TX;
-        } else {
-          wmask |= 0xFFFFu;
-          st = P_LUMADONE;
-        }
-        break;
-      case P_TX:
-        tx = static_cast<uint32_t>(v);
-        n = 4 << v; total = n * n; nb = (16 / n) * (16 / n);
-        plane = 0; blk = 0; si = 0; prev = 0; first = -1; last = -1; sum_abs = 0;
-        st = intra_mb ? P_LMODE : P_TOKEN;
-        break;
-      case P_LMODE:
-        if (blk < 8) lmode0 |= static_cast<uint32_t>(v) << (4 * blk);
-        else lmode1 |= static_cast<uint32_t>(v) << (4 * (blk - 8));
-        st = P_TOKEN;
-        break;
-      case P_CMODE:
-        cmode = static_cast<uint32_t>(v);
-        plane = 1; blk = 16; n = 8; total = 64;
-        si = 0; prev = 0; first = -1; last = -1; sum_abs = 0;
-        st = P_TOKEN;
-        break;
-      case P_TOKEN:
-        if (v == kTEOB) {
-          st = P_ENDBLK;
-        } else if (v == kTZero) {
-          prev = 0;
-          st = (++si == total) ? P_ENDBLK : P_TOKEN;
-        } else if (v == kTEscape) {
-          st = P_ESCCLS;
-        } else {
-          prev = v == 1 ? 1 : 2;
-          mag = v;
-        }
-        break;
-      case P_SIGN:
-        entry = coef_entry(c_zz[sidx_of(n)][si], blk, v ? -pend_mag : pend_mag);
-        has_emit = true;
-        last = si;
-        st = (++si == total) ? P_ENDBLK : P_TOKEN;
-        break;
-      case P_FSIGN:
-        entry = coef_entry(first_pos, blk, v ? -first_mag : first_mag);
-        has_emit = true;
-        st = P_BLKDONE;
-        break;
-      case P_ESCCLS:
-        if (v <= 1) { prev = 2; mag = 5 + v; }
-        else { cls = v; cls_bits = v - 1; clsval = 0; st = P_ESCBITS; }
-        break;
-      default:  // P_ESCBITS
-        clsval = (clsval << 1) | static_cast<uint32_t>(v);
-        if (--cls_bits == 0) { prev = 2; mag = 5 + static_cast<int32_t>((1u << (cls - 1)) | clsval); }
-        break;
-    }
-    if (mag > 0) {  // a coefficient magnitude is known (residual.go:99-117)
-      sum_abs += mag;
-      if (first < 0) {
-        first = si;
-        first_mag = mag;
-        first_pos = c_zz[sidx_of(n)][si];
-        last = si;
-        st = (++si == total) ? P_ENDBLK : P_TOKEN;
-      } else {
-        pend_mag = mag;
-        st = P_SIGN;
-      }
-    }
-    if (mvval >= 0) {  // an MV delta component is known (decode_inter.go:36-45)
-      const int d = zigzag_dec(static_cast<uint32_t>(mvval));
-      const uint32_t base = pi > 0 ? mv0 : pred;
-      const uint32_t nv = mv_set(mv_pick(mv0, mv1, mv2, mv3, pi), comp, clamp_mv(mv_comp(base, comp) + d));
-      if (pi == 0) mv0 = nv; else if (pi == 1) mv1 = nv; else if (pi == 2) mv2 = nv; else mv3 = nv;
-      comp ^= 1;
-      if (comp == 0 && ++pi == npart) {
-        mvcur = refsel == kRefGolden ? 0u : mv0;
-        st = P_CBP;
-      } else {
-        st = P_MVCLS;
-      }
-    }
-    if (has_emit) {
-      if (static_cast<uint32_t>(lane) == cn) cbuf = entry;
-      if (++cn == 64) { cdst[cbase + lane] = cbuf; cbase += 64; cn = 0; }
-      ncoef++;
-    }
+    wave_sync();
+    uint32_t* tmp = prow; prow = crow; crow = tmp;
   }
-  if (static_cast<uint32_t>(lane) < cn) cdst[cbase + lane] = cbuf;
-  if (lane == 0) status[t] = TileStatus{err, done, nsym, nintra};
+  if (static_cast<uint32_t>(lane) < out.cn) out.dst[out.base + lane] = out.buf;
+  if (lane == 0) status[t] = TileStatus{R.err, done, R.nsym, nintra};
+}
+
+__global__ __launch_bounds__(64) void ebiv_parse(const FrameJob* __restrict__ jobs, const WorkItem* __restrict__ items) {
+  const WorkItem it = item_of(items);
+  ConstJob& job = job_of(jobs, it.job);
+  parse_tile(job.pkt, job.desc, job.recs, job.coefs, job.status, static_cast<int>(it.idx));
 }
 
 // ============================================================================
@@ -618,21 +637,16 @@ __device__ __forceinline__ int tile_of(const uint8_t* desc, const FrameHdr* hdr,
 // ============================================================================
 // Inter reconstruction: one wavefront per macroblock
 // ============================================================================
-__global__ __launch_bounds__(256) void ebiv_recon_inter(const FrameJob* __restrict__ jobs,
-                                                        const WorkItem* __restrict__ items) {
+__device__ __forceinline__ void recon_inter_mb(const uint8_t* __restrict__ desc, const MBRec* __restrict__ recs,
+                                               const uint32_t* __restrict__ coefs,
+                                               const TileStatus* __restrict__ status, const Planes rec,
+                                               const Planes ref, const Planes gold, const int first_mb) {
   __shared__ WaveScratch scratch[4];
-  const WorkItem it = items[blockIdx.x];
-  const FrameJob& job = jobs[it.job];
-  const uint8_t* const desc = job.desc;
-  const MBRec* const recs = job.recs;
-  const uint32_t* const coefs = job.coefs;
-  const TileStatus* const status = job.status;
-  const Planes rec = job.rec, ref = job.ref, gold = job.gold;
   const FrameHdr* hdr = reinterpret_cast<const FrameHdr*>(desc);
   const TileDesc* tiles = reinterpret_cast<const TileDesc*>(desc + desc_tiles_off());
   const int lane = lane_id(), wave = threadIdx.x >> 6;
   const int mbc = hdr->mbc, nmb = mbc * hdr->mbr;
-  const int mb = static_cast<int>(it.idx) + wave;
+  const int mb = first_mb + wave;
   if (mb >= nmb) return;
   const int mbx = mb % mbc, mby = mb / mbc;
   const int t = tile_of(desc, hdr, mbx, mby);
@@ -696,6 +710,14 @@ __global__ __launch_bounds__(256) void ebiv_recon_inter(const FrameJob* __restri
       *reinterpret_cast<uint16_t*>(dp + static_cast<size_t>(mby * 8 + y) * rec.cw + mbx * 8 + x0) = static_cast<uint16_t>(half);
     }
   }
+}
+
+__global__ __launch_bounds__(256) void ebiv_recon_inter(const FrameJob* __restrict__ jobs,
+                                                        const WorkItem* __restrict__ items) {
+  const WorkItem it = item_of(items);
+  ConstJob& job = job_of(jobs, it.job);
+  recon_inter_mb(job.desc, job.recs, job.coefs, job.status, planes_of(job.rec), planes_of(job.ref), planes_of(job.gold),
+                 static_cast<int>(it.idx));
 }
 
 // ============================================================================
@@ -903,20 +925,13 @@ __device__ void intra_mb(IntraScratch& S, const uint32_t* __restrict__ coefs, co
 // One workgroup per tile.  Macroblock (x, y) depends on (x-1, y), (x-1, y-1),
 // (x, y-1) and (x+1, y-1), so step s = x + 2y (tile-relative) may run all its
 // macroblocks at once.
-__global__ __launch_bounds__(256) void ebiv_recon_intra(const FrameJob* __restrict__ jobs,
-                                                        const WorkItem* __restrict__ items) {
+__device__ __forceinline__ void recon_intra_tile(const uint8_t* __restrict__ desc, const MBRec* __restrict__ recs,
+                                                 const uint32_t* __restrict__ coefs,
+                                                 const TileStatus* __restrict__ status, const Planes rec, const int t) {
   __shared__ IntraScratch scratch[4];
-  const WorkItem it = items[blockIdx.x];
-  const FrameJob& job = jobs[it.job];
-  const uint8_t* const desc = job.desc;
-  const MBRec* const recs = job.recs;
-  const uint32_t* const coefs = job.coefs;
-  const TileStatus* const status = job.status;
-  const Planes rec = job.rec;
   const FrameHdr* hdr = reinterpret_cast<const FrameHdr*>(desc);
   const TileDesc* tiles = reinterpret_cast<const TileDesc*>(desc + desc_tiles_off());
   const int lane = lane_id(), wave = threadIdx.x >> 6;
-  const int t = static_cast<int>(it.idx);
   const TileDesc td = tiles[t];
   const TileStatus st = status[t];
   if (st.n_intra == 0) return;
@@ -944,6 +959,13 @@ __global__ __launch_bounds__(256) void ebiv_recon_intra(const FrameJob* __restri
     __syncthreads();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
+}
+
+__global__ __launch_bounds__(256) void ebiv_recon_intra(const FrameJob* __restrict__ jobs,
+                                                        const WorkItem* __restrict__ items) {
+  const WorkItem it = item_of(items);
+  ConstJob& job = job_of(jobs, it.job);
+  recon_intra_tile(job.desc, job.recs, job.coefs, job.status, planes_of(job.rec), static_cast<int>(it.idx));
 }
 
 // ============================================================================

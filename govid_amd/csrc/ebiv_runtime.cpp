@@ -378,6 +378,10 @@ struct Session {
     uint8_t* hd = ps->h_desc.h;
     std::memset(hd, 0, dsize);
     FrameHdr& hdr = *reinterpret_cast<FrameHdr*>(hd);
+    // A context no table installs reads as 0xFFFF on every lane of the
+    // entropy kernel's table search, which finds no symbol there: the
+    // "symbol decoded from unused context" error of rans.go:275-278.
+    std::memset(hdr.cum, 0xFF, sizeof hdr.cum);
     size_t consumed = 0;
     int rc = parse_tables(body + pos, static_cast<size_t>(tlen), key, hdr, &consumed);
     if (rc) return rc;
