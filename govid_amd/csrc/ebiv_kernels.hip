@@ -47,39 +47,39 @@ __device__ __forceinline__ uint32_t rd_lane(uint32_t v, int l) { return __builti
 // ============================================================================
 
 // Descending byte reader over a tile stream.  Each lane holds one dword of a
-// 256-byte window, so reading a byte is a readlane.  The next lower window is
-// loaded when the reader crosses into it and waited for there: a load kept in
-// flight across the symbol loop would make every loop-carried register copy
-// wait for it, and in-order completion would also make it wait for every
-// coefficient store before it.
+// 256-byte window, so reading a byte is a readlane.  Offsets are 32-bit and
+// relative to the tile's first byte (the window base may lie below it).  The
+// next lower window is loaded when the reader crosses into it and waited for
+// there: a load kept in flight across the symbol loop would make every
+// loop-carried register copy wait for it, and in-order completion would also
+// make it wait for every coefficient store before it.
 struct ByteWin {
-  uintptr_t tile;      // absolute address of tile byte 0
-  uintptr_t lo, hi;    // dword-aligned readable range [lo, hi)
-  uintptr_t wbase;     // 256-aligned base of the current window
+  const uint8_t* tile;  // tile byte 0
+  int32_t lo, hi;       // dword-aligned readable range [lo, hi), tile-relative
+  int32_t wbase;        // current window base, tile-relative (tile + wbase is 256-aligned)
   uint32_t cur;
 
-  __device__ __forceinline__ uint32_t fetch(uintptr_t wb, int lane) const {
-    uintptr_t a = wb + 4u * lane;
+  __device__ __forceinline__ void load(int lane) {
+    const int32_t a = wbase + 4 * lane;
     using GWord = __attribute__((address_space(1))) const uint32_t;  // packets live in device global memory
-    return (a >= lo && a < hi) ? *(GWord*)a : 0u;
+    cur = (a >= lo && a < hi) ? *(GWord*)(tile + a) : 0u;
   }
   __device__ void init(const uint8_t* t, uint32_t len, int lane) {
-    tile = reinterpret_cast<uintptr_t>(t);
-    lo = tile & ~uintptr_t(3);
-    hi = (tile + len + 3) & ~uintptr_t(3);
-    wbase = (tile + (len ? len - 1 : 0)) & ~uintptr_t(255);
-    cur = fetch(wbase, lane);
+    tile = t;
+    const int32_t mis = static_cast<int32_t>(reinterpret_cast<uintptr_t>(t) & 255u);
+    lo = -static_cast<int32_t>(reinterpret_cast<uintptr_t>(t) & 3u);
+    hi = (static_cast<int32_t>(len) - lo + 3) / 4 * 4 + lo;
+    wbase = ((static_cast<int32_t>(len ? len - 1 : 0) + mis) & ~255) - mis;
+    load(lane);
   }
   // Byte p of the tile; p only ever decreases between calls.
-  __device__ __forceinline__ uint32_t byte(uint32_t p, int lane) {
-    uintptr_t a = tile + p;
-    if (a < wbase) {
+  __device__ __forceinline__ uint32_t byte(int32_t p, int lane) {
+    if (p < wbase) {
       wbase -= 256;
-      cur = fetch(wbase, lane);
+      load(lane);
     }
-    uint32_t off = static_cast<uint32_t>(a - wbase);
-    uint32_t w = rd_lane(cur, static_cast<int>(off >> 2));
-    return (w >> ((off & 3u) * 8u)) & 0xFFu;
+    const uint32_t off = static_cast<uint32_t>(p - wbase);
+    return (rd_lane(cur, static_cast<int>(off >> 2)) >> ((off & 3u) * 8u)) & 0xFFu;
   }
 };
 
@@ -155,7 +155,7 @@ struct Rans {
         err = kErrCorrupt | (kTileUnderrun << 8);
         return false;
       }
-      x = (x << 8) | win.byte(--pos, lane);
+      x = (x << 8) | win.byte(static_cast<int32_t>(--pos), lane);
     }
     x0 = x1; x1 = x2; x2 = x3; x3 = x;
     nsym++;
@@ -216,8 +216,17 @@ struct CoefOut {
   }
 };
 
-__device__ __forceinline__ uint32_t cum_row(const uint16_t* cum, int ctx, int lane16) {
-  return cum[ctx * kCumStride + lane16];
+// LDS reads by 32-bit byte address: address arithmetic stays 32-bit and
+// uniform parts of an address stay scalar.
+using LdsU16 = __attribute__((address_space(3))) const uint16_t;
+__device__ __forceinline__ uint32_t lds_u16(uint32_t addr) { return *(LdsU16*)addr; }
+__device__ __forceinline__ uint32_t lds_addr(const void* p) {
+  return static_cast<uint32_t>(reinterpret_cast<uintptr_t>((__attribute__((address_space(3))) const void*)p));
+}
+
+// Context ctx's cumulative row; lanes is this lane's entry of context 0.
+__device__ __forceinline__ uint32_t cum_row(uint32_t lanes, int ctx) {
+  return lds_u16(lanes + static_cast<uint32_t>(ctx * kCumStride * 2));
 }
 
 // decodeBlock (residual.go:79-126) for one block: tokens in zigzag order under
@@ -225,26 +234,29 @@ __device__ __forceinline__ uint32_t cum_row(const uint16_t* cum, int ctx, int la
 // the first sign hidden in the level-sum parity when the span allows.  Levels
 // go to the coefficient list; the first coefficient goes last, once its sign
 // is known.  Returns false on a decode error.
-__device__ __forceinline__ bool parse_block(Rans& R, CoefOut& out, const uint16_t* cum, int lane16, int ctxbase,
-                                            uint32_t zzv, int total, int blk, uint32_t csign, int lane) {
-  int si = 0, prev = 0, band = 0, bnext = 1, first = -1, last = -1, first_pos = 0;
+__device__ __forceinline__ bool parse_block(Rans& R, CoefOut& out, uint32_t lanes, int ctxbase, uint32_t zzv,
+                                            int total, int blk, uint32_t csign, int lane) {
+  // This lane's entry of the block's first token context; the context of a
+  // token is a uniform row offset from it.
+  const uint32_t rows = lanes + static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(ctxbase) * kCumStride * 2);
+  // Row offsets in bytes: band * 3 rows, previous class * 1 row.
+  constexpr uint32_t kRow = kCumStride * 2;
+  uint32_t bandoff = 0, prevoff = 0;
+  int si = 0, band = 0, bnext = 1, first = -1, last = -1, first_pos = 0;
   int32_t sum_abs = 0, first_mag = 0;
   for (;;) {
-    const int t = R.decode(cum_row(cum, ctxbase + band * kNumPrevClass + prev, lane16), lane);
+    const int t = R.decode(lds_u16(rows + (bandoff + prevoff)), lane);
     if (t < 0) return false;
     if (t == kTEOB) break;
     if (t == kTZero) {
-      prev = 0;
+      prevoff = 0;
     } else {
-      int32_t mag;
+      int32_t mag = t;
+      prevoff = t == kTOne ? kRow : 2 * kRow;
       if (t == kTEscape) {
-        const int e = R.class_value(cum_row(cum, kCtxEscClass, lane16), lane);
+        const int e = R.class_value(cum_row(lanes, kCtxEscClass), lane);
         if (e < 0) return false;
         mag = 5 + e;
-        prev = 2;
-      } else {
-        mag = t;
-        prev = t == kTOne ? 1 : 2;
       }
       sum_abs += mag;
       if (first < 0) {
@@ -259,16 +271,15 @@ __device__ __forceinline__ bool parse_block(Rans& R, CoefOut& out, const uint16_
       last = si;
     }
     if (++si == total) break;
-    if (si == bnext) {  // bandOf thresholds 1, 3, 9, 21 (model.go)
+    if (si == bnext) {  // bandOf thresholds 1, 3, 9, 21 (model.go:165-180)
       band++;
+      bandoff += kNumPrevClass * kRow;
       bnext = band < 4 ? static_cast<int>((0x15090301u >> (8 * band)) & 0xFFu) : (1 << 20);
     }
   }
   if (first >= 0) {
-    bool neg;
-    if (last - first >= kSDHMinSpan) {
-      neg = (sum_abs & 1) != 0;
-    } else {
+    bool neg = (sum_abs & 1) != 0;
+    if (last - first < kSDHMinSpan) {
       const int s = R.decode(csign, lane);
       if (s < 0) return false;
       neg = s != 0;
@@ -298,16 +309,16 @@ struct MBState {
 // known; uncoded chroma planes once luma is complete (even when an earlier
 // coded chroma plane then fails); nothing when the failure comes before the
 // pattern.
-__device__ __forceinline__ bool parse_mb(Rans& R, CoefOut& out, MBState& m, const uint16_t* cum, int lane16,
+__device__ __forceinline__ bool parse_mb(Rans& R, CoefOut& out, MBState& m, uint32_t lanes,
                                          uint32_t zz4, uint32_t zz8, uint32_t zz16, uint32_t csign, bool intra,
                                          uint32_t pred_last, int lane) {
   bool luma_coded = true, cb_coded = true, cr_coded = true;
   if (!intra) {
-    const int refsel = R.decode(cum_row(cum, kCtxRef, lane16), lane);
+    const int refsel = R.decode(cum_row(lanes, kCtxRef), lane);
     if (refsel < 0) return false;
     m.refsel = static_cast<uint32_t>(refsel);
     const uint32_t pred = refsel == kRefGolden ? 0u : pred_last;
-    const int part = R.decode(cum_row(cum, kCtxPart, lane16), lane);
+    const int part = R.decode(cum_row(lanes, kCtxPart), lane);
     if (part < 0) return false;
     m.part = static_cast<uint32_t>(part);
     const int npart = static_cast<int>((kPartCountNibbles >> (4 * part)) & 15u);
@@ -315,7 +326,7 @@ __device__ __forceinline__ bool parse_mb(Rans& R, CoefOut& out, MBState& m, cons
       const uint32_t base = pi > 0 ? m.mv0 : pred;
       uint32_t mv = 0;
       for (int comp = 0; comp < 2; comp++) {
-        const int v = R.class_value(cum_row(cum, comp ? kCtxMVClassY : kCtxMVClassX, lane16), lane);
+        const int v = R.class_value(cum_row(lanes, comp ? kCtxMVClassY : kCtxMVClassX), lane);
         if (v < 0) return false;
         mv |= static_cast<uint32_t>(static_cast<uint16_t>(clamp_mv(mv_comp(base, comp) + zigzag_dec(static_cast<uint32_t>(v)))))
               << (16 * comp);
@@ -326,7 +337,7 @@ __device__ __forceinline__ bool parse_mb(Rans& R, CoefOut& out, MBState& m, cons
       else m.mv3 = mv;
     }
     m.mvcur = refsel == kRefGolden ? 0u : m.mv0;
-    const int cbp = R.decode(cum_row(cum, kCtxCBP, lane16), lane);
+    const int cbp = R.decode(cum_row(lanes, kCtxCBP), lane);
     if (cbp < 0) return false;
     m.cbp = static_cast<uint32_t>(cbp);
     luma_coded = (cbp & kCBPLuma) != 0;
@@ -337,7 +348,7 @@ __device__ __forceinline__ bool parse_mb(Rans& R, CoefOut& out, MBState& m, cons
   int nb = 0, total = 0, ctxbase = 0;
   uint32_t zzv = zz4;
   if (luma_coded) {
-    const int tx = R.decode(cum_row(cum, kCtxTxSize, lane16), lane);
+    const int tx = R.decode(cum_row(lanes, kCtxTxSize), lane);
     if (tx < 0) return false;
     m.tx = static_cast<uint32_t>(tx);
     const int n = 4 << tx;
@@ -351,7 +362,7 @@ __device__ __forceinline__ bool parse_mb(Rans& R, CoefOut& out, MBState& m, cons
     const int blk = chroma ? 16 + (k - nb) : k;
     if (k == nb) {  // luma complete
       if (intra) {
-        const int cmode = R.decode(cum_row(cum, kCtxChromaMode, lane16), lane);
+        const int cmode = R.decode(cum_row(lanes, kCtxChromaMode), lane);
         if (cmode < 0) return false;
         m.cmode = static_cast<uint32_t>(cmode);
       } else {
@@ -361,12 +372,12 @@ __device__ __forceinline__ bool parse_mb(Rans& R, CoefOut& out, MBState& m, cons
     }
     if (chroma && !(blk == 16 ? cb_coded : cr_coded)) continue;
     if (intra && !chroma) {
-      const int mode = R.decode(cum_row(cum, kCtxLumaMode, lane16), lane);
+      const int mode = R.decode(cum_row(lanes, kCtxLumaMode), lane);
       if (mode < 0) return false;
       if (k < 8) m.lmode0 |= static_cast<uint32_t>(mode) << (4 * k);
       else m.lmode1 |= static_cast<uint32_t>(mode) << (4 * (k - 8));
     }
-    if (!parse_block(R, out, cum, lane16, chroma ? kCtxTokenBase + kLumaTokenCtx : ctxbase, chroma ? zz8 : zzv,
+    if (!parse_block(R, out, lanes, chroma ? kCtxTokenBase + kLumaTokenCtx : ctxbase, chroma ? zz8 : zzv,
                      chroma ? 64 : total, blk, csign, lane))
       return false;
     m.wmask |= 1u << blk;
@@ -405,7 +416,7 @@ __device__ __forceinline__ void parse_tile(const uint8_t* __restrict__ pkt, cons
   const TileDesc td = tiles[t];
 
   // LDS: cumulative tables, then two MV rows of the tile width.
-  const uint16_t* cum = reinterpret_cast<const uint16_t*>(smem);
+  const uint32_t lanes = lds_addr(smem) + static_cast<uint32_t>(2 * lane16);
   const uint32_t* gcum = reinterpret_cast<const uint32_t*>(hdr->cum);
   constexpr int kCumWords = 72 * kCumStride / 2;
   for (int i = lane; i < kCumWords; i += 64) smem[i] = gcum[i];
@@ -427,10 +438,10 @@ __device__ __forceinline__ void parse_tile(const uint8_t* __restrict__ pkt, cons
     uint32_t sv[4];
 #pragma unroll
     for (int i = 3; i >= 0; i--) {  // pop32: big-endian from the top, state 3 first
-      uint32_t a = R.win.byte(--R.pos, lane);
-      uint32_t b = R.win.byte(--R.pos, lane);
-      uint32_t c = R.win.byte(--R.pos, lane);
-      uint32_t e = R.win.byte(--R.pos, lane);
+      uint32_t a = R.win.byte(static_cast<int32_t>(--R.pos), lane);
+      uint32_t b = R.win.byte(static_cast<int32_t>(--R.pos), lane);
+      uint32_t c = R.win.byte(static_cast<int32_t>(--R.pos), lane);
+      uint32_t e = R.win.byte(static_cast<int32_t>(--R.pos), lane);
       sv[i] = (a << 24) | (b << 16) | (c << 8) | e;
     }
     R.x0 = sv[0]; R.x1 = sv[1]; R.x2 = sv[2]; R.x3 = sv[3];
@@ -443,7 +454,7 @@ __device__ __forceinline__ void parse_tile(const uint8_t* __restrict__ pkt, cons
     if (lane < 16) zz8 = z[64 + lane];
     zz16 = z[128 + lane];
   }
-  const uint32_t csign = cum_row(cum, kCtxSign, lane16);
+  const uint32_t csign = cum_row(lanes, kCtxSign);
   // Let every start-up load land here, so no load is outstanding in the
   // macroblock loop: a wait there would also wait for every store before it.
   __builtin_amdgcn_s_waitcnt(0);
@@ -462,7 +473,7 @@ __device__ __forceinline__ void parse_tile(const uint8_t* __restrict__ pkt, cons
       bool coded = true;
       if (inter_frame) {
         // A failed mode decode reads as skip, which Go then reconstructs.
-        const int mode = R.decode(cum_row(cum, kCtxMBMode, lane16), lane);
+        const int mode = R.decode(cum_row(lanes, kCtxMBMode), lane);
         if (mode < 0 || mode == kMBSkip) {
           const uint32_t pred = predict_mv(prow, crow, mbx - td.x0, tw, mby == td.y0);
           m.kind = 0; m.mv0 = pred; m.mvcur = pred; m.wmask = 0x3FFFFu;
@@ -475,7 +486,7 @@ __device__ __forceinline__ void parse_tile(const uint8_t* __restrict__ pkt, cons
       if (coded) {
         const bool intra = m.kind == 2;
         const uint32_t pred = intra ? 0u : predict_mv(prow, crow, mbx - td.x0, tw, mby == td.y0);
-        ok = parse_mb(R, out, m, cum, lane16, zz4, zz8, zz16, csign, intra, pred, lane);
+        ok = parse_mb(R, out, m, lanes, zz4, zz8, zz16, csign, intra, pred, lane);
       }
       if (m.kind == 2) nintra++;
       if (lane == 0) {
