@@ -171,6 +171,41 @@ def test_intra_d45_reads_above_right():
                 assert pr2[r, c] == ext[r + c + 1]
 
 
+def test_intra_angular_modes_follow_a_plane():
+    """Angular prediction (predict.go:150-200) copies the reference along the
+    mode's direction: on a linear plane every directional mode reproduces the
+    plane at the point where that direction meets the reference row or column
+    (within rounding; up to 2 where a negative angle folds the side reference
+    into the main one through invAngle).  Below-left samples are never
+    available (predict.go:74-76) and repeat the lowest left sample.  A wrong
+    angle sign, axis or interpolation weight moves the sampled point and fails."""
+    H = W = 48
+    ys, xs = np.mgrid[0:H, 0:W]
+    p = (20 + 3 * xs + 2 * ys).astype(np.uint8)
+
+    def plane(x, y):
+        return 20 + 3 * x + 2 * y
+
+    ang = {4: (32, True), 5: (13, True), 6: (-13, True), 7: (32, False), 8: (13, False), 9: (-13, False)}
+    x0 = y0 = 16
+    for n in (4, 8, 16):
+        for mode, (a, vertical) in ang.items():
+            pr = O.predict_intra(p, x0, y0, n, mode, True, True, True)
+            for r in range(n):
+                for c in range(n):
+                    xc, yr = x0 + c, y0 + r
+                    if vertical:
+                        xt = xc + (r + 1) * a / 32
+                        folded = xt < x0 - 1
+                        want = plane(x0 - 1, yr - (xc - x0 + 1) * 32 / abs(a)) if folded else plane(xt, y0 - 1)
+                    else:
+                        yl = yr + (c + 1) * a / 32
+                        folded = yl < y0 - 1
+                        want = (plane(xc - (yr - y0 + 1) * 32 / abs(a), y0 - 1) if folded
+                                else plane(x0 - 1, min(yl, y0 + n - 1)))
+                    assert abs(pr[r, c] - want) <= (2 if folded else 0.5), (n, mode, r, c)
+
+
 def test_above_right_availability():
     """aboveRightAvailable (predict.go:50-58) on the 4x4 grid of a macroblock."""
     # tile [0, 64) x [0, 64), MB at (16, 16); blocks of 4
@@ -209,6 +244,21 @@ def test_mc_taps_on_a_ramp():
     for fx in range(8):
         o = O.mc_chroma(ramp, 20, 20, fx, 0, 8, 4)
         assert o[0, 0] == (64 * 40 + fx * 8 * 2 + 32) >> 6
+
+
+def test_mc_luma_2d_phase_on_a_plane():
+    """The 2-D luma path (horizontal pass on 4 rows, then vertical,
+    motioncomp.go:214-256) reproduces a linear plane at the quarter-pel point
+    (x + fx/4, y + fy/4), for all 16 phases."""
+    ys, xs = np.mgrid[0:40, 0:40]
+    p = (10 + 4 * xs + 2 * ys).astype(np.uint8)
+    for fx in range(4):
+        for fy in range(4):
+            o = O.mc_luma(p, 12, 14, fx, fy, 8, 8)
+            for r in range(8):
+                for c in range(8):
+                    want = 10 + 4 * (12 + c + fx / 4) + 2 * (14 + r + fy / 4)
+                    assert abs(o[r, c] - want) <= 0.5, (fx, fy, r, c)
 
 
 def test_mc_clamps_to_padded_plane():
