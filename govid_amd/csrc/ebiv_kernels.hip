@@ -219,7 +219,7 @@ struct CoefOut {
 // LDS reads by 32-bit byte address: address arithmetic stays 32-bit and
 // uniform parts of an address stay scalar.
 using LdsU16 = __attribute__((address_space(3))) const uint16_t;
-__device__ __forceinline__ uint32_t lds_u16(uint32_t addr) { return *(LdsU16*)addr; }
+__device__ __forceinline__ uint32_t lds_u16(uint32_t addr) { return *(LdsU16*)static_cast<uintptr_t>(addr); }
 __device__ __forceinline__ uint32_t lds_addr(const void* p) {
   return static_cast<uint32_t>(reinterpret_cast<uintptr_t>((__attribute__((address_space(3))) const void*)p));
 }
@@ -566,10 +566,22 @@ __device__ void load_coefs(WaveScratch& ws, const uint32_t* __restrict__ coefs, 
 }
 
 // Separable inverse DCT of every block (transform.go:129-194): column pass
-// into tmp, row pass back into coef, with 64-bit accumulation and the
-// per-pass (x + 4096) >> 13 rounding.  Zero coefficients add nothing, so the
-// result equals the reference's sparse form bit for bit.
-__device__ void idct_all(WaveScratch& ws, int n, bool luma, bool chroma, int lane) {
+// into tmp, row pass back into coef, with the per-pass (x + 4096) >> 13
+// rounding.  Zero coefficients add nothing, so the result equals the
+// reference's sparse form bit for bit.  The reference accumulates in 64 bits;
+// when every input is at most kIdct32Max in magnitude both passes' sums stay
+// below 2^31 (a column of the n = 16 matrix sums to at most 45,488 in
+// magnitude, so the first pass is at most 5.56x the input and the second
+// 45,488 x that), and 32-bit accumulation gives the same integers.
+constexpr int32_t kIdct32Max = 8190;
+template <typename Acc>
+__device__ __forceinline__ void idct_passes(WaveScratch& ws, int n, bool luma, bool chroma, int lane);
+__device__ void idct_all(WaveScratch& ws, int n, bool luma, bool chroma, bool narrow, int lane) {
+  if (narrow) idct_passes<int32_t>(ws, n, luma, chroma, lane);
+  else idct_passes<int64_t>(ws, n, luma, chroma, lane);
+}
+template <typename Acc>
+__device__ __forceinline__ void idct_passes(WaveScratch& ws, int n, bool luma, bool chroma, int lane) {
   const int32_t* cl = c_dct[sidx_of(n)];
   const int32_t* c8 = c_dct[1];
   const int nn = n * n;
@@ -581,8 +593,8 @@ __device__ void idct_all(WaveScratch& ws, int n, bool luma, bool chroma, int lan
     if (e < 256) { bn = n; base = (e / nn) * nn; local = e - base; c = cl; }
     else { bn = 8; base = 256 + ((e - 256) >> 6) * 64; local = (e - 256) & 63; c = c8; }
     int i = local / bn, l = local - i * bn;
-    int64_t acc = 0;
-    for (int k = 0; k < bn; k++) acc += static_cast<int64_t>(c[k * bn + i]) * ws.coef[base + k * bn + l];
+    Acc acc = 0;
+    for (int k = 0; k < bn; k++) acc += static_cast<Acc>(c[k * bn + i]) * ws.coef[base + k * bn + l];
     ws.tmp[e] = static_cast<int32_t>((acc + 4096) >> 13);
   }
   wave_sync();
@@ -594,8 +606,8 @@ __device__ void idct_all(WaveScratch& ws, int n, bool luma, bool chroma, int lan
     if (e < 256) { bn = n; base = (e / nn) * nn; local = e - base; c = cl; }
     else { bn = 8; base = 256 + ((e - 256) >> 6) * 64; local = (e - 256) & 63; c = c8; }
     int i = local / bn, mm = local - i * bn;
-    int64_t acc = 0;
-    for (int l = 0; l < bn; l++) acc += static_cast<int64_t>(c[l * bn + mm]) * ws.tmp[base + i * bn + l];
+    Acc acc = 0;
+    for (int l = 0; l < bn; l++) acc += static_cast<Acc>(c[l * bn + mm]) * ws.tmp[base + i * bn + l];
     ws.coef[e] = static_cast<int32_t>((acc + 4096) >> 13);
   }
   wave_sync();
@@ -648,11 +660,53 @@ __device__ __forceinline__ int tile_of(const uint8_t* desc, const FrameHdr* hdr,
 // ============================================================================
 // Inter reconstruction: one wavefront per macroblock
 // ============================================================================
+// Window rows use fixed strides so that lane-to-sample mappings divide by
+// constants.
+constexpr int kWinStride = 20;   // luma window row (up to 19 samples)
+constexpr int kCWinStride = 9;   // chroma window row (up to 9 samples)
+struct InterScratch {
+  WaveScratch ws;
+  int32_t hbuf[19 * 16];            // horizontal filter pass, 16 per row
+  uint8_t win[19 * kWinStride];     // luma reference window of one partition
+  uint8_t cwin[2][9 * kCWinStride];
+};
+
+// Largest |coefficient| of the macroblock's dequantised residual over the
+// wavefront.
+__device__ __forceinline__ int32_t wave_max_abs(const WaveScratch& ws, int lane) {
+  int32_t m = 0;
+  for (int e = lane; e < 384; e += 64) {
+    const int32_t v = ws.coef[e];
+    m = max(m, v < 0 ? -v : v);
+  }
+  for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
+  return m;
+}
+
+// Partition rectangle pi of a partition mode (partRects, model.go:91-96) as
+// x | y << 8 | w << 16 | h << 24.
+__device__ __forceinline__ uint32_t part_rect(int part, int pi) {
+  switch (part) {
+    case kPart16x8: return pi ? 0x08100800u : 0x08100000u;
+    case kPart8x16: return pi ? 0x10080008u : 0x10080000u;
+    case kPart8x8: return 0x08080000u | ((pi & 1) ? 8u : 0u) | ((pi & 2) ? 0x800u : 0u);
+    default: return 0x10100000u;
+  }
+}
+
+// One skip or inter macroblock (reconMCMB / reconInterMB / reconInterChroma,
+// decode_inter.go:64-152): per partition, the reference window (the
+// partition plus the 4-tap margin, coordinates clamped to the padded plane,
+// motioncomp.go:37-45) is staged in LDS and filtered separably (horizontal
+// sums over every window row, then the vertical sum and (v + 2048) >> 12; a
+// {0,64,0,0} tap row makes this equal the 1-D and copy cases of
+// motioncomp.go:49-97).  The residual (dequantised, inverse transformed) is
+// added and only the blocks of the write mask are stored.
 __device__ __forceinline__ void recon_inter_mb(const uint8_t* __restrict__ desc, const MBRec* __restrict__ recs,
                                                const uint32_t* __restrict__ coefs,
                                                const TileStatus* __restrict__ status, const Planes rec,
                                                const Planes ref, const Planes gold, const int first_mb) {
-  __shared__ WaveScratch scratch[4];
+  __shared__ InterScratch scratch[4];
   const FrameHdr* hdr = reinterpret_cast<const FrameHdr*>(desc);
   const TileDesc* tiles = reinterpret_cast<const TileDesc*>(desc + desc_tiles_off());
   const int lane = lane_id(), wave = threadIdx.x >> 6;
@@ -666,59 +720,102 @@ __device__ __forceinline__ void recon_inter_mb(const uint8_t* __restrict__ desc,
   if (local >= status[t].mbs_done) return;
   const MBRec r = recs[mb];
   if (r.kind == 2) return;
-  WaveScratch& ws = scratch[wave];
+  InterScratch& S = scratch[wave];
+  WaveScratch& ws = S.ws;
   const Planes& src = (r.kind == 1 && r.refsel == kRefGolden) ? gold : ref;
   const int n = 4 << (r.tx > 2 ? 2 : r.tx);
   const bool has_res = r.kind == 1 && r.coef_count > 0;
+  const bool luma_res = has_res && (r.cbp & kCBPLuma);
   if (has_res) {
     load_coefs(ws, coefs, r, n, hdr->qdc, hdr->qac, lane);
-    idct_all(ws, n, (r.cbp & kCBPLuma) != 0, (r.cbp & (kCBPCb | kCBPCr)) != 0, lane);
+    idct_all(ws, n, (r.cbp & kCBPLuma) != 0, (r.cbp & (kCBPCb | kCBPCr)) != 0, wave_max_abs(ws, lane) <= kIdct32Max,
+             lane);
   }
   const int part = r.kind == 1 ? r.part : 0;
-  // Luma: lane -> row lane/4, four pixels from column (lane%4)*4.
-  {
-    const int y = lane >> 2, x0 = (lane & 3) * 4;
-    const int pi = part_of(part, x0, y);
-    const int mvx = r.mv[pi][0], mvy = r.mv[pi][1];
-    const int fx = mvx & 3, fy = mvy & 3;
-    const int sx = mbx * 16 + x0 + (mvx >> 2), sy = mby * 16 + y + (mvy >> 2);
-    const int per = 16 / n;
-    const int blk = (y / n) * per + x0 / n;
-    if ((r.wmask >> blk) & 1u) {
-      uint32_t word = 0;
-#pragma unroll
-      for (int k = 0; k < 4; k++) {
-        int32_t p = luma_mc(src.y, src.pw, src.ph, sx + k, sy, fx, fy);
-        if (has_res && (r.cbp & kCBPLuma)) {
-          int xx = x0 + k;
-          p = clamp255(p + ws.coef[blk * n * n + (y % n) * n + (xx % n)]);
+  const int npart = static_cast<int>((kPartCountNibbles >> (4 * part)) & 15u);
+  for (int pi = 0; pi < npart; pi++) {
+    const uint32_t rc = part_rect(part, pi);
+    const int px = rc & 0xFF, py = (rc >> 8) & 0xFF, pwd = (rc >> 16) & 0xFF, pht = rc >> 24;
+    // (read from the record in memory: indexing the local copy by a
+    // variable would put it in scratch)
+    const uint32_t mvw = reinterpret_cast<const uint32_t*>(recs[mb].mv)[pi];
+    const int mvx = static_cast<int16_t>(mvw & 0xFFFFu), mvy = static_cast<int16_t>(mvw >> 16);
+    // ---- luma
+    {
+      const int fx = mvx & 3, fy = mvy & 3;
+      const int ox = mbx * 16 + px + (mvx >> 2) - 1, oy = mby * 16 + py + (mvy >> 2) - 1;
+      const int ww = pwd + 3, wh = pht + 3;
+      for (int i = lane; i < wh * kWinStride; i += 64) {
+        const int rr = i / kWinStride, cc = i - rr * kWinStride;
+        if (cc < ww)
+          S.win[i] = src.y[static_cast<size_t>(clampc(oy + rr, src.ph)) * src.pw + clampc(ox + cc, src.pw)];
+      }
+      wave_sync();
+      const int32_t h0 = c_taps[fx][0], h1 = c_taps[fx][1], h2 = c_taps[fx][2], h3 = c_taps[fx][3];
+      for (int i = lane; i < wh * 16; i += 64) {
+        const int rr = i >> 4, c = i & 15;
+        if (c < pwd) {
+          const uint8_t* w = S.win + rr * kWinStride + c;
+          S.hbuf[i] = h0 * w[0] + h1 * w[1] + h2 * w[2] + h3 * w[3];
         }
-        word |= static_cast<uint32_t>(p) << (8 * k);
       }
-      *reinterpret_cast<uint32_t*>(rec.y + static_cast<size_t>(mby * 16 + y) * rec.pw + mbx * 16 + x0) = word;
-    }
-  }
-  // Chroma: lanes 0-31 Cb, 32-63 Cr; row (lane%32)/4, two pixels from (lane%4)*2.
-  {
-    const int pl = lane >> 5, l = lane & 31;
-    const int y = l >> 2, x0 = (l & 3) * 2;
-    const int pi = part_of(part, x0 * 2, y * 2);
-    const int mvx = r.mv[pi][0], mvy = r.mv[pi][1];
-    const int fx = mvx & 7, fy = mvy & 7;
-    const int sx = mbx * 8 + x0 + (mvx >> 3), sy = mby * 8 + y + (mvy >> 3);
-    const uint8_t* sp = pl ? src.cr : src.cb;
-    uint8_t* dp = pl ? rec.cr : rec.cb;
-    const bool coded = has_res && (r.cbp & (pl ? kCBPCr : kCBPCb));
-    if ((r.wmask >> (16 + pl)) & 1u) {
-      uint32_t half = 0;
+      wave_sync();
+      const int32_t v0 = c_taps[fy][0], v1 = c_taps[fy][1], v2 = c_taps[fy][2], v3 = c_taps[fy][3];
+      const int gs = pwd == 16 ? 2 : 1;  // log2 of the 4-sample groups per row
+      if (lane < (pht << gs)) {
+        const int rr = lane >> gs, c0 = (lane & ((1 << gs) - 1)) * 4;
+        const int y = py + rr, x = px + c0;
+        const int blk = (y / n) * (16 / n) + x / n;
+        if ((r.wmask >> blk) & 1u) {
+          uint32_t word = 0;
 #pragma unroll
-      for (int k = 0; k < 2; k++) {
-        int32_t p = chroma_mc(sp, src.cw, src.ch, sx + k, sy, fx, fy);
-        if (coded) p = clamp255(p + ws.coef[256 + pl * 64 + y * 8 + x0 + k]);
-        else p = clamp255(p);
-        half |= static_cast<uint32_t>(p) << (8 * k);
+          for (int k = 0; k < 4; k++) {
+            const int32_t* h = S.hbuf + rr * 16 + c0 + k;
+            int32_t p = clamp255((v0 * h[0] + v1 * h[16] + v2 * h[32] + v3 * h[48] + 2048) >> 12);
+            if (luma_res) p = clamp255(p + ws.coef[blk * n * n + (y % n) * n + ((x + k) % n)]);
+            word |= static_cast<uint32_t>(p) << (8 * k);
+          }
+          *reinterpret_cast<uint32_t*>(rec.y + static_cast<size_t>(mby * 16 + y) * rec.pw + mbx * 16 + x) = word;
+        }
       }
-      *reinterpret_cast<uint16_t*>(dp + static_cast<size_t>(mby * 8 + y) * rec.cw + mbx * 8 + x0) = static_cast<uint16_t>(half);
+    }
+    // ---- chroma (eighth-pel bilinear, the same vector in chroma units)
+    {
+      const int fx = mvx & 7, fy = mvy & 7;
+      const int cpx = px >> 1, cpy = py >> 1, cpw = pwd >> 1, cph = pht >> 1;
+      const int ox = mbx * 8 + cpx + (mvx >> 3), oy = mby * 8 + cpy + (mvy >> 3);
+      const int ww = cpw + 1, wsz = kCWinStride * (cph + 1);
+      for (int i = lane; i < 2 * wsz; i += 64) {
+        const int pl = i >= wsz ? 1 : 0, j = i - pl * wsz;
+        const int rr = j / kCWinStride, cc = j - rr * kCWinStride;
+        if (cc < ww) {
+          const uint8_t* sp = pl ? src.cr : src.cb;
+          S.cwin[pl][j] = sp[static_cast<size_t>(clampc(oy + rr, src.ch)) * src.cw + clampc(ox + cc, src.cw)];
+        }
+      }
+      wave_sync();
+      const int32_t wa = (8 - fx) * (8 - fy), wb = fx * (8 - fy), wc = (8 - fx) * fy, wd = fx * fy;
+      const int ps = cpw == 8 ? 2 : 1;  // log2 of the 2-sample pairs per row
+      const int per_plane = cph << ps;
+      if (lane < 2 * per_plane) {
+        const int pl = lane >= per_plane ? 1 : 0, j = lane - pl * per_plane;
+        const int rr = j >> ps, c0 = (j & ((1 << ps) - 1)) * 2;
+        if ((r.wmask >> (16 + pl)) & 1u) {
+          const bool coded = has_res && (r.cbp & (pl ? kCBPCr : kCBPCb));
+          const uint8_t* w = S.cwin[pl] + rr * kCWinStride + c0;
+          uint32_t half = 0;
+#pragma unroll
+          for (int k = 0; k < 2; k++) {
+            int32_t p = (wa * w[k] + wb * w[k + 1] + wc * w[kCWinStride + k] + wd * w[kCWinStride + k + 1] + 32) >> 6;
+            if (coded) p = clamp255(p + ws.coef[256 + pl * 64 + (cpy + rr) * 8 + cpx + c0 + k]);
+            half |= static_cast<uint32_t>(p) << (8 * k);
+          }
+          uint8_t* dp = pl ? rec.cr : rec.cb;
+          *reinterpret_cast<uint16_t*>(dp + static_cast<size_t>(mby * 8 + cpy + rr) * rec.cw + mbx * 8 + cpx + c0) =
+              static_cast<uint16_t>(half);
+        }
+      }
+      wave_sync();
     }
   }
 }
@@ -835,7 +932,7 @@ __device__ void intra_mb(IntraScratch& S, const uint32_t* __restrict__ coefs, co
   WaveScratch& ws = S.ws;
   if (r.coef_count > 0) {
     load_coefs(ws, coefs, r, n, hdr->qdc, hdr->qac, lane);
-    idct_all(ws, n, true, true, lane);
+    idct_all(ws, n, true, true, wave_max_abs(ws, lane) <= kIdct32Max, lane);
   } else {
     for (int i = lane; i < 384; i += 64) ws.coef[i] = 0;
   }
