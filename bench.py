@@ -172,6 +172,60 @@ def run_batched(packets, dev, want, sessions, window, steps, warmup):
             "symbols": s1["symbols"] - s0["symbols"]}
 
 
+SERVE_STREAMS = ("rects_360p", "rects_720p", "parallax_1080p", "rects_2160p")
+
+
+def run_serving(requests_n, budget_bytes, window, check_every, seed=5):
+    """The serving mix: Poisson arrivals (200 requests/s) over the 360p, 720p,
+    1080p and 2160p streams; 20% seek to a random key frame part-way, 10% loop
+    restarts; lengths 30-600 frames; sessions admitted under an HBM budget.
+    Every `check_every`-th completed frame is compared with the oracle's
+    straight-through decode of that frame."""
+    from govid_amd.codec import Batch, DeviceBuffer
+    from govid_amd.container import Reader
+    from govid_amd.serve import Server, Stream, poisson_requests
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    streams, bufs = [], []
+    for nm in SERVE_STREAMS:
+        path = os.path.join(ROOT, "bench_data", nm + ".ebiv")
+        if not os.path.exists(path):
+            continue
+        rd = Reader.open(path)
+        pk = rd.packets()
+        dev = [DeviceBuffer(p) for p in pk]
+        bufs += dev
+        streams.append(Stream(nm, rd.header.width, rd.header.height, pk, [d.ptr for d in dev]))
+    want = {}
+    for st in streams:
+        ref = O.OracleCodec(threads=min(os.cpu_count() or 1, 16))
+        want[st.name] = [_crc(ref.decode(p)) for p in st.packets]
+    reqs = poisson_requests(streams, requests_n, rate_per_s=200.0, seed=seed, min_len=30, max_len=600)
+    checked = [0, 0]
+    count = [0]
+
+    def on_frame(fd):
+        count[0] += 1
+        if fd.status == 0 and count[0] % check_every == 0:
+            checked[0] += 1
+            st = next(r.stream for r in reqs if r.rid == fd.rid)
+            if _crc(fd.frame.planes()) != want[st.name][fd.index]:
+                checked[1] += 1
+
+    b = Batch()
+    srv = Server(b, budget_bytes, window=window, ring=window, on_frame=on_frame)
+    stats = srv.run(reqs, max_seconds=240)
+    b.close()
+    for d in bufs:
+        d.close()
+    stats.update({"streams": [f"{s.name} {s.width}x{s.height} x{len(s.packets)}" for s in streams],
+                  "budget_bytes": budget_bytes, "window": window,
+                  "frames_checked": checked[0], "frames_differing": checked[1]})
+    return stats
+
+
 def kavg_us(kt, k):
     return kt[f"{k}_ms"] / max(kt[f"{k}_n"], 1) * 1e3
 
@@ -187,6 +241,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-only", action="store_true", help="skip the plain pass (for profiling the batched one)")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-budget-gb", type=float, default=24.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this decoder runs on one GPU; runs on several GPUs are not part of this "
@@ -279,6 +336,10 @@ def main():
         }
     for d in dev:
         d.close()
+    if not args.no_serve:
+        sv = run_serving(args.serve_requests, int(args.serve_budget_gb * (1 << 30)), 4, check_every=17)
+        log(f"bench: serving {sv['frames_per_s']:.1f} frames/s, p99 {sv['p99_frame_ms']:.2f} ms")
+        out["serving"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in sv.items()}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(packets)
     print(json.dumps(out))

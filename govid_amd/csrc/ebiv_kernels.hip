@@ -1032,11 +1032,16 @@ __device__ void intra_mb(IntraScratch& S, const uint32_t* __restrict__ coefs, co
 
 // One workgroup per tile.  Macroblock (x, y) depends on (x-1, y), (x-1, y-1),
 // (x, y-1) and (x+1, y-1), so step s = x + 2y (tile-relative) may run all its
-// macroblocks at once.
+// macroblocks at once.  Only intra macroblocks are reconstructed here (the
+// others were written by the inter kernel before), so a first pass over the
+// tile's records marks the steps that hold one and the others are skipped.
+constexpr int kIntraWaves = 16;
+constexpr int kMaxStepWords = 16;  // steps tracked individually (512)
 __device__ __forceinline__ void recon_intra_tile(const uint8_t* __restrict__ desc, const MBRec* __restrict__ recs,
                                                  const uint32_t* __restrict__ coefs,
                                                  const TileStatus* __restrict__ status, const Planes rec, const int t) {
-  __shared__ IntraScratch scratch[4];
+  __shared__ IntraScratch scratch[kIntraWaves];
+  __shared__ uint32_t step_mask[kMaxStepWords];
   const FrameHdr* hdr = reinterpret_cast<const FrameHdr*>(desc);
   const TileDesc* tiles = reinterpret_cast<const TileDesc*>(desc + desc_tiles_off());
   const int lane = lane_id(), wave = threadIdx.x >> 6;
@@ -1046,11 +1051,25 @@ __device__ __forceinline__ void recon_intra_tile(const uint8_t* __restrict__ des
   const int tw = td.x1 - td.x0, th = td.y1 - td.y0;
   const int steps = (tw - 1) + 2 * (th - 1) + 1;
   const int mbc = hdr->mbc;
+  const bool tracked = steps <= 32 * kMaxStepWords;
+  if (threadIdx.x < kMaxStepWords) step_mask[threadIdx.x] = 0;
+  __syncthreads();
+  if (tracked) {
+    for (uint32_t i = threadIdx.x; i < st.mbs_done; i += blockDim.x) {
+      const int ry = static_cast<int>(i) / tw, rx = static_cast<int>(i) - ry * tw;
+      if (recs[(td.y0 + ry) * mbc + td.x0 + rx].kind == 2) {
+        const int s = rx + 2 * ry;
+        atomicOr(&step_mask[s >> 5], 1u << (s & 31));
+      }
+    }
+  }
+  __syncthreads();
   for (int s = 0; s < steps; s++) {
+    if (tracked && !((step_mask[s >> 5] >> (s & 31)) & 1u)) continue;  // uniform across the workgroup
     // Macroblocks on this step: rows ry with rx = s - 2*ry in [0, tw).
     int ry_lo = s - (tw - 1) > 0 ? (s - (tw - 1) + 1) / 2 : 0;
     int ry_hi = s / 2 < th - 1 ? s / 2 : th - 1;
-    for (int ry = ry_lo + wave; ry <= ry_hi; ry += 4) {
+    for (int ry = ry_lo + wave; ry <= ry_hi; ry += kIntraWaves) {
       int rx = s - 2 * ry;
       uint32_t local = static_cast<uint32_t>(ry * tw + rx);
       if (local >= st.mbs_done) continue;
@@ -1069,7 +1088,7 @@ __device__ __forceinline__ void recon_intra_tile(const uint8_t* __restrict__ des
   }
 }
 
-__global__ __launch_bounds__(256) void ebiv_recon_intra(const FrameJob* __restrict__ jobs,
+__global__ __launch_bounds__(64 * kIntraWaves) void ebiv_recon_intra(const FrameJob* __restrict__ jobs,
                                                         const WorkItem* __restrict__ items) {
   const WorkItem it = item_of(items);
   ConstJob& job = job_of(jobs, it.job);
@@ -1138,7 +1157,7 @@ void launch_recon_inter(hipStream_t s, int nitems, const FrameJob* jobs, const W
 }
 
 void launch_recon_intra(hipStream_t s, int nitems, const FrameJob* jobs, const WorkItem* items) {
-  hipLaunchKernelGGL(ebiv_recon_intra, dim3(nitems), dim3(256), 0, s, jobs, items);
+  hipLaunchKernelGGL(ebiv_recon_intra, dim3(nitems), dim3(64 * kIntraWaves), 0, s, jobs, items);
 }
 
 void launch_rgba(hipStream_t s, const uint8_t* y, int ys, const uint8_t* cb, const uint8_t* cr, int cs, int w, int h,
