@@ -235,13 +235,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--sessions", type=int, default=16)
+    ap.add_argument("--sessions", type=int, default=64)
     ap.add_argument("--window", type=int, default=8)
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-only", action="store_true", help="skip the plain pass (for profiling the batched one)")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-only", action="store_true", help="run only the serving mix")
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-budget-gb", type=float, default=24.0)
     args = ap.parse_args()
@@ -252,6 +253,10 @@ def main():
 
     from govid_amd.codec import DeviceBuffer
 
+    if args.serve_only:
+        sv = run_serving(args.serve_requests, int(args.serve_budget_gb * (1 << 30)), 4, check_every=17)
+        print(json.dumps({"mode": "serving", **{k: (round(v, 3) if isinstance(v, float) else v) for k, v in sv.items()}}))
+        return
     rd = load_stream()
     packets = rd.packets()
     hdr = rd.header
