@@ -20,12 +20,39 @@ struct TileDesc {
 };
 static_assert(sizeof(TileDesc) == 24, "TileDesc layout");
 
-constexpr int kCumStride = 20;  // u16 entries per context in FrameHdr::cum
+// The frame's 72 static rANS tables (rans.go:120-148) as the entropy kernel
+// holds them in registers: 15 vectors of 64 lanes, one packed entry per lane,
+//   entry = start << 13 | freq   (start <= 4095, freq <= 4096),
+// kNoSym past a context's alphabet and on every lane of an unused context.
+//   vector 0: sign (lanes 0-15), MB mode (16-31), transform size (32-47), CBP (48-63)
+//   vector 1: luma mode, chroma mode, reference, partition (16 lanes each)
+//   vector 2: escape class, MV class x, MV class y (16 lanes each)
+//   vector 3 + 3*set + band/2, set = luma transform index 0-2 or 3 for chroma:
+//     the token context (set, band, prev) at lanes 8*((band & 1)*3 + prev) + symbol
+// The bypass context is fixed at {2048, 2048} (rans.go:312-322) and has no entry.
+constexpr int kNumTableVecs = 15;
+constexpr uint32_t kNoSym = 0xFFFFFFFFu;
+// Vector and first lane of a context's entries; false for the bypass context.
+__host__ __device__ inline bool table_place(int ctx, int* vec, int* lane0) {
+  // contexts 0-11 in enum order: tx, luma mode, chroma mode, MB mode, ref,
+  // part, CBP, sign, escape class, MV x, MV y, bypass
+  constexpr int8_t kVec[12] = {0, 1, 1, 0, 1, 1, 0, 0, 2, 2, 2, -1};
+  constexpr int8_t kLane[12] = {32, 0, 16, 16, 32, 48, 48, 0, 0, 16, 32, -1};
+  if (ctx < 12) {
+    *vec = kVec[ctx];
+    *lane0 = kLane[ctx];
+    return ctx != 11;
+  }
+  int t = ctx - 12, set, band, prev;
+  if (t < 45) { set = t / 15; band = (t % 15) / 3; prev = t % 3; }
+  else { set = 3; band = (t - 45) / 3; prev = (t - 45) % 3; }
+  *vec = 3 + 3 * set + band / 2;
+  *lane0 = 8 * ((band & 1) * 3 + prev);
+  return true;
+}
 
 // Per-frame parameters: the coded header (frame.go:181-220), quantiser
-// (quant.go:38-51) and the 72 static rANS tables of the frame as cumulative
-// frequencies (rans.go:120-148).  cum[c][s] is the start of symbol s,
-// cum[c][n] = 4096 for an alphabet of n, and later entries are 0xFFFF.
+// (quant.go:38-51) and the frame's symbol tables in kernel layout (above).
 struct FrameHdr {
   int32_t mbc, mbr;       // macroblock grid of the padded frame
   int32_t ntiles;
@@ -34,7 +61,7 @@ struct FrameHdr {
   int32_t grid_cols, grid_rows;
   uint32_t used[3];       // bit c: context c has a usable table
   int32_t max_tile_w;     // widest tile in macroblocks (MV row buffers)
-  uint16_t cum[72][kCumStride];
+  uint32_t vt[kNumTableVecs][64];
 };
 
 // Per-frame device descriptor = FrameHdr, TileDesc[ntiles], uint16 col_tile[mbc],

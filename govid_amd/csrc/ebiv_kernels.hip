@@ -62,7 +62,11 @@ struct ByteWin {
   __device__ __forceinline__ void load(int lane) {
     const int32_t a = wbase + 4 * lane;
     using GWord = __attribute__((address_space(1))) const uint32_t;  // packets live in device global memory
-    cur = (a >= lo && a < hi) ? *(GWord*)(tile + a) : 0u;
+    // The address is clamped into the readable range so the load needs no
+    // branch; lanes outside it then read as zero.
+    const int32_t ac = a < lo ? lo : (a >= hi ? hi - 4 : a);
+    const uint32_t v = *(GWord*)(tile + ac);
+    cur = (a >= lo && a < hi) ? v : 0u;
   }
   __device__ void init(const uint8_t* t, uint32_t len, int lane) {
     tile = t;
@@ -137,9 +141,19 @@ __device__ __forceinline__ uint32_t mv_set(uint32_t v, int comp, int16_t c) {
 }
 
 // ----------------------------------------------------------------------------
-// Entropy decoding of one tile.  All state is wave-uniform; the lanes only
-// take part in the cumulative-table search (one lane per symbol), the byte
-// window and the coefficient staging buffer.
+// Entropy decoding of one tile.  All state is wave-uniform; the lanes take
+// part only in the symbol search (one lane per table entry), the byte window
+// and the coefficient staging buffer.
+//
+// Symbol search.  The frame's tables sit in 15 VGPRs (FrameHdr::vt): lane l
+// of a vector holds one packed entry start << 13 | freq.  For a slot the
+// entries of the context's lane group with start <= slot are exactly those
+// with entry <= (slot << 13 | 0x1FFF); their count minus one is the symbol
+// (the last symbol whose range starts at or below the slot, rans.go:120-148
+// slot2sym), and a count of zero means an unused context (every entry is
+// kNoSym).  One compare over the vector, one ballot, one popcount and one
+// readlane (start and frequency together) decode a symbol; no memory access
+// is on the dependency chain from one symbol to the next.
 // ----------------------------------------------------------------------------
 struct Rans {
   uint32_t x0, x1, x2, x3;  // x0 is state[counter & 3] (rans.go:279-282)
@@ -149,46 +163,48 @@ struct Rans {
   ByteWin win;
 
   // Renormalisation and state rotation after a symbol (rans.go:286-303).
+  // Most symbols need no byte, so the byte loop is the rare branch.
   __device__ __forceinline__ bool advance(uint32_t x, int lane) {
-    while (x < kRansLower) {
-      if (pos == 0) {
-        err = kErrCorrupt | (kTileUnderrun << 8);
-        return false;
-      }
-      x = (x << 8) | win.byte(static_cast<int32_t>(--pos), lane);
+    if (__builtin_expect(x < kRansLower, 0)) {
+      do {
+        if (pos == 0) {
+          err = kErrCorrupt | (kTileUnderrun << 8);
+          return false;
+        }
+        x = (x << 8) | win.byte(static_cast<int32_t>(--pos), lane);
+      } while (x < kRansLower);
     }
     x0 = x1; x1 = x2; x2 = x3; x3 = x;
     nsym++;
     return true;
   }
-  // ransDecoder.decode (rans.go:270-304).  c is the context's cumulative row:
-  // lane s holds the start of symbol s, lane n the total, other lanes 0xFFFF;
-  // an unused context is 0xFFFF on every lane.  Returns -1 on error.
-  __device__ __forceinline__ int decode(uint32_t c, int lane) {
+  // ransDecoder.decode (rans.go:270-304) for the context whose entries are
+  // the lanes [goff, goff + width) of V (gmask = 2^width - 1).  Returns the
+  // symbol, or -1 on error.
+  __device__ __forceinline__ int decode(uint32_t V, uint32_t goff, uint32_t gmask, int lane) {
     const uint32_t x = x0;
     const uint32_t slot = x & (kRansM - 1);
-    const int cnt = static_cast<int>(__popcll(__ballot(c <= slot)));
-    if (cnt == 0) {
+    const uint64_t b = __ballot(V <= ((slot << 13) | 0x1FFFu));
+    const uint32_t cnt = static_cast<uint32_t>(__builtin_popcount(static_cast<uint32_t>(b >> goff) & gmask));
+    if (__builtin_expect(cnt == 0, 0)) {
       err = kErrCorrupt | (kTileUnused << 8);
       return -1;
     }
-    const uint32_t start = rd_lane(c, cnt - 1);
-    const uint32_t freq = rd_lane(c, cnt) - start;
-    if (!advance(freq * (x >> kRansScaleBits) + slot - start, lane)) return -1;
-    return cnt - 1;
+    const uint32_t p = rd_lane(V, static_cast<int>(goff + cnt - 1));
+    if (!advance((p & 0x1FFFu) * (x >> kRansScaleBits) + slot - (p >> 13), lane)) return -1;
+    return static_cast<int>(cnt - 1);
   }
   // The bypass context, fixed at {2048, 2048} (rans.go:312-322).
   __device__ __forceinline__ int bit(int lane) {
     const uint32_t x = x0;
     const uint32_t slot = x & (kRansM - 1);
-    const uint32_t b = slot >> 11;
-    if (!advance(2048u * (x >> kRansScaleBits) + slot - (b << 11), lane)) return -1;
-    return static_cast<int>(b);
+    if (!advance(((x >> kRansScaleBits) << 11) + (slot & 2047u), lane)) return -1;
+    return static_cast<int>(slot >> 11);
   }
   // getClassValue (mv.go:95-108): a class symbol, then class-1 raw bits under
   // an implicit leading one.  Returns -1 on error.
-  __device__ __forceinline__ int class_value(uint32_t c, int lane) {
-    const int v = decode(c, lane);
+  __device__ __forceinline__ int class_value(uint32_t V, uint32_t goff, int lane) {
+    const int v = decode(V, goff, 0xFFFFu, lane);
     if (v <= 1) return v;
     int val = 1;
     for (int i = 1; i < v; i++) {
@@ -200,14 +216,23 @@ struct Rans {
   }
 };
 
+// The table vectors of one tile's wavefront.
+struct Tables {
+  uint32_t misc0, misc1, misc2;  // non-token contexts
+  uint32_t t[4][3];              // token contexts: set (luma tx 0-2, chroma), band pair
+};
+// Lane groups of the non-token contexts (vector, first lane).
+constexpr uint32_t kGSign = 0, kGMBMode = 16, kGTx = 32, kGCBP = 48;       // misc0
+constexpr uint32_t kGLumaMode = 0, kGChromaMode = 16, kGRef = 32, kGPart = 48;  // misc1
+constexpr uint32_t kGEsc = 0, kGMVX = 16, kGMVY = 32;                       // misc2
+
 // Coefficient list output: lane k stages entry base+k until 64 are queued.
 struct CoefOut {
   uint32_t* dst;
-  uint32_t base, cn, count;
+  uint32_t base, cn;  // entries stored, entries staged
   uint32_t buf;
   __device__ __forceinline__ void emit(uint32_t e, int lane) {
     if (static_cast<uint32_t>(lane) == cn) buf = e;
-    count++;
     if (++cn == 64) {
       dst[base + lane] = buf;
       base += 64;
@@ -216,45 +241,30 @@ struct CoefOut {
   }
 };
 
-// LDS reads by 32-bit byte address: address arithmetic stays 32-bit and
-// uniform parts of an address stay scalar.
-using LdsU16 = __attribute__((address_space(3))) const uint16_t;
-__device__ __forceinline__ uint32_t lds_u16(uint32_t addr) { return *(LdsU16*)static_cast<uintptr_t>(addr); }
-__device__ __forceinline__ uint32_t lds_addr(const void* p) {
-  return static_cast<uint32_t>(reinterpret_cast<uintptr_t>((__attribute__((address_space(3))) const void*)p));
-}
-
-// Context ctx's cumulative row; lanes is this lane's entry of context 0.
-__device__ __forceinline__ uint32_t cum_row(uint32_t lanes, int ctx) {
-  return lds_u16(lanes + static_cast<uint32_t>(ctx * kCumStride * 2));
-}
-
 // decodeBlock (residual.go:79-126) for one block: tokens in zigzag order under
 // (plane, size, band, previous class) contexts, escapes, explicit signs, and
 // the first sign hidden in the level-sum parity when the span allows.  Levels
 // go to the coefficient list; the first coefficient goes last, once its sign
-// is known.  Returns false on a decode error.
-__device__ __forceinline__ bool parse_block(Rans& R, CoefOut& out, uint32_t lanes, int ctxbase, uint32_t zzv,
-                                            int total, int blk, uint32_t csign, int lane) {
-  // This lane's entry of the block's first token context; the context of a
-  // token is a uniform row offset from it.
-  const uint32_t rows = lanes + static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(ctxbase) * kCumStride * 2);
-  // Row offsets in bytes: band * 3 rows, previous class * 1 row.
-  constexpr uint32_t kRow = kCumStride * 2;
-  uint32_t bandoff = 0, prevoff = 0;
+// is known.  va/vb/vc are the token vectors of the block's (plane, size) for
+// bands 0-1, 2-3 and 4; the context (band, prev) is the lane group
+// 8 * ((band & 1) * 3 + prev) of its band's vector.  Returns false on a decode
+// error.
+__device__ __forceinline__ bool parse_block(Rans& R, CoefOut& out, const Tables& T, uint32_t va, uint32_t vb,
+                                            uint32_t vc, uint32_t zzv, int total, int blk, int lane) {
+  uint32_t vcur = va, boff = 0, prevoff = 0;
   int si = 0, band = 0, bnext = 1, first = -1, last = -1, first_pos = 0;
   int32_t sum_abs = 0, first_mag = 0;
   for (;;) {
-    const int t = R.decode(lds_u16(rows + (bandoff + prevoff)), lane);
+    const int t = R.decode(vcur, boff + prevoff, 0xFFu, lane);
     if (t < 0) return false;
     if (t == kTEOB) break;
     if (t == kTZero) {
       prevoff = 0;
     } else {
       int32_t mag = t;
-      prevoff = t == kTOne ? kRow : 2 * kRow;
+      prevoff = t == kTOne ? 8u : 16u;  // classForMag: 1 -> 1, else 2 (model.go:181-191)
       if (t == kTEscape) {
-        const int e = R.class_value(cum_row(lanes, kCtxEscClass), lane);
+        const int e = R.class_value(T.misc2, kGEsc, lane);
         if (e < 0) return false;
         mag = 5 + e;
       }
@@ -264,7 +274,7 @@ __device__ __forceinline__ bool parse_block(Rans& R, CoefOut& out, uint32_t lane
         first_mag = mag;
         first_pos = zz_pos(zzv, si);
       } else {
-        const int s = R.decode(csign, lane);
+        const int s = R.decode(T.misc0, kGSign, 0xFFFFu, lane);
         if (s < 0) return false;
         out.emit(coef_entry(zz_pos(zzv, si), blk, s ? -mag : mag), lane);
       }
@@ -273,18 +283,327 @@ __device__ __forceinline__ bool parse_block(Rans& R, CoefOut& out, uint32_t lane
     if (++si == total) break;
     if (si == bnext) {  // bandOf thresholds 1, 3, 9, 21 (model.go:165-180)
       band++;
-      bandoff += kNumPrevClass * kRow;
+      boff ^= 24u;  // (band & 1) * 3 rows of 8 lanes
+      if (band == 2) vcur = vb;
+      if (band == 4) vcur = vc;
       bnext = band < 4 ? static_cast<int>((0x15090301u >> (8 * band)) & 0xFFu) : (1 << 20);
     }
   }
   if (first >= 0) {
     bool neg = (sum_abs & 1) != 0;
     if (last - first < kSDHMinSpan) {
-      const int s = R.decode(csign, lane);
+      const int s = R.decode(T.misc0, kGSign, 0xFFFFu, lane);
       if (s < 0) return false;
       neg = s != 0;
     }
     out.emit(coef_entry(first_pos, blk, neg ? -first_mag : first_mag), lane);
+  }
+  return true;
+}
+
+// Hand-scheduled token loop of decodeBlock (residual.go:86-115) for the common
+// case: every symbol of a (zero, one .. four) token and its sign.  It runs
+// until the block ends (EOB or the last scan position), an escape token has
+// been decoded (the caller decodes its magnitude and resumes), the byte window
+// needs to move, or a decode error.  All state is uniform and lives in SGPRs;
+// the only vector work is the table compare, lane reads of the table, scan and
+// byte window, and staging one coefficient entry per lane.
+//
+// Preconditions (checked by the caller): every rANS state is normalised
+// (>= 2^23), so a symbol needs at most two renormalisation bytes; the sign
+// context is in use (its two frequencies are f0 and f1); and the byte window
+// holds every byte the next iteration can consume: at most 4 bytes (a token
+// and a sign), which the loop re-checks against lim at each scan position.
+enum TokEvent : uint32_t { kTokDone = 0, kTokRefill = 1, kTokEscape = 2, kTokUnused = 3, kTokUnderrun = 4 };
+
+struct TokState {
+  uint32_t x0, x1, x2, x3, pos, nsym;
+  uint32_t si, stop, band, boff, prev;
+  uint32_t sumabs, firstsi, firstpm, last;
+  uint32_t cn, cbase;
+  uint32_t vcur, vbuf;  // per lane
+};
+
+// Renormalisation bytes of one symbol (rans.go:286-303): the byte below pos
+// from the lane window; underrun is an error.
+#define EBIV_RENORM(tag)                                  \
+  "L%=_" tag ":\n"                                    \
+      "s_cmp_eq_u32 %[pos], 0\n"                          \
+      "s_cbranch_scc1 L%=_underrun\n"                     \
+      "s_sub_u32 %[pos], %[pos], 1\n"                     \
+      "s_sub_u32 %[t1], %[pos], %[wbase]\n"               \
+      "s_lshr_b32 %[t2], %[t1], 2\n"                      \
+      "v_readlane_b32 %[t2], %[vwin], %[t2]\n"            \
+      "s_and_b32 %[t1], %[t1], 3\n"                       \
+      "s_lshl_b32 %[t1], %[t1], 3\n"                      \
+      "s_lshr_b32 %[t2], %[t2], %[t1]\n"                  \
+      "s_and_b32 %[t2], %[t2], 0xff\n"                    \
+      "s_lshl_b32 %[xn], %[xn], 8\n"                      \
+      "s_or_b32 %[xn], %[xn], %[t2]\n"                    \
+      "s_cmp_lt_u32 %[xn], 0x800000\n"                    \
+      "s_cbranch_scc1 L%=_" tag "\n"                      \
+      "s_branch L%=_" tag "done\n"
+
+__device__ __forceinline__ uint32_t token_loop(TokState& S, uint32_t resume, uint32_t lim, uint32_t wbase,
+                                               uint32_t total, uint32_t blk8, uint32_t f0, uint32_t f1,
+                                               uint32_t* dst, uint32_t vb, uint32_t vc, uint32_t zzv, uint32_t vwin,
+                                               uint32_t vlane4) {
+  uint32_t ev, slot, cnt, p, xn, t1, t2, mag, neg, vtmp, m0s;
+  uint64_t b;
+  asm volatile(
+      "s_nop 4\n"
+      "s_mov_b32 %[m0s], m0\n"
+      "s_cmp_lg_u32 %[resume], 0\n"
+      "s_cbranch_scc0 L%=_top\n"
+      "s_mov_b32 %[mag], %[resume]\n"
+      "s_mov_b32 %[prev], 16\n"
+      "s_branch L%=_nzcommon\n"
+
+      // ---- scan position si: the token under context (band, prev)
+      "L%=_top:\n"
+      "s_cmp_lt_i32 %[pos], %[lim]\n"
+      "s_cbranch_scc1 L%=_refill\n"
+      "s_and_b32 %[slot], %[x0], 0xfff\n"
+      "s_lshl_b32 %[t1], %[slot], 13\n"
+      "s_or_b32 %[t1], %[t1], 0x1fff\n"
+      "v_cmp_ge_u32_e64 %[b], %[t1], %[vcur]\n"
+      "s_add_u32 %[t2], %[boff], %[prev]\n"
+      "s_or_b32 %[t1], %[t2], 0x80000\n"
+      "s_lshr_b32 %[xn], %[x0], 12\n"
+      "s_bfe_u64 %[b], %[b], %[t1]\n"
+      "s_bcnt1_i32_b64 %[cnt], %[b]\n"
+      "s_cmp_eq_u32 %[cnt], 1\n"
+      "s_cbranch_scc0 L%=_nz\n"
+      // zero token: symbol 0 starts at 0
+      "v_readlane_b32 %[p], %[vcur], %[t2]\n"
+      "s_mov_b32 %[prev], 0\n"
+      "s_and_b32 %[p], %[p], 0x1fff\n"
+      "s_mul_i32 %[xn], %[p], %[xn]\n"
+      "s_add_u32 %[xn], %[xn], %[slot]\n"
+      "s_cmp_lt_u32 %[xn], 0x800000\n"
+      "s_cbranch_scc1 L%=_rz\n"
+      "L%=_rzdone:\n"
+      "s_mov_b32 %[x0], %[x1]\n"
+      "s_mov_b32 %[x1], %[x2]\n"
+      "s_mov_b32 %[x2], %[x3]\n"
+      "s_mov_b32 %[x3], %[xn]\n"
+      "s_add_u32 %[nsym], %[nsym], 1\n"
+      "s_add_u32 %[si], %[si], 1\n"
+      "s_cmp_lg_u32 %[si], %[stop]\n"
+      "s_cbranch_scc1 L%=_top\n"
+      "s_branch L%=_stop\n"
+
+      // ---- any other symbol of the token context
+      "L%=_nz:\n"
+      "s_cmp_eq_u32 %[cnt], 0\n"
+      "s_cbranch_scc1 L%=_unused\n"
+      "s_add_u32 %[t1], %[t2], %[cnt]\n"
+      "s_add_u32 %[t1], %[t1], -1\n"
+      "v_readlane_b32 %[p], %[vcur], %[t1]\n"
+      "s_add_u32 %[mag], %[cnt], -1\n"
+      "s_and_b32 %[t2], %[p], 0x1fff\n"
+      "s_lshr_b32 %[p], %[p], 13\n"
+      "s_mul_i32 %[xn], %[t2], %[xn]\n"
+      "s_sub_u32 %[t2], %[slot], %[p]\n"
+      "s_add_u32 %[xn], %[xn], %[t2]\n"
+      "s_cmp_lt_u32 %[xn], 0x800000\n"
+      "s_cbranch_scc1 L%=_rn\n"
+      "L%=_rndone:\n"
+      "s_mov_b32 %[x0], %[x1]\n"
+      "s_mov_b32 %[x1], %[x2]\n"
+      "s_mov_b32 %[x2], %[x3]\n"
+      "s_mov_b32 %[x3], %[xn]\n"
+      "s_add_u32 %[nsym], %[nsym], 1\n"
+      "s_cmp_ge_u32 %[cnt], 6\n"
+      "s_cbranch_scc1 L%=_escob\n"
+      // classForMag: 1 -> row 1 (8 lanes), 2..4 -> row 2
+      "s_cmp_eq_u32 %[mag], 1\n"
+      "s_cselect_b32 %[prev], 8, 16\n"
+
+      // ---- a nonzero level of magnitude mag at si
+      "L%=_nzcommon:\n"
+      "s_add_u32 %[sumabs], %[sumabs], %[mag]\n"
+      "s_cmp_eq_u32 %[firstsi], 0\n"
+      "s_cbranch_scc1 L%=_first\n"
+      // explicit sign: the two-symbol sign context from f0, f1
+      "s_sub_u32 %[neg], 0, %[mag]\n"
+      "s_and_b32 %[slot], %[x0], 0xfff\n"
+      "s_lshr_b32 %[xn], %[x0], 12\n"
+      "s_cmp_ge_u32 %[slot], %[f0]\n"
+      "s_cselect_b32 %[p], %[f0], 0\n"
+      "s_cselect_b32 %[t2], %[f1], %[f0]\n"
+      "s_cselect_b32 %[mag], %[neg], %[mag]\n"
+      "s_mul_i32 %[xn], %[t2], %[xn]\n"
+      "s_sub_u32 %[t2], %[slot], %[p]\n"
+      "s_add_u32 %[xn], %[xn], %[t2]\n"
+      "s_cmp_lt_u32 %[xn], 0x800000\n"
+      "s_cbranch_scc1 L%=_rs\n"
+      "L%=_rsdone:\n"
+      "s_mov_b32 %[x0], %[x1]\n"
+      "s_mov_b32 %[x1], %[x2]\n"
+      "s_mov_b32 %[x2], %[x3]\n"
+      "s_mov_b32 %[x3], %[xn]\n"
+      "s_add_u32 %[nsym], %[nsym], 1\n"
+      // coefficient entry: zigzag position | block << 8 | level << 13
+      "s_lshr_b32 %[t1], %[si], 2\n"
+      "v_readlane_b32 %[t2], %[zzv], %[t1]\n"
+      "s_and_b32 %[t1], %[si], 3\n"
+      "s_lshl_b32 %[t1], %[t1], 3\n"
+      "s_lshr_b32 %[t2], %[t2], %[t1]\n"
+      "s_and_b32 %[t2], %[t2], 0xff\n"
+      "s_lshl_b32 %[t1], %[mag], 13\n"
+      "s_or_b32 %[t1], %[t1], %[t2]\n"
+      "s_or_b32 %[t1], %[t1], %[blk8]\n"
+      "s_mov_b32 m0, %[cn]\n"
+      "v_writelane_b32 %[vbuf], %[t1], m0\n"
+      "s_add_u32 %[cn], %[cn], 1\n"
+      "s_cmp_eq_u32 %[cn], 64\n"
+      "s_cbranch_scc1 L%=_flush\n"
+      "L%=_flushdone:\n"
+      "s_mov_b32 %[last], %[si]\n"
+      "s_add_u32 %[si], %[si], 1\n"
+      "s_cmp_lg_u32 %[si], %[stop]\n"
+      "s_cbranch_scc1 L%=_top\n"
+      "s_branch L%=_stop\n"
+
+      // first nonzero of the block: its sign resolves at the end
+      "L%=_first:\n"
+      "s_lshr_b32 %[t1], %[si], 2\n"
+      "v_readlane_b32 %[t2], %[zzv], %[t1]\n"
+      "s_and_b32 %[t1], %[si], 3\n"
+      "s_lshl_b32 %[t1], %[t1], 3\n"
+      "s_lshr_b32 %[t2], %[t2], %[t1]\n"
+      "s_and_b32 %[t2], %[t2], 0xff\n"
+      "s_lshl_b32 %[t1], %[mag], 8\n"
+      "s_or_b32 %[firstpm], %[t1], %[t2]\n"
+      "s_add_u32 %[firstsi], %[si], 1\n"
+      "s_mov_b32 %[last], %[si]\n"
+      "s_add_u32 %[si], %[si], 1\n"
+      "s_cmp_lg_u32 %[si], %[stop]\n"
+      "s_cbranch_scc1 L%=_top\n"
+
+      // ---- si reached stop: the end of the block or a band boundary
+      // (bandOf thresholds 1, 3, 9, 21, model.go:165-180)
+      "L%=_stop:\n"
+      "s_cmp_eq_u32 %[si], %[total]\n"
+      "s_cbranch_scc1 L%=_done\n"
+      "s_add_u32 %[band], %[band], 1\n"
+      "s_xor_b32 %[boff], %[boff], 24\n"
+      "s_cmp_eq_u32 %[band], 2\n"
+      "s_cbranch_scc0 L%=_nb2\n"
+      "v_mov_b32 %[vcur], %[vb]\n"
+      "L%=_nb2:\n"
+      "s_cmp_eq_u32 %[band], 4\n"
+      "s_cbranch_scc0 L%=_nb4\n"
+      "v_mov_b32 %[vcur], %[vc]\n"
+      "s_mov_b32 %[stop], %[total]\n"
+      "s_branch L%=_top\n"
+      "L%=_nb4:\n"
+      "s_lshl_b32 %[t1], %[band], 3\n"
+      "s_lshr_b32 %[t1], 0x15090301, %[t1]\n"
+      "s_and_b32 %[t1], %[t1], 0xff\n"
+      "s_min_u32 %[stop], %[t1], %[total]\n"
+      "s_branch L%=_top\n"
+
+      // ---- escape or end of block (their symbol is consumed)
+      "L%=_escob:\n"
+      "s_cmp_eq_u32 %[cnt], 7\n"
+      "s_cbranch_scc1 L%=_done\n"
+      "s_mov_b32 %[ev], 2\n"
+      "s_branch L%=_exit\n"
+
+      // ---- 64 staged entries: one coalesced store
+      "L%=_flush:\n"
+      "s_lshl_b32 %[t1], %[cbase], 2\n"
+      "v_add_u32 %[vtmp], %[t1], %[vlane4]\n"
+      "global_store_dword %[vtmp], %[vbuf], %[dst]\n"
+      "s_add_u32 %[cbase], %[cbase], 64\n"
+      "s_mov_b32 %[cn], 0\n"
+      "s_branch L%=_flushdone\n"
+
+      // ---- renormalisation bytes (at most two; the loop checks underrun)
+      EBIV_RENORM("rz")
+      EBIV_RENORM("rn")
+      EBIV_RENORM("rs")
+
+      "L%=_refill:\n"
+      "s_mov_b32 %[ev], 1\n"
+      "s_branch L%=_exit\n"
+      "L%=_unused:\n"
+      "s_mov_b32 %[ev], 3\n"
+      "s_branch L%=_exit\n"
+      "L%=_underrun:\n"
+      "s_mov_b32 %[ev], 4\n"
+      "s_branch L%=_exit\n"
+      "L%=_done:\n"
+      "s_mov_b32 %[ev], 0\n"
+      "L%=_exit:\n"
+      "s_mov_b32 m0, %[m0s]\n"
+      "s_nop 4\n"
+      : [x0] "+s"(S.x0), [x1] "+s"(S.x1), [x2] "+s"(S.x2), [x3] "+s"(S.x3), [pos] "+s"(S.pos), [nsym] "+s"(S.nsym),
+        [si] "+s"(S.si), [stop] "+s"(S.stop), [band] "+s"(S.band), [boff] "+s"(S.boff), [prev] "+s"(S.prev),
+        [sumabs] "+s"(S.sumabs), [firstsi] "+s"(S.firstsi), [firstpm] "+s"(S.firstpm), [last] "+s"(S.last),
+        [cn] "+s"(S.cn), [cbase] "+s"(S.cbase), [vcur] "+v"(S.vcur), [vbuf] "+v"(S.vbuf), [ev] "=&s"(ev),
+        [slot] "=&s"(slot), [cnt] "=&s"(cnt), [p] "=&s"(p), [xn] "=&s"(xn), [t1] "=&s"(t1), [t2] "=&s"(t2),
+        [mag] "=&s"(mag), [neg] "=&s"(neg), [b] "=&s"(b), [vtmp] "=&v"(vtmp), [m0s] "=&s"(m0s)
+      : [resume] "s"(resume), [lim] "s"(lim), [wbase] "s"(wbase), [total] "s"(total), [blk8] "s"(blk8),
+        [f0] "s"(f0), [f1] "s"(f1), [dst] "s"(dst), [vb] "v"(vb), [vc] "v"(vc), [zzv] "v"(zzv), [vwin] "v"(vwin),
+        [vlane4] "v"(vlane4)
+      : "scc", "memory");
+  return ev;
+}
+
+// Per-tile inputs of the hand-scheduled token loop.
+struct FastTok {
+  bool sign_used;   // the sign context has a table
+  uint32_t f0, f1;  // its two frequencies
+  uint32_t vlane4;  // 4 * lane
+};
+
+// decodeBlock through token_loop: the block's state moves between the loop
+// and the generic decoder for escapes, window moves and errors.
+__device__ __forceinline__ bool parse_block_fast(Rans& R, CoefOut& out, const Tables& T, const FastTok& F,
+                                                 uint32_t va, uint32_t vb, uint32_t vc, uint32_t zzv, int total,
+                                                 int blk, int lane) {
+  TokState S;
+  S.x0 = R.x0; S.x1 = R.x1; S.x2 = R.x2; S.x3 = R.x3; S.pos = R.pos; S.nsym = R.nsym;
+  S.si = 0; S.stop = 1; S.band = 0; S.boff = 0; S.prev = 0;
+  S.sumabs = 0; S.firstsi = 0; S.firstpm = 0; S.last = 0;
+  S.cn = out.cn; S.cbase = out.base; S.vcur = va; S.vbuf = out.buf;
+  uint32_t resume = 0, ev;
+  for (;;) {
+    // The next scan position may consume 4 bytes below pos.
+    while (R.win.wbase > 0 && static_cast<int32_t>(S.pos) < R.win.wbase + 4) {
+      R.win.wbase -= 128;
+      R.win.load(lane);
+    }
+    const uint32_t lim = R.win.wbase > 0 ? static_cast<uint32_t>(R.win.wbase + 4) : 0x80000000u;
+    ev = token_loop(S, resume, lim, static_cast<uint32_t>(R.win.wbase), static_cast<uint32_t>(total),
+                    __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(blk) << 8), F.f0, F.f1, out.dst, vb, vc,
+                    zzv, R.win.cur, F.vlane4);
+    resume = 0;
+    if (ev == kTokRefill) continue;
+    R.x0 = S.x0; R.x1 = S.x1; R.x2 = S.x2; R.x3 = S.x3; R.pos = S.pos; R.nsym = S.nsym;
+    if (ev != kTokEscape) break;
+    const int e = R.class_value(T.misc2, kGEsc, lane);  // getEscapeValue (mv.go:124-128)
+    if (e < 0) break;
+    resume = 5 + static_cast<uint32_t>(e);
+    S.x0 = R.x0; S.x1 = R.x1; S.x2 = R.x2; S.x3 = R.x3; S.pos = R.pos; S.nsym = R.nsym;
+  }
+  out.cn = S.cn; out.base = S.cbase; out.buf = S.vbuf;
+  if (ev == kTokEscape) return false;  // the escape value failed (R.err is set)
+  if (ev == kTokUnused) { R.err = kErrCorrupt | (kTileUnused << 8); return false; }
+  if (ev == kTokUnderrun) { R.err = kErrCorrupt | (kTileUnderrun << 8); return false; }
+  if (S.firstsi) {
+    const int first = static_cast<int>(S.firstsi) - 1;
+    const int32_t first_mag = static_cast<int32_t>(S.firstpm >> 8);
+    bool neg = (S.sumabs & 1u) != 0;
+    if (static_cast<int>(S.last) - first < kSDHMinSpan) {
+      const int sg = R.decode(T.misc0, kGSign, 0xFFFFu, lane);
+      if (sg < 0) return false;
+      neg = sg != 0;
+    }
+    out.emit(coef_entry(static_cast<int>(S.firstpm & 0xFFu), blk, neg ? -first_mag : first_mag), lane);
   }
   return true;
 }
@@ -309,16 +628,16 @@ struct MBState {
 // known; uncoded chroma planes once luma is complete (even when an earlier
 // coded chroma plane then fails); nothing when the failure comes before the
 // pattern.
-__device__ __forceinline__ bool parse_mb(Rans& R, CoefOut& out, MBState& m, uint32_t lanes,
-                                         uint32_t zz4, uint32_t zz8, uint32_t zz16, uint32_t csign, bool intra,
-                                         uint32_t pred_last, int lane) {
+__device__ __forceinline__ bool parse_mb(Rans& R, CoefOut& out, MBState& m, const Tables& T, const FastTok& F,
+                                         uint32_t zz4, uint32_t zz8, uint32_t zz16, bool intra, uint32_t pred_last,
+                                         int lane) {
   bool luma_coded = true, cb_coded = true, cr_coded = true;
   if (!intra) {
-    const int refsel = R.decode(cum_row(lanes, kCtxRef), lane);
+    const int refsel = R.decode(T.misc1, kGRef, 0xFFFFu, lane);
     if (refsel < 0) return false;
     m.refsel = static_cast<uint32_t>(refsel);
     const uint32_t pred = refsel == kRefGolden ? 0u : pred_last;
-    const int part = R.decode(cum_row(lanes, kCtxPart), lane);
+    const int part = R.decode(T.misc1, kGPart, 0xFFFFu, lane);
     if (part < 0) return false;
     m.part = static_cast<uint32_t>(part);
     const int npart = static_cast<int>((kPartCountNibbles >> (4 * part)) & 15u);
@@ -326,7 +645,7 @@ __device__ __forceinline__ bool parse_mb(Rans& R, CoefOut& out, MBState& m, uint
       const uint32_t base = pi > 0 ? m.mv0 : pred;
       uint32_t mv = 0;
       for (int comp = 0; comp < 2; comp++) {
-        const int v = R.class_value(cum_row(lanes, comp ? kCtxMVClassY : kCtxMVClassX), lane);
+        const int v = R.class_value(T.misc2, comp ? kGMVY : kGMVX, lane);
         if (v < 0) return false;
         mv |= static_cast<uint32_t>(static_cast<uint16_t>(clamp_mv(mv_comp(base, comp) + zigzag_dec(static_cast<uint32_t>(v)))))
               << (16 * comp);
@@ -337,7 +656,7 @@ __device__ __forceinline__ bool parse_mb(Rans& R, CoefOut& out, MBState& m, uint
       else m.mv3 = mv;
     }
     m.mvcur = refsel == kRefGolden ? 0u : m.mv0;
-    const int cbp = R.decode(cum_row(lanes, kCtxCBP), lane);
+    const int cbp = R.decode(T.misc0, kGCBP, 0xFFFFu, lane);
     if (cbp < 0) return false;
     m.cbp = static_cast<uint32_t>(cbp);
     luma_coded = (cbp & kCBPLuma) != 0;
@@ -345,24 +664,27 @@ __device__ __forceinline__ bool parse_mb(Rans& R, CoefOut& out, MBState& m, uint
     cr_coded = (cbp & kCBPCr) != 0;
     if (!luma_coded) m.wmask |= 0xFFFFu;
   }
-  int nb = 0, total = 0, ctxbase = 0;
+  int nb = 0, total = 0, tx = 0;
   uint32_t zzv = zz4;
   if (luma_coded) {
-    const int tx = R.decode(cum_row(lanes, kCtxTxSize), lane);
+    tx = R.decode(T.misc0, kGTx, 0xFFFFu, lane);
     if (tx < 0) return false;
     m.tx = static_cast<uint32_t>(tx);
     const int n = 4 << tx;
     nb = (16 / n) * (16 / n);
     total = n * n;
     zzv = tx == 0 ? zz4 : (tx == 1 ? zz8 : zz16);
-    ctxbase = kCtxTokenBase + tx * kNumBands * kNumPrevClass;
   }
+  // The luma blocks' token vectors (set = transform index).
+  const uint32_t la = tx == 0 ? T.t[0][0] : (tx == 1 ? T.t[1][0] : T.t[2][0]);
+  const uint32_t lb = tx == 0 ? T.t[0][1] : (tx == 1 ? T.t[1][1] : T.t[2][1]);
+  const uint32_t lc = tx == 0 ? T.t[0][2] : (tx == 1 ? T.t[1][2] : T.t[2][2]);
   for (int k = 0; k < nb + 2; k++) {
     const bool chroma = k >= nb;
     const int blk = chroma ? 16 + (k - nb) : k;
     if (k == nb) {  // luma complete
       if (intra) {
-        const int cmode = R.decode(cum_row(lanes, kCtxChromaMode), lane);
+        const int cmode = R.decode(T.misc1, kGChromaMode, 0xFFFFu, lane);
         if (cmode < 0) return false;
         m.cmode = static_cast<uint32_t>(cmode);
       } else {
@@ -372,13 +694,19 @@ __device__ __forceinline__ bool parse_mb(Rans& R, CoefOut& out, MBState& m, uint
     }
     if (chroma && !(blk == 16 ? cb_coded : cr_coded)) continue;
     if (intra && !chroma) {
-      const int mode = R.decode(cum_row(lanes, kCtxLumaMode), lane);
+      const int mode = R.decode(T.misc1, kGLumaMode, 0xFFFFu, lane);
       if (mode < 0) return false;
       if (k < 8) m.lmode0 |= static_cast<uint32_t>(mode) << (4 * k);
       else m.lmode1 |= static_cast<uint32_t>(mode) << (4 * (k - 8));
     }
-    if (!parse_block(R, out, lanes, chroma ? kCtxTokenBase + kLumaTokenCtx : ctxbase, chroma ? zz8 : zzv,
-                     chroma ? 64 : total, blk, csign, lane))
+    const uint32_t va = chroma ? T.t[3][0] : la, vb = chroma ? T.t[3][1] : lb, vc = chroma ? T.t[3][2] : lc;
+    const uint32_t zb = chroma ? zz8 : zzv;
+    const int tb = chroma ? 64 : total;
+    // The token loop needs normalised states (a symbol then takes at most
+    // two bytes) and a sign table; otherwise the generic decoder runs.
+    const bool fast = F.sign_used && min(min(R.x0, R.x1), min(R.x2, R.x3)) >= kRansLower;
+    if (!(fast ? parse_block_fast(R, out, T, F, va, vb, vc, zb, tb, blk, lane)
+               : parse_block(R, out, T, va, vb, vc, zb, tb, blk, lane)))
       return false;
     m.wmask |= 1u << blk;
   }
@@ -412,18 +740,31 @@ __device__ __forceinline__ void parse_tile(const uint8_t* __restrict__ pkt, cons
   const FrameHdr* hdr = reinterpret_cast<const FrameHdr*>(desc);
   const TileDesc* tiles = reinterpret_cast<const TileDesc*>(desc + desc_tiles_off());
   const int lane = lane_id();
-  const int lane16 = lane < 16 ? lane : 16;
   const TileDesc td = tiles[t];
 
-  // LDS: cumulative tables, then two MV rows of the tile width.
-  const uint32_t lanes = lds_addr(smem) + static_cast<uint32_t>(2 * lane16);
-  const uint32_t* gcum = reinterpret_cast<const uint32_t*>(hdr->cum);
-  constexpr int kCumWords = 72 * kCumStride / 2;
-  for (int i = lane; i < kCumWords; i += 64) smem[i] = gcum[i];
-  uint32_t* prow = smem + kCumWords;
+  // The frame's symbol tables, one entry per lane of each vector.
+  using GWord = __attribute__((address_space(1))) const uint32_t;
+  const GWord* vt = (const GWord*)(&hdr->vt[0][lane]);
+  Tables T;
+  T.misc0 = vt[0];
+  T.misc1 = vt[64];
+  T.misc2 = vt[128];
+#pragma unroll
+  for (int s = 0; s < 4; s++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) T.t[s][j] = vt[64 * (3 + 3 * s + j)];
+  FastTok F;
+  {
+    const uint32_t e0 = rd_lane(T.misc0, static_cast<int>(kGSign)), e1 = rd_lane(T.misc0, static_cast<int>(kGSign) + 1);
+    F.sign_used = e0 != kNoSym;
+    F.f0 = e0 & 0x1FFFu;
+    F.f1 = e1 & 0x1FFFu;
+    F.vlane4 = 4u * static_cast<uint32_t>(lane);
+  }
+  // LDS: two MV rows of the tile width.
+  uint32_t* prow = smem;
   uint32_t* crow = prow + hdr->max_tile_w;
   const int tw = td.x1 - td.x0;
-  __syncthreads();
 
   if (td.byte_len < 16) {  // ransDecoder.reset (rans.go:237-250)
     if (lane == 0) status[t] = TileStatus{kErrCorrupt | (kTileShort << 8), 0, 0, 0};
@@ -454,14 +795,13 @@ __device__ __forceinline__ void parse_tile(const uint8_t* __restrict__ pkt, cons
     if (lane < 16) zz8 = z[64 + lane];
     zz16 = z[128 + lane];
   }
-  const uint32_t csign = cum_row(lanes, kCtxSign);
   // Let every start-up load land here, so no load is outstanding in the
   // macroblock loop: a wait there would also wait for every store before it.
   __builtin_amdgcn_s_waitcnt(0);
 
   CoefOut out;
   out.dst = coefs + td.coef_base;
-  out.base = 0; out.cn = 0; out.count = 0; out.buf = 0;
+  out.base = 0; out.cn = 0; out.buf = 0;
   const int mbc = hdr->mbc;
   const bool inter_frame = hdr->inter != 0;
   uint32_t done = 0, nintra = 0;
@@ -469,11 +809,11 @@ __device__ __forceinline__ void parse_tile(const uint8_t* __restrict__ pkt, cons
   for (int mby = td.y0; ok && mby < td.y1; mby++) {
     for (int mbx = td.x0; mbx < td.x1; mbx++) {
       MBState m{2, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-      const uint32_t coef_start = out.base + out.cn, count0 = out.count;
+      const uint32_t coef_start = out.base + out.cn;
       bool coded = true;
       if (inter_frame) {
         // A failed mode decode reads as skip, which Go then reconstructs.
-        const int mode = R.decode(cum_row(lanes, kCtxMBMode), lane);
+        const int mode = R.decode(T.misc0, kGMBMode, 0xFFFFu, lane);
         if (mode < 0 || mode == kMBSkip) {
           const uint32_t pred = predict_mv(prow, crow, mbx - td.x0, tw, mby == td.y0);
           m.kind = 0; m.mv0 = pred; m.mvcur = pred; m.wmask = 0x3FFFFu;
@@ -486,7 +826,7 @@ __device__ __forceinline__ void parse_tile(const uint8_t* __restrict__ pkt, cons
       if (coded) {
         const bool intra = m.kind == 2;
         const uint32_t pred = intra ? 0u : predict_mv(prow, crow, mbx - td.x0, tw, mby == td.y0);
-        ok = parse_mb(R, out, m, lanes, zz4, zz8, zz16, csign, intra, pred, lane);
+        ok = parse_mb(R, out, m, T, F, zz4, zz8, zz16, intra, pred, lane);
       }
       if (m.kind == 2) nintra++;
       if (lane == 0) {
@@ -502,7 +842,7 @@ __device__ __forceinline__ void parse_tile(const uint8_t* __restrict__ pkt, cons
         r.pad1 = 0;
         r.wmask = m.wmask;
         r.coef_off = td.coef_base + coef_start;
-        r.coef_count = out.count - count0;
+        r.coef_count = out.base + out.cn - coef_start;
         r.mv[0][0] = mv_comp(m.mv0, 0); r.mv[0][1] = mv_comp(m.mv0, 1);
         r.mv[1][0] = mv_comp(m.mv1, 0); r.mv[1][1] = mv_comp(m.mv1, 1);
         r.mv[2][0] = mv_comp(m.mv2, 0); r.mv[2][1] = mv_comp(m.mv2, 1);
@@ -1148,7 +1488,7 @@ hipError_t upload_tables(const int32_t dct[3][256], const uint8_t zz[3][256]) {
 }
 
 void launch_parse(hipStream_t s, int nitems, int max_tile_w, const FrameJob* jobs, const WorkItem* items) {
-  size_t lds = (72 * kCumStride / 2 + 2 * static_cast<size_t>(max_tile_w)) * sizeof(uint32_t);
+  size_t lds = 2 * static_cast<size_t>(max_tile_w) * sizeof(uint32_t);
   hipLaunchKernelGGL(ebiv_parse, dim3(nitems), dim3(64), lds, s, jobs, items);
 }
 

@@ -289,11 +289,13 @@ struct Session {
       // Go indexes slot2sym with start+k; an entry above ransM whose wrapped
       // sum still reaches ransM would panic there.  Such a stream is corrupt.
       for (int s = 0; s < n; s++) if (f[s] > kRansM) return false;
-      uint32_t cum = 0;
-      for (int s = 0; s < kCumStride; s++) {
-        if (s < n) { hdr.cum[c][s] = static_cast<uint16_t>(cum); cum += f[s]; }
-        else if (s == n) hdr.cum[c][s] = static_cast<uint16_t>(kRansM);
-        else hdr.cum[c][s] = 0xFFFF;
+      int vec = 0, lane0 = 0;
+      if (table_place(c, &vec, &lane0)) {
+        uint32_t start = 0;
+        for (int s = 0; s < n; s++) {
+          hdr.vt[vec][lane0 + s] = start << 13 | f[s];
+          start += f[s];
+        }
       }
       hdr.used[c >> 5] |= 1u << (c & 31);
       return true;
@@ -378,10 +380,10 @@ struct Session {
     uint8_t* hd = ps->h_desc.h;
     std::memset(hd, 0, dsize);
     FrameHdr& hdr = *reinterpret_cast<FrameHdr*>(hd);
-    // A context no table installs reads as 0xFFFF on every lane of the
+    // A context no table installs reads as kNoSym on every lane of the
     // entropy kernel's table search, which finds no symbol there: the
     // "symbol decoded from unused context" error of rans.go:275-278.
-    std::memset(hdr.cum, 0xFF, sizeof hdr.cum);
+    std::memset(hdr.vt, 0xFF, sizeof hdr.vt);
     size_t consumed = 0;
     int rc = parse_tables(body + pos, static_cast<size_t>(tlen), key, hdr, &consumed);
     if (rc) return rc;
