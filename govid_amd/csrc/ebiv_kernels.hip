@@ -113,7 +113,12 @@ __device__ __forceinline__ int16_t median3(int16_t a, int16_t b, int16_t c) {
 }
 
 // predictMV (mv.go:21-52) over the LDS rows of the tile's MV grid.
-__device__ __forceinline__ uint32_t predict_mv(const uint32_t* prow, const uint32_t* crow, int lx, int tw, bool top_row) {
+// The tile's MV rows live in LDS; typed as LDS pointers their accesses stay
+// LDS instructions (generic pointers would make them flat accesses, and a flat
+// load waits for every outstanding store).
+using LdsU32 = __attribute__((address_space(3))) uint32_t;
+
+__device__ __forceinline__ uint32_t predict_mv(const LdsU32* prow, const LdsU32* crow, int lx, int tw, bool top_row) {
   bool aok = lx > 0, bok = !top_row, cok = false;
   uint32_t a = 0, b = 0, c = 0;
   if (aok) a = crow[lx - 1];
@@ -219,7 +224,10 @@ struct Rans {
 // The table vectors of one tile's wavefront.
 struct Tables {
   uint32_t misc0, misc1, misc2;  // non-token contexts
-  uint32_t t[4][3];              // token contexts: set (luma tx 0-2, chroma), band pair
+  // token contexts t<set><band pair>: set = luma tx 0-2 or 3 for chroma.
+  // Named registers rather than an array: an array indexed by the transform
+  // size would be placed in scratch memory.
+  uint32_t t00, t01, t02, t10, t11, t12, t20, t21, t22, t30, t31, t32;
 };
 // Lane groups of the non-token contexts (vector, first lane).
 constexpr uint32_t kGSign = 0, kGMBMode = 16, kGTx = 32, kGCBP = 48;       // misc0
@@ -676,9 +684,16 @@ __device__ __forceinline__ bool parse_mb(Rans& R, CoefOut& out, MBState& m, cons
     zzv = tx == 0 ? zz4 : (tx == 1 ? zz8 : zz16);
   }
   // The luma blocks' token vectors (set = transform index).
-  const uint32_t la = tx == 0 ? T.t[0][0] : (tx == 1 ? T.t[1][0] : T.t[2][0]);
-  const uint32_t lb = tx == 0 ? T.t[0][1] : (tx == 1 ? T.t[1][1] : T.t[2][1]);
-  const uint32_t lc = tx == 0 ? T.t[0][2] : (tx == 1 ? T.t[1][2] : T.t[2][2]);
+  // The empty asm hands over the registers as opaque values: a selection
+  // between fields of one struct is otherwise turned into an indexed load,
+  // which puts the struct in scratch memory.
+  uint32_t a0 = T.t00, a1 = T.t10, a2 = T.t20, b0 = T.t01, b1 = T.t11, b2 = T.t21, c0 = T.t02, c1 = T.t12, c2 = T.t22;
+  uint32_t ca = T.t30, cb = T.t31, cc = T.t32;
+  asm volatile("" : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(b0), "+v"(b1), "+v"(b2));
+  asm volatile("" : "+v"(c0), "+v"(c1), "+v"(c2), "+v"(ca), "+v"(cb), "+v"(cc));
+  const uint32_t la = tx == 0 ? a0 : (tx == 1 ? a1 : a2);
+  const uint32_t lb = tx == 0 ? b0 : (tx == 1 ? b1 : b2);
+  const uint32_t lc = tx == 0 ? c0 : (tx == 1 ? c1 : c2);
   for (int k = 0; k < nb + 2; k++) {
     const bool chroma = k >= nb;
     const int blk = chroma ? 16 + (k - nb) : k;
@@ -699,7 +714,7 @@ __device__ __forceinline__ bool parse_mb(Rans& R, CoefOut& out, MBState& m, cons
       if (k < 8) m.lmode0 |= static_cast<uint32_t>(mode) << (4 * k);
       else m.lmode1 |= static_cast<uint32_t>(mode) << (4 * (k - 8));
     }
-    const uint32_t va = chroma ? T.t[3][0] : la, vb = chroma ? T.t[3][1] : lb, vc = chroma ? T.t[3][2] : lc;
+    const uint32_t va = chroma ? ca : la, vb = chroma ? cb : lb, vc = chroma ? cc : lc;
     const uint32_t zb = chroma ? zz8 : zzv;
     const int tb = chroma ? 64 : total;
     // The token loop needs normalised states (a symbol then takes at most
@@ -733,6 +748,27 @@ __device__ __forceinline__ WorkItem item_of(const WorkItem* items) {
 
 // decodeTile (decode_intra.go:121-135) for tile t: every macroblock in raster
 // order until the first error; one record per macroblock reached.
+constexpr uint32_t kRecStage = 16;  // macroblock records staged in LDS per store burst
+
+// base[off / 4] = v on the lanes where pred holds.  Written as one asm
+// statement: the compiler treats a store's operand registers as busy until
+// the store completes (vmcnt), so a compiler-visible store inside the
+// macroblock loop makes the next write of those registers wait for memory.
+// The hardware reads a dword store's operands at issue; the asm keeps the
+// compiler from waiting.
+__device__ __forceinline__ void store_dword_if(uint32_t* base, uint32_t off, uint32_t v, bool pred) {
+  uint64_t m, save;
+  asm volatile(
+      "s_nop 4\n"
+      "v_cmp_ne_u32_e64 %[m], %[p], 0\n"
+      "s_and_saveexec_b64 %[save], %[m]\n"
+      "global_store_dword %[off], %[v], %[base]\n"
+      "s_mov_b64 exec, %[save]\n"
+      : [m] "=&s"(m), [save] "=&s"(save)
+      : [p] "v"(static_cast<uint32_t>(pred)), [off] "v"(off), [v] "v"(v), [base] "s"(base)
+      : "memory", "scc");
+}
+
 __device__ __forceinline__ void parse_tile(const uint8_t* __restrict__ pkt, const uint8_t* __restrict__ desc,
                                            MBRec* __restrict__ recs, uint32_t* __restrict__ coefs,
                                            TileStatus* __restrict__ status, const int t) {
@@ -749,10 +785,10 @@ __device__ __forceinline__ void parse_tile(const uint8_t* __restrict__ pkt, cons
   T.misc0 = vt[0];
   T.misc1 = vt[64];
   T.misc2 = vt[128];
-#pragma unroll
-  for (int s = 0; s < 4; s++)
-#pragma unroll
-    for (int j = 0; j < 3; j++) T.t[s][j] = vt[64 * (3 + 3 * s + j)];
+  T.t00 = vt[64 * 3]; T.t01 = vt[64 * 4]; T.t02 = vt[64 * 5];
+  T.t10 = vt[64 * 6]; T.t11 = vt[64 * 7]; T.t12 = vt[64 * 8];
+  T.t20 = vt[64 * 9]; T.t21 = vt[64 * 10]; T.t22 = vt[64 * 11];
+  T.t30 = vt[64 * 12]; T.t31 = vt[64 * 13]; T.t32 = vt[64 * 14];
   FastTok F;
   {
     const uint32_t e0 = rd_lane(T.misc0, static_cast<int>(kGSign)), e1 = rd_lane(T.misc0, static_cast<int>(kGSign) + 1);
@@ -761,9 +797,24 @@ __device__ __forceinline__ void parse_tile(const uint8_t* __restrict__ pkt, cons
     F.f1 = e1 & 0x1FFFu;
     F.vlane4 = 4u * static_cast<uint32_t>(lane);
   }
-  // LDS: two MV rows of the tile width.
-  uint32_t* prow = smem;
-  uint32_t* crow = prow + hdr->max_tile_w;
+  // LDS: the macroblock records being staged, then two MV rows of the tile
+  // width.  Records go to memory kRecStage at a time, every lane storing
+  // dwords: a store per macroblock from one lane would have the next
+  // macroblock's record registers wait for it.
+  LdsU32* stage = (LdsU32*)smem;
+  LdsU32* prow = (LdsU32*)(smem + kRecStage * sizeof(MBRec) / 4);
+  LdsU32* crow = prow + hdr->max_tile_w;
+  uint32_t nst = 0, first_mb = 0;
+  auto flush = [&]() {
+    const LdsU32* src = stage;
+    uint32_t* dst = reinterpret_cast<uint32_t*>(recs + first_mb);
+    const uint32_t nd = nst * (sizeof(MBRec) / 4);
+    for (uint32_t i0 = 0; i0 < nd; i0 += 64) {
+      const uint32_t i = i0 + static_cast<uint32_t>(lane);
+      store_dword_if(dst + i0, 4u * static_cast<uint32_t>(lane), src[i < nd ? i : 0], i < nd);
+    }
+    nst = 0;
+  };
   const int tw = td.x1 - td.x0;
 
   if (td.byte_len < 16) {  // ransDecoder.reset (rans.go:237-250)
@@ -795,9 +846,13 @@ __device__ __forceinline__ void parse_tile(const uint8_t* __restrict__ pkt, cons
     if (lane < 16) zz8 = z[64 + lane];
     zz16 = z[128 + lane];
   }
-  // Let every start-up load land here, so no load is outstanding in the
-  // macroblock loop: a wait there would also wait for every store before it.
-  __builtin_amdgcn_s_waitcnt(0);
+  // Let every start-up load land here (an empty asm that takes the loaded
+  // registers), so no load into a register the macroblock loop uses is
+  // outstanding there: a wait in the loop would also wait for every store
+  // issued before it.
+  asm volatile("" : "+v"(T.misc0), "+v"(T.misc1), "+v"(T.misc2), "+v"(zz4), "+v"(zz8), "+v"(zz16));
+  asm volatile("" : "+v"(T.t00), "+v"(T.t01), "+v"(T.t02), "+v"(T.t10), "+v"(T.t11), "+v"(T.t12));
+  asm volatile("" : "+v"(T.t20), "+v"(T.t21), "+v"(T.t22), "+v"(T.t30), "+v"(T.t31), "+v"(T.t32));
 
   CoefOut out;
   out.dst = coefs + td.coef_base;
@@ -808,6 +863,7 @@ __device__ __forceinline__ void parse_tile(const uint8_t* __restrict__ pkt, cons
   bool ok = true;
   for (int mby = td.y0; ok && mby < td.y1; mby++) {
     for (int mbx = td.x0; mbx < td.x1; mbx++) {
+      if (nst == 0) first_mb = static_cast<uint32_t>(mby * mbc + mbx);
       MBState m{2, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
       const uint32_t coef_start = out.base + out.cn;
       bool coded = true;
@@ -850,13 +906,18 @@ __device__ __forceinline__ void parse_tile(const uint8_t* __restrict__ pkt, cons
         r.lmode[0] = m.lmode0;
         r.lmode[1] = m.lmode1;
         r.pad2 = 0;
-        recs[mby * mbc + mbx] = r;
+        const uint32_t* rw = reinterpret_cast<const uint32_t*>(&r);
+        LdsU32* sw = stage + nst * (sizeof(MBRec) / 4);
+#pragma unroll
+        for (int k = 0; k < static_cast<int>(sizeof(MBRec) / 4); k++) sw[k] = rw[k];
       }
       done++;
+      if (++nst == kRecStage) flush();
       if (!ok) break;
     }
+    if (nst) flush();
     wave_sync();
-    uint32_t* tmp = prow; prow = crow; crow = tmp;
+    LdsU32* tmp = prow; prow = crow; crow = tmp;
   }
   if (static_cast<uint32_t>(lane) < out.cn) out.dst[out.base + lane] = out.buf;
   if (lane == 0) status[t] = TileStatus{R.err, done, R.nsym, nintra};
@@ -1488,7 +1549,7 @@ hipError_t upload_tables(const int32_t dct[3][256], const uint8_t zz[3][256]) {
 }
 
 void launch_parse(hipStream_t s, int nitems, int max_tile_w, const FrameJob* jobs, const WorkItem* items) {
-  size_t lds = 2 * static_cast<size_t>(max_tile_w) * sizeof(uint32_t);
+  size_t lds = kRecStage * sizeof(MBRec) + 2 * static_cast<size_t>(max_tile_w) * sizeof(uint32_t);
   hipLaunchKernelGGL(ebiv_parse, dim3(nitems), dim3(64), lds, s, jobs, items);
 }
 
