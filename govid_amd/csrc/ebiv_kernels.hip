@@ -941,10 +941,13 @@ struct WaveScratch {
 
 // Dequantises the macroblock's coefficient list into dense per-block arrays:
 // luma block b of size n at [b*n*n, (b+1)*n*n), Cb at 256, Cr at 320.
-__device__ void load_coefs(WaveScratch& ws, const uint32_t* __restrict__ coefs, const MBRec& r, int n, int qdc, int qac,
-                           int lane) {
+// Returns the blocks holding a nonzero coefficient (bits 0-17) and, shifted
+// by 32, those holding one away from the DC position.
+__device__ uint64_t load_coefs(WaveScratch& ws, const uint32_t* __restrict__ coefs, const MBRec& r, int n, int qdc,
+                               int qac, int lane) {
   for (int i = lane; i < 384; i += 64) ws.coef[i] = 0;
   wave_sync();
+  uint32_t nzm = 0, acm = 0;
   uint32_t cnt = r.coef_count < 384u ? r.coef_count : 384u;
   for (uint32_t i = lane; i < cnt; i += 64) {
     uint32_t e = coefs[r.coef_off + i];
@@ -954,64 +957,119 @@ __device__ void load_coefs(WaveScratch& ws, const uint32_t* __restrict__ coefs, 
     int idx;
     if (blk < 16) {
       int nn = n * n;
-      idx = blk * nn + (pos < nn ? pos : nn - 1);
+      if (pos >= nn) pos = nn - 1;
+      idx = blk * nn + pos;
       if (idx >= 256) continue;
     } else if (blk < 18) {
-      idx = 256 + (blk - 16) * 64 + (pos & 63);
+      pos &= 63;
+      idx = 256 + (blk - 16) * 64 + pos;
     } else {
       continue;
     }
     ws.coef[idx] = v;
+    if (v != 0) {
+      nzm |= 1u << blk;
+      if (pos != 0) acm |= 1u << blk;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    nzm |= __shfl_xor(nzm, off);
+    acm |= __shfl_xor(acm, off);
+  }
+  wave_sync();
+  return static_cast<uint64_t>(nzm) | (static_cast<uint64_t>(acm) << 32);
+}
+
+// The DCT matrices of buildDCTMatrix (transform.go:15-40) for n = 4, 8, 16 at
+// offsets 0, 256 and 512, copied into LDS by each workgroup (a lane-indexed
+// read of constant memory is a global load per element).
+using LdsI32 = __attribute__((address_space(3))) int32_t;
+__device__ __forceinline__ void load_dct(LdsI32* dct) {
+  const int32_t* src = &c_dct[0][0];
+  for (int i = threadIdx.x; i < 768; i += blockDim.x) dct[i] = src[i];
+  __syncthreads();
+}
+
+// Separable inverse DCT (transform.go:129-194) of nblk n x n blocks at base:
+// column pass into tmp, row pass back into coef, each with the
+// (x + 4096) >> 13 rounding.  Zero coefficients add nothing, so this equals
+// the reference's sparse forms bit for bit; in particular an all-zero block
+// stays zero and a DC-only block is the constant
+//   (c0 * ((c0 * dc + 4096) >> 13) + 4096) >> 13,  c0 = C[0][i] for every i
+// (transform.go:141-160).  The reference accumulates in 64 bits; when every
+// input is at most kIdct32Max in magnitude both passes' sums stay below 2^31
+// (a column of the n = 16 matrix sums to at most 45,488 in magnitude, so the
+// first pass is at most 5.56x the input and the second 45,488 x that), and
+// 32-bit accumulation gives the same integers.
+constexpr int32_t kIdct32Max = 8190;
+template <int N, typename Acc>
+__device__ __forceinline__ void idct_blocks(WaveScratch& ws, const LdsI32* C, int base, int nblk, uint32_t nzm,
+                                            uint32_t acm, int lane) {
+  constexpr int NN = N * N;
+  const int total = nblk * NN;
+  const int32_t c0 = C[0];
+  for (int e = lane; e < total; e += 64) {
+    const int b = e / NN, local = e - b * NN;
+    if ((acm >> b) & 1u) {
+      const int i = local / N, l = local % N;
+      const int32_t* src = ws.coef + base + b * NN + l;
+      Acc acc = 0;
+#pragma unroll
+      for (int k = 0; k < N; k++) acc += static_cast<Acc>(C[k * N + i]) * src[k * N];
+      ws.tmp[base + e] = static_cast<int32_t>((acc + 4096) >> 13);
+    } else if (((nzm >> b) & 1u) && local == 0) {
+      ws.tmp[base + e] = static_cast<int32_t>((static_cast<Acc>(c0) * ws.coef[base + e] + 4096) >> 13);
+    }
+  }
+  wave_sync();
+  for (int e = lane; e < total; e += 64) {
+    const int b = e / NN, local = e - b * NN;
+    if ((acm >> b) & 1u) {
+      const int i = local / N, m = local % N;
+      const int32_t* t = ws.tmp + base + b * NN + i * N;
+      Acc acc = 0;
+#pragma unroll
+      for (int l = 0; l < N; l++) acc += static_cast<Acc>(C[l * N + m]) * t[l];
+      ws.coef[base + e] = static_cast<int32_t>((acc + 4096) >> 13);
+    } else if ((nzm >> b) & 1u) {
+      ws.coef[base + e] = static_cast<int32_t>((static_cast<Acc>(c0) * ws.tmp[base + b * NN] + 4096) >> 13);
+    }
   }
   wave_sync();
 }
 
-// Separable inverse DCT of every block (transform.go:129-194): column pass
-// into tmp, row pass back into coef, with the per-pass (x + 4096) >> 13
-// rounding.  Zero coefficients add nothing, so the result equals the
-// reference's sparse form bit for bit.  The reference accumulates in 64 bits;
-// when every input is at most kIdct32Max in magnitude both passes' sums stay
-// below 2^31 (a column of the n = 16 matrix sums to at most 45,488 in
-// magnitude, so the first pass is at most 5.56x the input and the second
-// 45,488 x that), and 32-bit accumulation gives the same integers.
-constexpr int32_t kIdct32Max = 8190;
 template <typename Acc>
-__device__ __forceinline__ void idct_passes(WaveScratch& ws, int n, bool luma, bool chroma, int lane);
-__device__ void idct_all(WaveScratch& ws, int n, bool luma, bool chroma, bool narrow, int lane) {
-  if (narrow) idct_passes<int32_t>(ws, n, luma, chroma, lane);
-  else idct_passes<int64_t>(ws, n, luma, chroma, lane);
+__device__ __forceinline__ void idct_mb(WaveScratch& ws, const LdsI32* dct, int n, uint32_t nzm, uint32_t acm,
+                                        int lane) {
+  if (nzm & 0xFFFFu) {
+    if (n == 4) idct_blocks<4, Acc>(ws, dct, 0, 16, nzm, acm, lane);
+    else if (n == 8) idct_blocks<8, Acc>(ws, dct + 256, 0, 4, nzm, acm, lane);
+    else idct_blocks<16, Acc>(ws, dct + 512, 0, 1, nzm, acm, lane);
+  }
+  if (nzm & 0x30000u) idct_blocks<8, Acc>(ws, dct + 256, 256, 2, nzm >> 16, acm >> 16, lane);
 }
-template <typename Acc>
-__device__ __forceinline__ void idct_passes(WaveScratch& ws, int n, bool luma, bool chroma, int lane) {
-  const int32_t* cl = c_dct[sidx_of(n)];
-  const int32_t* c8 = c_dct[1];
-  const int nn = n * n;
-  // column pass: tmp[i][l] = sum_k C[k][i] * src[k][l]
+
+// Largest |coefficient| of the macroblock's dequantised residual over the
+// wavefront.
+__device__ __forceinline__ int32_t wave_max_abs(const WaveScratch& ws, int lane) {
+  int32_t m = 0;
   for (int e = lane; e < 384; e += 64) {
-    if (e < 256 ? !luma : !chroma) continue;
-    int bn, base, local;
-    const int32_t* c;
-    if (e < 256) { bn = n; base = (e / nn) * nn; local = e - base; c = cl; }
-    else { bn = 8; base = 256 + ((e - 256) >> 6) * 64; local = (e - 256) & 63; c = c8; }
-    int i = local / bn, l = local - i * bn;
-    Acc acc = 0;
-    for (int k = 0; k < bn; k++) acc += static_cast<Acc>(c[k * bn + i]) * ws.coef[base + k * bn + l];
-    ws.tmp[e] = static_cast<int32_t>((acc + 4096) >> 13);
+    const int32_t v = ws.coef[e];
+    m = max(m, v < 0 ? -v : v);
   }
-  wave_sync();
-  // row pass: out[i][m] = sum_l C[l][m] * tmp[i][l]
-  for (int e = lane; e < 384; e += 64) {
-    if (e < 256 ? !luma : !chroma) continue;
-    int bn, base, local;
-    const int32_t* c;
-    if (e < 256) { bn = n; base = (e / nn) * nn; local = e - base; c = cl; }
-    else { bn = 8; base = 256 + ((e - 256) >> 6) * 64; local = (e - 256) & 63; c = c8; }
-    int i = local / bn, mm = local - i * bn;
-    Acc acc = 0;
-    for (int l = 0; l < bn; l++) acc += static_cast<Acc>(c[l * bn + mm]) * ws.tmp[base + i * bn + l];
-    ws.coef[e] = static_cast<int32_t>((acc + 4096) >> 13);
-  }
-  wave_sync();
+  for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
+  return m;
+}
+
+// Dequantisation and inverse transform of a macroblock's residual into
+// ws.coef (dense, zero where nothing is coded).
+__device__ __forceinline__ void residual_mb(WaveScratch& ws, const LdsI32* dct, const uint32_t* __restrict__ coefs,
+                                            const MBRec& r, int n, int qdc, int qac, int lane) {
+  const uint64_t masks = load_coefs(ws, coefs, r, n, qdc, qac, lane);
+  const uint32_t nzm = static_cast<uint32_t>(masks), acm = static_cast<uint32_t>(masks >> 32);
+  if (!nzm) return;
+  if (wave_max_abs(ws, lane) <= kIdct32Max) idct_mb<int32_t>(ws, dct, n, nzm, acm, lane);
+  else idct_mb<int64_t>(ws, dct, n, nzm, acm, lane);
 }
 
 __device__ __forceinline__ int clampc(int v, int hi) { return v < 0 ? 0 : (v >= hi ? hi - 1 : v); }
@@ -1072,17 +1130,6 @@ struct InterScratch {
   uint8_t cwin[2][9 * kCWinStride];
 };
 
-// Largest |coefficient| of the macroblock's dequantised residual over the
-// wavefront.
-__device__ __forceinline__ int32_t wave_max_abs(const WaveScratch& ws, int lane) {
-  int32_t m = 0;
-  for (int e = lane; e < 384; e += 64) {
-    const int32_t v = ws.coef[e];
-    m = max(m, v < 0 ? -v : v);
-  }
-  for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
-  return m;
-}
 
 // Partition rectangle pi of a partition mode (partRects, model.go:91-96) as
 // x | y << 8 | w << 16 | h << 24.
@@ -1108,6 +1155,9 @@ __device__ __forceinline__ void recon_inter_mb(const uint8_t* __restrict__ desc,
                                                const TileStatus* __restrict__ status, const Planes rec,
                                                const Planes ref, const Planes gold, const int first_mb) {
   __shared__ InterScratch scratch[4];
+  __shared__ int32_t s_dct[768];
+  LdsI32* dct = (LdsI32*)s_dct;
+  load_dct(dct);
   const FrameHdr* hdr = reinterpret_cast<const FrameHdr*>(desc);
   const TileDesc* tiles = reinterpret_cast<const TileDesc*>(desc + desc_tiles_off());
   const int lane = lane_id(), wave = threadIdx.x >> 6;
@@ -1127,11 +1177,7 @@ __device__ __forceinline__ void recon_inter_mb(const uint8_t* __restrict__ desc,
   const int n = 4 << (r.tx > 2 ? 2 : r.tx);
   const bool has_res = r.kind == 1 && r.coef_count > 0;
   const bool luma_res = has_res && (r.cbp & kCBPLuma);
-  if (has_res) {
-    load_coefs(ws, coefs, r, n, hdr->qdc, hdr->qac, lane);
-    idct_all(ws, n, (r.cbp & kCBPLuma) != 0, (r.cbp & (kCBPCb | kCBPCr)) != 0, wave_max_abs(ws, lane) <= kIdct32Max,
-             lane);
-  }
+  if (has_res) residual_mb(ws, dct, coefs, r, n, hdr->qdc, hdr->qac, lane);
   const int part = r.kind == 1 ? r.part : 0;
   const int npart = static_cast<int>((kPartCountNibbles >> (4 * part)) & 15u);
   for (int pi = 0; pi < npart; pi++) {
@@ -1327,13 +1373,12 @@ __device__ void prep_angular(const int32_t* ref, int32_t* ext, int32_t* dcout, i
   }
 }
 
-__device__ void intra_mb(IntraScratch& S, const uint32_t* __restrict__ coefs, const MBRec& r, const FrameHdr* hdr,
-                         const TileDesc& td, Planes rec, int mbx, int mby, int lane) {
+__device__ void intra_mb(IntraScratch& S, const LdsI32* dct, const uint32_t* __restrict__ coefs, const MBRec& r,
+                         const FrameHdr* hdr, const TileDesc& td, Planes rec, int mbx, int mby, int lane) {
   const int n = 4 << (r.tx > 2 ? 2 : r.tx);
   WaveScratch& ws = S.ws;
   if (r.coef_count > 0) {
-    load_coefs(ws, coefs, r, n, hdr->qdc, hdr->qac, lane);
-    idct_all(ws, n, true, true, wave_max_abs(ws, lane) <= kIdct32Max, lane);
+    residual_mb(ws, dct, coefs, r, n, hdr->qdc, hdr->qac, lane);
   } else {
     for (int i = lane; i < 384; i += 64) ws.coef[i] = 0;
   }
@@ -1443,6 +1488,8 @@ __device__ __forceinline__ void recon_intra_tile(const uint8_t* __restrict__ des
                                                  const TileStatus* __restrict__ status, const Planes rec, const int t) {
   __shared__ IntraScratch scratch[kIntraWaves];
   __shared__ uint32_t step_mask[kMaxStepWords];
+  __shared__ int32_t s_dct[768];
+  LdsI32* dct = (LdsI32*)s_dct;
   const FrameHdr* hdr = reinterpret_cast<const FrameHdr*>(desc);
   const TileDesc* tiles = reinterpret_cast<const TileDesc*>(desc + desc_tiles_off());
   const int lane = lane_id(), wave = threadIdx.x >> 6;
@@ -1454,7 +1501,7 @@ __device__ __forceinline__ void recon_intra_tile(const uint8_t* __restrict__ des
   const int mbc = hdr->mbc;
   const bool tracked = steps <= 32 * kMaxStepWords;
   if (threadIdx.x < kMaxStepWords) step_mask[threadIdx.x] = 0;
-  __syncthreads();
+  load_dct(dct);
   if (tracked) {
     for (uint32_t i = threadIdx.x; i < st.mbs_done; i += blockDim.x) {
       const int ry = static_cast<int>(i) / tw, rx = static_cast<int>(i) - ry * tw;
@@ -1477,7 +1524,7 @@ __device__ __forceinline__ void recon_intra_tile(const uint8_t* __restrict__ des
       int mbx = td.x0 + rx, mby = td.y0 + ry;
       const MBRec r = recs[mby * mbc + mbx];
       if (r.kind != 2) continue;
-      intra_mb(scratch[wave], coefs, r, hdr, td, rec, mbx, mby, lane);
+      intra_mb(scratch[wave], dct, coefs, r, hdr, td, rec, mbx, mby, lane);
     }
     // Macroblocks of the next step read what other wavefronts of this
     // workgroup stored in this one.  The barrier orders the stores, but the
