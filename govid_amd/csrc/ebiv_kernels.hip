@@ -56,7 +56,8 @@ __device__ __forceinline__ uint32_t rd_lane(uint32_t v, int l) { return __builti
 struct ByteWin {
   const uint8_t* tile;  // tile byte 0
   int32_t lo, hi;       // dword-aligned readable range [lo, hi), tile-relative
-  int32_t wbase;        // current window base, tile-relative (tile + wbase is 256-aligned)
+  int32_t wbase;        // current window base, tile-relative (tile + wbase is 4-aligned)
+  int32_t lim;          // max(wbase, 0): bytes at positions >= lim may be read from the window
   uint32_t cur;
 
   __device__ __forceinline__ void load(int lane) {
@@ -67,6 +68,7 @@ struct ByteWin {
     const int32_t ac = a < lo ? lo : (a >= hi ? hi - 4 : a);
     const uint32_t v = *(GWord*)(tile + ac);
     cur = (a >= lo && a < hi) ? v : 0u;
+    lim = wbase > 0 ? wbase : 0;
   }
   __device__ void init(const uint8_t* t, uint32_t len, int lane) {
     tile = t;
@@ -160,6 +162,44 @@ __device__ __forceinline__ uint32_t mv_set(uint32_t v, int comp, int16_t c) {
 // readlane (start and frequency together) decode a symbol; no memory access
 // is on the dependency chain from one symbol to the next.
 // ----------------------------------------------------------------------------
+// Common tail of the single-symbol decoders: xn is the updated state.
+// Renormalise from the lane window while pos > lim (lim = max(window base, 0)),
+// then rotate the states (rans.go:286-303).  st = 0 done, 1 unused context,
+// 2 renormalisation to be completed by advance() from xn.
+#define EBIV_DECODE_TAIL                               \
+  "s_cmp_lt_u32 %[xn], 0x800000\n"                    \
+  "s_cbranch_scc1 L%=_renorm\n"                       \
+  "L%=_rot:\n"                                        \
+  "s_mov_b32 %[x0], %[x1]\n"                          \
+  "s_mov_b32 %[x1], %[x2]\n"                          \
+  "s_mov_b32 %[x2], %[x3]\n"                          \
+  "s_mov_b32 %[x3], %[xn]\n"                          \
+  "s_add_u32 %[nsym], %[nsym], 1\n"                   \
+  "s_mov_b32 %[st], 0\n"                              \
+  "s_branch L%=_end\n"                                \
+  "L%=_renorm:\n"                                     \
+  "s_cmp_le_i32 %[pos], %[lim]\n"                     \
+  "s_cbranch_scc1 L%=_slow\n"                         \
+  "s_sub_u32 %[pos], %[pos], 1\n"                     \
+  "s_sub_u32 %[t1], %[pos], %[wbase]\n"               \
+  "s_lshr_b32 %[t2], %[t1], 2\n"                      \
+  "v_readlane_b32 %[t2], %[vwin], %[t2]\n"            \
+  "s_and_b32 %[t1], %[t1], 3\n"                       \
+  "s_lshl_b32 %[t1], %[t1], 3\n"                      \
+  "s_lshr_b32 %[t2], %[t2], %[t1]\n"                  \
+  "s_and_b32 %[t2], %[t2], 0xff\n"                    \
+  "s_lshl_b32 %[xn], %[xn], 8\n"                      \
+  "s_or_b32 %[xn], %[xn], %[t2]\n"                    \
+  "s_cmp_lt_u32 %[xn], 0x800000\n"                    \
+  "s_cbranch_scc1 L%=_renorm\n"                       \
+  "s_branch L%=_rot\n"                                \
+  "L%=_slow:\n"                                       \
+  "s_mov_b32 %[st], 2\n"                              \
+  "s_branch L%=_end\n"                                \
+  "L%=_unused:\n"                                     \
+  "s_mov_b32 %[st], 1\n"                              \
+  "L%=_end:\n"
+
 struct Rans {
   uint32_t x0, x1, x2, x3;  // x0 is state[counter & 3] (rans.go:279-282)
   uint32_t pos;             // tile bytes not yet consumed
@@ -184,9 +224,54 @@ struct Rans {
     return true;
   }
   // ransDecoder.decode (rans.go:270-304) for the context whose entries are
-  // the lanes [goff, goff + width) of V (gmask = 2^width - 1).  Returns the
-  // symbol, or -1 on error.
-  __device__ __forceinline__ int decode(uint32_t V, uint32_t goff, uint32_t gmask, int lane) {
+  // the lanes [goff, goff + width) of V.  Returns the symbol, or -1 on error.
+  // One hand-scheduled asm statement: table compare, ballot, popcount, lane
+  // read, state update and (when the window holds the bytes) the
+  // renormalisation; a renormalisation that reaches the window's base or the
+  // stream's start is completed by advance().
+  template <uint32_t kGoff, uint32_t kWidth>
+  __device__ __forceinline__ int decode(uint32_t V, int lane) {
+    static_assert(kGoff + kWidth <= 64 && kWidth <= 32, "lane group");
+    int32_t st, sym;
+    uint32_t xn, slot, q, t1, t2, p;
+    uint64_t b;
+    asm volatile(
+        "s_and_b32 %[slot], %[x0], 0xfff\n"
+        "s_lshl_b32 %[t1], %[slot], 13\n"
+        "s_or_b32 %[t1], %[t1], 0x1fff\n"
+        "v_cmp_ge_u32_e64 %[b], %[t1], %[V]\n"
+        "s_lshr_b32 %[q], %[x0], 12\n"
+        "s_bfe_u64 %[b], %[b], %[bfe]\n"
+        "s_bcnt1_i32_b64 %[sym], %[b]\n"
+        "s_cmp_eq_u32 %[sym], 0\n"
+        "s_cbranch_scc1 L%=_unused\n"
+        "s_add_u32 %[t1], %[sym], %[gm1]\n"
+        "v_readlane_b32 %[p], %[V], %[t1]\n"
+        "s_add_u32 %[sym], %[sym], -1\n"
+        "s_and_b32 %[t2], %[p], 0x1fff\n"
+        "s_lshr_b32 %[p], %[p], 13\n"
+        "s_mul_i32 %[xn], %[t2], %[q]\n"
+        "s_sub_u32 %[t2], %[slot], %[p]\n"
+        "s_add_u32 %[xn], %[xn], %[t2]\n"
+        EBIV_DECODE_TAIL
+        : [x0] "+s"(x0), [x1] "+s"(x1), [x2] "+s"(x2), [x3] "+s"(x3), [pos] "+s"(pos), [nsym] "+s"(nsym),
+          [st] "=&s"(st), [sym] "=&s"(sym), [xn] "=&s"(xn), [slot] "=&s"(slot), [q] "=&s"(q), [t1] "=&s"(t1),
+          [t2] "=&s"(t2), [p] "=&s"(p), [b] "=&s"(b)
+        : [V] "v"(V), [bfe] "i"(kGoff | (kWidth << 16)), [gm1] "i"(kGoff - 1), [lim] "s"(win.lim),
+          [wbase] "s"(win.wbase), [vwin] "v"(win.cur)
+        : "scc");
+    if (__builtin_expect(st != 0, 0)) {
+      if (st == 1) {
+        err = kErrCorrupt | (kTileUnused << 8);
+        return -1;
+      }
+      if (!advance(xn, lane)) return -1;
+    }
+    return sym;
+  }
+  // decode() with a lane group known only at run time (the fallback token
+  // loop).  Returns the symbol, or -1 on error.
+  __device__ __forceinline__ int decode_rt(uint32_t V, uint32_t goff, uint32_t gmask, int lane) {
     const uint32_t x = x0;
     const uint32_t slot = x & (kRansM - 1);
     const uint64_t b = __ballot(V <= ((slot << 13) | 0x1FFFu));
@@ -201,15 +286,29 @@ struct Rans {
   }
   // The bypass context, fixed at {2048, 2048} (rans.go:312-322).
   __device__ __forceinline__ int bit(int lane) {
-    const uint32_t x = x0;
-    const uint32_t slot = x & (kRansM - 1);
-    if (!advance(((x >> kRansScaleBits) << 11) + (slot & 2047u), lane)) return -1;
-    return static_cast<int>(slot >> 11);
+    int32_t st, sym;
+    uint32_t xn, t1, t2;
+    asm volatile(
+        "s_lshr_b32 %[xn], %[x0], 12\n"
+        "s_lshl_b32 %[xn], %[xn], 11\n"
+        "s_and_b32 %[t1], %[x0], 0x7ff\n"
+        "s_bfe_u32 %[sym], %[x0], 0x1000b\n"
+        "s_add_u32 %[xn], %[xn], %[t1]\n"
+        EBIV_DECODE_TAIL
+        : [x0] "+s"(x0), [x1] "+s"(x1), [x2] "+s"(x2), [x3] "+s"(x3), [pos] "+s"(pos), [nsym] "+s"(nsym),
+          [st] "=&s"(st), [sym] "=&s"(sym), [xn] "=&s"(xn), [t1] "=&s"(t1), [t2] "=&s"(t2)
+        : [lim] "s"(win.lim), [wbase] "s"(win.wbase), [vwin] "v"(win.cur)
+        : "scc");
+    if (__builtin_expect(st != 0, 0)) {
+      if (!advance(xn, lane)) return -1;
+    }
+    return sym;
   }
   // getClassValue (mv.go:95-108): a class symbol, then class-1 raw bits under
   // an implicit leading one.  Returns -1 on error.
-  __device__ __forceinline__ int class_value(uint32_t V, uint32_t goff, int lane) {
-    const int v = decode(V, goff, 0xFFFFu, lane);
+  template <uint32_t kGoff>
+  __device__ __forceinline__ int class_value(uint32_t V, int lane) {
+    const int v = decode<kGoff, 16>(V, lane);
     if (v <= 1) return v;
     int val = 1;
     for (int i = 1; i < v; i++) {
@@ -263,7 +362,7 @@ __device__ __forceinline__ bool parse_block(Rans& R, CoefOut& out, const Tables&
   int si = 0, band = 0, bnext = 1, first = -1, last = -1, first_pos = 0;
   int32_t sum_abs = 0, first_mag = 0;
   for (;;) {
-    const int t = R.decode(vcur, boff + prevoff, 0xFFu, lane);
+    const int t = R.decode_rt(vcur, boff + prevoff, 0xFFu, lane);
     if (t < 0) return false;
     if (t == kTEOB) break;
     if (t == kTZero) {
@@ -272,7 +371,7 @@ __device__ __forceinline__ bool parse_block(Rans& R, CoefOut& out, const Tables&
       int32_t mag = t;
       prevoff = t == kTOne ? 8u : 16u;  // classForMag: 1 -> 1, else 2 (model.go:181-191)
       if (t == kTEscape) {
-        const int e = R.class_value(T.misc2, kGEsc, lane);
+        const int e = R.class_value<kGEsc>(T.misc2, lane);
         if (e < 0) return false;
         mag = 5 + e;
       }
@@ -282,7 +381,7 @@ __device__ __forceinline__ bool parse_block(Rans& R, CoefOut& out, const Tables&
         first_mag = mag;
         first_pos = zz_pos(zzv, si);
       } else {
-        const int s = R.decode(T.misc0, kGSign, 0xFFFFu, lane);
+        const int s = R.decode<kGSign, 16>(T.misc0, lane);
         if (s < 0) return false;
         out.emit(coef_entry(zz_pos(zzv, si), blk, s ? -mag : mag), lane);
       }
@@ -300,7 +399,7 @@ __device__ __forceinline__ bool parse_block(Rans& R, CoefOut& out, const Tables&
   if (first >= 0) {
     bool neg = (sum_abs & 1) != 0;
     if (last - first < kSDHMinSpan) {
-      const int s = R.decode(T.misc0, kGSign, 0xFFFFu, lane);
+      const int s = R.decode<kGSign, 16>(T.misc0, lane);
       if (s < 0) return false;
       neg = s != 0;
     }
@@ -593,7 +692,7 @@ __device__ __forceinline__ bool parse_block_fast(Rans& R, CoefOut& out, const Ta
     if (ev == kTokRefill) continue;
     R.x0 = S.x0; R.x1 = S.x1; R.x2 = S.x2; R.x3 = S.x3; R.pos = S.pos; R.nsym = S.nsym;
     if (ev != kTokEscape) break;
-    const int e = R.class_value(T.misc2, kGEsc, lane);  // getEscapeValue (mv.go:124-128)
+    const int e = R.class_value<kGEsc>(T.misc2, lane);  // getEscapeValue (mv.go:124-128)
     if (e < 0) break;
     resume = 5 + static_cast<uint32_t>(e);
     S.x0 = R.x0; S.x1 = R.x1; S.x2 = R.x2; S.x3 = R.x3; S.pos = R.pos; S.nsym = R.nsym;
@@ -607,7 +706,7 @@ __device__ __forceinline__ bool parse_block_fast(Rans& R, CoefOut& out, const Ta
     const int32_t first_mag = static_cast<int32_t>(S.firstpm >> 8);
     bool neg = (S.sumabs & 1u) != 0;
     if (static_cast<int>(S.last) - first < kSDHMinSpan) {
-      const int sg = R.decode(T.misc0, kGSign, 0xFFFFu, lane);
+      const int sg = R.decode<kGSign, 16>(T.misc0, lane);
       if (sg < 0) return false;
       neg = sg != 0;
     }
@@ -641,11 +740,11 @@ __device__ __forceinline__ bool parse_mb(Rans& R, CoefOut& out, MBState& m, cons
                                          int lane) {
   bool luma_coded = true, cb_coded = true, cr_coded = true;
   if (!intra) {
-    const int refsel = R.decode(T.misc1, kGRef, 0xFFFFu, lane);
+    const int refsel = R.decode<kGRef, 16>(T.misc1, lane);
     if (refsel < 0) return false;
     m.refsel = static_cast<uint32_t>(refsel);
     const uint32_t pred = refsel == kRefGolden ? 0u : pred_last;
-    const int part = R.decode(T.misc1, kGPart, 0xFFFFu, lane);
+    const int part = R.decode<kGPart, 16>(T.misc1, lane);
     if (part < 0) return false;
     m.part = static_cast<uint32_t>(part);
     const int npart = static_cast<int>((kPartCountNibbles >> (4 * part)) & 15u);
@@ -653,7 +752,7 @@ __device__ __forceinline__ bool parse_mb(Rans& R, CoefOut& out, MBState& m, cons
       const uint32_t base = pi > 0 ? m.mv0 : pred;
       uint32_t mv = 0;
       for (int comp = 0; comp < 2; comp++) {
-        const int v = R.class_value(T.misc2, comp ? kGMVY : kGMVX, lane);
+        const int v = comp ? R.class_value<kGMVY>(T.misc2, lane) : R.class_value<kGMVX>(T.misc2, lane);
         if (v < 0) return false;
         mv |= static_cast<uint32_t>(static_cast<uint16_t>(clamp_mv(mv_comp(base, comp) + zigzag_dec(static_cast<uint32_t>(v)))))
               << (16 * comp);
@@ -664,7 +763,7 @@ __device__ __forceinline__ bool parse_mb(Rans& R, CoefOut& out, MBState& m, cons
       else m.mv3 = mv;
     }
     m.mvcur = refsel == kRefGolden ? 0u : m.mv0;
-    const int cbp = R.decode(T.misc0, kGCBP, 0xFFFFu, lane);
+    const int cbp = R.decode<kGCBP, 16>(T.misc0, lane);
     if (cbp < 0) return false;
     m.cbp = static_cast<uint32_t>(cbp);
     luma_coded = (cbp & kCBPLuma) != 0;
@@ -675,7 +774,7 @@ __device__ __forceinline__ bool parse_mb(Rans& R, CoefOut& out, MBState& m, cons
   int nb = 0, total = 0, tx = 0;
   uint32_t zzv = zz4;
   if (luma_coded) {
-    tx = R.decode(T.misc0, kGTx, 0xFFFFu, lane);
+    tx = R.decode<kGTx, 16>(T.misc0, lane);
     if (tx < 0) return false;
     m.tx = static_cast<uint32_t>(tx);
     const int n = 4 << tx;
@@ -699,7 +798,7 @@ __device__ __forceinline__ bool parse_mb(Rans& R, CoefOut& out, MBState& m, cons
     const int blk = chroma ? 16 + (k - nb) : k;
     if (k == nb) {  // luma complete
       if (intra) {
-        const int cmode = R.decode(T.misc1, kGChromaMode, 0xFFFFu, lane);
+        const int cmode = R.decode<kGChromaMode, 16>(T.misc1, lane);
         if (cmode < 0) return false;
         m.cmode = static_cast<uint32_t>(cmode);
       } else {
@@ -709,7 +808,7 @@ __device__ __forceinline__ bool parse_mb(Rans& R, CoefOut& out, MBState& m, cons
     }
     if (chroma && !(blk == 16 ? cb_coded : cr_coded)) continue;
     if (intra && !chroma) {
-      const int mode = R.decode(T.misc1, kGLumaMode, 0xFFFFu, lane);
+      const int mode = R.decode<kGLumaMode, 16>(T.misc1, lane);
       if (mode < 0) return false;
       if (k < 8) m.lmode0 |= static_cast<uint32_t>(mode) << (4 * k);
       else m.lmode1 |= static_cast<uint32_t>(mode) << (4 * (k - 8));
@@ -869,7 +968,7 @@ __device__ __forceinline__ void parse_tile(const uint8_t* __restrict__ pkt, cons
       bool coded = true;
       if (inter_frame) {
         // A failed mode decode reads as skip, which Go then reconstructs.
-        const int mode = R.decode(T.misc0, kGMBMode, 0xFFFFu, lane);
+        const int mode = R.decode<kGMBMode, 16>(T.misc0, lane);
         if (mode < 0 || mode == kMBSkip) {
           const uint32_t pred = predict_mv(prow, crow, mbx - td.x0, tw, mby == td.y0);
           m.kind = 0; m.mv0 = pred; m.mvcur = pred; m.wmask = 0x3FFFFu;
