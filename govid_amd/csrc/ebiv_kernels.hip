@@ -1472,8 +1472,24 @@ __device__ void prep_angular(const int32_t* ref, int32_t* ext, int32_t* dcout, i
   }
 }
 
+// Edge samples of the macroblocks reconstructed so far in a tile whose
+// macroblocks are all intra (a key frame): per MB column the bottom row of
+// its latest macroblock, per MB row the right column of its latest
+// macroblock, and per column the above-left corner of the next macroblock
+// in rows of either parity.  On the anti-diagonal schedule a macroblock's
+// neighbours (left, above-left, above, above-right) are exactly these
+// entries when it runs, and no macroblock of the same step writes an entry
+// another one of that step reads.
+constexpr int kFrontMax = 128;  // tile width and height (MBs) the LDS frontier covers
+struct Frontier {
+  uint8_t bot[kFrontMax][32];        // Y[16], Cb[8], Cr[8] of the bottom row
+  uint8_t right[kFrontMax][32];      // Y[16], Cb[8], Cr[8] of the right column
+  uint8_t corner[2][kFrontMax + 1][4];  // [row & 1][column]: Y, Cb, Cr
+};
+
 __device__ void intra_mb(IntraScratch& S, const LdsI32* dct, const uint32_t* __restrict__ coefs, const MBRec& r,
-                         const FrameHdr* hdr, const TileDesc& td, Planes rec, int mbx, int mby, int lane) {
+                         const FrameHdr* hdr, const TileDesc& td, Planes rec, int mbx, int mby, Frontier* fr,
+                         int lane) {
   const int n = 4 << (r.tx > 2 ? 2 : r.tx);
   WaveScratch& ws = S.ws;
   if (r.coef_count > 0) {
@@ -1481,8 +1497,25 @@ __device__ void intra_mb(IntraScratch& S, const LdsI32* dct, const uint32_t* __r
   } else {
     for (int i = lane; i < 384; i += 64) ws.coef[i] = 0;
   }
-  // Load the context: luma row above (x from -1 to 31) and left column, chroma likewise.
+  // Load the context: luma row above (x from -1 to 31) and left column,
+  // chroma likewise; from the frontier in a key frame, else from the frame
+  // (its inter macroblocks are final, its intra ones older in this launch).
   const int X = mbx * 16, Y = mby * 16;
+  if (fr) {
+    const int rx = mbx - td.x0, ry = mby - td.y0, tw = td.x1 - td.x0;
+    if (lane < 33) {
+      uint8_t v = 0;
+      if (lane == 0) v = fr->corner[ry & 1][rx][0];
+      else if (lane <= 16) v = fr->bot[rx][lane - 1];
+      else if (rx + 1 < tw) v = fr->bot[rx + 1][lane - 17];
+      S.L[0][lane] = v;
+    } else if (lane < 49) {
+      S.L[lane - 32][0] = fr->right[ry][lane - 33];
+    }
+    const int pl = lane >> 5, l = lane & 31;
+    if (l < 9) S.C[pl][0][l] = l == 0 ? fr->corner[ry & 1][rx][1 + pl] : fr->bot[rx][16 + 8 * pl + l - 1];
+    else if (l < 17) S.C[pl][l - 8][0] = fr->right[ry][16 + 8 * pl + l - 9];
+  } else {
   for (int i = lane; i < 33; i += 64) {
     int x = X - 1 + i, y = Y - 1;
     S.L[0][i] = (x >= 0 && x < rec.pw && y >= 0) ? rec.y[static_cast<size_t>(y) * rec.pw + x] : 0;
@@ -1502,6 +1535,7 @@ __device__ void intra_mb(IntraScratch& S, const LdsI32* dct, const uint32_t* __r
       int rr = l - 9, x = CX - 1, y = CY + rr;
       S.C[pl][1 + rr][0] = (x >= 0) ? cp[static_cast<size_t>(y) * rec.cw + x] : 0;
     }
+  }
   }
   wave_sync();
 
@@ -1567,12 +1601,28 @@ __device__ void intra_mb(IntraScratch& S, const LdsI32* dct, const uint32_t* __r
         int32_t pred = intra_pixel(S.ref[pl], S.ext[pl], S.dc[pl], 8, cmode, y, x0 + k);
         int32_t v = clamp255(pred + ws.coef[256 + pl * 64 + y * 8 + x0 + k]);
         half |= static_cast<uint32_t>(v) << (8 * k);
+        S.C[pl][y + 1][x0 + k + 1] = static_cast<uint8_t>(v);
       }
       uint8_t* dp = pl ? rec.cr : rec.cb;
       *reinterpret_cast<uint16_t*>(dp + static_cast<size_t>(mby * 8 + y) * rec.cw + mbx * 8 + x0) = static_cast<uint16_t>(half);
     }
   }
   wave_sync();
+  if (fr) {  // this macroblock's edges for its right and lower neighbours
+    const int rx = mbx - td.x0, ry = mby - td.y0;
+    if (lane < 16) {
+      fr->bot[rx][lane] = S.L[16][lane + 1];
+      fr->right[ry][lane] = S.L[lane + 1][16];
+    } else if (lane < 32) {
+      const int pl = (lane - 16) >> 3, k = (lane - 16) & 7;
+      fr->bot[rx][16 + 8 * pl + k] = S.C[pl][8][k + 1];
+      fr->right[ry][16 + 8 * pl + k] = S.C[pl][k + 1][8];
+    } else if (lane < 35) {
+      const int c = lane - 32;
+      fr->corner[(ry + 1) & 1][rx + 1][c] = c == 0 ? S.L[16][16] : S.C[c - 1][8][8];
+    }
+    wave_sync();
+  }
 }
 
 // One workgroup per tile.  Macroblock (x, y) depends on (x-1, y), (x-1, y-1),
@@ -1588,6 +1638,7 @@ __device__ __forceinline__ void recon_intra_tile(const uint8_t* __restrict__ des
   __shared__ IntraScratch scratch[kIntraWaves];
   __shared__ uint32_t step_mask[kMaxStepWords];
   __shared__ int32_t s_dct[768];
+  __shared__ Frontier front;
   LdsI32* dct = (LdsI32*)s_dct;
   const FrameHdr* hdr = reinterpret_cast<const FrameHdr*>(desc);
   const TileDesc* tiles = reinterpret_cast<const TileDesc*>(desc + desc_tiles_off());
@@ -1598,7 +1649,9 @@ __device__ __forceinline__ void recon_intra_tile(const uint8_t* __restrict__ des
   const int tw = td.x1 - td.x0, th = td.y1 - td.y0;
   const int steps = (tw - 1) + 2 * (th - 1) + 1;
   const int mbc = hdr->mbc;
-  const bool tracked = steps <= 32 * kMaxStepWords;
+  // A key frame's tile (every macroblock intra) keeps its context in LDS.
+  const bool keyf = hdr->inter == 0 && tw <= kFrontMax && th <= kFrontMax;
+  const bool tracked = !keyf && steps <= 32 * kMaxStepWords;
   if (threadIdx.x < kMaxStepWords) step_mask[threadIdx.x] = 0;
   load_dct(dct);
   if (tracked) {
@@ -1623,15 +1676,17 @@ __device__ __forceinline__ void recon_intra_tile(const uint8_t* __restrict__ des
       int mbx = td.x0 + rx, mby = td.y0 + ry;
       const MBRec r = recs[mby * mbc + mbx];
       if (r.kind != 2) continue;
-      intra_mb(scratch[wave], dct, coefs, r, hdr, td, rec, mbx, mby, lane);
+      intra_mb(scratch[wave], dct, coefs, r, hdr, td, rec, mbx, mby, keyf ? &front : nullptr, lane);
     }
     // Macroblocks of the next step read what other wavefronts of this
-    // workgroup stored in this one.  The barrier orders the stores, but the
-    // CU's vector L1 may still hold lines fetched before them (another
-    // wavefront's neighbour loads touch the same 128-byte lines), so the
-    // acquire at agent scope invalidates L1 before the next step's loads.
+    // workgroup stored in this one.  In a key frame that is the LDS frontier
+    // and the barrier suffices.  Otherwise they read the frame: the barrier
+    // orders the stores, but the CU's vector L1 may still hold lines fetched
+    // before them (another wavefront's neighbour loads touch the same
+    // 128-byte lines), so the acquire at agent scope invalidates L1 before
+    // the next step's loads.
     __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (!keyf) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
 }
 
