@@ -20,7 +20,13 @@ EXPORTS = (
     "ebiv_batch_create", "ebiv_batch_destroy", "ebiv_batch_last_error", "ebiv_batch_open", "ebiv_batch_close",
     "ebiv_batch_submit", "ebiv_batch_flush", "ebiv_batch_pending", "ebiv_batch_step", "ebiv_batch_stats",
     "ebiv_batch_set_timing", "ebiv_batch_kernel_times", "ebiv_batch_frame_rgba", "ebiv_session_bytes",
+    "ebiv_batch_set_schedule", "ebiv_batch_get_schedule",
 )
+
+
+class Schedule(C.Structure):
+    _fields_ = [("intra_waves_key", C.c_int32), ("intra_waves_few", C.c_int32),
+                ("intra_waves_many", C.c_int32), ("many_tiles", C.c_int32)]
 
 
 class Frame(C.Structure):
@@ -104,6 +110,10 @@ def bind(L):
     L.ebiv_batch_set_timing.argtypes = [vp, C.c_int]
     L.ebiv_batch_kernel_times.restype = C.c_int
     L.ebiv_batch_kernel_times.argtypes = [vp, C.POINTER(KernelTimes)]
+    L.ebiv_batch_set_schedule.restype = C.c_int
+    L.ebiv_batch_set_schedule.argtypes = [vp, C.POINTER(Schedule)]
+    L.ebiv_batch_get_schedule.restype = C.c_int
+    L.ebiv_batch_get_schedule.argtypes = [vp, C.POINTER(Schedule)]
     L.ebiv_batch_frame_rgba.restype = C.c_int
     L.ebiv_batch_frame_rgba.argtypes = [vp, C.POINTER(Frame), vp, C.c_int]
     L.ebiv_session_bytes.restype = sz

@@ -329,6 +329,18 @@ class Batch:
         self._L.ebiv_batch_kernel_times(self._h, C.byref(t))
         return {f: getattr(t, f) for f, _ in capi.KernelTimes._fields_}
 
+    def set_schedule(self, **kw) -> None:
+        """Launch shapes of the intra reconstruction (fields of capi.Schedule;
+        omitted ones keep their value).  Results do not depend on them."""
+        s = capi.Schedule(**{k: int(v) for k, v in kw.items()})
+        if self._L.ebiv_batch_set_schedule(self._h, C.byref(s)):
+            raise ValueError("bad schedule")
+
+    def schedule(self) -> dict:
+        s = capi.Schedule()
+        self._L.ebiv_batch_get_schedule(self._h, C.byref(s))
+        return {f: getattr(s, f) for f, _ in capi.Schedule._fields_}
+
     def _rgba(self, raw, dst, on_device):
         return self._L.ebiv_batch_frame_rgba(self._h, C.byref(raw), dst, on_device)
 

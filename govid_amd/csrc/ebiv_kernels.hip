@@ -1630,19 +1630,24 @@ __device__ void intra_mb(IntraScratch& S, const LdsI32* dct, const uint32_t* __r
 // macroblocks at once.  Only intra macroblocks are reconstructed here (the
 // others were written by the inter kernel before), so a first pass over the
 // tile's records marks the steps that hold one and the others are skipped.
-constexpr int kIntraWaves = 16;
+// The workgroup has blockDim.x / 64 wavefronts (up to kIntraMaxWaves); its
+// dynamic LDS holds one IntraScratch per wavefront and, when the launch
+// carries key frames (with_front), the Frontier in front of them.
+constexpr int kIntraMaxWaves = 16;
 constexpr int kMaxStepWords = 16;  // steps tracked individually (512)
 __device__ __forceinline__ void recon_intra_tile(const uint8_t* __restrict__ desc, const MBRec* __restrict__ recs,
                                                  const uint32_t* __restrict__ coefs,
-                                                 const TileStatus* __restrict__ status, const Planes rec, const int t) {
-  __shared__ IntraScratch scratch[kIntraWaves];
+                                                 const TileStatus* __restrict__ status, const Planes rec, const int t,
+                                                 const bool with_front) {
+  extern __shared__ __attribute__((aligned(16))) uint8_t intra_dyn[];
   __shared__ uint32_t step_mask[kMaxStepWords];
   __shared__ int32_t s_dct[768];
-  __shared__ Frontier front;
+  Frontier& front = *reinterpret_cast<Frontier*>(intra_dyn);
+  IntraScratch* scratch = reinterpret_cast<IntraScratch*>(intra_dyn + (with_front ? sizeof(Frontier) : 0));
   LdsI32* dct = (LdsI32*)s_dct;
   const FrameHdr* hdr = reinterpret_cast<const FrameHdr*>(desc);
   const TileDesc* tiles = reinterpret_cast<const TileDesc*>(desc + desc_tiles_off());
-  const int lane = lane_id(), wave = threadIdx.x >> 6;
+  const int lane = lane_id(), wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
   const TileDesc td = tiles[t];
   const TileStatus st = status[t];
   if (st.n_intra == 0) return;
@@ -1650,7 +1655,7 @@ __device__ __forceinline__ void recon_intra_tile(const uint8_t* __restrict__ des
   const int steps = (tw - 1) + 2 * (th - 1) + 1;
   const int mbc = hdr->mbc;
   // A key frame's tile (every macroblock intra) keeps its context in LDS.
-  const bool keyf = hdr->inter == 0 && tw <= kFrontMax && th <= kFrontMax;
+  const bool keyf = with_front && hdr->inter == 0 && tw <= kFrontMax && th <= kFrontMax;
   const bool tracked = !keyf && steps <= 32 * kMaxStepWords;
   if (threadIdx.x < kMaxStepWords) step_mask[threadIdx.x] = 0;
   load_dct(dct);
@@ -1669,7 +1674,7 @@ __device__ __forceinline__ void recon_intra_tile(const uint8_t* __restrict__ des
     // Macroblocks on this step: rows ry with rx = s - 2*ry in [0, tw).
     int ry_lo = s - (tw - 1) > 0 ? (s - (tw - 1) + 1) / 2 : 0;
     int ry_hi = s / 2 < th - 1 ? s / 2 : th - 1;
-    for (int ry = ry_lo + wave; ry <= ry_hi; ry += kIntraWaves) {
+    for (int ry = ry_lo + wave; ry <= ry_hi; ry += nwaves) {
       int rx = s - 2 * ry;
       uint32_t local = static_cast<uint32_t>(ry * tw + rx);
       if (local >= st.mbs_done) continue;
@@ -1690,11 +1695,13 @@ __device__ __forceinline__ void recon_intra_tile(const uint8_t* __restrict__ des
   }
 }
 
-__global__ __launch_bounds__(64 * kIntraWaves) void ebiv_recon_intra(const FrameJob* __restrict__ jobs,
-                                                        const WorkItem* __restrict__ items) {
+__global__ __launch_bounds__(64 * kIntraMaxWaves) void ebiv_recon_intra(const FrameJob* __restrict__ jobs,
+                                                                          const WorkItem* __restrict__ items,
+                                                                          int with_front) {
   const WorkItem it = item_of(items);
   ConstJob& job = job_of(jobs, it.job);
-  recon_intra_tile(job.desc, job.recs, job.coefs, job.status, planes_of(job.rec), static_cast<int>(it.idx));
+  recon_intra_tile(job.desc, job.recs, job.coefs, job.status, planes_of(job.rec), static_cast<int>(it.idx),
+                   with_front != 0);
 }
 
 // ============================================================================
@@ -1758,8 +1765,12 @@ void launch_recon_inter(hipStream_t s, int nitems, const FrameJob* jobs, const W
   hipLaunchKernelGGL(ebiv_recon_inter, dim3(nitems), dim3(256), 0, s, jobs, items);
 }
 
-void launch_recon_intra(hipStream_t s, int nitems, const FrameJob* jobs, const WorkItem* items) {
-  hipLaunchKernelGGL(ebiv_recon_intra, dim3(nitems), dim3(64 * kIntraWaves), 0, s, jobs, items);
+// waves: wavefronts per tile (1-16); key: the items include key-frame tiles,
+// which then use the LDS frontier.
+void launch_recon_intra(hipStream_t s, int nitems, const FrameJob* jobs, const WorkItem* items, int waves, bool key) {
+  waves = waves < 1 ? 1 : (waves > kIntraMaxWaves ? kIntraMaxWaves : waves);
+  const size_t lds = static_cast<size_t>(waves) * sizeof(IntraScratch) + (key ? sizeof(Frontier) : 0);
+  hipLaunchKernelGGL(ebiv_recon_intra, dim3(nitems), dim3(64 * waves), lds, s, jobs, items, key ? 1 : 0);
 }
 
 void launch_rgba(hipStream_t s, const uint8_t* y, int ys, const uint8_t* cb, const uint8_t* cr, int cs, int w, int h,

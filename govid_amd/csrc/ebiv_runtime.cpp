@@ -46,7 +46,7 @@ namespace ebiv {
 hipError_t upload_tables(const int32_t dct[3][256], const uint8_t zz[3][256]);
 void launch_parse(hipStream_t s, int nitems, int max_tile_w, const FrameJob* jobs, const WorkItem* items);
 void launch_recon_inter(hipStream_t s, int nitems, const FrameJob* jobs, const WorkItem* items);
-void launch_recon_intra(hipStream_t s, int nitems, const FrameJob* jobs, const WorkItem* items);
+void launch_recon_intra(hipStream_t s, int nitems, const FrameJob* jobs, const WorkItem* items, int waves, bool key);
 void launch_rgba(hipStream_t s, const uint8_t* y, int ys, const uint8_t* cb, const uint8_t* cr, int cs, int w, int h,
                  uint32_t* out);
 
@@ -586,8 +586,20 @@ bool assign_buffers(const Session& s, std::vector<StepFrame>& fr, const std::vec
 
 }  // namespace
 
+// Launch shapes of the reconstruction kernels (see ebiv_batch_set_schedule).
+struct Schedule {
+  int intra_waves_key = 16;   // wavefronts per key-frame tile
+  int intra_waves_few = 16;   // wavefronts per inter-frame tile, launches of few tiles
+  int intra_waves_many = 4;   // ... launches of at least many_tiles tiles
+  int many_tiles = 256;
+  int inter_intra_waves(size_t ntiles) const {
+    return ntiles >= static_cast<size_t>(many_tiles) ? intra_waves_many : intra_waves_few;
+  }
+};
+
 struct Engine {
   int device = 0;
+  Schedule sched;
   hipStream_t stream = nullptr;
   std::vector<std::unique_ptr<Session>> sessions;
   int next_id = 1;
@@ -779,12 +791,20 @@ int Engine::run_step(std::vector<Session*>& ss, std::vector<std::vector<StepFram
   for (size_t j = 0; j < njobs; j++) fill_job(h_jobs.h[j], jrefs[j]);
   size_t w = 0;
   for (size_t j = 0; j < njobs; j++) add_recon_items(j, false, w);
-  std::vector<size_t> inter_off(rounds.size()), intra_off(rounds.size()), round_end(rounds.size());
+  // Per round: inter macroblock groups, then the intra tiles of key-frame
+  // payloads (their own launch: many wavefronts per tile and the LDS
+  // frontier, for latency), then those of inter payloads (few wavefronts per
+  // tile, for throughput).
+  std::vector<size_t> inter_off(rounds.size()), intra_off(rounds.size()), key_end(rounds.size()),
+      round_end(rounds.size());
+  auto is_key = [&](size_t j) { return !payload_of(ss, frames, jrefs[j])->inter; };
   for (size_t r = 0; r < rounds.size(); r++) {
     inter_off[r] = w;
     for (size_t j : rounds[r]) add_recon_items(j, true, w);
     intra_off[r] = w;
-    for (size_t j : rounds[r]) add_recon_items(j, false, w);
+    for (size_t j : rounds[r]) if (is_key(j)) add_recon_items(j, false, w);
+    key_end[r] = w;
+    for (size_t j : rounds[r]) if (!is_key(j)) add_recon_items(j, false, w);
     round_end[r] = w;
   }
   if (njobs &&
@@ -828,9 +848,17 @@ int Engine::run_step(std::vector<Session*>& ss, std::vector<std::vector<StepFram
       launch_recon_inter(stream, static_cast<int>(intra_off[r] - inter_off[r]), d_jobs.d, d_items.d + inter_off[r]);
       mark_end(m);
     }
-    if (round_end[r] > intra_off[r]) {
+    if (key_end[r] > intra_off[r]) {
       const int m = mark_begin(2);
-      launch_recon_intra(stream, static_cast<int>(round_end[r] - intra_off[r]), d_jobs.d, d_items.d + intra_off[r]);
+      launch_recon_intra(stream, static_cast<int>(key_end[r] - intra_off[r]), d_jobs.d, d_items.d + intra_off[r],
+                         sched.intra_waves_key, true);
+      mark_end(m);
+    }
+    if (round_end[r] > key_end[r]) {
+      const size_t n = round_end[r] - key_end[r];
+      const int m = mark_begin(2);
+      launch_recon_intra(stream, static_cast<int>(n), d_jobs.d, d_items.d + key_end[r], sched.inter_intra_waves(n),
+                         false);
       mark_end(m);
     }
   }
@@ -892,7 +920,9 @@ int Engine::run_step(std::vector<Session*>& ss, std::vector<std::vector<StepFram
             hipMemcpyAsync(d_items.d, h_items.h, sizeof(WorkItem) * n, hipMemcpyHostToDevice, stream) != hipSuccess)
           return fail(EBIV_ERR_DEVICE, "job table upload failed");
         if (ninter) launch_recon_inter(stream, static_cast<int>(ninter), d_jobs.d, d_items.d);
-        launch_recon_intra(stream, ps->ntiles, d_jobs.d, d_items.d + ninter);
+        launch_recon_intra(stream, ps->ntiles, d_jobs.d, d_items.d + ninter,
+                           ps->inter ? sched.inter_intra_waves(static_cast<size_t>(ps->ntiles)) : sched.intra_waves_key,
+                           !ps->inter);
         if (hipGetLastError() != hipSuccess) return fail(EBIV_ERR_DEVICE, "kernel launch failed");
         // the pinned tables are rewritten by the next payload
         if (hipStreamSynchronize(stream) != hipSuccess) return fail(EBIV_ERR_DEVICE, "decode failed on the device");
@@ -1291,6 +1321,24 @@ int ebiv_batch_stats(const ebiv_batch* b, int session, ebiv_stats* out) {
       t.tiles += s->stats.tiles;
     }
   *out = t;
+  return EBIV_OK;
+}
+
+int ebiv_batch_set_schedule(ebiv_batch* b, const ebiv_schedule* sc) {
+  if (!b || !sc) return EBIV_ERR_ARGUMENT;
+  Schedule& d = b->eng.sched;
+  auto waves = [](int32_t v, int& dst) { if (v > 0) dst = v > 16 ? 16 : v; };
+  waves(sc->intra_waves_key, d.intra_waves_key);
+  waves(sc->intra_waves_few, d.intra_waves_few);
+  waves(sc->intra_waves_many, d.intra_waves_many);
+  if (sc->many_tiles > 0) d.many_tiles = sc->many_tiles;
+  return EBIV_OK;
+}
+
+int ebiv_batch_get_schedule(const ebiv_batch* b, ebiv_schedule* out) {
+  if (!b || !out) return EBIV_ERR_ARGUMENT;
+  const Schedule& d = b->eng.sched;
+  *out = ebiv_schedule{d.intra_waves_key, d.intra_waves_few, d.intra_waves_many, d.many_tiles};
   return EBIV_OK;
 }
 

@@ -30,12 +30,15 @@ def _oracle_outcomes(packets, ring, flush_at=()):
     return out
 
 
-def _run_batch(streams, max_out=4096, check_rgba=False):
+def _run_batch(streams, max_out=4096, check_rgba=False, schedule=None):
     """streams: list of (packets, ring, window).  Submits everything, steps
     until drained, and checks each completion against the oracle."""
     from govid_amd.codec import Batch
 
     b = Batch()
+    if schedule:
+        b.set_schedule(**schedule)
+        assert all(b.schedule()[k] == v for k, v in schedule.items())
     want, sids = [], []
     for k, (packets, ring, window) in enumerate(streams):
         sid = b.open(ring=ring, window=window)
@@ -80,6 +83,23 @@ def test_sessions_of_mixed_geometry_and_windows():
     ]
     print("mixed sessions", flush=True)
     _run_batch(streams, check_rgba=True)
+
+
+@pytest.mark.parametrize("schedule", [
+    dict(intra_waves_key=1, intra_waves_few=1, intra_waves_many=1),
+    dict(intra_waves_key=5, intra_waves_few=3, intra_waves_many=7, many_tiles=1),
+    dict(intra_waves_key=16, intra_waves_few=16, intra_waves_many=16, many_tiles=1000),
+])
+def test_launch_schedules_do_not_change_results(schedule):
+    """Intra reconstruction with 1 to 16 wavefronts per tile (key frames from
+    the LDS frontier, inter frames from the frame), bit for bit."""
+    streams = [
+        (_clip("rects", 96, 64, 8, qp=22, gop=4, tile_cols=2, tile_rows=2), 3, 4),
+        (_clip("parallax", 64, 48, 7, qp=28, gop=7, altref=3), 4, 4),
+        (_clip("synthframe", 80, 48, 4, qp=18, gop=2), 3, 4),
+    ]
+    print(f"schedule {schedule}", flush=True)
+    _run_batch(streams, schedule=schedule)
 
 
 def test_window_larger_than_stream_and_max_out_limit():
