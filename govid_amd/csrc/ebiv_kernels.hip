@@ -41,6 +41,18 @@ __device__ __forceinline__ void wave_sync() {
 __device__ __forceinline__ int32_t clamp255(int32_t v) { return v < 0 ? 0 : (v > 255 ? 255 : v); }
 __device__ __forceinline__ int sidx_of(int n) { return n == 4 ? 0 : (n == 8 ? 1 : 2); }
 __device__ __forceinline__ uint32_t rd_lane(uint32_t v, int l) { return __builtin_amdgcn_readlane(v, l); }
+// Uniform values for "s" operands of inline asm: the compiler may hold a
+// uniform value in a VGPR, and an "s" operand is not converted for it.
+__device__ __forceinline__ uint32_t sgpr(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+// (The builtin returns int: each half goes back through uint32_t so the low
+// half is zero-extended, not sign-extended, into the 64-bit address.)
+template <typename T>
+__device__ __forceinline__ T* sgpr(T* p) {
+  const uint64_t a = reinterpret_cast<uint64_t>(p);
+  const uint64_t lo = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<uint32_t>(a)));
+  const uint64_t hi = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<uint32_t>(a >> 32)));
+  return reinterpret_cast<T*>(lo | (hi << 32));
+}
 
 // ============================================================================
 // Entropy kernel
@@ -254,7 +266,7 @@ struct Rans {
         "s_sub_u32 %[t2], %[slot], %[p]\n"
         "s_add_u32 %[xn], %[xn], %[t2]\n"
         EBIV_DECODE_TAIL
-        : [x0] "+s"(x0), [x1] "+s"(x1), [x2] "+s"(x2), [x3] "+s"(x3), [pos] "+s"(pos), [nsym] "+s"(nsym),
+        : [x0] "+&s"(x0), [x1] "+&s"(x1), [x2] "+&s"(x2), [x3] "+&s"(x3), [pos] "+&s"(pos), [nsym] "+&s"(nsym),
           [st] "=&s"(st), [sym] "=&s"(sym), [xn] "=&s"(xn), [slot] "=&s"(slot), [q] "=&s"(q), [t1] "=&s"(t1),
           [t2] "=&s"(t2), [p] "=&s"(p), [b] "=&s"(b)
         : [V] "v"(V), [bfe] "i"(kGoff | (kWidth << 16)), [gm1] "i"(kGoff - 1), [lim] "s"(win.lim),
@@ -295,7 +307,7 @@ struct Rans {
         "s_bfe_u32 %[sym], %[x0], 0x1000b\n"
         "s_add_u32 %[xn], %[xn], %[t1]\n"
         EBIV_DECODE_TAIL
-        : [x0] "+s"(x0), [x1] "+s"(x1), [x2] "+s"(x2), [x3] "+s"(x3), [pos] "+s"(pos), [nsym] "+s"(nsym),
+        : [x0] "+&s"(x0), [x1] "+&s"(x1), [x2] "+&s"(x2), [x3] "+&s"(x3), [pos] "+&s"(pos), [nsym] "+&s"(nsym),
           [st] "=&s"(st), [sym] "=&s"(sym), [xn] "=&s"(xn), [t1] "=&s"(t1), [t2] "=&s"(t2)
         : [lim] "s"(win.lim), [wbase] "s"(win.wbase), [vwin] "v"(win.cur)
         : "scc");
@@ -647,14 +659,15 @@ __device__ __forceinline__ uint32_t token_loop(TokState& S, uint32_t resume, uin
       "L%=_exit:\n"
       "s_mov_b32 m0, %[m0s]\n"
       "s_nop 4\n"
-      : [x0] "+s"(S.x0), [x1] "+s"(S.x1), [x2] "+s"(S.x2), [x3] "+s"(S.x3), [pos] "+s"(S.pos), [nsym] "+s"(S.nsym),
-        [si] "+s"(S.si), [stop] "+s"(S.stop), [band] "+s"(S.band), [boff] "+s"(S.boff), [prev] "+s"(S.prev),
-        [sumabs] "+s"(S.sumabs), [firstsi] "+s"(S.firstsi), [firstpm] "+s"(S.firstpm), [last] "+s"(S.last),
-        [cn] "+s"(S.cn), [cbase] "+s"(S.cbase), [vcur] "+v"(S.vcur), [vbuf] "+v"(S.vbuf), [ev] "=&s"(ev),
+      : [x0] "+&s"(S.x0), [x1] "+&s"(S.x1), [x2] "+&s"(S.x2), [x3] "+&s"(S.x3), [pos] "+&s"(S.pos), [nsym] "+&s"(S.nsym),
+        [si] "+&s"(S.si), [stop] "+&s"(S.stop), [band] "+&s"(S.band), [boff] "+&s"(S.boff), [prev] "+&s"(S.prev),
+        [sumabs] "+&s"(S.sumabs), [firstsi] "+&s"(S.firstsi), [firstpm] "+&s"(S.firstpm), [last] "+&s"(S.last),
+        [cn] "+&s"(S.cn), [cbase] "+&s"(S.cbase), [vcur] "+&v"(S.vcur), [vbuf] "+&v"(S.vbuf), [ev] "=&s"(ev),
         [slot] "=&s"(slot), [cnt] "=&s"(cnt), [p] "=&s"(p), [xn] "=&s"(xn), [t1] "=&s"(t1), [t2] "=&s"(t2),
         [mag] "=&s"(mag), [neg] "=&s"(neg), [b] "=&s"(b), [vtmp] "=&v"(vtmp), [m0s] "=&s"(m0s)
-      : [resume] "s"(resume), [lim] "s"(lim), [wbase] "s"(wbase), [total] "s"(total), [blk8] "s"(blk8),
-        [f0] "s"(f0), [f1] "s"(f1), [dst] "s"(dst), [vb] "v"(vb), [vc] "v"(vc), [zzv] "v"(zzv), [vwin] "v"(vwin),
+      : [resume] "s"(sgpr(resume)), [lim] "s"(sgpr(lim)), [wbase] "s"(sgpr(wbase)), [total] "s"(sgpr(total)),
+        [blk8] "s"(sgpr(blk8)), [f0] "s"(sgpr(f0)), [f1] "s"(sgpr(f1)), [dst] "s"(sgpr(dst)), [vb] "v"(vb),
+        [vc] "v"(vc), [zzv] "v"(zzv), [vwin] "v"(vwin),
         [vlane4] "v"(vlane4)
       : "scc", "memory");
   return ev;
@@ -827,6 +840,706 @@ __device__ __forceinline__ bool parse_mb(Rans& R, CoefOut& out, MBState& m, cons
   return true;
 }
 
+// ============================================================================
+// Entropy kernel, hand-scheduled macroblock loop
+// ============================================================================
+// decodeTile (decode_intra.go:121-135) as one asm statement: macroblock
+// modes, MV prediction (mv.go:21-52), the inter header (decode_inter.go:18-62:
+// reference, partition, MV class values with raw bits, coded-block pattern),
+// transform size and intra modes (decode_intra.go:137-167), every token of
+// every block with escapes, explicit and hidden signs (residual.go:79-126),
+// coefficient entries, macroblock records and the MV rows.  All decoder state
+// is uniform and lives in SGPRs; vector registers hold the symbol tables, the
+// byte window, the scan tables, the MV rows (lane = tile column) and two
+// staging registers (64 coefficient entries; 5 records of 12 dwords).
+//
+// It returns at the end of the tile (kMbDone) or at a decode error
+// (kMbUnused, kMbUnderrun) with the failing macroblock's fields decoded so
+// far, which the caller turns into its record exactly as the generic path
+// does.  Preconditions: the tile is at most 64 macroblocks wide (MV rows in
+// one VGPR) and the sign context has a table (signs from two SGPRs).
+enum MbEvent : uint32_t { kMbDone = 0, kMbUnused = 3, kMbUnderrun = 4 };
+
+struct MbLoop {
+  // rANS, window, outputs (in/out)
+  uint32_t x0, x1, x2, x3, pos, nsym, wbase, lim, cn, cbase;
+  uint32_t mbx, mby, done, nintra, nst, firstmb, leftmv;
+  uint32_t vwin, vbuf, vstg, vprow, vcrow;  // per lane
+  // the failing macroblock (outputs)
+  uint32_t r0, r1, wmask, cstart, mv0, mv1, mv2, mv3, mvcur, lm0, lm1;
+};
+
+// Register roles that share one operand (never live together).
+#define MB_MAG "%[val]"
+#define MB_NEG "%[k2]"
+#define MB_MVT "%[sumabs]"
+#define MB_PI "%[si]"
+#define MB_NPART "%[stop]"
+#define MB_PRED "%[firstpm]"
+
+// Renormalisation after a symbol (xn = updated state), in line: branch out
+// when a byte is needed.  Out of line (MB_RN_OOL): the byte loop with
+// underrun check and window slide, then back to the rotation.
+#define MB_RN(tag)                               \
+  "s_cmp_lt_u32 %[xn], 0x800000\n"               \
+  "s_cbranch_scc1 L%=_" tag "_rn\n"              \
+  "L%=_" tag "_rot:\n"                           \
+  "s_mov_b32 %[x0], %[x1]\n"                     \
+  "s_mov_b32 %[x1], %[x2]\n"                     \
+  "s_mov_b32 %[x2], %[x3]\n"                     \
+  "s_mov_b32 %[x3], %[xn]\n"                     \
+  "s_add_u32 %[nsym], %[nsym], 1\n"
+#define MB_RN_OOL(tag, underrun)                 \
+  "L%=_" tag "_rn:\n"                            \
+  "s_cmp_eq_u32 %[pos], 0\n"                     \
+  "s_cbranch_scc1 L%=_" underrun "\n"            \
+  "s_sub_u32 %[pos], %[pos], 1\n"                \
+  "s_cmp_lt_i32 %[pos], %[wbase]\n"              \
+  "s_cbranch_scc1 L%=_" tag "_sl\n"              \
+  "L%=_" tag "_by:\n"                            \
+  "s_sub_u32 %[t1], %[pos], %[wbase]\n"          \
+  "s_lshr_b32 %[t2], %[t1], 2\n"                 \
+  "v_readlane_b32 %[t2], %[vwin], %[t2]\n"       \
+  "s_and_b32 %[t1], %[t1], 3\n"                  \
+  "s_lshl_b32 %[t1], %[t1], 3\n"                 \
+  "s_lshr_b32 %[t2], %[t2], %[t1]\n"             \
+  "s_and_b32 %[t2], %[t2], 0xff\n"               \
+  "s_lshl_b32 %[xn], %[xn], 8\n"                 \
+  "s_or_b32 %[xn], %[xn], %[t2]\n"               \
+  "s_cmp_lt_u32 %[xn], 0x800000\n"               \
+  "s_cbranch_scc1 L%=_" tag "_rn\n"              \
+  "s_branch L%=_" tag "_rot\n"                   \
+  "L%=_" tag "_sl:\n"                            \
+  "s_sub_u32 %[wbase], %[wbase], 256\n"          \
+  "s_max_i32 %[lim], %[wbase], 0\n"              \
+  "s_sub_u32 %[t1], %[wbase], %[wlo]\n"          \
+  "v_add_u32 %[vaddr], %[t1], %[vlane4]\n"       \
+  "v_max_i32 %[vt], %[vaddr], 0\n"               \
+  "v_min_i32 %[vt], %[hi2], %[vt]\n"             \
+  "global_load_dword %[vwin], %[vt], %[tb]\n"    \
+  "v_cmp_ge_u32_e32 vcc, %[hi2], %[vaddr]\n"     \
+  "s_waitcnt vmcnt(0)\n"                         \
+  "v_cndmask_b32_e32 %[vwin], 0, %[vwin], vcc\n" \
+  "s_branch L%=_" tag "_by\n"
+
+// Symbol of the context at lanes [goff, goff + 16) of vec into dst.
+#define MB_DEC(vec, goff, gm1, bfe, dst, tag, unused) \
+  "s_and_b32 %[slot], %[x0], 0xfff\n"            \
+  "s_lshl_b32 %[t1], %[slot], 13\n"              \
+  "s_or_b32 %[t1], %[t1], 0x1fff\n"              \
+  "v_cmp_ge_u32_e64 %[b], %[t1], " vec "\n"      \
+  "s_lshr_b32 %[xn], %[x0], 12\n"                \
+  "s_bfe_u64 %[b], %[b], " bfe "\n"              \
+  "s_bcnt1_i32_b64 " dst ", %[b]\n"              \
+  "s_cmp_eq_u32 " dst ", 0\n"                    \
+  "s_cbranch_scc1 L%=_" unused "\n"              \
+  "s_add_u32 %[t1], " dst ", " gm1 "\n"          \
+  "v_readlane_b32 %[p], " vec ", %[t1]\n"        \
+  "s_add_u32 " dst ", " dst ", -1\n"             \
+  "s_and_b32 %[t2], %[p], 0x1fff\n"              \
+  "s_lshr_b32 %[p], %[p], 13\n"                  \
+  "s_mul_i32 %[xn], %[t2], %[xn]\n"              \
+  "s_sub_u32 %[t2], %[slot], %[p]\n"             \
+  "s_add_u32 %[xn], %[xn], %[t2]\n"              \
+  MB_RN(tag)
+
+// A raw bit (the bypass context {2048, 2048}) into dst.
+#define MB_BIT(dst, tag)                         \
+  "s_lshr_b32 %[xn], %[x0], 12\n"                \
+  "s_lshl_b32 %[xn], %[xn], 11\n"                \
+  "s_and_b32 %[t1], %[x0], 0x7ff\n"              \
+  "s_bfe_u32 " dst ", %[x0], 0x1000b\n"          \
+  "s_add_u32 %[xn], %[xn], %[t1]\n"              \
+  MB_RN(tag)
+
+// getClassValue (mv.go:95-108) of the 16-symbol context at goff of vec into
+// %[sym]: class, then class-1 raw bits under a leading one.
+#define MB_CLASS(vec, goff, gm1, bfe, tag)       \
+  MB_DEC(vec, goff, gm1, bfe, "%[sym]", tag "d", "unused") \
+  "s_cmp_lt_u32 %[sym], 2\n"                     \
+  "s_cbranch_scc1 L%=_" tag "_cv\n"              \
+  "s_mov_b32 %[val], 1\n"                        \
+  "s_add_u32 %[k2], %[sym], -1\n"                \
+  "L%=_" tag "_bits:\n"                          \
+  MB_BIT("%[t3]", tag "b")                       \
+  "s_lshl_b32 %[val], %[val], 1\n"               \
+  "s_or_b32 %[val], %[val], %[t3]\n"             \
+  "s_add_u32 %[k2], %[k2], -1\n"                 \
+  "s_cmp_lg_u32 %[k2], 0\n"                      \
+  "s_cbranch_scc1 L%=_" tag "_bits\n"            \
+  "s_mov_b32 %[sym], %[val]\n"                   \
+  "L%=_" tag "_cv:\n"
+
+// Explicit sign from the sign context's two frequencies: mag <- -mag on 1.
+#define MB_SIGN(tag)                             \
+  "s_sub_u32 " MB_NEG ", 0, " MB_MAG "\n"        \
+  "s_and_b32 %[slot], %[x0], 0xfff\n"            \
+  "s_lshr_b32 %[xn], %[x0], 12\n"                \
+  "s_cmp_ge_u32 %[slot], %[f0]\n"                \
+  "s_cselect_b32 %[p], %[f0], 0\n"               \
+  "s_cselect_b32 %[t2], %[f1], %[f0]\n"          \
+  "s_cselect_b32 " MB_MAG ", " MB_NEG ", " MB_MAG "\n" \
+  "s_mul_i32 %[xn], %[t2], %[xn]\n"              \
+  "s_sub_u32 %[t2], %[slot], %[p]\n"             \
+  "s_add_u32 %[xn], %[xn], %[t2]\n"              \
+  MB_RN(tag)
+
+// Zigzag scan position of si into t2.
+#define MB_ZZ                                    \
+  "s_lshr_b32 %[t1], %[si], 2\n"                 \
+  "v_readlane_b32 %[t2], %[vzz], %[t1]\n"        \
+  "s_and_b32 %[t1], %[si], 3\n"                  \
+  "s_lshl_b32 %[t1], %[t1], 3\n"                 \
+  "s_lshr_b32 %[t2], %[t2], %[t1]\n"             \
+  "s_and_b32 %[t2], %[t2], 0xff\n"
+
+// Coefficient entry (position t2, level mag) into the staging register;
+// 64 staged entries go out in one store (out of line: MB_EMIT_OOL).
+#define MB_EMIT(tag)                             \
+  "s_lshl_b32 %[t1], " MB_MAG ", 13\n"           \
+  "s_or_b32 %[t1], %[t1], %[t2]\n"               \
+  "s_or_b32 %[t1], %[t1], %[blk8]\n"             \
+  "s_mov_b32 m0, %[cn]\n"                        \
+  "v_writelane_b32 %[vbuf], %[t1], m0\n"         \
+  "s_add_u32 %[cn], %[cn], 1\n"                  \
+  "s_cmp_eq_u32 %[cn], 64\n"                     \
+  "s_cbranch_scc1 L%=_" tag "_cf\n"              \
+  "L%=_" tag "_cfd:\n"
+#define MB_EMIT_OOL(tag)                         \
+  "L%=_" tag "_cf:\n"                            \
+  "s_lshl_b32 %[t1], %[cbase], 2\n"              \
+  "v_add_u32 %[vaddr], %[t1], %[vlane4]\n"       \
+  "global_store_dword %[vaddr], %[vbuf], %[dst]\n" \
+  "s_add_u32 %[cbase], %[cbase], 64\n"           \
+  "s_mov_b32 %[cn], 0\n"                         \
+  "s_branch L%=_" tag "_cfd\n"
+
+// Record dword k (value v) of the staged macroblock nst.
+#define MB_REC(k, v)                             \
+  "s_add_u32 m0, %[t2], " #k "\n"                \
+  "v_writelane_b32 %[vstg], " v ", m0\n"
+
+__device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t zz4, uint32_t zz8, uint32_t zz16,
+                                            uint32_t vlane4, uint32_t x0t, uint32_t x1t, uint32_t y0t, uint32_t y1t,
+                                            uint32_t mbc, uint32_t interf, uint32_t tcb, uint32_t f0, uint32_t f1,
+                                            uint32_t hi2, uint32_t wlo, const uint8_t* tb, uint32_t* dst,
+                                            MBRec* recs) {
+  uint32_t ev, slot, t1, t2, t3, p, xn, cnt, sym, val, k2;
+  uint32_t k, nb, total, blk8, cbpf, si, stop, boff, prev, sumabs, firstsi, firstpm, last, m0s;
+  uint64_t b;
+  uint32_t vcur, va, vb, vc, vzz, vaddr, vt;
+  asm volatile(
+      "s_nop 4\n"
+      "s_mov_b32 %[m0s], m0\n"
+
+      // ---------------------------------------------------------------- MB
+      "L%=_mb:\n"
+      "s_add_u32 %[cstart], %[cbase], %[cn]\n"
+      "s_mov_b32 %[r0], 2\n"
+      "s_mov_b32 %[r1], 0\n"
+      "s_mov_b32 %[wmask], 0\n"
+      "s_mov_b32 %[mv0], 0\n"
+      "s_mov_b32 %[mv1], 0\n"
+      "s_mov_b32 %[mv2], 0\n"
+      "s_mov_b32 %[mv3], 0\n"
+      "s_mov_b32 %[mvcur], 0\n"
+      "s_mov_b32 %[lm0], 0\n"
+      "s_mov_b32 %[lm1], 0\n"
+      "s_mov_b32 %[cbpf], 7\n"
+      "s_cmp_eq_u32 %[interf], 0\n"
+      "s_cbranch_scc1 L%=_tx\n"
+      // macroblock mode (decode_inter.go:9-16)
+      MB_DEC("%[vm0]", "16", "15", "0x100010", "%[sym]", "md", "mode_unused")
+      "s_cmp_eq_u32 %[sym], 0\n"
+      "s_cbranch_scc1 L%=_skip\n"
+      "s_cmp_eq_u32 %[sym], 2\n"
+      "s_cbranch_scc1 L%=_tx\n"
+      // ---- inter macroblock: reference, partition, vectors, pattern
+      "s_mov_b32 %[r0], 1\n"
+      MB_DEC("%[vm1]", "32", "31", "0x100020", "%[sym]", "rf", "unused")
+      "s_mov_b32 %[r1], %[sym]\n"
+      "s_mov_b32 " MB_PRED ", 0\n"
+      "s_cmp_eq_u32 %[sym], 1\n"
+      "s_cbranch_scc1 L%=_nopred\n"
+      "s_mov_b32 %[t3], 1\n"
+      "s_branch L%=_mvp\n"
+      "L%=_nopred:\n"
+      MB_DEC("%[vm1]", "48", "47", "0x100030", "%[sym]", "pt", "unused")
+      "s_lshl_b32 %[t1], %[sym], 8\n"
+      "s_or_b32 %[r0], %[r0], %[t1]\n"
+      "s_lshl_b32 %[t1], %[sym], 2\n"
+      "s_lshr_b32 " MB_NPART ", 0x4221, %[t1]\n"
+      "s_and_b32 " MB_NPART ", " MB_NPART ", 15\n"
+      "s_mov_b32 " MB_PI ", 0\n"
+      "L%=_mvl:\n"
+      "s_cmp_eq_u32 " MB_PI ", 0\n"
+      "s_cselect_b32 %[cnt], " MB_PRED ", %[mv0]\n"
+      // x component: base.x + zigzag(class value), clamped to +-4096
+      MB_CLASS("%[vm2]", "16", "15", "0x100010", "cx")
+      "s_sext_i32_i16 %[t1], %[cnt]\n"
+      "s_lshr_b32 %[t2], %[sym], 1\n"
+      "s_and_b32 %[sym], %[sym], 1\n"
+      "s_sub_u32 %[sym], 0, %[sym]\n"
+      "s_xor_b32 %[t2], %[t2], %[sym]\n"
+      "s_add_u32 %[t1], %[t1], %[t2]\n"
+      "s_max_i32 %[t1], %[t1], -4096\n"
+      "s_min_i32 %[t1], %[t1], 4096\n"
+      "s_and_b32 " MB_MVT ", %[t1], 0xffff\n"
+      MB_CLASS("%[vm2]", "32", "31", "0x100020", "cy")
+      "s_ashr_i32 %[t1], %[cnt], 16\n"
+      "s_lshr_b32 %[t2], %[sym], 1\n"
+      "s_and_b32 %[sym], %[sym], 1\n"
+      "s_sub_u32 %[sym], 0, %[sym]\n"
+      "s_xor_b32 %[t2], %[t2], %[sym]\n"
+      "s_add_u32 %[t1], %[t1], %[t2]\n"
+      "s_max_i32 %[t1], %[t1], -4096\n"
+      "s_min_i32 %[t1], %[t1], 4096\n"
+      "s_lshl_b32 %[t1], %[t1], 16\n"
+      "s_or_b32 " MB_MVT ", " MB_MVT ", %[t1]\n"
+      "s_cmp_eq_u32 " MB_PI ", 0\n"
+      "s_cselect_b32 %[mv0], " MB_MVT ", %[mv0]\n"
+      "s_cmp_eq_u32 " MB_PI ", 1\n"
+      "s_cselect_b32 %[mv1], " MB_MVT ", %[mv1]\n"
+      "s_cmp_eq_u32 " MB_PI ", 2\n"
+      "s_cselect_b32 %[mv2], " MB_MVT ", %[mv2]\n"
+      "s_cmp_eq_u32 " MB_PI ", 3\n"
+      "s_cselect_b32 %[mv3], " MB_MVT ", %[mv3]\n"
+      "s_add_u32 " MB_PI ", " MB_PI ", 1\n"
+      "s_cmp_lt_u32 " MB_PI ", " MB_NPART "\n"
+      "s_cbranch_scc1 L%=_mvl\n"
+      "s_cmp_eq_u32 %[r1], 1\n"
+      "s_cselect_b32 %[mvcur], 0, %[mv0]\n"
+      MB_DEC("%[vm0]", "48", "47", "0x100030", "%[cbpf]", "cb", "unused")
+      "s_lshl_b32 %[t1], %[cbpf], 16\n"
+      "s_or_b32 %[r0], %[r0], %[t1]\n"
+      "s_mov_b32 %[nb], 0\n"
+      "s_bitcmp1_b32 %[cbpf], 0\n"
+      "s_cbranch_scc1 L%=_tx\n"
+      "s_mov_b32 %[wmask], 0xffff\n"
+      "s_branch L%=_blocks\n"
+
+      // ---- transform size; the luma blocks' vectors and scan
+      "L%=_tx:\n"
+      MB_DEC("%[vm0]", "32", "31", "0x100020", "%[sym]", "tx", "unused")
+      "s_lshl_b32 %[t1], %[sym], 24\n"
+      "s_or_b32 %[r0], %[r0], %[t1]\n"
+      "s_cmp_eq_u32 %[sym], 0\n"
+      "s_cbranch_scc0 L%=_tx1\n"
+      "s_mov_b32 %[nb], 16\n"
+      "s_mov_b32 %[total], 16\n"
+      "v_mov_b32 %[va], %[t00]\n"
+      "v_mov_b32 %[vb], %[t01]\n"
+      "v_mov_b32 %[vc], %[t02]\n"
+      "v_mov_b32 %[vzz], %[zz4]\n"
+      "s_branch L%=_blocks\n"
+      "L%=_tx1:\n"
+      "s_cmp_eq_u32 %[sym], 1\n"
+      "s_cbranch_scc0 L%=_tx2\n"
+      "s_mov_b32 %[nb], 4\n"
+      "s_mov_b32 %[total], 64\n"
+      "v_mov_b32 %[va], %[t10]\n"
+      "v_mov_b32 %[vb], %[t11]\n"
+      "v_mov_b32 %[vc], %[t12]\n"
+      "v_mov_b32 %[vzz], %[zz8]\n"
+      "s_branch L%=_blocks\n"
+      "L%=_tx2:\n"
+      "s_mov_b32 %[nb], 1\n"
+      "s_movk_i32 %[total], 0x100\n"
+      "v_mov_b32 %[va], %[t20]\n"
+      "v_mov_b32 %[vb], %[t21]\n"
+      "v_mov_b32 %[vc], %[t22]\n"
+      "v_mov_b32 %[vzz], %[zz16]\n"
+
+      // ---- blocks: luma 0..nb-1, then Cb, Cr (when coded)
+      "L%=_blocks:\n"
+      "s_mov_b32 %[k], 0\n"
+      "L%=_bk:\n"
+      "s_cmp_eq_u32 %[k], %[nb]\n"
+      "s_cbranch_scc0 L%=_bk_nl\n"
+      // luma complete: the chroma mode (intra) or the uncoded chroma planes (inter)
+      "s_and_b32 %[t3], %[r0], 0xff\n"
+      "s_cmp_eq_u32 %[t3], 2\n"
+      "s_cbranch_scc0 L%=_bk_ic\n"
+      MB_DEC("%[vm1]", "16", "15", "0x100010", "%[sym]", "cm", "unused")
+      "s_lshl_b32 %[t1], %[sym], 8\n"
+      "s_or_b32 %[r1], %[r1], %[t1]\n"
+      "s_branch L%=_bk_nl\n"
+      "L%=_bk_ic:\n"
+      "s_bitcmp1_b32 %[cbpf], 1\n"
+      "s_cbranch_scc1 L%=_bk_icr\n"
+      "s_bitset1_b32 %[wmask], 16\n"
+      "L%=_bk_icr:\n"
+      "s_bitcmp1_b32 %[cbpf], 2\n"
+      "s_cbranch_scc1 L%=_bk_nl\n"
+      "s_bitset1_b32 %[wmask], 17\n"
+      "L%=_bk_nl:\n"
+      "s_cmp_lt_u32 %[k], %[nb]\n"
+      "s_cbranch_scc0 L%=_bk_ch\n"
+      "s_lshl_b32 %[blk8], %[k], 8\n"
+      "s_and_b32 %[t3], %[r0], 0xff\n"
+      "s_cmp_eq_u32 %[t3], 2\n"
+      "s_cbranch_scc0 L%=_tb\n"
+      // intra luma mode of block k (4 bits in decode order)
+      MB_DEC("%[vm1]", "0", "-1", "0x100000", "%[sym]", "lm", "unused")
+      "s_and_b32 %[t1], %[k], 7\n"
+      "s_lshl_b32 %[t1], %[t1], 2\n"
+      "s_lshl_b32 %[t1], %[sym], %[t1]\n"
+      "s_cmp_lt_u32 %[k], 8\n"
+      "s_cbranch_scc0 L%=_lmh\n"
+      "s_or_b32 %[lm0], %[lm0], %[t1]\n"
+      "s_branch L%=_tb\n"
+      "L%=_lmh:\n"
+      "s_or_b32 %[lm1], %[lm1], %[t1]\n"
+      "s_branch L%=_tb\n"
+      "L%=_bk_ch:\n"
+      "s_sub_u32 %[t1], %[k], %[nb]\n"
+      "s_add_u32 %[t3], %[t1], 1\n"
+      "s_lshr_b32 %[t3], %[cbpf], %[t3]\n"
+      "s_bitcmp1_b32 %[t3], 0\n"
+      "s_cbranch_scc0 L%=_bk_next\n"
+      "s_add_u32 %[t1], %[t1], 16\n"
+      "s_lshl_b32 %[blk8], %[t1], 8\n"
+      "s_mov_b32 %[total], 64\n"
+      "v_mov_b32 %[va], %[t30]\n"
+      "v_mov_b32 %[vb], %[t31]\n"
+      "v_mov_b32 %[vc], %[t32]\n"
+      "v_mov_b32 %[vzz], %[zz8]\n"
+
+      // ------------------------------------------------------ token block
+      "L%=_tb:\n"
+      "s_mov_b32 %[si], 0\n"
+      "s_mov_b32 %[stop], 1\n"
+      "s_mov_b32 %[boff], 0\n"
+      "s_mov_b32 %[prev], 0\n"
+      "s_mov_b32 %[sumabs], 0\n"
+      "s_mov_b32 %[firstsi], 0\n"
+      "v_mov_b32 %[vcur], %[va]\n"
+      "L%=_tok:\n"
+      "s_and_b32 %[slot], %[x0], 0xfff\n"
+      "s_lshl_b32 %[t1], %[slot], 13\n"
+      "s_or_b32 %[t1], %[t1], 0x1fff\n"
+      "v_cmp_ge_u32_e64 %[b], %[t1], %[vcur]\n"
+      "s_add_u32 %[t2], %[boff], %[prev]\n"
+      "s_or_b32 %[t1], %[t2], 0x80000\n"
+      "s_lshr_b32 %[xn], %[x0], 12\n"
+      "s_bfe_u64 %[b], %[b], %[t1]\n"
+      "s_bcnt1_i32_b64 %[cnt], %[b]\n"
+      "s_cmp_eq_u32 %[cnt], 1\n"
+      "s_cbranch_scc0 L%=_tnz\n"
+      // zero token: symbol 0 starts at 0
+      "v_readlane_b32 %[p], %[vcur], %[t2]\n"
+      "s_mov_b32 %[prev], 0\n"
+      "s_and_b32 %[p], %[p], 0x1fff\n"
+      "s_mul_i32 %[xn], %[p], %[xn]\n"
+      "s_add_u32 %[xn], %[xn], %[slot]\n"
+      MB_RN("tz")
+      "s_add_u32 %[si], %[si], 1\n"
+      "s_cmp_lg_u32 %[si], %[stop]\n"
+      "s_cbranch_scc1 L%=_tok\n"
+      "s_branch L%=_tstop\n"
+      "L%=_tnz:\n"
+      "s_cmp_eq_u32 %[cnt], 0\n"
+      "s_cbranch_scc1 L%=_unused\n"
+      "s_add_u32 %[t1], %[t2], %[cnt]\n"
+      "s_add_u32 %[t1], %[t1], -1\n"
+      "v_readlane_b32 %[p], %[vcur], %[t1]\n"
+      "s_add_u32 " MB_MAG ", %[cnt], -1\n"
+      "s_and_b32 %[t2], %[p], 0x1fff\n"
+      "s_lshr_b32 %[p], %[p], 13\n"
+      "s_mul_i32 %[xn], %[t2], %[xn]\n"
+      "s_sub_u32 %[t2], %[slot], %[p]\n"
+      "s_add_u32 %[xn], %[xn], %[t2]\n"
+      MB_RN("tn")
+      "s_cmp_ge_u32 %[cnt], 6\n"
+      "s_cbranch_scc1 L%=_tescob\n"
+      "s_cmp_eq_u32 " MB_MAG ", 1\n"
+      "s_cselect_b32 %[prev], 8, 16\n"
+      "L%=_tlev:\n"
+      "s_add_u32 %[sumabs], %[sumabs], " MB_MAG "\n"
+      "s_cmp_eq_u32 %[firstsi], 0\n"
+      "s_cbranch_scc1 L%=_tfirst\n"
+      MB_SIGN("ts")
+      MB_ZZ
+      MB_EMIT("e1")
+      "s_mov_b32 %[last], %[si]\n"
+      "s_add_u32 %[si], %[si], 1\n"
+      "s_cmp_lg_u32 %[si], %[stop]\n"
+      "s_cbranch_scc1 L%=_tok\n"
+      "s_branch L%=_tstop\n"
+      // first nonzero level: its sign resolves at the end of the block
+      "L%=_tfirst:\n"
+      MB_ZZ
+      "s_lshl_b32 %[t1], " MB_MAG ", 8\n"
+      "s_or_b32 %[firstpm], %[t1], %[t2]\n"
+      "s_add_u32 %[firstsi], %[si], 1\n"
+      "s_mov_b32 %[last], %[si]\n"
+      "s_add_u32 %[si], %[si], 1\n"
+      "s_cmp_lg_u32 %[si], %[stop]\n"
+      "s_cbranch_scc1 L%=_tok\n"
+      // si reached stop: the end of the block or a band boundary
+      // (bandOf thresholds 1, 3, 9, 21, model.go:165-180)
+      "L%=_tstop:\n"
+      "s_cmp_eq_u32 %[si], %[total]\n"
+      "s_cbranch_scc1 L%=_tend\n"
+      "s_xor_b32 %[boff], %[boff], 24\n"
+      "s_cmp_eq_u32 %[si], 1\n"
+      "s_cselect_b32 %[stop], 3, %[stop]\n"
+      "s_cmp_eq_u32 %[si], 9\n"
+      "s_cselect_b32 %[stop], 21, %[stop]\n"
+      "s_cmp_eq_u32 %[si], 3\n"
+      "s_cbranch_scc0 L%=_tb2\n"
+      "v_mov_b32 %[vcur], %[vb]\n"
+      "s_mov_b32 %[stop], 9\n"
+      "L%=_tb2:\n"
+      "s_cmp_eq_u32 %[si], 21\n"
+      "s_cbranch_scc0 L%=_tb4\n"
+      "v_mov_b32 %[vcur], %[vc]\n"
+      "s_mov_b32 %[stop], %[total]\n"
+      "L%=_tb4:\n"
+      "s_min_u32 %[stop], %[stop], %[total]\n"
+      "s_branch L%=_tok\n"
+      // escape (magnitude 5 + getEscapeValue, mv.go:124-128) or end of block
+      "L%=_tescob:\n"
+      "s_cmp_eq_u32 %[cnt], 7\n"
+      "s_cbranch_scc1 L%=_tend\n"
+      MB_CLASS("%[vm2]", "0", "-1", "0x100000", "es")
+      "s_add_u32 " MB_MAG ", %[sym], 5\n"
+      "s_mov_b32 %[prev], 16\n"
+      "s_branch L%=_tlev\n"
+      // end of block: the first level's sign, hidden in the level-sum parity
+      // when last - first >= 4, else explicit
+      "L%=_tend:\n"
+      "s_cmp_eq_u32 %[firstsi], 0\n"
+      "s_cbranch_scc1 L%=_tbdone\n"
+      "s_lshr_b32 " MB_MAG ", %[firstpm], 8\n"
+      "s_add_u32 %[t1], %[firstsi], -1\n"
+      "s_sub_u32 %[t1], %[last], %[t1]\n"
+      "s_cmp_lt_i32 %[t1], 4\n"
+      "s_cbranch_scc0 L%=_tpar\n"
+      MB_SIGN("sd")
+      "s_branch L%=_temit\n"
+      "L%=_tpar:\n"
+      "s_bitcmp1_b32 %[sumabs], 0\n"
+      "s_cbranch_scc0 L%=_temit\n"
+      "s_sub_u32 " MB_MAG ", 0, " MB_MAG "\n"
+      "L%=_temit:\n"
+      "s_and_b32 %[t2], %[firstpm], 0xff\n"
+      MB_EMIT("e2")
+      "L%=_tbdone:\n"
+      "s_lshr_b32 %[t1], %[blk8], 8\n"
+      "s_lshl_b32 %[t1], 1, %[t1]\n"
+      "s_or_b32 %[wmask], %[wmask], %[t1]\n"
+      "L%=_bk_next:\n"
+      "s_add_u32 %[k], %[k], 1\n"
+      "s_add_u32 %[t1], %[nb], 2\n"
+      "s_cmp_lt_u32 %[k], %[t1]\n"
+      "s_cbranch_scc1 L%=_bk\n"
+      "s_branch L%=_mbend\n"
+
+      // ---- skip macroblock: the predicted vector, every block written
+      "L%=_skip:\n"
+      "s_mov_b32 %[t3], 0\n"
+      "s_branch L%=_mvp\n"
+      "L%=_skipmv:\n"
+      "s_mov_b32 %[r0], 0\n"
+      "s_mov_b32 %[mv0], " MB_PRED "\n"
+      "s_mov_b32 %[mvcur], " MB_PRED "\n"
+      "s_mov_b32 %[wmask], 0x3ffff\n"
+      "s_branch L%=_mbend\n"
+
+      // ---- predictMV (mv.go:21-52) into pred; t3 = 0 skip, 1 inter, 2 error
+      "L%=_mvp:\n"
+      "s_sub_u32 %[t1], %[mbx], %[x0t]\n"
+      "s_cmp_gt_u32 %[mby], %[y0t]\n"
+      "s_cbranch_scc0 L%=_mvp_nob\n"
+      "v_readlane_b32 %[t2], %[vprow], %[t1]\n"
+      "s_add_u32 %[p], %[t1], 1\n"
+      "s_sub_u32 %[xn], %[x1t], %[x0t]\n"
+      "s_cmp_lt_u32 %[p], %[xn]\n"
+      "s_cbranch_scc0 L%=_mvp_cl\n"
+      "v_readlane_b32 %[xn], %[vprow], %[p]\n"
+      "s_branch L%=_mvp_med\n"
+      "L%=_mvp_cl:\n"
+      "s_cmp_eq_u32 %[t1], 0\n"
+      "s_cbranch_scc1 L%=_mvp_b\n"
+      "s_add_u32 %[p], %[t1], -1\n"
+      "v_readlane_b32 %[xn], %[vprow], %[p]\n"
+      "L%=_mvp_med:\n"
+      "s_cmp_eq_u32 %[t1], 0\n"
+      "s_cselect_b32 %[p], 0, %[leftmv]\n"
+      // component-wise median of a (p), b (t2), c (xn)
+      "s_sext_i32_i16 %[slot], %[p]\n"
+      "s_sext_i32_i16 %[cnt], %[t2]\n"
+      "s_sext_i32_i16 %[k2], %[xn]\n"
+      "s_min_i32 %[t1], %[slot], %[cnt]\n"
+      "s_max_i32 %[slot], %[slot], %[cnt]\n"
+      "s_min_i32 %[slot], %[slot], %[k2]\n"
+      "s_max_i32 %[t1], %[t1], %[slot]\n"
+      "s_ashr_i32 %[slot], %[p], 16\n"
+      "s_ashr_i32 %[cnt], %[t2], 16\n"
+      "s_ashr_i32 %[k2], %[xn], 16\n"
+      "s_min_i32 %[p], %[slot], %[cnt]\n"
+      "s_max_i32 %[slot], %[slot], %[cnt]\n"
+      "s_min_i32 %[slot], %[slot], %[k2]\n"
+      "s_max_i32 %[p], %[p], %[slot]\n"
+      "s_and_b32 %[t1], %[t1], 0xffff\n"
+      "s_lshl_b32 %[p], %[p], 16\n"
+      "s_or_b32 " MB_PRED ", %[t1], %[p]\n"
+      "s_branch L%=_mvp_end\n"
+      "L%=_mvp_b:\n"
+      "s_mov_b32 " MB_PRED ", %[t2]\n"
+      "s_branch L%=_mvp_end\n"
+      "L%=_mvp_nob:\n"
+      "s_cmp_eq_u32 %[t1], 0\n"
+      "s_cselect_b32 " MB_PRED ", 0, %[leftmv]\n"
+      "L%=_mvp_end:\n"
+      "s_cmp_eq_u32 %[t3], 0\n"
+      "s_cbranch_scc1 L%=_skipmv\n"
+      "s_cmp_eq_u32 %[t3], 1\n"
+      "s_cbranch_scc1 L%=_nopred\n"
+      "s_branch L%=_modeerr_mv\n"
+
+      // ---- end of a macroblock: MV row, record, next macroblock
+      "L%=_mbend:\n"
+      "s_and_b32 %[t3], %[r0], 0xff\n"
+      "s_cmp_eq_u32 %[t3], 2\n"
+      "s_cselect_b32 %[t3], 1, 0\n"
+      "s_add_u32 %[nintra], %[nintra], %[t3]\n"
+      "s_sub_u32 %[t1], %[mbx], %[x0t]\n"
+      "s_mov_b32 m0, %[t1]\n"
+      "v_writelane_b32 %[vcrow], %[mvcur], m0\n"
+      "s_mov_b32 %[leftmv], %[mvcur]\n"
+      "s_cmp_eq_u32 %[nst], 0\n"
+      "s_cbranch_scc0 L%=_rec\n"
+      "s_mul_i32 %[firstmb], %[mby], %[mbc]\n"
+      "s_add_u32 %[firstmb], %[firstmb], %[mbx]\n"
+      "L%=_rec:\n"
+      "s_mul_i32 %[t2], %[nst], 12\n"
+      "s_add_u32 %[t1], %[tcb], %[cstart]\n"
+      "s_add_u32 %[t3], %[cbase], %[cn]\n"
+      "s_sub_u32 %[t3], %[t3], %[cstart]\n"
+      MB_REC(0, "%[r0]")
+      MB_REC(1, "%[r1]")
+      MB_REC(2, "%[wmask]")
+      MB_REC(3, "%[t1]")
+      MB_REC(4, "%[t3]")
+      MB_REC(5, "%[mv0]")
+      MB_REC(6, "%[mv1]")
+      MB_REC(7, "%[mv2]")
+      MB_REC(8, "%[mv3]")
+      MB_REC(9, "%[lm0]")
+      MB_REC(10, "%[lm1]")
+      MB_REC(11, "0")
+      "s_add_u32 %[done], %[done], 1\n"
+      "s_add_u32 %[nst], %[nst], 1\n"
+      "s_cmp_eq_u32 %[nst], 5\n"
+      "s_cbranch_scc0 L%=_next\n"
+      "s_mov_b32 %[t3], 0\n"
+      "s_branch L%=_rflush\n"
+      "L%=_next:\n"
+      "s_add_u32 %[mbx], %[mbx], 1\n"
+      "s_cmp_lt_u32 %[mbx], %[x1t]\n"
+      "s_cbranch_scc1 L%=_mb\n"
+      // end of a row: its records out, the rows swap
+      "s_mov_b32 %[t3], 1\n"
+      "s_cmp_eq_u32 %[nst], 0\n"
+      "s_cbranch_scc0 L%=_rflush\n"
+      "L%=_rowend:\n"
+      "v_mov_b32 %[vprow], %[vcrow]\n"
+      "s_mov_b32 %[mbx], %[x0t]\n"
+      "s_add_u32 %[mby], %[mby], 1\n"
+      "s_cmp_lt_u32 %[mby], %[y1t]\n"
+      "s_cbranch_scc1 L%=_mb\n"
+      "s_mov_b32 %[ev], 0\n"
+      "s_branch L%=_exit\n"
+      // staged records out: lanes [0, 12 nst) to recs[firstmb ..]
+      // (t3: 0 continue with the next macroblock, 1 end of row)
+      "L%=_rflush:\n"
+      "s_mul_i32 %[t1], %[nst], 12\n"
+      "s_bfm_b64 %[b], %[t1], 0\n"
+      "s_mul_i32 %[t2], %[firstmb], 48\n"
+      "v_add_u32 %[vaddr], %[t2], %[vlane4]\n"
+      "s_mov_b64 exec, %[b]\n"
+      "global_store_dword %[vaddr], %[vstg], %[recs]\n"
+      "s_mov_b64 exec, -1\n"
+      "s_mov_b32 %[nst], 0\n"
+      "s_cmp_eq_u32 %[t3], 0\n"
+      "s_cbranch_scc1 L%=_next\n"
+      "s_branch L%=_rowend\n"
+
+      // ---- decode errors: the failing macroblock's fields stay as decoded
+      "L%=_mode_unused:\n"
+      "s_mov_b32 %[ev], 3\n"
+      "s_branch L%=_modeerr\n"
+      "L%=_mode_underrun:\n"
+      "s_mov_b32 %[ev], 4\n"
+      // a failed mode reads as a skip macroblock (Go reconstructs it)
+      "L%=_modeerr:\n"
+      "s_mov_b32 %[t3], 2\n"
+      "s_branch L%=_mvp\n"
+      "L%=_modeerr_mv:\n"
+      "s_mov_b32 %[r0], 0\n"
+      "s_mov_b32 %[mv0], " MB_PRED "\n"
+      "s_mov_b32 %[mvcur], " MB_PRED "\n"
+      "s_mov_b32 %[wmask], 0x3ffff\n"
+      "s_branch L%=_exit\n"
+      "L%=_unused:\n"
+      "s_mov_b32 %[ev], 3\n"
+      "s_branch L%=_exit\n"
+      "L%=_underrun:\n"
+      "s_mov_b32 %[ev], 4\n"
+      "s_branch L%=_exit\n"
+
+      // ---- out-of-line renormalisation and coefficient stores
+      MB_RN_OOL("md", "mode_underrun")
+      MB_RN_OOL("rf", "underrun")
+      MB_RN_OOL("pt", "underrun")
+      MB_RN_OOL("cxd", "underrun")
+      MB_RN_OOL("cxb", "underrun")
+      MB_RN_OOL("cyd", "underrun")
+      MB_RN_OOL("cyb", "underrun")
+      MB_RN_OOL("cb", "underrun")
+      MB_RN_OOL("tx", "underrun")
+      MB_RN_OOL("cm", "underrun")
+      MB_RN_OOL("lm", "underrun")
+      MB_RN_OOL("tz", "underrun")
+      MB_RN_OOL("tn", "underrun")
+      MB_RN_OOL("ts", "underrun")
+      MB_RN_OOL("esd", "underrun")
+      MB_RN_OOL("esb", "underrun")
+      MB_RN_OOL("sd", "underrun")
+      MB_EMIT_OOL("e1")
+      MB_EMIT_OOL("e2")
+
+      "L%=_exit:\n"
+      "s_mov_b32 m0, %[m0s]\n"
+      "s_nop 4\n"
+      : [x0] "+&s"(S.x0), [x1] "+&s"(S.x1), [x2] "+&s"(S.x2), [x3] "+&s"(S.x3), [pos] "+&s"(S.pos),
+        [nsym] "+&s"(S.nsym), [wbase] "+&s"(S.wbase), [lim] "+&s"(S.lim), [cn] "+&s"(S.cn), [cbase] "+&s"(S.cbase),
+        [mbx] "+&s"(S.mbx), [mby] "+&s"(S.mby), [done] "+&s"(S.done), [nintra] "+&s"(S.nintra), [nst] "+&s"(S.nst),
+        [firstmb] "+&s"(S.firstmb), [leftmv] "+&s"(S.leftmv), [vwin] "+&v"(S.vwin), [vbuf] "+&v"(S.vbuf),
+        [vstg] "+&v"(S.vstg), [vprow] "+&v"(S.vprow), [vcrow] "+&v"(S.vcrow), [r0] "=&s"(S.r0), [r1] "=&s"(S.r1),
+        [wmask] "=&s"(S.wmask), [cstart] "=&s"(S.cstart), [mv0] "=&s"(S.mv0), [mv1] "=&s"(S.mv1),
+        [mv2] "=&s"(S.mv2), [mv3] "=&s"(S.mv3), [mvcur] "=&s"(S.mvcur), [lm0] "=&s"(S.lm0), [lm1] "=&s"(S.lm1),
+        [ev] "=&s"(ev), [slot] "=&s"(slot), [t1] "=&s"(t1), [t2] "=&s"(t2), [t3] "=&s"(t3), [p] "=&s"(p),
+        [xn] "=&s"(xn), [cnt] "=&s"(cnt), [sym] "=&s"(sym), [val] "=&s"(val), [k2] "=&s"(k2), [k] "=&s"(k),
+        [nb] "=&s"(nb), [total] "=&s"(total), [blk8] "=&s"(blk8), [cbpf] "=&s"(cbpf), [si] "=&s"(si),
+        [stop] "=&s"(stop), [boff] "=&s"(boff), [prev] "=&s"(prev), [sumabs] "=&s"(sumabs),
+        [firstsi] "=&s"(firstsi), [firstpm] "=&s"(firstpm), [last] "=&s"(last), [m0s] "=&s"(m0s),
+        [b] "=&s"(b), [vcur] "=&v"(vcur), [va] "=&v"(va), [vb] "=&v"(vb), [vc] "=&v"(vc), [vzz] "=&v"(vzz),
+        [vaddr] "=&v"(vaddr), [vt] "=&v"(vt)
+      : [vm0] "v"(T.misc0), [vm1] "v"(T.misc1), [vm2] "v"(T.misc2), [t00] "v"(T.t00), [t01] "v"(T.t01),
+        [t02] "v"(T.t02), [t10] "v"(T.t10), [t11] "v"(T.t11), [t12] "v"(T.t12), [t20] "v"(T.t20),
+        [t21] "v"(T.t21), [t22] "v"(T.t22), [t30] "v"(T.t30), [t31] "v"(T.t31), [t32] "v"(T.t32),
+        [zz4] "v"(zz4), [zz8] "v"(zz8), [zz16] "v"(zz16), [vlane4] "v"(vlane4), [x0t] "s"(sgpr(x0t)),
+        [x1t] "s"(sgpr(x1t)), [y0t] "s"(sgpr(y0t)), [y1t] "s"(sgpr(y1t)), [mbc] "s"(sgpr(mbc)),
+        [interf] "s"(sgpr(interf)), [tcb] "s"(sgpr(tcb)), [f0] "s"(sgpr(f0)), [f1] "s"(sgpr(f1)),
+        [hi2] "s"(sgpr(hi2)), [wlo] "s"(sgpr(wlo)), [tb] "s"(sgpr(tb)), [dst] "s"(sgpr(dst)),
+        [recs] "s"(sgpr(recs))
+      : "scc", "vcc", "memory");
+  return ev;
+}
+
 // Pointers read from a job table are generic pointers; the job table is read
 // through the constant address space, and the compiler takes generic
 // pointers loaded from constant memory to be global ones, so accesses
@@ -864,7 +1577,7 @@ __device__ __forceinline__ void store_dword_if(uint32_t* base, uint32_t off, uin
       "global_store_dword %[off], %[v], %[base]\n"
       "s_mov_b64 exec, %[save]\n"
       : [m] "=&s"(m), [save] "=&s"(save)
-      : [p] "v"(static_cast<uint32_t>(pred)), [off] "v"(off), [v] "v"(v), [base] "s"(base)
+      : [p] "v"(static_cast<uint32_t>(pred)), [off] "v"(off), [v] "v"(v), [base] "s"(sgpr(base))
       : "memory", "scc");
 }
 
@@ -1022,10 +1735,150 @@ __device__ __forceinline__ void parse_tile(const uint8_t* __restrict__ pkt, cons
   if (lane == 0) status[t] = TileStatus{R.err, done, R.nsym, nintra};
 }
 
+
+// decodeTile through mb_loop (tiles at most 64 macroblocks wide with a sign
+// table); same outputs as parse_tile.
+__device__ __forceinline__ void parse_tile_asm(const uint8_t* __restrict__ pkt, const uint8_t* __restrict__ desc,
+                                           MBRec* __restrict__ recs, uint32_t* __restrict__ coefs,
+                                           TileStatus* __restrict__ status, const int t) {
+  extern __shared__ uint32_t smem[];
+  const FrameHdr* hdr = reinterpret_cast<const FrameHdr*>(desc);
+  const TileDesc* tiles = reinterpret_cast<const TileDesc*>(desc + desc_tiles_off());
+  const int lane = lane_id();
+  const TileDesc td = tiles[t];
+
+  // The frame's symbol tables, one entry per lane of each vector.
+  using GWord = __attribute__((address_space(1))) const uint32_t;
+  const GWord* vt = (const GWord*)(&hdr->vt[0][lane]);
+  Tables T;
+  T.misc0 = vt[0];
+  T.misc1 = vt[64];
+  T.misc2 = vt[128];
+  T.t00 = vt[64 * 3]; T.t01 = vt[64 * 4]; T.t02 = vt[64 * 5];
+  T.t10 = vt[64 * 6]; T.t11 = vt[64 * 7]; T.t12 = vt[64 * 8];
+  T.t20 = vt[64 * 9]; T.t21 = vt[64 * 10]; T.t22 = vt[64 * 11];
+  T.t30 = vt[64 * 12]; T.t31 = vt[64 * 13]; T.t32 = vt[64 * 14];
+  FastTok F;
+  {
+    const uint32_t e0 = rd_lane(T.misc0, static_cast<int>(kGSign)), e1 = rd_lane(T.misc0, static_cast<int>(kGSign) + 1);
+    F.sign_used = e0 != kNoSym;
+    F.f0 = e0 & 0x1FFFu;
+    F.f1 = e1 & 0x1FFFu;
+    F.vlane4 = 4u * static_cast<uint32_t>(lane);
+  }
+  // LDS: the macroblock records being staged, then two MV rows of the tile
+  // width.  Records go to memory kRecStage at a time, every lane storing
+  // dwords: a store per macroblock from one lane would have the next
+  // macroblock's record registers wait for it.
+  LdsU32* stage = (LdsU32*)smem;
+  LdsU32* prow = (LdsU32*)(smem + kRecStage * sizeof(MBRec) / 4);
+  LdsU32* crow = prow + hdr->max_tile_w;
+  uint32_t nst = 0, first_mb = 0;
+  auto flush = [&]() {
+    const LdsU32* src = stage;
+    uint32_t* dst = reinterpret_cast<uint32_t*>(recs + first_mb);
+    const uint32_t nd = nst * (sizeof(MBRec) / 4);
+    for (uint32_t i0 = 0; i0 < nd; i0 += 64) {
+      const uint32_t i = i0 + static_cast<uint32_t>(lane);
+      store_dword_if(dst + i0, 4u * static_cast<uint32_t>(lane), src[i < nd ? i : 0], i < nd);
+    }
+    nst = 0;
+  };
+  const int tw = td.x1 - td.x0;
+
+  if (td.byte_len < 16) {  // ransDecoder.reset (rans.go:237-250)
+    if (lane == 0) status[t] = TileStatus{kErrCorrupt | (kTileShort << 8), 0, 0, 0};
+    return;
+  }
+  Rans R;
+  R.win.init(pkt + td.byte_off, td.byte_len, lane);
+  R.pos = td.byte_len;
+  R.nsym = 0;
+  R.err = 0;
+  {
+    uint32_t sv[4];
+#pragma unroll
+    for (int i = 3; i >= 0; i--) {  // pop32: big-endian from the top, state 3 first
+      uint32_t a = R.win.byte(static_cast<int32_t>(--R.pos), lane);
+      uint32_t b = R.win.byte(static_cast<int32_t>(--R.pos), lane);
+      uint32_t c = R.win.byte(static_cast<int32_t>(--R.pos), lane);
+      uint32_t e = R.win.byte(static_cast<int32_t>(--R.pos), lane);
+      sv[i] = (a << 24) | (b << 16) | (c << 8) | e;
+    }
+    R.x0 = sv[0]; R.x1 = sv[1]; R.x2 = sv[2]; R.x3 = sv[3];
+  }
+  // Scan tables in registers: lane l holds positions 4l..4l+3 of each size.
+  uint32_t zz4 = 0, zz8 = 0, zz16 = 0;
+  {
+    const uint32_t* z = reinterpret_cast<const uint32_t*>(c_zz);
+    if (lane < 4) zz4 = z[lane];
+    if (lane < 16) zz8 = z[64 + lane];
+    zz16 = z[128 + lane];
+  }
+  // Let every start-up load land here (an empty asm that takes the loaded
+  // registers), so no load into a register the macroblock loop uses is
+  // outstanding there: a wait in the loop would also wait for every store
+  // issued before it.
+  asm volatile("" : "+v"(T.misc0), "+v"(T.misc1), "+v"(T.misc2), "+v"(zz4), "+v"(zz8), "+v"(zz16));
+  asm volatile("" : "+v"(T.t00), "+v"(T.t01), "+v"(T.t02), "+v"(T.t10), "+v"(T.t11), "+v"(T.t12));
+  asm volatile("" : "+v"(T.t20), "+v"(T.t21), "+v"(T.t22), "+v"(T.t30), "+v"(T.t31), "+v"(T.t32));
+
+  CoefOut out;
+  out.dst = coefs + td.coef_base;
+  out.base = 0; out.cn = 0; out.buf = 0;
+  const int mbc = hdr->mbc;
+  const bool inter_frame = hdr->inter != 0;
+  uint32_t done = 0, nintra = 0;
+  bool ok = true;
+  {
+    // The hand-scheduled loop over the whole tile.
+    MbLoop S;
+    S.x0 = R.x0; S.x1 = R.x1; S.x2 = R.x2; S.x3 = R.x3; S.pos = R.pos; S.nsym = R.nsym;
+    S.wbase = static_cast<uint32_t>(R.win.wbase); S.lim = static_cast<uint32_t>(R.win.lim);
+    S.cn = 0; S.cbase = 0;
+    S.mbx = td.x0; S.mby = td.y0; S.done = 0; S.nintra = 0; S.nst = 0; S.firstmb = 0; S.leftmv = 0;
+    S.vwin = R.win.cur; S.vbuf = 0; S.vstg = 0; S.vprow = 0; S.vcrow = 0;
+    S.x0 = sgpr(S.x0); S.x1 = sgpr(S.x1); S.x2 = sgpr(S.x2); S.x3 = sgpr(S.x3); S.pos = sgpr(S.pos);
+    S.nsym = sgpr(S.nsym); S.wbase = sgpr(S.wbase); S.lim = sgpr(S.lim); S.mbx = sgpr(S.mbx); S.mby = sgpr(S.mby);
+    const uint32_t ev = mb_loop(S, T, zz4, zz8, zz16, F.vlane4, td.x0, td.x1, td.y0, td.y1, static_cast<uint32_t>(mbc),
+                                inter_frame ? 1u : 0u, td.coef_base, F.f0, F.f1,
+                                static_cast<uint32_t>(R.win.hi - R.win.lo - 4), static_cast<uint32_t>(R.win.lo),
+                                R.win.tile + R.win.lo, out.dst, recs);
+    R.x0 = S.x0; R.x1 = S.x1; R.x2 = S.x2; R.x3 = S.x3; R.pos = S.pos; R.nsym = S.nsym;
+    R.win.wbase = static_cast<int32_t>(S.wbase); R.win.lim = static_cast<int32_t>(S.lim); R.win.cur = S.vwin;
+    out.cn = S.cn; out.base = S.cbase; out.buf = S.vbuf;
+    done = S.done;
+    nintra = S.nintra;
+    if (ev != kMbDone) {
+      R.err = kErrCorrupt | ((ev == kMbUnused ? kTileUnused : kTileUnderrun) << 8);
+      // the records still staged, then the failing macroblock's as decoded
+      const uint32_t nd = S.nst * (sizeof(MBRec) / 4);
+      store_dword_if(reinterpret_cast<uint32_t*>(recs + S.firstmb), 4u * static_cast<uint32_t>(lane), S.vstg,
+                     static_cast<uint32_t>(lane) < nd);
+      if ((S.r0 & 0xFFu) == 2) nintra++;
+      if (lane == 0) {
+        uint32_t* rw = reinterpret_cast<uint32_t*>(recs + S.mby * static_cast<uint32_t>(mbc) + S.mbx);
+        rw[0] = S.r0; rw[1] = S.r1; rw[2] = S.wmask; rw[3] = td.coef_base + S.cstart;
+        rw[4] = S.cbase + S.cn - S.cstart;
+        rw[5] = S.mv0; rw[6] = S.mv1; rw[7] = S.mv2; rw[8] = S.mv3; rw[9] = S.lm0; rw[10] = S.lm1; rw[11] = 0;
+      }
+      done++;
+    }
+  }
+  if (static_cast<uint32_t>(lane) < out.cn) out.dst[out.base + lane] = out.buf;
+  if (lane == 0) status[t] = TileStatus{R.err, done, R.nsym, nintra};
+}
+
 __global__ __launch_bounds__(64) void ebiv_parse(const FrameJob* __restrict__ jobs, const WorkItem* __restrict__ items) {
   const WorkItem it = item_of(items);
   ConstJob& job = job_of(jobs, it.job);
-  parse_tile(job.pkt, job.desc, job.recs, job.coefs, job.status, static_cast<int>(it.idx));
+  const FrameHdr* hdr = reinterpret_cast<const FrameHdr*>(job.desc);
+  const TileDesc* tiles = reinterpret_cast<const TileDesc*>(job.desc + desc_tiles_off());
+  const int t = static_cast<int>(it.idx);
+  if (tiles[t].x1 - tiles[t].x0 <= 64 && hdr->vt[0][kGSign] != kNoSym)
+    parse_tile_asm(job.pkt, job.desc, job.recs, job.coefs, job.status, t);
+  else
+    parse_tile(job.pkt, job.desc, job.recs, job.coefs, job.status, t);
 }
 
 // ============================================================================
