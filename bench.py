@@ -245,6 +245,7 @@ def main():
     ap.add_argument("--serve-only", action="store_true", help="run only the serving mix")
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-budget-gb", type=float, default=24.0)
+    ap.add_argument("--serve-window", type=int, default=4, help="frames per session per engine step when serving")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this decoder runs on one GPU; runs on several GPUs are not part of this "
@@ -254,7 +255,7 @@ def main():
     from govid_amd.codec import DeviceBuffer
 
     if args.serve_only:
-        sv = run_serving(args.serve_requests, int(args.serve_budget_gb * (1 << 30)), 4, check_every=17)
+        sv = run_serving(args.serve_requests, int(args.serve_budget_gb * (1 << 30)), args.serve_window, check_every=17)
         print(json.dumps({"mode": "serving", **{k: (round(v, 3) if isinstance(v, float) else v) for k, v in sv.items()}}))
         return
     rd = load_stream()
@@ -342,7 +343,7 @@ def main():
     for d in dev:
         d.close()
     if not args.no_serve:
-        sv = run_serving(args.serve_requests, int(args.serve_budget_gb * (1 << 30)), 4, check_every=17)
+        sv = run_serving(args.serve_requests, int(args.serve_budget_gb * (1 << 30)), args.serve_window, check_every=17)
         log(f"bench: serving {sv['frames_per_s']:.1f} frames/s, p99 {sv['p99_frame_ms']:.2f} ms")
         out["serving"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in sv.items()}
     if not args.no_cpu:

@@ -856,8 +856,7 @@ __device__ __forceinline__ bool parse_mb(Rans& R, CoefOut& out, MBState& m, cons
 // It returns at the end of the tile (kMbDone) or at a decode error
 // (kMbUnused, kMbUnderrun) with the failing macroblock's fields decoded so
 // far, which the caller turns into its record exactly as the generic path
-// does.  Preconditions: the tile is at most 64 macroblocks wide (MV rows in
-// one VGPR) and the sign context has a table (signs from two SGPRs).
+// does.  Precondition: the sign context has a table (signs from two SGPRs).
 enum MbEvent : uint32_t { kMbDone = 0, kMbUnused = 3, kMbUnderrun = 4 };
 
 struct MbLoop {
@@ -865,6 +864,7 @@ struct MbLoop {
   uint32_t x0, x1, x2, x3, pos, nsym, wbase, lim, cn, cbase;
   uint32_t mbx, mby, done, nintra, nst, firstmb, leftmv;
   uint32_t vwin, vbuf, vstg, vprow, vcrow;  // per lane
+  uint32_t prowa, crowa;                    // LDS MV rows (wide tiles)
   // the failing macroblock (outputs)
   uint32_t r0, r1, wmask, cstart, mv0, mv1, mv2, mv3, mvcur, lm0, lm1;
 };
@@ -1019,6 +1019,33 @@ struct MbLoop {
   "s_add_u32 m0, %[t2], " #k "\n"                \
   "v_writelane_b32 %[vstg], " v ", m0\n"
 
+// MV rows of the tile (mv.go:21-52 reads left, above, above-right or
+// above-left): for tiles at most 64 macroblocks wide in two VGPRs (lane =
+// tile column), otherwise in LDS at byte addresses prowa / crowa.
+#define MB_ROW_RD_VGPR(dst, idx) "v_readlane_b32 " dst ", %[vprow], " idx "\n"
+#define MB_ROW_WR_VGPR                           \
+  "s_mov_b32 m0, %[t1]\n"                        \
+  "v_writelane_b32 %[vcrow], %[mvcur], m0\n"
+#define MB_ROW_SWAP_VGPR "v_mov_b32 %[vprow], %[vcrow]\n"
+#define MB_ROW_RD_LDS(dst, idx)                  \
+  "s_lshl_b32 %[k2], " idx ", 2\n"               \
+  "s_add_u32 %[k2], %[k2], %[prowa]\n"           \
+  "v_mov_b32 %[vt], %[k2]\n"                     \
+  "ds_read_b32 %[vt], %[vt]\n"                   \
+  "s_waitcnt lgkmcnt(0)\n"                       \
+  "v_readfirstlane_b32 " dst ", %[vt]\n"
+#define MB_ROW_WR_LDS                            \
+  "s_lshl_b32 %[t1], %[t1], 2\n"                 \
+  "s_add_u32 %[t1], %[t1], %[crowa]\n"           \
+  "v_mov_b32 %[vaddr], %[t1]\n"                  \
+  "v_mov_b32 %[vt], %[mvcur]\n"                  \
+  "ds_write_b32 %[vaddr], %[vt]\n"
+#define MB_ROW_SWAP_LDS                          \
+  "s_mov_b32 %[t1], %[prowa]\n"                  \
+  "s_mov_b32 %[prowa], %[crowa]\n"               \
+  "s_mov_b32 %[crowa], %[t1]\n"
+
+template <bool kWide>
 __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t zz4, uint32_t zz8, uint32_t zz16,
                                             uint32_t vlane4, uint32_t x0t, uint32_t x1t, uint32_t y0t, uint32_t y1t,
                                             uint32_t mbc, uint32_t interf, uint32_t tcb, uint32_t f0, uint32_t f1,
@@ -1028,6 +1055,10 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
   uint32_t k, nb, total, blk8, cbpf, si, stop, boff, prev, sumabs, firstsi, firstpm, last, m0s;
   uint64_t b;
   uint32_t vcur, va, vb, vc, vzz, vaddr, vt;
+  if constexpr (!kWide) {
+#define MB_ROW_RD MB_ROW_RD_VGPR
+#define MB_ROW_WR MB_ROW_WR_VGPR
+#define MB_ROW_SWAP MB_ROW_SWAP_VGPR
   asm volatile(
       "s_nop 4\n"
       "s_mov_b32 %[m0s], m0\n"
@@ -1352,18 +1383,18 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       "s_sub_u32 %[t1], %[mbx], %[x0t]\n"
       "s_cmp_gt_u32 %[mby], %[y0t]\n"
       "s_cbranch_scc0 L%=_mvp_nob\n"
-      "v_readlane_b32 %[t2], %[vprow], %[t1]\n"
+      MB_ROW_RD("%[t2]", "%[t1]")
       "s_add_u32 %[p], %[t1], 1\n"
       "s_sub_u32 %[xn], %[x1t], %[x0t]\n"
       "s_cmp_lt_u32 %[p], %[xn]\n"
       "s_cbranch_scc0 L%=_mvp_cl\n"
-      "v_readlane_b32 %[xn], %[vprow], %[p]\n"
+      MB_ROW_RD("%[xn]", "%[p]")
       "s_branch L%=_mvp_med\n"
       "L%=_mvp_cl:\n"
       "s_cmp_eq_u32 %[t1], 0\n"
       "s_cbranch_scc1 L%=_mvp_b\n"
       "s_add_u32 %[p], %[t1], -1\n"
-      "v_readlane_b32 %[xn], %[vprow], %[p]\n"
+      MB_ROW_RD("%[xn]", "%[p]")
       "L%=_mvp_med:\n"
       "s_cmp_eq_u32 %[t1], 0\n"
       "s_cselect_b32 %[p], 0, %[leftmv]\n"
@@ -1406,8 +1437,7 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       "s_cselect_b32 %[t3], 1, 0\n"
       "s_add_u32 %[nintra], %[nintra], %[t3]\n"
       "s_sub_u32 %[t1], %[mbx], %[x0t]\n"
-      "s_mov_b32 m0, %[t1]\n"
-      "v_writelane_b32 %[vcrow], %[mvcur], m0\n"
+      MB_ROW_WR
       "s_mov_b32 %[leftmv], %[mvcur]\n"
       "s_cmp_eq_u32 %[nst], 0\n"
       "s_cbranch_scc0 L%=_rec\n"
@@ -1445,7 +1475,7 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       "s_cmp_eq_u32 %[nst], 0\n"
       "s_cbranch_scc0 L%=_rflush\n"
       "L%=_rowend:\n"
-      "v_mov_b32 %[vprow], %[vcrow]\n"
+      MB_ROW_SWAP
       "s_mov_b32 %[mbx], %[x0t]\n"
       "s_add_u32 %[mby], %[mby], 1\n"
       "s_cmp_lt_u32 %[mby], %[y1t]\n"
@@ -1518,7 +1548,8 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
         [nsym] "+&s"(S.nsym), [wbase] "+&s"(S.wbase), [lim] "+&s"(S.lim), [cn] "+&s"(S.cn), [cbase] "+&s"(S.cbase),
         [mbx] "+&s"(S.mbx), [mby] "+&s"(S.mby), [done] "+&s"(S.done), [nintra] "+&s"(S.nintra), [nst] "+&s"(S.nst),
         [firstmb] "+&s"(S.firstmb), [leftmv] "+&s"(S.leftmv), [vwin] "+&v"(S.vwin), [vbuf] "+&v"(S.vbuf),
-        [vstg] "+&v"(S.vstg), [vprow] "+&v"(S.vprow), [vcrow] "+&v"(S.vcrow), [r0] "=&s"(S.r0), [r1] "=&s"(S.r1),
+        [vstg] "+&v"(S.vstg), [vprow] "+&v"(S.vprow), [vcrow] "+&v"(S.vcrow),
+        [prowa] "+&s"(S.prowa), [crowa] "+&s"(S.crowa), [r0] "=&s"(S.r0), [r1] "=&s"(S.r1),
         [wmask] "=&s"(S.wmask), [cstart] "=&s"(S.cstart), [mv0] "=&s"(S.mv0), [mv1] "=&s"(S.mv1),
         [mv2] "=&s"(S.mv2), [mv3] "=&s"(S.mv3), [mvcur] "=&s"(S.mvcur), [lm0] "=&s"(S.lm0), [lm1] "=&s"(S.lm1),
         [ev] "=&s"(ev), [slot] "=&s"(slot), [t1] "=&s"(t1), [t2] "=&s"(t2), [t3] "=&s"(t3), [p] "=&s"(p),
@@ -1537,6 +1568,526 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
         [hi2] "s"(sgpr(hi2)), [wlo] "s"(sgpr(wlo)), [tb] "s"(sgpr(tb)), [dst] "s"(sgpr(dst)),
         [recs] "s"(sgpr(recs))
       : "scc", "vcc", "memory");
+#undef MB_ROW_RD
+#undef MB_ROW_WR
+#undef MB_ROW_SWAP
+  } else {
+#define MB_ROW_RD MB_ROW_RD_LDS
+#define MB_ROW_WR MB_ROW_WR_LDS
+#define MB_ROW_SWAP MB_ROW_SWAP_LDS
+  asm volatile(
+      "s_nop 4\n"
+      "s_mov_b32 %[m0s], m0\n"
+
+      // ---------------------------------------------------------------- MB
+      "L%=_mb:\n"
+      "s_add_u32 %[cstart], %[cbase], %[cn]\n"
+      "s_mov_b32 %[r0], 2\n"
+      "s_mov_b32 %[r1], 0\n"
+      "s_mov_b32 %[wmask], 0\n"
+      "s_mov_b32 %[mv0], 0\n"
+      "s_mov_b32 %[mv1], 0\n"
+      "s_mov_b32 %[mv2], 0\n"
+      "s_mov_b32 %[mv3], 0\n"
+      "s_mov_b32 %[mvcur], 0\n"
+      "s_mov_b32 %[lm0], 0\n"
+      "s_mov_b32 %[lm1], 0\n"
+      "s_mov_b32 %[cbpf], 7\n"
+      "s_cmp_eq_u32 %[interf], 0\n"
+      "s_cbranch_scc1 L%=_tx\n"
+      // macroblock mode (decode_inter.go:9-16)
+      MB_DEC("%[vm0]", "16", "15", "0x100010", "%[sym]", "md", "mode_unused")
+      "s_cmp_eq_u32 %[sym], 0\n"
+      "s_cbranch_scc1 L%=_skip\n"
+      "s_cmp_eq_u32 %[sym], 2\n"
+      "s_cbranch_scc1 L%=_tx\n"
+      // ---- inter macroblock: reference, partition, vectors, pattern
+      "s_mov_b32 %[r0], 1\n"
+      MB_DEC("%[vm1]", "32", "31", "0x100020", "%[sym]", "rf", "unused")
+      "s_mov_b32 %[r1], %[sym]\n"
+      "s_mov_b32 " MB_PRED ", 0\n"
+      "s_cmp_eq_u32 %[sym], 1\n"
+      "s_cbranch_scc1 L%=_nopred\n"
+      "s_mov_b32 %[t3], 1\n"
+      "s_branch L%=_mvp\n"
+      "L%=_nopred:\n"
+      MB_DEC("%[vm1]", "48", "47", "0x100030", "%[sym]", "pt", "unused")
+      "s_lshl_b32 %[t1], %[sym], 8\n"
+      "s_or_b32 %[r0], %[r0], %[t1]\n"
+      "s_lshl_b32 %[t1], %[sym], 2\n"
+      "s_lshr_b32 " MB_NPART ", 0x4221, %[t1]\n"
+      "s_and_b32 " MB_NPART ", " MB_NPART ", 15\n"
+      "s_mov_b32 " MB_PI ", 0\n"
+      "L%=_mvl:\n"
+      "s_cmp_eq_u32 " MB_PI ", 0\n"
+      "s_cselect_b32 %[cnt], " MB_PRED ", %[mv0]\n"
+      // x component: base.x + zigzag(class value), clamped to +-4096
+      MB_CLASS("%[vm2]", "16", "15", "0x100010", "cx")
+      "s_sext_i32_i16 %[t1], %[cnt]\n"
+      "s_lshr_b32 %[t2], %[sym], 1\n"
+      "s_and_b32 %[sym], %[sym], 1\n"
+      "s_sub_u32 %[sym], 0, %[sym]\n"
+      "s_xor_b32 %[t2], %[t2], %[sym]\n"
+      "s_add_u32 %[t1], %[t1], %[t2]\n"
+      "s_max_i32 %[t1], %[t1], -4096\n"
+      "s_min_i32 %[t1], %[t1], 4096\n"
+      "s_and_b32 " MB_MVT ", %[t1], 0xffff\n"
+      MB_CLASS("%[vm2]", "32", "31", "0x100020", "cy")
+      "s_ashr_i32 %[t1], %[cnt], 16\n"
+      "s_lshr_b32 %[t2], %[sym], 1\n"
+      "s_and_b32 %[sym], %[sym], 1\n"
+      "s_sub_u32 %[sym], 0, %[sym]\n"
+      "s_xor_b32 %[t2], %[t2], %[sym]\n"
+      "s_add_u32 %[t1], %[t1], %[t2]\n"
+      "s_max_i32 %[t1], %[t1], -4096\n"
+      "s_min_i32 %[t1], %[t1], 4096\n"
+      "s_lshl_b32 %[t1], %[t1], 16\n"
+      "s_or_b32 " MB_MVT ", " MB_MVT ", %[t1]\n"
+      "s_cmp_eq_u32 " MB_PI ", 0\n"
+      "s_cselect_b32 %[mv0], " MB_MVT ", %[mv0]\n"
+      "s_cmp_eq_u32 " MB_PI ", 1\n"
+      "s_cselect_b32 %[mv1], " MB_MVT ", %[mv1]\n"
+      "s_cmp_eq_u32 " MB_PI ", 2\n"
+      "s_cselect_b32 %[mv2], " MB_MVT ", %[mv2]\n"
+      "s_cmp_eq_u32 " MB_PI ", 3\n"
+      "s_cselect_b32 %[mv3], " MB_MVT ", %[mv3]\n"
+      "s_add_u32 " MB_PI ", " MB_PI ", 1\n"
+      "s_cmp_lt_u32 " MB_PI ", " MB_NPART "\n"
+      "s_cbranch_scc1 L%=_mvl\n"
+      "s_cmp_eq_u32 %[r1], 1\n"
+      "s_cselect_b32 %[mvcur], 0, %[mv0]\n"
+      MB_DEC("%[vm0]", "48", "47", "0x100030", "%[cbpf]", "cb", "unused")
+      "s_lshl_b32 %[t1], %[cbpf], 16\n"
+      "s_or_b32 %[r0], %[r0], %[t1]\n"
+      "s_mov_b32 %[nb], 0\n"
+      "s_bitcmp1_b32 %[cbpf], 0\n"
+      "s_cbranch_scc1 L%=_tx\n"
+      "s_mov_b32 %[wmask], 0xffff\n"
+      "s_branch L%=_blocks\n"
+
+      // ---- transform size; the luma blocks' vectors and scan
+      "L%=_tx:\n"
+      MB_DEC("%[vm0]", "32", "31", "0x100020", "%[sym]", "tx", "unused")
+      "s_lshl_b32 %[t1], %[sym], 24\n"
+      "s_or_b32 %[r0], %[r0], %[t1]\n"
+      "s_cmp_eq_u32 %[sym], 0\n"
+      "s_cbranch_scc0 L%=_tx1\n"
+      "s_mov_b32 %[nb], 16\n"
+      "s_mov_b32 %[total], 16\n"
+      "v_mov_b32 %[va], %[t00]\n"
+      "v_mov_b32 %[vb], %[t01]\n"
+      "v_mov_b32 %[vc], %[t02]\n"
+      "v_mov_b32 %[vzz], %[zz4]\n"
+      "s_branch L%=_blocks\n"
+      "L%=_tx1:\n"
+      "s_cmp_eq_u32 %[sym], 1\n"
+      "s_cbranch_scc0 L%=_tx2\n"
+      "s_mov_b32 %[nb], 4\n"
+      "s_mov_b32 %[total], 64\n"
+      "v_mov_b32 %[va], %[t10]\n"
+      "v_mov_b32 %[vb], %[t11]\n"
+      "v_mov_b32 %[vc], %[t12]\n"
+      "v_mov_b32 %[vzz], %[zz8]\n"
+      "s_branch L%=_blocks\n"
+      "L%=_tx2:\n"
+      "s_mov_b32 %[nb], 1\n"
+      "s_movk_i32 %[total], 0x100\n"
+      "v_mov_b32 %[va], %[t20]\n"
+      "v_mov_b32 %[vb], %[t21]\n"
+      "v_mov_b32 %[vc], %[t22]\n"
+      "v_mov_b32 %[vzz], %[zz16]\n"
+
+      // ---- blocks: luma 0..nb-1, then Cb, Cr (when coded)
+      "L%=_blocks:\n"
+      "s_mov_b32 %[k], 0\n"
+      "L%=_bk:\n"
+      "s_cmp_eq_u32 %[k], %[nb]\n"
+      "s_cbranch_scc0 L%=_bk_nl\n"
+      // luma complete: the chroma mode (intra) or the uncoded chroma planes (inter)
+      "s_and_b32 %[t3], %[r0], 0xff\n"
+      "s_cmp_eq_u32 %[t3], 2\n"
+      "s_cbranch_scc0 L%=_bk_ic\n"
+      MB_DEC("%[vm1]", "16", "15", "0x100010", "%[sym]", "cm", "unused")
+      "s_lshl_b32 %[t1], %[sym], 8\n"
+      "s_or_b32 %[r1], %[r1], %[t1]\n"
+      "s_branch L%=_bk_nl\n"
+      "L%=_bk_ic:\n"
+      "s_bitcmp1_b32 %[cbpf], 1\n"
+      "s_cbranch_scc1 L%=_bk_icr\n"
+      "s_bitset1_b32 %[wmask], 16\n"
+      "L%=_bk_icr:\n"
+      "s_bitcmp1_b32 %[cbpf], 2\n"
+      "s_cbranch_scc1 L%=_bk_nl\n"
+      "s_bitset1_b32 %[wmask], 17\n"
+      "L%=_bk_nl:\n"
+      "s_cmp_lt_u32 %[k], %[nb]\n"
+      "s_cbranch_scc0 L%=_bk_ch\n"
+      "s_lshl_b32 %[blk8], %[k], 8\n"
+      "s_and_b32 %[t3], %[r0], 0xff\n"
+      "s_cmp_eq_u32 %[t3], 2\n"
+      "s_cbranch_scc0 L%=_tb\n"
+      // intra luma mode of block k (4 bits in decode order)
+      MB_DEC("%[vm1]", "0", "-1", "0x100000", "%[sym]", "lm", "unused")
+      "s_and_b32 %[t1], %[k], 7\n"
+      "s_lshl_b32 %[t1], %[t1], 2\n"
+      "s_lshl_b32 %[t1], %[sym], %[t1]\n"
+      "s_cmp_lt_u32 %[k], 8\n"
+      "s_cbranch_scc0 L%=_lmh\n"
+      "s_or_b32 %[lm0], %[lm0], %[t1]\n"
+      "s_branch L%=_tb\n"
+      "L%=_lmh:\n"
+      "s_or_b32 %[lm1], %[lm1], %[t1]\n"
+      "s_branch L%=_tb\n"
+      "L%=_bk_ch:\n"
+      "s_sub_u32 %[t1], %[k], %[nb]\n"
+      "s_add_u32 %[t3], %[t1], 1\n"
+      "s_lshr_b32 %[t3], %[cbpf], %[t3]\n"
+      "s_bitcmp1_b32 %[t3], 0\n"
+      "s_cbranch_scc0 L%=_bk_next\n"
+      "s_add_u32 %[t1], %[t1], 16\n"
+      "s_lshl_b32 %[blk8], %[t1], 8\n"
+      "s_mov_b32 %[total], 64\n"
+      "v_mov_b32 %[va], %[t30]\n"
+      "v_mov_b32 %[vb], %[t31]\n"
+      "v_mov_b32 %[vc], %[t32]\n"
+      "v_mov_b32 %[vzz], %[zz8]\n"
+
+      // ------------------------------------------------------ token block
+      "L%=_tb:\n"
+      "s_mov_b32 %[si], 0\n"
+      "s_mov_b32 %[stop], 1\n"
+      "s_mov_b32 %[boff], 0\n"
+      "s_mov_b32 %[prev], 0\n"
+      "s_mov_b32 %[sumabs], 0\n"
+      "s_mov_b32 %[firstsi], 0\n"
+      "v_mov_b32 %[vcur], %[va]\n"
+      "L%=_tok:\n"
+      "s_and_b32 %[slot], %[x0], 0xfff\n"
+      "s_lshl_b32 %[t1], %[slot], 13\n"
+      "s_or_b32 %[t1], %[t1], 0x1fff\n"
+      "v_cmp_ge_u32_e64 %[b], %[t1], %[vcur]\n"
+      "s_add_u32 %[t2], %[boff], %[prev]\n"
+      "s_or_b32 %[t1], %[t2], 0x80000\n"
+      "s_lshr_b32 %[xn], %[x0], 12\n"
+      "s_bfe_u64 %[b], %[b], %[t1]\n"
+      "s_bcnt1_i32_b64 %[cnt], %[b]\n"
+      "s_cmp_eq_u32 %[cnt], 1\n"
+      "s_cbranch_scc0 L%=_tnz\n"
+      // zero token: symbol 0 starts at 0
+      "v_readlane_b32 %[p], %[vcur], %[t2]\n"
+      "s_mov_b32 %[prev], 0\n"
+      "s_and_b32 %[p], %[p], 0x1fff\n"
+      "s_mul_i32 %[xn], %[p], %[xn]\n"
+      "s_add_u32 %[xn], %[xn], %[slot]\n"
+      MB_RN("tz")
+      "s_add_u32 %[si], %[si], 1\n"
+      "s_cmp_lg_u32 %[si], %[stop]\n"
+      "s_cbranch_scc1 L%=_tok\n"
+      "s_branch L%=_tstop\n"
+      "L%=_tnz:\n"
+      "s_cmp_eq_u32 %[cnt], 0\n"
+      "s_cbranch_scc1 L%=_unused\n"
+      "s_add_u32 %[t1], %[t2], %[cnt]\n"
+      "s_add_u32 %[t1], %[t1], -1\n"
+      "v_readlane_b32 %[p], %[vcur], %[t1]\n"
+      "s_add_u32 " MB_MAG ", %[cnt], -1\n"
+      "s_and_b32 %[t2], %[p], 0x1fff\n"
+      "s_lshr_b32 %[p], %[p], 13\n"
+      "s_mul_i32 %[xn], %[t2], %[xn]\n"
+      "s_sub_u32 %[t2], %[slot], %[p]\n"
+      "s_add_u32 %[xn], %[xn], %[t2]\n"
+      MB_RN("tn")
+      "s_cmp_ge_u32 %[cnt], 6\n"
+      "s_cbranch_scc1 L%=_tescob\n"
+      "s_cmp_eq_u32 " MB_MAG ", 1\n"
+      "s_cselect_b32 %[prev], 8, 16\n"
+      "L%=_tlev:\n"
+      "s_add_u32 %[sumabs], %[sumabs], " MB_MAG "\n"
+      "s_cmp_eq_u32 %[firstsi], 0\n"
+      "s_cbranch_scc1 L%=_tfirst\n"
+      MB_SIGN("ts")
+      MB_ZZ
+      MB_EMIT("e1")
+      "s_mov_b32 %[last], %[si]\n"
+      "s_add_u32 %[si], %[si], 1\n"
+      "s_cmp_lg_u32 %[si], %[stop]\n"
+      "s_cbranch_scc1 L%=_tok\n"
+      "s_branch L%=_tstop\n"
+      // first nonzero level: its sign resolves at the end of the block
+      "L%=_tfirst:\n"
+      MB_ZZ
+      "s_lshl_b32 %[t1], " MB_MAG ", 8\n"
+      "s_or_b32 %[firstpm], %[t1], %[t2]\n"
+      "s_add_u32 %[firstsi], %[si], 1\n"
+      "s_mov_b32 %[last], %[si]\n"
+      "s_add_u32 %[si], %[si], 1\n"
+      "s_cmp_lg_u32 %[si], %[stop]\n"
+      "s_cbranch_scc1 L%=_tok\n"
+      // si reached stop: the end of the block or a band boundary
+      // (bandOf thresholds 1, 3, 9, 21, model.go:165-180)
+      "L%=_tstop:\n"
+      "s_cmp_eq_u32 %[si], %[total]\n"
+      "s_cbranch_scc1 L%=_tend\n"
+      "s_xor_b32 %[boff], %[boff], 24\n"
+      "s_cmp_eq_u32 %[si], 1\n"
+      "s_cselect_b32 %[stop], 3, %[stop]\n"
+      "s_cmp_eq_u32 %[si], 9\n"
+      "s_cselect_b32 %[stop], 21, %[stop]\n"
+      "s_cmp_eq_u32 %[si], 3\n"
+      "s_cbranch_scc0 L%=_tb2\n"
+      "v_mov_b32 %[vcur], %[vb]\n"
+      "s_mov_b32 %[stop], 9\n"
+      "L%=_tb2:\n"
+      "s_cmp_eq_u32 %[si], 21\n"
+      "s_cbranch_scc0 L%=_tb4\n"
+      "v_mov_b32 %[vcur], %[vc]\n"
+      "s_mov_b32 %[stop], %[total]\n"
+      "L%=_tb4:\n"
+      "s_min_u32 %[stop], %[stop], %[total]\n"
+      "s_branch L%=_tok\n"
+      // escape (magnitude 5 + getEscapeValue, mv.go:124-128) or end of block
+      "L%=_tescob:\n"
+      "s_cmp_eq_u32 %[cnt], 7\n"
+      "s_cbranch_scc1 L%=_tend\n"
+      MB_CLASS("%[vm2]", "0", "-1", "0x100000", "es")
+      "s_add_u32 " MB_MAG ", %[sym], 5\n"
+      "s_mov_b32 %[prev], 16\n"
+      "s_branch L%=_tlev\n"
+      // end of block: the first level's sign, hidden in the level-sum parity
+      // when last - first >= 4, else explicit
+      "L%=_tend:\n"
+      "s_cmp_eq_u32 %[firstsi], 0\n"
+      "s_cbranch_scc1 L%=_tbdone\n"
+      "s_lshr_b32 " MB_MAG ", %[firstpm], 8\n"
+      "s_add_u32 %[t1], %[firstsi], -1\n"
+      "s_sub_u32 %[t1], %[last], %[t1]\n"
+      "s_cmp_lt_i32 %[t1], 4\n"
+      "s_cbranch_scc0 L%=_tpar\n"
+      MB_SIGN("sd")
+      "s_branch L%=_temit\n"
+      "L%=_tpar:\n"
+      "s_bitcmp1_b32 %[sumabs], 0\n"
+      "s_cbranch_scc0 L%=_temit\n"
+      "s_sub_u32 " MB_MAG ", 0, " MB_MAG "\n"
+      "L%=_temit:\n"
+      "s_and_b32 %[t2], %[firstpm], 0xff\n"
+      MB_EMIT("e2")
+      "L%=_tbdone:\n"
+      "s_lshr_b32 %[t1], %[blk8], 8\n"
+      "s_lshl_b32 %[t1], 1, %[t1]\n"
+      "s_or_b32 %[wmask], %[wmask], %[t1]\n"
+      "L%=_bk_next:\n"
+      "s_add_u32 %[k], %[k], 1\n"
+      "s_add_u32 %[t1], %[nb], 2\n"
+      "s_cmp_lt_u32 %[k], %[t1]\n"
+      "s_cbranch_scc1 L%=_bk\n"
+      "s_branch L%=_mbend\n"
+
+      // ---- skip macroblock: the predicted vector, every block written
+      "L%=_skip:\n"
+      "s_mov_b32 %[t3], 0\n"
+      "s_branch L%=_mvp\n"
+      "L%=_skipmv:\n"
+      "s_mov_b32 %[r0], 0\n"
+      "s_mov_b32 %[mv0], " MB_PRED "\n"
+      "s_mov_b32 %[mvcur], " MB_PRED "\n"
+      "s_mov_b32 %[wmask], 0x3ffff\n"
+      "s_branch L%=_mbend\n"
+
+      // ---- predictMV (mv.go:21-52) into pred; t3 = 0 skip, 1 inter, 2 error
+      "L%=_mvp:\n"
+      "s_sub_u32 %[t1], %[mbx], %[x0t]\n"
+      "s_cmp_gt_u32 %[mby], %[y0t]\n"
+      "s_cbranch_scc0 L%=_mvp_nob\n"
+      MB_ROW_RD("%[t2]", "%[t1]")
+      "s_add_u32 %[p], %[t1], 1\n"
+      "s_sub_u32 %[xn], %[x1t], %[x0t]\n"
+      "s_cmp_lt_u32 %[p], %[xn]\n"
+      "s_cbranch_scc0 L%=_mvp_cl\n"
+      MB_ROW_RD("%[xn]", "%[p]")
+      "s_branch L%=_mvp_med\n"
+      "L%=_mvp_cl:\n"
+      "s_cmp_eq_u32 %[t1], 0\n"
+      "s_cbranch_scc1 L%=_mvp_b\n"
+      "s_add_u32 %[p], %[t1], -1\n"
+      MB_ROW_RD("%[xn]", "%[p]")
+      "L%=_mvp_med:\n"
+      "s_cmp_eq_u32 %[t1], 0\n"
+      "s_cselect_b32 %[p], 0, %[leftmv]\n"
+      // component-wise median of a (p), b (t2), c (xn)
+      "s_sext_i32_i16 %[slot], %[p]\n"
+      "s_sext_i32_i16 %[cnt], %[t2]\n"
+      "s_sext_i32_i16 %[k2], %[xn]\n"
+      "s_min_i32 %[t1], %[slot], %[cnt]\n"
+      "s_max_i32 %[slot], %[slot], %[cnt]\n"
+      "s_min_i32 %[slot], %[slot], %[k2]\n"
+      "s_max_i32 %[t1], %[t1], %[slot]\n"
+      "s_ashr_i32 %[slot], %[p], 16\n"
+      "s_ashr_i32 %[cnt], %[t2], 16\n"
+      "s_ashr_i32 %[k2], %[xn], 16\n"
+      "s_min_i32 %[p], %[slot], %[cnt]\n"
+      "s_max_i32 %[slot], %[slot], %[cnt]\n"
+      "s_min_i32 %[slot], %[slot], %[k2]\n"
+      "s_max_i32 %[p], %[p], %[slot]\n"
+      "s_and_b32 %[t1], %[t1], 0xffff\n"
+      "s_lshl_b32 %[p], %[p], 16\n"
+      "s_or_b32 " MB_PRED ", %[t1], %[p]\n"
+      "s_branch L%=_mvp_end\n"
+      "L%=_mvp_b:\n"
+      "s_mov_b32 " MB_PRED ", %[t2]\n"
+      "s_branch L%=_mvp_end\n"
+      "L%=_mvp_nob:\n"
+      "s_cmp_eq_u32 %[t1], 0\n"
+      "s_cselect_b32 " MB_PRED ", 0, %[leftmv]\n"
+      "L%=_mvp_end:\n"
+      "s_cmp_eq_u32 %[t3], 0\n"
+      "s_cbranch_scc1 L%=_skipmv\n"
+      "s_cmp_eq_u32 %[t3], 1\n"
+      "s_cbranch_scc1 L%=_nopred\n"
+      "s_branch L%=_modeerr_mv\n"
+
+      // ---- end of a macroblock: MV row, record, next macroblock
+      "L%=_mbend:\n"
+      "s_and_b32 %[t3], %[r0], 0xff\n"
+      "s_cmp_eq_u32 %[t3], 2\n"
+      "s_cselect_b32 %[t3], 1, 0\n"
+      "s_add_u32 %[nintra], %[nintra], %[t3]\n"
+      "s_sub_u32 %[t1], %[mbx], %[x0t]\n"
+      MB_ROW_WR
+      "s_mov_b32 %[leftmv], %[mvcur]\n"
+      "s_cmp_eq_u32 %[nst], 0\n"
+      "s_cbranch_scc0 L%=_rec\n"
+      "s_mul_i32 %[firstmb], %[mby], %[mbc]\n"
+      "s_add_u32 %[firstmb], %[firstmb], %[mbx]\n"
+      "L%=_rec:\n"
+      "s_mul_i32 %[t2], %[nst], 12\n"
+      "s_add_u32 %[t1], %[tcb], %[cstart]\n"
+      "s_add_u32 %[t3], %[cbase], %[cn]\n"
+      "s_sub_u32 %[t3], %[t3], %[cstart]\n"
+      MB_REC(0, "%[r0]")
+      MB_REC(1, "%[r1]")
+      MB_REC(2, "%[wmask]")
+      MB_REC(3, "%[t1]")
+      MB_REC(4, "%[t3]")
+      MB_REC(5, "%[mv0]")
+      MB_REC(6, "%[mv1]")
+      MB_REC(7, "%[mv2]")
+      MB_REC(8, "%[mv3]")
+      MB_REC(9, "%[lm0]")
+      MB_REC(10, "%[lm1]")
+      MB_REC(11, "0")
+      "s_add_u32 %[done], %[done], 1\n"
+      "s_add_u32 %[nst], %[nst], 1\n"
+      "s_cmp_eq_u32 %[nst], 5\n"
+      "s_cbranch_scc0 L%=_next\n"
+      "s_mov_b32 %[t3], 0\n"
+      "s_branch L%=_rflush\n"
+      "L%=_next:\n"
+      "s_add_u32 %[mbx], %[mbx], 1\n"
+      "s_cmp_lt_u32 %[mbx], %[x1t]\n"
+      "s_cbranch_scc1 L%=_mb\n"
+      // end of a row: its records out, the rows swap
+      "s_mov_b32 %[t3], 1\n"
+      "s_cmp_eq_u32 %[nst], 0\n"
+      "s_cbranch_scc0 L%=_rflush\n"
+      "L%=_rowend:\n"
+      MB_ROW_SWAP
+      "s_mov_b32 %[mbx], %[x0t]\n"
+      "s_add_u32 %[mby], %[mby], 1\n"
+      "s_cmp_lt_u32 %[mby], %[y1t]\n"
+      "s_cbranch_scc1 L%=_mb\n"
+      "s_mov_b32 %[ev], 0\n"
+      "s_branch L%=_exit\n"
+      // staged records out: lanes [0, 12 nst) to recs[firstmb ..]
+      // (t3: 0 continue with the next macroblock, 1 end of row)
+      "L%=_rflush:\n"
+      "s_mul_i32 %[t1], %[nst], 12\n"
+      "s_bfm_b64 %[b], %[t1], 0\n"
+      "s_mul_i32 %[t2], %[firstmb], 48\n"
+      "v_add_u32 %[vaddr], %[t2], %[vlane4]\n"
+      "s_mov_b64 exec, %[b]\n"
+      "global_store_dword %[vaddr], %[vstg], %[recs]\n"
+      "s_mov_b64 exec, -1\n"
+      "s_mov_b32 %[nst], 0\n"
+      "s_cmp_eq_u32 %[t3], 0\n"
+      "s_cbranch_scc1 L%=_next\n"
+      "s_branch L%=_rowend\n"
+
+      // ---- decode errors: the failing macroblock's fields stay as decoded
+      "L%=_mode_unused:\n"
+      "s_mov_b32 %[ev], 3\n"
+      "s_branch L%=_modeerr\n"
+      "L%=_mode_underrun:\n"
+      "s_mov_b32 %[ev], 4\n"
+      // a failed mode reads as a skip macroblock (Go reconstructs it)
+      "L%=_modeerr:\n"
+      "s_mov_b32 %[t3], 2\n"
+      "s_branch L%=_mvp\n"
+      "L%=_modeerr_mv:\n"
+      "s_mov_b32 %[r0], 0\n"
+      "s_mov_b32 %[mv0], " MB_PRED "\n"
+      "s_mov_b32 %[mvcur], " MB_PRED "\n"
+      "s_mov_b32 %[wmask], 0x3ffff\n"
+      "s_branch L%=_exit\n"
+      "L%=_unused:\n"
+      "s_mov_b32 %[ev], 3\n"
+      "s_branch L%=_exit\n"
+      "L%=_underrun:\n"
+      "s_mov_b32 %[ev], 4\n"
+      "s_branch L%=_exit\n"
+
+      // ---- out-of-line renormalisation and coefficient stores
+      MB_RN_OOL("md", "mode_underrun")
+      MB_RN_OOL("rf", "underrun")
+      MB_RN_OOL("pt", "underrun")
+      MB_RN_OOL("cxd", "underrun")
+      MB_RN_OOL("cxb", "underrun")
+      MB_RN_OOL("cyd", "underrun")
+      MB_RN_OOL("cyb", "underrun")
+      MB_RN_OOL("cb", "underrun")
+      MB_RN_OOL("tx", "underrun")
+      MB_RN_OOL("cm", "underrun")
+      MB_RN_OOL("lm", "underrun")
+      MB_RN_OOL("tz", "underrun")
+      MB_RN_OOL("tn", "underrun")
+      MB_RN_OOL("ts", "underrun")
+      MB_RN_OOL("esd", "underrun")
+      MB_RN_OOL("esb", "underrun")
+      MB_RN_OOL("sd", "underrun")
+      MB_EMIT_OOL("e1")
+      MB_EMIT_OOL("e2")
+
+      "L%=_exit:\n"
+      "s_mov_b32 m0, %[m0s]\n"
+      "s_nop 4\n"
+      : [x0] "+&s"(S.x0), [x1] "+&s"(S.x1), [x2] "+&s"(S.x2), [x3] "+&s"(S.x3), [pos] "+&s"(S.pos),
+        [nsym] "+&s"(S.nsym), [wbase] "+&s"(S.wbase), [lim] "+&s"(S.lim), [cn] "+&s"(S.cn), [cbase] "+&s"(S.cbase),
+        [mbx] "+&s"(S.mbx), [mby] "+&s"(S.mby), [done] "+&s"(S.done), [nintra] "+&s"(S.nintra), [nst] "+&s"(S.nst),
+        [firstmb] "+&s"(S.firstmb), [leftmv] "+&s"(S.leftmv), [vwin] "+&v"(S.vwin), [vbuf] "+&v"(S.vbuf),
+        [vstg] "+&v"(S.vstg), [vprow] "+&v"(S.vprow), [vcrow] "+&v"(S.vcrow),
+        [prowa] "+&s"(S.prowa), [crowa] "+&s"(S.crowa), [r0] "=&s"(S.r0), [r1] "=&s"(S.r1),
+        [wmask] "=&s"(S.wmask), [cstart] "=&s"(S.cstart), [mv0] "=&s"(S.mv0), [mv1] "=&s"(S.mv1),
+        [mv2] "=&s"(S.mv2), [mv3] "=&s"(S.mv3), [mvcur] "=&s"(S.mvcur), [lm0] "=&s"(S.lm0), [lm1] "=&s"(S.lm1),
+        [ev] "=&s"(ev), [slot] "=&s"(slot), [t1] "=&s"(t1), [t2] "=&s"(t2), [t3] "=&s"(t3), [p] "=&s"(p),
+        [xn] "=&s"(xn), [cnt] "=&s"(cnt), [sym] "=&s"(sym), [val] "=&s"(val), [k2] "=&s"(k2), [k] "=&s"(k),
+        [nb] "=&s"(nb), [total] "=&s"(total), [blk8] "=&s"(blk8), [cbpf] "=&s"(cbpf), [si] "=&s"(si),
+        [stop] "=&s"(stop), [boff] "=&s"(boff), [prev] "=&s"(prev), [sumabs] "=&s"(sumabs),
+        [firstsi] "=&s"(firstsi), [firstpm] "=&s"(firstpm), [last] "=&s"(last), [m0s] "=&s"(m0s),
+        [b] "=&s"(b), [vcur] "=&v"(vcur), [va] "=&v"(va), [vb] "=&v"(vb), [vc] "=&v"(vc), [vzz] "=&v"(vzz),
+        [vaddr] "=&v"(vaddr), [vt] "=&v"(vt)
+      : [vm0] "v"(T.misc0), [vm1] "v"(T.misc1), [vm2] "v"(T.misc2), [t00] "v"(T.t00), [t01] "v"(T.t01),
+        [t02] "v"(T.t02), [t10] "v"(T.t10), [t11] "v"(T.t11), [t12] "v"(T.t12), [t20] "v"(T.t20),
+        [t21] "v"(T.t21), [t22] "v"(T.t22), [t30] "v"(T.t30), [t31] "v"(T.t31), [t32] "v"(T.t32),
+        [zz4] "v"(zz4), [zz8] "v"(zz8), [zz16] "v"(zz16), [vlane4] "v"(vlane4), [x0t] "s"(sgpr(x0t)),
+        [x1t] "s"(sgpr(x1t)), [y0t] "s"(sgpr(y0t)), [y1t] "s"(sgpr(y1t)), [mbc] "s"(sgpr(mbc)),
+        [interf] "s"(sgpr(interf)), [tcb] "s"(sgpr(tcb)), [f0] "s"(sgpr(f0)), [f1] "s"(sgpr(f1)),
+        [hi2] "s"(sgpr(hi2)), [wlo] "s"(sgpr(wlo)), [tb] "s"(sgpr(tb)), [dst] "s"(sgpr(dst)),
+        [recs] "s"(sgpr(recs))
+      : "scc", "vcc", "memory");
+#undef MB_ROW_RD
+#undef MB_ROW_WR
+#undef MB_ROW_SWAP
+  }
   return ev;
 }
 
@@ -1736,8 +2287,8 @@ __device__ __forceinline__ void parse_tile(const uint8_t* __restrict__ pkt, cons
 }
 
 
-// decodeTile through mb_loop (tiles at most 64 macroblocks wide with a sign
-// table); same outputs as parse_tile.
+// decodeTile through mb_loop (frames with a sign table); same outputs as
+// parse_tile.
 __device__ __forceinline__ void parse_tile_asm(const uint8_t* __restrict__ pkt, const uint8_t* __restrict__ desc,
                                            MBRec* __restrict__ recs, uint32_t* __restrict__ coefs,
                                            TileStatus* __restrict__ status, const int t) {
@@ -1838,12 +2389,19 @@ __device__ __forceinline__ void parse_tile_asm(const uint8_t* __restrict__ pkt, 
     S.cn = 0; S.cbase = 0;
     S.mbx = td.x0; S.mby = td.y0; S.done = 0; S.nintra = 0; S.nst = 0; S.firstmb = 0; S.leftmv = 0;
     S.vwin = R.win.cur; S.vbuf = 0; S.vstg = 0; S.vprow = 0; S.vcrow = 0;
+    S.prowa = sgpr(static_cast<uint32_t>(reinterpret_cast<uintptr_t>(prow)));
+    S.crowa = sgpr(static_cast<uint32_t>(reinterpret_cast<uintptr_t>(crow)));
     S.x0 = sgpr(S.x0); S.x1 = sgpr(S.x1); S.x2 = sgpr(S.x2); S.x3 = sgpr(S.x3); S.pos = sgpr(S.pos);
     S.nsym = sgpr(S.nsym); S.wbase = sgpr(S.wbase); S.lim = sgpr(S.lim); S.mbx = sgpr(S.mbx); S.mby = sgpr(S.mby);
-    const uint32_t ev = mb_loop(S, T, zz4, zz8, zz16, F.vlane4, td.x0, td.x1, td.y0, td.y1, static_cast<uint32_t>(mbc),
-                                inter_frame ? 1u : 0u, td.coef_base, F.f0, F.f1,
-                                static_cast<uint32_t>(R.win.hi - R.win.lo - 4), static_cast<uint32_t>(R.win.lo),
-                                R.win.tile + R.win.lo, out.dst, recs);
+    // MV rows in two VGPRs up to 64 macroblocks of tile width, else in LDS
+    const uint32_t hi2 = static_cast<uint32_t>(R.win.hi - R.win.lo - 4), wlo = static_cast<uint32_t>(R.win.lo);
+    const uint32_t ev =
+        tw <= 64 ? mb_loop<false>(S, T, zz4, zz8, zz16, F.vlane4, td.x0, td.x1, td.y0, td.y1,
+                                  static_cast<uint32_t>(mbc), inter_frame ? 1u : 0u, td.coef_base, F.f0, F.f1, hi2,
+                                  wlo, R.win.tile + R.win.lo, out.dst, recs)
+                 : mb_loop<true>(S, T, zz4, zz8, zz16, F.vlane4, td.x0, td.x1, td.y0, td.y1,
+                                 static_cast<uint32_t>(mbc), inter_frame ? 1u : 0u, td.coef_base, F.f0, F.f1, hi2,
+                                 wlo, R.win.tile + R.win.lo, out.dst, recs);
     R.x0 = S.x0; R.x1 = S.x1; R.x2 = S.x2; R.x3 = S.x3; R.pos = S.pos; R.nsym = S.nsym;
     R.win.wbase = static_cast<int32_t>(S.wbase); R.win.lim = static_cast<int32_t>(S.lim); R.win.cur = S.vwin;
     out.cn = S.cn; out.base = S.cbase; out.buf = S.vbuf;
@@ -1875,7 +2433,7 @@ __global__ __launch_bounds__(64) void ebiv_parse(const FrameJob* __restrict__ jo
   const FrameHdr* hdr = reinterpret_cast<const FrameHdr*>(job.desc);
   const TileDesc* tiles = reinterpret_cast<const TileDesc*>(job.desc + desc_tiles_off());
   const int t = static_cast<int>(it.idx);
-  if (tiles[t].x1 - tiles[t].x0 <= 64 && hdr->vt[0][kGSign] != kNoSym)
+  if (hdr->vt[0][kGSign] != kNoSym)
     parse_tile_asm(job.pkt, job.desc, job.recs, job.coefs, job.status, t);
   else
     parse_tile(job.pkt, job.desc, job.recs, job.coefs, job.status, t);

@@ -77,6 +77,9 @@ def run_clip(clip, frames=None, label=""):
     dict(workload="synthframe", w=64, h=48, frames=3, qp=20, gop=1),
     dict(workload="rects", w=96, h=64, frames=6, qp=22, gop=6, tile_cols=2, tile_rows=2),
     dict(workload="parallax", w=256, h=128, frames=6, qp=22, gop=6, tile_cols=2, tile_rows=1),
+    # single tiles wider than 64 macroblocks (MV rows in LDS), 65 and 72 wide
+    dict(workload="rects", w=1040, h=64, frames=4, qp=22, gop=4, tile_cols=1, tile_rows=1),
+    dict(workload="synthpan", w=1152, h=80, frames=5, qp=18, gop=5, tile_cols=1, tile_rows=1),
 ])
 def test_parse_matches_oracle(case):
     c = dict(case)
