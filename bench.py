@@ -172,12 +172,13 @@ def run_batched(packets, dev, want, sessions, window, steps, warmup):
             "symbols": s1["symbols"] - s0["symbols"]}
 
 
-SERVE_STREAMS = ("rects_360p", "rects_720p", "parallax_1080p", "rects_2160p")
+SERVE_STREAMS = ("rects_360p", "rects_720p", "rects_1080p", "rects_2160p")
 
 
 def run_serving(requests_n, budget_bytes, window, check_every, seed=5):
     """The serving mix: Poisson arrivals (200 requests/s) over the 360p, 720p,
-    1080p and 2160p streams; 20% seek to a random key frame part-way, 10% loop
+    1080p (single tile each) and 2160p (8x6 tiles) streams of the CPU-case
+    content at q22, gop 30; 20% seek to a random key frame part-way, 10% loop
     restarts; lengths 30-600 frames; sessions admitted under an HBM budget.
     Every `check_every`-th completed frame is compared with the oracle's
     straight-through decode of that frame."""
@@ -245,7 +246,8 @@ def main():
     ap.add_argument("--serve-only", action="store_true", help="run only the serving mix")
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-budget-gb", type=float, default=24.0)
-    ap.add_argument("--serve-window", type=int, default=4, help="frames per session per engine step when serving")
+    ap.add_argument("--serve-window", type=int, default=1,
+                    help="frames per session per engine step when serving (1 keeps p99 frame latency lowest)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this decoder runs on one GPU; runs on several GPUs are not part of this "

@@ -4,9 +4,10 @@ encoder, deterministically (seeded), so GPU runs need not re-encode them.
   parallax_1080p.ebiv — 1920x1080, 120 frames: three Perlin parallax layers
   (0.5 / 1 / 2 px per frame), 2-frame flash every 24 frames, noise sigma 1.5;
   q22, gop 60, 4x3 tiles, single-pass encode.
-  rects_360p / rects_720p / rects_2160p.ebiv — the serving mix's sizes with
-  the CPU-case content (gradient, 4 moving noise rectangles, sigma 2): q22,
-  gop 30; 1x1 tiles at 640x360 and 1280x720, 8x6 at 3840x2160.
+  rects_360p / rects_720p / rects_1080p / rects_2160p.ebiv — the serving
+  mix's sizes with the CPU-case content (gradient, 4 moving noise
+  rectangles, sigma 2): q22, gop 30; 1x1 tiles at 640x360, 1280x720 and
+  1920x1080, 8x6 at 3840x2160.
 
 Run: python3 bench_data/make_streams.py [name ...]
 """
@@ -28,6 +29,8 @@ STREAMS = {
                        enc=dict(qp=22, gop=30, tile_cols=1, tile_rows=1)),
     "rects_720p": dict(workload="rects", w=1280, h=720, frames=120, seed=0,
                        enc=dict(qp=22, gop=30, tile_cols=1, tile_rows=1)),
+    "rects_1080p": dict(workload="rects", w=1920, h=1080, frames=120, seed=0,
+                        enc=dict(qp=22, gop=30, tile_cols=1, tile_rows=1)),
     "rects_2160p": dict(workload="rects", w=3840, h=2160, frames=60, seed=0,
                         enc=dict(qp=22, gop=30, tile_cols=8, tile_rows=6)),
 }
