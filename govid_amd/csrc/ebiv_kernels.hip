@@ -2652,6 +2652,113 @@ __device__ __forceinline__ uint32_t part_rect(int part, int pi) {
   }
 }
 
+// 4-tap luma filter rows (motioncomp.go:30-35) as packed int8 for v_dot4.
+__constant__ uint32_t c_taps8[4] = {0x00004000u, 0xFD1237FAu, 0xF82828F8u, 0xFA3712FDu};
+
+__device__ __forceinline__ uint32_t clamp_pack4(const int32_t (&v)[4]) {
+  return static_cast<uint32_t>(clamp255(v[0])) | (static_cast<uint32_t>(clamp255(v[1])) << 8) |
+         (static_cast<uint32_t>(clamp255(v[2])) << 16) | (static_cast<uint32_t>(clamp255(v[3])) << 24);
+}
+
+// Luma of one partition away from the plane's left and right edges
+// (motioncomp.go:49-78): each lane makes 4 samples of one row.  The source
+// row segment is read as aligned dwords, samples are biased by 128 into
+// signed bytes, and each horizontal tap sum is one v_dot4 plus 8192
+// (the taps sum to 64); rows are clamped to the padded plane as in
+// clampCoord.  The vertical sum and (v + 2048) >> 12 follow, so every phase
+// equals the reference's 1-D, 2-D and copy forms bit for bit; the full-pel
+// copy takes the bytes directly.  Residual and write mask as in the general
+// path.
+__device__ __forceinline__ void luma_part_fast(const Planes& src, const Planes& rec, const WaveScratch& ws,
+                                               bool luma_res, int n, uint32_t wmask, int mbx, int mby, int px, int py,
+                                               int pwd, int pht, int mvx, int mvy, int lane) {
+  const int r = lane >> 2, c = (lane & 3) * 4;
+  if (r >= pht || c >= pwd) return;
+  const int y = py + r, x = px + c;  // macroblock-relative
+  const int blk = (y / n) * (16 / n) + x / n;
+  if (!((wmask >> blk) & 1u)) return;
+  const int fx = mvx & 3, fy = mvy & 3;
+  const int sy = mby * 16 + y + (mvy >> 2);
+  const int sx = mbx * 16 + x + (mvx >> 2) - 1;  // first tap of sample 0
+  const int sh = sx & 3;                          // the same for every lane
+  const uint8_t* col = src.y + (sx & ~3);
+  int32_t v[4];
+  if (fx == 0 && fy == 0) {
+    const uint32_t* row = reinterpret_cast<const uint32_t*>(col + static_cast<size_t>(clampc(sy, src.ph)) * src.pw);
+    const uint32_t d0 = row[0], d1 = row[1];
+    const uint32_t w = sh == 3 ? d1 : __builtin_amdgcn_alignbyte(d1, d0, sh + 1);
+#pragma unroll
+    for (int j = 0; j < 4; j++) v[j] = static_cast<int32_t>((w >> (8 * j)) & 0xFFu);
+  } else {
+    const int32_t tx = static_cast<int32_t>(c_taps8[fx]);
+    int32_t acc[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int32_t ty = c_taps[fy][k];
+      if (ty == 0) continue;
+      const uint32_t* row =
+          reinterpret_cast<const uint32_t*>(col + static_cast<size_t>(clampc(sy - 1 + k, src.ph)) * src.pw);
+      const uint32_t d0 = row[0] ^ 0x80808080u, d1 = row[1] ^ 0x80808080u, d2 = row[2] ^ 0x80808080u;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int s = sh + j;
+        const uint32_t w = s < 4 ? __builtin_amdgcn_alignbyte(d1, d0, s) : __builtin_amdgcn_alignbyte(d2, d1, s - 4);
+        acc[j] += ty * __builtin_amdgcn_sdot4(static_cast<int32_t>(w), tx, 8192, false);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) v[j] = clamp255((acc[j] + 2048) >> 12);
+  }
+  if (luma_res) {
+    const int32_t* res = ws.coef + blk * n * n + (y % n) * n + (x % n);
+#pragma unroll
+    for (int j = 0; j < 4; j++) v[j] += res[j];
+  }
+  *reinterpret_cast<uint32_t*>(rec.y + static_cast<size_t>(mby * 16 + y) * rec.pw + mbx * 16 + x) = clamp_pack4(v);
+}
+
+// Chroma of one partition away from the plane's left and right edges
+// (motioncomp.go:82-97): lanes 0-15 Cb, 16-31 Cr, 4 samples of one row each.
+// The eighth-pel bilinear sum is evaluated as two horizontal 2-tap sums
+// (one v_dot4 each, weights 8 - fx and fx, bias 128 * 8) and the vertical
+// blend (8 - fy, fy); (sum + 32) >> 6 is the reference's rounding.
+__device__ __forceinline__ void chroma_part_fast(const Planes& src, const Planes& rec, const WaveScratch& ws,
+                                                 bool has_res, uint32_t cbp, uint32_t wmask, int mbx, int mby,
+                                                 int cpx, int cpy, int cpw, int cph, int mvx, int mvy, int lane) {
+  const int pl = lane >> 4, r = (lane >> 1) & 7, c = (lane & 1) * 4;
+  if (lane >= 32 || r >= cph || c >= cpw) return;
+  if (!((wmask >> (16 + pl)) & 1u)) return;
+  const int fx = mvx & 7, fy = mvy & 7;
+  const int y = cpy + r, x = cpx + c;  // macroblock-relative chroma
+  const int sy = mby * 8 + y + (mvy >> 3);
+  const int sx = mbx * 8 + x + (mvx >> 3);
+  const int sh = sx & 3;
+  const uint8_t* plane = pl ? src.cr : src.cb;
+  const uint32_t* r0 = reinterpret_cast<const uint32_t*>(plane + static_cast<size_t>(clampc(sy, src.ch)) * src.cw + (sx & ~3));
+  const uint32_t* r1 =
+      reinterpret_cast<const uint32_t*>(plane + static_cast<size_t>(clampc(sy + 1, src.ch)) * src.cw + (sx & ~3));
+  const uint32_t a0 = r0[0] ^ 0x80808080u, a1 = r0[1] ^ 0x80808080u;
+  const uint32_t b0 = r1[0] ^ 0x80808080u, b1 = r1[1] ^ 0x80808080u;
+  const int32_t tx = (8 - fx) | (fx << 8);
+  int32_t v[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int s = sh + j;
+    const uint32_t wa = s < 4 ? __builtin_amdgcn_alignbyte(a1, a0, s) : __builtin_amdgcn_alignbyte(a1, a1, s - 4);
+    const uint32_t wb = s < 4 ? __builtin_amdgcn_alignbyte(b1, b0, s) : __builtin_amdgcn_alignbyte(b1, b1, s - 4);
+    const int32_t h0 = __builtin_amdgcn_sdot4(static_cast<int32_t>(wa), tx, 1024, false);
+    const int32_t h1 = __builtin_amdgcn_sdot4(static_cast<int32_t>(wb), tx, 1024, false);
+    v[j] = ((8 - fy) * h0 + fy * h1 + 32) >> 6;
+  }
+  if (has_res && (cbp & (pl ? kCBPCr : kCBPCb))) {
+    const int32_t* res = ws.coef + 256 + pl * 64 + y * 8 + x;
+#pragma unroll
+    for (int j = 0; j < 4; j++) v[j] += res[j];
+  }
+  uint8_t* dp = pl ? rec.cr : rec.cb;
+  *reinterpret_cast<uint32_t*>(dp + static_cast<size_t>(mby * 8 + y) * rec.cw + mbx * 8 + x) = clamp_pack4(v);
+}
+
 // One skip or inter macroblock (reconMCMB / reconInterMB / reconInterChroma,
 // decode_inter.go:64-152): per partition, the reference window (the
 // partition plus the 4-tap margin, coordinates clamped to the padded plane,
@@ -2662,8 +2769,8 @@ __device__ __forceinline__ uint32_t part_rect(int part, int pi) {
 // added and only the blocks of the write mask are stored.
 __device__ __forceinline__ void recon_inter_mb(const uint8_t* __restrict__ desc, const MBRec* __restrict__ recs,
                                                const uint32_t* __restrict__ coefs,
-                                               const TileStatus* __restrict__ status, const Planes rec,
-                                               const Planes ref, const Planes gold, const int first_mb) {
+                                               const TileStatus* __restrict__ status, ConstJob& job,
+                                               const int first_mb) {
   __shared__ InterScratch scratch[4];
   __shared__ int32_t s_dct[768];
   LdsI32* dct = (LdsI32*)s_dct;
@@ -2683,11 +2790,18 @@ __device__ __forceinline__ void recon_inter_mb(const uint8_t* __restrict__ desc,
   if (r.kind == 2) return;
   InterScratch& S = scratch[wave];
   WaveScratch& ws = S.ws;
-  const Planes& src = (r.kind == 1 && r.refsel == kRefGolden) ? gold : ref;
+  // (the reference is chosen by address in constant memory: selecting
+  // between two copies of the plane table would put them in scratch)
+  const bool golden = sgpr(static_cast<uint32_t>(r.kind == 1 && r.refsel == kRefGolden)) != 0;
+  const Planes rec = planes_of(job.rec);
+  const Planes src = planes_of(golden ? job.gold : job.ref);
   const int n = 4 << (r.tx > 2 ? 2 : r.tx);
   const bool has_res = r.kind == 1 && r.coef_count > 0;
   const bool luma_res = has_res && (r.cbp & kCBPLuma);
-  if (has_res) residual_mb(ws, dct, coefs, r, n, hdr->qdc, hdr->qac, lane);
+  if (has_res) {
+    residual_mb(ws, dct, coefs, r, n, hdr->qdc, hdr->qac, lane);
+    wave_sync();  // the residual is read across lanes below
+  }
   const int part = r.kind == 1 ? r.part : 0;
   const int npart = static_cast<int>((kPartCountNibbles >> (4 * part)) & 15u);
   for (int pi = 0; pi < npart; pi++) {
@@ -2698,7 +2812,9 @@ __device__ __forceinline__ void recon_inter_mb(const uint8_t* __restrict__ desc,
     const uint32_t mvw = reinterpret_cast<const uint32_t*>(recs[mb].mv)[pi];
     const int mvx = static_cast<int16_t>(mvw & 0xFFFFu), mvy = static_cast<int16_t>(mvw >> 16);
     // ---- luma
-    {
+    if (const int sx0 = mbx * 16 + px + (mvx >> 2) - 1; sx0 >= 0 && sx0 + pwd + 8 <= src.pw) {
+      luma_part_fast(src, rec, ws, luma_res, n, r.wmask, mbx, mby, px, py, pwd, pht, mvx, mvy, lane);
+    } else {
       const int fx = mvx & 3, fy = mvy & 3;
       const int ox = mbx * 16 + px + (mvx >> 2) - 1, oy = mby * 16 + py + (mvy >> 2) - 1;
       const int ww = pwd + 3, wh = pht + 3;
@@ -2737,7 +2853,10 @@ __device__ __forceinline__ void recon_inter_mb(const uint8_t* __restrict__ desc,
       }
     }
     // ---- chroma (eighth-pel bilinear, the same vector in chroma units)
-    {
+    if (const int csx = mbx * 8 + (px >> 1) + (mvx >> 3); csx >= 0 && csx + (pwd >> 1) + 8 <= src.cw) {
+      chroma_part_fast(src, rec, ws, has_res, r.cbp, r.wmask, mbx, mby, px >> 1, py >> 1, pwd >> 1, pht >> 1, mvx,
+                       mvy, lane);
+    } else {
       const int fx = mvx & 7, fy = mvy & 7;
       const int cpx = px >> 1, cpy = py >> 1, cpw = pwd >> 1, cph = pht >> 1;
       const int ox = mbx * 8 + cpx + (mvx >> 3), oy = mby * 8 + cpy + (mvy >> 3);
@@ -2781,8 +2900,7 @@ __global__ __launch_bounds__(256) void ebiv_recon_inter(const FrameJob* __restri
                                                         const WorkItem* __restrict__ items) {
   const WorkItem it = item_of(items);
   ConstJob& job = job_of(jobs, it.job);
-  recon_inter_mb(job.desc, job.recs, job.coefs, job.status, planes_of(job.rec), planes_of(job.ref), planes_of(job.gold),
-                 static_cast<int>(it.idx));
+  recon_inter_mb(job.desc, job.recs, job.coefs, job.status, job, static_cast<int>(it.idx));
 }
 
 // ============================================================================

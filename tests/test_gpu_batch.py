@@ -102,6 +102,16 @@ def test_launch_schedules_do_not_change_results(schedule):
     _run_batch(streams, schedule=schedule)
 
 
+def test_tuner_checks_every_schedule_and_picks_one():
+    from govid_amd import tune
+
+    pk = clip_kw("rects", 96, 64, 6, qp=22, gop=3, tile_cols=2, tile_rows=2).packets
+    res = tune.tune(pk, sessions=3, window=2, passes=1)
+    assert len(res["tried"]) >= 2
+    assert res["best"] in [t["schedule"] for t in res["tried"]]
+    assert all(t["frames"] == 3 * len(pk) for t in res["tried"])
+
+
 def test_window_larger_than_stream_and_max_out_limit():
     pk = _clip("synthpan", 64, 48, 6, qp=22, gop=6)
     print("window 8 over 6 frames", flush=True)
