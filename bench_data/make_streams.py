@@ -4,10 +4,9 @@ encoder, deterministically (seeded), so GPU runs need not re-encode them.
   parallax_1080p.ebiv — 1920x1080, 120 frames: three Perlin parallax layers
   (0.5 / 1 / 2 px per frame), 2-frame flash every 24 frames, noise sigma 1.5;
   q22, gop 60, 4x3 tiles, single-pass encode.
-  rects_360p / rects_720p / rects_1080p / rects_2160p.ebiv — the serving
-  mix's sizes with the CPU-case content (gradient, 4 moving noise
-  rectangles, sigma 2): q22, gop 30; 1x1 tiles at 640x360, 1280x720 and
-  1920x1080, 8x6 at 3840x2160.
+  parallax_360p / parallax_720p / parallax_2160p.ebiv — the serving mix's
+  other sizes with the same content and encoding (640x360 and 1280x720 with
+  4x3 tiles, 120 frames; 3840x2160 with 8x6 tiles, 60 frames).
 
 Run: python3 bench_data/make_streams.py [name ...]
 """
@@ -25,14 +24,12 @@ from govid_amd import clips, encoder  # noqa: E402
 STREAMS = {
     "parallax_1080p": dict(workload="parallax", w=1920, h=1080, frames=120, seed=0,
                            enc=dict(qp=22, gop=60, tile_cols=4, tile_rows=3)),
-    "rects_360p": dict(workload="rects", w=640, h=360, frames=120, seed=0,
-                       enc=dict(qp=22, gop=30, tile_cols=1, tile_rows=1)),
-    "rects_720p": dict(workload="rects", w=1280, h=720, frames=120, seed=0,
-                       enc=dict(qp=22, gop=30, tile_cols=1, tile_rows=1)),
-    "rects_1080p": dict(workload="rects", w=1920, h=1080, frames=120, seed=0,
-                        enc=dict(qp=22, gop=30, tile_cols=1, tile_rows=1)),
-    "rects_2160p": dict(workload="rects", w=3840, h=2160, frames=60, seed=0,
-                        enc=dict(qp=22, gop=30, tile_cols=8, tile_rows=6)),
+    "parallax_360p": dict(workload="parallax", w=640, h=360, frames=120, seed=0,
+                          enc=dict(qp=22, gop=60, tile_cols=4, tile_rows=3)),
+    "parallax_720p": dict(workload="parallax", w=1280, h=720, frames=120, seed=0,
+                          enc=dict(qp=22, gop=60, tile_cols=4, tile_rows=3)),
+    "parallax_2160p": dict(workload="parallax", w=3840, h=2160, frames=60, seed=0,
+                           enc=dict(qp=22, gop=60, tile_cols=8, tile_rows=6)),
 }
 
 

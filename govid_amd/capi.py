@@ -45,7 +45,7 @@ class Completion(C.Structure):
 class KernelTimes(C.Structure):
     _fields_ = [("parse_ms", C.c_double), ("inter_ms", C.c_double), ("intra_ms", C.c_double),
                 ("rgba_ms", C.c_double), ("parse_n", C.c_uint64), ("inter_n", C.c_uint64),
-                ("intra_n", C.c_uint64), ("rgba_n", C.c_uint64)]
+                ("intra_n", C.c_uint64), ("rgba_n", C.c_uint64), ("inter_mbs", C.c_uint64)]
 
 
 class Stats(C.Structure):

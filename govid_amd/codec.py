@@ -206,6 +206,7 @@ class DeviceBuffer:
     """A block of HBM holding bytes (packets kept resident for decode_device)."""
 
     def __init__(self, data: bytes, device: int = 0):
+        self.ptr = 0
         self._L = native.gpu_lib()
         p = C.c_void_p()
         rc = self._L.ebiv_device_alloc(device, max(len(data), 1), C.byref(p))

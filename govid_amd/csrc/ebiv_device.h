@@ -73,7 +73,10 @@ __host__ __device__ inline size_t desc_rowtile_off(int ntiles, int mbc) { return
 __host__ __device__ inline size_t desc_size(int ntiles, int mbc, int mbr) { return align16(desc_rowtile_off(ntiles, mbc) + 2 * size_t(mbr)); }
 
 // Macroblock record written by the entropy kernel and read by the
-// reconstruction kernels.  kind: 0 skip, 1 inter, 2 intra (decode_inter.go:8-62).
+// reconstruction kernels.  kind: 0 skip, 1 inter, 2 intra (decode_inter.go:8-62),
+// kKindUnreached for a macroblock after the one on which its tile's stream
+// failed (no pixel of it is written).
+constexpr uint32_t kKindUnreached = 3;
 struct alignas(16) MBRec {
   uint8_t kind, part, cbp, tx;
   uint8_t refsel, cmode, pad0, pad1;

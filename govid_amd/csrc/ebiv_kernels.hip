@@ -2109,6 +2109,16 @@ __device__ __forceinline__ WorkItem item_of(const WorkItem* items) {
   return WorkItem{it.job, it.idx};
 }
 
+// Marks the records of tile td from local macroblock `done` on as never
+// reached (kind kKindUnreached), so that reconstruction needs no tile status.
+__device__ void mark_unreached(MBRec* __restrict__ recs, uint32_t mbc, const TileDesc& td, uint32_t done, int lane) {
+  const uint32_t tw = td.x1 - td.x0, total = tw * (td.y1 - td.y0);
+  for (uint32_t i = done + static_cast<uint32_t>(lane); i < total; i += 64) {
+    const uint32_t yy = i / tw, xx = i - yy * tw;
+    *reinterpret_cast<uint32_t*>(recs + (td.y0 + yy) * mbc + td.x0 + xx) = kKindUnreached;
+  }
+}
+
 // decodeTile (decode_intra.go:121-135) for tile t: every macroblock in raster
 // order until the first error; one record per macroblock reached.
 constexpr uint32_t kRecStage = 16;  // macroblock records staged in LDS per store burst
@@ -2181,6 +2191,7 @@ __device__ __forceinline__ void parse_tile(const uint8_t* __restrict__ pkt, cons
   const int tw = td.x1 - td.x0;
 
   if (td.byte_len < 16) {  // ransDecoder.reset (rans.go:237-250)
+    mark_unreached(recs, static_cast<uint32_t>(hdr->mbc), td, 0, lane);
     if (lane == 0) status[t] = TileStatus{kErrCorrupt | (kTileShort << 8), 0, 0, 0};
     return;
   }
@@ -2283,6 +2294,7 @@ __device__ __forceinline__ void parse_tile(const uint8_t* __restrict__ pkt, cons
     LdsU32* tmp = prow; prow = crow; crow = tmp;
   }
   if (static_cast<uint32_t>(lane) < out.cn) out.dst[out.base + lane] = out.buf;
+  if (R.err) mark_unreached(recs, static_cast<uint32_t>(hdr->mbc), td, done, lane);
   if (lane == 0) status[t] = TileStatus{R.err, done, R.nsym, nintra};
 }
 
@@ -2338,6 +2350,7 @@ __device__ __forceinline__ void parse_tile_asm(const uint8_t* __restrict__ pkt, 
   const int tw = td.x1 - td.x0;
 
   if (td.byte_len < 16) {  // ransDecoder.reset (rans.go:237-250)
+    mark_unreached(recs, static_cast<uint32_t>(hdr->mbc), td, 0, lane);
     if (lane == 0) status[t] = TileStatus{kErrCorrupt | (kTileShort << 8), 0, 0, 0};
     return;
   }
@@ -2424,6 +2437,7 @@ __device__ __forceinline__ void parse_tile_asm(const uint8_t* __restrict__ pkt, 
     }
   }
   if (static_cast<uint32_t>(lane) < out.cn) out.dst[out.base + lane] = out.buf;
+  if (R.err) mark_unreached(recs, static_cast<uint32_t>(hdr->mbc), td, done, lane);
   if (lane == 0) status[t] = TileStatus{R.err, done, R.nsym, nintra};
 }
 
@@ -2662,66 +2676,88 @@ __device__ __forceinline__ uint32_t clamp_pack4(const int32_t (&v)[4]) {
 
 // Luma of one partition away from the plane's left and right edges
 // (motioncomp.go:49-78): each lane makes 4 samples of one row.  The source
-// row segment is read as aligned dwords, samples are biased by 128 into
-// signed bytes, and each horizontal tap sum is one v_dot4 plus 8192
-// (the taps sum to 64); rows are clamped to the padded plane as in
-// clampCoord.  The vertical sum and (v + 2048) >> 12 follow, so every phase
-// equals the reference's 1-D, 2-D and copy forms bit for bit; the full-pel
-// copy takes the bytes directly.  Residual and write mask as in the general
-// path.
+// row segment is read as aligned dwords and shifted once by the partition's
+// byte phase (the same for every lane: output columns step by 4), after
+// which sample j's four taps are bytes j..j+3.  Samples are biased by 128
+// into signed bytes, so each horizontal tap sum is one v_dot4 plus 8192 (the
+// taps sum to 64), and the bias is added once at the end (the vertical taps
+// also sum to 64); rows are clamped to the padded plane as in clampCoord.
+// The vertical sum and (v + 2048) >> 12 follow, so every phase equals the
+// reference's 1-D, 2-D and copy forms bit for bit; the full-pel copy takes
+// the bytes directly.  Residual and write mask as in the general path.
 __device__ __forceinline__ void luma_part_fast(const Planes& src, const Planes& rec, const WaveScratch& ws,
-                                               bool luma_res, int n, uint32_t wmask, int mbx, int mby, int px, int py,
+                                               bool luma_res, int lgn, uint32_t wmask, int mbx, int mby, int px, int py,
                                                int pwd, int pht, int mvx, int mvy, int lane) {
   const int r = lane >> 2, c = (lane & 3) * 4;
   if (r >= pht || c >= pwd) return;
   const int y = py + r, x = px + c;  // macroblock-relative
-  const int blk = (y / n) * (16 / n) + x / n;
+  const int blk = ((y >> lgn) << (4 - lgn)) + (x >> lgn);
   if (!((wmask >> blk) & 1u)) return;
   const int fx = mvx & 3, fy = mvy & 3;
   const int sy = mby * 16 + y + (mvy >> 2);
-  const int sx = mbx * 16 + x + (mvx >> 2) - 1;  // first tap of sample 0
-  const int sh = sx & 3;                          // the same for every lane
-  const uint8_t* col = src.y + (sx & ~3);
+  const int sx0 = mbx * 16 + px + (mvx >> 2);  // sample 0 of the partition row (interior: sx0 >= 1)
+  const uint32_t pw = static_cast<uint32_t>(src.pw);
   int32_t v[4];
   if (fx == 0 && fy == 0) {
-    const uint32_t* row = reinterpret_cast<const uint32_t*>(col + static_cast<size_t>(clampc(sy, src.ph)) * src.pw);
-    const uint32_t d0 = row[0], d1 = row[1];
-    const uint32_t w = sh == 3 ? d1 : __builtin_amdgcn_alignbyte(d1, d0, sh + 1);
+    const int sh = sx0 & 3;  // uniform
+    const uint32_t col = static_cast<uint32_t>((sx0 & ~3) + c);
+    const uint32_t* row = reinterpret_cast<const uint32_t*>(src.y + (static_cast<uint32_t>(clampc(sy, src.ph)) * pw + col));
+    const uint32_t w = __builtin_amdgcn_alignbyte(row[1], row[0], sh);
 #pragma unroll
     for (int j = 0; j < 4; j++) v[j] = static_cast<int32_t>((w >> (8 * j)) & 0xFFu);
   } else {
+    const int sh = (sx0 - 1) & 3;  // uniform: byte phase of the first tap
+    const uint32_t col = static_cast<uint32_t>(((sx0 - 1) & ~3) + c);
     const int32_t tx = static_cast<int32_t>(c_taps8[fx]);
-    int32_t acc[4] = {0, 0, 0, 0};
+    // h[j]: the horizontal tap sum of sample j on one row, less 128 * 64
+    auto hsum = [&](const uint32_t* row, int32_t (&h)[4]) {
+      const uint32_t d0 = row[0], d1 = row[1], d2 = row[2];
+      const uint32_t e0 = __builtin_amdgcn_alignbyte(d1, d0, sh) ^ 0x80808080u;
+      const uint32_t e1 = __builtin_amdgcn_alignbyte(d2, d1, sh) ^ 0x80808080u;
+      h[0] = __builtin_amdgcn_sdot4(static_cast<int32_t>(e0), tx, 0, false);
+      h[1] = __builtin_amdgcn_sdot4(static_cast<int32_t>(__builtin_amdgcn_alignbyte(e1, e0, 1)), tx, 0, false);
+      h[2] = __builtin_amdgcn_sdot4(static_cast<int32_t>(__builtin_amdgcn_alignbyte(e1, e0, 2)), tx, 0, false);
+      h[3] = __builtin_amdgcn_sdot4(static_cast<int32_t>(__builtin_amdgcn_alignbyte(e1, e0, 3)), tx, 0, false);
+    };
+    auto row_at = [&](int yy) {
+      return reinterpret_cast<const uint32_t*>(src.y + (static_cast<uint32_t>(clampc(yy, src.ph)) * pw + col));
+    };
+    if (fy == 0) {  // tap row {0, 64, 0, 0}: (64 h + 2048) >> 12 = (h + 32) >> 6
+      int32_t h[4];
+      hsum(row_at(sy), h);
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const int32_t ty = c_taps[fy][k];
-      if (ty == 0) continue;
-      const uint32_t* row =
-          reinterpret_cast<const uint32_t*>(col + static_cast<size_t>(clampc(sy - 1 + k, src.ph)) * src.pw);
-      const uint32_t d0 = row[0] ^ 0x80808080u, d1 = row[1] ^ 0x80808080u, d2 = row[2] ^ 0x80808080u;
+      for (int j = 0; j < 4; j++) v[j] = clamp255((h[j] + (8192 + 32)) >> 6);
+    } else {  // every tap of a fractional row is nonzero: the four rows are loaded together
+      const uint32_t *q0 = row_at(sy - 1), *q1 = row_at(sy), *q2 = row_at(sy + 1), *q3 = row_at(sy + 2);
+      const int32_t t0 = c_taps[fy][0], t1 = c_taps[fy][1], t2 = c_taps[fy][2], t3 = c_taps[fy][3];
+      int32_t h0[4], h1[4], h2[4], h3[4];
+      hsum(q0, h0);
+      hsum(q1, h1);
+      hsum(q2, h2);
+      hsum(q3, h3);
+      // |h| < 2^14 and |tap| <= 64: 24-bit products are exact
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const int s = sh + j;
-        const uint32_t w = s < 4 ? __builtin_amdgcn_alignbyte(d1, d0, s) : __builtin_amdgcn_alignbyte(d2, d1, s - 4);
-        acc[j] += ty * __builtin_amdgcn_sdot4(static_cast<int32_t>(w), tx, 8192, false);
-      }
+      for (int j = 0; j < 4; j++)
+        v[j] = clamp255((__mul24(t0, h0[j]) + __mul24(t1, h1[j]) + __mul24(t2, h2[j]) + __mul24(t3, h3[j]) +
+                         (8192 * 64 + 2048)) >> 12);
     }
-#pragma unroll
-    for (int j = 0; j < 4; j++) v[j] = clamp255((acc[j] + 2048) >> 12);
   }
   if (luma_res) {
-    const int32_t* res = ws.coef + blk * n * n + (y % n) * n + (x % n);
+    const int m = (1 << lgn) - 1;
+    const int32_t* res = ws.coef + (blk << (2 * lgn)) + ((y & m) << lgn) + (x & m);
 #pragma unroll
     for (int j = 0; j < 4; j++) v[j] += res[j];
   }
-  *reinterpret_cast<uint32_t*>(rec.y + static_cast<size_t>(mby * 16 + y) * rec.pw + mbx * 16 + x) = clamp_pack4(v);
+  *reinterpret_cast<uint32_t*>(rec.y + (static_cast<uint32_t>(mby * 16 + y) * static_cast<uint32_t>(rec.pw) +
+                                        static_cast<uint32_t>(mbx * 16 + x))) = clamp_pack4(v);
 }
 
 // Chroma of one partition away from the plane's left and right edges
 // (motioncomp.go:82-97): lanes 0-15 Cb, 16-31 Cr, 4 samples of one row each.
 // The eighth-pel bilinear sum is evaluated as two horizontal 2-tap sums
-// (one v_dot4 each, weights 8 - fx and fx, bias 128 * 8) and the vertical
-// blend (8 - fy, fy); (sum + 32) >> 6 is the reference's rounding.
+// (one v_dot4 each on samples biased by -128, weights 8 - fx and fx) and the
+// vertical blend (8 - fy, fy), the bias returned as 128 * 64 at the end;
+// (sum + 32) >> 6 is the reference's rounding.
 __device__ __forceinline__ void chroma_part_fast(const Planes& src, const Planes& rec, const WaveScratch& ws,
                                                  bool has_res, uint32_t cbp, uint32_t wmask, int mbx, int mby,
                                                  int cpx, int cpy, int cpw, int cph, int mvx, int mvy, int lane) {
@@ -2731,24 +2767,29 @@ __device__ __forceinline__ void chroma_part_fast(const Planes& src, const Planes
   const int fx = mvx & 7, fy = mvy & 7;
   const int y = cpy + r, x = cpx + c;  // macroblock-relative chroma
   const int sy = mby * 8 + y + (mvy >> 3);
-  const int sx = mbx * 8 + x + (mvx >> 3);
-  const int sh = sx & 3;
+  const int sx0 = mbx * 8 + cpx + (mvx >> 3);
+  const int sh = sx0 & 3;  // uniform
+  const uint32_t col = static_cast<uint32_t>((sx0 & ~3) + c), cw = static_cast<uint32_t>(src.cw);
   const uint8_t* plane = pl ? src.cr : src.cb;
-  const uint32_t* r0 = reinterpret_cast<const uint32_t*>(plane + static_cast<size_t>(clampc(sy, src.ch)) * src.cw + (sx & ~3));
+  const uint32_t* r0 = reinterpret_cast<const uint32_t*>(plane + (static_cast<uint32_t>(clampc(sy, src.ch)) * cw + col));
   const uint32_t* r1 =
-      reinterpret_cast<const uint32_t*>(plane + static_cast<size_t>(clampc(sy + 1, src.ch)) * src.cw + (sx & ~3));
-  const uint32_t a0 = r0[0] ^ 0x80808080u, a1 = r0[1] ^ 0x80808080u;
-  const uint32_t b0 = r1[0] ^ 0x80808080u, b1 = r1[1] ^ 0x80808080u;
+      reinterpret_cast<const uint32_t*>(plane + (static_cast<uint32_t>(clampc(sy + 1, src.ch)) * cw + col));
+  const uint32_t a0 = r0[0], a1 = r0[1], b0 = r1[0], b1 = r1[1];
+  // bytes sh..sh+3 and sh+4.. of each row, biased
+  const uint32_t ea = __builtin_amdgcn_alignbyte(a1, a0, sh) ^ 0x80808080u;
+  const uint32_t fa = __builtin_amdgcn_alignbyte(a1, a1, sh) ^ 0x80808080u;
+  const uint32_t eb = __builtin_amdgcn_alignbyte(b1, b0, sh) ^ 0x80808080u;
+  const uint32_t fb = __builtin_amdgcn_alignbyte(b1, b1, sh) ^ 0x80808080u;
   const int32_t tx = (8 - fx) | (fx << 8);
+  const int32_t wy0 = 8 - fy, wy1 = fy;
   int32_t v[4];
+  v[0] = (wy0 * __builtin_amdgcn_sdot4(static_cast<int32_t>(ea), tx, 0, false) +
+          wy1 * __builtin_amdgcn_sdot4(static_cast<int32_t>(eb), tx, 0, false) + (8192 + 32)) >> 6;
 #pragma unroll
-  for (int j = 0; j < 4; j++) {
-    const int s = sh + j;
-    const uint32_t wa = s < 4 ? __builtin_amdgcn_alignbyte(a1, a0, s) : __builtin_amdgcn_alignbyte(a1, a1, s - 4);
-    const uint32_t wb = s < 4 ? __builtin_amdgcn_alignbyte(b1, b0, s) : __builtin_amdgcn_alignbyte(b1, b1, s - 4);
-    const int32_t h0 = __builtin_amdgcn_sdot4(static_cast<int32_t>(wa), tx, 1024, false);
-    const int32_t h1 = __builtin_amdgcn_sdot4(static_cast<int32_t>(wb), tx, 1024, false);
-    v[j] = ((8 - fy) * h0 + fy * h1 + 32) >> 6;
+  for (int j = 1; j < 4; j++) {
+    const uint32_t wa = __builtin_amdgcn_alignbyte(fa, ea, j), wb = __builtin_amdgcn_alignbyte(fb, eb, j);
+    v[j] = (wy0 * __builtin_amdgcn_sdot4(static_cast<int32_t>(wa), tx, 0, false) +
+            wy1 * __builtin_amdgcn_sdot4(static_cast<int32_t>(wb), tx, 0, false) + (8192 + 32)) >> 6;
   }
   if (has_res && (cbp & (pl ? kCBPCr : kCBPCb))) {
     const int32_t* res = ws.coef + 256 + pl * 64 + y * 8 + x;
@@ -2756,7 +2797,8 @@ __device__ __forceinline__ void chroma_part_fast(const Planes& src, const Planes
     for (int j = 0; j < 4; j++) v[j] += res[j];
   }
   uint8_t* dp = pl ? rec.cr : rec.cb;
-  *reinterpret_cast<uint32_t*>(dp + static_cast<size_t>(mby * 8 + y) * rec.cw + mbx * 8 + x) = clamp_pack4(v);
+  *reinterpret_cast<uint32_t*>(dp + (static_cast<uint32_t>(mby * 8 + y) * static_cast<uint32_t>(rec.cw) +
+                                     static_cast<uint32_t>(mbx * 8 + x))) = clamp_pack4(v);
 }
 
 // One skip or inter macroblock (reconMCMB / reconInterMB / reconInterChroma,
@@ -2768,26 +2810,27 @@ __device__ __forceinline__ void chroma_part_fast(const Planes& src, const Planes
 // motioncomp.go:49-97).  The residual (dequantised, inverse transformed) is
 // added and only the blocks of the write mask are stored.
 __device__ __forceinline__ void recon_inter_mb(const uint8_t* __restrict__ desc, const MBRec* __restrict__ recs,
-                                               const uint32_t* __restrict__ coefs,
-                                               const TileStatus* __restrict__ status, ConstJob& job,
+                                               const uint32_t* __restrict__ coefs, ConstJob& job,
                                                const int first_mb) {
   __shared__ InterScratch scratch[4];
   __shared__ int32_t s_dct[768];
   LdsI32* dct = (LdsI32*)s_dct;
-  load_dct(dct);
   const FrameHdr* hdr = reinterpret_cast<const FrameHdr*>(desc);
-  const TileDesc* tiles = reinterpret_cast<const TileDesc*>(desc + desc_tiles_off());
-  const int lane = lane_id(), wave = threadIdx.x >> 6;
+  // Everything per macroblock is wave-uniform: scalar registers and loads.
+  const int lane = lane_id(), wave = static_cast<int>(sgpr(threadIdx.x >> 6));
   const int mbc = hdr->mbc, nmb = mbc * hdr->mbr;
   const int mb = first_mb + wave;
-  if (mb >= nmb) return;
-  const int mbx = mb % mbc, mby = mb / mbc;
-  const int t = tile_of(desc, hdr, mbx, mby);
-  const TileDesc td = tiles[t];
-  const uint32_t local = static_cast<uint32_t>((mby - td.y0) * (td.x1 - td.x0) + (mbx - td.x0));
-  if (local >= status[t].mbs_done) return;
-  const MBRec r = recs[mb];
-  if (r.kind == 2) return;
+  // Macroblocks a failed tile stream never reached carry kind kKindUnreached.
+  const MBRec r = recs[mb < nmb ? mb : nmb - 1];
+  const bool active = mb < nmb && r.kind < 2;
+  const bool has_res = active && r.kind == 1 && r.coef_count > 0;
+  // The DCT matrices are staged only by workgroups with a residual to invert.
+  __shared__ uint32_t s_res[4];
+  if (lane == 0) s_res[wave] = has_res ? 1u : 0u;
+  __syncthreads();
+  if (s_res[0] | s_res[1] | s_res[2] | s_res[3]) load_dct(dct);
+  if (!active) return;
+  const int mby = static_cast<int>(sgpr(static_cast<uint32_t>(mb / mbc))), mbx = mb - mby * mbc;
   InterScratch& S = scratch[wave];
   WaveScratch& ws = S.ws;
   // (the reference is chosen by address in constant memory: selecting
@@ -2795,8 +2838,8 @@ __device__ __forceinline__ void recon_inter_mb(const uint8_t* __restrict__ desc,
   const bool golden = sgpr(static_cast<uint32_t>(r.kind == 1 && r.refsel == kRefGolden)) != 0;
   const Planes rec = planes_of(job.rec);
   const Planes src = planes_of(golden ? job.gold : job.ref);
-  const int n = 4 << (r.tx > 2 ? 2 : r.tx);
-  const bool has_res = r.kind == 1 && r.coef_count > 0;
+  const int lgn = 2 + (r.tx > 2 ? 2 : r.tx);  // log2 of the luma transform size
+  const int n = 1 << lgn;
   const bool luma_res = has_res && (r.cbp & kCBPLuma);
   if (has_res) {
     residual_mb(ws, dct, coefs, r, n, hdr->qdc, hdr->qac, lane);
@@ -2804,16 +2847,21 @@ __device__ __forceinline__ void recon_inter_mb(const uint8_t* __restrict__ desc,
   }
   const int part = r.kind == 1 ? r.part : 0;
   const int npart = static_cast<int>((kPartCountNibbles >> (4 * part)) & 15u);
+  // (the vectors as named words: indexing the record copy by the partition
+  // number would put it in scratch)
+  auto mv_word = [&](int k) {
+    return static_cast<uint32_t>(static_cast<uint16_t>(r.mv[k][0])) |
+           (static_cast<uint32_t>(static_cast<uint16_t>(r.mv[k][1])) << 16);
+  };
+  const uint32_t mvw0 = mv_word(0), mvw1 = mv_word(1), mvw2 = mv_word(2), mvw3 = mv_word(3);
   for (int pi = 0; pi < npart; pi++) {
     const uint32_t rc = part_rect(part, pi);
     const int px = rc & 0xFF, py = (rc >> 8) & 0xFF, pwd = (rc >> 16) & 0xFF, pht = rc >> 24;
-    // (read from the record in memory: indexing the local copy by a
-    // variable would put it in scratch)
-    const uint32_t mvw = reinterpret_cast<const uint32_t*>(recs[mb].mv)[pi];
+    const uint32_t mvw = sgpr(pi == 0 ? mvw0 : (pi == 1 ? mvw1 : (pi == 2 ? mvw2 : mvw3)));
     const int mvx = static_cast<int16_t>(mvw & 0xFFFFu), mvy = static_cast<int16_t>(mvw >> 16);
     // ---- luma
     if (const int sx0 = mbx * 16 + px + (mvx >> 2) - 1; sx0 >= 0 && sx0 + pwd + 8 <= src.pw) {
-      luma_part_fast(src, rec, ws, luma_res, n, r.wmask, mbx, mby, px, py, pwd, pht, mvx, mvy, lane);
+      luma_part_fast(src, rec, ws, luma_res, lgn, r.wmask, mbx, mby, px, py, pwd, pht, mvx, mvy, lane);
     } else {
       const int fx = mvx & 3, fy = mvy & 3;
       const int ox = mbx * 16 + px + (mvx >> 2) - 1, oy = mby * 16 + py + (mvy >> 2) - 1;
@@ -2838,14 +2886,14 @@ __device__ __forceinline__ void recon_inter_mb(const uint8_t* __restrict__ desc,
       if (lane < (pht << gs)) {
         const int rr = lane >> gs, c0 = (lane & ((1 << gs) - 1)) * 4;
         const int y = py + rr, x = px + c0;
-        const int blk = (y / n) * (16 / n) + x / n;
+        const int blk = ((y >> lgn) << (4 - lgn)) + (x >> lgn);
         if ((r.wmask >> blk) & 1u) {
           uint32_t word = 0;
 #pragma unroll
           for (int k = 0; k < 4; k++) {
             const int32_t* h = S.hbuf + rr * 16 + c0 + k;
             int32_t p = clamp255((v0 * h[0] + v1 * h[16] + v2 * h[32] + v3 * h[48] + 2048) >> 12);
-            if (luma_res) p = clamp255(p + ws.coef[blk * n * n + (y % n) * n + ((x + k) % n)]);
+            if (luma_res) p = clamp255(p + ws.coef[(blk << (2 * lgn)) + ((y & (n - 1)) << lgn) + ((x + k) & (n - 1))]);
             word |= static_cast<uint32_t>(p) << (8 * k);
           }
           *reinterpret_cast<uint32_t*>(rec.y + static_cast<size_t>(mby * 16 + y) * rec.pw + mbx * 16 + x) = word;
@@ -2900,7 +2948,7 @@ __global__ __launch_bounds__(256) void ebiv_recon_inter(const FrameJob* __restri
                                                         const WorkItem* __restrict__ items) {
   const WorkItem it = item_of(items);
   ConstJob& job = job_of(jobs, it.job);
-  recon_inter_mb(job.desc, job.recs, job.coefs, job.status, job, static_cast<int>(it.idx));
+  recon_inter_mb(job.desc, job.recs, job.coefs, job, static_cast<int>(it.idx));
 }
 
 // ============================================================================

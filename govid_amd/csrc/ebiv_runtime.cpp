@@ -158,8 +158,8 @@ struct PayloadSlot {
   bool inter = false;
   DevArray<MBRec> recs;
   DevArray<uint32_t> coefs;
-  DevArray<TileStatus> d_status;
-  PinnedArray<TileStatus> h_status;
+  size_t status_off = 0;  // first tile's entry in the step's status array
+  PinnedArray<TileStatus> h_status;  // the last step's tile statuses
 };
 
 }  // namespace
@@ -440,8 +440,7 @@ struct Session {
     ps->desc_bytes = dsize;
     ps->inter = inter;
     const size_t nmb = static_cast<size_t>(mbc) * mbr;
-    if (!ps->recs.reserve(nmb) || !ps->coefs.reserve(nmb * kCoefPerMB) || !ps->d_status.reserve(nt) ||
-        !ps->h_status.reserve(nt))
+    if (!ps->recs.reserve(nmb) || !ps->coefs.reserve(nmb * kCoefPerMB) || !ps->h_status.reserve(nt))
       return fail(EBIV_ERR_DEVICE, "payload buffer allocation failed");
     return EBIV_OK;
   }
@@ -609,11 +608,15 @@ struct Engine {
   DevArray<FrameJob> d_jobs;
   PinnedArray<WorkItem> h_items;
   DevArray<WorkItem> d_items;
+  // Tile statuses of every payload of a step, contiguous: one download
+  DevArray<TileStatus> d_step_status;
+  PinnedArray<TileStatus> h_step_status;
   DevArray<uint32_t> d_rgba;
   std::string msg;
   bool timing = false;
   std::vector<hipEvent_t> evs;
   std::vector<int> marks;  // kind per event pair
+  std::vector<uint64_t> mark_mbs;
   ebiv_kernel_times ktimes{};
 
   ~Engine() {
@@ -634,8 +637,9 @@ struct Engine {
   }
 
   // Brackets a launch of kind k (0 parse, 1 inter, 2 intra, 3 rgba) with
-  // events while timing is on; returns the pair index or -1.
-  int mark_begin(int kind) {
+  // events while timing is on; returns the pair index or -1.  `mbs`: the
+  // macroblocks an inter launch covers.
+  int mark_begin(int kind, uint64_t mbs = 0) {
     if (!timing) return -1;
     const size_t i = marks.size();
     while (evs.size() < 2 * i + 2) {
@@ -645,6 +649,7 @@ struct Engine {
     }
     if (hipEventRecord(evs[2 * i], stream) != hipSuccess) return -1;
     marks.push_back(kind);
+    mark_mbs.push_back(mbs);
     return static_cast<int>(i);
   }
   void mark_end(int m) {
@@ -656,12 +661,13 @@ struct Engine {
       if (hipEventElapsedTime(&ms, evs[2 * i], evs[2 * i + 1]) != hipSuccess) continue;
       switch (marks[i]) {
         case 0: ktimes.parse_ms += ms; ktimes.parse_n++; break;
-        case 1: ktimes.inter_ms += ms; ktimes.inter_n++; break;
+        case 1: ktimes.inter_ms += ms; ktimes.inter_n++; ktimes.inter_mbs += mark_mbs[i]; break;
         case 2: ktimes.intra_ms += ms; ktimes.intra_n++; break;
         default: ktimes.rgba_ms += ms; ktimes.rgba_n++; break;
       }
     }
     marks.clear();
+    mark_mbs.clear();
   }
 
   int run_step(std::vector<Session*>& ss, std::vector<std::vector<StepFrame>>& frames);
@@ -749,7 +755,7 @@ int Engine::run_step(std::vector<Session*>& ss, std::vector<std::vector<StepFram
     j.desc = d_stage.d + ps->desc_off;
     j.recs = ps->recs.d;
     j.coefs = ps->coefs.d;
-    j.status = ps->d_status.d;
+    j.status = d_step_status.d + ps->status_off;
     j.pad[0] = j.pad[1] = 0;
     if (r.ov) {  // the override predicts only from its key (codec.go:178-181)
       j.rec = s.pool[f.gtarget].p;
@@ -778,15 +784,16 @@ int Engine::run_step(std::vector<Session*>& ss, std::vector<std::vector<StepFram
   size_t nitems = 0, nparse = 0;
   int max_tw = 1;
   for (const JobRef& jr : jrefs) {
-    const PayloadSlot* ps = payload_of(ss, frames, jr);
+    PayloadSlot* ps = payload_of(ss, frames, jr);
     const Session& s = *ss[jr.si];
+    ps->status_off = nparse;
     nparse += static_cast<size_t>(ps->ntiles);
     nitems += 2 * static_cast<size_t>(ps->ntiles) + (ps->inter ? (static_cast<size_t>(s.mbc) * s.mbr + 3) / 4 : 0);
     max_tw = std::max(max_tw, ps->max_tile_w);
   }
   const size_t njobs = jrefs.size();
   if (!h_jobs.reserve(njobs + 1) || !d_jobs.reserve(njobs + 1) || !h_items.reserve(nitems + 1) ||
-      !d_items.reserve(nitems + 1))
+      !d_items.reserve(nitems + 1) || !d_step_status.reserve(nparse + 1) || !h_step_status.reserve(nparse + 1))
     return fail(EBIV_ERR_DEVICE, "job table allocation failed");
   for (size_t j = 0; j < njobs; j++) fill_job(h_jobs.h[j], jrefs[j]);
   size_t w = 0;
@@ -844,7 +851,10 @@ int Engine::run_step(std::vector<Session*>& ss, std::vector<std::vector<StepFram
     for (size_t j : rounds[r])
       if (jrefs[j].ov && !golden_copy(jrefs[j])) return fail(EBIV_ERR_DEVICE, "golden copy failed");
     if (intra_off[r] > inter_off[r]) {
-      const int m = mark_begin(1);
+      uint64_t mbs = 0;
+      for (size_t j : rounds[r])
+        if (payload_of(ss, frames, jrefs[j])->inter) mbs += static_cast<uint64_t>(ss[jrefs[j].si]->mbc) * ss[jrefs[j].si]->mbr;
+      const int m = mark_begin(1, mbs);
       launch_recon_inter(stream, static_cast<int>(intra_off[r] - inter_off[r]), d_jobs.d, d_items.d + inter_off[r]);
       mark_end(m);
     }
@@ -862,15 +872,16 @@ int Engine::run_step(std::vector<Session*>& ss, std::vector<std::vector<StepFram
       mark_end(m);
     }
   }
-  for (const JobRef& jr : jrefs) {
-    PayloadSlot* ps = payload_of(ss, frames, jr);
-    if (hipMemcpyAsync(ps->h_status.h, ps->d_status.d, sizeof(TileStatus) * ps->ntiles, hipMemcpyDeviceToHost, stream) !=
-        hipSuccess)
-      return fail(EBIV_ERR_DEVICE, "status download failed");
-  }
+  if (nparse && hipMemcpyAsync(h_step_status.h, d_step_status.d, sizeof(TileStatus) * nparse, hipMemcpyDeviceToHost,
+                                stream) != hipSuccess)
+    return fail(EBIV_ERR_DEVICE, "status download failed");
   if (hipGetLastError() != hipSuccess) return fail(EBIV_ERR_DEVICE, "kernel launch failed");
   if (hipStreamSynchronize(stream) != hipSuccess) return fail(EBIV_ERR_DEVICE, "decode failed on the device");
   collect_marks();
+  for (const JobRef& jr : jrefs) {
+    PayloadSlot* ps = payload_of(ss, frames, jr);
+    std::memcpy(ps->h_status.h, h_step_status.h + ps->status_off, sizeof(TileStatus) * ps->ntiles);
+  }
 
   // ---- outcomes; where a payload failed, the session's later payloads are
   // reconstructed again against the buffers Go would have used
@@ -1108,10 +1119,8 @@ int ebiv_codec_debug_dump(ebiv_codec* c, int slot, void* recs, size_t recs_bytes
   if (coefs && ps->coefs.d &&
       hipMemcpy(coefs, ps->coefs.d, std::min(coef_count, nmb * kCoefPerMB) * 4, hipMemcpyDeviceToHost) != hipSuccess)
     return -EBIV_ERR_DEVICE;
-  if (status && ps->d_status.d &&
-      hipMemcpy(status, ps->d_status.d, std::min(status_bytes, ps->d_status.cap * sizeof(TileStatus)),
-                hipMemcpyDeviceToHost) != hipSuccess)
-    return -EBIV_ERR_DEVICE;
+  if (status && ps->h_status.h)
+    std::memcpy(status, ps->h_status.h, std::min(status_bytes, ps->h_status.cap * sizeof(TileStatus)));
   return static_cast<int>(nmb);
 }
 

@@ -85,10 +85,12 @@ int ebiv_codec_stats(const ebiv_codec *c, ebiv_stats *out);
 
 /* Per-kernel device time of this codec's decodes, measured with HIP events
  * on the codec's stream while timing is enabled (no reference counterpart:
- * measurement only).  Durations in milliseconds, summed over launches. */
+ * measurement only).  Durations in milliseconds, summed over launches;
+ * inter_mbs: macroblocks covered by the timed inter launches. */
 typedef struct {
     double parse_ms, inter_ms, intra_ms, rgba_ms;
     uint64_t parse_n, inter_n, intra_n, rgba_n;
+    uint64_t inter_mbs;
 } ebiv_kernel_times;
 
 int ebiv_codec_set_timing(ebiv_codec *c, int enabled);
