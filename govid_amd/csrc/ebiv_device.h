@@ -124,9 +124,12 @@ struct FrameJob {
   MBRec* recs;
   uint32_t* coefs;
   TileStatus* status;
-  uint32_t pad[2];
+  int32_t mbc, mbr;  // the frame's macroblock grid
   Planes rec, ref, gold;
 };
+
+// Macroblocks per inter-reconstruction work item: one per wavefront.
+constexpr uint32_t kInterGroupMbs = 4;
 
 // One workgroup's share of a launch: job index and a tile or macroblock index.
 struct WorkItem {

@@ -2562,15 +2562,16 @@ __device__ __forceinline__ void idct_blocks(WaveScratch& ws, const LdsI32* C, in
   wave_sync();
 }
 
+// cl: the n x n matrix, cc: the 8 x 8 one.
 template <typename Acc>
-__device__ __forceinline__ void idct_mb(WaveScratch& ws, const LdsI32* dct, int n, uint32_t nzm, uint32_t acm,
-                                        int lane) {
+__device__ __forceinline__ void idct_mb(WaveScratch& ws, const LdsI32* cl, const LdsI32* cc, int n, uint32_t nzm,
+                                        uint32_t acm, int lane) {
   if (nzm & 0xFFFFu) {
-    if (n == 4) idct_blocks<4, Acc>(ws, dct, 0, 16, nzm, acm, lane);
-    else if (n == 8) idct_blocks<8, Acc>(ws, dct + 256, 0, 4, nzm, acm, lane);
-    else idct_blocks<16, Acc>(ws, dct + 512, 0, 1, nzm, acm, lane);
+    if (n == 4) idct_blocks<4, Acc>(ws, cl, 0, 16, nzm, acm, lane);
+    else if (n == 8) idct_blocks<8, Acc>(ws, cl, 0, 4, nzm, acm, lane);
+    else idct_blocks<16, Acc>(ws, cl, 0, 1, nzm, acm, lane);
   }
-  if (nzm & 0x30000u) idct_blocks<8, Acc>(ws, dct + 256, 256, 2, nzm >> 16, acm >> 16, lane);
+  if (nzm & 0x30000u) idct_blocks<8, Acc>(ws, cc, 256, 2, nzm >> 16, acm >> 16, lane);
 }
 
 // Largest |coefficient| of the macroblock's dequantised residual over the
@@ -2587,73 +2588,46 @@ __device__ __forceinline__ int32_t wave_max_abs(const WaveScratch& ws, int lane)
 
 // Dequantisation and inverse transform of a macroblock's residual into
 // ws.coef (dense, zero where nothing is coded).
+// dct: the three matrices as laid out by load_dct.
 __device__ __forceinline__ void residual_mb(WaveScratch& ws, const LdsI32* dct, const uint32_t* __restrict__ coefs,
                                             const MBRec& r, int n, int qdc, int qac, int lane) {
   const uint64_t masks = load_coefs(ws, coefs, r, n, qdc, qac, lane);
   const uint32_t nzm = static_cast<uint32_t>(masks), acm = static_cast<uint32_t>(masks >> 32);
   if (!nzm) return;
-  if (wave_max_abs(ws, lane) <= kIdct32Max) idct_mb<int32_t>(ws, dct, n, nzm, acm, lane);
-  else idct_mb<int64_t>(ws, dct, n, nzm, acm, lane);
+  const LdsI32* cl = dct + (n == 4 ? 0 : (n == 8 ? 256 : 512));
+  if (wave_max_abs(ws, lane) <= kIdct32Max) idct_mb<int32_t>(ws, cl, dct + 256, n, nzm, acm, lane);
+  else idct_mb<int64_t>(ws, cl, dct + 256, n, nzm, acm, lane);
+}
+
+// The same for one wavefront that stages only the matrices it needs in its
+// own LDS (mats: the n x n matrix at 0, the 8 x 8 one at 256): no
+// workgroup barrier.
+__device__ __forceinline__ void residual_mb_wave(WaveScratch& ws, LdsI32* mats, const uint32_t* __restrict__ coefs,
+                                                 const MBRec& r, int n, int qdc, int qac, int lane) {
+  const uint64_t masks = load_coefs(ws, coefs, r, n, qdc, qac, lane);
+  const uint32_t nzm = static_cast<uint32_t>(masks), acm = static_cast<uint32_t>(masks >> 32);
+  if (!nzm) return;
+  if (nzm & 0xFFFFu) {
+    const int32_t* src = &c_dct[n == 4 ? 0 : (n == 8 ? 1 : 2)][0];
+    for (int i = lane; i < n * n; i += 64) mats[i] = src[i];
+  }
+  if (nzm & 0x30000u) mats[256 + lane] = c_dct[1][lane];
+  wave_sync();
+  if (wave_max_abs(ws, lane) <= kIdct32Max) idct_mb<int32_t>(ws, mats, mats + 256, n, nzm, acm, lane);
+  else idct_mb<int64_t>(ws, mats, mats + 256, n, nzm, acm, lane);
 }
 
 __device__ __forceinline__ int clampc(int v, int hi) { return v < 0 ? 0 : (v >= hi ? hi - 1 : v); }
 
-// Luma quarter-pel sample (motioncomp.go:49-78).  The 1-D and full-pel cases
-// are the 2-D filter with a {0,64,0,0} tap row, which reproduces their
-// (v+32)>>6 and copy results exactly, and clamped coordinates equal the
-// interior fast paths wherever those apply.
-__device__ __forceinline__ int32_t luma_mc(const uint8_t* __restrict__ p, int pw, int ph, int x, int y, int fx, int fy) {
-  const int32_t* tx = c_taps[fx];
-  const int32_t* ty = c_taps[fy];
-  int x0 = clampc(x - 1, pw), x1 = clampc(x, pw), x2 = clampc(x + 1, pw), x3 = clampc(x + 2, pw);
-  int32_t v = 0;
-#pragma unroll
-  for (int k = 0; k < 4; k++) {
-    const uint8_t* row = p + static_cast<size_t>(clampc(y - 1 + k, ph)) * pw;
-    int32_t h = tx[0] * row[x0] + tx[1] * row[x1] + tx[2] * row[x2] + tx[3] * row[x3];
-    v += ty[k] * h;
-  }
-  return clamp255((v + 2048) >> 12);
-}
-
-// Chroma eighth-pel bilinear sample (motioncomp.go:82-97).
-__device__ __forceinline__ int32_t chroma_mc(const uint8_t* __restrict__ p, int cw, int ch, int x, int y, int fx, int fy) {
-  int xa = clampc(x, cw), xb = clampc(x + 1, cw);
-  const uint8_t* r0 = p + static_cast<size_t>(clampc(y, ch)) * cw;
-  const uint8_t* r1 = p + static_cast<size_t>(clampc(y + 1, ch)) * cw;
-  int32_t v = (8 - fx) * (8 - fy) * r0[xa] + fx * (8 - fy) * r0[xb] + (8 - fx) * fy * r1[xa] + fx * fy * r1[xb];
-  return (v + 32) >> 6;
-}
-
-__device__ __forceinline__ int part_of(int part, int x, int y) {  // partRects, model.go:91-96
-  switch (part) {
-    case kPart16x8: return y >= 8 ? 1 : 0;
-    case kPart8x16: return x >= 8 ? 1 : 0;
-    case kPart8x8: return (y >= 8 ? 2 : 0) + (x >= 8 ? 1 : 0);
-    default: return 0;
-  }
-}
-
-__device__ __forceinline__ int tile_of(const uint8_t* desc, const FrameHdr* hdr, int mbx, int mby) {
-  const uint16_t* ct = reinterpret_cast<const uint16_t*>(desc + desc_coltile_off(hdr->ntiles));
-  const uint16_t* rt = reinterpret_cast<const uint16_t*>(desc + desc_rowtile_off(hdr->ntiles, hdr->mbc));
-  return rt[mby] * hdr->grid_cols + ct[mbx];
-}
-
 // ============================================================================
 // Inter reconstruction: one wavefront per macroblock
 // ============================================================================
-// Window rows use fixed strides so that lane-to-sample mappings divide by
-// constants.
-constexpr int kWinStride = 20;   // luma window row (up to 19 samples)
-constexpr int kCWinStride = 9;   // chroma window row (up to 9 samples)
+// Per wavefront: the residual and the inverse transform's work area, and the
+// DCT matrices it needs.
 struct InterScratch {
   WaveScratch ws;
-  int32_t hbuf[19 * 16];            // horizontal filter pass, 16 per row
-  uint8_t win[19 * kWinStride];     // luma reference window of one partition
-  uint8_t cwin[2][9 * kCWinStride];
+  int32_t mats[320];
 };
-
 
 // Partition rectangle pi of a partition mode (partRects, model.go:91-96) as
 // x | y << 8 | w << 16 | h << 24.
@@ -2674,67 +2648,120 @@ __device__ __forceinline__ uint32_t clamp_pack4(const int32_t (&v)[4]) {
          (static_cast<uint32_t>(clamp255(v[2])) << 16) | (static_cast<uint32_t>(clamp255(v[3])) << 24);
 }
 
-// Luma of one partition away from the plane's left and right edges
-// (motioncomp.go:49-78): each lane makes 4 samples of one row.  The source
-// row segment is read as aligned dwords and shifted once by the partition's
-// byte phase (the same for every lane: output columns step by 4), after
-// which sample j's four taps are bytes j..j+3.  Samples are biased by 128
-// into signed bytes, so each horizontal tap sum is one v_dot4 plus 8192 (the
-// taps sum to 64), and the bias is added once at the end (the vertical taps
-// also sum to 64); rows are clamped to the padded plane as in clampCoord.
-// The vertical sum and (v + 2048) >> 12 follow, so every phase equals the
-// reference's 1-D, 2-D and copy forms bit for bit; the full-pel copy takes
-// the bytes directly.  Residual and write mask as in the general path.
-__device__ __forceinline__ void luma_part_fast(const Planes& src, const Planes& rec, const WaveScratch& ws,
-                                               bool luma_res, int lgn, uint32_t wmask, int mbx, int mby, int px, int py,
-                                               int pwd, int pht, int mvx, int mvy, int lane) {
-  const int r = lane >> 2, c = (lane & 3) * 4;
-  if (r >= pht || c >= pwd) return;
-  const int y = py + r, x = px + c;  // macroblock-relative
-  const int blk = ((y >> lgn) << (4 - lgn)) + (x >> lgn);
-  if (!((wmask >> blk) & 1u)) return;
+// Bytes col .. col + 4K - 1 of a plane row of width w as K dwords, each
+// column clamped to [0, w) as clampCoord clamps it (motioncomp.go:37-45).
+// col is a multiple of 4, so a span inside the row is K aligned dword
+// loads; a span over an edge (lanes at the plane's left or right border)
+// is gathered byte by byte.
+template <int K>
+__device__ __forceinline__ void load_span(const uint8_t* __restrict__ row, int col, int w, uint32_t (&d)[K]) {
+  if (col >= 0 && col + 4 * K <= w) {
+    const uint32_t* p = reinterpret_cast<const uint32_t*>(row + col);
+#pragma unroll
+    for (int k = 0; k < K; k++) d[k] = p[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      uint32_t v = 0;
+#pragma unroll
+      for (int b = 0; b < 4; b++) v |= static_cast<uint32_t>(row[clampc(col + 4 * k + b, w)]) << (8 * b);
+      d[k] = v;
+    }
+  }
+}
+
+// Horizontal 4-tap sums of samples 0-3 from a row segment d0..d2 whose
+// first tap is byte sh, on samples biased by -128 (each sum is 128 * 64 low).
+__device__ __forceinline__ void luma_hsum4(const uint32_t (&d)[3], int sh, int32_t tx, int32_t (&h)[4]) {
+  const uint32_t e0 = __builtin_amdgcn_alignbyte(d[1], d[0], sh) ^ 0x80808080u;
+  const uint32_t e1 = __builtin_amdgcn_alignbyte(d[2], d[1], sh) ^ 0x80808080u;
+  h[0] = __builtin_amdgcn_sdot4(static_cast<int32_t>(e0), tx, 0, false);
+  h[1] = __builtin_amdgcn_sdot4(static_cast<int32_t>(__builtin_amdgcn_alignbyte(e1, e0, 1)), tx, 0, false);
+  h[2] = __builtin_amdgcn_sdot4(static_cast<int32_t>(__builtin_amdgcn_alignbyte(e1, e0, 2)), tx, 0, false);
+  h[3] = __builtin_amdgcn_sdot4(static_cast<int32_t>(__builtin_amdgcn_alignbyte(e1, e0, 3)), tx, 0, false);
+}
+
+// Motion compensation of one partition, its residual added and the blocks
+// of the write mask stored (decode_inter.go:64-152, motioncomp.go:37-97).
+// Luma: lane = 4 samples of partition row lane / 4.  Chroma: lanes 0-15 Cb,
+// 16-31 Cr, 4 samples of one row each.  Every reference load of the
+// partition is issued before any sample is computed.
+//  - Luma: the row segment of a lane's samples is read as aligned dwords
+//    and shifted once by the partition's byte phase (the same for every
+//    lane, as output columns step by 4); sample j's taps are then bytes
+//    j..j+3.  Samples are biased by -128 into signed bytes, each horizontal
+//    tap sum is one v_dot4, and the bias returns as 128 * 64 per tap row
+//    (the taps sum to 64).  Rows are clamped to the padded plane.  The
+//    vertical sum and (v + 2048) >> 12 follow; a {0, 64, 0, 0} tap row
+//    makes that equal the reference's 1-D forms, (v + 32) >> 6, and the
+//    full-pel case takes the bytes directly.
+//  - Chroma: the eighth-pel bilinear sum as two horizontal 2-tap sums (one
+//    v_dot4 each, weights 8 - fx and fx) and the vertical blend (8 - fy,
+//    fy); (sum + 32) >> 6 is the reference's rounding.
+__device__ __forceinline__ void mc_part(const Planes& src, const Planes& rec, const WaveScratch& ws, const MBRec& r,
+                                        bool has_res, int lgn, int mbx, int mby, int px, int py, int pwd, int pht,
+                                        int mvx, int mvy, int lane) {
+  // ---- luma loads
   const int fx = mvx & 3, fy = mvy & 3;
+  const int lr = lane >> 2, lc = (lane & 3) * 4;
+  const int y = py + lr, x = px + lc;  // macroblock-relative
   const int sy = mby * 16 + y + (mvy >> 2);
-  const int sx0 = mbx * 16 + px + (mvx >> 2);  // sample 0 of the partition row (interior: sx0 >= 1)
-  const uint32_t pw = static_cast<uint32_t>(src.pw);
+  const int sx0 = mbx * 16 + px + (mvx >> 2);
+  const bool copy = (fx | fy) == 0;
+  const int sft = copy ? sx0 : sx0 - 1;  // first byte read of the partition row
+  const int sh = sft & 3;
+  const int col = (sft & ~3) + lc;
+  auto lrow = [&](int yy) {
+    return src.y + static_cast<uint32_t>(clampc(yy, src.ph)) * static_cast<uint32_t>(src.pw);
+  };
+  uint32_t q0[3] = {0, 0, 0}, q1[3] = {0, 0, 0}, q2[3] = {0, 0, 0}, q3[3] = {0, 0, 0};
+  if (copy) {
+    uint32_t t[2];
+    load_span<2>(lrow(sy), col, src.pw, t);
+    q1[0] = t[0];
+    q1[1] = t[1];
+  } else if (fy == 0) {
+    load_span<3>(lrow(sy), col, src.pw, q1);
+  } else {
+    load_span<3>(lrow(sy - 1), col, src.pw, q0);
+    load_span<3>(lrow(sy), col, src.pw, q1);
+    load_span<3>(lrow(sy + 1), col, src.pw, q2);
+    load_span<3>(lrow(sy + 2), col, src.pw, q3);
+  }
+  // ---- chroma loads (lanes 32-63 repeat lanes 0-31)
+  const int cpx = px >> 1, cpy = py >> 1, cpw = pwd >> 1, cph = pht >> 1;
+  const int pl = (lane >> 4) & 1, cr = (lane >> 1) & 7, cc = (lane & 1) * 4;
+  const int cy = cpy + cr, cx = cpx + cc;
+  const int csy = mby * 8 + cy + (mvy >> 3);
+  const int csx0 = mbx * 8 + cpx + (mvx >> 3);
+  const int csh = csx0 & 3;
+  const int ccol = (csx0 & ~3) + cc;
+  const uint8_t* plane = pl ? src.cr : src.cb;
+  uint32_t ca[2], cb[2];
+  load_span<2>(plane + static_cast<uint32_t>(clampc(csy, src.ch)) * static_cast<uint32_t>(src.cw), ccol, src.cw, ca);
+  load_span<2>(plane + static_cast<uint32_t>(clampc(csy + 1, src.ch)) * static_cast<uint32_t>(src.cw), ccol, src.cw,
+               cb);
+
+  // ---- luma samples
   int32_t v[4];
-  if (fx == 0 && fy == 0) {
-    const int sh = sx0 & 3;  // uniform
-    const uint32_t col = static_cast<uint32_t>((sx0 & ~3) + c);
-    const uint32_t* row = reinterpret_cast<const uint32_t*>(src.y + (static_cast<uint32_t>(clampc(sy, src.ph)) * pw + col));
-    const uint32_t w = __builtin_amdgcn_alignbyte(row[1], row[0], sh);
+  if (copy) {
+    const uint32_t w = __builtin_amdgcn_alignbyte(q1[1], q1[0], sh);
 #pragma unroll
     for (int j = 0; j < 4; j++) v[j] = static_cast<int32_t>((w >> (8 * j)) & 0xFFu);
   } else {
-    const int sh = (sx0 - 1) & 3;  // uniform: byte phase of the first tap
-    const uint32_t col = static_cast<uint32_t>(((sx0 - 1) & ~3) + c);
     const int32_t tx = static_cast<int32_t>(c_taps8[fx]);
-    // h[j]: the horizontal tap sum of sample j on one row, less 128 * 64
-    auto hsum = [&](const uint32_t* row, int32_t (&h)[4]) {
-      const uint32_t d0 = row[0], d1 = row[1], d2 = row[2];
-      const uint32_t e0 = __builtin_amdgcn_alignbyte(d1, d0, sh) ^ 0x80808080u;
-      const uint32_t e1 = __builtin_amdgcn_alignbyte(d2, d1, sh) ^ 0x80808080u;
-      h[0] = __builtin_amdgcn_sdot4(static_cast<int32_t>(e0), tx, 0, false);
-      h[1] = __builtin_amdgcn_sdot4(static_cast<int32_t>(__builtin_amdgcn_alignbyte(e1, e0, 1)), tx, 0, false);
-      h[2] = __builtin_amdgcn_sdot4(static_cast<int32_t>(__builtin_amdgcn_alignbyte(e1, e0, 2)), tx, 0, false);
-      h[3] = __builtin_amdgcn_sdot4(static_cast<int32_t>(__builtin_amdgcn_alignbyte(e1, e0, 3)), tx, 0, false);
-    };
-    auto row_at = [&](int yy) {
-      return reinterpret_cast<const uint32_t*>(src.y + (static_cast<uint32_t>(clampc(yy, src.ph)) * pw + col));
-    };
-    if (fy == 0) {  // tap row {0, 64, 0, 0}: (64 h + 2048) >> 12 = (h + 32) >> 6
+    if (fy == 0) {
       int32_t h[4];
-      hsum(row_at(sy), h);
+      luma_hsum4(q1, sh, tx, h);
 #pragma unroll
       for (int j = 0; j < 4; j++) v[j] = clamp255((h[j] + (8192 + 32)) >> 6);
-    } else {  // every tap of a fractional row is nonzero: the four rows are loaded together
-      const uint32_t *q0 = row_at(sy - 1), *q1 = row_at(sy), *q2 = row_at(sy + 1), *q3 = row_at(sy + 2);
+    } else {
       const int32_t t0 = c_taps[fy][0], t1 = c_taps[fy][1], t2 = c_taps[fy][2], t3 = c_taps[fy][3];
       int32_t h0[4], h1[4], h2[4], h3[4];
-      hsum(q0, h0);
-      hsum(q1, h1);
-      hsum(q2, h2);
-      hsum(q3, h3);
+      luma_hsum4(q0, sh, tx, h0);
+      luma_hsum4(q1, sh, tx, h1);
+      luma_hsum4(q2, sh, tx, h2);
+      luma_hsum4(q3, sh, tx, h3);
       // |h| < 2^14 and |tap| <= 64: 24-bit products are exact
 #pragma unroll
       for (int j = 0; j < 4; j++)
@@ -2742,107 +2769,74 @@ __device__ __forceinline__ void luma_part_fast(const Planes& src, const Planes& 
                          (8192 * 64 + 2048)) >> 12);
     }
   }
-  if (luma_res) {
-    const int m = (1 << lgn) - 1;
-    const int32_t* res = ws.coef + (blk << (2 * lgn)) + ((y & m) << lgn) + (x & m);
+  const int blk = ((y >> lgn) << (4 - lgn)) + (x >> lgn);
+  if (lr < pht && lc < pwd && ((r.wmask >> blk) & 1u)) {
+    if (has_res && (r.cbp & kCBPLuma)) {
+      const int m = (1 << lgn) - 1;
+      const int32_t* res = ws.coef + (blk << (2 * lgn)) + ((y & m) << lgn) + (x & m);
 #pragma unroll
-    for (int j = 0; j < 4; j++) v[j] += res[j];
+      for (int j = 0; j < 4; j++) v[j] += res[j];
+    }
+    *reinterpret_cast<uint32_t*>(rec.y + (static_cast<uint32_t>(mby * 16 + y) * static_cast<uint32_t>(rec.pw) +
+                                          static_cast<uint32_t>(mbx * 16 + x))) = clamp_pack4(v);
   }
-  *reinterpret_cast<uint32_t*>(rec.y + (static_cast<uint32_t>(mby * 16 + y) * static_cast<uint32_t>(rec.pw) +
-                                        static_cast<uint32_t>(mbx * 16 + x))) = clamp_pack4(v);
-}
-
-// Chroma of one partition away from the plane's left and right edges
-// (motioncomp.go:82-97): lanes 0-15 Cb, 16-31 Cr, 4 samples of one row each.
-// The eighth-pel bilinear sum is evaluated as two horizontal 2-tap sums
-// (one v_dot4 each on samples biased by -128, weights 8 - fx and fx) and the
-// vertical blend (8 - fy, fy), the bias returned as 128 * 64 at the end;
-// (sum + 32) >> 6 is the reference's rounding.
-__device__ __forceinline__ void chroma_part_fast(const Planes& src, const Planes& rec, const WaveScratch& ws,
-                                                 bool has_res, uint32_t cbp, uint32_t wmask, int mbx, int mby,
-                                                 int cpx, int cpy, int cpw, int cph, int mvx, int mvy, int lane) {
-  const int pl = lane >> 4, r = (lane >> 1) & 7, c = (lane & 1) * 4;
-  if (lane >= 32 || r >= cph || c >= cpw) return;
-  if (!((wmask >> (16 + pl)) & 1u)) return;
-  const int fx = mvx & 7, fy = mvy & 7;
-  const int y = cpy + r, x = cpx + c;  // macroblock-relative chroma
-  const int sy = mby * 8 + y + (mvy >> 3);
-  const int sx0 = mbx * 8 + cpx + (mvx >> 3);
-  const int sh = sx0 & 3;  // uniform
-  const uint32_t col = static_cast<uint32_t>((sx0 & ~3) + c), cw = static_cast<uint32_t>(src.cw);
-  const uint8_t* plane = pl ? src.cr : src.cb;
-  const uint32_t* r0 = reinterpret_cast<const uint32_t*>(plane + (static_cast<uint32_t>(clampc(sy, src.ch)) * cw + col));
-  const uint32_t* r1 =
-      reinterpret_cast<const uint32_t*>(plane + (static_cast<uint32_t>(clampc(sy + 1, src.ch)) * cw + col));
-  const uint32_t a0 = r0[0], a1 = r0[1], b0 = r1[0], b1 = r1[1];
-  // bytes sh..sh+3 and sh+4.. of each row, biased
-  const uint32_t ea = __builtin_amdgcn_alignbyte(a1, a0, sh) ^ 0x80808080u;
-  const uint32_t fa = __builtin_amdgcn_alignbyte(a1, a1, sh) ^ 0x80808080u;
-  const uint32_t eb = __builtin_amdgcn_alignbyte(b1, b0, sh) ^ 0x80808080u;
-  const uint32_t fb = __builtin_amdgcn_alignbyte(b1, b1, sh) ^ 0x80808080u;
-  const int32_t tx = (8 - fx) | (fx << 8);
-  const int32_t wy0 = 8 - fy, wy1 = fy;
-  int32_t v[4];
-  v[0] = (wy0 * __builtin_amdgcn_sdot4(static_cast<int32_t>(ea), tx, 0, false) +
-          wy1 * __builtin_amdgcn_sdot4(static_cast<int32_t>(eb), tx, 0, false) + (8192 + 32)) >> 6;
+  // ---- chroma samples
+  if (lane < 32 && cr < cph && cc < cpw && ((r.wmask >> (16 + pl)) & 1u)) {
+    const int cfx = mvx & 7, cfy = mvy & 7;
+    const uint32_t ea = __builtin_amdgcn_alignbyte(ca[1], ca[0], csh) ^ 0x80808080u;
+    const uint32_t fa = __builtin_amdgcn_alignbyte(ca[1], ca[1], csh) ^ 0x80808080u;
+    const uint32_t eb = __builtin_amdgcn_alignbyte(cb[1], cb[0], csh) ^ 0x80808080u;
+    const uint32_t fb = __builtin_amdgcn_alignbyte(cb[1], cb[1], csh) ^ 0x80808080u;
+    const int32_t tx = (8 - cfx) | (cfx << 8);
+    const int32_t wy0 = 8 - cfy, wy1 = cfy;
+    int32_t u[4];
+    u[0] = (wy0 * __builtin_amdgcn_sdot4(static_cast<int32_t>(ea), tx, 0, false) +
+            wy1 * __builtin_amdgcn_sdot4(static_cast<int32_t>(eb), tx, 0, false) + (8192 + 32)) >> 6;
 #pragma unroll
-  for (int j = 1; j < 4; j++) {
-    const uint32_t wa = __builtin_amdgcn_alignbyte(fa, ea, j), wb = __builtin_amdgcn_alignbyte(fb, eb, j);
-    v[j] = (wy0 * __builtin_amdgcn_sdot4(static_cast<int32_t>(wa), tx, 0, false) +
-            wy1 * __builtin_amdgcn_sdot4(static_cast<int32_t>(wb), tx, 0, false) + (8192 + 32)) >> 6;
-  }
-  if (has_res && (cbp & (pl ? kCBPCr : kCBPCb))) {
-    const int32_t* res = ws.coef + 256 + pl * 64 + y * 8 + x;
+    for (int j = 1; j < 4; j++) {
+      const uint32_t wa = __builtin_amdgcn_alignbyte(fa, ea, j), wb = __builtin_amdgcn_alignbyte(fb, eb, j);
+      u[j] = (wy0 * __builtin_amdgcn_sdot4(static_cast<int32_t>(wa), tx, 0, false) +
+              wy1 * __builtin_amdgcn_sdot4(static_cast<int32_t>(wb), tx, 0, false) + (8192 + 32)) >> 6;
+    }
+    if (has_res && (r.cbp & (pl ? kCBPCr : kCBPCb))) {
+      const int32_t* res = ws.coef + 256 + pl * 64 + cy * 8 + cx;
 #pragma unroll
-    for (int j = 0; j < 4; j++) v[j] += res[j];
+      for (int j = 0; j < 4; j++) u[j] += res[j];
+    }
+    uint8_t* dp = pl ? rec.cr : rec.cb;
+    *reinterpret_cast<uint32_t*>(dp + (static_cast<uint32_t>(mby * 8 + cy) * static_cast<uint32_t>(rec.cw) +
+                                       static_cast<uint32_t>(mbx * 8 + cx))) = clamp_pack4(u);
   }
-  uint8_t* dp = pl ? rec.cr : rec.cb;
-  *reinterpret_cast<uint32_t*>(dp + (static_cast<uint32_t>(mby * 8 + y) * static_cast<uint32_t>(rec.cw) +
-                                     static_cast<uint32_t>(mbx * 8 + x))) = clamp_pack4(v);
 }
 
 // One skip or inter macroblock (reconMCMB / reconInterMB / reconInterChroma,
-// decode_inter.go:64-152): per partition, the reference window (the
-// partition plus the 4-tap margin, coordinates clamped to the padded plane,
-// motioncomp.go:37-45) is staged in LDS and filtered separably (horizontal
-// sums over every window row, then the vertical sum and (v + 2048) >> 12; a
-// {0,64,0,0} tap row makes this equal the 1-D and copy cases of
-// motioncomp.go:49-97).  The residual (dequantised, inverse transformed) is
-// added and only the blocks of the write mask are stored.
+// decode_inter.go:64-152): the residual (dequantised, inverse transformed)
+// first, then motion compensation per partition with the residual added;
+// only the blocks of the write mask are stored.
 __device__ __forceinline__ void recon_inter_mb(const uint8_t* __restrict__ desc, const MBRec* __restrict__ recs,
                                                const uint32_t* __restrict__ coefs, ConstJob& job,
                                                const int first_mb) {
   __shared__ InterScratch scratch[4];
-  __shared__ int32_t s_dct[768];
-  LdsI32* dct = (LdsI32*)s_dct;
-  const FrameHdr* hdr = reinterpret_cast<const FrameHdr*>(desc);
   // Everything per macroblock is wave-uniform: scalar registers and loads.
   const int lane = lane_id(), wave = static_cast<int>(sgpr(threadIdx.x >> 6));
-  const int mbc = hdr->mbc, nmb = mbc * hdr->mbr;
+  const int mbc = job.mbc, nmb = mbc * job.mbr;
   const int mb = first_mb + wave;
+  if (mb >= nmb) return;
   // Macroblocks a failed tile stream never reached carry kind kKindUnreached.
-  const MBRec r = recs[mb < nmb ? mb : nmb - 1];
-  const bool active = mb < nmb && r.kind < 2;
-  const bool has_res = active && r.kind == 1 && r.coef_count > 0;
-  // The DCT matrices are staged only by workgroups with a residual to invert.
-  __shared__ uint32_t s_res[4];
-  if (lane == 0) s_res[wave] = has_res ? 1u : 0u;
-  __syncthreads();
-  if (s_res[0] | s_res[1] | s_res[2] | s_res[3]) load_dct(dct);
-  if (!active) return;
+  const MBRec r = recs[mb];
+  if (r.kind >= 2) return;
+  const bool has_res = r.kind == 1 && r.coef_count > 0;
   const int mby = static_cast<int>(sgpr(static_cast<uint32_t>(mb / mbc))), mbx = mb - mby * mbc;
   InterScratch& S = scratch[wave];
-  WaveScratch& ws = S.ws;
   // (the reference is chosen by address in constant memory: selecting
   // between two copies of the plane table would put them in scratch)
   const bool golden = sgpr(static_cast<uint32_t>(r.kind == 1 && r.refsel == kRefGolden)) != 0;
   const Planes rec = planes_of(job.rec);
   const Planes src = planes_of(golden ? job.gold : job.ref);
   const int lgn = 2 + (r.tx > 2 ? 2 : r.tx);  // log2 of the luma transform size
-  const int n = 1 << lgn;
-  const bool luma_res = has_res && (r.cbp & kCBPLuma);
   if (has_res) {
-    residual_mb(ws, dct, coefs, r, n, hdr->qdc, hdr->qac, lane);
+    const FrameHdr* hdr = reinterpret_cast<const FrameHdr*>(desc);
+    residual_mb_wave(S.ws, (LdsI32*)S.mats, coefs, r, 1 << lgn, hdr->qdc, hdr->qac, lane);
     wave_sync();  // the residual is read across lanes below
   }
   const int part = r.kind == 1 ? r.part : 0;
@@ -2859,96 +2853,16 @@ __device__ __forceinline__ void recon_inter_mb(const uint8_t* __restrict__ desc,
     const int px = rc & 0xFF, py = (rc >> 8) & 0xFF, pwd = (rc >> 16) & 0xFF, pht = rc >> 24;
     const uint32_t mvw = sgpr(pi == 0 ? mvw0 : (pi == 1 ? mvw1 : (pi == 2 ? mvw2 : mvw3)));
     const int mvx = static_cast<int16_t>(mvw & 0xFFFFu), mvy = static_cast<int16_t>(mvw >> 16);
-    // ---- luma
-    if (const int sx0 = mbx * 16 + px + (mvx >> 2) - 1; sx0 >= 0 && sx0 + pwd + 8 <= src.pw) {
-      luma_part_fast(src, rec, ws, luma_res, lgn, r.wmask, mbx, mby, px, py, pwd, pht, mvx, mvy, lane);
-    } else {
-      const int fx = mvx & 3, fy = mvy & 3;
-      const int ox = mbx * 16 + px + (mvx >> 2) - 1, oy = mby * 16 + py + (mvy >> 2) - 1;
-      const int ww = pwd + 3, wh = pht + 3;
-      for (int i = lane; i < wh * kWinStride; i += 64) {
-        const int rr = i / kWinStride, cc = i - rr * kWinStride;
-        if (cc < ww)
-          S.win[i] = src.y[static_cast<size_t>(clampc(oy + rr, src.ph)) * src.pw + clampc(ox + cc, src.pw)];
-      }
-      wave_sync();
-      const int32_t h0 = c_taps[fx][0], h1 = c_taps[fx][1], h2 = c_taps[fx][2], h3 = c_taps[fx][3];
-      for (int i = lane; i < wh * 16; i += 64) {
-        const int rr = i >> 4, c = i & 15;
-        if (c < pwd) {
-          const uint8_t* w = S.win + rr * kWinStride + c;
-          S.hbuf[i] = h0 * w[0] + h1 * w[1] + h2 * w[2] + h3 * w[3];
-        }
-      }
-      wave_sync();
-      const int32_t v0 = c_taps[fy][0], v1 = c_taps[fy][1], v2 = c_taps[fy][2], v3 = c_taps[fy][3];
-      const int gs = pwd == 16 ? 2 : 1;  // log2 of the 4-sample groups per row
-      if (lane < (pht << gs)) {
-        const int rr = lane >> gs, c0 = (lane & ((1 << gs) - 1)) * 4;
-        const int y = py + rr, x = px + c0;
-        const int blk = ((y >> lgn) << (4 - lgn)) + (x >> lgn);
-        if ((r.wmask >> blk) & 1u) {
-          uint32_t word = 0;
-#pragma unroll
-          for (int k = 0; k < 4; k++) {
-            const int32_t* h = S.hbuf + rr * 16 + c0 + k;
-            int32_t p = clamp255((v0 * h[0] + v1 * h[16] + v2 * h[32] + v3 * h[48] + 2048) >> 12);
-            if (luma_res) p = clamp255(p + ws.coef[(blk << (2 * lgn)) + ((y & (n - 1)) << lgn) + ((x + k) & (n - 1))]);
-            word |= static_cast<uint32_t>(p) << (8 * k);
-          }
-          *reinterpret_cast<uint32_t*>(rec.y + static_cast<size_t>(mby * 16 + y) * rec.pw + mbx * 16 + x) = word;
-        }
-      }
-    }
-    // ---- chroma (eighth-pel bilinear, the same vector in chroma units)
-    if (const int csx = mbx * 8 + (px >> 1) + (mvx >> 3); csx >= 0 && csx + (pwd >> 1) + 8 <= src.cw) {
-      chroma_part_fast(src, rec, ws, has_res, r.cbp, r.wmask, mbx, mby, px >> 1, py >> 1, pwd >> 1, pht >> 1, mvx,
-                       mvy, lane);
-    } else {
-      const int fx = mvx & 7, fy = mvy & 7;
-      const int cpx = px >> 1, cpy = py >> 1, cpw = pwd >> 1, cph = pht >> 1;
-      const int ox = mbx * 8 + cpx + (mvx >> 3), oy = mby * 8 + cpy + (mvy >> 3);
-      const int ww = cpw + 1, wsz = kCWinStride * (cph + 1);
-      for (int i = lane; i < 2 * wsz; i += 64) {
-        const int pl = i >= wsz ? 1 : 0, j = i - pl * wsz;
-        const int rr = j / kCWinStride, cc = j - rr * kCWinStride;
-        if (cc < ww) {
-          const uint8_t* sp = pl ? src.cr : src.cb;
-          S.cwin[pl][j] = sp[static_cast<size_t>(clampc(oy + rr, src.ch)) * src.cw + clampc(ox + cc, src.cw)];
-        }
-      }
-      wave_sync();
-      const int32_t wa = (8 - fx) * (8 - fy), wb = fx * (8 - fy), wc = (8 - fx) * fy, wd = fx * fy;
-      const int ps = cpw == 8 ? 2 : 1;  // log2 of the 2-sample pairs per row
-      const int per_plane = cph << ps;
-      if (lane < 2 * per_plane) {
-        const int pl = lane >= per_plane ? 1 : 0, j = lane - pl * per_plane;
-        const int rr = j >> ps, c0 = (j & ((1 << ps) - 1)) * 2;
-        if ((r.wmask >> (16 + pl)) & 1u) {
-          const bool coded = has_res && (r.cbp & (pl ? kCBPCr : kCBPCb));
-          const uint8_t* w = S.cwin[pl] + rr * kCWinStride + c0;
-          uint32_t half = 0;
-#pragma unroll
-          for (int k = 0; k < 2; k++) {
-            int32_t p = (wa * w[k] + wb * w[k + 1] + wc * w[kCWinStride + k] + wd * w[kCWinStride + k + 1] + 32) >> 6;
-            if (coded) p = clamp255(p + ws.coef[256 + pl * 64 + (cpy + rr) * 8 + cpx + c0 + k]);
-            half |= static_cast<uint32_t>(p) << (8 * k);
-          }
-          uint8_t* dp = pl ? rec.cr : rec.cb;
-          *reinterpret_cast<uint16_t*>(dp + static_cast<size_t>(mby * 8 + cpy + rr) * rec.cw + mbx * 8 + cpx + c0) =
-              static_cast<uint16_t>(half);
-        }
-      }
-      wave_sync();
-    }
+    mc_part(src, rec, S.ws, r, has_res, lgn, mbx, mby, px, py, pwd, pht, mvx, mvy, lane);
   }
 }
 
-__global__ __launch_bounds__(256) void ebiv_recon_inter(const FrameJob* __restrict__ jobs,
+__global__ __launch_bounds__(256, 8) void ebiv_recon_inter(const FrameJob* __restrict__ jobs,
                                                         const WorkItem* __restrict__ items) {
   const WorkItem it = item_of(items);
   ConstJob& job = job_of(jobs, it.job);
-  recon_inter_mb(job.desc, job.recs, job.coefs, job, static_cast<int>(it.idx));
+  for (uint32_t k = 0; k < kInterGroupMbs; k += 4)
+    recon_inter_mb(job.desc, job.recs, job.coefs, job, static_cast<int>(it.idx + k));
 }
 
 // ============================================================================

@@ -756,7 +756,8 @@ int Engine::run_step(std::vector<Session*>& ss, std::vector<std::vector<StepFram
     j.recs = ps->recs.d;
     j.coefs = ps->coefs.d;
     j.status = d_step_status.d + ps->status_off;
-    j.pad[0] = j.pad[1] = 0;
+    j.mbc = s.mbc;
+    j.mbr = s.mbr;
     if (r.ov) {  // the override predicts only from its key (codec.go:178-181)
       j.rec = s.pool[f.gtarget].p;
       j.ref = s.pool[f.target].p;
@@ -776,7 +777,7 @@ int Engine::run_step(std::vector<Session*>& ss, std::vector<std::vector<StepFram
     if (inter_part) {
       if (!ps->inter) return;
       const uint32_t nmb = static_cast<uint32_t>(s.mbc * s.mbr);
-      for (uint32_t m = 0; m < nmb; m += 4) h_items.h[w++] = WorkItem{static_cast<uint32_t>(j), m};
+      for (uint32_t m = 0; m < nmb; m += kInterGroupMbs) h_items.h[w++] = WorkItem{static_cast<uint32_t>(j), m};
     } else {
       for (int t = 0; t < ps->ntiles; t++) h_items.h[w++] = WorkItem{static_cast<uint32_t>(j), static_cast<uint32_t>(t)};
     }
@@ -788,7 +789,7 @@ int Engine::run_step(std::vector<Session*>& ss, std::vector<std::vector<StepFram
     const Session& s = *ss[jr.si];
     ps->status_off = nparse;
     nparse += static_cast<size_t>(ps->ntiles);
-    nitems += 2 * static_cast<size_t>(ps->ntiles) + (ps->inter ? (static_cast<size_t>(s.mbc) * s.mbr + 3) / 4 : 0);
+    nitems += 2 * static_cast<size_t>(ps->ntiles) + (ps->inter ? (static_cast<size_t>(s.mbc) * s.mbr + kInterGroupMbs - 1) / kInterGroupMbs : 0);
     max_tw = std::max(max_tw, ps->max_tile_w);
   }
   const size_t njobs = jrefs.size();
@@ -923,7 +924,7 @@ int Engine::run_step(std::vector<Session*>& ss, std::vector<std::vector<StepFram
         fill_job(h_jobs.h[0], jr);
         size_t n = 0;
         if (ps->inter)
-          for (uint32_t m = 0; m < static_cast<uint32_t>(s.mbc * s.mbr); m += 4) h_items.h[n++] = WorkItem{0, m};
+          for (uint32_t m = 0; m < static_cast<uint32_t>(s.mbc * s.mbr); m += kInterGroupMbs) h_items.h[n++] = WorkItem{0, m};
         const size_t ninter = n;
         for (int t = 0; t < ps->ntiles; t++) h_items.h[n++] = WorkItem{0, static_cast<uint32_t>(t)};
         if (ov && !golden_copy(jr)) return fail(EBIV_ERR_DEVICE, "golden copy failed");
