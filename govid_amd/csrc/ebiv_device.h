@@ -128,7 +128,8 @@ struct FrameJob {
   Planes rec, ref, gold;
 };
 
-// Macroblocks per inter-reconstruction work item: one per wavefront.
+// Macroblocks per inter-reconstruction work item: one per wavefront of a
+// 4-wavefront workgroup.
 constexpr uint32_t kInterGroupMbs = 4;
 
 // One workgroup's share of a launch: job index and a tile or macroblock index.

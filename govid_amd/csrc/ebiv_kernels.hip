@@ -2643,9 +2643,17 @@ __device__ __forceinline__ uint32_t part_rect(int part, int pi) {
 // 4-tap luma filter rows (motioncomp.go:30-35) as packed int8 for v_dot4.
 __constant__ uint32_t c_taps8[4] = {0x00004000u, 0xFD1237FAu, 0xF82828F8u, 0xFA3712FDu};
 
+// Four samples clamped to [0, 255] and packed into a dword, sample 0 in the
+// low byte.  The empty asm statements keep the compiler from fusing a
+// preceding shift, the clamp and the packing into v_ashr_pk_u8_i32: in
+// this position it produced wrong upper bytes (samples 2 and 3 of a dword,
+// seen as chroma mismatches against the oracle), so the fusion is blocked
+// outright.
 __device__ __forceinline__ uint32_t clamp_pack4(const int32_t (&v)[4]) {
-  return static_cast<uint32_t>(clamp255(v[0])) | (static_cast<uint32_t>(clamp255(v[1])) << 8) |
-         (static_cast<uint32_t>(clamp255(v[2])) << 16) | (static_cast<uint32_t>(clamp255(v[3])) << 24);
+  int32_t a = v[0], b = v[1], c = v[2], d = v[3];
+  asm("" : "+v"(a), "+v"(b), "+v"(c), "+v"(d));
+  return static_cast<uint32_t>(clamp255(a)) | (static_cast<uint32_t>(clamp255(b)) << 8) |
+         (static_cast<uint32_t>(clamp255(c)) << 16) | (static_cast<uint32_t>(clamp255(d)) << 24);
 }
 
 // Bytes col .. col + 4K - 1 of a plane row of width w as K dwords, each
@@ -2653,8 +2661,16 @@ __device__ __forceinline__ uint32_t clamp_pack4(const int32_t (&v)[4]) {
 // col is a multiple of 4, so a span inside the row is K aligned dword
 // loads; a span over an edge (lanes at the plane's left or right border)
 // is gathered byte by byte.
+// (Returned by value: output parameters of two inlined calls that the
+// compiler merges would become a pointer into scratch.)
 template <int K>
-__device__ __forceinline__ void load_span(const uint8_t* __restrict__ row, int col, int w, uint32_t (&d)[K]) {
+struct Span {
+  uint32_t d[K];
+};
+template <int K>
+__device__ __forceinline__ Span<K> load_span(const uint8_t* __restrict__ row, int col, int w) {
+  Span<K> out;
+  uint32_t(&d)[K] = out.d;
   if (col >= 0 && col + 4 * K <= w) {
     const uint32_t* p = reinterpret_cast<const uint32_t*>(row + col);
 #pragma unroll
@@ -2668,13 +2684,14 @@ __device__ __forceinline__ void load_span(const uint8_t* __restrict__ row, int c
       d[k] = v;
     }
   }
+  return out;
 }
 
 // Horizontal 4-tap sums of samples 0-3 from a row segment d0..d2 whose
 // first tap is byte sh, on samples biased by -128 (each sum is 128 * 64 low).
-__device__ __forceinline__ void luma_hsum4(const uint32_t (&d)[3], int sh, int32_t tx, int32_t (&h)[4]) {
-  const uint32_t e0 = __builtin_amdgcn_alignbyte(d[1], d[0], sh) ^ 0x80808080u;
-  const uint32_t e1 = __builtin_amdgcn_alignbyte(d[2], d[1], sh) ^ 0x80808080u;
+__device__ __forceinline__ void luma_hsum4(const Span<3>& s, int sh, int32_t tx, int32_t (&h)[4]) {
+  const uint32_t e0 = __builtin_amdgcn_alignbyte(s.d[1], s.d[0], sh) ^ 0x80808080u;
+  const uint32_t e1 = __builtin_amdgcn_alignbyte(s.d[2], s.d[1], sh) ^ 0x80808080u;
   h[0] = __builtin_amdgcn_sdot4(static_cast<int32_t>(e0), tx, 0, false);
   h[1] = __builtin_amdgcn_sdot4(static_cast<int32_t>(__builtin_amdgcn_alignbyte(e1, e0, 1)), tx, 0, false);
   h[2] = __builtin_amdgcn_sdot4(static_cast<int32_t>(__builtin_amdgcn_alignbyte(e1, e0, 2)), tx, 0, false);
@@ -2698,9 +2715,15 @@ __device__ __forceinline__ void luma_hsum4(const uint32_t (&d)[3], int sh, int32
 //  - Chroma: the eighth-pel bilinear sum as two horizontal 2-tap sums (one
 //    v_dot4 each, weights 8 - fx and fx) and the vertical blend (8 - fy,
 //    fy); (sum + 32) >> 6 is the reference's rounding.
-__device__ __forceinline__ void mc_part(const Planes& src, const Planes& rec, const WaveScratch& ws, const MBRec& r,
-                                        bool has_res, int lgn, int mbx, int mby, int px, int py, int pwd, int pht,
-                                        int mvx, int mvy, int lane) {
+// The reference rows of one lane (luma: up to four 12-byte spans; chroma:
+// one 8-byte span of each of two rows).
+struct McRows {
+  Span<3> q0, q1, q2, q3;
+  Span<2> ca, cb;
+};
+
+__device__ __forceinline__ void mc_load(const Planes& src, int mbx, int mby, int px, int py, int mvx, int mvy,
+                                        int lane, McRows& m) {
   // ---- luma loads
   const int fx = mvx & 3, fy = mvy & 3;
   const int lr = lane >> 2, lc = (lane & 3) * 4;
@@ -2714,54 +2737,60 @@ __device__ __forceinline__ void mc_part(const Planes& src, const Planes& rec, co
   auto lrow = [&](int yy) {
     return src.y + static_cast<uint32_t>(clampc(yy, src.ph)) * static_cast<uint32_t>(src.pw);
   };
-  uint32_t q0[3] = {0, 0, 0}, q1[3] = {0, 0, 0}, q2[3] = {0, 0, 0}, q3[3] = {0, 0, 0};
-  if (copy) {
-    uint32_t t[2];
-    load_span<2>(lrow(sy), col, src.pw, t);
-    q1[0] = t[0];
-    q1[1] = t[1];
-  } else if (fy == 0) {
-    load_span<3>(lrow(sy), col, src.pw, q1);
+  if (fy == 0) {  // one row (the full-pel copy uses its first 8 bytes)
+    m.q1 = load_span<3>(lrow(sy), col, src.pw);
+    asm volatile("");  // keeps the compiler from merging this load with the 2-D path's last one
   } else {
-    load_span<3>(lrow(sy - 1), col, src.pw, q0);
-    load_span<3>(lrow(sy), col, src.pw, q1);
-    load_span<3>(lrow(sy + 1), col, src.pw, q2);
-    load_span<3>(lrow(sy + 2), col, src.pw, q3);
+    m.q0 = load_span<3>(lrow(sy - 1), col, src.pw);
+    m.q1 = load_span<3>(lrow(sy), col, src.pw);
+    m.q2 = load_span<3>(lrow(sy + 1), col, src.pw);
+    m.q3 = load_span<3>(lrow(sy + 2), col, src.pw);
   }
   // ---- chroma loads (lanes 32-63 repeat lanes 0-31)
+  const int cpx = px >> 1, cpy = py >> 1;
+  const int pl = (lane >> 4) & 1, cr = (lane >> 1) & 7, cc = (lane & 1) * 4;
+  const int csy = mby * 8 + cpy + cr + (mvy >> 3);
+  const int csx0 = mbx * 8 + cpx + (mvx >> 3);
+  const int ccol = (csx0 & ~3) + cc;
+  const uint8_t* plane = pl ? src.cr : src.cb;
+  m.ca = load_span<2>(plane + static_cast<uint32_t>(clampc(csy, src.ch)) * static_cast<uint32_t>(src.cw), ccol, src.cw);
+  m.cb = load_span<2>(plane + static_cast<uint32_t>(clampc(csy + 1, src.ch)) * static_cast<uint32_t>(src.cw), ccol,
+                      src.cw);
+}
+
+__device__ __forceinline__ void mc_finish(const Planes& rec, const WaveScratch& ws, const MBRec& r, bool has_res,
+                                          int lgn, int mbx, int mby, int px, int py, int pwd, int pht, int mvx,
+                                          int mvy, int lane, const McRows& m) {
+  const int fx = mvx & 3, fy = mvy & 3;
+  const int lr = lane >> 2, lc = (lane & 3) * 4;
+  const int y = py + lr, x = px + lc;  // macroblock-relative
+  const int sx0 = mbx * 16 + px + (mvx >> 2);
+  const bool copy = (fx | fy) == 0;
+  const int sh = (copy ? sx0 : sx0 - 1) & 3;
   const int cpx = px >> 1, cpy = py >> 1, cpw = pwd >> 1, cph = pht >> 1;
   const int pl = (lane >> 4) & 1, cr = (lane >> 1) & 7, cc = (lane & 1) * 4;
   const int cy = cpy + cr, cx = cpx + cc;
-  const int csy = mby * 8 + cy + (mvy >> 3);
-  const int csx0 = mbx * 8 + cpx + (mvx >> 3);
-  const int csh = csx0 & 3;
-  const int ccol = (csx0 & ~3) + cc;
-  const uint8_t* plane = pl ? src.cr : src.cb;
-  uint32_t ca[2], cb[2];
-  load_span<2>(plane + static_cast<uint32_t>(clampc(csy, src.ch)) * static_cast<uint32_t>(src.cw), ccol, src.cw, ca);
-  load_span<2>(plane + static_cast<uint32_t>(clampc(csy + 1, src.ch)) * static_cast<uint32_t>(src.cw), ccol, src.cw,
-               cb);
-
+  const int csh = (mbx * 8 + cpx + (mvx >> 3)) & 3;
   // ---- luma samples
   int32_t v[4];
   if (copy) {
-    const uint32_t w = __builtin_amdgcn_alignbyte(q1[1], q1[0], sh);
+    const uint32_t w = __builtin_amdgcn_alignbyte(m.q1.d[1], m.q1.d[0], sh);
 #pragma unroll
     for (int j = 0; j < 4; j++) v[j] = static_cast<int32_t>((w >> (8 * j)) & 0xFFu);
   } else {
     const int32_t tx = static_cast<int32_t>(c_taps8[fx]);
     if (fy == 0) {
       int32_t h[4];
-      luma_hsum4(q1, sh, tx, h);
+      luma_hsum4(m.q1, sh, tx, h);
 #pragma unroll
       for (int j = 0; j < 4; j++) v[j] = clamp255((h[j] + (8192 + 32)) >> 6);
     } else {
       const int32_t t0 = c_taps[fy][0], t1 = c_taps[fy][1], t2 = c_taps[fy][2], t3 = c_taps[fy][3];
       int32_t h0[4], h1[4], h2[4], h3[4];
-      luma_hsum4(q0, sh, tx, h0);
-      luma_hsum4(q1, sh, tx, h1);
-      luma_hsum4(q2, sh, tx, h2);
-      luma_hsum4(q3, sh, tx, h3);
+      luma_hsum4(m.q0, sh, tx, h0);
+      luma_hsum4(m.q1, sh, tx, h1);
+      luma_hsum4(m.q2, sh, tx, h2);
+      luma_hsum4(m.q3, sh, tx, h3);
       // |h| < 2^14 and |tap| <= 64: 24-bit products are exact
 #pragma unroll
       for (int j = 0; j < 4; j++)
@@ -2783,10 +2812,10 @@ __device__ __forceinline__ void mc_part(const Planes& src, const Planes& rec, co
   // ---- chroma samples
   if (lane < 32 && cr < cph && cc < cpw && ((r.wmask >> (16 + pl)) & 1u)) {
     const int cfx = mvx & 7, cfy = mvy & 7;
-    const uint32_t ea = __builtin_amdgcn_alignbyte(ca[1], ca[0], csh) ^ 0x80808080u;
-    const uint32_t fa = __builtin_amdgcn_alignbyte(ca[1], ca[1], csh) ^ 0x80808080u;
-    const uint32_t eb = __builtin_amdgcn_alignbyte(cb[1], cb[0], csh) ^ 0x80808080u;
-    const uint32_t fb = __builtin_amdgcn_alignbyte(cb[1], cb[1], csh) ^ 0x80808080u;
+    const uint32_t ea = __builtin_amdgcn_alignbyte(m.ca.d[1], m.ca.d[0], csh) ^ 0x80808080u;
+    const uint32_t fa = __builtin_amdgcn_alignbyte(m.ca.d[1], m.ca.d[1], csh) ^ 0x80808080u;
+    const uint32_t eb = __builtin_amdgcn_alignbyte(m.cb.d[1], m.cb.d[0], csh) ^ 0x80808080u;
+    const uint32_t fb = __builtin_amdgcn_alignbyte(m.cb.d[1], m.cb.d[1], csh) ^ 0x80808080u;
     const int32_t tx = (8 - cfx) | (cfx << 8);
     const int32_t wy0 = 8 - cfy, wy1 = cfy;
     int32_t u[4];
@@ -2809,25 +2838,29 @@ __device__ __forceinline__ void mc_part(const Planes& src, const Planes& rec, co
   }
 }
 
+__device__ __forceinline__ void mc_part(const Planes& src, const Planes& rec, const WaveScratch& ws, const MBRec& r,
+                                        bool has_res, int lgn, int mbx, int mby, int px, int py, int pwd, int pht,
+                                        int mvx, int mvy, int lane) {
+  McRows m{};
+  mc_load(src, mbx, mby, px, py, mvx, mvy, lane, m);
+  mc_finish(rec, ws, r, has_res, lgn, mbx, mby, px, py, pwd, pht, mvx, mvy, lane, m);
+}
+
 // One skip or inter macroblock (reconMCMB / reconInterMB / reconInterChroma,
 // decode_inter.go:64-152): the residual (dequantised, inverse transformed)
 // first, then motion compensation per partition with the residual added;
 // only the blocks of the write mask are stored.
 __device__ __forceinline__ void recon_inter_mb(const uint8_t* __restrict__ desc, const MBRec* __restrict__ recs,
-                                               const uint32_t* __restrict__ coefs, ConstJob& job,
-                                               const int first_mb) {
-  __shared__ InterScratch scratch[4];
+                                               const uint32_t* __restrict__ coefs, ConstJob& job, const int mb,
+                                               InterScratch& S, const int lane) {
   // Everything per macroblock is wave-uniform: scalar registers and loads.
-  const int lane = lane_id(), wave = static_cast<int>(sgpr(threadIdx.x >> 6));
   const int mbc = job.mbc, nmb = mbc * job.mbr;
-  const int mb = first_mb + wave;
   if (mb >= nmb) return;
   // Macroblocks a failed tile stream never reached carry kind kKindUnreached.
   const MBRec r = recs[mb];
   if (r.kind >= 2) return;
   const bool has_res = r.kind == 1 && r.coef_count > 0;
   const int mby = static_cast<int>(sgpr(static_cast<uint32_t>(mb / mbc))), mbx = mb - mby * mbc;
-  InterScratch& S = scratch[wave];
   // (the reference is chosen by address in constant memory: selecting
   // between two copies of the plane table would put them in scratch)
   const bool golden = sgpr(static_cast<uint32_t>(r.kind == 1 && r.refsel == kRefGolden)) != 0;
@@ -2858,11 +2891,12 @@ __device__ __forceinline__ void recon_inter_mb(const uint8_t* __restrict__ desc,
 }
 
 __global__ __launch_bounds__(256, 8) void ebiv_recon_inter(const FrameJob* __restrict__ jobs,
-                                                        const WorkItem* __restrict__ items) {
+                                                           const WorkItem* __restrict__ items) {
+  __shared__ InterScratch scratch[4];
   const WorkItem it = item_of(items);
   ConstJob& job = job_of(jobs, it.job);
-  for (uint32_t k = 0; k < kInterGroupMbs; k += 4)
-    recon_inter_mb(job.desc, job.recs, job.coefs, job, static_cast<int>(it.idx + k));
+  const int lane = lane_id(), wave = static_cast<int>(sgpr(threadIdx.x >> 6));
+  recon_inter_mb(job.desc, job.recs, job.coefs, job, static_cast<int>(it.idx) + wave, scratch[wave], lane);
 }
 
 // ============================================================================
