@@ -263,12 +263,14 @@ def main():
     ap.add_argument("--serve-window", type=int, default=1,
                     help="frames per session per engine step when serving (1 keeps p99 frame latency lowest)")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="only run a plain 256 MiB device-to-device copy (scale for HBM byte counters)")
+                    help="only run a plain 2 GiB device-to-device copy (scale for HBM byte counters)")
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
 
-        n = 256 << 20
+        # 2 GiB per buffer, well above the 256 MB last-level cache, so each
+        # copy streams from and to HBM
+        n = 2 << 30
         x = torch.randint(0, 255, (n,), dtype=torch.uint8, device="cuda")
         y = torch.empty_like(x)
         for _ in range(3):

@@ -2109,6 +2109,16 @@ __device__ __forceinline__ WorkItem item_of(const WorkItem* items) {
   return WorkItem{it.job, it.idx};
 }
 
+// Workgroups are dispatched to the 8 XCDs of the chip round robin
+// (workgroup b to XCD b % 8), and each XCD has its own L2.  This bijection
+// of [0, n) gives XCD x a contiguous range of the work items instead, so
+// neighbouring macroblocks (which share reference lines) meet in one L2.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_contiguous(uint32_t b, uint32_t n) {
+  const uint32_t q = n / kXcds, r = n % kXcds, x = b % kXcds, k = b / kXcds;
+  return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
 // Marks the records of tile td from local macroblock `done` on as never
 // reached (kind kKindUnreached), so that reconstruction needs no tile status.
 __device__ void mark_unreached(MBRec* __restrict__ recs, uint32_t mbc, const TileDesc& td, uint32_t done, int lane) {
@@ -2893,7 +2903,7 @@ __device__ __forceinline__ void recon_inter_mb(const uint8_t* __restrict__ desc,
 __global__ __launch_bounds__(256, 8) void ebiv_recon_inter(const FrameJob* __restrict__ jobs,
                                                            const WorkItem* __restrict__ items) {
   __shared__ InterScratch scratch[4];
-  const WorkItem it = item_of(items);
+  const ConstItem& it = ((ConstItem*)items)[xcd_contiguous(blockIdx.x, gridDim.x)];
   ConstJob& job = job_of(jobs, it.job);
   const int lane = lane_id(), wave = static_cast<int>(sgpr(threadIdx.x >> 6));
   recon_inter_mb(job.desc, job.recs, job.coefs, job, static_cast<int>(it.idx) + wave, scratch[wave], lane);
