@@ -30,6 +30,21 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` in `workload` from the committed
+    rocprofv3 counter summary (profiles/hbm_traffic.json): FETCH_SIZE and
+    WRITE_SIZE in KiB per dispatch, each scaled by what the same counter
+    reads for a plain device copy of known size.  None when not measured."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "hbm_traffic.json")) as fh:
+            t = json.load(fh)
+        k = t["workloads"][workload][kernel]
+        cal = t["calibration"]
+        return round(1024 * (k["fetch_kib"] * cal["fetch_scale"] + k["write_kib"] * cal["write_scale"]))
+    except (OSError, KeyError, ValueError):
+        return None
 STREAM = os.path.join(ROOT, "bench_data", "parallax_1080p.ebiv")
 
 
@@ -320,9 +335,11 @@ def main():
     parse_avg_s = kavg_us(kt, "parse") / 1e6
     roof = {"bound": "hbm", "kernel": "ebiv_parse", "achieved": round(parse_bytes / parse_avg_s / 1e9, 3),
             "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(parse_bytes / parse_avg_s / HBM_PEAK, 6),
-            "traffic": None, "bytes_per_launch": round(parse_bytes), "avg_launch_us": round(parse_avg_s * 1e6, 2),
+            "traffic": measured_traffic("plain_1080p_parallax", "ebiv_parse"), "bytes_per_launch": round(parse_bytes), "avg_launch_us": round(parse_avg_s * 1e6, 2),
             "note": "latency-bound serial rANS chains (one wavefront per tile); reported in Msymbols/s"}
-    recon = {"ebiv_recon_inter": inter_roofline(kt), "ebiv_recon_intra": {"avg_launch_us": round(kavg_us(kt, "intra"), 2)}}
+    recon = {"ebiv_recon_inter": dict(inter_roofline(kt) or {},
+                                      traffic=measured_traffic("plain_1080p_parallax", "ebiv_recon_inter")),
+             "ebiv_recon_intra": {"avg_launch_us": round(kavg_us(kt, "intra"), 2)}}
 
     out = {
         "metric": "EBIV 1080p decoded frames/s bit-exact, batched vs plain GPU; p50/p99 frame ms",
@@ -363,7 +380,9 @@ def main():
             "frames_differing": bt["bad"],
             "kernel_avg_us": {k: round(kavg_us(bk, k), 2) for k in ("parse", "inter", "intra")},
             "kernel_launches": {k: bk[f"{k}_n"] for k in ("parse", "inter", "intra")},
-            "recon": {"ebiv_recon_inter": inter_roofline(bk)},
+            "recon": {"ebiv_recon_inter": dict(inter_roofline(bk) or {},
+                                               traffic=measured_traffic("batched_64x1080p_parallax_w8",
+                                                                        "ebiv_recon_inter"))},
         }
     for d in dev:
         d.close()
