@@ -39,7 +39,8 @@ def build_gpu() -> str:
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "libebivgpu.so")
     tmp = out + ".tmp"
-    srcs = [os.path.join(CSRC, "ebiv_kernels.hip"), os.path.join(CSRC, "ebiv_runtime.cpp")]
+    srcs = [os.path.join(CSRC, "ebiv_kernels.hip"), os.path.join(CSRC, "ebiv_runtime.cpp"),
+            os.path.join(CSRC, "ebiv_container.cpp")]
     _run([HIPCC, *HIP_FLAGS, "-o", tmp, *srcs])
     os.replace(tmp, out)
     return out

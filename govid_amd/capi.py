@@ -8,6 +8,9 @@ ERR_CORRUPT = 1
 ERR_UNSUPPORTED = 2
 ERR_NOKEYFRAME = 3
 ERR_DIMENSIONS = 4
+ERR_BAD_MAGIC = 5
+ERR_VERSION = 6
+EOF = 7
 ERR_ARGUMENT = 16
 ERR_DEVICE = 17
 
@@ -21,7 +24,22 @@ EXPORTS = (
     "ebiv_batch_submit", "ebiv_batch_flush", "ebiv_batch_pending", "ebiv_batch_step", "ebiv_batch_stats",
     "ebiv_batch_set_timing", "ebiv_batch_kernel_times", "ebiv_batch_frame_rgba", "ebiv_session_bytes",
     "ebiv_batch_set_schedule", "ebiv_batch_get_schedule",
+    "ebiv_reader_open", "ebiv_reader_destroy", "ebiv_reader_header", "ebiv_reader_frame_count", "ebiv_reader_index",
+    "ebiv_reader_index_scanned", "ebiv_reader_duration", "ebiv_reader_next_packet", "ebiv_reader_seek",
+    "ebiv_reader_close", "ebiv_frame_time", "ebiv_frame_index_at",
 )
+
+
+class FileHeader(C.Structure):
+    _fields_ = [("version", C.c_uint16), ("tile_cols", C.c_uint16), ("tile_rows", C.c_uint16),
+                ("reserved", C.c_uint16), ("width", C.c_int32), ("height", C.c_int32),
+                ("fps_num", C.c_uint32), ("fps_den", C.c_uint32), ("chroma_format", C.c_uint8),
+                ("bit_depth", C.c_uint8), ("color_range", C.c_uint8), ("color_primaries", C.c_uint8),
+                ("flags", C.c_uint32)]
+
+
+class IndexEntry(C.Structure):
+    _fields_ = [("offset", C.c_uint64), ("size", C.c_uint32), ("keyframe", C.c_int32)]
 
 
 class Schedule(C.Structure):
@@ -118,4 +136,29 @@ def bind(L):
     L.ebiv_batch_frame_rgba.argtypes = [vp, C.POINTER(Frame), vp, C.c_int]
     L.ebiv_session_bytes.restype = sz
     L.ebiv_session_bytes.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
+    L.ebiv_reader_open.restype = C.c_int
+    L.ebiv_reader_open.argtypes = [C.c_char_p, sz, C.POINTER(vp), C.c_char_p, sz]
+    L.ebiv_reader_destroy.restype = None
+    L.ebiv_reader_destroy.argtypes = [vp]
+    L.ebiv_reader_header.restype = C.c_int
+    L.ebiv_reader_header.argtypes = [vp, C.POINTER(FileHeader)]
+    L.ebiv_reader_frame_count.restype = C.c_int64
+    L.ebiv_reader_frame_count.argtypes = [vp]
+    L.ebiv_reader_index.restype = C.c_int
+    L.ebiv_reader_index.argtypes = [vp, C.POINTER(IndexEntry), sz, C.POINTER(sz)]
+    L.ebiv_reader_index_scanned.restype = C.c_int
+    L.ebiv_reader_index_scanned.argtypes = [vp]
+    L.ebiv_reader_duration.restype = C.c_int64
+    L.ebiv_reader_duration.argtypes = [vp]
+    L.ebiv_reader_next_packet.restype = C.c_int
+    L.ebiv_reader_next_packet.argtypes = [vp, C.POINTER(C.c_void_p), C.POINTER(sz), C.POINTER(C.c_int64),
+                                          C.POINTER(C.c_int32)]
+    L.ebiv_reader_seek.restype = C.c_int
+    L.ebiv_reader_seek.argtypes = [vp, C.c_int64, C.POINTER(C.c_int64)]
+    L.ebiv_reader_close.restype = C.c_int
+    L.ebiv_reader_close.argtypes = [vp]
+    L.ebiv_frame_time.restype = C.c_int64
+    L.ebiv_frame_time.argtypes = [C.c_uint32, C.c_uint32, C.c_int64]
+    L.ebiv_frame_index_at.restype = C.c_int64
+    L.ebiv_frame_index_at.argtypes = [C.c_uint32, C.c_uint32, C.c_int64]
     return L

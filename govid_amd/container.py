@@ -218,3 +218,84 @@ class Reader:
 
     def close(self):
         self.closed = True
+
+
+class NativeReader:
+    """The same reader in C++ behind the C ABI (ebiv_reader_*,
+    govid_amd/csrc/ebiv_container.cpp), with Reader's interface."""
+
+    _ERRORS = {1: ErrCorruptContainer, 2: ErrUnsupportedContainer, 5: ErrBadMagic, 6: ErrVersion}
+
+    def __init__(self, data: bytes):
+        import ctypes as C
+
+        from . import capi, native
+
+        self._C, self._capi = C, capi
+        self._L = native.gpu_lib()
+        h = C.c_void_p()
+        msg = C.create_string_buffer(256)
+        rc = self._L.ebiv_reader_open(bytes(data), len(data), C.byref(h), msg, 256)
+        if rc:
+            raise self._ERRORS.get(rc, ContainerError)(msg.value.decode(errors="replace"))
+        self._h = h
+        fh = capi.FileHeader()
+        self._L.ebiv_reader_header(h, C.byref(fh))
+        self.header = FileHeader(fh.version, fh.width, fh.height, fh.fps_num, fh.fps_den, fh.chroma_format,
+                                 fh.bit_depth, fh.color_range, fh.color_primaries, fh.tile_cols, fh.tile_rows,
+                                 fh.flags)
+        n = C.c_size_t()
+        self._L.ebiv_reader_index(h, None, 0, C.byref(n))
+        arr = (capi.IndexEntry * max(n.value, 1))()
+        self._L.ebiv_reader_index(h, arr, n.value, C.byref(n))
+        self.index = [IndexEntry(e.offset, e.size, bool(e.keyframe)) for e in arr[:n.value]]
+        self.index_scanned = bool(self._L.ebiv_reader_index_scanned(h))
+
+    @classmethod
+    def open(cls, path: str) -> "NativeReader":
+        with open(path, "rb") as f:
+            return cls(f.read())
+
+    def frame_count(self) -> int:
+        return int(self._L.ebiv_reader_frame_count(self._h))
+
+    def duration(self) -> int:
+        return int(self._L.ebiv_reader_duration(self._h))
+
+    def next_packet(self):
+        """NextPacket; None at EOF."""
+        C = self._C
+        p, n, ts, key = C.c_void_p(), C.c_size_t(), C.c_int64(), C.c_int32()
+        rc = self._L.ebiv_reader_next_packet(self._h, C.byref(p), C.byref(n), C.byref(ts), C.byref(key))
+        if rc == self._capi.EOF:
+            return None
+        if rc:
+            raise ErrCorruptContainer("packet outside the file")
+        return Packet(C.string_at(p.value, n.value), ts.value, bool(key.value))
+
+    def seek(self, t: int) -> int:
+        C = self._C
+        out = C.c_int64()
+        rc = self._L.ebiv_reader_seek(self._h, int(t), C.byref(out))
+        if rc == self._capi.EOF:
+            raise EOFError("reader closed or empty")
+        return out.value
+
+    def packets(self):
+        out = []
+        self.seek(0)
+        while True:
+            p = self.next_packet()
+            if p is None:
+                break
+            out.append(p.data)
+        return out
+
+    def close(self):
+        self._L.ebiv_reader_close(self._h)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._L.ebiv_reader_destroy(h)
+            self._h = None

@@ -29,6 +29,9 @@ enum {
     EBIV_ERR_UNSUPPORTED = 2,
     EBIV_ERR_NOKEYFRAME = 3,
     EBIV_ERR_DIMENSIONS = 4,
+    EBIV_ERR_BAD_MAGIC = 5, /* ErrBadMagic (container) */
+    EBIV_ERR_VERSION = 6,   /* ErrVersion (container) */
+    EBIV_EOF = 7,           /* io.EOF from the container reader */
     EBIV_ERR_ARGUMENT = 16, /* null handle, bad pointer or size */
     EBIV_ERR_DEVICE = 17,   /* HIP runtime failure (allocation, launch) */
 };
@@ -179,6 +182,48 @@ int ebiv_batch_get_schedule(const ebiv_batch *b, ebiv_schedule *out);
 /* HBM one session holds at this geometry: frame pool and payload buffers
  * (for admission against a device memory budget). */
 size_t ebiv_session_bytes(int width, int height, int ring_size, int window);
+
+/* ---- Container reader (host only; needs no GPU) ----
+ * ebiv.NewDemuxer / Reader (ebiv/reader.go:32-258) over an in-memory copy of
+ * an EBIV file: the 48-byte file header (ebiv/format.go:151-247), the footer
+ * index with its CRC-32, and the record scan that rebuilds a missing or
+ * damaged index.  Errors: EBIV_ERR_BAD_MAGIC, EBIV_ERR_VERSION,
+ * EBIV_ERR_CORRUPT, EBIV_ERR_UNSUPPORTED as Go's sentinels; EBIV_EOF as
+ * io.EOF.  Packet data stays valid until ebiv_reader_destroy. */
+typedef struct ebiv_reader ebiv_reader;
+
+typedef struct { /* FileHeader (ebiv/format.go:156-169) */
+    uint16_t version, tile_cols, tile_rows, reserved;
+    int32_t width, height;
+    uint32_t fps_num, fps_den;
+    uint8_t chroma_format, bit_depth, color_range, color_primaries;
+    uint32_t flags;
+} ebiv_file_header;
+
+typedef struct { /* IndexEntry (ebiv/format.go:281-285) */
+    uint64_t offset; /* of the frame record, prefix included */
+    uint32_t size;   /* payload bytes */
+    int32_t keyframe;
+} ebiv_index_entry;
+
+/* NewDemuxer (reader.go:32-61); msg (may be null) receives the error text. */
+int ebiv_reader_open(const uint8_t *data, size_t len, ebiv_reader **out, char *msg, size_t msg_cap);
+void ebiv_reader_destroy(ebiv_reader *r);
+int ebiv_reader_header(const ebiv_reader *r, ebiv_file_header *out);   /* FileHeader, reader.go:163 */
+int64_t ebiv_reader_frame_count(const ebiv_reader *r);                /* FrameCount, reader.go:166 */
+/* Index (reader.go:170): copies at most max entries, *n = the count. */
+int ebiv_reader_index(const ebiv_reader *r, ebiv_index_entry *out, size_t max, size_t *n);
+int ebiv_reader_index_scanned(const ebiv_reader *r); /* 1: index rebuilt by the record scan */
+int64_t ebiv_reader_duration(const ebiv_reader *r);  /* Duration, reader.go:183-184 (ns) */
+/* NextPacket (reader.go:189-217): EBIV_OK, or EBIV_EOF past the end or
+ * after close (sticky). */
+int ebiv_reader_next_packet(ebiv_reader *r, const uint8_t **data, size_t *len, int64_t *ts_ns, int32_t *keyframe);
+/* Seek (reader.go:222-238): to the last key frame at or before t_ns. */
+int ebiv_reader_seek(ebiv_reader *r, int64_t t_ns, int64_t *actual_ns);
+int ebiv_reader_close(ebiv_reader *r); /* Close (reader.go:242-245) */
+/* frameTime / frameIndexAt (ebiv/format.go:250-275), nanoseconds. */
+int64_t ebiv_frame_time(uint32_t fps_num, uint32_t fps_den, int64_t i);
+int64_t ebiv_frame_index_at(uint32_t fps_num, uint32_t fps_den, int64_t t_ns);
 
 /* Device memory helpers for callers that keep packets resident in HBM. */
 int ebiv_device_alloc(int device, size_t bytes, void **out);
