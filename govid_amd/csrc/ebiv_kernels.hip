@@ -3170,6 +3170,11 @@ __device__ void intra_mb(IntraScratch& S, const LdsI32* dct, const uint32_t* __r
 // carries key frames (with_front), the Frontier in front of them.
 constexpr int kIntraMaxWaves = 16;
 constexpr int kMaxStepWords = 16;  // steps tracked individually (512)
+// Inter frames: up to one wavefront's worth of intra macroblocks per tile
+// (in tiles of at most kFrontMax x kFrontMax macroblocks) are listed with
+// their dependency level.
+constexpr int kMaxIntraList = 64;
+constexpr int kIntraBitWords = kFrontMax * kFrontMax / 32 + 1;
 __device__ __forceinline__ void recon_intra_tile(const uint8_t* __restrict__ desc, const MBRec* __restrict__ recs,
                                                  const uint32_t* __restrict__ coefs,
                                                  const TileStatus* __restrict__ status, const Planes rec, const int t,
@@ -3177,6 +3182,9 @@ __device__ __forceinline__ void recon_intra_tile(const uint8_t* __restrict__ des
   extern __shared__ __attribute__((aligned(16))) uint8_t intra_dyn[];
   __shared__ uint32_t step_mask[kMaxStepWords];
   __shared__ int32_t s_dct[768];
+  __shared__ uint32_t s_bits[kIntraBitWords];  // intra macroblocks of the tile (local index)
+  __shared__ uint32_t s_list[kMaxIntraList];   // local index | level << 16, raster order
+  __shared__ uint32_t s_count[2];              // list length, highest level
   Frontier& front = *reinterpret_cast<Frontier*>(intra_dyn);
   IntraScratch* scratch = reinterpret_cast<IntraScratch*>(intra_dyn + (with_front ? sizeof(Frontier) : 0));
   LdsI32* dct = (LdsI32*)s_dct;
@@ -3191,9 +3199,95 @@ __device__ __forceinline__ void recon_intra_tile(const uint8_t* __restrict__ des
   const int mbc = hdr->mbc;
   // A key frame's tile (every macroblock intra) keeps its context in LDS.
   const bool keyf = with_front && hdr->inter == 0 && tw <= kFrontMax && th <= kFrontMax;
-  const bool tracked = !keyf && steps <= 32 * kMaxStepWords;
+  // Otherwise, a tile with few intra macroblocks runs them by dependency
+  // level: level 0 has no intra neighbour among left, above-left, above and
+  // above-right, level k + 1 has one of level k at most.  Macroblocks of one
+  // level neither read nor write each other's pixels, so a level is one
+  // parallel round, usually one or two rounds against one per anti-diagonal.
+  const bool levels = !keyf && tw <= kFrontMax && th <= kFrontMax && st.n_intra <= kMaxIntraList;
+  const bool tracked = !keyf && !levels && steps <= 32 * kMaxStepWords;
   if (threadIdx.x < kMaxStepWords) step_mask[threadIdx.x] = 0;
-  load_dct(dct);
+  if (levels)
+    for (int i = threadIdx.x; i < (tw * th + 31) / 32; i += blockDim.x) s_bits[i] = 0;
+  load_dct(dct);  // (its barrier also orders the clears above)
+  if (levels) {
+    for (uint32_t i = threadIdx.x; i < st.mbs_done; i += blockDim.x) {
+      const int ry = static_cast<int>(i) / tw, rx = static_cast<int>(i) - ry * tw;
+      if (recs[(td.y0 + ry) * mbc + td.x0 + rx].kind == 2) atomicOr(&s_bits[i >> 5], 1u << (i & 31));
+    }
+    __syncthreads();
+    if (wave == 0) {
+      // The list in raster order, lane k = the k-th intra macroblock.
+      uint32_t mine = 0xFFFFFFFFu, n = 0;
+      const int nwords = (tw * th + 31) / 32;
+      for (int w0 = 0; w0 < nwords; w0 += 64) {
+        const uint32_t word = w0 + lane < nwords ? s_bits[w0 + lane] : 0u;
+        const int cnt = __builtin_popcount(word);
+        int pre = cnt;  // inclusive prefix of the counts over the lanes
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+          const int o = __shfl_up(pre, d);
+          if (lane >= d) pre += o;
+        }
+        const int base = static_cast<int>(n) + pre - cnt;
+        // each lane's set bits go to lanes base .. base + cnt - 1
+        uint32_t w = word;
+        for (int j = 0; w; j++) {
+          const uint32_t b = static_cast<uint32_t>(__builtin_ctz(w));
+          w &= w - 1;
+          const int dst = base + j;
+          if (dst < kMaxIntraList) s_list[dst] = static_cast<uint32_t>((w0 + lane) * 32) + b;
+        }
+        n += static_cast<uint32_t>(__shfl(pre, 63));
+      }
+      wave_sync();
+      if (n > static_cast<uint32_t>(kMaxIntraList)) n = 0xFFFFFFFFu;  // more than the tile status said: steps
+      if (static_cast<uint32_t>(lane) < n) mine = s_list[lane];
+      // Levels in raster order: a macroblock's intra neighbours come earlier.
+      uint32_t lvl = 0, maxl = 0;
+      for (uint32_t k = 0; n != 0xFFFFFFFFu && k < n; k++) {
+        const int loc = static_cast<int>(__builtin_amdgcn_readlane(mine, static_cast<int>(k)));
+        const int ry = loc / tw, rx = loc - ry * tw;
+        uint32_t l = 0;
+        auto dep = [&](bool present, int nloc) {  // an intra neighbour's level + 1
+          if (!present) return;
+          const uint64_t hit = __ballot(mine == static_cast<uint32_t>(nloc));
+          if (hit) {
+            const uint32_t v = __builtin_amdgcn_readlane(lvl, static_cast<int>(__builtin_ctzll(hit))) + 1u;
+            l = v > l ? v : l;
+          }
+        };
+        dep(rx > 0, loc - 1);                       // left
+        dep(rx > 0 && ry > 0, loc - tw - 1);        // above-left
+        dep(ry > 0, loc - tw);                      // above
+        dep(ry > 0 && rx + 1 < tw, loc - tw + 1);   // above-right
+        if (static_cast<uint32_t>(lane) == k) lvl = l;
+        maxl = l > maxl ? l : maxl;
+      }
+      if (n != 0xFFFFFFFFu && static_cast<uint32_t>(lane) < n) s_list[lane] = mine | (lvl << 16);
+      if (lane == 0) {
+        s_count[0] = n;
+        s_count[1] = maxl;
+      }
+    }
+    __syncthreads();
+    const uint32_t n = s_count[0], maxl = s_count[1];
+    for (uint32_t lv = 0; n != 0xFFFFFFFFu && lv <= maxl; lv++) {
+      for (uint32_t k = wave; k < n; k += nwaves) {
+        const uint32_t e = s_list[k];
+        if ((e >> 16) != lv) continue;
+        const int local = static_cast<int>(e & 0xFFFFu);
+        const int ry = local / tw, rx = local - ry * tw;
+        const int mbx = td.x0 + rx, mby = td.y0 + ry;
+        intra_mb(scratch[wave], dct, coefs, recs[mby * mbc + mbx], hdr, td, rec, mbx, mby, nullptr, lane);
+      }
+      // the next level reads this one's pixels from the frame (see below)
+      __syncthreads();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    if (n != 0xFFFFFFFFu) return;
+    // (otherwise every anti-diagonal below, untracked)
+  }
   if (tracked) {
     for (uint32_t i = threadIdx.x; i < st.mbs_done; i += blockDim.x) {
       const int ry = static_cast<int>(i) / tw, rx = static_cast<int>(i) - ry * tw;
