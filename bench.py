@@ -150,8 +150,7 @@ def run_batched(packets, dev, want, sessions, window, steps, warmup):
 
     def one_pass(check):
         for sid, order in zip(sids, orders):
-            for i in order:
-                b.submit(sid, packets[i], dev[i].ptr, tag=i)
+            b.submit_many(sid, [packets[i] for i in order], [dev[i].ptr for i in order], tags=order)
         lat, bad, n = [], 0, 0
         while b.pending():
             a = time.perf_counter()

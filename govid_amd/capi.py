@@ -23,7 +23,7 @@ EXPORTS = (
     "ebiv_batch_create", "ebiv_batch_destroy", "ebiv_batch_last_error", "ebiv_batch_open", "ebiv_batch_close",
     "ebiv_batch_submit", "ebiv_batch_flush", "ebiv_batch_pending", "ebiv_batch_step", "ebiv_batch_stats",
     "ebiv_batch_set_timing", "ebiv_batch_kernel_times", "ebiv_batch_frame_rgba", "ebiv_session_bytes",
-    "ebiv_batch_set_schedule", "ebiv_batch_get_schedule",
+    "ebiv_batch_set_schedule", "ebiv_batch_get_schedule", "ebiv_batch_submit_many",
     "ebiv_reader_open", "ebiv_reader_destroy", "ebiv_reader_header", "ebiv_reader_frame_count", "ebiv_reader_index",
     "ebiv_reader_index_scanned", "ebiv_reader_duration", "ebiv_reader_next_packet", "ebiv_reader_seek",
     "ebiv_reader_close", "ebiv_frame_time", "ebiv_frame_index_at",
@@ -136,6 +136,9 @@ def bind(L):
     L.ebiv_batch_frame_rgba.argtypes = [vp, C.POINTER(Frame), vp, C.c_int]
     L.ebiv_session_bytes.restype = sz
     L.ebiv_session_bytes.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
+    L.ebiv_batch_submit_many.restype = C.c_int
+    L.ebiv_batch_submit_many.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                         C.POINTER(sz), C.POINTER(C.c_int64), C.POINTER(C.c_uint64)]
     L.ebiv_reader_open.restype = C.c_int
     L.ebiv_reader_open.argtypes = [C.c_char_p, sz, C.POINTER(vp), C.c_char_p, sz]
     L.ebiv_reader_destroy.restype = None

@@ -12,6 +12,7 @@ Decoding always runs on the GPU through libebivgpu.so; there is no other path.
 from __future__ import annotations
 
 import ctypes as C
+import struct
 from dataclasses import dataclass
 
 import numpy as np
@@ -232,14 +233,29 @@ def device_count() -> int:
     return int(native.gpu_lib().ebiv_device_count())
 
 
-@dataclass
 class Completion:
-    """One decoded (or failed) packet of a batch step."""
-    session: int
-    tag: int
-    status: int
-    frame: Frame | None
-    error: EbivError | None
+    """One decoded (or failed) packet of a batch step.  ``frame`` (None on
+    failure) and ``error`` are built on first access."""
+
+    __slots__ = ("session", "tag", "status", "_batch", "_raw", "_msg", "_frame")
+
+    def __init__(self, session: int, tag: int, status: int, batch, raw: bytes | None, msg: bytes = b""):
+        self.session, self.tag, self.status = session, tag, status
+        self._batch, self._raw, self._msg, self._frame = batch, raw, msg, None
+
+    @property
+    def frame(self) -> Frame | None:
+        if self.status or self._raw is None:
+            return None
+        if self._frame is None:
+            self._frame = Frame(self._batch, capi.Frame.from_buffer_copy(self._raw))
+        return self._frame
+
+    @property
+    def error(self) -> EbivError | None:
+        if not self.status:
+            return None
+        return _ERRORS.get(self.status, DeviceError)(self._msg.split(b"\0", 1)[0].decode(errors="replace"))
 
 
 class Batch:
@@ -290,6 +306,19 @@ class Batch:
         if rc:
             raise DeviceError(f"ebiv_batch_submit failed ({rc})")
 
+    def submit_many(self, session: int, packets, dev_ptrs=None, tags=None, timestamps=None) -> None:
+        """submit() of a list of packets (bytes) in one C call."""
+        n = len(packets)
+        data = [bytes(p) for p in packets]
+        hp = (C.c_void_p * n)(*[C.cast(C.c_char_p(d), C.c_void_p).value for d in data])
+        dp = (C.c_void_p * n)(*dev_ptrs) if dev_ptrs is not None else None
+        ln = (C.c_size_t * n)(*[len(d) for d in data])
+        ts = (C.c_int64 * n)(*timestamps) if timestamps is not None else None
+        tg = (C.c_uint64 * n)(*tags) if tags is not None else None
+        rc = self._L.ebiv_batch_submit_many(self._h, session, n, hp, dp, ln, ts, tg)
+        if rc:
+            raise DeviceError(f"ebiv_batch_submit_many failed ({rc})")
+
     def flush(self, session: int) -> None:
         """Codec.Flush for one session; its queued packets are dropped."""
         if self._L.ebiv_batch_flush(self._h, session):
@@ -305,16 +334,20 @@ class Batch:
         rc = self._L.ebiv_batch_step(self._h, self._out, max_out, C.byref(n))
         if rc:
             raise DeviceError(f"{self._L.ebiv_batch_last_error(self._h).decode()} ({rc})")
+        # one copy of the completions, then plain slices (the output array is
+        # reused by the next step)
+        size, fo, fs = C.sizeof(capi.Completion), capi.Completion.frame.offset, C.sizeof(capi.Frame)
+        mo, ms = capi.Completion.message.offset, capi.Completion.message.size
+        raw = C.string_at(C.addressof(self._out), size * n.value)
+        head = struct.Struct("<iiQ")
         res = []
         for i in range(n.value):
-            c = self._out[i]
-            if c.status == capi.OK:
-                fr = capi.Frame()
-                C.pointer(fr)[0] = c.frame
-                res.append(Completion(c.session, c.tag, 0, Frame(self, fr), None))
+            o = i * size
+            session, status, tag = head.unpack_from(raw, o)
+            if status == capi.OK:
+                res.append(Completion(session, tag, 0, self, raw[o + fo:o + fo + fs]))
             else:
-                err = _ERRORS.get(c.status, DeviceError)(c.message.decode(errors="replace"))
-                res.append(Completion(c.session, c.tag, c.status, None, err))
+                res.append(Completion(session, tag, status, self, None, raw[o + mo:o + mo + ms]))
         return res
 
     def stats(self, session: int = 0) -> dict:

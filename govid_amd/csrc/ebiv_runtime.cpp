@@ -1235,6 +1235,20 @@ int ebiv_batch_submit(ebiv_batch* b, int session, const uint8_t* host_data, cons
   return EBIV_OK;
 }
 
+int ebiv_batch_submit_many(ebiv_batch* b, int session, int n, const uint8_t* const* host_data,
+                           const uint8_t* const* dev_data, const size_t* len, const int64_t* ts, const uint64_t* tags) {
+  if (!b || n < 0 || (n > 0 && (!host_data || !len))) return EBIV_ERR_ARGUMENT;
+  if (!b->eng.find(session)) return EBIV_ERR_ARGUMENT;
+  for (int i = 0; i < n; i++)
+    if (!host_data[i] && len[i]) return EBIV_ERR_ARGUMENT;  // (checked before anything is queued)
+  for (int i = 0; i < n; i++) {
+    const int rc = ebiv_batch_submit(b, session, host_data[i], dev_data ? dev_data[i] : nullptr, len[i],
+                                     ts ? ts[i] : 0, tags ? tags[i] : 0);
+    if (rc) return rc;
+  }
+  return EBIV_OK;
+}
+
 int ebiv_batch_flush(ebiv_batch* b, int session) {
   Session* s = b ? b->eng.find(session) : nullptr;
   if (!s) return EBIV_ERR_ARGUMENT;

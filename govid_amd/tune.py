@@ -57,8 +57,7 @@ def measure(packets, dev, schedule: dict, sessions: int, window: int, passes: in
 
         def one_pass(record):
             for sid, order in zip(sids, orders):
-                for i in order:
-                    b.submit(sid, packets[i], dev[i].ptr, tag=i)
+                b.submit_many(sid, [packets[i] for i in order], [dev[i].ptr for i in order], tags=order)
             n = 0
             while b.pending():
                 for c in b.step():

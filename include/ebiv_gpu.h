@@ -149,6 +149,13 @@ int ebiv_batch_close(ebiv_batch *b, int session);
  * packet completes; when null the engine stages the packet itself. */
 int ebiv_batch_submit(ebiv_batch *b, int session, const uint8_t *host_data, const uint8_t *dev_data, size_t len,
                       int64_t timestamp_ns, uint64_t tag);
+/* n packets of one session in one call, in order (as n ebiv_batch_submit
+ * calls; no counterpart in the reference).  dev_data, ts and tags may be
+ * null (no HBM copies, timestamps 0, tags 0); nothing is queued when an
+ * argument is invalid. */
+int ebiv_batch_submit_many(ebiv_batch *b, int session, int n, const uint8_t *const *host_data,
+                           const uint8_t *const *dev_data, const size_t *len, const int64_t *ts,
+                           const uint64_t *tags);
 
 /* Codec.Flush for a session (codec.go:115-117), dropping its queued packets:
  * the seek path (source.go:45-50 and 226-234 seek the demuxer, then flush). */

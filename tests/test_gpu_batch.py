@@ -30,7 +30,7 @@ def _oracle_outcomes(packets, ring, flush_at=()):
     return out
 
 
-def _run_batch(streams, max_out=4096, check_rgba=False, schedule=None):
+def _run_batch(streams, max_out=4096, check_rgba=False, schedule=None, bulk=False):
     """streams: list of (packets, ring, window).  Submits everything, steps
     until drained, and checks each completion against the oracle."""
     from govid_amd.codec import Batch
@@ -44,8 +44,11 @@ def _run_batch(streams, max_out=4096, check_rgba=False, schedule=None):
         sid = b.open(ring=ring, window=window)
         sids.append(sid)
         want.append(_oracle_outcomes(packets, max(ring, window)))
-        for i, p in enumerate(packets):
-            b.submit(sid, p, tag=(k << 32) | i)
+        if bulk:  # one call per stream (ebiv_batch_submit_many)
+            b.submit_many(sid, packets, tags=[(k << 32) | i for i in range(len(packets))])
+        else:
+            for i, p in enumerate(packets):
+                b.submit(sid, p, tag=(k << 32) | i)
     seen = [0] * len(streams)
     steps = 0
     while b.pending():
@@ -110,6 +113,15 @@ def test_tuner_checks_every_schedule_and_picks_one():
     assert len(res["tried"]) >= 2
     assert res["best"] in [t["schedule"] for t in res["tried"]]
     assert all(t["frames"] == 3 * len(pk) for t in res["tried"])
+
+
+def test_bulk_submission_matches_oracle():
+    streams = [
+        (_clip("rects", 96, 64, 10, qp=22, gop=5, tile_cols=2, tile_rows=2), 3, 3),
+        (_clip("parallax", 64, 48, 9, qp=28, gop=9, altref=4), 4, 2),
+    ]
+    print("bulk submit", flush=True)
+    _run_batch(streams, bulk=True)
 
 
 def test_window_larger_than_stream_and_max_out_limit():
