@@ -2900,13 +2900,28 @@ __device__ __forceinline__ void recon_inter_mb(const uint8_t* __restrict__ desc,
   }
 }
 
+// items[0, njobs): {job, first macroblock group of the job in this launch},
+// ascending; the launch has one workgroup per group of kInterGroupMbs
+// macroblocks.
 __global__ __launch_bounds__(256, 8) void ebiv_recon_inter(const FrameJob* __restrict__ jobs,
-                                                           const WorkItem* __restrict__ items) {
+                                                           const WorkItem* __restrict__ items, int njobs) {
   __shared__ InterScratch scratch[4];
-  const ConstItem& it = ((ConstItem*)items)[xcd_contiguous(blockIdx.x, gridDim.x)];
-  ConstJob& job = job_of(jobs, it.job);
+  const uint32_t g = xcd_contiguous(blockIdx.x, gridDim.x);
   const int lane = lane_id(), wave = static_cast<int>(sgpr(threadIdx.x >> 6));
-  recon_inter_mb(job.desc, job.recs, job.coefs, job, static_cast<int>(it.idx) + wave, scratch[wave], lane);
+  // this group's job: the last item whose first group is <= g (one vector
+  // load and a ballot per 64 jobs)
+  uint32_t k = 0;
+  for (int base = 0; base < njobs; base += 64) {
+    const int e = base + lane;
+    const uint32_t first = e < njobs ? items[e].idx : 0xFFFFFFFFu;
+    const uint32_t c = static_cast<uint32_t>(__popcll(__ballot(first <= g)));
+    k += c;
+    if (c < 64) break;
+  }
+  const ConstItem& it = ((ConstItem*)items)[sgpr(k - 1)];
+  ConstJob& job = job_of(jobs, it.job);
+  const int first_mb = static_cast<int>((g - it.idx) * kInterGroupMbs);
+  recon_inter_mb(job.desc, job.recs, job.coefs, job, first_mb + wave, scratch[wave], lane);
 }
 
 // ============================================================================
@@ -3390,8 +3405,8 @@ void launch_parse(hipStream_t s, int nitems, int max_tile_w, const FrameJob* job
   hipLaunchKernelGGL(ebiv_parse, dim3(nitems), dim3(64), lds, s, jobs, items);
 }
 
-void launch_recon_inter(hipStream_t s, int nitems, const FrameJob* jobs, const WorkItem* items) {
-  hipLaunchKernelGGL(ebiv_recon_inter, dim3(nitems), dim3(256), 0, s, jobs, items);
+void launch_recon_inter(hipStream_t s, int ngroups, const FrameJob* jobs, const WorkItem* items, int njobs) {
+  hipLaunchKernelGGL(ebiv_recon_inter, dim3(ngroups), dim3(256), 0, s, jobs, items, njobs);
 }
 
 // waves: wavefronts per tile (1-16); key: the items include key-frame tiles,

@@ -45,7 +45,7 @@ namespace ebiv {
 
 hipError_t upload_tables(const int32_t dct[3][256], const uint8_t zz[3][256]);
 void launch_parse(hipStream_t s, int nitems, int max_tile_w, const FrameJob* jobs, const WorkItem* items);
-void launch_recon_inter(hipStream_t s, int nitems, const FrameJob* jobs, const WorkItem* items);
+void launch_recon_inter(hipStream_t s, int ngroups, const FrameJob* jobs, const WorkItem* items, int njobs);
 void launch_recon_intra(hipStream_t s, int nitems, const FrameJob* jobs, const WorkItem* items, int waves, bool key);
 void launch_rgba(hipStream_t s, const uint8_t* y, int ys, const uint8_t* cb, const uint8_t* cr, int cs, int w, int h,
                  uint32_t* out);
@@ -770,14 +770,20 @@ int Engine::run_step(std::vector<Session*>& ss, std::vector<std::vector<StepFram
   };
   // Items: the entropy launch's tiles, then per round its inter macroblock
   // groups and its intra tiles.
+  // Inter launches take one item per job: {job, its first macroblock group
+  // in the launch}; the kernel finds a workgroup's job among them.
+  auto inter_groups = [&](size_t j) -> uint32_t {
+    const Session& s = *ss[jrefs[j].si];
+    return (static_cast<uint32_t>(s.mbc * s.mbr) + kInterGroupMbs - 1) / kInterGroupMbs;
+  };
+  uint32_t inter_prefix = 0;  // groups so far in the current inter launch
   auto add_recon_items = [&](size_t j, bool inter_part, size_t& w) {
     const JobRef& jr = jrefs[j];
-    const Session& s = *ss[jr.si];
     const PayloadSlot* ps = payload_of(ss, frames, jr);
     if (inter_part) {
       if (!ps->inter) return;
-      const uint32_t nmb = static_cast<uint32_t>(s.mbc * s.mbr);
-      for (uint32_t m = 0; m < nmb; m += kInterGroupMbs) h_items.h[w++] = WorkItem{static_cast<uint32_t>(j), m};
+      h_items.h[w++] = WorkItem{static_cast<uint32_t>(j), inter_prefix};
+      inter_prefix += inter_groups(j);
     } else {
       for (int t = 0; t < ps->ntiles; t++) h_items.h[w++] = WorkItem{static_cast<uint32_t>(j), static_cast<uint32_t>(t)};
     }
@@ -789,7 +795,8 @@ int Engine::run_step(std::vector<Session*>& ss, std::vector<std::vector<StepFram
     const Session& s = *ss[jr.si];
     ps->status_off = nparse;
     nparse += static_cast<size_t>(ps->ntiles);
-    nitems += 2 * static_cast<size_t>(ps->ntiles) + (ps->inter ? (static_cast<size_t>(s.mbc) * s.mbr + kInterGroupMbs - 1) / kInterGroupMbs : 0);
+    (void)s;
+    nitems += 2 * static_cast<size_t>(ps->ntiles) + (ps->inter ? 1 : 0);
     max_tw = std::max(max_tw, ps->max_tile_w);
   }
   const size_t njobs = jrefs.size();
@@ -805,10 +812,13 @@ int Engine::run_step(std::vector<Session*>& ss, std::vector<std::vector<StepFram
   // tile, for throughput).
   std::vector<size_t> inter_off(rounds.size()), intra_off(rounds.size()), key_end(rounds.size()),
       round_end(rounds.size());
+  std::vector<uint32_t> inter_groups_of(rounds.size());
   auto is_key = [&](size_t j) { return !payload_of(ss, frames, jrefs[j])->inter; };
   for (size_t r = 0; r < rounds.size(); r++) {
     inter_off[r] = w;
+    inter_prefix = 0;
     for (size_t j : rounds[r]) add_recon_items(j, true, w);
+    inter_groups_of[r] = inter_prefix;
     intra_off[r] = w;
     for (size_t j : rounds[r]) if (is_key(j)) add_recon_items(j, false, w);
     key_end[r] = w;
@@ -856,7 +866,8 @@ int Engine::run_step(std::vector<Session*>& ss, std::vector<std::vector<StepFram
       for (size_t j : rounds[r])
         if (payload_of(ss, frames, jrefs[j])->inter) mbs += static_cast<uint64_t>(ss[jrefs[j].si]->mbc) * ss[jrefs[j].si]->mbr;
       const int m = mark_begin(1, mbs);
-      launch_recon_inter(stream, static_cast<int>(intra_off[r] - inter_off[r]), d_jobs.d, d_items.d + inter_off[r]);
+      launch_recon_inter(stream, static_cast<int>(inter_groups_of[r]), d_jobs.d, d_items.d + inter_off[r],
+                         static_cast<int>(intra_off[r] - inter_off[r]));
       mark_end(m);
     }
     if (key_end[r] > intra_off[r]) {
@@ -923,15 +934,15 @@ int Engine::run_step(std::vector<Session*>& ss, std::vector<std::vector<StepFram
         const Session& s = *ss[i];
         fill_job(h_jobs.h[0], jr);
         size_t n = 0;
-        if (ps->inter)
-          for (uint32_t m = 0; m < static_cast<uint32_t>(s.mbc * s.mbr); m += kInterGroupMbs) h_items.h[n++] = WorkItem{0, m};
+        if (ps->inter) h_items.h[n++] = WorkItem{0, 0};
         const size_t ninter = n;
+        const uint32_t groups = (static_cast<uint32_t>(s.mbc * s.mbr) + kInterGroupMbs - 1) / kInterGroupMbs;
         for (int t = 0; t < ps->ntiles; t++) h_items.h[n++] = WorkItem{0, static_cast<uint32_t>(t)};
         if (ov && !golden_copy(jr)) return fail(EBIV_ERR_DEVICE, "golden copy failed");
         if (hipMemcpyAsync(d_jobs.d, h_jobs.h, sizeof(FrameJob), hipMemcpyHostToDevice, stream) != hipSuccess ||
             hipMemcpyAsync(d_items.d, h_items.h, sizeof(WorkItem) * n, hipMemcpyHostToDevice, stream) != hipSuccess)
           return fail(EBIV_ERR_DEVICE, "job table upload failed");
-        if (ninter) launch_recon_inter(stream, static_cast<int>(ninter), d_jobs.d, d_items.d);
+        if (ninter) launch_recon_inter(stream, static_cast<int>(groups), d_jobs.d, d_items.d, 1);
         launch_recon_intra(stream, ps->ntiles, d_jobs.d, d_items.d + ninter,
                            ps->inter ? sched.inter_intra_waves(static_cast<size_t>(ps->ntiles)) : sched.intra_waves_key,
                            !ps->inter);
