@@ -33,7 +33,11 @@
 #include <cstring>
 #include <deque>
 #include <memory>
+#include <atomic>
+#include <condition_variable>
+#include <functional>
 #include <mutex>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -596,6 +600,77 @@ struct Schedule {
   }
 };
 
+// Host worker threads for per-session work of a step (header, table and
+// tile-framing parse of every queued packet).  parallel_for(n, fn) runs
+// fn(0..n-1) on the workers and the calling thread and returns when all
+// are done; each worker makes the engine's device current first.
+class HostPool {
+ public:
+  explicit HostPool(int device) : device_(device) {
+    const unsigned hc = std::thread::hardware_concurrency();
+    const int n = static_cast<int>(hc > 1 ? (hc < 8 ? hc : 8) : 1) - 1;
+    for (int i = 0; i < n; i++) workers_.emplace_back([this] { run(); });
+  }
+  ~HostPool() {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      quit_ = true;
+    }
+    cv_.notify_all();
+    for (auto& t : workers_) t.join();
+  }
+  void parallel_for(int n, const std::function<void(int)>& fn) {
+    if (n <= 1 || workers_.empty()) {
+      for (int i = 0; i < n; i++) fn(i);
+      return;
+    }
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      fn_ = &fn;
+      n_ = n;
+      next_.store(0);
+      active_ = static_cast<int>(workers_.size());
+      gen_++;
+    }
+    cv_.notify_all();
+    drain();
+    std::unique_lock<std::mutex> l(mu_);
+    done_cv_.wait(l, [this] { return active_ == 0; });
+    fn_ = nullptr;
+  }
+
+ private:
+  void drain() {
+    for (int i = next_.fetch_add(1); i < n_; i = next_.fetch_add(1)) (*fn_)(i);
+  }
+  void run() {
+    (void)hipSetDevice(device_);
+    uint64_t seen = 0;
+    for (;;) {
+      {
+        std::unique_lock<std::mutex> l(mu_);
+        cv_.wait(l, [&] { return quit_ || gen_ != seen; });
+        if (quit_) return;
+        seen = gen_;
+      }
+      drain();
+      {
+        std::lock_guard<std::mutex> g(mu_);
+        if (--active_ == 0) done_cv_.notify_one();
+      }
+    }
+  }
+  int device_;
+  std::vector<std::thread> workers_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_cv_;
+  const std::function<void(int)>* fn_ = nullptr;
+  int n_ = 0, active_ = 0;
+  std::atomic<int> next_{0};
+  uint64_t gen_ = 0;
+  bool quit_ = false;
+};
+
 struct Engine {
   int device = 0;
   Schedule sched;
@@ -619,7 +694,10 @@ struct Engine {
   std::vector<uint64_t> mark_mbs;
   ebiv_kernel_times ktimes{};
 
+  std::unique_ptr<HostPool> pool;  // created on the first batch step
+
   ~Engine() {
+    pool.reset();
     if (stream) (void)hipStreamSynchronize(stream);
     sessions.clear();
     for (auto e : evs) (void)hipEventDestroy(e);
@@ -1285,6 +1363,7 @@ int ebiv_batch_step(ebiv_batch* b, ebiv_completion* out, int max_out, int* n_out
   std::vector<std::vector<StepFrame>> frames;
   std::vector<std::vector<Session::Queued>> held;
   int budget = max_out;
+  // Take up to a window of packets from every session ...
   for (auto& sp : e.sessions) {
     Session& s = *sp;
     if (s.queue.empty() || budget <= 0) continue;
@@ -1294,23 +1373,37 @@ int ebiv_batch_step(ebiv_batch* b, ebiv_completion* out, int max_out, int* n_out
     const int window = std::min(s.window, budget);
     frames.back().reserve(static_cast<size_t>(window));
     held.back().reserve(static_cast<size_t>(window));
-    int next_slot = 0;
-    while (!s.queue.empty() && static_cast<int>(frames.back().size()) < window) {
+    while (!s.queue.empty() && static_cast<int>(held.back().size()) < window) {
       held.back().push_back(std::move(s.queue.front()));
       s.queue.pop_front();
-      const Session::Queued& q = held.back().back();
-      frames.back().emplace_back();
-      StepFrame& f = frames.back().back();
+      budget--;
+    }
+  }
+  if (ss.empty()) return EBIV_OK;
+  // ... and parse them, sessions in parallel, each in stream order.  A key
+  // frame with a golden override ends its session's window: the packets
+  // after it go back to the front of the queue.
+  if (!e.pool) e.pool.reset(new HostPool(e.device));
+  e.pool->parallel_for(static_cast<int>(ss.size()), [&](int i) {
+    Session& s = *ss[static_cast<size_t>(i)];
+    std::vector<StepFrame>& fr = frames[static_cast<size_t>(i)];
+    std::vector<Session::Queued>& h = held[static_cast<size_t>(i)];
+    int next_slot = 0;
+    size_t k = 0;
+    while (k < h.size()) {
+      const Session::Queued& q = h[k++];
+      fr.emplace_back();
+      StepFrame& f = fr.back();
       f.host = q.host.data();
       f.dev = q.dev;
       f.len = q.host.size();
       f.ts = q.ts;
       f.tag = q.tag;
-      budget--;
       if (!s.prepare_frame(f, &next_slot, e.stream)) break;
     }
-  }
-  if (ss.empty()) return EBIV_OK;
+    for (size_t j = h.size(); j > k; j--) s.queue.push_front(std::move(h[j - 1]));
+    h.resize(k);
+  });
   if (int rc = e.run_step(ss, frames)) return rc;
   int n = 0;
   for (size_t i = 0; i < ss.size(); i++)
