@@ -97,6 +97,11 @@ static_assert(sizeof(MBRec) == 48, "MBRec layout");
 // block id (0-15 luma in decode order, 16 Cb, 17 Cr), bits 13-31 the signed
 // level (|level| <= 32772 always fits 19 bits).
 constexpr int kCoefPerMB = 384;
+// Residual of an intra macroblock, formed by the inter-reconstruction launch
+// for the intra launch: luma 256 then Cb 64, Cr 64, each value clamped to
+// int16 (the reconstruction clamps pred + residual to [0, 255] with pred in
+// [0, 255], so clamping the residual to [-32768, 32767] changes no sample).
+constexpr int kResPerMB = 384;
 
 // Per-tile outcome of the entropy kernel.
 struct TileStatus {
@@ -124,6 +129,7 @@ struct FrameJob {
   MBRec* recs;
   uint32_t* coefs;
   TileStatus* status;
+  int16_t* res;      // intra macroblocks' residuals, 384 per macroblock (see kResPerMB)
   int32_t mbc, mbr;  // the frame's macroblock grid
   Planes rec, ref, gold;
 };

@@ -2514,15 +2514,11 @@ __device__ uint64_t load_coefs(WaveScratch& ws, const uint32_t* __restrict__ coe
   return static_cast<uint64_t>(nzm) | (static_cast<uint64_t>(acm) << 32);
 }
 
-// The DCT matrices of buildDCTMatrix (transform.go:15-40) for n = 4, 8, 16 at
-// offsets 0, 256 and 512, copied into LDS by each workgroup (a lane-indexed
-// read of constant memory is a global load per element).
+// The DCT matrices of buildDCTMatrix (transform.go:15-40) for n = 4, 8, 16
+// (c_dct rows 0, 1, 2) are staged into LDS by the wavefront that inverts a
+// residual (a lane-indexed read of constant memory is a global load per
+// element).
 using LdsI32 = __attribute__((address_space(3))) int32_t;
-__device__ __forceinline__ void load_dct(LdsI32* dct) {
-  const int32_t* src = &c_dct[0][0];
-  for (int i = threadIdx.x; i < 768; i += blockDim.x) dct[i] = src[i];
-  __syncthreads();
-}
 
 // Separable inverse DCT (transform.go:129-194) of nblk n x n blocks at base:
 // column pass into tmp, row pass back into coef, each with the
@@ -2597,21 +2593,9 @@ __device__ __forceinline__ int32_t wave_max_abs(const WaveScratch& ws, int lane)
 }
 
 // Dequantisation and inverse transform of a macroblock's residual into
-// ws.coef (dense, zero where nothing is coded).
-// dct: the three matrices as laid out by load_dct.
-__device__ __forceinline__ void residual_mb(WaveScratch& ws, const LdsI32* dct, const uint32_t* __restrict__ coefs,
-                                            const MBRec& r, int n, int qdc, int qac, int lane) {
-  const uint64_t masks = load_coefs(ws, coefs, r, n, qdc, qac, lane);
-  const uint32_t nzm = static_cast<uint32_t>(masks), acm = static_cast<uint32_t>(masks >> 32);
-  if (!nzm) return;
-  const LdsI32* cl = dct + (n == 4 ? 0 : (n == 8 ? 256 : 512));
-  if (wave_max_abs(ws, lane) <= kIdct32Max) idct_mb<int32_t>(ws, cl, dct + 256, n, nzm, acm, lane);
-  else idct_mb<int64_t>(ws, cl, dct + 256, n, nzm, acm, lane);
-}
-
-// The same for one wavefront that stages only the matrices it needs in its
-// own LDS (mats: the n x n matrix at 0, the 8 x 8 one at 256): no
-// workgroup barrier.
+// ws.coef (dense, zero where nothing is coded).  The wavefront stages only
+// the matrices it needs in its own LDS (mats: the n x n matrix at 0, the
+// 8 x 8 one at 256): no workgroup barrier.
 __device__ __forceinline__ void residual_mb_wave(WaveScratch& ws, LdsI32* mats, const uint32_t* __restrict__ coefs,
                                                  const MBRec& r, int n, int qdc, int qac, int lane) {
   const uint64_t masks = load_coefs(ws, coefs, r, n, qdc, qac, lane);
@@ -2868,7 +2852,22 @@ __device__ __forceinline__ void recon_inter_mb(const uint8_t* __restrict__ desc,
   if (mb >= nmb) return;
   // Macroblocks a failed tile stream never reached carry kind kKindUnreached.
   const MBRec r = recs[mb];
-  if (r.kind >= 2) return;
+  if (r.kind == 2) {  // intra: only its residual, for the intra launch
+    if (r.coef_count > 0) {
+      const FrameHdr* hdr = reinterpret_cast<const FrameHdr*>(desc);
+      residual_mb_wave(S.ws, (LdsI32*)S.mats, coefs, r, 4 << (r.tx > 2 ? 2 : r.tx), hdr->qdc, hdr->qac, lane);
+      wave_sync();
+      uint32_t* dst = reinterpret_cast<uint32_t*>(job.res + static_cast<size_t>(mb) * kResPerMB);
+#pragma unroll
+      for (int k = 0; k < kResPerMB / 128; k++) {
+        const int i = 2 * (lane + 64 * k);
+        const int32_t a = max(-32768, min(32767, S.ws.coef[i])), b = max(-32768, min(32767, S.ws.coef[i + 1]));
+        dst[lane + 64 * k] = (static_cast<uint32_t>(a) & 0xFFFFu) | (static_cast<uint32_t>(b) << 16);
+      }
+    }
+    return;
+  }
+  if (r.kind > 2) return;
   const bool has_res = r.kind == 1 && r.coef_count > 0;
   const int mby = static_cast<int>(sgpr(static_cast<uint32_t>(mb / mbc))), mbx = mb - mby * mbc;
   // (the reference is chosen by address in constant memory: selecting
@@ -3037,13 +3036,20 @@ struct Frontier {
   uint8_t corner[2][kFrontMax + 1][4];  // [row & 1][column]: Y, Cb, Cr
 };
 
-__device__ void intra_mb(IntraScratch& S, const LdsI32* dct, const uint32_t* __restrict__ coefs, const MBRec& r,
-                         const FrameHdr* hdr, const TileDesc& td, Planes rec, int mbx, int mby, Frontier* fr,
-                         int lane) {
+// res: the macroblock's residual as the inter launch formed it.
+__device__ void intra_mb(IntraScratch& S, const int16_t* __restrict__ res, const MBRec& r, const TileDesc& td,
+                         Planes rec, int mbx, int mby, Frontier* fr, int lane) {
   const int n = 4 << (r.tx > 2 ? 2 : r.tx);
   WaveScratch& ws = S.ws;
   if (r.coef_count > 0) {
-    residual_mb(ws, dct, coefs, r, n, hdr->qdc, hdr->qac, lane);
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(res);
+#pragma unroll
+    for (int k = 0; k < kResPerMB / 128; k++) {
+      const uint32_t w = src[lane + 64 * k];
+      const int i = 2 * (lane + 64 * k);
+      ws.coef[i] = static_cast<int16_t>(w & 0xFFFFu);
+      ws.coef[i + 1] = static_cast<int16_t>(w >> 16);
+    }
   } else {
     for (int i = lane; i < 384; i += 64) ws.coef[i] = 0;
   }
@@ -3191,18 +3197,16 @@ constexpr int kMaxStepWords = 16;  // steps tracked individually (512)
 constexpr int kMaxIntraList = 64;
 constexpr int kIntraBitWords = kFrontMax * kFrontMax / 32 + 1;
 __device__ __forceinline__ void recon_intra_tile(const uint8_t* __restrict__ desc, const MBRec* __restrict__ recs,
-                                                 const uint32_t* __restrict__ coefs,
+                                                 const int16_t* __restrict__ res,
                                                  const TileStatus* __restrict__ status, const Planes rec, const int t,
                                                  const bool with_front) {
   extern __shared__ __attribute__((aligned(16))) uint8_t intra_dyn[];
   __shared__ uint32_t step_mask[kMaxStepWords];
-  __shared__ int32_t s_dct[768];
   __shared__ uint32_t s_bits[kIntraBitWords];  // intra macroblocks of the tile (local index)
   __shared__ uint32_t s_list[kMaxIntraList];   // local index | level << 16, raster order
   __shared__ uint32_t s_count[2];              // list length, highest level
   Frontier& front = *reinterpret_cast<Frontier*>(intra_dyn);
   IntraScratch* scratch = reinterpret_cast<IntraScratch*>(intra_dyn + (with_front ? sizeof(Frontier) : 0));
-  LdsI32* dct = (LdsI32*)s_dct;
   const FrameHdr* hdr = reinterpret_cast<const FrameHdr*>(desc);
   const TileDesc* tiles = reinterpret_cast<const TileDesc*>(desc + desc_tiles_off());
   const int lane = lane_id(), wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
@@ -3224,7 +3228,7 @@ __device__ __forceinline__ void recon_intra_tile(const uint8_t* __restrict__ des
   if (threadIdx.x < kMaxStepWords) step_mask[threadIdx.x] = 0;
   if (levels)
     for (int i = threadIdx.x; i < (tw * th + 31) / 32; i += blockDim.x) s_bits[i] = 0;
-  load_dct(dct);  // (its barrier also orders the clears above)
+  __syncthreads();  // (orders the clears above)
   if (levels) {
     for (uint32_t i = threadIdx.x; i < st.mbs_done; i += blockDim.x) {
       const int ry = static_cast<int>(i) / tw, rx = static_cast<int>(i) - ry * tw;
@@ -3294,7 +3298,8 @@ __device__ __forceinline__ void recon_intra_tile(const uint8_t* __restrict__ des
         const int local = static_cast<int>(e & 0xFFFFu);
         const int ry = local / tw, rx = local - ry * tw;
         const int mbx = td.x0 + rx, mby = td.y0 + ry;
-        intra_mb(scratch[wave], dct, coefs, recs[mby * mbc + mbx], hdr, td, rec, mbx, mby, nullptr, lane);
+        intra_mb(scratch[wave], res + static_cast<size_t>(mby * mbc + mbx) * kResPerMB, recs[mby * mbc + mbx], td, rec,
+                 mbx, mby, nullptr, lane);
       }
       // the next level reads this one's pixels from the frame (see below)
       __syncthreads();
@@ -3325,7 +3330,8 @@ __device__ __forceinline__ void recon_intra_tile(const uint8_t* __restrict__ des
       int mbx = td.x0 + rx, mby = td.y0 + ry;
       const MBRec r = recs[mby * mbc + mbx];
       if (r.kind != 2) continue;
-      intra_mb(scratch[wave], dct, coefs, r, hdr, td, rec, mbx, mby, keyf ? &front : nullptr, lane);
+      intra_mb(scratch[wave], res + static_cast<size_t>(mby * mbc + mbx) * kResPerMB, r, td, rec, mbx, mby,
+               keyf ? &front : nullptr, lane);
     }
     // Macroblocks of the next step read what other wavefronts of this
     // workgroup stored in this one.  In a key frame that is the LDS frontier
@@ -3344,7 +3350,7 @@ __global__ __launch_bounds__(64 * kIntraMaxWaves) void ebiv_recon_intra(const Fr
                                                                           int with_front) {
   const WorkItem it = item_of(items);
   ConstJob& job = job_of(jobs, it.job);
-  recon_intra_tile(job.desc, job.recs, job.coefs, job.status, planes_of(job.rec), static_cast<int>(it.idx),
+  recon_intra_tile(job.desc, job.recs, job.res, job.status, planes_of(job.rec), static_cast<int>(it.idx),
                    with_front != 0);
 }
 

@@ -162,6 +162,7 @@ struct PayloadSlot {
   bool inter = false;
   DevArray<MBRec> recs;
   DevArray<uint32_t> coefs;
+  DevArray<int16_t> res;
   size_t status_off = 0;  // first tile's entry in the step's status array
   PinnedArray<TileStatus> h_status;  // the last step's tile statuses
 };
@@ -444,7 +445,8 @@ struct Session {
     ps->desc_bytes = dsize;
     ps->inter = inter;
     const size_t nmb = static_cast<size_t>(mbc) * mbr;
-    if (!ps->recs.reserve(nmb) || !ps->coefs.reserve(nmb * kCoefPerMB) || !ps->h_status.reserve(nt))
+    if (!ps->recs.reserve(nmb) || !ps->coefs.reserve(nmb * kCoefPerMB) || !ps->res.reserve(nmb * kResPerMB) ||
+        !ps->h_status.reserve(nt))
       return fail(EBIV_ERR_DEVICE, "payload buffer allocation failed");
     return EBIV_OK;
   }
@@ -833,6 +835,7 @@ int Engine::run_step(std::vector<Session*>& ss, std::vector<std::vector<StepFram
     j.desc = d_stage.d + ps->desc_off;
     j.recs = ps->recs.d;
     j.coefs = ps->coefs.d;
+    j.res = ps->res.d;
     j.status = d_step_status.d + ps->status_off;
     j.mbc = s.mbc;
     j.mbr = s.mbr;
@@ -858,8 +861,7 @@ int Engine::run_step(std::vector<Session*>& ss, std::vector<std::vector<StepFram
   auto add_recon_items = [&](size_t j, bool inter_part, size_t& w) {
     const JobRef& jr = jrefs[j];
     const PayloadSlot* ps = payload_of(ss, frames, jr);
-    if (inter_part) {
-      if (!ps->inter) return;
+    if (inter_part) {  // (key frames too: the launch forms intra residuals)
       h_items.h[w++] = WorkItem{static_cast<uint32_t>(j), inter_prefix};
       inter_prefix += inter_groups(j);
     } else {
@@ -874,7 +876,7 @@ int Engine::run_step(std::vector<Session*>& ss, std::vector<std::vector<StepFram
     ps->status_off = nparse;
     nparse += static_cast<size_t>(ps->ntiles);
     (void)s;
-    nitems += 2 * static_cast<size_t>(ps->ntiles) + (ps->inter ? 1 : 0);
+    nitems += 2 * static_cast<size_t>(ps->ntiles) + 1;
     max_tw = std::max(max_tw, ps->max_tile_w);
   }
   const size_t njobs = jrefs.size();
@@ -1012,7 +1014,7 @@ int Engine::run_step(std::vector<Session*>& ss, std::vector<std::vector<StepFram
         const Session& s = *ss[i];
         fill_job(h_jobs.h[0], jr);
         size_t n = 0;
-        if (ps->inter) h_items.h[n++] = WorkItem{0, 0};
+        h_items.h[n++] = WorkItem{0, 0};
         const size_t ninter = n;
         const uint32_t groups = (static_cast<uint32_t>(s.mbc * s.mbr) + kInterGroupMbs - 1) / kInterGroupMbs;
         for (int t = 0; t < ps->ntiles; t++) h_items.h[n++] = WorkItem{0, static_cast<uint32_t>(t)};
@@ -1482,7 +1484,7 @@ size_t ebiv_session_bytes(int width, int height, int ring_size, int window) {
   const size_t rs = static_cast<size_t>(std::max(ring_size >= 2 ? ring_size : 3, window));
   const size_t pool = (rs + 2 + static_cast<size_t>(window) + 1) * (ysz + 2 * csz);
   // payload slots: a window's payloads plus an override
-  const size_t payload = (static_cast<size_t>(window) + 1) * mbc * mbr * (sizeof(MBRec) + kCoefPerMB * 4);
+  const size_t payload = (static_cast<size_t>(window) + 1) * mbc * mbr * (sizeof(MBRec) + kCoefPerMB * 4 + kResPerMB * 2);
   return pool + payload;
 }
 
