@@ -1014,6 +1014,194 @@ struct MbLoop {
   "s_mov_b32 %[cn], 0\n"                         \
   "s_branch L%=_" tag "_cfd\n"
 
+// ---- token loop specialised by rANS phase ----------------------------------
+// Four copies of the hot token loop, one per interleave phase P: the next
+// symbol uses state register X[P] (x0..x3), which the decode updates in
+// place, so a symbol costs no register rotation.  Leaving the loop (escape,
+// end of block) rotates the registers back to phase 0, where the rest of
+// the loop keeps x0 as the next symbol's state.
+#define MBP_RN(tag, R)                           \
+  "s_cmp_lt_u32 " R ", 0x800000\n"               \
+  "s_cbranch_scc1 L%=_" tag "_rn\n"              \
+  "L%=_" tag "_rot:\n"                           \
+  "s_add_u32 %[nsym], %[nsym], 1\n"
+#define MBP_RN_OOL(tag, underrun, R)             \
+  "L%=_" tag "_rn:\n"                            \
+  "s_cmp_eq_u32 %[pos], 0\n"                     \
+  "s_cbranch_scc1 L%=_" underrun "\n"            \
+  "s_sub_u32 %[pos], %[pos], 1\n"                \
+  "s_cmp_lt_i32 %[pos], %[wbase]\n"              \
+  "s_cbranch_scc1 L%=_" tag "_sl\n"              \
+  "L%=_" tag "_by:\n"                            \
+  "s_sub_u32 %[t1], %[pos], %[wbase]\n"          \
+  "s_lshr_b32 %[t2], %[t1], 2\n"                 \
+  "v_readlane_b32 %[t2], %[vwin], %[t2]\n"       \
+  "s_and_b32 %[t1], %[t1], 3\n"                  \
+  "s_lshl_b32 %[t1], %[t1], 3\n"                 \
+  "s_lshr_b32 %[t2], %[t2], %[t1]\n"             \
+  "s_and_b32 %[t2], %[t2], 0xff\n"               \
+  "s_lshl_b32 " R ", " R ", 8\n"                 \
+  "s_or_b32 " R ", " R ", %[t2]\n"               \
+  "s_cmp_lt_u32 " R ", 0x800000\n"               \
+  "s_cbranch_scc1 L%=_" tag "_rn\n"              \
+  "s_branch L%=_" tag "_rot\n"                   \
+  "L%=_" tag "_sl:\n"                            \
+  "s_sub_u32 %[wbase], %[wbase], 256\n"          \
+  "s_max_i32 %[lim], %[wbase], 0\n"              \
+  "s_sub_u32 %[t1], %[wbase], %[wlo]\n"          \
+  "v_add_u32 %[vaddr], %[t1], %[vlane4]\n"       \
+  "v_max_i32 %[vt], %[vaddr], 0\n"               \
+  "v_min_i32 %[vt], %[hi2], %[vt]\n"             \
+  "global_load_dword %[vwin], %[vt], %[tb]\n"    \
+  "v_cmp_ge_u32_e32 vcc, %[hi2], %[vaddr]\n"     \
+  "s_waitcnt vmcnt(0)\n"                         \
+  "v_cndmask_b32_e32 %[vwin], 0, %[vwin], vcc\n" \
+  "s_branch L%=_" tag "_by\n"
+// Explicit sign (as MB_SIGN) with state register R updated in place.
+#define MBP_SIGN(tag, R)                         \
+  "s_sub_u32 " MB_NEG ", 0, " MB_MAG "\n"        \
+  "s_and_b32 %[slot], " R ", 0xfff\n"            \
+  "s_lshr_b32 " R ", " R ", 12\n"                \
+  "s_cmp_ge_u32 %[slot], %[f0]\n"                \
+  "s_cselect_b32 %[p], %[f0], 0\n"               \
+  "s_cselect_b32 %[t2], %[f1], %[f0]\n"          \
+  "s_cselect_b32 " MB_MAG ", " MB_NEG ", " MB_MAG "\n" \
+  "s_mul_i32 " R ", %[t2], " R "\n"              \
+  "s_sub_u32 %[t2], %[slot], %[p]\n"             \
+  "s_add_u32 " R ", " R ", %[t2]\n"              \
+  MBP_RN(tag, R)
+// The token loop at phase P (state RA); RB is the next phase's state (the
+// sign's).  NP1, NP2: the phases after one and two symbols.
+#define MBP_TOK(P, NP1, NP2, RA, RB)             \
+  "L%=_tok" P ":\n"                              \
+  "s_and_b32 %[slot], " RA ", 0xfff\n"           \
+  "s_lshl_b32 %[t1], %[slot], 13\n"              \
+  "s_or_b32 %[t1], %[t1], 0x1fff\n"              \
+  "v_cmp_ge_u32_e64 %[b], %[t1], %[vcur]\n"      \
+  "s_add_u32 %[t2], %[boff], %[prev]\n"          \
+  "s_or_b32 %[t1], %[t2], 0x80000\n"             \
+  "s_lshr_b32 " RA ", " RA ", 12\n"              \
+  "s_bfe_u64 %[b], %[b], %[t1]\n"                \
+  "s_bcnt1_i32_b64 %[cnt], %[b]\n"               \
+  "s_cmp_eq_u32 %[cnt], 1\n"                     \
+  "s_cbranch_scc0 L%=_tnz" P "\n"                \
+  "v_readlane_b32 %[p], %[vcur], %[t2]\n"        \
+  "s_mov_b32 %[prev], 0\n"                       \
+  "s_and_b32 %[p], %[p], 0x1fff\n"               \
+  "s_mul_i32 " RA ", %[p], " RA "\n"             \
+  "s_add_u32 " RA ", " RA ", %[slot]\n"          \
+  MBP_RN("tz" P, RA)                             \
+  "s_add_u32 %[si], %[si], 1\n"                  \
+  "s_cmp_lg_u32 %[si], %[stop]\n"                \
+  "s_cbranch_scc1 L%=_tok" NP1 "\n"              \
+  "s_branch L%=_tstop" NP1 "\n"                  \
+  "L%=_tnz" P ":\n"                              \
+  "s_cmp_eq_u32 %[cnt], 0\n"                     \
+  "s_cbranch_scc1 L%=_unused\n"                  \
+  "s_add_u32 %[t1], %[t2], %[cnt]\n"             \
+  "s_add_u32 %[t1], %[t1], -1\n"                 \
+  "v_readlane_b32 %[p], %[vcur], %[t1]\n"        \
+  "s_add_u32 " MB_MAG ", %[cnt], -1\n"           \
+  "s_and_b32 %[t2], %[p], 0x1fff\n"              \
+  "s_lshr_b32 %[p], %[p], 13\n"                  \
+  "s_mul_i32 " RA ", %[t2], " RA "\n"            \
+  "s_sub_u32 %[t2], %[slot], %[p]\n"             \
+  "s_add_u32 " RA ", " RA ", %[t2]\n"            \
+  MBP_RN("tn" P, RA)                             \
+  "s_cmp_ge_u32 %[cnt], 6\n"                     \
+  "s_cbranch_scc1 L%=_nesc" NP1 "\n"             \
+  "s_cmp_eq_u32 " MB_MAG ", 1\n"                 \
+  "s_cselect_b32 %[prev], 8, 16\n"               \
+  "L%=_tlev" NP1 ":\n"                           \
+  "s_add_u32 %[sumabs], %[sumabs], " MB_MAG "\n" \
+  "s_cmp_eq_u32 %[firstsi], 0\n"                 \
+  "s_cbranch_scc1 L%=_tfirst" NP1 "\n"           \
+  MBP_SIGN("ts" P, RB)                           \
+  MB_ZZ                                          \
+  MB_EMIT("e1" P)                                \
+  "s_mov_b32 %[last], %[si]\n"                   \
+  "s_add_u32 %[si], %[si], 1\n"                  \
+  "s_cmp_lg_u32 %[si], %[stop]\n"                \
+  "s_cbranch_scc1 L%=_tok" NP2 "\n"              \
+  "s_branch L%=_tstop" NP2 "\n"                  \
+  "L%=_tfirst" NP1 ":\n"                         \
+  MB_ZZ                                          \
+  "s_lshl_b32 %[t1], " MB_MAG ", 8\n"            \
+  "s_or_b32 %[firstpm], %[t1], %[t2]\n"          \
+  "s_add_u32 %[firstsi], %[si], 1\n"             \
+  "s_mov_b32 %[last], %[si]\n"                   \
+  "s_add_u32 %[si], %[si], 1\n"                  \
+  "s_cmp_lg_u32 %[si], %[stop]\n"                \
+  "s_cbranch_scc1 L%=_tok" NP1 "\n"              \
+  "s_branch L%=_tstop" NP1 "\n"
+// si reached stop at phase P: the end of the block (back to phase 0, then
+// the end-of-block code) or a band boundary (bandOf thresholds 1, 3, 9, 21,
+// model.go:165-180).
+#define MBP_TSTOP(P)                             \
+  "L%=_tstop" P ":\n"                            \
+  "s_cmp_eq_u32 %[si], %[total]\n"               \
+  "s_cbranch_scc1 L%=_nend" P "\n"               \
+  "s_xor_b32 %[boff], %[boff], 24\n"             \
+  "s_cmp_eq_u32 %[si], 1\n"                      \
+  "s_cselect_b32 %[stop], 3, %[stop]\n"          \
+  "s_cmp_eq_u32 %[si], 9\n"                      \
+  "s_cselect_b32 %[stop], 21, %[stop]\n"         \
+  "s_cmp_eq_u32 %[si], 3\n"                      \
+  "s_cbranch_scc0 L%=_tb2" P "\n"                \
+  "v_mov_b32 %[vcur], %[vb]\n"                   \
+  "s_mov_b32 %[stop], 9\n"                       \
+  "L%=_tb2" P ":\n"                              \
+  "s_cmp_eq_u32 %[si], 21\n"                     \
+  "s_cbranch_scc0 L%=_tb4" P "\n"                \
+  "v_mov_b32 %[vcur], %[vc]\n"                   \
+  "s_mov_b32 %[stop], %[total]\n"                \
+  "L%=_tb4" P ":\n"                              \
+  "s_min_u32 %[stop], %[stop], %[total]\n"       \
+  "s_branch L%=_tok" P "\n"
+// Back to phase 0 from phase 1, 2, 3 (t1 as the spare register).
+#define MBP_NORM1                                \
+  "s_mov_b32 %[t1], %[x0]\n"                     \
+  "s_mov_b32 %[x0], %[x1]\n"                     \
+  "s_mov_b32 %[x1], %[x2]\n"                     \
+  "s_mov_b32 %[x2], %[x3]\n"                     \
+  "s_mov_b32 %[x3], %[t1]\n"
+#define MBP_NORM2                                \
+  "s_mov_b32 %[t1], %[x0]\n"                     \
+  "s_mov_b32 %[x0], %[x2]\n"                     \
+  "s_mov_b32 %[x2], %[t1]\n"                     \
+  "s_mov_b32 %[t1], %[x1]\n"                     \
+  "s_mov_b32 %[x1], %[x3]\n"                     \
+  "s_mov_b32 %[x3], %[t1]\n"
+#define MBP_NORM3                                \
+  "s_mov_b32 %[t1], %[x3]\n"                     \
+  "s_mov_b32 %[x3], %[x2]\n"                     \
+  "s_mov_b32 %[x2], %[x1]\n"                     \
+  "s_mov_b32 %[x1], %[x0]\n"                     \
+  "s_mov_b32 %[x0], %[t1]\n"
+#define MBP_TOKLOOP                              \
+  MBP_TOK("0", "1", "2", "%[x0]", "%[x1]")       \
+  MBP_TOK("1", "2", "3", "%[x1]", "%[x2]")       \
+  MBP_TOK("2", "3", "0", "%[x2]", "%[x3]")       \
+  MBP_TOK("3", "0", "1", "%[x3]", "%[x0]")       \
+  MBP_TSTOP("0") MBP_TSTOP("1") MBP_TSTOP("2") MBP_TSTOP("3") \
+  "L%=_nesc0:\n" "s_branch L%=_tescob\n"         \
+  "L%=_nesc1:\n" MBP_NORM1 "s_branch L%=_tescob\n" \
+  "L%=_nesc2:\n" MBP_NORM2 "s_branch L%=_tescob\n" \
+  "L%=_nesc3:\n" MBP_NORM3 "s_branch L%=_tescob\n" \
+  "L%=_nend0:\n" "s_branch L%=_tend\n"           \
+  "L%=_nend1:\n" MBP_NORM1 "s_branch L%=_tend\n" \
+  "L%=_nend2:\n" MBP_NORM2 "s_branch L%=_tend\n" \
+  "L%=_nend3:\n" MBP_NORM3 "s_branch L%=_tend\n"
+#define MBP_TOKLOOP_OOL                          \
+  MBP_RN_OOL("tz0", "underrun", "%[x0]") MBP_RN_OOL("tn0", "underrun", "%[x0]") \
+  MBP_RN_OOL("ts0", "underrun", "%[x1]") MB_EMIT_OOL("e10")                     \
+  MBP_RN_OOL("tz1", "underrun", "%[x1]") MBP_RN_OOL("tn1", "underrun", "%[x1]") \
+  MBP_RN_OOL("ts1", "underrun", "%[x2]") MB_EMIT_OOL("e11")                     \
+  MBP_RN_OOL("tz2", "underrun", "%[x2]") MBP_RN_OOL("tn2", "underrun", "%[x2]") \
+  MBP_RN_OOL("ts2", "underrun", "%[x3]") MB_EMIT_OOL("e12")                     \
+  MBP_RN_OOL("tz3", "underrun", "%[x3]") MBP_RN_OOL("tn3", "underrun", "%[x3]") \
+  MBP_RN_OOL("ts3", "underrun", "%[x0]") MB_EMIT_OOL("e13")
+
 // Record dword k (value v) of the staged macroblock nst.
 #define MB_REC(k, v)                             \
   "s_add_u32 m0, %[t2], " #k "\n"                \
@@ -1245,90 +1433,7 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       "s_mov_b32 %[sumabs], 0\n"
       "s_mov_b32 %[firstsi], 0\n"
       "v_mov_b32 %[vcur], %[va]\n"
-      "L%=_tok:\n"
-      "s_and_b32 %[slot], %[x0], 0xfff\n"
-      "s_lshl_b32 %[t1], %[slot], 13\n"
-      "s_or_b32 %[t1], %[t1], 0x1fff\n"
-      "v_cmp_ge_u32_e64 %[b], %[t1], %[vcur]\n"
-      "s_add_u32 %[t2], %[boff], %[prev]\n"
-      "s_or_b32 %[t1], %[t2], 0x80000\n"
-      "s_lshr_b32 %[xn], %[x0], 12\n"
-      "s_bfe_u64 %[b], %[b], %[t1]\n"
-      "s_bcnt1_i32_b64 %[cnt], %[b]\n"
-      "s_cmp_eq_u32 %[cnt], 1\n"
-      "s_cbranch_scc0 L%=_tnz\n"
-      // zero token: symbol 0 starts at 0
-      "v_readlane_b32 %[p], %[vcur], %[t2]\n"
-      "s_mov_b32 %[prev], 0\n"
-      "s_and_b32 %[p], %[p], 0x1fff\n"
-      "s_mul_i32 %[xn], %[p], %[xn]\n"
-      "s_add_u32 %[xn], %[xn], %[slot]\n"
-      MB_RN("tz")
-      "s_add_u32 %[si], %[si], 1\n"
-      "s_cmp_lg_u32 %[si], %[stop]\n"
-      "s_cbranch_scc1 L%=_tok\n"
-      "s_branch L%=_tstop\n"
-      "L%=_tnz:\n"
-      "s_cmp_eq_u32 %[cnt], 0\n"
-      "s_cbranch_scc1 L%=_unused\n"
-      "s_add_u32 %[t1], %[t2], %[cnt]\n"
-      "s_add_u32 %[t1], %[t1], -1\n"
-      "v_readlane_b32 %[p], %[vcur], %[t1]\n"
-      "s_add_u32 " MB_MAG ", %[cnt], -1\n"
-      "s_and_b32 %[t2], %[p], 0x1fff\n"
-      "s_lshr_b32 %[p], %[p], 13\n"
-      "s_mul_i32 %[xn], %[t2], %[xn]\n"
-      "s_sub_u32 %[t2], %[slot], %[p]\n"
-      "s_add_u32 %[xn], %[xn], %[t2]\n"
-      MB_RN("tn")
-      "s_cmp_ge_u32 %[cnt], 6\n"
-      "s_cbranch_scc1 L%=_tescob\n"
-      "s_cmp_eq_u32 " MB_MAG ", 1\n"
-      "s_cselect_b32 %[prev], 8, 16\n"
-      "L%=_tlev:\n"
-      "s_add_u32 %[sumabs], %[sumabs], " MB_MAG "\n"
-      "s_cmp_eq_u32 %[firstsi], 0\n"
-      "s_cbranch_scc1 L%=_tfirst\n"
-      MB_SIGN("ts")
-      MB_ZZ
-      MB_EMIT("e1")
-      "s_mov_b32 %[last], %[si]\n"
-      "s_add_u32 %[si], %[si], 1\n"
-      "s_cmp_lg_u32 %[si], %[stop]\n"
-      "s_cbranch_scc1 L%=_tok\n"
-      "s_branch L%=_tstop\n"
-      // first nonzero level: its sign resolves at the end of the block
-      "L%=_tfirst:\n"
-      MB_ZZ
-      "s_lshl_b32 %[t1], " MB_MAG ", 8\n"
-      "s_or_b32 %[firstpm], %[t1], %[t2]\n"
-      "s_add_u32 %[firstsi], %[si], 1\n"
-      "s_mov_b32 %[last], %[si]\n"
-      "s_add_u32 %[si], %[si], 1\n"
-      "s_cmp_lg_u32 %[si], %[stop]\n"
-      "s_cbranch_scc1 L%=_tok\n"
-      // si reached stop: the end of the block or a band boundary
-      // (bandOf thresholds 1, 3, 9, 21, model.go:165-180)
-      "L%=_tstop:\n"
-      "s_cmp_eq_u32 %[si], %[total]\n"
-      "s_cbranch_scc1 L%=_tend\n"
-      "s_xor_b32 %[boff], %[boff], 24\n"
-      "s_cmp_eq_u32 %[si], 1\n"
-      "s_cselect_b32 %[stop], 3, %[stop]\n"
-      "s_cmp_eq_u32 %[si], 9\n"
-      "s_cselect_b32 %[stop], 21, %[stop]\n"
-      "s_cmp_eq_u32 %[si], 3\n"
-      "s_cbranch_scc0 L%=_tb2\n"
-      "v_mov_b32 %[vcur], %[vb]\n"
-      "s_mov_b32 %[stop], 9\n"
-      "L%=_tb2:\n"
-      "s_cmp_eq_u32 %[si], 21\n"
-      "s_cbranch_scc0 L%=_tb4\n"
-      "v_mov_b32 %[vcur], %[vc]\n"
-      "s_mov_b32 %[stop], %[total]\n"
-      "L%=_tb4:\n"
-      "s_min_u32 %[stop], %[stop], %[total]\n"
-      "s_branch L%=_tok\n"
+      MBP_TOKLOOP
       // escape (magnitude 5 + getEscapeValue, mv.go:124-128) or end of block
       "L%=_tescob:\n"
       "s_cmp_eq_u32 %[cnt], 7\n"
@@ -1336,7 +1441,7 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       MB_CLASS("%[vm2]", "0", "-1", "0x100000", "es")
       "s_add_u32 " MB_MAG ", %[sym], 5\n"
       "s_mov_b32 %[prev], 16\n"
-      "s_branch L%=_tlev\n"
+      "s_branch L%=_tlev0\n"
       // end of block: the first level's sign, hidden in the level-sum parity
       // when last - first >= 4, else explicit
       "L%=_tend:\n"
@@ -1532,13 +1637,10 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       MB_RN_OOL("tx", "underrun")
       MB_RN_OOL("cm", "underrun")
       MB_RN_OOL("lm", "underrun")
-      MB_RN_OOL("tz", "underrun")
-      MB_RN_OOL("tn", "underrun")
-      MB_RN_OOL("ts", "underrun")
       MB_RN_OOL("esd", "underrun")
       MB_RN_OOL("esb", "underrun")
       MB_RN_OOL("sd", "underrun")
-      MB_EMIT_OOL("e1")
+      MBP_TOKLOOP_OOL
       MB_EMIT_OOL("e2")
 
       "L%=_exit:\n"
@@ -1761,90 +1863,7 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       "s_mov_b32 %[sumabs], 0\n"
       "s_mov_b32 %[firstsi], 0\n"
       "v_mov_b32 %[vcur], %[va]\n"
-      "L%=_tok:\n"
-      "s_and_b32 %[slot], %[x0], 0xfff\n"
-      "s_lshl_b32 %[t1], %[slot], 13\n"
-      "s_or_b32 %[t1], %[t1], 0x1fff\n"
-      "v_cmp_ge_u32_e64 %[b], %[t1], %[vcur]\n"
-      "s_add_u32 %[t2], %[boff], %[prev]\n"
-      "s_or_b32 %[t1], %[t2], 0x80000\n"
-      "s_lshr_b32 %[xn], %[x0], 12\n"
-      "s_bfe_u64 %[b], %[b], %[t1]\n"
-      "s_bcnt1_i32_b64 %[cnt], %[b]\n"
-      "s_cmp_eq_u32 %[cnt], 1\n"
-      "s_cbranch_scc0 L%=_tnz\n"
-      // zero token: symbol 0 starts at 0
-      "v_readlane_b32 %[p], %[vcur], %[t2]\n"
-      "s_mov_b32 %[prev], 0\n"
-      "s_and_b32 %[p], %[p], 0x1fff\n"
-      "s_mul_i32 %[xn], %[p], %[xn]\n"
-      "s_add_u32 %[xn], %[xn], %[slot]\n"
-      MB_RN("tz")
-      "s_add_u32 %[si], %[si], 1\n"
-      "s_cmp_lg_u32 %[si], %[stop]\n"
-      "s_cbranch_scc1 L%=_tok\n"
-      "s_branch L%=_tstop\n"
-      "L%=_tnz:\n"
-      "s_cmp_eq_u32 %[cnt], 0\n"
-      "s_cbranch_scc1 L%=_unused\n"
-      "s_add_u32 %[t1], %[t2], %[cnt]\n"
-      "s_add_u32 %[t1], %[t1], -1\n"
-      "v_readlane_b32 %[p], %[vcur], %[t1]\n"
-      "s_add_u32 " MB_MAG ", %[cnt], -1\n"
-      "s_and_b32 %[t2], %[p], 0x1fff\n"
-      "s_lshr_b32 %[p], %[p], 13\n"
-      "s_mul_i32 %[xn], %[t2], %[xn]\n"
-      "s_sub_u32 %[t2], %[slot], %[p]\n"
-      "s_add_u32 %[xn], %[xn], %[t2]\n"
-      MB_RN("tn")
-      "s_cmp_ge_u32 %[cnt], 6\n"
-      "s_cbranch_scc1 L%=_tescob\n"
-      "s_cmp_eq_u32 " MB_MAG ", 1\n"
-      "s_cselect_b32 %[prev], 8, 16\n"
-      "L%=_tlev:\n"
-      "s_add_u32 %[sumabs], %[sumabs], " MB_MAG "\n"
-      "s_cmp_eq_u32 %[firstsi], 0\n"
-      "s_cbranch_scc1 L%=_tfirst\n"
-      MB_SIGN("ts")
-      MB_ZZ
-      MB_EMIT("e1")
-      "s_mov_b32 %[last], %[si]\n"
-      "s_add_u32 %[si], %[si], 1\n"
-      "s_cmp_lg_u32 %[si], %[stop]\n"
-      "s_cbranch_scc1 L%=_tok\n"
-      "s_branch L%=_tstop\n"
-      // first nonzero level: its sign resolves at the end of the block
-      "L%=_tfirst:\n"
-      MB_ZZ
-      "s_lshl_b32 %[t1], " MB_MAG ", 8\n"
-      "s_or_b32 %[firstpm], %[t1], %[t2]\n"
-      "s_add_u32 %[firstsi], %[si], 1\n"
-      "s_mov_b32 %[last], %[si]\n"
-      "s_add_u32 %[si], %[si], 1\n"
-      "s_cmp_lg_u32 %[si], %[stop]\n"
-      "s_cbranch_scc1 L%=_tok\n"
-      // si reached stop: the end of the block or a band boundary
-      // (bandOf thresholds 1, 3, 9, 21, model.go:165-180)
-      "L%=_tstop:\n"
-      "s_cmp_eq_u32 %[si], %[total]\n"
-      "s_cbranch_scc1 L%=_tend\n"
-      "s_xor_b32 %[boff], %[boff], 24\n"
-      "s_cmp_eq_u32 %[si], 1\n"
-      "s_cselect_b32 %[stop], 3, %[stop]\n"
-      "s_cmp_eq_u32 %[si], 9\n"
-      "s_cselect_b32 %[stop], 21, %[stop]\n"
-      "s_cmp_eq_u32 %[si], 3\n"
-      "s_cbranch_scc0 L%=_tb2\n"
-      "v_mov_b32 %[vcur], %[vb]\n"
-      "s_mov_b32 %[stop], 9\n"
-      "L%=_tb2:\n"
-      "s_cmp_eq_u32 %[si], 21\n"
-      "s_cbranch_scc0 L%=_tb4\n"
-      "v_mov_b32 %[vcur], %[vc]\n"
-      "s_mov_b32 %[stop], %[total]\n"
-      "L%=_tb4:\n"
-      "s_min_u32 %[stop], %[stop], %[total]\n"
-      "s_branch L%=_tok\n"
+      MBP_TOKLOOP
       // escape (magnitude 5 + getEscapeValue, mv.go:124-128) or end of block
       "L%=_tescob:\n"
       "s_cmp_eq_u32 %[cnt], 7\n"
@@ -1852,7 +1871,7 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       MB_CLASS("%[vm2]", "0", "-1", "0x100000", "es")
       "s_add_u32 " MB_MAG ", %[sym], 5\n"
       "s_mov_b32 %[prev], 16\n"
-      "s_branch L%=_tlev\n"
+      "s_branch L%=_tlev0\n"
       // end of block: the first level's sign, hidden in the level-sum parity
       // when last - first >= 4, else explicit
       "L%=_tend:\n"
@@ -2048,13 +2067,10 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       MB_RN_OOL("tx", "underrun")
       MB_RN_OOL("cm", "underrun")
       MB_RN_OOL("lm", "underrun")
-      MB_RN_OOL("tz", "underrun")
-      MB_RN_OOL("tn", "underrun")
-      MB_RN_OOL("ts", "underrun")
       MB_RN_OOL("esd", "underrun")
       MB_RN_OOL("esb", "underrun")
       MB_RN_OOL("sd", "underrun")
-      MB_EMIT_OOL("e1")
+      MBP_TOKLOOP_OOL
       MB_EMIT_OOL("e2")
 
       "L%=_exit:\n"
