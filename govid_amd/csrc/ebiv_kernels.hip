@@ -1071,7 +1071,8 @@ struct MbLoop {
   "s_add_u32 " R ", " R ", %[t2]\n"              \
   MBP_RN(tag, R)
 // The token loop at phase P (state RA); RB is the next phase's state (the
-// sign's).  NP1, NP2: the phases after one and two symbols.
+// sign's).  NP1, NP2: the phases after one and two symbols.  A zero token is
+// symbol 0, whose table entry is its frequency alone (start 0).
 #define MBP_TOK(P, NP1, NP2, RA, RB)             \
   "L%=_tok" P ":\n"                              \
   "s_and_b32 %[slot], " RA ", 0xfff\n"           \
@@ -1087,7 +1088,6 @@ struct MbLoop {
   "s_cbranch_scc0 L%=_tnz" P "\n"                \
   "v_readlane_b32 %[p], %[vcur], %[t2]\n"        \
   "s_mov_b32 %[prev], 0\n"                       \
-  "s_and_b32 %[p], %[p], 0x1fff\n"               \
   "s_mul_i32 " RA ", %[p], " RA "\n"             \
   "s_add_u32 " RA ", " RA ", %[slot]\n"          \
   MBP_RN("tz" P, RA)                             \
@@ -2472,7 +2472,16 @@ __global__ __launch_bounds__(64) void ebiv_parse(const FrameJob* __restrict__ jo
   ConstJob& job = job_of(jobs, it.job);
   const FrameHdr* hdr = reinterpret_cast<const FrameHdr*>(job.desc);
   const TileDesc* tiles = reinterpret_cast<const TileDesc*>(job.desc + desc_tiles_off());
-  const int t = static_cast<int>(it.idx);
+  // Bits 30-31 of the item: the wave's issue priority (the host gives the
+  // biggest tiles, the longest serial chains, the highest).
+  const uint32_t prio = it.idx >> 30;
+  if (prio == 3)
+    __builtin_amdgcn_s_setprio(3);
+  else if (prio == 2)
+    __builtin_amdgcn_s_setprio(2);
+  else if (prio == 1)
+    __builtin_amdgcn_s_setprio(1);
+  const int t = static_cast<int>(it.idx & 0x3FFFFFFFu);
   if (hdr->vt[0][kGSign] != kNoSym)
     parse_tile_asm(job.pkt, job.desc, job.recs, job.coefs, job.status, t);
   else

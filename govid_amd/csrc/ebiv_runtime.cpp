@@ -870,22 +870,37 @@ int Engine::run_step(std::vector<Session*>& ss, std::vector<std::vector<StepFram
   };
   size_t nitems = 0, nparse = 0;
   int max_tw = 1;
+  uint32_t max_tile_bytes = 1;
+  auto tiles_of = [&](const PayloadSlot* ps) {
+    return reinterpret_cast<const TileDesc*>(ps->h_desc.h + desc_tiles_off());
+  };
   for (const JobRef& jr : jrefs) {
     PayloadSlot* ps = payload_of(ss, frames, jr);
-    const Session& s = *ss[jr.si];
     ps->status_off = nparse;
     nparse += static_cast<size_t>(ps->ntiles);
-    (void)s;
     nitems += 2 * static_cast<size_t>(ps->ntiles) + 1;
     max_tw = std::max(max_tw, ps->max_tile_w);
+    for (int t = 0; t < ps->ntiles; t++) max_tile_bytes = std::max(max_tile_bytes, tiles_of(ps)[t].byte_len);
   }
   const size_t njobs = jrefs.size();
   if (!h_jobs.reserve(njobs + 1) || !d_jobs.reserve(njobs + 1) || !h_items.reserve(nitems + 1) ||
       !d_items.reserve(nitems + 1) || !d_step_status.reserve(nparse + 1) || !h_step_status.reserve(nparse + 1))
     return fail(EBIV_ERR_DEVICE, "job table allocation failed");
   for (size_t j = 0; j < njobs; j++) fill_job(h_jobs.h[j], jrefs[j]);
+  // Entropy items: one per tile, with an issue priority in bits 30-31 of
+  // the tile index.  The launch lasts as long as its longest serial tile
+  // chain, and the chains of the biggest tiles (key frames) would otherwise
+  // share their SIMD's issue slots evenly with many short ones; the biggest
+  // get priority 3, those above 1/4 and 1/8 of the biggest 2 and 1.
   size_t w = 0;
-  for (size_t j = 0; j < njobs; j++) add_recon_items(j, false, w);
+  for (size_t j = 0; j < njobs; j++) {
+    const PayloadSlot* ps = payload_of(ss, frames, jrefs[j]);
+    for (int t = 0; t < ps->ntiles; t++) {
+      const uint64_t len = tiles_of(ps)[t].byte_len;
+      const uint32_t prio = 2 * len >= max_tile_bytes ? 3 : 4 * len >= max_tile_bytes ? 2 : 8 * len >= max_tile_bytes ? 1 : 0;
+      h_items.h[w++] = WorkItem{static_cast<uint32_t>(j), static_cast<uint32_t>(t) | prio << 30};
+    }
+  }
   // Per round: inter macroblock groups, then the intra tiles of key-frame
   // payloads (their own launch: many wavefronts per tile and the LDS
   // frontier, for latency), then those of inter payloads (few wavefronts per
