@@ -9,10 +9,25 @@ Workload: one 1920x1080 4:2:0 clip, 120 frames (three Perlin parallax layers,
            starting at a different key frame and wrapping round (a loop
            restart), decoding up to W frames per session per engine step.
            A step is one pass of every session over the clip.
+  mixed    64 sessions, one stream each: 24 x 640x360, 24 x 1280x720 and
+           16 x 1920x1080 (rects content), each with its own tile grid
+           {1x1, 2x2, 4x3}, gop {30, 60, 120}, q {18, 22, 28} and length
+           60-240 frames.  A step is one pass of every session over its stream.
+  entropy  32 sessions of 1920x1080 noise (sigma 20, random global motion in
+           +-16 px), q12, gop 15, 4x3 tiles: the symbol-heavy case, reported
+           in Msymbols/s.  The sessions replay one 30-frame stream from
+           staggered key frames.
+  serving  Poisson arrivals of play / seek / loop sessions of 360p-2160p
+           streams under an HBM budget; p50/p99 frame latency.
 
-Packets are resident in HBM before the timed regions and frames stay in HBM.
-Every frame of the plain pass and of the batched warmup pass is checked bit
-for bit against the CPU oracle (plane CRCs) outside the timed regions.
+The mixed and entropy streams are generated by bench_data/make_streams.py
+into bench_data/gen/; a section whose streams are absent is reported as
+skipped.  Packets are resident in HBM before the timed regions and frames
+stay in HBM.  Every frame of the plain pass and of each batched warmup pass,
+and a sample of the served frames, is checked bit for bit (plane CRCs)
+against the expected CRCs stored beside each stream, outside the timed
+regions; those are the encoder's reconstruction, which the CPU tests check
+against the oracle decoder.
 
 Prints one JSON line.
 """
@@ -45,37 +60,45 @@ def measured_traffic(workload: str, kernel: str):
         return round(1024 * (k["fetch_kib"] * cal["fetch_scale"] + k["write_kib"] * cal["write_scale"]))
     except (OSError, KeyError, ValueError):
         return None
-STREAM = os.path.join(ROOT, "bench_data", "parallax_1080p.ebiv")
+BENCH_DATA = os.path.join(ROOT, "bench_data")
+GEN = os.path.join(BENCH_DATA, "gen")
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def load_stream():
+def stream_path(name, gen=False):
+    return os.path.join(GEN if gen else BENCH_DATA, name + ".ebiv")
+
+
+def load_stream(name="parallax_1080p"):
+    """Reader and expected plane CRCs of a committed benchmark stream."""
     from govid_amd.container import Reader
 
-    if not os.path.exists(STREAM):
-        log("bench: stream missing, generating it (several minutes)")
-        sys.path.insert(0, os.path.join(ROOT, "bench_data"))
+    path = stream_path(name)
+    if not os.path.exists(path):
+        log(f"bench: {name} missing, generating it (several minutes)")
+        sys.path.insert(0, BENCH_DATA)
         import make_streams
 
-        make_streams.make("parallax_1080p", threads=os.cpu_count() or 8)
-    return Reader.open(STREAM)
+        make_streams.make(name, threads=os.cpu_count() or 8)
+    rd = Reader.open(path)
+    return rd, expected_crcs(path, rd.data)
+
+
+def expected_crcs(path, data):
+    """Per-frame (Y, Cb, Cr) CRC-32s stored beside a stream by
+    bench_data/make_streams.py (the encoder's reconstruction)."""
+    with open(path[:-len(".ebiv")] + ".crc.json") as fh:
+        d = json.load(fh)
+    if d["stream_crc32"] != zlib.crc32(data):
+        raise RuntimeError(f"{path}: the CRC file belongs to another version of the stream")
+    return [tuple(c) for c in d["crc32"]]
 
 
 def _crc(planes):
     return tuple(zlib.crc32(p.tobytes()) for p in planes)
-
-
-def oracle_crcs(packets):
-    """Plane CRCs of every frame from the oracle (C restatement of the Go
-    decoder), straight through the clip."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
-
-    ref = O.OracleCodec(threads=min(os.cpu_count() or 1, 16))
-    return [_crc(ref.decode(p)) for p in packets]
 
 
 def cpu_baseline(packets, budget_s=20.0):
@@ -138,19 +161,29 @@ def run_plain(packets, dev, steps, warmup):
 
 
 def run_batched(packets, dev, want, sessions, window, steps, warmup):
-    from govid_amd.codec import Batch
-
+    """S sessions over one clip, each from a different key frame."""
     keys = [i for i, p in enumerate(packets) if p[0] & 3 == 0]
-    b = Batch()
-    sids = [b.open(ring=window, window=window) for _ in range(sessions)]
-    orders = []
+    plan = []
     for k in range(sessions):
         st = keys[k % len(keys)]
-        orders.append(list(range(st, len(packets))) + list(range(0, st)))
+        plan.append((packets, dev, want, list(range(st, len(packets))) + list(range(0, st))))
+    return run_sessions(plan, window, steps, warmup)
+
+
+def run_sessions(plan, window, steps, warmup):
+    """One Batch engine; plan = [(packets, device buffers, expected CRCs,
+    frame order)] per session.  A pass submits every session's frames in
+    its order and steps the engine until all have completed; the first
+    warmup pass checks every frame."""
+    from govid_amd.codec import Batch
+
+    b = Batch()
+    sids = [b.open(ring=window, window=window) for _ in plan]
 
     def one_pass(check):
-        for sid, order in zip(sids, orders):
+        for sid, (packets, dev, _, order) in zip(sids, plan):
             b.submit_many(sid, [packets[i] for i in order], [dev[i].ptr for i in order], tags=order)
+        wants = {sid: w for sid, (_, _, w, _) in zip(sids, plan)}
         lat, bad, n = [], 0, 0
         while b.pending():
             a = time.perf_counter()
@@ -160,7 +193,7 @@ def run_batched(packets, dev, want, sessions, window, steps, warmup):
                 n += 1
                 if c.status:
                     bad += 1
-                elif check and _crc(c.frame.planes()) != want[c.tag]:
+                elif check and _crc(c.frame.planes()) != wants[c.session][c.tag]:
                     bad += 1
         return lat, bad, n
 
@@ -177,13 +210,83 @@ def run_batched(packets, dev, want, sessions, window, steps, warmup):
         lat += l
         bad += bw
         frames += n
-        log(f"bench: batched step {step} done")
+        log(f"bench: batched step {step} done ({n} frames)")
     elapsed = time.perf_counter() - t0
     kt = b.kernel_times()
     s1 = b.stats()
     b.close()
     return {"elapsed": elapsed, "frames": frames, "lat": lat, "kt": kt, "bad": bad,
             "symbols": s1["symbols"] - s0["symbols"]}
+
+
+def batched_summary(bt, steps, **extra):
+    bk = bt["kt"]
+    return dict(extra, **{
+        "frames_per_s": round(bt["frames"] / bt["elapsed"], 2),
+        "frames_per_step": bt["frames"] // steps,
+        "engine_step_p50_ms": round(statistics.median(bt["lat"]) * 1e3, 3),
+        "engine_step_p99_ms": round(pct(bt["lat"], 0.99) * 1e3, 3),
+        "msymbols_per_s": round(bt["symbols"] / bt["elapsed"] / 1e6, 2),
+        "frames_differing": bt["bad"],
+        "kernel_avg_us": {k: round(kavg_us(bk, k), 2) for k in ("parse", "inter", "intra")},
+        "kernel_launches": {k: bk[f"{k}_n"] for k in ("parse", "inter", "intra")},
+        "recon": {"ebiv_recon_inter": inter_roofline(bk) or {}},
+    })
+
+
+def open_gen_streams(names):
+    """Readers, expected CRCs and device buffers of generated streams, or
+    None when any is absent."""
+    from govid_amd.codec import DeviceBuffer
+    from govid_amd.container import Reader
+
+    out = []
+    for nm in names:
+        path = stream_path(nm, gen=True)
+        if not os.path.exists(path):
+            return None
+        rd = Reader.open(path)
+        pk = rd.packets()
+        out.append((nm, rd, pk, [DeviceBuffer(p) for p in pk], expected_crcs(path, rd.data)))
+    return out
+
+
+def run_mixed(window, steps, warmup):
+    """The mixed batch: 64 sessions, one generated stream each."""
+    names = [f"mixed_{i:02d}" for i in range(64)]
+    st = open_gen_streams(names)
+    if st is None:
+        return {"skipped": "streams absent: python3 bench_data/make_streams.py --jobs 8 mixed"}
+    plan = [(pk, dev, want, list(range(len(pk)))) for _, _, pk, dev, want in st]
+    bt = run_sessions(plan, window, steps, warmup)
+    for *_, dev, _ in st:
+        for d in dev:
+            d.close()
+    sizes = {}
+    for _, rd, pk, _, _ in st:
+        k = f"{rd.header.width}x{rd.header.height}"
+        sizes[k] = sizes.get(k, 0) + 1
+    hd = sum(len(pk) for _, rd, pk, _, _ in st if rd.header.width == 1920) * steps
+    out = batched_summary(bt, steps, sessions=len(st), window=window, streams=sizes,
+                          frames_per_pass=sum(len(pk) for _, _, pk, _, _ in st))
+    out["frames_1080p_share"] = round(hd / max(bt["frames"], 1), 3)
+    return out
+
+
+def run_entropy(sessions, window, steps, warmup):
+    """The entropy-bound batch: S sessions of the 1080p q12 noise stream."""
+    st = open_gen_streams(["noisepan_1080p"])
+    if st is None:
+        return {"skipped": "stream absent: python3 bench_data/make_streams.py noisepan_1080p"}
+    _, rd, pk, dev, want = st[0]
+    bt = run_batched(pk, dev, want, sessions, window, steps, warmup)
+    for d in dev:
+        d.close()
+    out = batched_summary(bt, steps, sessions=sessions, window=window,
+                          stream=f"1920x1080 noise q12 gop15 4x3 tiles, {len(pk)} frames, "
+                                 f"{sum(map(len, pk)) // len(pk)} B per frame")
+    out["symbols_per_frame"] = round(bt["symbols"] / max(bt["frames"], 1))
+    return out
 
 
 SERVE_STREAMS = ("parallax_360p", "parallax_720p", "parallax_1080p", "parallax_2160p")
@@ -194,27 +297,18 @@ def run_serving(requests_n, budget_bytes, window, check_every, seed=5):
     1080p streams (4x3 tiles) and a 2160p stream (8x6 tiles), all of the
     parallax content at q22, gop 60; 20% seek to a random key frame part-way, 10% loop
     restarts; lengths 30-600 frames; sessions admitted under an HBM budget.
-    Every `check_every`-th completed frame is compared with the oracle's
-    straight-through decode of that frame."""
+    Every `check_every`-th completed frame is compared with the stream's
+    expected CRCs for that frame."""
     from govid_amd.codec import Batch, DeviceBuffer
-    from govid_amd.container import Reader
     from govid_amd.serve import Server, Stream, poisson_requests
 
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
-
-    streams, bufs = [], []
+    streams, bufs, want = [], [], {}
     for nm in SERVE_STREAMS:
-        path = os.path.join(ROOT, "bench_data", nm + ".ebiv")
-        rd = Reader.open(path)
+        rd, want[nm] = load_stream(nm)
         pk = rd.packets()
         dev = [DeviceBuffer(p) for p in pk]
         bufs += dev
         streams.append(Stream(nm, rd.header.width, rd.header.height, pk, [d.ptr for d in dev]))
-    want = {}
-    for st in streams:
-        ref = O.OracleCodec(threads=min(os.cpu_count() or 1, 16))
-        want[st.name] = [_crc(ref.decode(p)) for p in st.packets]
     reqs = poisson_requests(streams, requests_n, rate_per_s=200.0, seed=seed, min_len=30, max_len=600)
     checked = [0, 0]
     count = [0]
@@ -271,6 +365,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-only", action="store_true", help="skip the plain pass (for profiling the batched one)")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-entropy", action="store_true")
+    ap.add_argument("--entropy-sessions", type=int, default=32)
     ap.add_argument("--serve-only", action="store_true", help="run only the serving mix")
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-budget-gb", type=float, default=24.0)
@@ -303,13 +400,12 @@ def main():
         sv = run_serving(args.serve_requests, int(args.serve_budget_gb * (1 << 30)), args.serve_window, check_every=17)
         print(json.dumps({"mode": "serving", **{k: (round(v, 3) if isinstance(v, float) else v) for k, v in sv.items()}}))
         return
-    rd = load_stream()
+    rd, want = load_stream()
     packets = rd.packets()
     hdr = rd.header
     W, H = hdr.width, hdr.height
     log(f"bench: {len(packets)} packets, {sum(map(len, packets))} bytes, {W}x{H}")
 
-    want = oracle_crcs(packets)
     dev = [DeviceBuffer(p) for p in packets]
     if args.batched_only:
         bt = run_batched(packets, dev, want, args.sessions, args.window, args.steps, args.warmup)
@@ -317,7 +413,7 @@ def main():
                           "frames_differing": bt["bad"], "sessions": args.sessions, "window": args.window}))
         return
     bad = 0 if args.no_verify else verify_plain(packets, want)
-    log(f"bench: plain verify -> {bad} frames differ from the oracle")
+    log(f"bench: plain verify -> {bad} frames differ from the expected CRCs")
 
     pl = run_plain(packets, dev, args.steps, args.warmup)
     fps = pl["frames"] / pl["elapsed"]
@@ -367,24 +463,18 @@ def main():
         bt = run_batched(packets, dev, want, args.sessions, args.window, args.steps, args.warmup)
         bfps = bt["frames"] / bt["elapsed"]
         log(f"bench: batched {bfps:.1f} frames/s")
-        bk = bt["kt"]
-        out["batched"] = {
-            "frames_per_s": round(bfps, 2),
-            "vs_plain": round(bfps / fps, 2),
-            "sessions": args.sessions, "window": args.window,
-            "frames_per_step": bt["frames"] // args.steps,
-            "engine_step_p50_ms": round(statistics.median(bt["lat"]) * 1e3, 3),
-            "engine_step_p99_ms": round(pct(bt["lat"], 0.99) * 1e3, 3),
-            "msymbols_per_s": round(bt["symbols"] / bt["elapsed"] / 1e6, 2),
-            "frames_differing": bt["bad"],
-            "kernel_avg_us": {k: round(kavg_us(bk, k), 2) for k in ("parse", "inter", "intra")},
-            "kernel_launches": {k: bk[f"{k}_n"] for k in ("parse", "inter", "intra")},
-            "recon": {"ebiv_recon_inter": dict(inter_roofline(bk) or {},
-                                               traffic=measured_traffic("batched_64x1080p_parallax_w8",
-                                                                        "ebiv_recon_inter"))},
-        }
+        out["batched"] = batched_summary(bt, args.steps, vs_plain=round(bfps / fps, 2),
+                                         sessions=args.sessions, window=args.window)
+        out["batched"]["recon"]["ebiv_recon_inter"]["traffic"] = measured_traffic(
+            "batched_64x1080p_parallax_w8", "ebiv_recon_inter")
     for d in dev:
         d.close()
+    if not args.no_mixed:
+        out["mixed"] = run_mixed(args.window, args.steps, args.warmup)
+        log(f"bench: mixed {out['mixed'].get('frames_per_s', out['mixed'].get('skipped'))}")
+    if not args.no_entropy:
+        out["entropy"] = run_entropy(args.entropy_sessions, args.window, args.steps, args.warmup)
+        log(f"bench: entropy {out['entropy'].get('msymbols_per_s', out['entropy'].get('skipped'))} Msymbols/s")
     if not args.no_serve:
         sv = run_serving(args.serve_requests, int(args.serve_budget_gb * (1 << 30)), args.serve_window, check_every=17)
         log(f"bench: serving {sv['frames_per_s']:.1f} frames/s, p99 {sv['p99_frame_ms']:.2f} ms")
