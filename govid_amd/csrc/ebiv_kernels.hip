@@ -1057,6 +1057,13 @@ struct MbLoop {
   "s_waitcnt vmcnt(0)\n"                         \
   "v_cndmask_b32_e32 %[vwin], 0, %[vwin], vcc\n" \
   "s_branch L%=_" tag "_by\n"
+// As MBP_RN, without counting the symbol: the token symbols of a block are
+// counted once at its end (one per scan position passed, plus the end of
+// block token).
+#define MBP_RN_NC(tag, R)                        \
+  "s_cmp_lt_u32 " R ", 0x800000\n"               \
+  "s_cbranch_scc1 L%=_" tag "_rn\n"              \
+  "L%=_" tag "_rot:\n"
 // Explicit sign (as MB_SIGN) with state register R updated in place.
 #define MBP_SIGN(tag, R)                         \
   "s_sub_u32 " MB_NEG ", 0, " MB_MAG "\n"        \
@@ -1090,7 +1097,7 @@ struct MbLoop {
   "s_mov_b32 %[prev], 0\n"                       \
   "s_mul_i32 " RA ", %[p], " RA "\n"             \
   "s_add_u32 " RA ", " RA ", %[slot]\n"          \
-  MBP_RN("tz" P, RA)                             \
+  MBP_RN_NC("tz" P, RA)                             \
   "s_add_u32 %[si], %[si], 1\n"                  \
   "s_cmp_lg_u32 %[si], %[stop]\n"                \
   "s_cbranch_scc1 L%=_tok" NP1 "\n"              \
@@ -1107,7 +1114,7 @@ struct MbLoop {
   "s_mul_i32 " RA ", %[t2], " RA "\n"            \
   "s_sub_u32 %[t2], %[slot], %[p]\n"             \
   "s_add_u32 " RA ", " RA ", %[t2]\n"            \
-  MBP_RN("tn" P, RA)                             \
+  MBP_RN_NC("tn" P, RA)                             \
   "s_cmp_ge_u32 %[cnt], 6\n"                     \
   "s_cbranch_scc1 L%=_nesc" NP1 "\n"             \
   "s_cmp_eq_u32 " MB_MAG ", 1\n"                 \
@@ -1437,14 +1444,17 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       // escape (magnitude 5 + getEscapeValue, mv.go:124-128) or end of block
       "L%=_tescob:\n"
       "s_cmp_eq_u32 %[cnt], 7\n"
-      "s_cbranch_scc1 L%=_tend\n"
+      "s_cbranch_scc1 L%=_teob\n"
       MB_CLASS("%[vm2]", "0", "-1", "0x100000", "es")
       "s_add_u32 " MB_MAG ", %[sym], 5\n"
       "s_mov_b32 %[prev], 16\n"
       "s_branch L%=_tlev0\n"
       // end of block: the first level's sign, hidden in the level-sum parity
       // when last - first >= 4, else explicit
+      "L%=_teob:\n"
+      "s_add_u32 %[nsym], %[nsym], 1\n"
       "L%=_tend:\n"
+      "s_add_u32 %[nsym], %[nsym], %[si]\n"
       "s_cmp_eq_u32 %[firstsi], 0\n"
       "s_cbranch_scc1 L%=_tbdone\n"
       "s_lshr_b32 " MB_MAG ", %[firstpm], 8\n"
@@ -1867,14 +1877,17 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       // escape (magnitude 5 + getEscapeValue, mv.go:124-128) or end of block
       "L%=_tescob:\n"
       "s_cmp_eq_u32 %[cnt], 7\n"
-      "s_cbranch_scc1 L%=_tend\n"
+      "s_cbranch_scc1 L%=_teob\n"
       MB_CLASS("%[vm2]", "0", "-1", "0x100000", "es")
       "s_add_u32 " MB_MAG ", %[sym], 5\n"
       "s_mov_b32 %[prev], 16\n"
       "s_branch L%=_tlev0\n"
       // end of block: the first level's sign, hidden in the level-sum parity
       // when last - first >= 4, else explicit
+      "L%=_teob:\n"
+      "s_add_u32 %[nsym], %[nsym], 1\n"
       "L%=_tend:\n"
+      "s_add_u32 %[nsym], %[nsym], %[si]\n"
       "s_cmp_eq_u32 %[firstsi], 0\n"
       "s_cbranch_scc1 L%=_tbdone\n"
       "s_lshr_b32 " MB_MAG ", %[firstpm], 8\n"
