@@ -85,6 +85,8 @@ struct alignas(16) MBRec {
   // bit 17 Cr.  A fully decoded macroblock has every bit it uses set; only the
   // macroblock on which a tile's stream failed carries a partial mask.
   uint32_t wmask;
+  // Coefficient entries: scan index (bits 0-7) | block << 8 | level << 13,
+  // the block 0-15 luma in decode order, 16 Cb, 17 Cr.
   uint32_t coef_off;    // first coefficient entry of this macroblock
   uint32_t coef_count;  // number of coefficient entries
   int16_t mv[4][2];     // partition vectors (quarter-pel), skip/whole use mv[0]

@@ -101,12 +101,6 @@ struct ByteWin {
   }
 };
 
-// Zigzag scan position si of an n x n block from per-size register tables:
-// lane l of zzv holds positions 4l..4l+3 (scan.go:17-35).
-__device__ __forceinline__ int zz_pos(uint32_t zzv, int si) {
-  return static_cast<int>((rd_lane(zzv, si >> 2) >> ((si & 3) * 8)) & 0xFFu);
-}
-
 __device__ __forceinline__ int zigzag_dec(uint32_t u) { return static_cast<int>(u >> 1) ^ -static_cast<int>(u & 1u); }
 __device__ __forceinline__ int16_t clamp_mv(int v) { return static_cast<int16_t>(v < -kMVClamp ? -kMVClamp : (v > kMVClamp ? kMVClamp : v)); }
 __device__ __forceinline__ int dtoken_ctx(int plane, int tx, int band, int prev) {
@@ -391,11 +385,11 @@ __device__ __forceinline__ bool parse_block(Rans& R, CoefOut& out, const Tables&
       if (first < 0) {
         first = si;
         first_mag = mag;
-        first_pos = zz_pos(zzv, si);
+        first_pos = si;
       } else {
         const int s = R.decode<kGSign, 16>(T.misc0, lane);
         if (s < 0) return false;
-        out.emit(coef_entry(zz_pos(zzv, si), blk, s ? -mag : mag), lane);
+        out.emit(coef_entry(si, blk, s ? -mag : mag), lane);
       }
       last = si;
     }
@@ -563,15 +557,9 @@ __device__ __forceinline__ uint32_t token_loop(TokState& S, uint32_t resume, uin
       "s_mov_b32 %[x2], %[x3]\n"
       "s_mov_b32 %[x3], %[xn]\n"
       "s_add_u32 %[nsym], %[nsym], 1\n"
-      // coefficient entry: zigzag position | block << 8 | level << 13
-      "s_lshr_b32 %[t1], %[si], 2\n"
-      "v_readlane_b32 %[t2], %[zzv], %[t1]\n"
-      "s_and_b32 %[t1], %[si], 3\n"
-      "s_lshl_b32 %[t1], %[t1], 3\n"
-      "s_lshr_b32 %[t2], %[t2], %[t1]\n"
-      "s_and_b32 %[t2], %[t2], 0xff\n"
+      // coefficient entry: scan index | block << 8 | level << 13
       "s_lshl_b32 %[t1], %[mag], 13\n"
-      "s_or_b32 %[t1], %[t1], %[t2]\n"
+      "s_or_b32 %[t1], %[t1], %[si]\n"
       "s_or_b32 %[t1], %[t1], %[blk8]\n"
       "s_mov_b32 m0, %[cn]\n"
       "v_writelane_b32 %[vbuf], %[t1], m0\n"
@@ -587,14 +575,8 @@ __device__ __forceinline__ uint32_t token_loop(TokState& S, uint32_t resume, uin
 
       // first nonzero of the block: its sign resolves at the end
       "L%=_first:\n"
-      "s_lshr_b32 %[t1], %[si], 2\n"
-      "v_readlane_b32 %[t2], %[zzv], %[t1]\n"
-      "s_and_b32 %[t1], %[si], 3\n"
-      "s_lshl_b32 %[t1], %[t1], 3\n"
-      "s_lshr_b32 %[t2], %[t2], %[t1]\n"
-      "s_and_b32 %[t2], %[t2], 0xff\n"
       "s_lshl_b32 %[t1], %[mag], 8\n"
-      "s_or_b32 %[firstpm], %[t1], %[t2]\n"
+      "s_or_b32 %[firstpm], %[t1], %[si]\n"
       "s_add_u32 %[firstsi], %[si], 1\n"
       "s_mov_b32 %[last], %[si]\n"
       "s_add_u32 %[si], %[si], 1\n"
@@ -984,20 +966,12 @@ struct MbLoop {
   "s_add_u32 %[xn], %[xn], %[t2]\n"              \
   MB_RN(tag)
 
-// Zigzag scan position of si into t2.
-#define MB_ZZ                                    \
-  "s_lshr_b32 %[t1], %[si], 2\n"                 \
-  "v_readlane_b32 %[t2], %[vzz], %[t1]\n"        \
-  "s_and_b32 %[t1], %[si], 3\n"                  \
-  "s_lshl_b32 %[t1], %[t1], 3\n"                 \
-  "s_lshr_b32 %[t2], %[t2], %[t1]\n"             \
-  "s_and_b32 %[t2], %[t2], 0xff\n"
-
-// Coefficient entry (position t2, level mag) into the staging register;
+// Coefficient entry (scan index POS, level mag) into the staging register;
 // 64 staged entries go out in one store (out of line: MB_EMIT_OOL).
-#define MB_EMIT(tag)                             \
+#define MB_EMIT(tag) MB_EMIT_AT(tag, "%[t2]")
+#define MB_EMIT_AT(tag, POS)                     \
   "s_lshl_b32 %[t1], " MB_MAG ", 13\n"           \
-  "s_or_b32 %[t1], %[t1], %[t2]\n"               \
+  "s_or_b32 %[t1], %[t1], " POS "\n"             \
   "s_or_b32 %[t1], %[t1], %[blk8]\n"             \
   "s_mov_b32 m0, %[cn]\n"                        \
   "v_writelane_b32 %[vbuf], %[t1], m0\n"         \
@@ -1124,17 +1098,15 @@ struct MbLoop {
   "s_cmp_eq_u32 %[firstsi], 0\n"                 \
   "s_cbranch_scc1 L%=_tfirst" NP1 "\n"           \
   MBP_SIGN("ts" P, RB)                           \
-  MB_ZZ                                          \
-  MB_EMIT("e1" P)                                \
+  MB_EMIT_AT("e1" P, "%[si]")                    \
   "s_mov_b32 %[last], %[si]\n"                   \
   "s_add_u32 %[si], %[si], 1\n"                  \
   "s_cmp_lg_u32 %[si], %[stop]\n"                \
   "s_cbranch_scc1 L%=_tok" NP2 "\n"              \
   "s_branch L%=_tstop" NP2 "\n"                  \
   "L%=_tfirst" NP1 ":\n"                         \
-  MB_ZZ                                          \
   "s_lshl_b32 %[t1], " MB_MAG ", 8\n"            \
-  "s_or_b32 %[firstpm], %[t1], %[t2]\n"          \
+  "s_or_b32 %[firstpm], %[t1], %[si]\n"          \
   "s_add_u32 %[firstsi], %[si], 1\n"             \
   "s_mov_b32 %[last], %[si]\n"                   \
   "s_add_u32 %[si], %[si], 1\n"                  \
@@ -1249,7 +1221,7 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
   uint32_t ev, slot, t1, t2, t3, p, xn, cnt, sym, val, k2;
   uint32_t k, nb, total, blk8, cbpf, si, stop, boff, prev, sumabs, firstsi, firstpm, last, m0s;
   uint64_t b;
-  uint32_t vcur, va, vb, vc, vzz, vaddr, vt;
+  uint32_t vcur, va, vb, vc, vaddr, vt;
   if constexpr (!kWide) {
 #define MB_ROW_RD MB_ROW_RD_VGPR
 #define MB_ROW_WR MB_ROW_WR_VGPR
@@ -1356,7 +1328,6 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       "v_mov_b32 %[va], %[t00]\n"
       "v_mov_b32 %[vb], %[t01]\n"
       "v_mov_b32 %[vc], %[t02]\n"
-      "v_mov_b32 %[vzz], %[zz4]\n"
       "s_branch L%=_blocks\n"
       "L%=_tx1:\n"
       "s_cmp_eq_u32 %[sym], 1\n"
@@ -1366,7 +1337,6 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       "v_mov_b32 %[va], %[t10]\n"
       "v_mov_b32 %[vb], %[t11]\n"
       "v_mov_b32 %[vc], %[t12]\n"
-      "v_mov_b32 %[vzz], %[zz8]\n"
       "s_branch L%=_blocks\n"
       "L%=_tx2:\n"
       "s_mov_b32 %[nb], 1\n"
@@ -1374,7 +1344,6 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       "v_mov_b32 %[va], %[t20]\n"
       "v_mov_b32 %[vb], %[t21]\n"
       "v_mov_b32 %[vc], %[t22]\n"
-      "v_mov_b32 %[vzz], %[zz16]\n"
 
       // ---- blocks: luma 0..nb-1, then Cb, Cr (when coded)
       "L%=_blocks:\n"
@@ -1429,7 +1398,6 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       "v_mov_b32 %[va], %[t30]\n"
       "v_mov_b32 %[vb], %[t31]\n"
       "v_mov_b32 %[vc], %[t32]\n"
-      "v_mov_b32 %[vzz], %[zz8]\n"
 
       // ------------------------------------------------------ token block
       "L%=_tb:\n"
@@ -1669,12 +1637,12 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
         [nb] "=&s"(nb), [total] "=&s"(total), [blk8] "=&s"(blk8), [cbpf] "=&s"(cbpf), [si] "=&s"(si),
         [stop] "=&s"(stop), [boff] "=&s"(boff), [prev] "=&s"(prev), [sumabs] "=&s"(sumabs),
         [firstsi] "=&s"(firstsi), [firstpm] "=&s"(firstpm), [last] "=&s"(last), [m0s] "=&s"(m0s),
-        [b] "=&s"(b), [vcur] "=&v"(vcur), [va] "=&v"(va), [vb] "=&v"(vb), [vc] "=&v"(vc), [vzz] "=&v"(vzz),
+        [b] "=&s"(b), [vcur] "=&v"(vcur), [va] "=&v"(va), [vb] "=&v"(vb), [vc] "=&v"(vc),
         [vaddr] "=&v"(vaddr), [vt] "=&v"(vt)
       : [vm0] "v"(T.misc0), [vm1] "v"(T.misc1), [vm2] "v"(T.misc2), [t00] "v"(T.t00), [t01] "v"(T.t01),
         [t02] "v"(T.t02), [t10] "v"(T.t10), [t11] "v"(T.t11), [t12] "v"(T.t12), [t20] "v"(T.t20),
         [t21] "v"(T.t21), [t22] "v"(T.t22), [t30] "v"(T.t30), [t31] "v"(T.t31), [t32] "v"(T.t32),
-        [zz4] "v"(zz4), [zz8] "v"(zz8), [zz16] "v"(zz16), [vlane4] "v"(vlane4), [x0t] "s"(sgpr(x0t)),
+        [vlane4] "v"(vlane4), [x0t] "s"(sgpr(x0t)),
         [x1t] "s"(sgpr(x1t)), [y0t] "s"(sgpr(y0t)), [y1t] "s"(sgpr(y1t)), [mbc] "s"(sgpr(mbc)),
         [interf] "s"(sgpr(interf)), [tcb] "s"(sgpr(tcb)), [f0] "s"(sgpr(f0)), [f1] "s"(sgpr(f1)),
         [hi2] "s"(sgpr(hi2)), [wlo] "s"(sgpr(wlo)), [tb] "s"(sgpr(tb)), [dst] "s"(sgpr(dst)),
@@ -1789,7 +1757,6 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       "v_mov_b32 %[va], %[t00]\n"
       "v_mov_b32 %[vb], %[t01]\n"
       "v_mov_b32 %[vc], %[t02]\n"
-      "v_mov_b32 %[vzz], %[zz4]\n"
       "s_branch L%=_blocks\n"
       "L%=_tx1:\n"
       "s_cmp_eq_u32 %[sym], 1\n"
@@ -1799,7 +1766,6 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       "v_mov_b32 %[va], %[t10]\n"
       "v_mov_b32 %[vb], %[t11]\n"
       "v_mov_b32 %[vc], %[t12]\n"
-      "v_mov_b32 %[vzz], %[zz8]\n"
       "s_branch L%=_blocks\n"
       "L%=_tx2:\n"
       "s_mov_b32 %[nb], 1\n"
@@ -1807,7 +1773,6 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       "v_mov_b32 %[va], %[t20]\n"
       "v_mov_b32 %[vb], %[t21]\n"
       "v_mov_b32 %[vc], %[t22]\n"
-      "v_mov_b32 %[vzz], %[zz16]\n"
 
       // ---- blocks: luma 0..nb-1, then Cb, Cr (when coded)
       "L%=_blocks:\n"
@@ -1862,7 +1827,6 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       "v_mov_b32 %[va], %[t30]\n"
       "v_mov_b32 %[vb], %[t31]\n"
       "v_mov_b32 %[vc], %[t32]\n"
-      "v_mov_b32 %[vzz], %[zz8]\n"
 
       // ------------------------------------------------------ token block
       "L%=_tb:\n"
@@ -2102,12 +2066,12 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
         [nb] "=&s"(nb), [total] "=&s"(total), [blk8] "=&s"(blk8), [cbpf] "=&s"(cbpf), [si] "=&s"(si),
         [stop] "=&s"(stop), [boff] "=&s"(boff), [prev] "=&s"(prev), [sumabs] "=&s"(sumabs),
         [firstsi] "=&s"(firstsi), [firstpm] "=&s"(firstpm), [last] "=&s"(last), [m0s] "=&s"(m0s),
-        [b] "=&s"(b), [vcur] "=&v"(vcur), [va] "=&v"(va), [vb] "=&v"(vb), [vc] "=&v"(vc), [vzz] "=&v"(vzz),
+        [b] "=&s"(b), [vcur] "=&v"(vcur), [va] "=&v"(va), [vb] "=&v"(vb), [vc] "=&v"(vc),
         [vaddr] "=&v"(vaddr), [vt] "=&v"(vt)
       : [vm0] "v"(T.misc0), [vm1] "v"(T.misc1), [vm2] "v"(T.misc2), [t00] "v"(T.t00), [t01] "v"(T.t01),
         [t02] "v"(T.t02), [t10] "v"(T.t10), [t11] "v"(T.t11), [t12] "v"(T.t12), [t20] "v"(T.t20),
         [t21] "v"(T.t21), [t22] "v"(T.t22), [t30] "v"(T.t30), [t31] "v"(T.t31), [t32] "v"(T.t32),
-        [zz4] "v"(zz4), [zz8] "v"(zz8), [zz16] "v"(zz16), [vlane4] "v"(vlane4), [x0t] "s"(sgpr(x0t)),
+        [vlane4] "v"(vlane4), [x0t] "s"(sgpr(x0t)),
         [x1t] "s"(sgpr(x1t)), [y0t] "s"(sgpr(y0t)), [y1t] "s"(sgpr(y1t)), [mbc] "s"(sgpr(mbc)),
         [interf] "s"(sgpr(interf)), [tcb] "s"(sgpr(tcb)), [f0] "s"(sgpr(f0)), [f1] "s"(sgpr(f1)),
         [hi2] "s"(sgpr(hi2)), [wlo] "s"(sgpr(wlo)), [tb] "s"(sgpr(tb)), [dst] "s"(sgpr(dst)),
@@ -2251,19 +2215,13 @@ __device__ __forceinline__ void parse_tile(const uint8_t* __restrict__ pkt, cons
     }
     R.x0 = sv[0]; R.x1 = sv[1]; R.x2 = sv[2]; R.x3 = sv[3];
   }
-  // Scan tables in registers: lane l holds positions 4l..4l+3 of each size.
-  uint32_t zz4 = 0, zz8 = 0, zz16 = 0;
-  {
-    const uint32_t* z = reinterpret_cast<const uint32_t*>(c_zz);
-    if (lane < 4) zz4 = z[lane];
-    if (lane < 16) zz8 = z[64 + lane];
-    zz16 = z[128 + lane];
-  }
+  // The loop emits scan indices (reconstruction maps them to positions).
+  const uint32_t zz4 = 0, zz8 = 0, zz16 = 0;
   // Let every start-up load land here (an empty asm that takes the loaded
   // registers), so no load into a register the macroblock loop uses is
   // outstanding there: a wait in the loop would also wait for every store
   // issued before it.
-  asm volatile("" : "+v"(T.misc0), "+v"(T.misc1), "+v"(T.misc2), "+v"(zz4), "+v"(zz8), "+v"(zz16));
+  asm volatile("" : "+v"(T.misc0), "+v"(T.misc1), "+v"(T.misc2));
   asm volatile("" : "+v"(T.t00), "+v"(T.t01), "+v"(T.t02), "+v"(T.t10), "+v"(T.t11), "+v"(T.t12));
   asm volatile("" : "+v"(T.t20), "+v"(T.t21), "+v"(T.t22), "+v"(T.t30), "+v"(T.t31), "+v"(T.t32));
 
@@ -2410,19 +2368,13 @@ __device__ __forceinline__ void parse_tile_asm(const uint8_t* __restrict__ pkt, 
     }
     R.x0 = sv[0]; R.x1 = sv[1]; R.x2 = sv[2]; R.x3 = sv[3];
   }
-  // Scan tables in registers: lane l holds positions 4l..4l+3 of each size.
-  uint32_t zz4 = 0, zz8 = 0, zz16 = 0;
-  {
-    const uint32_t* z = reinterpret_cast<const uint32_t*>(c_zz);
-    if (lane < 4) zz4 = z[lane];
-    if (lane < 16) zz8 = z[64 + lane];
-    zz16 = z[128 + lane];
-  }
+  // The loop emits scan indices (reconstruction maps them to positions).
+  const uint32_t zz4 = 0, zz8 = 0, zz16 = 0;
   // Let every start-up load land here (an empty asm that takes the loaded
   // registers), so no load into a register the macroblock loop uses is
   // outstanding there: a wait in the loop would also wait for every store
   // issued before it.
-  asm volatile("" : "+v"(T.misc0), "+v"(T.misc1), "+v"(T.misc2), "+v"(zz4), "+v"(zz8), "+v"(zz16));
+  asm volatile("" : "+v"(T.misc0), "+v"(T.misc1), "+v"(T.misc2));
   asm volatile("" : "+v"(T.t00), "+v"(T.t01), "+v"(T.t02), "+v"(T.t10), "+v"(T.t11), "+v"(T.t12));
   asm volatile("" : "+v"(T.t20), "+v"(T.t21), "+v"(T.t22), "+v"(T.t30), "+v"(T.t31), "+v"(T.t32));
 
@@ -2521,19 +2473,22 @@ __device__ uint64_t load_coefs(WaveScratch& ws, const uint32_t* __restrict__ coe
   wave_sync();
   uint32_t nzm = 0, acm = 0;
   uint32_t cnt = r.coef_count < 384u ? r.coef_count : 384u;
+  const int zl = n == 4 ? 0 : (n == 8 ? 1 : 2);
   for (uint32_t i = lane; i < cnt; i += 64) {
     uint32_t e = coefs[r.coef_off + i];
-    int pos = e & 0xFF, blk = (e >> 8) & 31;
+    int si = e & 0xFF, blk = (e >> 8) & 31;
     int32_t level = static_cast<int32_t>(e) >> 13;
-    int32_t v = level * (pos == 0 ? qdc : qac);
-    int idx;
+    int32_t v = level * (si == 0 ? qdc : qac);  // scan index 0 is the DC position
+    int idx, pos;
     if (blk < 16) {
       int nn = n * n;
-      if (pos >= nn) pos = nn - 1;
+      if (si >= nn) si = nn - 1;
+      pos = c_zz[zl][si];
       idx = blk * nn + pos;
       if (idx >= 256) continue;
     } else if (blk < 18) {
-      pos &= 63;
+      si &= 63;
+      pos = c_zz[1][si];
       idx = 256 + (blk - 16) * 64 + pos;
     } else {
       continue;

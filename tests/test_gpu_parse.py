@@ -9,6 +9,15 @@ from helpers import clip_kw
 pytestmark = pytest.mark.gpu
 
 PART_COUNT = [1, 2, 2, 4]
+ZIGZAG = {n: O.zigzag(n) for n in (4, 8, 16)}
+
+
+def to_position(e, tx):
+    """A GPU coefficient entry with its scan index replaced by the raster
+    position (oracle zigzag of the block's size; chroma blocks are 8x8)."""
+    si, blk = e & 255, (e >> 8) & 31
+    n = 8 if blk >= 16 else 4 << tx
+    return (e & ~255) | ZIGZAG[n][si]
 
 
 def compare_frame(gpu, ref, label=""):
@@ -45,7 +54,8 @@ def compare_frame(gpu, ref, label=""):
                 if r["mv"][0].tolist() != t["mv"][:2]:
                     errs.append(f"skip mv {r['mv'][0].tolist()} != {t['mv'][:2]}")
             off, cnt = int(r["coef_off"]), int(r["coef_count"])
-            g = sorted(coefs[off:off + cnt].tolist())
+            # the GPU stores scan indices, the oracle's trace raster positions
+            g = sorted(to_position(e, int(r["tx"])) for e in coefs[off:off + cnt].tolist())
             if g != t["coef"]:
                 gs, ts = set(g), set(t["coef"])
                 dec = lambda e: (e & 255, (e >> 8) & 31, np.int32(np.uint32(e)) >> 13)
