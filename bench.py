@@ -359,7 +359,12 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--sessions", type=int, default=64)
-    ap.add_argument("--window", type=int, default=8)
+    # frames per session per engine step in the batched sections: a wider
+    # window puts more independent tile chains into each entropy launch and
+    # spreads the per-step costs over more frames (8 / 16 / 32 on the 64 x 1080p
+    # clip: 37.6k / 38.4k / 40.2k frames/s; on the mixed batch 27.5k / 35.4k /
+    # 41.0k), at one step's latency per frame
+    ap.add_argument("--window", type=int, default=32)
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
@@ -465,6 +470,8 @@ def main():
         log(f"bench: batched {bfps:.1f} frames/s")
         out["batched"] = batched_summary(bt, args.steps, vs_plain=round(bfps / fps, 2),
                                          sessions=args.sessions, window=args.window)
+        # measured at window 8; an inter launch holds one frame position of
+        # every session (64 frames) at any window, so the figure per launch holds
         out["batched"]["recon"]["ebiv_recon_inter"]["traffic"] = measured_traffic(
             "batched_64x1080p_parallax_w8", "ebiv_recon_inter")
     for d in dev:
