@@ -10,9 +10,15 @@ schedules on the caller's own stream and reports the fastest.  Every
 candidate's decoded frames are compared with the first candidate's (frame
 CRCs per packet index): schedules must not change results.
 
-    python -m govid_amd.tune STREAM.ebiv [--sessions 64] [--window 4] [--passes 2]
+The window (frames per session per engine step) is searched first when
+several are given: a wider one puts more independent tile chains into each
+entropy launch and spreads each step's fixed costs over more frames, at one
+step's latency per frame and (ring + window) frame buffers per session.  It
+is picked by wall time per frame, the schedule then by kernel time.
 
-prints one JSON object: the schedules tried with their kernel time per
+    python -m govid_amd.tune STREAM.ebiv [--sessions 64] [--window 4 | --windows 8,16,32] [--passes 2]
+
+prints one JSON object: the windows and schedules tried with their time per
 frame, and the fastest.
 """
 from __future__ import annotations
@@ -83,10 +89,25 @@ def measure(packets, dev, schedule: dict, sessions: int, window: int, passes: in
             "frames": frames}, crcs
 
 
-def tune(source, sessions: int = 64, window: int = 4, passes: int = 2, exhaustive: bool = False) -> dict:
-    """`source`: a container path or a list of packets."""
+def tune(source, sessions: int = 64, window: int = 4, passes: int = 2, exhaustive: bool = False,
+         windows=None) -> dict:
+    """`source`: a container path or a list of packets; `windows`: candidate
+    windows (searched first, under the engine's default schedule)."""
     packets = Reader.open(source).packets() if isinstance(source, str) else list(source)
     dev = [DeviceBuffer(p) for p in packets]
+    window_results, ref = [], None
+    if windows:
+        base = Batch()
+        d0 = base.schedule()
+        base.close()
+        for w in windows:
+            r, crcs = measure(packets, dev, d0, sessions, w, passes, check=True)
+            if ref is None:
+                ref = crcs
+            elif crcs != ref:
+                raise RuntimeError(f"window {w} changed decoded frames")
+            window_results.append(dict(window=w, **r))
+        window = min(window_results, key=lambda r: r["wall_ms_per_frame"])["window"]
     if exhaustive:
         cands = [dict(intra_waves_key=a, intra_waves_few=b, intra_waves_many=c, many_tiles=d)
                  for a, b, c, d in itertools.product(KEY_WAVES, FEW_WAVES, MANY_WAVES, MANY_TILES)]
@@ -101,7 +122,7 @@ def tune(source, sessions: int = 64, window: int = 4, passes: int = 2, exhaustiv
                 c = dict(d0, **{field: v})
                 if c not in cands:
                     cands.append(c)
-    results, ref = [], None
+    results = []
     for c in cands:
         r, crcs = measure(packets, dev, c, sessions, window, passes, check=True)
         if ref is None:
@@ -113,7 +134,8 @@ def tune(source, sessions: int = 64, window: int = 4, passes: int = 2, exhaustiv
         d.close()
     best = min(results, key=lambda r: r["kernel_ms_per_frame"])
     name = source if isinstance(source, str) else f"{len(packets)} packets"
-    return {"stream": name, "sessions": sessions, "window": window, "tried": results, "best": best["schedule"]}
+    return {"stream": name, "sessions": sessions, "window": window, "windows_tried": window_results,
+            "tried": results, "best": best["schedule"]}
 
 
 def main(argv=None):
@@ -121,10 +143,12 @@ def main(argv=None):
     ap.add_argument("stream")
     ap.add_argument("--sessions", type=int, default=64)
     ap.add_argument("--window", type=int, default=4)
+    ap.add_argument("--windows", default="", help="comma-separated candidate windows (overrides --window)")
     ap.add_argument("--passes", type=int, default=2)
     ap.add_argument("--exhaustive", action="store_true", help="every combination instead of one field at a time")
     a = ap.parse_args(argv)
-    print(json.dumps(tune(a.stream, a.sessions, a.window, a.passes, a.exhaustive)))
+    wins = [int(w) for w in a.windows.split(",") if w] or None
+    print(json.dumps(tune(a.stream, a.sessions, a.window, a.passes, a.exhaustive, windows=wins)))
 
 
 if __name__ == "__main__":

@@ -115,6 +115,19 @@ def test_tuner_checks_every_schedule_and_picks_one():
     assert all(t["frames"] == 3 * len(pk) for t in res["tried"])
 
 
+def test_tuner_window_search_keeps_results():
+    """Every candidate window decodes the same frames (the tuner compares the
+    CRCs of every frame) and the chosen one is among them."""
+    from govid_amd import tune
+
+    pk = clip_kw("rects", 96, 64, 7, qp=22, gop=3, tile_cols=2, tile_rows=2).packets
+    print("tuner windows 1, 2, 5", flush=True)
+    res = tune.tune(pk, sessions=3, passes=1, windows=(1, 2, 5))
+    assert [w["window"] for w in res["windows_tried"]] == [1, 2, 5]
+    assert res["window"] in (1, 2, 5)
+    assert all(w["frames"] == 3 * len(pk) for w in res["windows_tried"])
+
+
 def test_bulk_submission_matches_oracle():
     streams = [
         (_clip("rects", 96, 64, 10, qp=22, gop=5, tile_cols=2, tile_rows=2), 3, 3),
