@@ -364,8 +364,9 @@ class Batch:
         return {f: getattr(t, f) for f, _ in capi.KernelTimes._fields_}
 
     def set_schedule(self, **kw) -> None:
-        """Launch shapes of the intra reconstruction (fields of capi.Schedule;
-        omitted ones keep their value).  Results do not depend on them."""
+        """Launch shapes of the reconstruction (fields of capi.Schedule:
+        intra wavefronts per tile, recon_lanes streams; omitted ones keep
+        their value).  Results do not depend on them."""
         s = capi.Schedule(**{k: int(v) for k, v in kw.items()})
         if self._L.ebiv_batch_set_schedule(self._h, C.byref(s)):
             raise ValueError("bad schedule")

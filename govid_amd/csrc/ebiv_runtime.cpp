@@ -597,6 +597,7 @@ struct Schedule {
   int intra_waves_few = 16;   // wavefronts per inter-frame tile, launches of few tiles
   int intra_waves_many = 4;   // ... launches of at least many_tiles tiles
   int many_tiles = 256;
+  int recon_lanes = 1;        // HIP streams the sessions' reconstruction is split over
   int inter_intra_waves(size_t ntiles) const {
     return ntiles >= static_cast<size_t>(many_tiles) ? intra_waves_many : intra_waves_few;
   }
@@ -691,6 +692,11 @@ struct Engine {
   DevArray<uint32_t> d_rgba;
   std::string msg;
   bool timing = false;
+  // Reconstruction lanes beyond `stream` (default priority, created on first
+  // use) and the events that fork them from the entropy launch and join them.
+  static constexpr int kMaxLanes = 4;
+  hipStream_t lanes[kMaxLanes - 1] = {};
+  hipEvent_t lane_fork = nullptr, lane_join[kMaxLanes - 1] = {};
   std::vector<hipEvent_t> evs;
   std::vector<int> marks;  // kind per event pair
   std::vector<uint64_t> mark_mbs;
@@ -701,8 +707,14 @@ struct Engine {
   ~Engine() {
     pool.reset();
     if (stream) (void)hipStreamSynchronize(stream);
+    for (auto l : lanes) if (l) (void)hipStreamSynchronize(l);
     sessions.clear();
     for (auto e : evs) (void)hipEventDestroy(e);
+    for (int i = 0; i < kMaxLanes - 1; i++) {
+      if (lane_join[i]) (void)hipEventDestroy(lane_join[i]);
+      if (lanes[i]) (void)hipStreamDestroy(lanes[i]);
+    }
+    if (lane_fork) (void)hipEventDestroy(lane_fork);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -719,21 +731,33 @@ struct Engine {
   // Brackets a launch of kind k (0 parse, 1 inter, 2 intra, 3 rgba) with
   // events while timing is on; returns the pair index or -1.  `mbs`: the
   // macroblocks an inter launch covers.
-  int mark_begin(int kind, uint64_t mbs = 0) {
+  // Streams and events of `n` reconstruction lanes (lane 0 is `stream`).
+  bool ensure_lanes(int n) {
+    if (!lane_fork && hipEventCreateWithFlags(&lane_fork, hipEventDisableTiming) != hipSuccess) return false;
+    for (int i = 0; i + 1 < n; i++) {
+      if (!lanes[i] && hipStreamCreateWithFlags(&lanes[i], hipStreamNonBlocking) != hipSuccess) return false;
+      if (!lane_join[i] && hipEventCreateWithFlags(&lane_join[i], hipEventDisableTiming) != hipSuccess) return false;
+    }
+    return true;
+  }
+  hipStream_t lane_stream(int g) const { return g == 0 ? stream : lanes[g - 1]; }
+
+  int mark_begin(int kind, uint64_t mbs = 0, hipStream_t st = nullptr) {
     if (!timing) return -1;
+    if (!st) st = stream;
     const size_t i = marks.size();
     while (evs.size() < 2 * i + 2) {
       hipEvent_t e;
       if (hipEventCreate(&e) != hipSuccess) return -1;
       evs.push_back(e);
     }
-    if (hipEventRecord(evs[2 * i], stream) != hipSuccess) return -1;
+    if (hipEventRecord(evs[2 * i], st) != hipSuccess) return -1;
     marks.push_back(kind);
     mark_mbs.push_back(mbs);
     return static_cast<int>(i);
   }
-  void mark_end(int m) {
-    if (m >= 0) (void)hipEventRecord(evs[2 * static_cast<size_t>(m) + 1], stream);
+  void mark_end(int m, hipStream_t st = nullptr) {
+    if (m >= 0) (void)hipEventRecord(evs[2 * static_cast<size_t>(m) + 1], st ? st : stream);
   }
   void collect_marks() {
     for (size_t i = 0; i < marks.size(); i++) {
@@ -901,25 +925,39 @@ int Engine::run_step(std::vector<Session*>& ss, std::vector<std::vector<StepFram
       h_items.h[w++] = WorkItem{static_cast<uint32_t>(j), static_cast<uint32_t>(t) | prio << 30};
     }
   }
-  // Per round: inter macroblock groups, then the intra tiles of key-frame
-  // payloads (their own launch: many wavefronts per tile and the LDS
-  // frontier, for latency), then those of inter payloads (few wavefronts per
-  // tile, for throughput).
-  std::vector<size_t> inter_off(rounds.size()), intra_off(rounds.size()), key_end(rounds.size()),
-      round_end(rounds.size());
-  std::vector<uint32_t> inter_groups_of(rounds.size());
+  // Per round and reconstruction lane: inter macroblock groups, then the
+  // intra tiles of key-frame payloads (their own launch: many wavefronts per
+  // tile and the LDS frontier, for latency), then those of inter payloads
+  // (few wavefronts per tile, for throughput).  Sessions are dealt to lanes
+  // round robin; a lane's launches run in order on its own stream, and the
+  // lanes overlap (one lane's latency-bound intra launches beside another's
+  // inter launches).  Sessions share nothing between the entropy launch and
+  // the end of the step, so the split leaves every result unchanged.
+  const int nl = std::max(1, std::min({sched.recon_lanes, static_cast<int>(kMaxLanes), static_cast<int>(ss.size())}));
+  if (nl > 1 && !ensure_lanes(nl)) return fail(EBIV_ERR_DEVICE, "stream creation failed");
+  const size_t nrl = rounds.size() * static_cast<size_t>(nl);
+  std::vector<size_t> inter_off(nrl), intra_off(nrl), key_end(nrl), round_end(nrl);
+  std::vector<uint32_t> inter_groups_of(nrl);
+  std::vector<uint64_t> inter_mbs_of(nrl);
   auto is_key = [&](size_t j) { return !payload_of(ss, frames, jrefs[j])->inter; };
-  for (size_t r = 0; r < rounds.size(); r++) {
-    inter_off[r] = w;
-    inter_prefix = 0;
-    for (size_t j : rounds[r]) add_recon_items(j, true, w);
-    inter_groups_of[r] = inter_prefix;
-    intra_off[r] = w;
-    for (size_t j : rounds[r]) if (is_key(j)) add_recon_items(j, false, w);
-    key_end[r] = w;
-    for (size_t j : rounds[r]) if (!is_key(j)) add_recon_items(j, false, w);
-    round_end[r] = w;
-  }
+  auto lane_of = [&](size_t j) { return static_cast<int>(jrefs[j].si % static_cast<size_t>(nl)); };
+  for (size_t r = 0; r < rounds.size(); r++)
+    for (int g = 0; g < nl; g++) {
+      const size_t q = r * nl + g;
+      inter_off[q] = w;
+      inter_prefix = 0;
+      for (size_t j : rounds[r])
+        if (lane_of(j) == g) {
+          add_recon_items(j, true, w);
+          if (!is_key(j)) inter_mbs_of[q] += static_cast<uint64_t>(ss[jrefs[j].si]->mbc) * ss[jrefs[j].si]->mbr;
+        }
+      inter_groups_of[q] = inter_prefix;
+      intra_off[q] = w;
+      for (size_t j : rounds[r]) if (lane_of(j) == g && is_key(j)) add_recon_items(j, false, w);
+      key_end[q] = w;
+      for (size_t j : rounds[r]) if (lane_of(j) == g && !is_key(j)) add_recon_items(j, false, w);
+      round_end[q] = w;
+    }
   if (njobs &&
       (hipMemcpyAsync(d_jobs.d, h_jobs.h, sizeof(FrameJob) * njobs, hipMemcpyHostToDevice, stream) != hipSuccess ||
        hipMemcpyAsync(d_items.d, h_items.h, sizeof(WorkItem) * w, hipMemcpyHostToDevice, stream) != hipSuccess))
@@ -947,38 +985,48 @@ int Engine::run_step(std::vector<Session*>& ss, std::vector<std::vector<StepFram
   }
   // A golden override decodes over a copy of the golden it replaces, so a
   // failing override leaves Go's partial rewrite (codec.go:178-181).
-  auto golden_copy = [&](const JobRef& jr) -> bool {
+  auto golden_copy = [&](const JobRef& jr, hipStream_t st) -> bool {
     const Session& s = *ss[jr.si];
     const StepFrame& f = frames[jr.si][jr.fi];
     const DevBuf& src = s.pool[f.gold_in];
-    return hipMemcpyAsync(s.pool[f.gtarget].base, src.base, src.bytes, hipMemcpyDeviceToDevice, stream) == hipSuccess;
+    return hipMemcpyAsync(s.pool[f.gtarget].base, src.base, src.bytes, hipMemcpyDeviceToDevice, st) == hipSuccess;
   };
+  if (nl > 1) {
+    if (hipEventRecord(lane_fork, stream) != hipSuccess) return fail(EBIV_ERR_DEVICE, "event record failed");
+    for (int g = 1; g < nl; g++)
+      if (hipStreamWaitEvent(lanes[g - 1], lane_fork, 0) != hipSuccess) return fail(EBIV_ERR_DEVICE, "stream wait failed");
+  }
   for (size_t r = 0; r < rounds.size(); r++) {
-    for (size_t j : rounds[r])
-      if (jrefs[j].ov && !golden_copy(jrefs[j])) return fail(EBIV_ERR_DEVICE, "golden copy failed");
-    if (intra_off[r] > inter_off[r]) {
-      uint64_t mbs = 0;
+    for (int g = 0; g < nl; g++) {
+      const size_t q = r * nl + g;
+      hipStream_t st = lane_stream(g);
       for (size_t j : rounds[r])
-        if (payload_of(ss, frames, jrefs[j])->inter) mbs += static_cast<uint64_t>(ss[jrefs[j].si]->mbc) * ss[jrefs[j].si]->mbr;
-      const int m = mark_begin(1, mbs);
-      launch_recon_inter(stream, static_cast<int>(inter_groups_of[r]), d_jobs.d, d_items.d + inter_off[r],
-                         static_cast<int>(intra_off[r] - inter_off[r]));
-      mark_end(m);
-    }
-    if (key_end[r] > intra_off[r]) {
-      const int m = mark_begin(2);
-      launch_recon_intra(stream, static_cast<int>(key_end[r] - intra_off[r]), d_jobs.d, d_items.d + intra_off[r],
-                         sched.intra_waves_key, true);
-      mark_end(m);
-    }
-    if (round_end[r] > key_end[r]) {
-      const size_t n = round_end[r] - key_end[r];
-      const int m = mark_begin(2);
-      launch_recon_intra(stream, static_cast<int>(n), d_jobs.d, d_items.d + key_end[r], sched.inter_intra_waves(n),
-                         false);
-      mark_end(m);
+        if (lane_of(j) == g && jrefs[j].ov && !golden_copy(jrefs[j], st)) return fail(EBIV_ERR_DEVICE, "golden copy failed");
+      if (intra_off[q] > inter_off[q]) {
+        const int m = mark_begin(1, inter_mbs_of[q], st);
+        launch_recon_inter(st, static_cast<int>(inter_groups_of[q]), d_jobs.d, d_items.d + inter_off[q],
+                           static_cast<int>(intra_off[q] - inter_off[q]));
+        mark_end(m, st);
+      }
+      if (key_end[q] > intra_off[q]) {
+        const int m = mark_begin(2, 0, st);
+        launch_recon_intra(st, static_cast<int>(key_end[q] - intra_off[q]), d_jobs.d, d_items.d + intra_off[q],
+                           sched.intra_waves_key, true);
+        mark_end(m, st);
+      }
+      if (round_end[q] > key_end[q]) {
+        const size_t n = round_end[q] - key_end[q];
+        const int m = mark_begin(2, 0, st);
+        launch_recon_intra(st, static_cast<int>(n), d_jobs.d, d_items.d + key_end[q], sched.inter_intra_waves(n),
+                           false);
+        mark_end(m, st);
+      }
     }
   }
+  for (int g = 1; g < nl; g++)
+    if (hipEventRecord(lane_join[g - 1], lanes[g - 1]) != hipSuccess ||
+        hipStreamWaitEvent(stream, lane_join[g - 1], 0) != hipSuccess)
+      return fail(EBIV_ERR_DEVICE, "lane join failed");
   if (nparse && hipMemcpyAsync(h_step_status.h, d_step_status.d, sizeof(TileStatus) * nparse, hipMemcpyDeviceToHost,
                                 stream) != hipSuccess)
     return fail(EBIV_ERR_DEVICE, "status download failed");
@@ -1033,7 +1081,7 @@ int Engine::run_step(std::vector<Session*>& ss, std::vector<std::vector<StepFram
         const size_t ninter = n;
         const uint32_t groups = (static_cast<uint32_t>(s.mbc * s.mbr) + kInterGroupMbs - 1) / kInterGroupMbs;
         for (int t = 0; t < ps->ntiles; t++) h_items.h[n++] = WorkItem{0, static_cast<uint32_t>(t)};
-        if (ov && !golden_copy(jr)) return fail(EBIV_ERR_DEVICE, "golden copy failed");
+        if (ov && !golden_copy(jr, stream)) return fail(EBIV_ERR_DEVICE, "golden copy failed");
         if (hipMemcpyAsync(d_jobs.d, h_jobs.h, sizeof(FrameJob), hipMemcpyHostToDevice, stream) != hipSuccess ||
             hipMemcpyAsync(d_items.d, h_items.h, sizeof(WorkItem) * n, hipMemcpyHostToDevice, stream) != hipSuccess)
           return fail(EBIV_ERR_DEVICE, "job table upload failed");
@@ -1477,13 +1525,14 @@ int ebiv_batch_set_schedule(ebiv_batch* b, const ebiv_schedule* sc) {
   waves(sc->intra_waves_few, d.intra_waves_few);
   waves(sc->intra_waves_many, d.intra_waves_many);
   if (sc->many_tiles > 0) d.many_tiles = sc->many_tiles;
+  if (sc->recon_lanes > 0) d.recon_lanes = sc->recon_lanes > Engine::kMaxLanes ? Engine::kMaxLanes : sc->recon_lanes;
   return EBIV_OK;
 }
 
 int ebiv_batch_get_schedule(const ebiv_batch* b, ebiv_schedule* out) {
   if (!b || !out) return EBIV_ERR_ARGUMENT;
   const Schedule& d = b->eng.sched;
-  *out = ebiv_schedule{d.intra_waves_key, d.intra_waves_few, d.intra_waves_many, d.many_tiles};
+  *out = ebiv_schedule{d.intra_waves_key, d.intra_waves_few, d.intra_waves_many, d.many_tiles, d.recon_lanes};
   return EBIV_OK;
 }
 

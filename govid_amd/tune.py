@@ -37,6 +37,7 @@ KEY_WAVES = (8, 16)
 FEW_WAVES = (4, 8, 16)
 MANY_WAVES = (2, 4, 8)
 MANY_TILES = (128, 256, 512)
+RECON_LANES = (1, 2, 4)
 
 
 def _crc(planes) -> int:
@@ -109,15 +110,16 @@ def tune(source, sessions: int = 64, window: int = 4, passes: int = 2, exhaustiv
             window_results.append(dict(window=w, **r))
         window = min(window_results, key=lambda r: r["wall_ms_per_frame"])["window"]
     if exhaustive:
-        cands = [dict(intra_waves_key=a, intra_waves_few=b, intra_waves_many=c, many_tiles=d)
-                 for a, b, c, d in itertools.product(KEY_WAVES, FEW_WAVES, MANY_WAVES, MANY_TILES)]
+        cands = [dict(intra_waves_key=a, intra_waves_few=b, intra_waves_many=c, many_tiles=d, recon_lanes=e)
+                 for a, b, c, d, e in itertools.product(KEY_WAVES, FEW_WAVES, MANY_WAVES, MANY_TILES, RECON_LANES)]
     else:  # one field at a time around the engine's defaults
         base = Batch()
         d0 = base.schedule()
         base.close()
         cands = [dict(d0)]
         for field, vals in (("intra_waves_key", KEY_WAVES), ("intra_waves_few", FEW_WAVES),
-                            ("intra_waves_many", MANY_WAVES), ("many_tiles", MANY_TILES)):
+                            ("intra_waves_many", MANY_WAVES), ("many_tiles", MANY_TILES),
+                            ("recon_lanes", RECON_LANES)):
             for v in vals:
                 c = dict(d0, **{field: v})
                 if c not in cands:

@@ -173,15 +173,17 @@ int ebiv_batch_set_timing(ebiv_batch *b, int enabled);
 int ebiv_batch_kernel_times(const ebiv_batch *b, ebiv_kernel_times *out);
 int ebiv_batch_frame_rgba(ebiv_batch *b, const ebiv_frame *f, uint8_t *dst, int dst_on_device);
 
-/* Launch shapes of the batch engine's intra reconstruction.  No counterpart
- * in the reference (its tile pool, ebiv/pool.go:35-72, has no launch shape);
- * for callers that tune the engine to their mix (govid_amd/tune.py).
+/* Launch shapes of the batch engine's reconstruction.  No counterpart in the
+ * reference (its tile pool, ebiv/pool.go:35-72, has no launch shape); for
+ * callers that tune the engine to their mix (govid_amd/tune.py).
  * intra_waves_key: wavefronts per key-frame tile; intra_waves_few and
  * intra_waves_many: per inter-frame tile, for launches of fewer than / at
- * least many_tiles tiles.  Wavefront counts are 1-16; a field <= 0 keeps its
- * value.  Results do not depend on the schedule. */
+ * least many_tiles tiles (1-16).  recon_lanes: HIP streams (1-4) the
+ * sessions' reconstruction launches are dealt to after the step's entropy
+ * launch, so that one group's launches overlap another's.  A field <= 0 keeps
+ * its value.  Results do not depend on the schedule. */
 typedef struct {
-    int32_t intra_waves_key, intra_waves_few, intra_waves_many, many_tiles;
+    int32_t intra_waves_key, intra_waves_few, intra_waves_many, many_tiles, recon_lanes;
 } ebiv_schedule;
 int ebiv_batch_set_schedule(ebiv_batch *b, const ebiv_schedule *s);
 int ebiv_batch_get_schedule(const ebiv_batch *b, ebiv_schedule *out);

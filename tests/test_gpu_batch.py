@@ -92,10 +92,13 @@ def test_sessions_of_mixed_geometry_and_windows():
     dict(intra_waves_key=1, intra_waves_few=1, intra_waves_many=1),
     dict(intra_waves_key=5, intra_waves_few=3, intra_waves_many=7, many_tiles=1),
     dict(intra_waves_key=16, intra_waves_few=16, intra_waves_many=16, many_tiles=1000),
+    dict(recon_lanes=2),
+    dict(recon_lanes=4, intra_waves_few=3),
 ])
 def test_launch_schedules_do_not_change_results(schedule):
     """Intra reconstruction with 1 to 16 wavefronts per tile (key frames from
-    the LDS frontier, inter frames from the frame), bit for bit."""
+    the LDS frontier, inter frames from the frame), and reconstruction split
+    over 2 or 4 streams, bit for bit."""
     streams = [
         (_clip("rects", 96, 64, 8, qp=22, gop=4, tile_cols=2, tile_rows=2), 3, 4),
         (_clip("parallax", 64, 48, 7, qp=28, gop=7, altref=3), 4, 4),
