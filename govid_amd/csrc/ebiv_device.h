@@ -136,9 +136,10 @@ struct FrameJob {
   Planes rec, ref, gold;
 };
 
-// Macroblocks per inter-reconstruction work item: one per wavefront of a
+// Macroblocks per inter-reconstruction work item: two per wavefront of a
 // 4-wavefront workgroup.
-constexpr uint32_t kInterGroupMbs = 4;
+constexpr uint32_t kInterMbsPerWave = 2;
+constexpr uint32_t kInterGroupMbs = 4 * kInterMbsPerWave;
 
 // One workgroup's share of a launch: job index and a tile or macroblock index.
 struct WorkItem {

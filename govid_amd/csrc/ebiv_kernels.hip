@@ -2837,14 +2837,12 @@ __device__ __forceinline__ void mc_part(const Planes& src, const Planes& rec, co
 // decode_inter.go:64-152): the residual (dequantised, inverse transformed)
 // first, then motion compensation per partition with the residual added;
 // only the blocks of the write mask are stored.
-__device__ __forceinline__ void recon_inter_mb(const uint8_t* __restrict__ desc, const MBRec* __restrict__ recs,
+__device__ __forceinline__ void recon_inter_mb(const uint8_t* __restrict__ desc, const MBRec r,
                                                const uint32_t* __restrict__ coefs, ConstJob& job, const int mb,
                                                InterScratch& S, const int lane) {
   // Everything per macroblock is wave-uniform: scalar registers and loads.
-  const int mbc = job.mbc, nmb = mbc * job.mbr;
-  if (mb >= nmb) return;
   // Macroblocks a failed tile stream never reached carry kind kKindUnreached.
-  const MBRec r = recs[mb];
+  const int mbc = job.mbc;
   if (r.kind == 2) {  // intra: only its residual, for the intra launch
     if (r.coef_count > 0) {
       const FrameHdr* hdr = reinterpret_cast<const FrameHdr*>(desc);
@@ -2913,7 +2911,18 @@ __global__ __launch_bounds__(256, 8) void ebiv_recon_inter(const FrameJob* __res
   const ConstItem& it = ((ConstItem*)items)[sgpr(k - 1)];
   ConstJob& job = job_of(jobs, it.job);
   const int first_mb = static_cast<int>((g - it.idx) * kInterGroupMbs);
-  recon_inter_mb(job.desc, job.recs, job.coefs, job, first_mb + wave, scratch[wave], lane);
+  // The wavefront's two macroblocks first_mb + wave and first_mb + wave + 4:
+  // both records are loaded before the first is reconstructed, so only the
+  // first waits for the work item, job and record loads in a row.
+  static_assert(kInterMbsPerWave == 2, "two macroblocks per wavefront");
+  const MBRec* __restrict__ recs = job.recs;
+  const int nmb = job.mbc * job.mbr;
+  const int mb0 = first_mb + wave, mb1 = mb0 + 4;
+  if (mb0 >= nmb) return;
+  const MBRec r0 = recs[mb0];
+  const MBRec r1 = recs[mb1 < nmb ? mb1 : mb0];
+  recon_inter_mb(job.desc, r0, job.coefs, job, mb0, scratch[wave], lane);
+  if (mb1 < nmb) recon_inter_mb(job.desc, r1, job.coefs, job, mb1, scratch[wave], lane);
 }
 
 // ============================================================================
