@@ -1051,11 +1051,13 @@ struct MbLoop {
   "s_sub_u32 %[t2], %[slot], %[p]\n"             \
   "s_add_u32 " R ", " R ", %[t2]\n"              \
   MBP_RN(tag, R)
-// The token loop at phase P (state RA); RB is the next phase's state (the
-// sign's).  NP1, NP2: the phases after one and two symbols.  A zero token is
-// symbol 0, whose table entry is its frequency alone (start 0).
-#define MBP_TOK(P, NP1, NP2, RA, RB)             \
-  "L%=_tok" P ":\n"                              \
+// The token loop at phase P (state RA) in state F: "a" before the block's
+// first level (whose sign resolves at the end of the block), "b" after it.
+// RB is the next phase's state (the sign's); NP1, NP2: the phases after one
+// and two symbols.  A zero token is symbol 0, whose table entry is its
+// frequency alone (start 0).
+#define MBP_TOK(P, NP1, NP2, RA, RB, F)          \
+  "L%=_tok" P #F ":\n"                            \
   "s_and_b32 %[slot], " RA ", 0xfff\n"           \
   "s_lshl_b32 %[t1], %[slot], 13\n"              \
   "s_or_b32 %[t1], %[t1], 0x1fff\n"              \
@@ -1066,17 +1068,17 @@ struct MbLoop {
   "s_bfe_u64 %[b], %[b], %[t1]\n"                \
   "s_bcnt1_i32_b64 %[cnt], %[b]\n"               \
   "s_cmp_eq_u32 %[cnt], 1\n"                     \
-  "s_cbranch_scc0 L%=_tnz" P "\n"                \
+  "s_cbranch_scc0 L%=_tnz" P #F "\n"              \
   "v_readlane_b32 %[p], %[vcur], %[t2]\n"        \
   "s_mov_b32 %[prev], 0\n"                       \
   "s_mul_i32 " RA ", %[p], " RA "\n"             \
   "s_add_u32 " RA ", " RA ", %[slot]\n"          \
-  MBP_RN_NC("tz" P, RA)                             \
+  MBP_RN_NC("tz" P #F, RA)                        \
   "s_add_u32 %[si], %[si], 1\n"                  \
   "s_cmp_lg_u32 %[si], %[stop]\n"                \
-  "s_cbranch_scc1 L%=_tok" NP1 "\n"              \
-  "s_branch L%=_tstop" NP1 "\n"                  \
-  "L%=_tnz" P ":\n"                              \
+  "s_cbranch_scc1 L%=_tok" NP1 #F "\n"            \
+  "s_branch L%=_tstop" NP1 #F "\n"                \
+  "L%=_tnz" P #F ":\n"                            \
   "s_cmp_eq_u32 %[cnt], 0\n"                     \
   "s_cbranch_scc1 L%=_unused\n"                  \
   "s_add_u32 %[t1], %[t2], %[cnt]\n"             \
@@ -1088,36 +1090,51 @@ struct MbLoop {
   "s_mul_i32 " RA ", %[t2], " RA "\n"            \
   "s_sub_u32 %[t2], %[slot], %[p]\n"             \
   "s_add_u32 " RA ", " RA ", %[t2]\n"            \
-  MBP_RN_NC("tn" P, RA)                             \
+  MBP_RN_NC("tn" P #F, RA)                        \
   "s_cmp_ge_u32 %[cnt], 6\n"                     \
   "s_cbranch_scc1 L%=_nesc" NP1 "\n"             \
   "s_cmp_eq_u32 " MB_MAG ", 1\n"                 \
   "s_cselect_b32 %[prev], 8, 16\n"               \
-  "L%=_tlev" NP1 ":\n"                           \
   "s_add_u32 %[sumabs], %[sumabs], " MB_MAG "\n" \
-  "s_cmp_eq_u32 %[firstsi], 0\n"                 \
-  "s_cbranch_scc1 L%=_tfirst" NP1 "\n"           \
-  MBP_SIGN("ts" P, RB)                           \
-  MB_EMIT_AT("e1" P, "%[si]")                    \
-  "s_mov_b32 %[last], %[si]\n"                   \
-  "s_add_u32 %[si], %[si], 1\n"                  \
-  "s_cmp_lg_u32 %[si], %[stop]\n"                \
-  "s_cbranch_scc1 L%=_tok" NP2 "\n"              \
-  "s_branch L%=_tstop" NP2 "\n"                  \
-  "L%=_tfirst" NP1 ":\n"                         \
+  MBP_LEVEL_##F(P, NP1, NP2, RB)
+// A level in state a: the block's first, its sign deferred; state b follows.
+#define MBP_LEVEL_a(P, NP1, NP2, RB)             \
+  MBP_FIRST(NP1)
+#define MBP_FIRST(NP)                            \
   "s_lshl_b32 %[t1], " MB_MAG ", 8\n"            \
   "s_or_b32 %[firstpm], %[t1], %[si]\n"          \
   "s_add_u32 %[firstsi], %[si], 1\n"             \
   "s_mov_b32 %[last], %[si]\n"                   \
   "s_add_u32 %[si], %[si], 1\n"                  \
   "s_cmp_lg_u32 %[si], %[stop]\n"                \
-  "s_cbranch_scc1 L%=_tok" NP1 "\n"              \
-  "s_branch L%=_tstop" NP1 "\n"
-// si reached stop at phase P: the end of the block (back to phase 0, then
-// the end-of-block code) or a band boundary (bandOf thresholds 1, 3, 9, 21,
-// model.go:165-180).
-#define MBP_TSTOP(P)                             \
-  "L%=_tstop" P ":\n"                            \
+  "s_cbranch_scc1 L%=_tok" NP "b\n"              \
+  "s_branch L%=_tstop" NP "b\n"
+// A later level in state b: its explicit sign, then its entry.
+#define MBP_LEVEL_b(P, NP1, NP2, RB)             \
+  MBP_SIGN("ts" P, RB)                           \
+  MBP_LATER(P, NP2)
+#define MBP_LATER(TAG, NP)                       \
+  MB_EMIT_AT("e1" TAG, "%[si]")                  \
+  "s_mov_b32 %[last], %[si]\n"                   \
+  "s_add_u32 %[si], %[si], 1\n"                  \
+  "s_cmp_lg_u32 %[si], %[stop]\n"                \
+  "s_cbranch_scc1 L%=_tok" NP "b\n"              \
+  "s_branch L%=_tstop" NP "b\n"
+// An escaped level (magnitude in MB_MAG) at phase 0, in either state.
+#define MBP_ESCLEVEL                             \
+  "L%=_tlev0:\n"                                 \
+  "s_add_u32 %[sumabs], %[sumabs], " MB_MAG "\n" \
+  "s_cmp_eq_u32 %[firstsi], 0\n"                 \
+  "s_cbranch_scc1 L%=_tfirstg\n"                 \
+  MBP_SIGN("tsg", "%[x0]")                       \
+  MBP_LATER("g", "1")                            \
+  "L%=_tfirstg:\n"                               \
+  MBP_FIRST("0")
+// si reached stop at phase P in state F: the end of the block (back to phase
+// 0, then the end-of-block code) or a band boundary (bandOf thresholds 1, 3,
+// 9, 21, model.go:165-180).
+#define MBP_TSTOP(P, F)                          \
+  "L%=_tstop" P F ":\n"                          \
   "s_cmp_eq_u32 %[si], %[total]\n"               \
   "s_cbranch_scc1 L%=_nend" P "\n"               \
   "s_xor_b32 %[boff], %[boff], 24\n"             \
@@ -1126,17 +1143,17 @@ struct MbLoop {
   "s_cmp_eq_u32 %[si], 9\n"                      \
   "s_cselect_b32 %[stop], 21, %[stop]\n"         \
   "s_cmp_eq_u32 %[si], 3\n"                      \
-  "s_cbranch_scc0 L%=_tb2" P "\n"                \
+  "s_cbranch_scc0 L%=_tb2" P F "\n"              \
   "v_mov_b32 %[vcur], %[vb]\n"                   \
   "s_mov_b32 %[stop], 9\n"                       \
-  "L%=_tb2" P ":\n"                              \
+  "L%=_tb2" P F ":\n"                            \
   "s_cmp_eq_u32 %[si], 21\n"                     \
-  "s_cbranch_scc0 L%=_tb4" P "\n"                \
+  "s_cbranch_scc0 L%=_tb4" P F "\n"              \
   "v_mov_b32 %[vcur], %[vc]\n"                   \
   "s_mov_b32 %[stop], %[total]\n"                \
-  "L%=_tb4" P ":\n"                              \
+  "L%=_tb4" P F ":\n"                            \
   "s_min_u32 %[stop], %[stop], %[total]\n"       \
-  "s_branch L%=_tok" P "\n"
+  "s_branch L%=_tok" P F "\n"
 // Back to phase 0 from phase 1, 2, 3 (t1 as the spare register).
 #define MBP_NORM1                                \
   "s_mov_b32 %[t1], %[x0]\n"                     \
@@ -1158,11 +1175,17 @@ struct MbLoop {
   "s_mov_b32 %[x1], %[x0]\n"                     \
   "s_mov_b32 %[x0], %[t1]\n"
 #define MBP_TOKLOOP                              \
-  MBP_TOK("0", "1", "2", "%[x0]", "%[x1]")       \
-  MBP_TOK("1", "2", "3", "%[x1]", "%[x2]")       \
-  MBP_TOK("2", "3", "0", "%[x2]", "%[x3]")       \
-  MBP_TOK("3", "0", "1", "%[x3]", "%[x0]")       \
-  MBP_TSTOP("0") MBP_TSTOP("1") MBP_TSTOP("2") MBP_TSTOP("3") \
+  MBP_TOK("0", "1", "2", "%[x0]", "%[x1]", a)    \
+  MBP_TOK("1", "2", "3", "%[x1]", "%[x2]", a)    \
+  MBP_TOK("2", "3", "0", "%[x2]", "%[x3]", a)    \
+  MBP_TOK("3", "0", "1", "%[x3]", "%[x0]", a)    \
+  MBP_TOK("0", "1", "2", "%[x0]", "%[x1]", b)    \
+  MBP_TOK("1", "2", "3", "%[x1]", "%[x2]", b)    \
+  MBP_TOK("2", "3", "0", "%[x2]", "%[x3]", b)    \
+  MBP_TOK("3", "0", "1", "%[x3]", "%[x0]", b)    \
+  MBP_TSTOP("0", "a") MBP_TSTOP("1", "a") MBP_TSTOP("2", "a") MBP_TSTOP("3", "a") \
+  MBP_TSTOP("0", "b") MBP_TSTOP("1", "b") MBP_TSTOP("2", "b") MBP_TSTOP("3", "b") \
+  MBP_ESCLEVEL                                   \
   "L%=_nesc0:\n" "s_branch L%=_tescob\n"         \
   "L%=_nesc1:\n" MBP_NORM1 "s_branch L%=_tescob\n" \
   "L%=_nesc2:\n" MBP_NORM2 "s_branch L%=_tescob\n" \
@@ -1172,14 +1195,19 @@ struct MbLoop {
   "L%=_nend2:\n" MBP_NORM2 "s_branch L%=_tend\n" \
   "L%=_nend3:\n" MBP_NORM3 "s_branch L%=_tend\n"
 #define MBP_TOKLOOP_OOL                          \
-  MBP_RN_OOL("tz0", "underrun", "%[x0]") MBP_RN_OOL("tn0", "underrun", "%[x0]") \
-  MBP_RN_OOL("ts0", "underrun", "%[x1]") MB_EMIT_OOL("e10")                     \
-  MBP_RN_OOL("tz1", "underrun", "%[x1]") MBP_RN_OOL("tn1", "underrun", "%[x1]") \
-  MBP_RN_OOL("ts1", "underrun", "%[x2]") MB_EMIT_OOL("e11")                     \
-  MBP_RN_OOL("tz2", "underrun", "%[x2]") MBP_RN_OOL("tn2", "underrun", "%[x2]") \
-  MBP_RN_OOL("ts2", "underrun", "%[x3]") MB_EMIT_OOL("e12")                     \
-  MBP_RN_OOL("tz3", "underrun", "%[x3]") MBP_RN_OOL("tn3", "underrun", "%[x3]") \
-  MBP_RN_OOL("ts3", "underrun", "%[x0]") MB_EMIT_OOL("e13")
+  MBP_RN_OOL("tz0a", "underrun", "%[x0]") MBP_RN_OOL("tn0a", "underrun", "%[x0]") \
+  MBP_RN_OOL("tz1a", "underrun", "%[x1]") MBP_RN_OOL("tn1a", "underrun", "%[x1]") \
+  MBP_RN_OOL("tz2a", "underrun", "%[x2]") MBP_RN_OOL("tn2a", "underrun", "%[x2]") \
+  MBP_RN_OOL("tz3a", "underrun", "%[x3]") MBP_RN_OOL("tn3a", "underrun", "%[x3]") \
+  MBP_RN_OOL("tz0b", "underrun", "%[x0]") MBP_RN_OOL("tn0b", "underrun", "%[x0]") \
+  MBP_RN_OOL("tz1b", "underrun", "%[x1]") MBP_RN_OOL("tn1b", "underrun", "%[x1]") \
+  MBP_RN_OOL("tz2b", "underrun", "%[x2]") MBP_RN_OOL("tn2b", "underrun", "%[x2]") \
+  MBP_RN_OOL("tz3b", "underrun", "%[x3]") MBP_RN_OOL("tn3b", "underrun", "%[x3]") \
+  MBP_RN_OOL("ts0", "underrun", "%[x1]") MB_EMIT_OOL("e10")                       \
+  MBP_RN_OOL("ts1", "underrun", "%[x2]") MB_EMIT_OOL("e11")                       \
+  MBP_RN_OOL("ts2", "underrun", "%[x3]") MB_EMIT_OOL("e12")                       \
+  MBP_RN_OOL("ts3", "underrun", "%[x0]") MB_EMIT_OOL("e13")                       \
+  MBP_RN_OOL("tsg", "underrun", "%[x0]") MB_EMIT_OOL("e1g")
 
 // Record dword k (value v) of the staged macroblock nst.
 #define MB_REC(k, v)                             \
