@@ -470,10 +470,8 @@ def main():
         log(f"bench: batched {bfps:.1f} frames/s")
         out["batched"] = batched_summary(bt, args.steps, vs_plain=round(bfps / fps, 2),
                                          sessions=args.sessions, window=args.window)
-        # measured at window 8; an inter launch holds one frame position of
-        # every session (64 frames) at any window, so the figure per launch holds
         out["batched"]["recon"]["ebiv_recon_inter"]["traffic"] = measured_traffic(
-            "batched_64x1080p_parallax_w8", "ebiv_recon_inter")
+            "batched_64x1080p_parallax", "ebiv_recon_inter")
     for d in dev:
         d.close()
     if not args.no_mixed:
