@@ -5,7 +5,7 @@
 // transform.go, mv.go, writer.go) so that streams exercising every decode
 // syntax element can be produced where Go is absent.  Pass 1 (dead-zone
 // quantisation priced with estimateBits) is what WithFastEncode runs; the
-// second, real-cost RDOQ pass (quant.go:554-650) is run when fast == 0.
+// second, real-cost RDOQ pass (quant.go:102-199) is run when fast == 0.
 // Its reconstruction is computed by its own integer paths, independent of
 // the decoder, so "encoder reconstruction == decoded output" is a real check.
 //
@@ -168,7 +168,7 @@ struct Quant {
     ac = T().qstep[qp];
     dc = qp >= 6 ? T().qstep[qp - 6] : ac;
   }
-  void quantize(const int32_t* co, int32_t* lv, int n) const {  // quant.go:58-70
+  void quantize(const int32_t* co, int32_t* lv, int n) const {  // quant.go:55-70
     for (int i = 0; i < n * n; i++) {
       int32_t step = i == 0 ? dc : ac, c = co[i];
       lv[i] = c >= 0 ? (c * 8 + 3 * step) / (step * 8) : -((-c * 8 + 3 * step) / (step * 8));
@@ -189,14 +189,14 @@ double go_log2(double x) {
   return std::log(frac) * (1.0 / M_LN2) + static_cast<double>(e);
 }
 
-double coeff_rate(int m, int ctx, const CostModel& cost) {  // quant.go:654-660
+double coeff_rate(int m, int ctx, const CostModel& cost) {  // quant.go:202-208
   if (m <= 4) return cost[ctx][m];
   unsigned v = static_cast<unsigned>(m - 5);
   int bl = bits_len(v);
   return cost[ctx][kTEscape] + cost[kCtxEscClass][bl] + static_cast<double>(std::max(0, bl - 1));
 }
 
-// quantizeRDOQ, quant.go:554-650.
+// quantizeRDOQ, quant.go:102-199.
 void quantize_rdoq(const Quant& q, const int32_t* coeffs, int32_t* levels, int n, int plane, int tx, double lambda,
                    const CostModel& cost) {
   const int* scan = T().zz[sidx(n)];
@@ -255,7 +255,7 @@ void quantize_rdoq(const Quant& q, const int32_t* coeffs, int32_t* levels, int n
   for (int si = 0; si < total; si++) levels[scan[si]] = si <= best_t ? lvl_scan[si] : 0;
 }
 
-// applySignDataHiding, residual.go:364-410.
+// applySignDataHiding, residual.go:134-180.
 void apply_sdh(int32_t* lv, int n) {
   const int* scan = T().zz[sidx(n)];
   int total = n * n, first = -1, last = -1;
@@ -287,7 +287,7 @@ void put_class_value(TileStream& s, int ctx, unsigned v) {
 }
 inline unsigned zigzag_enc(int v) { return static_cast<unsigned>((v << 1) ^ (v >> 31)); }
 
-// encodeBlock, residual.go:260-305.  Returns true for an all-zero block.
+// encodeBlock, residual.go:30-75.  Returns true for an all-zero block.
 bool encode_block(TileStream& s, int plane, int tx, const int32_t* lv, int n) {
   const int* scan = T().zz[sidx(n)];
   int total = n * n, first = -1, last = -1;
@@ -1335,7 +1335,7 @@ size_t ebivenc_recon_count(void* h) { return static_cast<Encoder*>(h)->recons.si
 const uint8_t* ebivenc_recon(void* h, size_t i) { return static_cast<Encoder*>(h)->recons[i].data(); }
 void ebivenc_free(void* h) { delete static_cast<Encoder*>(h); }
 
-// Test hooks over the encoder's own block grammar (residual.go:260-305,
+// Test hooks over the encoder's own block grammar (residual.go:30-75,
 // rans.go:199-214) so the decoder can be checked on hand-written level
 // patterns: encodes `count` blocks of n*n levels (row-major) after sign data
 // hiding (when sdh != 0), returning the serialized key-frame tables and the
@@ -1412,7 +1412,7 @@ int ebivenc_forward_dct(const int32_t* src, int32_t* dst, int n) {
   return 0;
 }
 
-// quantizer.quantize / dequantize (quant.go:58-80) for the quantiser test.
+// quantizer.quantize / dequantize (quant.go:55-80) for the quantiser test.
 int ebivenc_quantize(int qp, const int32_t* coeffs, int32_t* levels, int n) {
   ebiv::Quant q(qp);
   q.quantize(coeffs, levels, n);

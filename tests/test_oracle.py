@@ -283,7 +283,8 @@ def test_predict_mv_cases():
     g[0, 0] = (100, 100)
     assert O.predict_mv(g, 0, 0, 0, 4, 0, 4) == (0, 0)
     assert O.predict_mv(g, 1, 0, 0, 4, 0, 4) == (100, 100)          # only left
-    assert O.predict_mv(g, 0, 1, 0, 4, 0, 4) == (100, 100) or True  # above + above-right
+    # above + above-right, no left: the missing left contributes a zero vector
+    assert O.predict_mv(g, 0, 1, 0, 4, 0, 4) == (12, 2)              # median(0,100,12), (0,100,2)
     assert O.predict_mv(g, 1, 1, 0, 4, 0, 4) == (4, 2)               # median(4,12,-6), (-8,2,10)
     # at the tile's right edge above-left substitutes for above-right
     assert O.predict_mv(g, 1, 1, 0, 2, 0, 4) == (12, 2)              # median(4,12,100), (-8,2,100)
