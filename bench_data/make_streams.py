@@ -16,12 +16,15 @@ benchmark skips a section whose streams are absent):
   4x3 tiles; 30 frames (two GOPs), which the benchmark's 32 sessions replay
   from staggered key frames (about 1.2 MB per frame, so a full 120-frame
   stream per session would be 4.6 GB of input).
+  serve_360p / serve_720p / serve_1080p / serve_2160p.ebiv — the serving
+  mix: rects content, q22, gop 30, one tile (8x6 tiles at 3840x2160), 600
+  frames each.
   mixed_00 .. mixed_63.ebiv — the mixed batch: 24 streams of 640x360, 24 of
   1280x720 and 16 of 1920x1080, rects content, each with its own draw of
   tile grid {1x1, 2x2, 4x3}, gop {30, 60, 120}, q {18, 22, 28} and length
   60-240 frames (seeded; see mixed_specs).
 
-Run: python3 bench_data/make_streams.py [--jobs N] [name ... | noisepan_1080p | mixed]
+Run: python3 bench_data/make_streams.py [--jobs N] [name ... | noisepan_1080p | mixed | serve]
 """
 from __future__ import annotations
 
@@ -75,6 +78,21 @@ def mixed_specs():
 STREAMS.update(mixed_specs())
 
 
+def serving_specs():
+    """The serving mix's streams: rects content encoded like the 320x180
+    clip (q22, gop 30, one tile) at 640x360, 1280x720 and 1920x1080, and
+    3840x2160 with 8x6 tiles; 600 frames each, so that plays of 30-600
+    frames need no clamping."""
+    out = {}
+    for w, h, tc, tr in ((640, 360, 1, 1), (1280, 720, 1, 1), (1920, 1080, 1, 1), (3840, 2160, 8, 6)):
+        out[f"serve_{h}p"] = dict(workload="rects", w=w, h=h, frames=600, seed=200 + h, dir=GEN,
+                                  enc=dict(qp=22, gop=30, tile_cols=tc, tile_rows=tr))
+    return out
+
+
+STREAMS.update(serving_specs())
+
+
 def path_of(name: str) -> str:
     return os.path.join(STREAMS[name].get("dir", HERE), name + ".ebiv")
 
@@ -113,7 +131,8 @@ def make(name: str, threads: int = 8) -> str:
 
 
 def _make_one(name: str) -> str:
-    return make(name, threads=1)
+    # one thread per stream, four for the 3840x2160 one (48 tiles)
+    return make(name, threads=4 if STREAMS[name]["w"] * STREAMS[name]["h"] > 4_000_000 else 1)
 
 
 if __name__ == "__main__":
@@ -123,7 +142,10 @@ if __name__ == "__main__":
         jobs, args = int(args[1]), args[2:]
     names = []
     for a in args or [k for k, v in STREAMS.items() if "dir" not in v]:
-        names += sorted(k for k in STREAMS if k.startswith("mixed_")) if a == "mixed" else [a]
+        if a in ("mixed", "serve"):
+            names += sorted(k for k in STREAMS if k.startswith(a + "_"))
+        else:
+            names.append(a)
     if jobs > 1:
         from multiprocessing import Pool
 

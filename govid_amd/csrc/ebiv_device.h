@@ -1,8 +1,7 @@
 // ebiv_device.h — data layouts shared by the host runtime and the gfx950
-// kernels.  Plain structs only; both sides include it.
+// kernels.  Plain structs and constexpr helpers only (constexpr functions are
+// callable from device code), so host-only code builds against it too.
 #pragma once
-
-#include <hip/hip_runtime.h>
 
 #include <cstddef>
 #include <cstdint>
@@ -33,7 +32,7 @@ static_assert(sizeof(TileDesc) == 24, "TileDesc layout");
 constexpr int kNumTableVecs = 15;
 constexpr uint32_t kNoSym = 0xFFFFFFFFu;
 // Vector and first lane of a context's entries; false for the bypass context.
-__host__ __device__ inline bool table_place(int ctx, int* vec, int* lane0) {
+constexpr bool table_place(int ctx, int* vec, int* lane0) {
   // contexts 0-11 in enum order: tx, luma mode, chroma mode, MB mode, ref,
   // part, CBP, sign, escape class, MV x, MV y, bypass
   constexpr int8_t kVec[12] = {0, 1, 1, 0, 1, 1, 0, 0, 2, 2, 2, -1};
@@ -66,11 +65,11 @@ struct FrameHdr {
 
 // Per-frame device descriptor = FrameHdr, TileDesc[ntiles], uint16 col_tile[mbc],
 // uint16 row_tile[mbr], laid out back to back (each part 16-byte aligned).
-__host__ __device__ inline size_t align16(size_t v) { return (v + 15) & ~size_t(15); }
-__host__ __device__ inline size_t desc_tiles_off() { return align16(sizeof(FrameHdr)); }
-__host__ __device__ inline size_t desc_coltile_off(int ntiles) { return align16(desc_tiles_off() + sizeof(TileDesc) * ntiles); }
-__host__ __device__ inline size_t desc_rowtile_off(int ntiles, int mbc) { return align16(desc_coltile_off(ntiles) + 2 * size_t(mbc)); }
-__host__ __device__ inline size_t desc_size(int ntiles, int mbc, int mbr) { return align16(desc_rowtile_off(ntiles, mbc) + 2 * size_t(mbr)); }
+constexpr size_t align16(size_t v) { return (v + 15) & ~size_t(15); }
+constexpr size_t desc_tiles_off() { return align16(sizeof(FrameHdr)); }
+constexpr size_t desc_coltile_off(int ntiles) { return align16(desc_tiles_off() + sizeof(TileDesc) * ntiles); }
+constexpr size_t desc_rowtile_off(int ntiles, int mbc) { return align16(desc_coltile_off(ntiles) + 2 * size_t(mbc)); }
+constexpr size_t desc_size(int ntiles, int mbc, int mbr) { return align16(desc_rowtile_off(ntiles, mbc) + 2 * size_t(mbr)); }
 
 // Macroblock record written by the entropy kernel and read by the
 // reconstruction kernels.  kind: 0 skip, 1 inter, 2 intra (decode_inter.go:8-62),

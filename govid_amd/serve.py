@@ -18,8 +18,10 @@ decodes the next packet, `seek` repositions the demuxer and flushes the codec
   batching, not a queue in front of a single decoder.
 
 Per frame the server records the latency from the moment its packet was
-submitted to the moment its step returned.  It checks nothing itself; callers
-compare frames (the tests do, against the oracle).
+submitted to the moment its step returned; per request, the admission wait
+from its arrival to its admission, and the time from its arrival to its first
+frame.  It checks nothing itself; callers compare frames (the tests do,
+against the oracle).
 """
 from __future__ import annotations
 
@@ -106,11 +108,14 @@ class Server:
         self.frames: list = []
         self.max_active = 0
         self.peak_bytes = 0
+        self.waits: list = []        # admission wait per request, seconds
+        self.first_frame: list = []  # arrival to first completed frame, seconds
 
     class _Run:
         def __init__(self, req, sid, nbytes, order):
             self.req, self.sid, self.nbytes = req, sid, nbytes
             self.order = order
+            self.first_done = False
             self.next = 0          # next playback position to submit
             self.completed = 0
             self.submit_time = {}  # playback position -> submit time
@@ -127,6 +132,7 @@ class Server:
             if self.used + cost > self.budget:
                 break
             self.waiting.popleft()
+            self.waits.append(max(0.0, now - req.arrival))
             sid = self.engine.open(ring=self.ring, window=self.window)
             self.used += cost
             self.peak_bytes = max(self.peak_bytes, self.used)
@@ -183,6 +189,9 @@ class Server:
                 if self.on_frame:
                     self.on_frame(fd)
                 run.completed += 1
+                if not run.first_done:
+                    run.first_done = True
+                    self.first_frame.append(t_done - run.req.arrival)
                 if run.completed == len(run.order):
                     finished.append(run)
             for run in finished:
@@ -191,13 +200,17 @@ class Server:
                 del self.active[run.sid]
                 self.done.append(run.req.rid)
         elapsed = self.clock() - t0
-        lats = sorted(f.latency_s for f in self.frames)
+        def pct(v, q):
+            v = sorted(v)
+            return v[min(len(v) - 1, int(q * len(v)))] if v else 0.0
 
-        def pct(q):
-            return lats[min(len(lats) - 1, int(q * len(lats)))] if lats else 0.0
-
+        lats = [f.latency_s for f in self.frames]
         return {"frames": len(self.frames), "elapsed_s": elapsed, "frames_per_s": len(self.frames) / elapsed,
-                "p50_frame_ms": pct(0.5) * 1e3, "p99_frame_ms": pct(0.99) * 1e3, "steps": steps,
+                "p50_frame_ms": pct(lats, 0.5) * 1e3, "p99_frame_ms": pct(lats, 0.99) * 1e3,
+                "admission_wait_p50_ms": pct(self.waits, 0.5) * 1e3,
+                "admission_wait_p99_ms": pct(self.waits, 0.99) * 1e3,
+                "first_frame_p50_ms": pct(self.first_frame, 0.5) * 1e3,
+                "first_frame_p99_ms": pct(self.first_frame, 0.99) * 1e3, "steps": steps,
                 "requests": len(pending), "max_active_sessions": self.max_active,
                 "peak_session_bytes": self.peak_bytes, "errors": sum(1 for f in self.frames if f.status)}
 
