@@ -24,7 +24,7 @@ benchmark skips a section whose streams are absent):
   tile grid {1x1, 2x2, 4x3}, gop {30, 60, 120}, q {18, 22, 28} and length
   60-240 frames (seeded; see mixed_specs).
 
-Run: python3 bench_data/make_streams.py [--jobs N] [name ... | noisepan_1080p | mixed | serve]
+Run: python3 bench_data/make_streams.py [--jobs N] [--missing] [name ... | noisepan_1080p | mixed | serve]
 """
 from __future__ import annotations
 
@@ -140,12 +140,17 @@ if __name__ == "__main__":
     jobs = 1
     if args[:1] == ["--jobs"]:
         jobs, args = int(args[1]), args[2:]
+    skip = args[:1] == ["--missing"]  # only streams whose file is absent
+    if skip:
+        args = args[1:]
     names = []
     for a in args or [k for k, v in STREAMS.items() if "dir" not in v]:
         if a in ("mixed", "serve"):
             names += sorted(k for k in STREAMS if k.startswith(a + "_"))
         else:
             names.append(a)
+    if skip:
+        names = [n for n in names if not os.path.exists(path_of(n))]
     if jobs > 1:
         from multiprocessing import Pool
 
