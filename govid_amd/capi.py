@@ -17,7 +17,7 @@ ERR_DEVICE = 17
 # Every symbol include/ebiv_gpu.h declares; tests check the library exports them.
 EXPORTS = (
     "ebiv_codec_create", "ebiv_codec_decode", "ebiv_codec_decode_device", "ebiv_codec_flush",
-    "ebiv_codec_destroy", "ebiv_codec_last_error", "ebiv_codec_stats", "ebiv_frame_rgba",
+    "ebiv_codec_destroy", "ebiv_codec_last_error", "ebiv_codec_stats", "ebiv_frame_rgba", "ebiv_frame_rgba_noalloc",
     "ebiv_device_alloc", "ebiv_device_free", "ebiv_memcpy_h2d", "ebiv_memcpy_d2h", "ebiv_device_count",
     "ebiv_codec_set_timing", "ebiv_codec_kernel_times", "ebiv_codec_debug_dump",
     "ebiv_batch_create", "ebiv_batch_destroy", "ebiv_batch_last_error", "ebiv_batch_open", "ebiv_batch_close",
@@ -88,6 +88,8 @@ def bind(L):
     L.ebiv_codec_stats.argtypes = [vp, C.POINTER(Stats)]
     L.ebiv_frame_rgba.restype = C.c_int
     L.ebiv_frame_rgba.argtypes = [vp, C.POINTER(Frame), vp, C.c_int]
+    L.ebiv_frame_rgba_noalloc.restype = C.c_int
+    L.ebiv_frame_rgba_noalloc.argtypes = [vp, C.POINTER(Frame), vp]
     L.ebiv_device_alloc.restype = C.c_int
     L.ebiv_device_alloc.argtypes = [C.c_int, sz, C.POINTER(vp)]
     L.ebiv_device_free.restype = C.c_int

@@ -91,15 +91,32 @@ const StaticTables& statics() {
 std::mutex g_init_mu;
 std::vector<bool> g_dev_ready;
 
+// Staging of ebiv_frame_rgba_noalloc: RGBA of one frame of at most
+// kRgbaScratchBytes, converted on the device and copied to the caller's host
+// memory.  A file-scope device array, so that entry point allocates nothing:
+// each device holds its own copy (in the library's code object) until the
+// process ends.  In host code `scratch` names a host-side stand-in, never the
+// device memory: its device address is taken once per device in
+// ensure_device (g_scratch) and used everywhere instead.
+constexpr size_t kRgbaScratchBytes = static_cast<size_t>(3840) * 2160 * 4;
+static __device__ __align__(256) uint8_t scratch[kRgbaScratchBytes];
+std::vector<uint8_t*> g_scratch;  // per device: scratch's device address
+
 int ensure_device(int dev) {
   std::lock_guard<std::mutex> lk(g_init_mu);
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || dev < 0 || dev >= count) return EBIV_ERR_DEVICE;
-  if (static_cast<int>(g_dev_ready.size()) < count) g_dev_ready.resize(count, false);
+  if (static_cast<int>(g_dev_ready.size()) < count) {
+    g_dev_ready.resize(count, false);
+    g_scratch.resize(count, nullptr);
+  }
   if (hipSetDevice(dev) != hipSuccess) return EBIV_ERR_DEVICE;
   if (!g_dev_ready[dev]) {
     const StaticTables& t = statics();
     if (upload_tables(t.dct, t.zz) != hipSuccess) return EBIV_ERR_DEVICE;
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return EBIV_ERR_DEVICE;
+    g_scratch[dev] = static_cast<uint8_t*>(p);
     g_dev_ready[dev] = true;
   }
   return EBIV_OK;
@@ -1333,6 +1350,30 @@ static int frame_rgba(Engine& e, const ebiv_frame* f, uint8_t* dst, int dst_on_d
 int ebiv_frame_rgba(ebiv_codec* c, const ebiv_frame* f, uint8_t* dst, int dst_on_device) {
   if (!c) return EBIV_ERR_ARGUMENT;
   return frame_rgba(c->eng, f, dst, dst_on_device);
+}
+
+int ebiv_frame_rgba_noalloc(ebiv_codec* c, const ebiv_frame* f, uint8_t* dst) {
+  if (!c) return EBIV_ERR_ARGUMENT;
+  Session& s = *c->s;
+  Engine& e = c->eng;
+  if (!f || !dst || !f->y || f->width <= 0 || f->height <= 0) return s.fail(EBIV_ERR_ARGUMENT, "null frame or destination");
+  const size_t bytes = static_cast<size_t>(f->width) * static_cast<size_t>(f->height) * 4;
+  if (bytes > kRgbaScratchBytes)
+    return s.fail(EBIV_ERR_ARGUMENT, "frame too large for ebiv_frame_rgba_noalloc (at most 3840x2160 pixels)");
+  if (hipSetDevice(e.device) != hipSuccess) return s.fail(EBIV_ERR_DEVICE, "hipSetDevice failed");
+  uint8_t* stage = e.device < static_cast<int>(g_scratch.size()) ? g_scratch[e.device] : nullptr;
+  if (!stage) return s.fail(EBIV_ERR_DEVICE, "device not initialised");
+  const DevBuf* b = buffer_of(e, f);
+  if (!b) return s.fail(EBIV_ERR_ARGUMENT, "frame not returned by this codec");
+  // The kernel writes all `bytes` of the staging area that are copied out,
+  // so nothing needs clearing between calls.
+  launch_rgba(e.stream, b->p.y, b->p.pw, b->p.cb, b->p.cr, b->p.cw, f->width, f->height,
+              reinterpret_cast<uint32_t*>(stage));
+  if (hipGetLastError() != hipSuccess) return s.fail(EBIV_ERR_DEVICE, "kernel launch failed");
+  if (hipMemcpyAsync(dst, stage, bytes, hipMemcpyDeviceToHost, e.stream) != hipSuccess ||
+      hipStreamSynchronize(e.stream) != hipSuccess)
+    return s.fail(EBIV_ERR_DEVICE, "rgba download failed");
+  return EBIV_OK;
 }
 
 // ---- batched engine ------------------------------------------------------------

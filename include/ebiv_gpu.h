@@ -111,6 +111,16 @@ int ebiv_codec_debug_dump(ebiv_codec *c, int slot, void *recs, size_t recs_bytes
  * dst receives width*height*4 bytes; dst_on_device selects HBM or host. */
 int ebiv_frame_rgba(ebiv_codec *c, const ebiv_frame *f, uint8_t *dst, int dst_on_device);
 
+/* Frame.RGBA of a frame this codec returned into host memory (dst receives
+ * width*height*4 bytes), allocating nothing.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (HIP graph
+ * capture, tight serving loops).  The conversion is staged in a fixed device
+ * buffer of 3840*2160*4 bytes that the library holds on each device it runs on
+ * until the process ends; larger frames return EBIV_ERR_ARGUMENT.  Every call
+ * on one device shares that buffer: calls that overlap on one device (from
+ * several threads or codecs) race. */
+int ebiv_frame_rgba_noalloc(ebiv_codec *c, const ebiv_frame *f, uint8_t *dst);
+
 /* ---- Batched decode -----------------------------------------------------------
  *
  * Many streams decoded together on one device.  govid drives one Codec per
