@@ -40,7 +40,7 @@ def build_gpu() -> str:
     out = os.path.join(LIB, "libebivgpu.so")
     tmp = out + ".tmp"
     srcs = [os.path.join(CSRC, "ebiv_kernels.hip"), os.path.join(CSRC, "ebiv_runtime.cpp"),
-            os.path.join(CSRC, "ebiv_container.cpp")]
+            os.path.join(CSRC, "ebiv_host.cpp"), os.path.join(CSRC, "ebiv_container.cpp")]
     _run([HIPCC, *HIP_FLAGS, "-o", tmp, *srcs])
     os.replace(tmp, out)
     return out

@@ -42,7 +42,7 @@ constexpr bool table_place(int ctx, int* vec, int* lane0) {
     *lane0 = kLane[ctx];
     return ctx != 11;
   }
-  int t = ctx - 12, set, band, prev;
+  int t = ctx - 12, set = 0, band = 0, prev = 0;
   if (t < 45) { set = t / 15; band = (t % 15) / 3; prev = t % 3; }
   else { set = 3; band = (t - 45) / 3; prev = (t - 45) % 3; }
   *vec = 3 + 3 * set + band / 2;

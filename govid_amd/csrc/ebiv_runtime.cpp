@@ -44,6 +44,7 @@
 #include "../../include/ebiv_gpu.h"
 #include "ebiv_device.h"
 #include "ebiv_format.h"
+#include "ebiv_host.h"
 
 namespace ebiv {
 
@@ -60,7 +61,6 @@ namespace {
 struct StaticTables {
   int32_t dct[3][256];
   uint8_t zz[3][256];
-  int32_t qstep[kMaxQP + 1];
   StaticTables() {
     std::memset(dct, 0, sizeof dct);
     std::memset(zz, 0, sizeof zz);
@@ -80,7 +80,6 @@ struct StaticTables {
           dct[s][kk * n + i] = static_cast<int32_t>(std::lround(alpha * std::cos((2.0 * i + 1.0) * kk * M_PI / (2.0 * n)) * 8192.0));
       }
     }
-    for (int qp = 0; qp <= kMaxQP; qp++) qstep[qp] = std::max<int32_t>(1, static_cast<int32_t>(std::lround(std::pow(2.0, qp / 6.0))));
   }
 };
 const StaticTables& statics() {
@@ -154,24 +153,17 @@ struct PinnedArray {
   ~PinnedArray() { release(); }
 };
 
-// prevFreqs slot: Go keeps the backing array across key-frame wipes
-// (rans.go:397-405, 473-481), so a re-grown slice shows its old values.
-struct PrevFreq {
-  uint32_t v[kMaxAlphabet];
-  int len = 0;
-  bool alloc = false;
-};
-
 struct DevBuf {
   uint8_t* base = nullptr;
   size_t bytes = 0;
   Planes p{};
 };
 
-// Device scratch of one coded payload in flight: its host descriptor, the
-// entropy kernel's macroblock records and coefficients, the tile statuses.
+// Device buffers of one coded payload in flight: the entropy kernel's
+// macroblock records and coefficients, the intra residuals, the tile
+// statuses; and where the step staged its descriptor (HostPayload::desc).
 struct PayloadSlot {
-  PinnedArray<uint8_t> h_desc;
+  const uint8_t* hdesc = nullptr;  // the host descriptor (HostCodec::payloads)
   size_t desc_bytes = 0;
   size_t desc_off = 0;  // offset in the step's staging buffer
   int ntiles = 0;
@@ -183,27 +175,14 @@ struct PayloadSlot {
   size_t status_off = 0;  // first tile's entry in the step's status array
   PinnedArray<TileStatus> h_status;  // the last step's tile statuses
 };
-
 }  // namespace
 
-// One frame of a session within a step.
-struct StepFrame {
-  int rc = EBIV_OK;
-  std::string msg;
-  bool device_work = false;  // a coded payload or a raw copy runs on the device
-  bool raw = false;
-  bool key = false, inter = false, has_ov = false;
-  int ring_slot = -1;
-  int main_slot = -1, ov_slot = -1;  // payload slots
-  int ov_rc = EBIV_OK;               // host-side failure of the override payload
-  std::string ov_msg;
-  std::vector<PrevFreq> saved_prev;  // table history before the override's tables
-  const uint8_t* host = nullptr;
+// One frame of a session within a step: its host state and the device
+// buffers it reads and writes.
+struct StepFrame : HostFrame {
   const uint8_t* dev = nullptr;  // packet bytes on the device
   size_t staged_off = 0;
   bool staged = false;
-  size_t len = 0;
-  size_t body_off = 0;
   int64_t ts = 0;
   uint64_t tag = 0;
   // buffer assignment
@@ -211,21 +190,16 @@ struct StepFrame {
   int out_buf = -1;
 };
 
-struct Session {
+// One stream's codec: the host state machine (HostCodec) plus its device
+// frame pool, payload buffers and host ring.
+struct Session : HostCodec {
   int id = 0;
-  int ring_size = 3;
-  int window = 1;
-  bool ready = false;
-  int W = 0, H = 0, CW = 0, CH = 0, mbc = 0, mbr = 0;
-  int ystride = 0, cstride = 0;  // host ring image strides (geometry.go:22-30)
-  int next = 0;
+  hipStream_t stream = nullptr;  // the engine's stream (frame pool clears)
   std::vector<DevBuf> pool;
   std::vector<int> ring;  // pool index per slot, -1 = never written
   int ref = -1, golden = -1;
   std::vector<uint8_t*> host_ring;
-  PrevFreq prev[kNumContexts];
   std::vector<std::unique_ptr<PayloadSlot>> slots;
-  std::string msg;
   ebiv_stats stats{0, 0, 0};
   struct Queued {
     std::vector<uint8_t> host;
@@ -235,53 +209,52 @@ struct Session {
   };
   std::deque<Queued> queue;
 
-  ~Session() {
-    for (auto& b : pool) if (b.base) (void)hipFree(b.base);
+  ~Session() override {
+    release_pool();
     for (auto* h : host_ring) if (h) (void)hipHostFree(h);
   }
 
-  int fail(int code, const std::string& m) {
-    msg = m;
-    return code;
+  void release_pool() {
+    for (auto& b : pool) if (b.base) (void)hipFree(b.base);
+    pool.clear();
   }
 
-  // configure, codec.go:129-153.
-  int configure(int w, int h, hipStream_t stream) {
-    if (static_cast<long>(w) * h > kMaxDecodePixels)
-      return fail(EBIV_ERR_UNSUPPORTED, "ebiv: unsupported stream feature: frame exceeds the decoder's pixel bound");
-    if (ready) {
-      if (w != W || h != H) return fail(EBIV_ERR_DIMENSIONS, "ebiv: frame dimensions do not match the stream");
-      return EBIV_OK;
-    }
-    W = w; H = h; CW = (w + 1) / 2; CH = (h + 1) / 2;
-    mbc = (w + kMB - 1) / kMB;
-    mbr = (h + kMB - 1) / kMB;
-    ystride = align_up(W, 64);
-    cstride = align_up(CW, 64);
+  // The frame pool of configure (codec.go:129-153).  Live at a step's
+  // start: ring images, ref and golden.  Written by a step: one buffer per
+  // frame of the window and a golden-override target.  A failed allocation
+  // frees what it took, and the next key frame tries again.
+  int allocate_frames() override {
+    release_pool();
     const int pw = mbc * kMB, ph = mbr * kMB, cw = mbc * kCMB, ch = mbr * kCMB;
     const size_t ysz = static_cast<size_t>(pw) * ph, csz = align256(static_cast<size_t>(cw) * ch);
-    // Live at a step's start: ring images, ref and golden.  Written by a
-    // step: one buffer per frame of the window and a golden-override target.
     const size_t npool = static_cast<size_t>(ring_size) + 2 + static_cast<size_t>(window) + 1;
     pool.assign(npool, DevBuf{});
     for (auto& b : pool) {
       b.bytes = ysz + 2 * csz;
-      if (hipMalloc(&b.base, b.bytes) != hipSuccess) { b.base = nullptr; return fail(EBIV_ERR_DEVICE, "frame pool allocation failed"); }
-      if (hipMemsetAsync(b.base, 0, b.bytes, stream) != hipSuccess) return fail(EBIV_ERR_DEVICE, "frame pool clear failed");
+      if (hipMalloc(&b.base, b.bytes) != hipSuccess) {
+        b.base = nullptr;
+        release_pool();
+        return fail(EBIV_ERR_DEVICE, "frame pool allocation failed");
+      }
       b.p = Planes{b.base, b.base + ysz, b.base + ysz + csz, pw, ph, cw, ch};
+      if (hipMemsetAsync(b.base, 0, b.bytes, stream) != hipSuccess) {
+        release_pool();
+        return fail(EBIV_ERR_DEVICE, "frame pool clear failed");
+      }
     }
-    ring.assign(ring_size, -1);
+    if (hipStreamSynchronize(stream) != hipSuccess) {
+      release_pool();
+      return fail(EBIV_ERR_DEVICE, "frame pool clear failed");
+    }
+    ring.assign(static_cast<size_t>(ring_size), -1);
     ref = 0;
     golden = 1;
-    if (hipStreamSynchronize(stream) != hipSuccess) return fail(EBIV_ERR_DEVICE, "frame pool clear failed");
-    next = 0;
-    ready = true;
     return EBIV_OK;
   }
 
   int ensure_host_ring() {
     if (!host_ring.empty()) return EBIV_OK;
-    host_ring.assign(ring_size, nullptr);
+    host_ring.assign(static_cast<size_t>(ring_size), nullptr);
     const size_t hsz = static_cast<size_t>(ystride) * H + 2 * static_cast<size_t>(cstride) * CH;
     for (auto*& hr : host_ring) {
       if (hipHostMalloc(reinterpret_cast<void**>(&hr), hsz, hipHostMallocDefault) != hipSuccess) {
@@ -295,179 +268,24 @@ struct Session {
 
   PayloadSlot* slot(int i) {
     while (static_cast<int>(slots.size()) <= i) slots.emplace_back(new PayloadSlot());
-    return slots[i].get();
+    return slots[static_cast<size_t>(i)].get();
   }
 
-  // parseTablesInto (rans.go:396-471) into the frame header's cumulative
-  // tables, updating prev exactly as Go does (including on failure).
-  int parse_tables(const uint8_t* b, size_t len, bool key, FrameHdr& hdr, size_t* consumed) {
-    if (key) for (auto& p : prev) p.len = 0;
-    size_t pos = 0;
-    hdr.used[0] = hdr.used[1] = hdr.used[2] = 0;
-    auto install = [&](int c, const uint32_t* f, int n) -> bool {
-      uint32_t sum = 0;
-      for (int s = 0; s < n; s++) sum += f[s];
-      if (sum != kRansM) return false;
-      // Go indexes slot2sym with start+k; an entry above ransM whose wrapped
-      // sum still reaches ransM would panic there.  Such a stream is corrupt.
-      for (int s = 0; s < n; s++) if (f[s] > kRansM) return false;
-      int vec = 0, lane0 = 0;
-      if (table_place(c, &vec, &lane0)) {
-        uint32_t start = 0;
-        for (int s = 0; s < n; s++) {
-          hdr.vt[vec][lane0 + s] = start << 13 | f[s];
-          start += f[s];
-        }
-      }
-      hdr.used[c >> 5] |= 1u << (c & 31);
-      return true;
-    };
-    for (int c = 0; c < kNumContexts; c++) {
-      const int n = alphabet_size(c);
-      if (pos >= len) return fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: truncated table modes");
-      const uint8_t mode = b[pos++];
-      if (c == kCtxBypass && mode != 0) return fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: bypass context must not ship a table");
-      const bool have_ref = prev[c].len == n;
-      PrevFreq& p = prev[c];
-      switch (mode) {
-        case 0:
-          break;
-        case 1:
-          if (key || !have_ref) return fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: context references a previous table that does not exist");
-          if (!install(c, p.v, n)) return fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: context frequencies do not sum to 4096");
-          break;
-        case 2: {
-          if (!p.alloc) { std::memset(p.v, 0, sizeof p.v); p.alloc = true; }
-          p.len = n;
-          for (int s = 0; s < n; s++) {
-            uint64_t v = 0;
-            const int k = uvarint(b + pos, len - pos, &v);
-            if (k <= 0) return fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: truncated frequency table body");
-            pos += static_cast<size_t>(k);
-            p.v[s] = static_cast<uint32_t>(v);
-          }
-          if (!install(c, p.v, n)) return fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: context frequencies do not sum to 4096");
-          break;
-        }
-        case 3:
-          if (key || !have_ref) return fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: context deltas a previous table that does not exist");
-          for (int s = 0; s < n; s++) {
-            int64_t v = 0;
-            const int k = varint(b + pos, len - pos, &v);
-            if (k <= 0) return fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: truncated frequency table delta");
-            pos += static_cast<size_t>(k);
-            const int64_t nv = static_cast<int64_t>(p.v[s]) + v;
-            if (nv < 0 || nv > static_cast<int64_t>(kRansM)) return fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: context delta out of range");
-            p.v[s] = static_cast<uint32_t>(nv);
-          }
-          if (!install(c, p.v, n)) return fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: context frequencies do not sum to 4096");
-          break;
-        default:
-          return fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: unknown table mode");
-      }
-    }
-    const uint32_t bypass[2] = {kRansM / 2, kRansM / 2};  // installBypass
-    install(kCtxBypass, bypass, 2);
-    *consumed = pos;
-    return EBIV_OK;
-  }
-
-  // decodeState.decodeCoded up to the tile fan-out (decode_intra.go:55-64,
-  // frame.go:197-274): fills payload slot `si`'s descriptor.  body is a
-  // sub-range of the packet starting at pkt_base.
-  int prepare_coded(const uint8_t* pkt_base, const uint8_t* body, size_t len, bool inter, bool key, int si) {
-    if (len < 1) return fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: coded header truncated");
-    const int qp = body[0];
-    size_t pos = 1;
-    uint64_t cols = 0, rows = 0, tlen = 0;
-    int n = uvarint(body + pos, len - pos, &cols);
-    if (n <= 0) return fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: coded header tile columns");
-    pos += static_cast<size_t>(n);
-    n = uvarint(body + pos, len - pos, &rows);
-    if (n <= 0) return fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: coded header tile rows");
-    pos += static_cast<size_t>(n);
-    if (qp > kMaxQP || cols == 0 || rows == 0 || cols > 65535 || rows > 65535)
-      return fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: coded header values out of range");
-    n = uvarint(body + pos, len - pos, &tlen);
-    if (n <= 0 || tlen > static_cast<uint64_t>(len - pos - static_cast<size_t>(n)))
-      return fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: bad table length");
-    pos += static_cast<size_t>(n);
-
-    const int gc = static_cast<int>(std::min<uint64_t>(cols, static_cast<uint64_t>(mbc)));
-    const int gr = static_cast<int>(std::min<uint64_t>(rows, static_cast<uint64_t>(mbr)));
-    const int nt = gc * gr;
-    const size_t dsize = desc_size(nt, mbc, mbr);
+  // Device buffers of payload slot si, sized for this geometry.
+  int reserve_payload(int si) override {
     PayloadSlot* ps = slot(si);
-    if (!ps->h_desc.reserve(dsize)) return fail(EBIV_ERR_DEVICE, "descriptor allocation failed");
-    uint8_t* hd = ps->h_desc.h;
-    std::memset(hd, 0, dsize);
-    FrameHdr& hdr = *reinterpret_cast<FrameHdr*>(hd);
-    // A context no table installs reads as kNoSym on every lane of the
-    // entropy kernel's table search, which finds no symbol there: the
-    // "symbol decoded from unused context" error of rans.go:275-278.
-    std::memset(hdr.vt, 0xFF, sizeof hdr.vt);
-    size_t consumed = 0;
-    int rc = parse_tables(body + pos, static_cast<size_t>(tlen), key, hdr, &consumed);
-    if (rc) return rc;
-    if (consumed != tlen) return fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: trailing table bytes");
-    pos += static_cast<size_t>(tlen);
-
-    const uint64_t want = cols * rows;
-    std::vector<std::pair<size_t, size_t>> tiles;
-    while (tiles.size() < want) {
-      uint64_t l = 0;
-      n = uvarint(body + pos, len - pos, &l);
-      if (n <= 0 || l > static_cast<uint64_t>(len - pos - static_cast<size_t>(n)))
-        return fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: bad tile length");
-      pos += static_cast<size_t>(n);
-      tiles.emplace_back(static_cast<size_t>(body + pos - pkt_base), static_cast<size_t>(l));
-      pos += static_cast<size_t>(l);
-    }
-    if (static_cast<uint64_t>(nt) != tiles.size()) return fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream");
-
-    hdr.mbc = mbc;
-    hdr.mbr = mbr;
-    hdr.ntiles = nt;
-    hdr.inter = inter ? 1 : 0;
-    hdr.qac = statics().qstep[qp];
-    hdr.qdc = qp >= 6 ? statics().qstep[qp - 6] : statics().qstep[qp];
-    hdr.grid_cols = gc;
-    hdr.grid_rows = gr;
-    TileDesc* td = reinterpret_cast<TileDesc*>(hd + desc_tiles_off());
-    uint16_t* ct = reinterpret_cast<uint16_t*>(hd + desc_coltile_off(nt));
-    uint16_t* rt = reinterpret_cast<uint16_t*>(hd + desc_rowtile_off(nt, mbc));
-    uint32_t coef = 0;
-    int maxw = 1;
-    for (int i = 0; i < nt; i++) {  // tileGrid.bounds, frame.go:170-177
-      const int tc = i % gc, tr = i / gc;
-      TileDesc& t = td[i];
-      t.byte_off = static_cast<uint32_t>(tiles[i].first);
-      t.byte_len = static_cast<uint32_t>(tiles[i].second);
-      t.x0 = static_cast<uint16_t>(mbc * tc / gc);
-      t.x1 = static_cast<uint16_t>(mbc * (tc + 1) / gc);
-      t.y0 = static_cast<uint16_t>(mbr * tr / gr);
-      t.y1 = static_cast<uint16_t>(mbr * (tr + 1) / gr);
-      t.coef_base = coef;
-      t.pad = 0;
-      coef += static_cast<uint32_t>((t.x1 - t.x0) * (t.y1 - t.y0) * kCoefPerMB);
-      maxw = std::max(maxw, t.x1 - t.x0);
-    }
-    for (int tc = 0; tc < gc; tc++)
-      for (int x = mbc * tc / gc; x < mbc * (tc + 1) / gc; x++) ct[x] = static_cast<uint16_t>(tc);
-    for (int tr = 0; tr < gr; tr++)
-      for (int y = mbr * tr / gr; y < mbr * (tr + 1) / gr; y++) rt[y] = static_cast<uint16_t>(tr);
-    hdr.max_tile_w = maxw;
-    ps->ntiles = nt;
-    ps->max_tile_w = maxw;
-    ps->desc_bytes = dsize;
-    ps->inter = inter;
+    const HostPayload& hp = payloads[static_cast<size_t>(si)];
+    ps->hdesc = hp.desc.data();
+    ps->desc_bytes = hp.desc.size();
+    ps->ntiles = hp.ntiles;
+    ps->max_tile_w = hp.max_tile_w;
+    ps->inter = hp.inter;
     const size_t nmb = static_cast<size_t>(mbc) * mbr;
     if (!ps->recs.reserve(nmb) || !ps->coefs.reserve(nmb * kCoefPerMB) || !ps->res.reserve(nmb * kResPerMB) ||
-        !ps->h_status.reserve(nt))
+        !ps->h_status.reserve(static_cast<size_t>(hp.ntiles)))
       return fail(EBIV_ERR_DEVICE, "payload buffer allocation failed");
     return EBIV_OK;
   }
-
   // First failing tile in tile order (decode_intra.go:86-93).
   int tile_result(const PayloadSlot* ps) {
     for (int i = 0; i < ps->ntiles; i++) {
@@ -487,80 +305,6 @@ struct Session {
     return EBIV_OK;
   }
 
-  // Host half of Codec.Decode up to the device work (codec.go:70-100,
-  // 162-173): header, configure, ring slot, raw check, override split and
-  // payload preparation.  Payload slots are numbered from *next_slot.
-  // Returns false when the frame must close the session's window for this
-  // step: the override's tables are parsed only if its key frame decodes.
-  bool prepare_frame(StepFrame& f, int* next_slot, hipStream_t stream) {
-    const bool more = prepare_frame_inner(f, next_slot, stream);
-    if (f.rc && f.msg.empty()) f.msg = msg;
-    return more;
-  }
-
-  bool prepare_frame_inner(StepFrame& f, int* next_slot, hipStream_t stream) {
-    const uint8_t* data = f.host;
-    const size_t len = f.len;
-    if (len < 1) { f.rc = fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: empty frame payload"); return true; }
-    const int type = data[0] & 3, coding = (data[0] >> 2) & 7;
-    const bool gov = ((data[0] >> 5) & 1) != 0;
-    if (type != 0 && type != 1) { f.rc = fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: frame type"); return true; }
-    if (gov && type != 0) { f.rc = fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: golden override on a non-key frame"); return true; }
-    size_t hlen = 1;
-    if (type == 0) {
-      if (len < 6) { f.rc = fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: key frame header truncated"); return true; }
-      const int w = data[1] | data[2] << 8, h = data[3] | data[4] << 8;
-      if (w == 0 || h == 0) { f.rc = fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: key frame dimensions"); return true; }
-      if ((data[5] >> 4) != 0) { f.rc = fail(EBIV_ERR_UNSUPPORTED, "ebiv: unsupported stream feature: chroma format"); return true; }
-      if ((data[5] & 15) != 0) { f.rc = fail(EBIV_ERR_UNSUPPORTED, "ebiv: unsupported stream feature: bit depth code"); return true; }
-      hlen = 6;
-      if ((f.rc = configure(w, h, stream))) return true;
-    } else if (!ready) {
-      f.rc = fail(EBIV_ERR_NOKEYFRAME, "ebiv: inter frame before any key frame");
-      return true;
-    }
-    f.ring_slot = next;
-    if (++next == ring_size) next = 0;
-    f.key = type == 0;
-    if (coding == 0) {  // decodeRaw, codec.go:195-207
-      const size_t want = static_cast<size_t>(W) * H + 2 * static_cast<size_t>(CW) * CH;
-      if (len - hlen != want) { f.rc = fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: raw frame size"); return true; }
-      f.raw = true;
-      f.device_work = true;
-      f.body_off = hlen;
-      return true;
-    }
-    if (coding != 1 && coding != 2) { f.rc = fail(EBIV_ERR_UNSUPPORTED, "ebiv: unsupported stream feature: coding mode"); return true; }
-    f.inter = coding == 2;
-    const uint8_t* body = data + hlen;
-    size_t blen = len - hlen;
-    const uint8_t* ov = nullptr;
-    size_t ovlen = 0;
-    if (gov) {
-      uint64_t kl = 0;
-      const int k = uvarint(body, blen, &kl);
-      if (k <= 0 || kl > static_cast<uint64_t>(blen - static_cast<size_t>(k))) {
-        f.rc = fail(EBIV_ERR_CORRUPT, "ebiv: corrupt stream: golden override split");
-        return true;
-      }
-      ov = body + k + kl;
-      ovlen = blen - static_cast<size_t>(k) - static_cast<size_t>(kl);
-      body += k;
-      blen = static_cast<size_t>(kl);
-    }
-    if ((f.rc = prepare_coded(data, body, blen, f.inter, f.key, *next_slot))) return true;
-    f.main_slot = (*next_slot)++;
-    f.device_work = true;
-    if (f.key && ov) {
-      f.has_ov = true;
-      f.saved_prev.assign(prev, prev + kNumContexts);
-      f.ov_rc = prepare_coded(data, ov, ovlen, true, false, *next_slot);
-      if (f.ov_rc) f.ov_msg = msg;
-      else f.ov_slot = (*next_slot)++;
-      return false;
-    }
-    return true;
-  }
 };
 
 namespace {
@@ -835,7 +579,7 @@ int Engine::run_step(std::vector<Session*>& ss, std::vector<std::vector<StepFram
         for (int sl : {f.main_slot, f.ov_slot}) {
           if (sl < 0) continue;
           PayloadSlot* ps = ss[i]->slot(sl);
-          std::memcpy(h_stage.h + ps->desc_off, ps->h_desc.h, ps->desc_bytes);
+          std::memcpy(h_stage.h + ps->desc_off, ps->hdesc, ps->desc_bytes);
         }
       }
     if (hipMemcpyAsync(d_stage.d, h_stage.h, stage, hipMemcpyHostToDevice, stream) != hipSuccess)
@@ -913,7 +657,7 @@ int Engine::run_step(std::vector<Session*>& ss, std::vector<std::vector<StepFram
   int max_tw = 1;
   uint32_t max_tile_bytes = 1;
   auto tiles_of = [&](const PayloadSlot* ps) {
-    return reinterpret_cast<const TileDesc*>(ps->h_desc.h + desc_tiles_off());
+    return reinterpret_cast<const TileDesc*>(ps->hdesc + desc_tiles_off());
   };
   for (const JobRef& jr : jrefs) {
     PayloadSlot* ps = payload_of(ss, frames, jr);
@@ -1119,27 +863,14 @@ int Engine::run_step(std::vector<Session*>& ss, std::vector<std::vector<StepFram
     for (size_t k = 0; k < frames[i].size(); k++) {
       StepFrame& f = frames[i][k];
       if (f.rc) continue;  // failed on the host: nothing ran on the device
-      if (f.raw) {
-        s.ring[f.ring_slot] = f.target;
-        f.out_buf = f.target;
-        s.stats.frames++;
+      const int st = s.resolve(f, okm[i][k] != 0, oko[i][k] != 0);
+      if (f.golden_written) s.golden = f.has_ov ? f.gtarget : f.target;
+      if (st) {
+        f.rc = st;
         continue;
       }
-      if (!okm[i][k]) {
-        f.rc = EBIV_ERR_CORRUPT;
-        // Go never reached the override, so never parsed its tables.
-        if (f.has_ov) std::copy(f.saved_prev.begin(), f.saved_prev.end(), s.prev);
-        continue;
-      }
-      if (f.has_ov) {
-        if (f.ov_rc) { f.rc = f.ov_rc; f.msg = f.ov_msg; continue; }
-        s.golden = f.gtarget;
-        if (!oko[i][k]) { f.rc = EBIV_ERR_CORRUPT; f.msg = f.ov_msg; continue; }
-      } else if (f.key) {
-        s.golden = f.target;
-      }
-      s.ring[f.ring_slot] = f.target;
-      s.ref = f.target;
+      s.ring[static_cast<size_t>(f.ring_slot)] = f.target;
+      if (!f.raw) s.ref = f.target;
       f.out_buf = f.target;
       s.stats.frames++;
     }
@@ -1221,7 +952,7 @@ int decode_one(ebiv_codec* c, const uint8_t* data, const uint8_t* dev, size_t le
   f.len = len;
   f.ts = ts;
   int next_slot = 0;
-  s.prepare_frame(f, &next_slot, e.stream);
+  s.prepare_frame(f, &next_slot);
   if (f.rc) return s.fail(f.rc, f.msg);
   if (int rc = e.run_step(ss, frames)) return s.fail(rc, e.msg);
   if (f.rc) return s.fail(f.rc, f.msg);
@@ -1248,6 +979,7 @@ int ebiv_codec_create(int ring_size, int device, ebiv_codec** out) {
   c->s = c->eng.sessions.back().get();
   c->s->ring_size = ring_size >= 2 ? ring_size : 3;
   c->s->window = 1;
+  c->s->stream = c->eng.stream;
   *out = c;
   return EBIV_OK;
 }
@@ -1398,6 +1130,7 @@ int ebiv_batch_open(ebiv_batch* b, int ring_size, int window, int* session) {
   s->window = window;
   // every frame a step returns stays valid until the next step returns
   s->ring_size = std::max(ring_size >= 2 ? ring_size : 3, window);
+  s->stream = b->eng.stream;
   *session = s->id;
   b->eng.sessions.push_back(std::move(s));
   return EBIV_OK;
@@ -1505,7 +1238,7 @@ int ebiv_batch_step(ebiv_batch* b, ebiv_completion* out, int max_out, int* n_out
       f.len = q.host.size();
       f.ts = q.ts;
       f.tag = q.tag;
-      if (!s.prepare_frame(f, &next_slot, e.stream)) break;
+      if (!s.prepare_frame(f, &next_slot)) break;
     }
     for (size_t j = h.size(); j > k; j--) s.queue.push_front(std::move(h[j - 1]));
     h.resize(k);

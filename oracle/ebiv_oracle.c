@@ -858,6 +858,8 @@ struct orc_codec {
     int trace_on;
     orc_mbtrace *trace;
     int ntrace;
+    int in_tiles;  /* the last decode_coded_into failed in its tiles */
+    int tile_fail; /* last decode: 0, 1 its own payload's tiles failed, 2 the override's */
 };
 
 static int fail_msg(orc_codec *c, int code, const char *m)
@@ -1075,6 +1077,7 @@ static int decode_coded_into(orc_codec *c, fbuf *rec, fbuf *ref, fbuf *golden, c
         c->nsym += j.nsym[i];
         if (!rc && j.errs[i]) rc = fail_msg(c, j.errs[i], j.msgs[i] ? j.msgs[i] : "corrupt tile");
     }
+    c->in_tiles = rc != 0;
     free(j.errs); free(j.msgs); free(j.nsym); free(tb); free(tp); free(tl);
     return rc;
 }
@@ -1092,8 +1095,12 @@ static void fill_frame(orc_codec *c, uint8_t *img, orc_frame *out)
     out->width = c->W; out->height = c->H; out->cwidth = c->CW; out->cheight = c->CH;
 }
 
+int orc_codec_last_tile_failure(const orc_codec *c) { return c->tile_fail; }
+
 int orc_codec_decode(orc_codec *c, const uint8_t *data, size_t len, orc_frame *out)
 {
+    c->tile_fail = 0;
+    c->in_tiles = 0;
     /* parseFrameHeader, format.go:376-409 */
     if (len < 1) return fail_msg(c, ORC_ERR_CORRUPT, "empty frame payload");
     int type = data[0] & 3, coding = (data[0] >> 2) & 7, gov = (data[0] >> 5) & 1;
@@ -1146,11 +1153,18 @@ int orc_codec_decode(orc_codec *c, const uint8_t *data, size_t len, orc_frame *o
         blen = (size_t)kl;
     }
     int rc = decode_coded_into(c, c->work, c->ref, c->golden, body, blen, inter, key);
-    if (rc) return rc;
+    if (rc) {
+        c->tile_fail = c->in_tiles ? 1 : 0;
+        return rc;
+    }
     if (key) {
         if (ov) {
+            c->in_tiles = 0;
             rc = decode_coded_into(c, c->golden, c->work, c->work, ov, ovlen, 1, 0);
-            if (rc) return rc;
+            if (rc) {
+                c->tile_fail = c->in_tiles ? 2 : 0;
+                return rc;
+            }
         } else {
             fbuf *g = c->golden, *w = c->work; /* copyFrom, frame.go:42-46 */
             memcpy(g->y.data, w->y.data, (size_t)w->y.stride * w->y.h);

@@ -48,6 +48,9 @@ int orc_codec_decode(orc_codec *c, const uint8_t *data, size_t len, orc_frame *o
 void orc_codec_flush(orc_codec *c);
 /* Message of the last failed decode. */
 const char *orc_codec_last_error(const orc_codec *c);
+/* Where the last decode failed: 0 not in tiles (or no failure), 1 in the
+ * frame's own payload's tiles, 2 in its golden override's tiles. */
+int orc_codec_last_tile_failure(const orc_codec *c);
 /* Worker threads for tile decode (ebiv/pool.go: NumCPU workers; 1 = inline). */
 void orc_codec_set_threads(orc_codec *c, int threads);
 /* Number of rANS decode calls (bypass bits included) made by this codec. */

@@ -52,6 +52,8 @@ def lib():
         L.orc_codec_decode.restype = C.c_int
         L.orc_codec_decode.argtypes = [vp, C.c_char_p, C.c_size_t, C.POINTER(_Frame)]
         L.orc_codec_flush.argtypes = [vp]
+        L.orc_codec_last_tile_failure.restype = C.c_int
+        L.orc_codec_last_tile_failure.argtypes = [vp]
         L.orc_codec_last_error.restype = C.c_char_p
         L.orc_codec_last_error.argtypes = [vp]
         L.orc_codec_set_threads.argtypes = [vp, C.c_int]
@@ -136,6 +138,11 @@ class OracleCodec:
 
     def flush(self):
         self._L.orc_codec_flush(self._h)
+
+    def last_tile_failure(self) -> int:
+        """0: the last decode did not fail in a tile; 1: its own payload's
+        tiles failed; 2: its golden override's tiles failed."""
+        return int(self._L.orc_codec_last_tile_failure(self._h))
 
     def enable_trace(self, on: bool = True):
         self._L.orc_codec_enable_trace(self._h, int(on))
