@@ -23,7 +23,7 @@ EXPORTS = (
     "ebiv_batch_create", "ebiv_batch_destroy", "ebiv_batch_last_error", "ebiv_batch_open", "ebiv_batch_close",
     "ebiv_batch_submit", "ebiv_batch_flush", "ebiv_batch_pending", "ebiv_batch_step", "ebiv_batch_stats",
     "ebiv_batch_set_timing", "ebiv_batch_kernel_times", "ebiv_batch_frame_rgba", "ebiv_session_bytes",
-    "ebiv_batch_set_schedule", "ebiv_batch_get_schedule", "ebiv_batch_submit_many",
+    "ebiv_batch_set_schedule", "ebiv_batch_get_schedule", "ebiv_batch_submit_many", "ebiv_batch_set_surfaces",
     "ebiv_reader_open", "ebiv_reader_destroy", "ebiv_reader_header", "ebiv_reader_frame_count", "ebiv_reader_index",
     "ebiv_reader_index_scanned", "ebiv_reader_duration", "ebiv_reader_next_packet", "ebiv_reader_seek",
     "ebiv_reader_close", "ebiv_frame_time", "ebiv_frame_index_at",
@@ -57,7 +57,7 @@ class Frame(C.Structure):
 
 class Completion(C.Structure):
     _fields_ = [("session", C.c_int32), ("status", C.c_int32), ("tag", C.c_uint64), ("frame", Frame),
-                ("message", C.c_char * 96)]
+                ("message", C.c_char * 96), ("surface", C.c_int32), ("reserved", C.c_int32)]
 
 
 class KernelTimes(C.Structure):
@@ -130,6 +130,8 @@ def bind(L):
     L.ebiv_batch_set_timing.argtypes = [vp, C.c_int]
     L.ebiv_batch_kernel_times.restype = C.c_int
     L.ebiv_batch_kernel_times.argtypes = [vp, C.POINTER(KernelTimes)]
+    L.ebiv_batch_set_surfaces.restype = C.c_int
+    L.ebiv_batch_set_surfaces.argtypes = [vp, C.c_int, C.POINTER(vp), C.c_int, sz]
     L.ebiv_batch_set_schedule.restype = C.c_int
     L.ebiv_batch_set_schedule.argtypes = [vp, C.POINTER(Schedule)]
     L.ebiv_batch_get_schedule.restype = C.c_int

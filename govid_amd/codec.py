@@ -237,10 +237,11 @@ class Completion:
     """One decoded (or failed) packet of a batch step.  ``frame`` (None on
     failure) and ``error`` are built on first access."""
 
-    __slots__ = ("session", "tag", "status", "_batch", "_raw", "_msg", "_frame")
+    __slots__ = ("session", "tag", "status", "surface", "_batch", "_raw", "_msg", "_frame")
 
-    def __init__(self, session: int, tag: int, status: int, batch, raw: bytes | None, msg: bytes = b""):
-        self.session, self.tag, self.status = session, tag, status
+    def __init__(self, session: int, tag: int, status: int, batch, raw: bytes | None, msg: bytes = b"",
+                 surface: int = -1):
+        self.session, self.tag, self.status, self.surface = session, tag, status, surface
         self._batch, self._raw, self._msg, self._frame = batch, raw, msg, None
 
     @property
@@ -338,17 +339,28 @@ class Batch:
         # reused by the next step)
         size, fo, fs = C.sizeof(capi.Completion), capi.Completion.frame.offset, C.sizeof(capi.Frame)
         mo, ms = capi.Completion.message.offset, capi.Completion.message.size
+        so = capi.Completion.surface.offset
         raw = C.string_at(C.addressof(self._out), size * n.value)
         head = struct.Struct("<iiQ")
+        surf = struct.Struct("<i")
         res = []
         for i in range(n.value):
             o = i * size
             session, status, tag = head.unpack_from(raw, o)
             if status == capi.OK:
-                res.append(Completion(session, tag, 0, self, raw[o + fo:o + fo + fs]))
+                res.append(Completion(session, tag, 0, self, raw[o + fo:o + fo + fs], b"",
+                                      surf.unpack_from(raw, o + so)[0]))
             else:
                 res.append(Completion(session, tag, status, self, None, raw[o + mo:o + mo + ms]))
         return res
+
+    def set_surfaces(self, session: int, ptrs, pitch: int) -> None:
+        """Presentation: the session's frames are also written as RGBA into
+        these device surfaces (rows of `pitch` bytes), in turn; a
+        completion's ``surface`` names the one its frame went to."""
+        arr = (C.c_void_p * max(len(ptrs), 1))(*ptrs)
+        if self._L.ebiv_batch_set_surfaces(self._h, session, arr, len(ptrs), pitch):
+            raise DeviceError("ebiv_batch_set_surfaces failed")
 
     def stats(self, session: int = 0) -> dict:
         s = capi.Stats()

@@ -146,6 +146,17 @@ struct WorkItem {
   uint32_t idx;
 };
 
+// One YCbCr 4:2:0 -> RGBA conversion (frame.go:8-67): a decoded frame's
+// padded planes in, w x h RGBA pixels out at dst with `pitch` bytes per row.
+struct RgbaJob {
+  const uint8_t* y;
+  const uint8_t* cb;
+  const uint8_t* cr;
+  uint8_t* dst;
+  int64_t pitch;
+  int32_t ys, cs, w, h;
+};
+
 // Error detail codes of TileStatus::err >> 8 (low byte is the Status).
 enum TileErr { kTileOK = 0, kTileShort = 1, kTileUnused = 2, kTileUnderrun = 3 };
 

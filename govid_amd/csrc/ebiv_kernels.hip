@@ -3397,32 +3397,53 @@ __device__ __forceinline__ uint32_t rgba_px(int32_t yv, int32_t cbv, int32_t crv
          0xFF000000u;
 }
 
-__global__ __launch_bounds__(256) void ebiv_rgba(const uint8_t* __restrict__ y, int ys, const uint8_t* __restrict__ cb,
-                                                 const uint8_t* __restrict__ cr, int cs, int w, int h,
-                                                 uint32_t* __restrict__ out) {
-  const int qw = (w + 3) >> 2;
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= qw * h) return;
-  const int row = idx / qw, q = idx - row * qw;
-  const int col = q * 4;
-  const uint8_t* yr = y + static_cast<size_t>(row) * ys;
-  const uint8_t* br = cb + static_cast<size_t>(row >> 1) * cs;
-  const uint8_t* rr = cr + static_cast<size_t>(row >> 1) * cs;
-  uint32_t* o = out + static_cast<size_t>(row) * w + col;
-  if (col + 4 <= w) {
-    uint4 v;
-    v.x = rgba_px(yr[col], br[col >> 1], rr[col >> 1]);
-    v.y = rgba_px(yr[col + 1], br[col >> 1], rr[col >> 1]);
-    v.z = rgba_px(yr[col + 2], br[(col + 2) >> 1], rr[(col + 2) >> 1]);
-    v.w = rgba_px(yr[col + 3], br[(col + 2) >> 1], rr[(col + 2) >> 1]);
-    if ((reinterpret_cast<uintptr_t>(o) & 15) == 0) {
-      *reinterpret_cast<uint4*>(o) = v;
-    } else {
-      o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
-    }
-  } else {
-    for (int k = 0; col + k < w; k++) o[k] = rgba_px(yr[col + k], br[(col + k) >> 1], rr[(col + k) >> 1]);
+// Eight pixels of one row per thread: Y as one 8-byte load and Cb, Cr as one
+// dword each (the decoder's padded planes: rows of a multiple of 16 luma and
+// 8 chroma bytes, so these loads are aligned and stay inside the plane), and
+// two 16-byte stores where the destination row allows, else dword or byte
+// stores.  HBM-bound: 1.5 bytes read and 4 written per pixel.
+__device__ __forceinline__ void rgba_span(const uint8_t* __restrict__ y, int ys, const uint8_t* __restrict__ cb,
+                                          const uint8_t* __restrict__ cr, int cs, int w, int h, uint8_t* __restrict__ dst,
+                                          int64_t pitch, uint32_t idx) {
+  const uint32_t q8 = static_cast<uint32_t>(w + 7) >> 3;
+  if (idx >= q8 * static_cast<uint32_t>(h)) return;
+  const uint32_t row = idx / q8;
+  const int col = static_cast<int>(idx - row * q8) * 8;
+  const uint2 yv = *reinterpret_cast<const uint2*>(y + static_cast<size_t>(row) * ys + col);
+  const size_t co = static_cast<size_t>(row >> 1) * cs + (col >> 1);
+  const uint32_t bv = *reinterpret_cast<const uint32_t*>(cb + co);
+  const uint32_t rv = *reinterpret_cast<const uint32_t*>(cr + co);
+  uint32_t px[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const uint32_t yb = ((k < 4 ? yv.x : yv.y) >> (8 * (k & 3))) & 0xFFu;
+    px[k] = rgba_px(static_cast<int32_t>(yb), static_cast<int32_t>((bv >> (8 * (k >> 1))) & 0xFFu),
+                    static_cast<int32_t>((rv >> (8 * (k >> 1))) & 0xFFu));
   }
+  uint8_t* o = dst + static_cast<int64_t>(row) * pitch + static_cast<int64_t>(col) * 4;
+  const int n = w - col < 8 ? w - col : 8;
+  if (n == 8 && (reinterpret_cast<uintptr_t>(o) & 15) == 0) {
+    reinterpret_cast<uint4*>(o)[0] = make_uint4(px[0], px[1], px[2], px[3]);
+    reinterpret_cast<uint4*>(o)[1] = make_uint4(px[4], px[5], px[6], px[7]);
+  } else if ((reinterpret_cast<uintptr_t>(o) & 3) == 0) {
+    for (int k = 0; k < n; k++) reinterpret_cast<uint32_t*>(o)[k] = px[k];
+  } else {
+    for (int k = 0; k < n; k++)
+      for (int b = 0; b < 4; b++) o[4 * k + b] = static_cast<uint8_t>(px[k] >> (8 * b));
+  }
+}
+
+// One frame, its parameters as kernel arguments.
+__global__ __launch_bounds__(256) void ebiv_rgba(RgbaJob j) {
+  rgba_span(j.y, j.ys, j.cb, j.cr, j.cs, j.w, j.h, j.dst, j.pitch, blockIdx.x * 256u + threadIdx.x);
+}
+
+// Many frames in one launch (the batch step's presentation): blockIdx.y is
+// the frame, read from a job table through the constant address space.
+using ConstRgbaJob = __attribute__((address_space(4))) const RgbaJob;
+__global__ __launch_bounds__(256) void ebiv_rgba_frames(const RgbaJob* __restrict__ jobs) {
+  const ConstRgbaJob& j = ((ConstRgbaJob*)jobs)[blockIdx.y];
+  rgba_span(j.y, j.ys, j.cb, j.cr, j.cs, j.w, j.h, j.dst, j.pitch, blockIdx.x * 256u + threadIdx.x);
 }
 
 }  // namespace ebiv
@@ -3453,10 +3474,14 @@ void launch_recon_intra(hipStream_t s, int nitems, const FrameJob* jobs, const W
   hipLaunchKernelGGL(ebiv_recon_intra, dim3(nitems), dim3(64 * waves), lds, s, jobs, items, key ? 1 : 0);
 }
 
-void launch_rgba(hipStream_t s, const uint8_t* y, int ys, const uint8_t* cb, const uint8_t* cr, int cs, int w, int h,
-                 uint32_t* out) {
-  int n = ((w + 3) / 4) * h;
-  hipLaunchKernelGGL(ebiv_rgba, dim3((n + 255) / 256), dim3(256), 0, s, y, ys, cb, cr, cs, w, h, out);
+void launch_rgba(hipStream_t s, const RgbaJob& j) {
+  const uint32_t n = static_cast<uint32_t>((j.w + 7) / 8) * static_cast<uint32_t>(j.h);
+  hipLaunchKernelGGL(ebiv_rgba, dim3((n + 255) / 256), dim3(256), 0, s, j);
+}
+
+// max_px8: the most 8-pixel spans of any frame in the table.
+void launch_rgba_frames(hipStream_t s, const RgbaJob* jobs, int njobs, uint32_t max_px8) {
+  hipLaunchKernelGGL(ebiv_rgba_frames, dim3((max_px8 + 255) / 256, njobs), dim3(256), 0, s, jobs);
 }
 
 }  // namespace ebiv

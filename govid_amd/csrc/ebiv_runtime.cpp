@@ -52,8 +52,8 @@ hipError_t upload_tables(const int32_t dct[3][256], const uint8_t zz[3][256]);
 void launch_parse(hipStream_t s, int nitems, int max_tile_w, const FrameJob* jobs, const WorkItem* items);
 void launch_recon_inter(hipStream_t s, int ngroups, const FrameJob* jobs, const WorkItem* items, int njobs);
 void launch_recon_intra(hipStream_t s, int nitems, const FrameJob* jobs, const WorkItem* items, int waves, bool key);
-void launch_rgba(hipStream_t s, const uint8_t* y, int ys, const uint8_t* cb, const uint8_t* cr, int cs, int w, int h,
-                 uint32_t* out);
+void launch_rgba(hipStream_t s, const RgbaJob& j);
+void launch_rgba_frames(hipStream_t s, const RgbaJob* jobs, int njobs, uint32_t max_px8);
 
 namespace {
 
@@ -201,6 +201,10 @@ struct Session : HostCodec {
   std::vector<uint8_t*> host_ring;
   std::vector<std::unique_ptr<PayloadSlot>> slots;
   ebiv_stats stats{0, 0, 0};
+  // presentation surfaces (ebiv_batch_set_surfaces)
+  std::vector<uint8_t*> surfaces;
+  size_t surface_pitch = 0;
+  int next_surface = 0;
   struct Queued {
     std::vector<uint8_t> host;
     const uint8_t* dev = nullptr;
@@ -441,6 +445,7 @@ struct Engine {
   hipStream_t stream = nullptr;
   std::vector<std::unique_ptr<Session>> sessions;
   int next_id = 1;
+  size_t rotate = 0;  // first session served by the next batch step
   PinnedArray<uint8_t> h_stage;  // descriptors and host packets of a step
   DevArray<uint8_t> d_stage;
   PinnedArray<FrameJob> h_jobs;
@@ -451,6 +456,8 @@ struct Engine {
   DevArray<TileStatus> d_step_status;
   PinnedArray<TileStatus> h_step_status;
   DevArray<uint32_t> d_rgba;
+  PinnedArray<RgbaJob> h_present;  // a step's presentation conversions
+  DevArray<RgbaJob> d_present;
   std::string msg;
   bool timing = false;
   // Reconstruction lanes beyond `stream` (default priority, created on first
@@ -1057,6 +1064,10 @@ static const DevBuf* buffer_of(const Engine& e, const ebiv_frame* f) {
   return nullptr;
 }
 
+static RgbaJob rgba_job(const DevBuf& b, int w, int h, uint8_t* dst, int64_t pitch) {
+  return RgbaJob{b.p.y, b.p.cb, b.p.cr, dst, pitch, b.p.pw, b.p.cw, w, h};
+}
+
 static int frame_rgba(Engine& e, const ebiv_frame* f, uint8_t* dst, int dst_on_device) {
   if (!f || !dst || !f->y || f->width <= 0 || f->height <= 0) return EBIV_ERR_ARGUMENT;
   if (hipSetDevice(e.device) != hipSuccess) return EBIV_ERR_DEVICE;
@@ -1069,7 +1080,7 @@ static int frame_rgba(Engine& e, const ebiv_frame* f, uint8_t* dst, int dst_on_d
     out = e.d_rgba.d;
   }
   const int m = e.mark_begin(3);
-  launch_rgba(e.stream, b->p.y, b->p.pw, b->p.cb, b->p.cr, b->p.cw, f->width, f->height, out);
+  launch_rgba(e.stream, rgba_job(*b, f->width, f->height, reinterpret_cast<uint8_t*>(out), 4 * static_cast<int64_t>(f->width)));
   e.mark_end(m);
   if (hipGetLastError() != hipSuccess) return EBIV_ERR_DEVICE;
   if (!dst_on_device && hipMemcpyAsync(dst, e.d_rgba.d, px * 4, hipMemcpyDeviceToHost, e.stream) != hipSuccess)
@@ -1099,8 +1110,7 @@ int ebiv_frame_rgba_noalloc(ebiv_codec* c, const ebiv_frame* f, uint8_t* dst) {
   if (!b) return s.fail(EBIV_ERR_ARGUMENT, "frame not returned by this codec");
   // The kernel writes all `bytes` of the staging area that are copied out,
   // so nothing needs clearing between calls.
-  launch_rgba(e.stream, b->p.y, b->p.pw, b->p.cb, b->p.cr, b->p.cw, f->width, f->height,
-              reinterpret_cast<uint32_t*>(stage));
+  launch_rgba(e.stream, rgba_job(*b, f->width, f->height, stage, 4 * static_cast<int64_t>(f->width)));
   if (hipGetLastError() != hipSuccess) return s.fail(EBIV_ERR_DEVICE, "kernel launch failed");
   if (hipMemcpyAsync(dst, stage, bytes, hipMemcpyDeviceToHost, e.stream) != hipSuccess ||
       hipStreamSynchronize(e.stream) != hipSuccess)
@@ -1202,9 +1212,12 @@ int ebiv_batch_step(ebiv_batch* b, ebiv_completion* out, int max_out, int* n_out
   std::vector<std::vector<StepFrame>> frames;
   std::vector<std::vector<Session::Queued>> held;
   int budget = max_out;
-  // Take up to a window of packets from every session ...
-  for (auto& sp : e.sessions) {
-    Session& s = *sp;
+  // Take up to a window of packets from every session, starting one session
+  // further on each step, so a max_out below the queued work is shared ...
+  const size_t ns = e.sessions.size();
+  const size_t first = ns ? e.rotate++ % ns : 0;
+  for (size_t n = 0; n < ns; n++) {
+    Session& s = *e.sessions[(first + n) % ns];
     if (s.queue.empty() || budget <= 0) continue;
     ss.push_back(&s);
     frames.emplace_back();
@@ -1245,6 +1258,11 @@ int ebiv_batch_step(ebiv_batch* b, ebiv_completion* out, int max_out, int* n_out
   });
   if (int rc = e.run_step(ss, frames)) return rc;
   int n = 0;
+  size_t npresent = 0, nframes = 0;
+  uint32_t max_px8 = 0;
+  for (auto& fr : frames) nframes += fr.size();
+  // (reserve does not keep contents: once, before the table is filled)
+  if (!e.h_present.reserve(nframes + 1)) return e.fail(EBIV_ERR_DEVICE, "presentation table allocation failed");
   for (size_t i = 0; i < ss.size(); i++)
     for (auto& f : frames[i]) {
       ebiv_completion& c = out[n++];
@@ -1252,13 +1270,36 @@ int ebiv_batch_step(ebiv_batch* b, ebiv_completion* out, int max_out, int* n_out
       c.session = ss[i]->id;
       c.status = f.rc;
       c.tag = f.tag;
+      c.surface = -1;
       if (f.rc == EBIV_OK) {
-        fill_frame(*ss[i], f.out_buf, f.ring_slot, f.ts, true, &c.frame);
+        Session& s = *ss[i];
+        fill_frame(s, f.out_buf, f.ring_slot, f.ts, true, &c.frame);
+        if (!s.surfaces.empty() && s.surface_pitch >= 4 * static_cast<size_t>(s.W)) {
+          c.surface = s.next_surface;
+          s.next_surface = (s.next_surface + 1) % static_cast<int>(s.surfaces.size());
+          e.h_present.h[npresent++] = rgba_job(s.pool[static_cast<size_t>(f.out_buf)], s.W, s.H,
+                                               s.surfaces[static_cast<size_t>(c.surface)],
+                                               static_cast<int64_t>(s.surface_pitch));
+          max_px8 = std::max(max_px8, static_cast<uint32_t>((s.W + 7) / 8) * static_cast<uint32_t>(s.H));
+        }
       } else {
         std::strncpy(c.message, f.msg.c_str(), sizeof c.message - 1);
         ss[i]->msg = f.msg;
       }
     }
+  // ---- presentation: every presented frame of the step in one launch
+  if (npresent) {
+    if (!e.d_present.reserve(npresent)) return e.fail(EBIV_ERR_DEVICE, "presentation table allocation failed");
+    if (hipMemcpyAsync(e.d_present.d, e.h_present.h, sizeof(RgbaJob) * npresent, hipMemcpyHostToDevice, e.stream) !=
+        hipSuccess)
+      return e.fail(EBIV_ERR_DEVICE, "presentation table upload failed");
+    const int m = e.mark_begin(3);
+    launch_rgba_frames(e.stream, e.d_present.d, static_cast<int>(npresent), max_px8);
+    e.mark_end(m);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(e.stream) != hipSuccess)
+      return e.fail(EBIV_ERR_DEVICE, "presentation failed on the device");
+    e.collect_marks();
+  }
   *n_out = n;
   return EBIV_OK;
 }
@@ -1288,6 +1329,19 @@ int ebiv_batch_stats(const ebiv_batch* b, int session, ebiv_stats* out) {
       t.tiles += s->stats.tiles;
     }
   *out = t;
+  return EBIV_OK;
+}
+
+int ebiv_batch_set_surfaces(ebiv_batch* b, int session, uint8_t* const* surfaces, int count, size_t pitch) {
+  Session* s = b ? b->eng.find(session) : nullptr;
+  // fewer surfaces than the window would let one step overwrite its own frames
+  if (!s || count < 0 || count > 64 || (count > 0 && (count < s->window || !surfaces || pitch == 0)))
+    return EBIV_ERR_ARGUMENT;
+  for (int i = 0; i < count; i++)
+    if (!surfaces[i]) return EBIV_ERR_ARGUMENT;
+  s->surfaces.assign(surfaces, surfaces + count);
+  s->surface_pitch = pitch;
+  s->next_surface = 0;
   return EBIV_OK;
 }
 

@@ -71,8 +71,10 @@ int ebiv_codec_decode(ebiv_codec *c, const uint8_t *data, size_t len, int64_t ti
 
 /* The same decode for a packet already resident in HBM: host_data is read
  * for the headers and tables, dev_data (same bytes, device memory) by the
- * kernels.  *out describes HBM planes (padded macroblock-aligned planes; the
- * visible region starts at offset 0). */
+ * kernels, which read tile streams as aligned dwords: dev_data must stay
+ * readable for 4 bytes past len (buffers from ebiv_device_alloc are).  *out
+ * describes HBM planes (padded macroblock-aligned planes; the visible region
+ * starts at offset 0). */
 int ebiv_codec_decode_device(ebiv_codec *c, const uint8_t *host_data, const uint8_t *dev_data, size_t len,
                              int64_t timestamp_ns, ebiv_frame *out);
 
@@ -142,6 +144,8 @@ typedef struct {
     uint64_t tag;       /* the submit call's tag */
     ebiv_frame frame;   /* valid when status == EBIV_OK */
     char message[96];   /* error text when status != EBIV_OK */
+    int32_t surface;    /* presentation surface the frame's RGBA was written to, or -1 */
+    int32_t reserved;
 } ebiv_completion;
 
 int ebiv_batch_create(int device, ebiv_batch **out);
@@ -155,8 +159,9 @@ int ebiv_batch_open(ebiv_batch *b, int ring_size, int window, int *session);
 int ebiv_batch_close(ebiv_batch *b, int session);
 
 /* Queues a packet.  host_data is copied; dev_data, when not null, holds the
- * same bytes in HBM (from ebiv_device_alloc) and must stay valid until the
- * packet completes; when null the engine stages the packet itself. */
+ * same bytes in HBM, readable for 4 bytes past len (as ebiv_device_alloc's
+ * buffers are), and must stay valid until the packet completes; when null
+ * the engine stages the packet itself. */
 int ebiv_batch_submit(ebiv_batch *b, int session, const uint8_t *host_data, const uint8_t *dev_data, size_t len,
                       int64_t timestamp_ns, uint64_t tag);
 /* n packets of one session in one call, in order (as n ebiv_batch_submit
@@ -174,9 +179,24 @@ int ebiv_batch_flush(ebiv_batch *b, int session);
 /* Queued packets of one session (or of all when session is 0). */
 int ebiv_batch_pending(const ebiv_batch *b, int session);
 
-/* Runs one step; writes at most max_out completions.  Returns a runtime
- * status (decode errors are reported per completion). */
+/* Runs one step; writes at most max_out completions.  When max_out is less
+ * than the queued work, the sessions served first rotate from step to step.
+ * Returns a runtime status (decode errors are reported per completion). */
 int ebiv_batch_step(ebiv_batch *b, ebiv_completion *out, int max_out, int *n_out);
+
+/* Zero-copy presentation.  govid's Ebitengine bridge uploads each frame's
+ * Frame.RGBA() to a GPU image (ebitengine/video_image.go:30-52); here the
+ * batch step that decodes a session's frames also writes their RGBA
+ * (frame.go:8-67, bit-exact) straight into caller-owned device surfaces, in
+ * one launch for the whole step.  surfaces: count device pointers (at least
+ * the session's window, at most 64), each with room for height rows of pitch
+ * bytes; the session's successful
+ * frames take them in turn, and a completion's `surface` names the one its
+ * frame went to (-1: none, also when pitch < 4 * width).  A surface is
+ * rewritten count frames later.  count 0 turns presentation off.  No
+ * counterpart in the reference's API (govid cannot preset Frame.RGBA from
+ * outside its package, frame.go:76-91). */
+int ebiv_batch_set_surfaces(ebiv_batch *b, int session, uint8_t *const *surfaces, int count, size_t pitch);
 
 int ebiv_batch_stats(const ebiv_batch *b, int session, ebiv_stats *out);
 int ebiv_batch_set_timing(ebiv_batch *b, int enabled);
