@@ -1,31 +1,49 @@
 """EBIV decode benchmark on one MI355X.
 
-Workload: one 1920x1080 4:2:0 clip, 120 frames (three Perlin parallax layers,
-2-frame flash every 24 frames, noise sigma 1.5), EBIV q22, gop 60, 4x3 tiles.
+Workload of `value`: one 1920x1080 4:2:0 clip, 120 frames (three Perlin
+parallax layers, 2-frame flash every 24 frames, noise sigma 1.5), EBIV q22,
+gop 60, 4x3 tiles, decoded by the plain drop-in (one Codec, one packet per
+Codec.Decode call) with packets resident in HBM and frames left in HBM.  A
+step is one pass over the clip.
 
-  plain    the drop-in: one Codec, one packet per Codec.Decode call (batch 1).
-           A step is one pass over the clip.  This is `value`.
-  batched  the same clip as S concurrent sessions of one Batch engine, each
-           starting at a different key frame and wrapping round (a loop
-           restart), decoding up to W frames per session per engine step.
-           A step is one pass of every session over the clip.
-  mixed    64 sessions, one stream each: 24 x 640x360, 24 x 1280x720 and
-           16 x 1920x1080 (rects content), each with its own tile grid
-           {1x1, 2x2, 4x3}, gop {30, 60, 120}, q {18, 22, 28} and length
-           60-240 frames.  A step is one pass of every session over its stream.
-  entropy  32 sessions of 1920x1080 noise (sigma 20, random global motion in
-           +-16 px), q12, gop 15, 4x3 tiles: the symbol-heavy case, reported
-           in Msymbols/s.  The sessions replay one 30-frame stream from
-           staggered key frames.
-  serving  Poisson arrivals of play / seek / loop sessions of 360p-2160p
-           streams under an HBM budget; p50/p99 frame latency.
+Sections beside it (each in its own object of the one JSON line):
+  drop_in_host  the same clip through ebiv_codec_decode: packets from host
+                memory, frames downloaded to the host ring (what the cgo
+                shim sees; PCIe-inclusive, never `value`).
+  rgba          Frame.RGBA on the device for every frame of the clip, timed
+                against its 5.5 * W * H algorithmic bytes.
+  batched       the clip as S sessions of one Batch engine from staggered key
+                frames, up to W frames per session per engine step; also with
+                presentation into RGBA surfaces.
+  mixed         64 sessions: 24 x 640x360, 24 x 1280x720, 16 x 1920x1080
+                (rects content), each with its own tile grid {1x1, 2x2, 4x3},
+                gop {30, 60, 120}, q {18, 22, 28} and length 60-240 frames.
+  entropy       32 sessions of 1920x1080 noise (sigma 20, random global
+                motion in +-16 px), q12, gop 15, 4x3 tiles, replaying one
+                30-frame stream from staggered key frames: Msymbols/s.
+  serving       Poisson arrivals at 200 sessions/s for a fixed time over
+                640x360, 1280x720, 1920x1080 (q22, gop 30, one tile) and
+                3840x2160 (8x6 tiles) streams of 600 frames; plays of 30-600
+                frames, 20% seeks, 10% loop restarts; admission under an HBM
+                budget; frame latency and admission wait.
+  cpu_baseline  the C restatement of the govid decoder (oracle) on this
+                host's cores, tile-parallel like ebiv/pool.go, and one thread.
 
-The mixed and entropy streams are generated by bench_data/make_streams.py
-into bench_data/gen/; a section whose streams are absent is reported as
-skipped.  Packets are resident in HBM before the timed regions and frames
-stay in HBM.  Every frame of the plain pass and of each batched warmup pass,
-and a sample of the served frames, is checked bit for bit (plane CRCs)
-against the expected CRCs stored beside each stream, outside the timed
+Rooflines (SURVEY 8(d) definitions; peaks are MI355X's published figures):
+  roofline      the reconstruction kernels (inter + intra) of the plain pass:
+                sum over frames of 3*W*H (inter) or 1.5*W*H (key) bytes over
+                the summed device time of those kernels, against 8 TB/s.
+  parse_issue   the entropy kernel, which is bound by scalar instruction
+                issue (one dependent chain per tile): its SALU instructions
+                per rANS symbol (rocprofv3 SQ_INSTS_SALU, profiles/) times the
+                symbols decoded, over its device time, against one SALU
+                instruction per CU per cycle (256 CUs at 2.4 GHz).
+
+The mixed, entropy and serving streams are generated by
+bench_data/make_streams.py into bench_data/gen/; a section whose streams are
+absent is reported as skipped.  Every frame of the plain pass and of each
+batched warmup pass, and a sample of the served frames, is checked bit for bit
+(plane CRCs) against the CRCs stored beside each stream, outside the timed
 regions; those are the encoder's reconstruction, which the CPU tests check
 against the oracle decoder.
 
@@ -44,22 +62,9 @@ import zlib
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
-
-
-def measured_traffic(workload: str, kernel: str):
-    """HBM bytes per launch of `kernel` in `workload` from the committed
-    rocprofv3 counter summary (profiles/hbm_traffic.json): FETCH_SIZE and
-    WRITE_SIZE in KiB per dispatch, each scaled by what the same counter
-    reads for a plain device copy of known size.  None when not measured."""
-    try:
-        with open(os.path.join(ROOT, "profiles", "hbm_traffic.json")) as fh:
-            t = json.load(fh)
-        k = t["workloads"][workload][kernel]
-        cal = t["calibration"]
-        return round(1024 * (k["fetch_kib"] * cal["fetch_scale"] + k["write_kib"] * cal["write_scale"]))
-    except (OSError, KeyError, ValueError):
-        return None
+HBM_PEAK = 8.0e12           # MI355X HBM3E, bytes/s
+CUS, CLOCK = 256, 2.4e9     # MI355X compute units and peak engine clock
+SALU_PEAK = CUS * CLOCK     # scalar instructions per second, one per CU per cycle
 BENCH_DATA = os.path.join(ROOT, "bench_data")
 GEN = os.path.join(BENCH_DATA, "gen")
 
@@ -68,23 +73,44 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def _profile(name):
+    try:
+        with open(os.path.join(ROOT, "profiles", name)) as fh:
+            return json.load(fh)
+    except (OSError, ValueError):
+        return None
+
+
+def measured_traffic(workload: str, kernels):
+    """HBM bytes per launch of `kernels` (summed) in `workload` from the
+    committed rocprofv3 counter summary (profiles/hbm_traffic.json):
+    FETCH_SIZE and WRITE_SIZE in KiB per dispatch, each scaled by what the
+    same counter reads for a plain device copy of known size.  None when not
+    measured."""
+    t = _profile("hbm_traffic.json")
+    try:
+        cal = t["calibration"]
+        tot = 0.0
+        for k in kernels:
+            m = t["workloads"][workload][k]
+            tot += 1024 * (m["fetch_kib"] * cal["fetch_scale"] + m["write_kib"] * cal["write_scale"])
+        return round(tot)
+    except (TypeError, KeyError):
+        return None
+
+
+def salu_per_symbol(workload: str):
+    """SQ_INSTS_SALU of the entropy kernel per rANS symbol, from the
+    committed counter summary (profiles/instruction_mix.json)."""
+    t = _profile("instruction_mix.json")
+    try:
+        return float(t["workloads"][workload]["ebiv_parse"]["salu_per_symbol"])
+    except (TypeError, KeyError, ValueError):
+        return None
+
+
 def stream_path(name, gen=False):
     return os.path.join(GEN if gen else BENCH_DATA, name + ".ebiv")
-
-
-def load_stream(name="parallax_1080p"):
-    """Reader and expected plane CRCs of a committed benchmark stream."""
-    from govid_amd.container import Reader
-
-    path = stream_path(name)
-    if not os.path.exists(path):
-        log(f"bench: {name} missing, generating it (several minutes)")
-        sys.path.insert(0, BENCH_DATA)
-        import make_streams
-
-        make_streams.make(name, threads=os.cpu_count() or 8)
-    rd = Reader.open(path)
-    return rd, expected_crcs(path, rd.data)
 
 
 def expected_crcs(path, data):
@@ -97,28 +123,84 @@ def expected_crcs(path, data):
     return [tuple(c) for c in d["crc32"]]
 
 
+def load_stream(name="parallax_1080p", gen=False):
+    """Reader and expected plane CRCs of a benchmark stream, or None when a
+    generated one is absent."""
+    from govid_amd.container import Reader
+
+    path = stream_path(name, gen)
+    if not os.path.exists(path):
+        if gen:
+            return None
+        log(f"bench: {name} missing, generating it (several minutes)")
+        sys.path.insert(0, BENCH_DATA)
+        import make_streams
+
+        make_streams.make(name, threads=host_threads())
+    rd = Reader.open(path)
+    return rd, expected_crcs(path, rd.data)
+
+
 def _crc(planes):
     return tuple(zlib.crc32(p.tobytes()) for p in planes)
 
 
-def cpu_baseline(packets, budget_s=20.0):
-    """The oracle on this host: tile-parallel like ebiv/pool.go."""
+def pct(v, q):
+    v = sorted(v)
+    return v[min(len(v) - 1, int(q * len(v)))] if v else 0.0
+
+
+def host_threads():
+    """Host threads this process may use: its CPU affinity, at most 16 (the
+    GPU box's CPU share per GPU)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(packets, tiled_s=16.0, single_s=8.0):
+    """The oracle (a C restatement of govid's ebiv decoder) on this host:
+    tile-parallel like ebiv/pool.go over the usable threads, then one
+    thread, each on a bounded sample of the clip from its first frame."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
-    cores = min(os.cpu_count() or 1, 16)
-    dec = O.OracleCodec(threads=cores)
-    t0 = time.perf_counter()
-    n = 0
-    while n < len(packets):
-        dec.decode(packets[n])
-        n += 1
-        if time.perf_counter() - t0 > budget_s and n >= 8:
-            break
-    dt = time.perf_counter() - t0
+    def run(threads, budget):
+        dec = O.OracleCodec(threads=threads)
+        t0 = time.perf_counter()
+        n = 0
+        while n < len(packets):
+            dec.decode(packets[n])
+            n += 1
+            if time.perf_counter() - t0 > budget and n >= 8:
+                break
+        dt = time.perf_counter() - t0
+        dec.close()
+        return n, dt
+
+    cores = host_threads()
+    n, dt = run(cores, tiled_s)
+    n1, dt1 = run(1, single_s)
     return {"value": round(n / dt, 2), "unit": "frames/s", "cores": cores, "kind": "port",
-            "sample": f"first {n} of {len(packets)} frames of the same 1080p clip (C restatement of the "
-                      f"govid ebiv tiled decoder, one tile per job over {cores} threads)"}
+            "cpu_model": cpu_model(), "ms_per_frame": round(dt / n * 1e3, 3),
+            "sample": f"first {n} of {len(packets)} frames of the same 1080p clip: C restatement of the govid ebiv "
+                      f"tiled decoder, one job per tile over {cores} threads",
+            "single_thread": {"value": round(n1 / dt1, 2), "unit": "frames/s", "cores": 1,
+                              "ms_per_frame": round(dt1 / n1 * 1e3, 3),
+                              "sample": f"first {n1} frames of the same clip, one thread"}}
 
 
 def verify_plain(packets, want):
@@ -130,9 +212,44 @@ def verify_plain(packets, want):
     return bad
 
 
-def pct(v, q):
-    v = sorted(v)
-    return v[min(len(v) - 1, int(q * len(v)))]
+def kavg_us(kt, k):
+    return kt[f"{k}_ms"] / max(kt[f"{k}_n"], 1) * 1e3
+
+
+def recon_bytes(packets, w, h):
+    """SURVEY 8(d) algorithmic bytes of reconstructing `packets` once:
+    1.5*W*H written per key frame, 1.5*W*H read + 1.5*W*H written per inter
+    frame (padding, overrides and intermediate buffers not counted)."""
+    px = w * h
+    return sum(1.5 * px if p[0] & 3 == 0 else 3.0 * px for p in packets)
+
+
+def recon_roofline(kt, bytes_total, traffic=None, workload=""):
+    t = (kt["inter_ms"] + kt["intra_ms"]) / 1e3
+    if not t:
+        return None
+    launches = kt["inter_n"] + kt["intra_n"]
+    return {"bound": "hbm", "kernel": "ebiv_recon_inter + ebiv_recon_intra", "workload": workload,
+            "achieved": round(bytes_total / t / 1e9, 2), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": round(bytes_total / t / HBM_PEAK, 5), "traffic": traffic,
+            "algorithmic_bytes": round(bytes_total), "device_ms": round(t * 1e3, 3),
+            "launches": {"inter": kt["inter_n"], "intra": kt["intra_n"]},
+            "avg_launch_us": {"inter": round(kavg_us(kt, "inter"), 2), "intra": round(kavg_us(kt, "intra"), 2)},
+            "per_launch_pair_bytes": round(bytes_total / max(launches / 2, 1))}
+
+
+def parse_issue(kt, symbols, workload):
+    spp = salu_per_symbol(workload)
+    t = kt["parse_ms"] / 1e3
+    if not t:
+        return None
+    out = {"bound": "salu", "kernel": "ebiv_parse", "workload": workload, "unit": "G SALU instructions/s",
+           "peak": SALU_PEAK / 1e9, "msymbols_per_s_device": round(symbols / t / 1e6, 2),
+           "avg_launch_us": round(kavg_us(kt, "parse"), 2), "salu_per_symbol": spp}
+    if spp:
+        out["achieved"] = round(spp * symbols / t / 1e9, 2)
+        out["frac"] = round(spp * symbols / t / SALU_PEAK, 4)
+    return out
 
 
 def run_plain(packets, dev, steps, warmup):
@@ -160,25 +277,85 @@ def run_plain(packets, dev, steps, warmup):
             "symbols": s1["symbols"] - s0["symbols"]}
 
 
-def run_batched(packets, dev, want, sessions, window, steps, warmup):
+def run_plain_host(packets, steps):
+    """The clip through ebiv_codec_decode: host packets in, host planes out."""
+    from govid_amd.codec import Codec
+
+    codec = Codec(ring=3)
+    for p in packets:
+        codec.decode(p)
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for p in packets:
+            a = time.perf_counter()
+            codec.decode(p)
+            lat.append(time.perf_counter() - a)
+    elapsed = time.perf_counter() - t0
+    codec.close()
+    n = steps * len(packets)
+    return {"frames_per_s": round(n / elapsed, 2), "p50_frame_ms": round(statistics.median(lat) * 1e3, 3),
+            "p99_frame_ms": round(pct(lat, 0.99) * 1e3, 3), "frames": n,
+            "note": "ebiv_codec_decode: packet staged from host memory, visible planes downloaded to the pinned "
+                    "host ring each frame (PCIe included)"}
+
+
+def run_rgba(packets, dev, w, h):
+    """Frame.RGBA on the device for every frame of the clip (timed launches)."""
+    from govid_amd.codec import Codec, DeviceBuffer
+
+    codec = Codec(ring=3)
+    out = DeviceBuffer(bytes(w * h * 4))
+    frames = [codec.decode_device(p, d.ptr) for p, d in zip(packets[:3], dev[:3])]
+    frames[-1].rgba_device(out.ptr)  # warm
+    codec.set_timing(True)
+    for p, d in zip(packets, dev):
+        codec.decode_device(p, d.ptr).rgba_device(out.ptr)
+    kt = codec.kernel_times()
+    codec.close()
+    out.close()
+    per = 5.5 * w * h
+    t = kt["rgba_ms"] / max(kt["rgba_n"], 1) / 1e3
+    return {"bound": "hbm", "kernel": "ebiv_rgba", "launches": kt["rgba_n"], "avg_launch_us": round(t * 1e6, 2),
+            "bytes_per_launch": round(per), "achieved": round(per / t / 1e9, 2), "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s", "frac": round(per / t / HBM_PEAK, 4),
+            "note": "one 1920x1080 frame per launch: 1.5 B/px read, 4 B/px written"}
+
+
+def run_batched(packets, dev, want, sessions, window, steps, warmup, present=False):
     """S sessions over one clip, each from a different key frame."""
     keys = [i for i, p in enumerate(packets) if p[0] & 3 == 0]
     plan = []
     for k in range(sessions):
         st = keys[k % len(keys)]
         plan.append((packets, dev, want, list(range(st, len(packets))) + list(range(0, st))))
-    return run_sessions(plan, window, steps, warmup)
+    return run_sessions(plan, window, steps, warmup, present)
 
 
-def run_sessions(plan, window, steps, warmup):
+SCHEDULE = {}  # engine schedule fields set from the command line
+
+
+def run_sessions(plan, window, steps, warmup, present=False):
     """One Batch engine; plan = [(packets, device buffers, expected CRCs,
     frame order)] per session.  A pass submits every session's frames in
     its order and steps the engine until all have completed; the first
-    warmup pass checks every frame."""
-    from govid_amd.codec import Batch
+    warmup pass checks every frame.  With `present`, every session also
+    writes its frames' RGBA into `window` device surfaces."""
+    from govid_amd.codec import Batch, DeviceBuffer
 
     b = Batch()
+    if SCHEDULE:
+        b.set_schedule(**SCHEDULE)
     sids = [b.open(ring=window, window=window) for _ in plan]
+    surfaces = []
+    if present:  # (geometry from each stream's first packet, a key frame)
+        for sid, (packets, _, _, _) in zip(sids, plan):
+            k = packets[0]
+            w, h = k[1] | k[2] << 8, k[3] | k[4] << 8
+            pitch = (4 * w + 255) // 256 * 256
+            bufs = [DeviceBuffer(bytes(pitch * h)) for _ in range(window)]
+            surfaces += bufs
+            b.set_surfaces(sid, [x.ptr for x in bufs], pitch)
 
     def one_pass(check):
         for sid, (packets, dev, _, order) in zip(sids, plan):
@@ -215,13 +392,15 @@ def run_sessions(plan, window, steps, warmup):
     kt = b.kernel_times()
     s1 = b.stats()
     b.close()
+    for x in surfaces:
+        x.close()
     return {"elapsed": elapsed, "frames": frames, "lat": lat, "kt": kt, "bad": bad,
             "symbols": s1["symbols"] - s0["symbols"]}
 
 
-def batched_summary(bt, steps, **extra):
+def batched_summary(bt, steps, recon_b=None, workload="", traffic_wl=None, **extra):
     bk = bt["kt"]
-    return dict(extra, **{
+    out = dict(extra, **{
         "frames_per_s": round(bt["frames"] / bt["elapsed"], 2),
         "frames_per_step": bt["frames"] // steps,
         "engine_step_p50_ms": round(statistics.median(bt["lat"]) * 1e3, 3),
@@ -230,24 +409,28 @@ def batched_summary(bt, steps, **extra):
         "frames_differing": bt["bad"],
         "kernel_avg_us": {k: round(kavg_us(bk, k), 2) for k in ("parse", "inter", "intra")},
         "kernel_launches": {k: bk[f"{k}_n"] for k in ("parse", "inter", "intra")},
-        "recon": {"ebiv_recon_inter": inter_roofline(bk) or {}},
+        "kernel_device_ms": {k: round(bk[f"{k}_ms"], 3) for k in ("parse", "inter", "intra")},
     })
+    if recon_b is not None:
+        tr = measured_traffic(traffic_wl, ("ebiv_recon_inter", "ebiv_recon_intra")) if traffic_wl else None
+        out["roofline"] = recon_roofline(bk, recon_b, tr, workload)
+        out["parse_issue"] = parse_issue(bk, bt["symbols"], traffic_wl or workload)
+    return out
 
 
 def open_gen_streams(names):
-    """Readers, expected CRCs and device buffers of generated streams, or
-    None when any is absent."""
+    """Readers, packets, device buffers and expected CRCs of generated
+    streams, or None when any is absent."""
     from govid_amd.codec import DeviceBuffer
-    from govid_amd.container import Reader
 
     out = []
     for nm in names:
-        path = stream_path(nm, gen=True)
-        if not os.path.exists(path):
+        got = load_stream(nm, gen=True)
+        if got is None:
             return None
-        rd = Reader.open(path)
+        rd, want = got
         pk = rd.packets()
-        out.append((nm, rd, pk, [DeviceBuffer(p) for p in pk], expected_crcs(path, rd.data)))
+        out.append((nm, rd, pk, [DeviceBuffer(p) for p in pk], want))
     return out
 
 
@@ -266,8 +449,9 @@ def run_mixed(window, steps, warmup):
     for _, rd, pk, _, _ in st:
         k = f"{rd.header.width}x{rd.header.height}"
         sizes[k] = sizes.get(k, 0) + 1
+    rb = steps * sum(recon_bytes(pk, rd.header.width, rd.header.height) for _, rd, pk, _, _ in st)
     hd = sum(len(pk) for _, rd, pk, _, _ in st if rd.header.width == 1920) * steps
-    out = batched_summary(bt, steps, sessions=len(st), window=window, streams=sizes,
+    out = batched_summary(bt, steps, rb, "mixed 64 streams", None, sessions=len(st), window=window, streams=sizes,
                           frames_per_pass=sum(len(pk) for _, _, pk, _, _ in st))
     out["frames_1080p_share"] = round(hd / max(bt["frames"], 1), 3)
     return out
@@ -282,34 +466,40 @@ def run_entropy(sessions, window, steps, warmup):
     bt = run_batched(pk, dev, want, sessions, window, steps, warmup)
     for d in dev:
         d.close()
-    out = batched_summary(bt, steps, sessions=sessions, window=window,
+    rb = steps * sessions * recon_bytes(pk, 1920, 1080)
+    out = batched_summary(bt, steps, rb, "entropy 32x1080p noise q12", "entropy_32x1080p_noise_q12",
+                          sessions=sessions, window=window,
                           stream=f"1920x1080 noise q12 gop15 4x3 tiles, {len(pk)} frames, "
                                  f"{sum(map(len, pk)) // len(pk)} B per frame")
     out["symbols_per_frame"] = round(bt["symbols"] / max(bt["frames"], 1))
     return out
 
 
-SERVE_STREAMS = ("parallax_360p", "parallax_720p", "parallax_1080p", "parallax_2160p")
+SERVE_STREAMS = ("serve_360p", "serve_720p", "serve_1080p", "serve_2160p")
 
 
-def run_serving(requests_n, budget_bytes, window, check_every, seed=5):
-    """The serving mix: Poisson arrivals (200 requests/s) over 360p, 720p and
-    1080p streams (4x3 tiles) and a 2160p stream (8x6 tiles), all of the
-    parallax content at q22, gop 60; 20% seek to a random key frame part-way, 10% loop
-    restarts; lengths 30-600 frames; sessions admitted under an HBM budget.
-    Every `check_every`-th completed frame is compared with the stream's
-    expected CRCs for that frame."""
+def run_serving(duration_s, rate, budget_bytes, window, check_every, seed=5):
+    """Config 5: Poisson arrivals at `rate` sessions/s for `duration_s`
+    seconds over the four serving streams (600 frames each); plays of 30-600
+    frames, 20% seek to a random key frame part-way, 10% loop restarts;
+    sessions admitted under an HBM budget and served to completion.  Every
+    `check_every`-th completed frame is compared with the stream's expected
+    CRCs for that frame."""
     from govid_amd.codec import Batch, DeviceBuffer
     from govid_amd.serve import Server, Stream, poisson_requests
 
     streams, bufs, want = [], [], {}
     for nm in SERVE_STREAMS:
-        rd, want[nm] = load_stream(nm)
+        got = load_stream(nm, gen=True)
+        if got is None:
+            return {"skipped": "streams absent: python3 bench_data/make_streams.py serve"}
+        rd, want[nm] = got
         pk = rd.packets()
         dev = [DeviceBuffer(p) for p in pk]
         bufs += dev
         streams.append(Stream(nm, rd.header.width, rd.header.height, pk, [d.ptr for d in dev]))
-    reqs = poisson_requests(streams, requests_n, rate_per_s=200.0, seed=seed, min_len=30, max_len=600)
+    reqs = poisson_requests(streams, None, rate_per_s=rate, seed=seed, min_len=30, max_len=600, until_s=duration_s)
+    by_rid = {r.rid: r.stream.name for r in reqs}
     checked = [0, 0]
     count = [0]
 
@@ -317,40 +507,23 @@ def run_serving(requests_n, budget_bytes, window, check_every, seed=5):
         count[0] += 1
         if fd.status == 0 and count[0] % check_every == 0:
             checked[0] += 1
-            st = next(r.stream for r in reqs if r.rid == fd.rid)
-            if _crc(fd.frame.planes()) != want[st.name][fd.index]:
+            if _crc(fd.frame.planes()) != want[by_rid[fd.rid]][fd.index]:
                 checked[1] += 1
 
     b = Batch()
     srv = Server(b, budget_bytes, window=window, ring=window, on_frame=on_frame)
-    stats = srv.run(reqs, max_seconds=240)
+    stats = srv.run(reqs, max_seconds=600, progress=lambda s: log(f"bench: serving {s}"))
     b.close()
     for d in bufs:
         d.close()
+    kinds = {"seek": sum(1 for r in reqs if r.seek_at is not None),
+             "loop": sum(1 for r in reqs if len(r.frame_order()) > len(r.stream.packets) and r.seek_at is None)}
+    kinds["play"] = len(reqs) - kinds["seek"] - kinds["loop"]
     stats.update({"streams": [f"{s.name} {s.width}x{s.height} x{len(s.packets)}" for s in streams],
-                  "budget_bytes": budget_bytes, "window": window,
+                  "arrival_rate_per_s": rate, "arrival_window_s": duration_s, "request_kinds": kinds,
+                  "budget_bytes": budget_bytes, "window": window, "target_p99_frame_ms": 16.7,
                   "frames_checked": checked[0], "frames_differing": checked[1]})
     return stats
-
-
-def kavg_us(kt, k):
-    return kt[f"{k}_ms"] / max(kt[f"{k}_n"], 1) * 1e3
-
-
-# Inter reconstruction moves, per macroblock, its 384 predicted samples
-# (256 luma + 2 x 64 chroma) in from the reference and 384 reconstructed
-# samples out; filter margins re-read from cache are not algorithmic bytes.
-INTER_BYTES_PER_MB = 768
-
-
-def inter_roofline(kt):
-    n, t = kt["inter_n"], kt["inter_ms"] / 1e3
-    if not n or not t:
-        return None
-    per_launch = INTER_BYTES_PER_MB * kt["inter_mbs"] / n
-    return {"bytes_per_launch": round(per_launch), "macroblocks_per_launch": round(kt["inter_mbs"] / n, 1),
-            "avg_launch_us": round(t / n * 1e6, 2), "achieved_GBps": round(per_launch * n / t / 1e9, 2),
-            "frac": round(per_launch * n / t / HBM_PEAK, 4)}
 
 
 def main():
@@ -361,23 +534,24 @@ def main():
     ap.add_argument("--sessions", type=int, default=64)
     # frames per session per engine step in the batched sections: a wider
     # window puts more independent tile chains into each entropy launch and
-    # spreads the per-step costs over more frames (8 / 16 / 32 on the 64 x 1080p
-    # clip: 37.6k / 38.4k / 40.2k frames/s; on the mixed batch 27.5k / 35.4k /
-    # 41.0k), at one step's latency per frame
+    # spreads the per-step costs over more frames
     ap.add_argument("--window", type=int, default=32)
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-present", action="store_true")
     ap.add_argument("--batched-only", action="store_true", help="skip the plain pass (for profiling the batched one)")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-entropy", action="store_true")
+    ap.add_argument("--no-host", action="store_true")
     ap.add_argument("--entropy-sessions", type=int, default=32)
     ap.add_argument("--serve-only", action="store_true", help="run only the serving mix")
-    ap.add_argument("--serve-requests", type=int, default=48)
-    ap.add_argument("--serve-budget-gb", type=float, default=24.0)
+    ap.add_argument("--serve-seconds", type=float, default=4.0, help="length of the Poisson arrival window")
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-budget-gb", type=float, default=16.0)
     ap.add_argument("--serve-window", type=int, default=1,
-                    help="frames per session per engine step when serving (1 keeps p99 frame latency lowest)")
+                    help="frames per session per engine step when serving (1 keeps frame latency lowest)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 2 GiB device-to-device copy (scale for HBM byte counters)")
     args = ap.parse_args()
@@ -401,8 +575,9 @@ def main():
 
     from govid_amd.codec import DeviceBuffer
 
+    serve_budget = int(args.serve_budget_gb * (1 << 30))
     if args.serve_only:
-        sv = run_serving(args.serve_requests, int(args.serve_budget_gb * (1 << 30)), args.serve_window, check_every=17)
+        sv = run_serving(args.serve_seconds, args.serve_rate, serve_budget, args.serve_window, check_every=17)
         print(json.dumps({"mode": "serving", **{k: (round(v, 3) if isinstance(v, float) else v) for k, v in sv.items()}}))
         return
     rd, want = load_stream()
@@ -424,22 +599,7 @@ def main():
     fps = pl["frames"] / pl["elapsed"]
     kt = pl["kt"]
     log(f"bench: plain {fps:.1f} frames/s")
-
-    # Dominant kernel of the plain path: the entropy kernel.  Algorithmic
-    # bytes per launch: the frame's coded bytes read plus its macroblock
-    # records written (48 B per macroblock), averaged over the clip; the
-    # coefficient entries it also writes (4 B per nonzero level) are left out.
-    nmb = ((W + 15) // 16) * ((H + 15) // 16)
-    coded = sum(map(len, packets)) / len(packets)
-    parse_bytes = coded + 48 * nmb
-    parse_avg_s = kavg_us(kt, "parse") / 1e6
-    roof = {"bound": "hbm", "kernel": "ebiv_parse", "achieved": round(parse_bytes / parse_avg_s / 1e9, 3),
-            "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(parse_bytes / parse_avg_s / HBM_PEAK, 6),
-            "traffic": measured_traffic("plain_1080p_parallax", "ebiv_parse"), "bytes_per_launch": round(parse_bytes), "avg_launch_us": round(parse_avg_s * 1e6, 2),
-            "note": "latency-bound serial rANS chains (one wavefront per tile); reported in Msymbols/s"}
-    recon = {"ebiv_recon_inter": dict(inter_roofline(kt) or {},
-                                      traffic=measured_traffic("plain_1080p_parallax", "ebiv_recon_inter")),
-             "ebiv_recon_intra": {"avg_launch_us": round(kavg_us(kt, "intra"), 2)}}
+    rb = args.steps * recon_bytes(packets, W, H)
 
     out = {
         "metric": "EBIV 1080p decoded frames/s bit-exact, batched vs plain GPU; p50/p99 frame ms",
@@ -454,24 +614,44 @@ def main():
         "vs_baseline": None,
         "dtype": "u8",
         "data": "synthetic",
-        "config": {"workload": "plain drop-in, 1920x1080 parallax clip, 120 frames, q22 gop60 4x3 tiles, batch 1",
+        "config": {"workload": "plain drop-in, 1920x1080 parallax clip, 120 frames, q22 gop60 4x3 tiles, batch 1, "
+                               "packets and frames in HBM",
                    "frames_per_step": len(packets), "mode": "plain"},
         "p50_frame_ms": round(statistics.median(pl["lat"]) * 1e3, 3),
         "p99_frame_ms": round(pct(pl["lat"], 0.99) * 1e3, 3),
         "msymbols_per_s": round(pl["symbols"] / pl["elapsed"] / 1e6, 2),
         "bitexact_frames_differing": bad,
         "kernel_avg_us": {k: round(kavg_us(kt, k), 2) for k in ("parse", "inter", "intra")},
-        "roofline": roof,
-        "recon": recon,
+        "roofline": recon_roofline(kt, rb, measured_traffic("plain_1080p_parallax",
+                                                            ("ebiv_recon_inter", "ebiv_recon_intra")),
+                                   "plain 1080p parallax"),
+        "parse_issue": parse_issue(kt, pl["symbols"], "plain_1080p_parallax"),
     }
+    if not args.no_host:
+        out["drop_in_host"] = run_plain_host(packets, 1)
+        log(f"bench: host-plane drop-in {out['drop_in_host']['frames_per_s']} frames/s")
+    out["rgba"] = run_rgba(packets, dev, W, H)
     if not args.no_batched:
         bt = run_batched(packets, dev, want, args.sessions, args.window, args.steps, args.warmup)
         bfps = bt["frames"] / bt["elapsed"]
         log(f"bench: batched {bfps:.1f} frames/s")
-        out["batched"] = batched_summary(bt, args.steps, vs_plain=round(bfps / fps, 2),
-                                         sessions=args.sessions, window=args.window)
-        out["batched"]["recon"]["ebiv_recon_inter"]["traffic"] = measured_traffic(
-            "batched_64x1080p_parallax", "ebiv_recon_inter")
+        rbb = args.steps * args.sessions * recon_bytes(packets, W, H)
+        out["batched"] = batched_summary(bt, args.steps, rbb, "batched 64x1080p parallax", "batched_64x1080p_parallax",
+                                         vs_plain=round(bfps / fps, 2), sessions=args.sessions, window=args.window)
+        if not args.no_present:
+            bp = run_batched(packets, dev, want, args.sessions, args.window, args.steps, 0, present=True)
+            k = bp["kt"]
+            per = 5.5 * W * H
+            t = k["rgba_ms"] / 1e3
+            out["batched_present"] = {
+                "frames_per_s": round(bp["frames"] / bp["elapsed"], 2), "frames_differing": bp["bad"],
+                "rgba": {"bound": "hbm", "kernel": "ebiv_rgba_frames", "launches": k["rgba_n"],
+                         "avg_launch_us": round(kavg_us(k, "rgba"), 2),
+                         "bytes_per_launch": round(per * bp["frames"] / max(k["rgba_n"], 1)),
+                         "achieved": round(per * bp["frames"] / t / 1e9, 2) if t else None,
+                         "frac": round(per * bp["frames"] / t / HBM_PEAK, 4) if t else None},
+                "note": "the batched pass with every frame also written as RGBA into the session's device "
+                        "surfaces by the step (one conversion launch per step)"}
     for d in dev:
         d.close()
     if not args.no_mixed:
@@ -481,8 +661,8 @@ def main():
         out["entropy"] = run_entropy(args.entropy_sessions, args.window, args.steps, args.warmup)
         log(f"bench: entropy {out['entropy'].get('msymbols_per_s', out['entropy'].get('skipped'))} Msymbols/s")
     if not args.no_serve:
-        sv = run_serving(args.serve_requests, int(args.serve_budget_gb * (1 << 30)), args.serve_window, check_every=17)
-        log(f"bench: serving {sv['frames_per_s']:.1f} frames/s, p99 {sv['p99_frame_ms']:.2f} ms")
+        sv = run_serving(args.serve_seconds, args.serve_rate, serve_budget, args.serve_window, check_every=17)
+        log(f"bench: serving {sv.get('frames_per_s', sv.get('skipped'))}")
         out["serving"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in sv.items()}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(packets)

@@ -2460,11 +2460,10 @@ __device__ __forceinline__ void parse_tile_asm(const uint8_t* __restrict__ pkt, 
   if (lane == 0) status[t] = TileStatus{R.err, done, R.nsym, nintra};
 }
 
-__global__ __launch_bounds__(64) void ebiv_parse(const FrameJob* __restrict__ jobs, const WorkItem* __restrict__ items) {
-  const WorkItem it = item_of(items);
+// One tile: its job's packet and descriptor, the entropy outputs.
+__device__ __forceinline__ void parse_item(const FrameJob* __restrict__ jobs, const WorkItem it) {
   ConstJob& job = job_of(jobs, it.job);
   const FrameHdr* hdr = reinterpret_cast<const FrameHdr*>(job.desc);
-  const TileDesc* tiles = reinterpret_cast<const TileDesc*>(job.desc + desc_tiles_off());
   // Bits 30-31 of the item: the wave's issue priority (the host gives the
   // biggest tiles, the longest serial chains, the highest).
   const uint32_t prio = it.idx >> 30;
@@ -2479,6 +2478,13 @@ __global__ __launch_bounds__(64) void ebiv_parse(const FrameJob* __restrict__ jo
     parse_tile_asm(job.pkt, job.desc, job.recs, job.coefs, job.status, t);
   else
     parse_tile(job.pkt, job.desc, job.recs, job.coefs, job.status, t);
+}
+
+// The entropy launch: one wavefront per tile, item blockIdx.x (the host
+// orders the items longest tile first: a tile's coded size predicts its
+// serial chain, and wavefronts are dispatched in block order).
+__global__ __launch_bounds__(64) void ebiv_parse(const FrameJob* __restrict__ jobs, const WorkItem* __restrict__ items) {
+  parse_item(jobs, item_of(items));
 }
 
 // ============================================================================

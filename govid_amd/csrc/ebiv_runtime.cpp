@@ -684,15 +684,23 @@ int Engine::run_step(std::vector<Session*>& ss, std::vector<std::vector<StepFram
   // chain, and the chains of the biggest tiles (key frames) would otherwise
   // share their SIMD's issue slots evenly with many short ones; the biggest
   // get priority 3, those above 1/4 and 1/8 of the biggest 2 and 1.
-  size_t w = 0;
+  // Tiles in order of decreasing coded size: wavefronts are dispatched in
+  // block order, so the longest chains start first and the short ones fill
+  // in behind them.
+  std::vector<std::pair<uint32_t, WorkItem>> order;
+  order.reserve(nparse);
   for (size_t j = 0; j < njobs; j++) {
     const PayloadSlot* ps = payload_of(ss, frames, jrefs[j]);
     for (int t = 0; t < ps->ntiles; t++) {
-      const uint64_t len = tiles_of(ps)[t].byte_len;
-      const uint32_t prio = 2 * len >= max_tile_bytes ? 3 : 4 * len >= max_tile_bytes ? 2 : 8 * len >= max_tile_bytes ? 1 : 0;
-      h_items.h[w++] = WorkItem{static_cast<uint32_t>(j), static_cast<uint32_t>(t) | prio << 30};
+      const uint32_t len = tiles_of(ps)[t].byte_len;
+      const uint64_t l = len;
+      const uint32_t prio = 2 * l >= max_tile_bytes ? 3 : 4 * l >= max_tile_bytes ? 2 : 8 * l >= max_tile_bytes ? 1 : 0;
+      order.emplace_back(len, WorkItem{static_cast<uint32_t>(j), static_cast<uint32_t>(t) | prio << 30});
     }
   }
+  std::stable_sort(order.begin(), order.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+  size_t w = 0;
+  for (const auto& o : order) h_items.h[w++] = o.second;
   // Per round and reconstruction lane: inter macroblock groups, then the
   // intra tiles of key-frame payloads (their own launch: many wavefronts per
   // tile and the LDS frontier, for latency), then those of inter payloads

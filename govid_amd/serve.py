@@ -157,14 +157,21 @@ class Server:
                 run.submit_time[run.next] = now
                 run.next += 1
 
-    def run(self, requests, max_seconds: float = 600.0):
-        """Serves `requests` (sorted by arrival) to completion; returns stats."""
+    def run(self, requests, max_seconds: float = 600.0, progress=None, progress_every: float = 5.0):
+        """Serves `requests` (sorted by arrival) to completion; returns stats.
+        `progress`, when given, is called with a one-line status every
+        `progress_every` seconds."""
         pending = sorted(requests, key=lambda r: r.arrival)
         t0 = self.clock()
         k = 0
         steps = 0
+        next_report = progress_every
         while k < len(pending) or self.waiting or self.active:
             now = self.clock() - t0
+            if progress and now >= next_report:
+                next_report += progress_every
+                progress(f"{now:.0f}s: {len(self.frames)} frames, {len(self.active)} active, "
+                         f"{len(self.waiting)} waiting, {len(pending) - k} to arrive")
             if now > max_seconds:
                 raise TimeoutError("serving did not finish in time")
             while k < len(pending) and pending[k].arrival <= now:
@@ -215,19 +222,24 @@ class Server:
                 "peak_session_bytes": self.peak_bytes, "errors": sum(1 for f in self.frames if f.status)}
 
 
-def poisson_requests(streams, n: int, rate_per_s: float, seed: int, seek_frac: float = 0.2, loop_frac: float = 0.1,
-                     min_len: int = 30, max_len: int = 600):
-    """A seeded request mix: Poisson arrivals at `rate_per_s`; streams drawn
-    uniformly; lengths uniform in [min_len, max_len] (clamped to the stream
-    for plays, beyond it for loop restarts); `seek_frac` of the requests start
-    at a random key frame, `loop_frac` play past the end and restart."""
+def poisson_requests(streams, n: int | None, rate_per_s: float, seed: int, seek_frac: float = 0.2,
+                     loop_frac: float = 0.1, min_len: int = 30, max_len: int = 600, until_s: float | None = None):
+    """A seeded request mix: Poisson arrivals at `rate_per_s` (n requests, or
+    every arrival before `until_s` seconds); streams drawn uniformly; lengths
+    uniform in [min_len, max_len] (clamped to the stream for plays, beyond it
+    for loop restarts); `seek_frac` of the requests start at a random key
+    frame, `loop_frac` play past the end and restart."""
     import numpy as np
 
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
-    for rid in range(n):
+    rid = -1
+    while n is None or rid + 1 < n:
+        rid += 1
         t += float(rng.exponential(1.0 / rate_per_s))
+        if until_s is not None and t >= until_s:
+            break
         st = streams[int(rng.integers(0, len(streams)))]
         nf = len(st.packets)
         kind = rng.random()

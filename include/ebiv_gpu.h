@@ -203,9 +203,9 @@ int ebiv_batch_set_timing(ebiv_batch *b, int enabled);
 int ebiv_batch_kernel_times(const ebiv_batch *b, ebiv_kernel_times *out);
 int ebiv_batch_frame_rgba(ebiv_batch *b, const ebiv_frame *f, uint8_t *dst, int dst_on_device);
 
-/* Launch shapes of the batch engine's reconstruction.  No counterpart in the
- * reference (its tile pool, ebiv/pool.go:35-72, has no launch shape); for
- * callers that tune the engine to their mix (govid_amd/tune.py).
+/* Launch shapes of the batch engine.  No counterpart in the reference (its
+ * tile pool, ebiv/pool.go:35-72, has no launch shape); for callers that tune
+ * the engine to their mix (govid_amd/tune.py).
  * intra_waves_key: wavefronts per key-frame tile; intra_waves_few and
  * intra_waves_many: per inter-frame tile, for launches of fewer than / at
  * least many_tiles tiles (1-16).  recon_lanes: HIP streams (1-4) the

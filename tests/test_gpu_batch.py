@@ -97,8 +97,8 @@ def test_sessions_of_mixed_geometry_and_windows():
 ])
 def test_launch_schedules_do_not_change_results(schedule):
     """Intra reconstruction with 1 to 16 wavefronts per tile (key frames from
-    the LDS frontier, inter frames from the frame), and reconstruction split
-    over 2 or 4 streams, bit for bit."""
+    the LDS frontier, inter frames from the frame), reconstruction split over
+    2 or 4 streams, bit for bit."""
     streams = [
         (_clip("rects", 96, 64, 8, qp=22, gop=4, tile_cols=2, tile_rows=2), 3, 4),
         (_clip("parallax", 64, 48, 7, qp=28, gop=7, altref=3), 4, 4),
