@@ -552,6 +552,7 @@ def main():
     ap.add_argument("--serve-budget-gb", type=float, default=16.0)
     ap.add_argument("--serve-window", type=int, default=1,
                     help="frames per session per engine step when serving (1 keeps frame latency lowest)")
+    ap.add_argument("--no-pipeline", action="store_true", help="batch steps one at a time (A/B of the pipeline)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 2 GiB device-to-device copy (scale for HBM byte counters)")
     args = ap.parse_args()
@@ -575,6 +576,8 @@ def main():
 
     from govid_amd.codec import DeviceBuffer
 
+    if args.no_pipeline:
+        SCHEDULE["pipeline"] = 2
     serve_budget = int(args.serve_budget_gb * (1 << 30))
     if args.serve_only:
         sv = run_serving(args.serve_seconds, args.serve_rate, serve_budget, args.serve_window, check_every=17)

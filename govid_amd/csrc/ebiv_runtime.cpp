@@ -311,50 +311,6 @@ struct Session : HostCodec {
 
 };
 
-namespace {
-
-// Assigns a session's frames their target buffers and inputs by walking them
-// in stream order as Go's state machine does (codec.go:174-190).  ok_main /
-// ok_ov are each payload's outcome (all true on the speculative pass).
-// Targets are never buffers live at the step's start, nor each other.
-bool assign_buffers(const Session& s, std::vector<StepFrame>& fr, const std::vector<char>& ok_main,
-                    const std::vector<char>& ok_ov, std::vector<int>& taken) {
-  auto live = [&](int b) {
-    if (b == s.ref || b == s.golden) return true;
-    for (int r : s.ring) if (r == b) return true;
-    for (int t : taken) if (t == b) return true;
-    return false;
-  };
-  auto pick = [&]() {
-    for (int i = 0; i < static_cast<int>(s.pool.size()); i++)
-      if (!live(i)) { taken.push_back(i); return i; }
-    return -1;
-  };
-  int ref = s.ref, golden = s.golden;
-  for (size_t k = 0; k < fr.size(); k++) {
-    StepFrame& f = fr[k];
-    if (f.rc || !f.device_work) continue;
-    if (f.target < 0 && (f.target = pick()) < 0) return false;
-    if (f.raw) continue;  // a raw frame writes its ring image only
-    f.ref_in = ref;
-    f.gold_in = golden;
-    if (f.ov_slot >= 0 && f.gtarget < 0 && (f.gtarget = pick()) < 0) return false;
-    if (!ok_main[k]) continue;
-    if (f.key) {
-      if (f.has_ov) {
-        if (f.ov_rc) continue;
-        golden = f.gtarget;  // decoded, or partially rewritten on failure
-        if (!ok_ov[k]) continue;
-      } else {
-        golden = f.target;
-      }
-    }
-    ref = f.target;
-  }
-  return true;
-}
-
-}  // namespace
 
 // Launch shapes of the reconstruction kernels (see ebiv_batch_set_schedule).
 struct Schedule {
@@ -363,6 +319,7 @@ struct Schedule {
   int intra_waves_many = 4;   // ... launches of at least many_tiles tiles
   int many_tiles = 256;
   int recon_lanes = 1;        // HIP streams the sessions' reconstruction is split over
+  bool pipeline = true;       // next step's entropy launch beside this step's reconstruction
   int inter_intra_waves(size_t ntiles) const {
     return ntiles >= static_cast<size_t>(many_tiles) ? intra_waves_many : intra_waves_few;
   }
@@ -439,50 +396,109 @@ class HostPool {
   bool quit_ = false;
 };
 
-struct Engine {
-  int device = 0;
-  Schedule sched;
-  hipStream_t stream = nullptr;
-  std::vector<std::unique_ptr<Session>> sessions;
-  int next_id = 1;
-  size_t rotate = 0;  // first session served by the next batch step
-  PinnedArray<uint8_t> h_stage;  // descriptors and host packets of a step
+struct JobRef {
+  size_t si, fi;
+  bool ov;
+};
+
+// One step of the engine in flight: the frames it takes from each session,
+// their packets, the staging and job tables of its launches and its tile
+// statuses.  The engine keeps two, so that one step's entropy launch can run
+// while the previous step reconstructs.
+struct StepBuf {
+  std::vector<Session*> ss;  // null: a session closed while the step was in flight
+  std::vector<std::vector<StepFrame>> frames;
+  std::vector<std::vector<Session::Queued>> held;  // the frames' packets
+  std::vector<JobRef> jrefs;
+  size_t nparse = 0;
+  int max_tw = 1;
+  PinnedArray<uint8_t> h_stage;  // descriptors and host packets
   DevArray<uint8_t> d_stage;
   PinnedArray<FrameJob> h_jobs;
   DevArray<FrameJob> d_jobs;
-  PinnedArray<WorkItem> h_items;
+  PinnedArray<WorkItem> h_items;  // entropy items, then reconstruction items
   DevArray<WorkItem> d_items;
-  // Tile statuses of every payload of a step, contiguous: one download
-  DevArray<TileStatus> d_step_status;
-  PinnedArray<TileStatus> h_step_status;
+  DevArray<TileStatus> d_status;  // every payload's tiles, contiguous: one download
+  PinnedArray<TileStatus> h_status;
+  hipEvent_t parsed = nullptr;  // the statuses are on the host
+  std::vector<std::vector<char>> okm, oko;  // per frame: its payloads' tiles all decoded
+  bool live = false;  // entropy launched, step not yet returned
+
+  ~StepBuf() {
+    if (parsed) (void)hipEventDestroy(parsed);
+  }
+  PayloadSlot* payload(const JobRef& r) {
+    const StepFrame& f = frames[r.si][r.fi];
+    return ss[r.si]->slot(r.ov ? f.ov_slot : f.main_slot);
+  }
+  void clear() {
+    ss.clear();
+    frames.clear();
+    held.clear();
+    jrefs.clear();
+    okm.clear();
+    oko.clear();
+    nparse = 0;
+    max_tw = 1;
+    live = false;
+  }
+  size_t frames_of(int session) const {
+    size_t n = 0;
+    for (size_t i = 0; i < ss.size(); i++)
+      if (ss[i] && (session == 0 || ss[i]->id == session)) n += frames[i].size();
+    return n;
+  }
+};
+
+struct Engine {
+  int device = 0;
+  Schedule sched;
+  hipStream_t stream = nullptr;   // reconstruction (lane 0), and everything in the plain codec
+  hipStream_t pstream = nullptr;  // the batch engine's entropy launches
+  std::vector<std::unique_ptr<Session>> sessions;
+  int next_id = 1;
+  size_t rotate = 0;  // first session served by the next batch step
+  StepBuf steps[2];
+  int cur = 0;  // the step whose entropy launch is in flight (or is taken next)
+  std::vector<ebiv_completion> carry;  // completions beyond a call's max_out
   DevArray<uint32_t> d_rgba;
   PinnedArray<RgbaJob> h_present;  // a step's presentation conversions
   DevArray<RgbaJob> d_present;
   std::string msg;
   bool timing = false;
   // Reconstruction lanes beyond `stream` (default priority, created on first
-  // use) and the events that fork them from the entropy launch and join them.
+  // use) and the events that join them.
   static constexpr int kMaxLanes = 4;
   hipStream_t lanes[kMaxLanes - 1] = {};
   hipEvent_t lane_fork = nullptr, lane_join[kMaxLanes - 1] = {};
-  std::vector<hipEvent_t> evs;
-  std::vector<int> marks;  // kind per event pair
-  std::vector<uint64_t> mark_mbs;
+  struct Mark {
+    int kind;
+    uint64_t mbs;
+    hipEvent_t a, b;
+  };
+  std::vector<Mark> marks;      // timed launches not yet collected
+  std::vector<hipEvent_t> spare;  // events of collected marks
   ebiv_kernel_times ktimes{};
 
   std::unique_ptr<HostPool> pool;  // created on the first batch step
 
   ~Engine() {
     pool.reset();
+    if (pstream) (void)hipStreamSynchronize(pstream);
     if (stream) (void)hipStreamSynchronize(stream);
     for (auto l : lanes) if (l) (void)hipStreamSynchronize(l);
     sessions.clear();
-    for (auto e : evs) (void)hipEventDestroy(e);
+    for (auto& m : marks) {
+      (void)hipEventDestroy(m.a);
+      (void)hipEventDestroy(m.b);
+    }
+    for (auto e : spare) (void)hipEventDestroy(e);
     for (int i = 0; i < kMaxLanes - 1; i++) {
       if (lane_join[i]) (void)hipEventDestroy(lane_join[i]);
       if (lanes[i]) (void)hipStreamDestroy(lanes[i]);
     }
     if (lane_fork) (void)hipEventDestroy(lane_fork);
+    if (pstream) (void)hipStreamDestroy(pstream);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -496,9 +512,6 @@ struct Engine {
     return nullptr;
   }
 
-  // Brackets a launch of kind k (0 parse, 1 inter, 2 intra, 3 rgba) with
-  // events while timing is on; returns the pair index or -1.  `mbs`: the
-  // macroblocks an inter launch covers.
   // Streams and events of `n` reconstruction lanes (lane 0 is `stream`).
   bool ensure_lanes(int n) {
     if (!lane_fork && hipEventCreateWithFlags(&lane_fork, hipEventDisableTiming) != hipSuccess) return false;
@@ -510,57 +523,123 @@ struct Engine {
   }
   hipStream_t lane_stream(int g) const { return g == 0 ? stream : lanes[g - 1]; }
 
+  // Brackets a launch of kind k (0 parse, 1 inter, 2 intra, 3 rgba) on
+  // stream st with events while timing is on; returns the mark or -1.
+  // `mbs`: the macroblocks an inter launch covers.
+  hipEvent_t take_event() {
+    if (!spare.empty()) {
+      hipEvent_t e = spare.back();
+      spare.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    return hipEventCreate(&e) == hipSuccess ? e : nullptr;
+  }
   int mark_begin(int kind, uint64_t mbs = 0, hipStream_t st = nullptr) {
     if (!timing) return -1;
-    if (!st) st = stream;
-    const size_t i = marks.size();
-    while (evs.size() < 2 * i + 2) {
-      hipEvent_t e;
-      if (hipEventCreate(&e) != hipSuccess) return -1;
-      evs.push_back(e);
+    Mark m{kind, mbs, take_event(), take_event()};
+    if (!m.a || !m.b || hipEventRecord(m.a, st ? st : stream) != hipSuccess) {
+      if (m.a) spare.push_back(m.a);
+      if (m.b) spare.push_back(m.b);
+      return -1;
     }
-    if (hipEventRecord(evs[2 * i], st) != hipSuccess) return -1;
-    marks.push_back(kind);
-    mark_mbs.push_back(mbs);
-    return static_cast<int>(i);
+    marks.push_back(m);
+    return static_cast<int>(marks.size()) - 1;
   }
   void mark_end(int m, hipStream_t st = nullptr) {
-    if (m >= 0) (void)hipEventRecord(evs[2 * static_cast<size_t>(m) + 1], st ? st : stream);
+    if (m >= 0) (void)hipEventRecord(marks[static_cast<size_t>(m)].b, st ? st : stream);
   }
+  // Adds the timed launches that have finished; the others stay pending.
   void collect_marks() {
-    for (size_t i = 0; i < marks.size(); i++) {
-      float ms = 0;
-      if (hipEventElapsedTime(&ms, evs[2 * i], evs[2 * i + 1]) != hipSuccess) continue;
-      switch (marks[i]) {
-        case 0: ktimes.parse_ms += ms; ktimes.parse_n++; break;
-        case 1: ktimes.inter_ms += ms; ktimes.inter_n++; ktimes.inter_mbs += mark_mbs[i]; break;
-        case 2: ktimes.intra_ms += ms; ktimes.intra_n++; break;
-        default: ktimes.rgba_ms += ms; ktimes.rgba_n++; break;
+    std::vector<Mark> keep;
+    for (const Mark& m : marks) {
+      if (hipEventQuery(m.b) != hipSuccess) {
+        keep.push_back(m);
+        continue;
       }
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, m.a, m.b) == hipSuccess) {
+        switch (m.kind) {
+          case 0: ktimes.parse_ms += ms; ktimes.parse_n++; break;
+          case 1: ktimes.inter_ms += ms; ktimes.inter_n++; ktimes.inter_mbs += m.mbs; break;
+          case 2: ktimes.intra_ms += ms; ktimes.intra_n++; break;
+          default: ktimes.rgba_ms += ms; ktimes.rgba_n++; break;
+        }
+      }
+      spare.push_back(m.a);
+      spare.push_back(m.b);
     }
-    marks.clear();
-    mark_mbs.clear();
+    marks.swap(keep);
   }
 
-  int run_step(std::vector<Session*>& ss, std::vector<std::vector<StepFrame>>& frames);
+  // Phases of a step (see run_plain and the batch step).
+  int launch_entropy(StepBuf& S, hipStream_t ps, bool speculative);
+  int finish_entropy(StepBuf& S);
+  int launch_recon(StepBuf& S);
+  void commit(StepBuf& S);
+  int wait_recon();
+  int run_plain(StepBuf& S);
 };
 
 namespace {
 
-struct JobRef {
-  size_t si, fi;
-  bool ov;
-};
-
-PayloadSlot* payload_of(std::vector<Session*>& ss, std::vector<std::vector<StepFrame>>& frames, const JobRef& r) {
-  const StepFrame& f = frames[r.si][r.fi];
-  return ss[r.si]->slot(r.ov ? f.ov_slot : f.main_slot);
+// Assigns a session's frames their target buffers and inputs by walking them
+// in stream order as Go's state machine does (codec.go:174-190).  ok_main /
+// ok_ov are each payload's outcome.  Targets are never buffers live at the
+// step's start, nor each other.
+bool assign_buffers(const Session& s, std::vector<StepFrame>& fr, const std::vector<char>& ok_main,
+                    const std::vector<char>& ok_ov) {
+  std::vector<int> taken;
+  auto live = [&](int b) {
+    if (b == s.ref || b == s.golden) return true;
+    for (int r : s.ring) if (r == b) return true;
+    for (int t : taken) if (t == b) return true;
+    return false;
+  };
+  auto pick = [&]() {
+    for (int i = 0; i < static_cast<int>(s.pool.size()); i++)
+      if (!live(i)) { taken.push_back(i); return i; }
+    return -1;
+  };
+  int ref = s.ref, golden = s.golden;
+  for (size_t k = 0; k < fr.size(); k++) {
+    StepFrame& f = fr[k];
+    if (f.rc || !f.device_work) continue;
+    if ((f.target = pick()) < 0) return false;
+    if (f.raw) continue;  // a raw frame writes its ring image only
+    f.ref_in = ref;
+    f.gold_in = golden;
+    if (f.ov_slot >= 0 && (f.gtarget = pick()) < 0) return false;
+    if (!ok_main[k]) continue;
+    if (f.key) {
+      if (f.has_ov) {
+        if (f.ov_rc) continue;
+        golden = f.gtarget;  // decoded, or partially rewritten on failure
+        if (!ok_ov[k]) continue;
+      } else {
+        golden = f.target;
+      }
+    }
+    ref = f.target;
+  }
+  return true;
 }
+
+// Whether a payload is reconstructed: its frame's own payload when its tiles
+// decoded; an override when its key decoded (a failing override still
+// rewrites golden up to where it failed, codec.go:176-181).
+bool reconstructs(const StepBuf& S, const JobRef& r) { return S.okm[r.si][r.fi] != 0; }
 
 }  // namespace
 
-int Engine::run_step(std::vector<Session*>& ss, std::vector<std::vector<StepFrame>>& frames) {
-  // ---- staging: host packets and descriptors, one upload
+// Stages the step's packets and descriptors, builds its job table (with the
+// reconstruction's frame buffers when `speculative`: assigned as if every
+// payload decodes, which is exact for a one-frame step), and launches the
+// entropy kernel over every tile of every payload on `ps`, then the status
+// download, then records S.parsed.
+int Engine::launch_entropy(StepBuf& S, hipStream_t ps, bool speculative) {
+  auto& ss = S.ss;
+  auto& frames = S.frames;
   size_t stage = 0;
   for (size_t i = 0; i < ss.size(); i++)
     for (auto& f : frames[i]) {
@@ -572,179 +651,252 @@ int Engine::run_step(std::vector<Session*>& ss, std::vector<std::vector<StepFram
       }
       for (int sl : {f.main_slot, f.ov_slot}) {
         if (sl < 0) continue;
-        PayloadSlot* ps = ss[i]->slot(sl);
-        ps->desc_off = stage;
-        stage = align256(stage + ps->desc_bytes);
+        PayloadSlot* p = ss[i]->slot(sl);
+        p->desc_off = stage;
+        stage = align256(stage + p->desc_bytes);
       }
     }
   if (stage) {
-    if (!h_stage.reserve(stage) || !d_stage.reserve(stage)) return fail(EBIV_ERR_DEVICE, "staging allocation failed");
+    if (!S.h_stage.reserve(stage) || !S.d_stage.reserve(stage)) return fail(EBIV_ERR_DEVICE, "staging allocation failed");
     for (size_t i = 0; i < ss.size(); i++)
       for (auto& f : frames[i]) {
         if (f.rc || !f.device_work) continue;
-        if (f.staged) std::memcpy(h_stage.h + f.staged_off, f.host, f.len);
+        if (f.staged) std::memcpy(S.h_stage.h + f.staged_off, f.host, f.len);
         for (int sl : {f.main_slot, f.ov_slot}) {
           if (sl < 0) continue;
-          PayloadSlot* ps = ss[i]->slot(sl);
-          std::memcpy(h_stage.h + ps->desc_off, ps->hdesc, ps->desc_bytes);
+          PayloadSlot* p = ss[i]->slot(sl);
+          std::memcpy(S.h_stage.h + p->desc_off, p->hdesc, p->desc_bytes);
         }
       }
-    if (hipMemcpyAsync(d_stage.d, h_stage.h, stage, hipMemcpyHostToDevice, stream) != hipSuccess)
+    if (hipMemcpyAsync(S.d_stage.d, S.h_stage.h, stage, hipMemcpyHostToDevice, ps) != hipSuccess)
       return fail(EBIV_ERR_DEVICE, "staging upload failed");
     for (size_t i = 0; i < ss.size(); i++)
       for (auto& f : frames[i])
-        if (f.staged) f.dev = d_stage.d + f.staged_off;
+        if (f.staged) f.dev = S.d_stage.d + f.staged_off;
   }
+  if (speculative)
+    for (size_t i = 0; i < ss.size(); i++) {
+      const std::vector<char> ok(frames[i].size(), 1);
+      if (!assign_buffers(*ss[i], frames[i], ok, ok)) return fail(EBIV_ERR_DEVICE, "frame pool exhausted");
+    }
 
-  // ---- speculative buffer assignment: every payload taken to decode
-  std::vector<std::vector<int>> taken(ss.size());
-  for (size_t i = 0; i < ss.size(); i++) {
-    const std::vector<char> ok(frames[i].size(), 1);
-    if (!assign_buffers(*ss[i], frames[i], ok, ok, taken[i])) return fail(EBIV_ERR_DEVICE, "frame pool exhausted");
-  }
-
-  // ---- jobs: one per coded payload; a session's payloads in stream order
-  std::vector<JobRef> jrefs;
-  std::vector<std::vector<size_t>> rounds;  // job indices of each reconstruction round
-  for (size_t i = 0; i < ss.size(); i++) {
-    size_t u = 0;
+  // ---- jobs: one per coded payload, a session's in stream order
+  S.jrefs.clear();
+  for (size_t i = 0; i < ss.size(); i++)
     for (size_t k = 0; k < frames[i].size(); k++) {
       const StepFrame& f = frames[i][k];
       if (f.rc || !f.device_work || f.raw) continue;
-      for (bool ov : {false, true}) {
-        if (ov && f.ov_slot < 0) continue;
-        if (rounds.size() <= u) rounds.emplace_back();
-        rounds[u++].push_back(jrefs.size());
-        jrefs.push_back({i, k, ov});
-      }
+      S.jrefs.push_back({i, k, false});
+      if (f.ov_slot >= 0) S.jrefs.push_back({i, k, true});
     }
-  }
-  auto fill_job = [&](FrameJob& j, const JobRef& r) {
-    const Session& s = *ss[r.si];
-    const StepFrame& f = frames[r.si][r.fi];
-    PayloadSlot* ps = payload_of(ss, frames, r);
-    j.pkt = f.dev;
-    j.desc = d_stage.d + ps->desc_off;
-    j.recs = ps->recs.d;
-    j.coefs = ps->coefs.d;
-    j.res = ps->res.d;
-    j.status = d_step_status.d + ps->status_off;
-    j.mbc = s.mbc;
-    j.mbr = s.mbr;
-    if (r.ov) {  // the override predicts only from its key (codec.go:178-181)
-      j.rec = s.pool[f.gtarget].p;
-      j.ref = s.pool[f.target].p;
-      j.gold = s.pool[f.target].p;
-    } else {
-      j.rec = s.pool[f.target].p;
-      j.ref = s.pool[f.ref_in].p;
-      j.gold = s.pool[f.gold_in].p;
-    }
-  };
-  // Items: the entropy launch's tiles, then per round its inter macroblock
-  // groups and its intra tiles.
-  // Inter launches take one item per job: {job, its first macroblock group
-  // in the launch}; the kernel finds a workgroup's job among them.
-  auto inter_groups = [&](size_t j) -> uint32_t {
-    const Session& s = *ss[jrefs[j].si];
-    return (static_cast<uint32_t>(s.mbc * s.mbr) + kInterGroupMbs - 1) / kInterGroupMbs;
-  };
-  uint32_t inter_prefix = 0;  // groups so far in the current inter launch
-  auto add_recon_items = [&](size_t j, bool inter_part, size_t& w) {
-    const JobRef& jr = jrefs[j];
-    const PayloadSlot* ps = payload_of(ss, frames, jr);
-    if (inter_part) {  // (key frames too: the launch forms intra residuals)
-      h_items.h[w++] = WorkItem{static_cast<uint32_t>(j), inter_prefix};
-      inter_prefix += inter_groups(j);
-    } else {
-      for (int t = 0; t < ps->ntiles; t++) h_items.h[w++] = WorkItem{static_cast<uint32_t>(j), static_cast<uint32_t>(t)};
-    }
-  };
   size_t nitems = 0, nparse = 0;
   int max_tw = 1;
   uint32_t max_tile_bytes = 1;
-  auto tiles_of = [&](const PayloadSlot* ps) {
-    return reinterpret_cast<const TileDesc*>(ps->hdesc + desc_tiles_off());
-  };
-  for (const JobRef& jr : jrefs) {
-    PayloadSlot* ps = payload_of(ss, frames, jr);
-    ps->status_off = nparse;
-    nparse += static_cast<size_t>(ps->ntiles);
-    nitems += 2 * static_cast<size_t>(ps->ntiles) + 1;
-    max_tw = std::max(max_tw, ps->max_tile_w);
-    for (int t = 0; t < ps->ntiles; t++) max_tile_bytes = std::max(max_tile_bytes, tiles_of(ps)[t].byte_len);
+  auto tiles_of = [](const PayloadSlot* p) { return reinterpret_cast<const TileDesc*>(p->hdesc + desc_tiles_off()); };
+  for (const JobRef& jr : S.jrefs) {
+    PayloadSlot* p = S.payload(jr);
+    p->status_off = nparse;
+    nparse += static_cast<size_t>(p->ntiles);
+    nitems += 2 * static_cast<size_t>(p->ntiles) + 1;
+    max_tw = std::max(max_tw, p->max_tile_w);
+    for (int t = 0; t < p->ntiles; t++) max_tile_bytes = std::max(max_tile_bytes, tiles_of(p)[t].byte_len);
   }
-  const size_t njobs = jrefs.size();
-  if (!h_jobs.reserve(njobs + 1) || !d_jobs.reserve(njobs + 1) || !h_items.reserve(nitems + 1) ||
-      !d_items.reserve(nitems + 1) || !d_step_status.reserve(nparse + 1) || !h_step_status.reserve(nparse + 1))
+  S.nparse = nparse;
+  S.max_tw = max_tw;
+  const size_t njobs = S.jrefs.size();
+  if (!S.h_jobs.reserve(njobs + 1) || !S.d_jobs.reserve(njobs + 1) || !S.h_items.reserve(nitems + 1) ||
+      !S.d_items.reserve(nitems + 1) || !S.d_status.reserve(nparse + 1) || !S.h_status.reserve(nparse + 1))
     return fail(EBIV_ERR_DEVICE, "job table allocation failed");
-  for (size_t j = 0; j < njobs; j++) fill_job(h_jobs.h[j], jrefs[j]);
+  for (size_t j = 0; j < njobs; j++) {
+    const JobRef& r = S.jrefs[j];
+    const Session& s = *ss[r.si];
+    const StepFrame& f = frames[r.si][r.fi];
+    PayloadSlot* p = S.payload(r);
+    FrameJob& jb = S.h_jobs.h[j];
+    std::memset(&jb, 0, sizeof jb);
+    jb.pkt = f.dev;
+    jb.desc = S.d_stage.d + p->desc_off;
+    jb.recs = p->recs.d;
+    jb.coefs = p->coefs.d;
+    jb.res = p->res.d;
+    jb.status = S.d_status.d + p->status_off;
+    jb.mbc = s.mbc;
+    jb.mbr = s.mbr;
+  }
   // Entropy items: one per tile, with an issue priority in bits 30-31 of
   // the tile index.  The launch lasts as long as its longest serial tile
   // chain, and the chains of the biggest tiles (key frames) would otherwise
   // share their SIMD's issue slots evenly with many short ones; the biggest
-  // get priority 3, those above 1/4 and 1/8 of the biggest 2 and 1.
-  // Tiles in order of decreasing coded size: wavefronts are dispatched in
-  // block order, so the longest chains start first and the short ones fill
-  // in behind them.
+  // get priority 3, those above 1/4 and 1/8 of the biggest 2 and 1.  Tiles go
+  // in order of decreasing coded size: wavefronts are dispatched in block
+  // order, so the longest chains start first and the short ones fill in
+  // behind them.
   std::vector<std::pair<uint32_t, WorkItem>> order;
   order.reserve(nparse);
   for (size_t j = 0; j < njobs; j++) {
-    const PayloadSlot* ps = payload_of(ss, frames, jrefs[j]);
-    for (int t = 0; t < ps->ntiles; t++) {
-      const uint32_t len = tiles_of(ps)[t].byte_len;
+    const PayloadSlot* p = S.payload(S.jrefs[j]);
+    for (int t = 0; t < p->ntiles; t++) {
+      const uint32_t len = tiles_of(p)[t].byte_len;
       const uint64_t l = len;
       const uint32_t prio = 2 * l >= max_tile_bytes ? 3 : 4 * l >= max_tile_bytes ? 2 : 8 * l >= max_tile_bytes ? 1 : 0;
       order.emplace_back(len, WorkItem{static_cast<uint32_t>(j), static_cast<uint32_t>(t) | prio << 30});
     }
   }
   std::stable_sort(order.begin(), order.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
-  size_t w = 0;
-  for (const auto& o : order) h_items.h[w++] = o.second;
-  // Per round and reconstruction lane: inter macroblock groups, then the
-  // intra tiles of key-frame payloads (their own launch: many wavefronts per
-  // tile and the LDS frontier, for latency), then those of inter payloads
-  // (few wavefronts per tile, for throughput).  Sessions are dealt to lanes
-  // round robin; a lane's launches run in order on its own stream, and the
-  // lanes overlap (one lane's latency-bound intra launches beside another's
-  // inter launches).  Sessions share nothing between the entropy launch and
-  // the end of the step, so the split leaves every result unchanged.
+  for (size_t w = 0; w < order.size(); w++) S.h_items.h[w] = order[w].second;
+  if (speculative)
+    for (size_t j = 0; j < njobs; j++) {
+      const JobRef& r = S.jrefs[j];
+      const Session& s = *ss[r.si];
+      const StepFrame& f = frames[r.si][r.fi];
+      FrameJob& jb = S.h_jobs.h[j];
+      jb.rec = s.pool[static_cast<size_t>(r.ov ? f.gtarget : f.target)].p;
+      jb.ref = s.pool[static_cast<size_t>(r.ov ? f.target : f.ref_in)].p;
+      jb.gold = s.pool[static_cast<size_t>(r.ov ? f.target : f.gold_in)].p;
+    }
+  if (njobs &&
+      (hipMemcpyAsync(S.d_jobs.d, S.h_jobs.h, sizeof(FrameJob) * njobs, hipMemcpyHostToDevice, ps) != hipSuccess ||
+       hipMemcpyAsync(S.d_items.d, S.h_items.h, sizeof(WorkItem) * nparse, hipMemcpyHostToDevice, ps) != hipSuccess))
+    return fail(EBIV_ERR_DEVICE, "job table upload failed");
+  if (nparse) {
+    const int m = mark_begin(0, 0, ps);
+    launch_parse(ps, static_cast<int>(nparse), max_tw, S.d_jobs.d, S.d_items.d);
+    mark_end(m, ps);
+    if (hipMemcpyAsync(S.h_status.h, S.d_status.d, sizeof(TileStatus) * nparse, hipMemcpyDeviceToHost, ps) !=
+        hipSuccess)
+      return fail(EBIV_ERR_DEVICE, "status download failed");
+  }
+  if (hipGetLastError() != hipSuccess) return fail(EBIV_ERR_DEVICE, "kernel launch failed");
+  if (!S.parsed && hipEventCreateWithFlags(&S.parsed, hipEventDisableTiming) != hipSuccess) {
+    S.parsed = nullptr;
+    return fail(EBIV_ERR_DEVICE, "event creation failed");
+  }
+  if (hipEventRecord(S.parsed, ps) != hipSuccess) return fail(EBIV_ERR_DEVICE, "event record failed");
+  S.live = true;
+  return EBIV_OK;
+}
+
+// Waits for the step's tile statuses and turns them into each payload's
+// outcome (the first failing tile in tile order, decode_intra.go:86-93).
+int Engine::finish_entropy(StepBuf& S) {
+  if (hipEventSynchronize(S.parsed) != hipSuccess) return fail(EBIV_ERR_DEVICE, "decode failed on the device");
+  S.okm.assign(S.ss.size(), {});
+  S.oko.assign(S.ss.size(), {});
+  for (size_t i = 0; i < S.ss.size(); i++) {
+    S.okm[i].assign(S.frames[i].size(), 1);
+    S.oko[i].assign(S.frames[i].size(), 1);
+  }
+  for (const JobRef& jr : S.jrefs) {
+    if (!S.ss[jr.si]) continue;
+    Session& s = *S.ss[jr.si];
+    StepFrame& f = S.frames[jr.si][jr.fi];
+    PayloadSlot* p = S.payload(jr);
+    std::memcpy(p->h_status.h, S.h_status.h + p->status_off, sizeof(TileStatus) * static_cast<size_t>(p->ntiles));
+    if (s.tile_result(p) == EBIV_OK) continue;
+    if (jr.ov) {
+      S.oko[jr.si][jr.fi] = 0;
+      f.ov_msg = s.msg;
+    } else {
+      S.okm[jr.si][jr.fi] = 0;
+      f.msg = s.msg;
+    }
+  }
+  return EBIV_OK;
+}
+
+// Reconstructs the step.  Buffers are assigned from the payloads' real
+// outcomes (unless launch_entropy assigned them for a one-frame step), so
+// every payload is reconstructed once, against the references Go uses.
+// Runs in rounds: round r reconstructs the r-th reconstructed payload of
+// every session.  Per round and lane: inter macroblock groups (key frames
+// too: the launch forms their intra residuals), then the intra tiles of
+// key-frame payloads (their own launch: many wavefronts per tile and the LDS
+// frontier, for latency), then those of inter payloads (few wavefronts per
+// tile, for throughput).  Sessions are dealt to lanes round robin; a lane's
+// launches run in order on its own stream, and the lanes overlap.  Sessions
+// share nothing, so the split leaves every result unchanged.
+int Engine::launch_recon(StepBuf& S) {
+  auto& ss = S.ss;
+  auto& frames = S.frames;
+  const bool exact = !S.okm.empty();  // statuses known: buffers from the real outcomes
+  if (exact) {
+    for (size_t i = 0; i < ss.size(); i++)
+      if (ss[i] && !assign_buffers(*ss[i], frames[i], S.okm[i], S.oko[i])) return fail(EBIV_ERR_DEVICE, "frame pool exhausted");
+    for (size_t j = 0; j < S.jrefs.size(); j++) {
+      const JobRef& r = S.jrefs[j];
+      if (!ss[r.si]) continue;
+      const Session& s = *ss[r.si];
+      const StepFrame& f = frames[r.si][r.fi];
+      FrameJob& jb = S.h_jobs.h[j];
+      jb.rec = s.pool[static_cast<size_t>(r.ov ? f.gtarget : f.target)].p;
+      jb.ref = s.pool[static_cast<size_t>(r.ov ? f.target : f.ref_in)].p;
+      jb.gold = s.pool[static_cast<size_t>(r.ov ? f.target : f.gold_in)].p;
+    }
+  }
   const int nl = std::max(1, std::min({sched.recon_lanes, static_cast<int>(kMaxLanes), static_cast<int>(ss.size())}));
   if (nl > 1 && !ensure_lanes(nl)) return fail(EBIV_ERR_DEVICE, "stream creation failed");
+  auto lane_of = [&](size_t si) { return static_cast<int>(si % static_cast<size_t>(nl)); };
+  // rounds: the reconstructed jobs of each session in order
+  std::vector<std::vector<size_t>> rounds;
+  {
+    std::vector<size_t> u(ss.size(), 0);
+    for (size_t j = 0; j < S.jrefs.size(); j++) {
+      const JobRef& r = S.jrefs[j];
+      if (!ss[r.si] || (exact && !reconstructs(S, r))) continue;
+      if (rounds.size() <= u[r.si]) rounds.emplace_back();
+      rounds[u[r.si]++].push_back(j);
+    }
+  }
+  auto inter_groups = [&](size_t j) -> uint32_t {
+    const Session& s = *ss[S.jrefs[j].si];
+    return (static_cast<uint32_t>(s.mbc * s.mbr) + kInterGroupMbs - 1) / kInterGroupMbs;
+  };
+  auto is_key = [&](size_t j) { return !S.payload(S.jrefs[j])->inter; };
+  size_t w = S.nparse;
   const size_t nrl = rounds.size() * static_cast<size_t>(nl);
   std::vector<size_t> inter_off(nrl), intra_off(nrl), key_end(nrl), round_end(nrl);
   std::vector<uint32_t> inter_groups_of(nrl);
   std::vector<uint64_t> inter_mbs_of(nrl);
-  auto is_key = [&](size_t j) { return !payload_of(ss, frames, jrefs[j])->inter; };
-  auto lane_of = [&](size_t j) { return static_cast<int>(jrefs[j].si % static_cast<size_t>(nl)); };
   for (size_t r = 0; r < rounds.size(); r++)
     for (int g = 0; g < nl; g++) {
-      const size_t q = r * nl + g;
+      const size_t q = r * static_cast<size_t>(nl) + static_cast<size_t>(g);
+      // Inter launches take one item per job: {job, its first macroblock
+      // group in the launch}; the kernel finds a workgroup's job among them.
       inter_off[q] = w;
-      inter_prefix = 0;
+      uint32_t prefix = 0;
       for (size_t j : rounds[r])
-        if (lane_of(j) == g) {
-          add_recon_items(j, true, w);
-          if (!is_key(j)) inter_mbs_of[q] += static_cast<uint64_t>(ss[jrefs[j].si]->mbc) * ss[jrefs[j].si]->mbr;
+        if (lane_of(S.jrefs[j].si) == g) {
+          S.h_items.h[w++] = WorkItem{static_cast<uint32_t>(j), prefix};
+          prefix += inter_groups(j);
+          if (!is_key(j)) inter_mbs_of[q] += static_cast<uint64_t>(ss[S.jrefs[j].si]->mbc) * ss[S.jrefs[j].si]->mbr;
         }
-      inter_groups_of[q] = inter_prefix;
+      inter_groups_of[q] = prefix;
       intra_off[q] = w;
-      for (size_t j : rounds[r]) if (lane_of(j) == g && is_key(j)) add_recon_items(j, false, w);
-      key_end[q] = w;
-      for (size_t j : rounds[r]) if (lane_of(j) == g && !is_key(j)) add_recon_items(j, false, w);
+      for (int pass = 0; pass < 2; pass++) {  // key-frame tiles, then inter-frame tiles
+        for (size_t j : rounds[r])
+          if (lane_of(S.jrefs[j].si) == g && is_key(j) == (pass == 0))
+            for (int t = 0; t < S.payload(S.jrefs[j])->ntiles; t++)
+              S.h_items.h[w++] = WorkItem{static_cast<uint32_t>(j), static_cast<uint32_t>(t)};
+        if (pass == 0) key_end[q] = w;
+      }
       round_end[q] = w;
     }
-  if (njobs &&
-      (hipMemcpyAsync(d_jobs.d, h_jobs.h, sizeof(FrameJob) * njobs, hipMemcpyHostToDevice, stream) != hipSuccess ||
-       hipMemcpyAsync(d_items.d, h_items.h, sizeof(WorkItem) * w, hipMemcpyHostToDevice, stream) != hipSuccess))
+  if (exact && !S.jrefs.empty() &&
+      hipMemcpyAsync(S.d_jobs.d, S.h_jobs.h, sizeof(FrameJob) * S.jrefs.size(), hipMemcpyHostToDevice, stream) !=
+          hipSuccess)
+    return fail(EBIV_ERR_DEVICE, "job table upload failed");
+  if (w > S.nparse &&
+      hipMemcpyAsync(S.d_items.d + S.nparse, S.h_items.h + S.nparse, sizeof(WorkItem) * (w - S.nparse),
+                     hipMemcpyHostToDevice, stream) != hipSuccess)
     return fail(EBIV_ERR_DEVICE, "job table upload failed");
 
   // ---- raw frames: packed planes into their buffers (codec.go:195-207)
-  for (size_t i = 0; i < ss.size(); i++)
+  for (size_t i = 0; i < ss.size(); i++) {
+    if (!ss[i]) continue;
     for (auto& f : frames[i]) {
       if (f.rc || !f.raw) continue;
       const Session& s = *ss[i];
-      const DevBuf& b = s.pool[f.target];
+      const DevBuf& b = s.pool[static_cast<size_t>(f.target)];
       const uint8_t* src = f.dev + f.body_off;
       const size_t ysz = static_cast<size_t>(s.W) * s.H, csz = static_cast<size_t>(s.CW) * s.CH;
       if (hipMemcpy2DAsync(b.p.y, b.p.pw, src, s.W, s.W, s.H, hipMemcpyDeviceToDevice, stream) != hipSuccess ||
@@ -752,48 +904,44 @@ int Engine::run_step(std::vector<Session*>& ss, std::vector<std::vector<StepFram
           hipMemcpy2DAsync(b.p.cr, b.p.cw, src + ysz + csz, s.CW, s.CW, s.CH, hipMemcpyDeviceToDevice, stream) != hipSuccess)
         return fail(EBIV_ERR_DEVICE, "raw frame copy failed");
     }
-
-  // ---- one entropy launch over every tile of every payload
-  if (nparse) {
-    const int m = mark_begin(0);
-    launch_parse(stream, static_cast<int>(nparse), max_tw, d_jobs.d, d_items.d);
-    mark_end(m);
+  }
+  if (nl > 1) {  // the lanes start after the uploads above
+    if (hipEventRecord(lane_fork, stream) != hipSuccess) return fail(EBIV_ERR_DEVICE, "event record failed");
+    for (int g = 1; g < nl; g++)
+      if (hipStreamWaitEvent(lanes[g - 1], lane_fork, 0) != hipSuccess) return fail(EBIV_ERR_DEVICE, "stream wait failed");
   }
   // A golden override decodes over a copy of the golden it replaces, so a
   // failing override leaves Go's partial rewrite (codec.go:178-181).
   auto golden_copy = [&](const JobRef& jr, hipStream_t st) -> bool {
     const Session& s = *ss[jr.si];
     const StepFrame& f = frames[jr.si][jr.fi];
-    const DevBuf& src = s.pool[f.gold_in];
-    return hipMemcpyAsync(s.pool[f.gtarget].base, src.base, src.bytes, hipMemcpyDeviceToDevice, st) == hipSuccess;
+    const DevBuf& src = s.pool[static_cast<size_t>(f.gold_in)];
+    return hipMemcpyAsync(s.pool[static_cast<size_t>(f.gtarget)].base, src.base, src.bytes, hipMemcpyDeviceToDevice,
+                          st) == hipSuccess;
   };
-  if (nl > 1) {
-    if (hipEventRecord(lane_fork, stream) != hipSuccess) return fail(EBIV_ERR_DEVICE, "event record failed");
-    for (int g = 1; g < nl; g++)
-      if (hipStreamWaitEvent(lanes[g - 1], lane_fork, 0) != hipSuccess) return fail(EBIV_ERR_DEVICE, "stream wait failed");
-  }
   for (size_t r = 0; r < rounds.size(); r++) {
     for (int g = 0; g < nl; g++) {
-      const size_t q = r * nl + g;
+      const size_t q = r * static_cast<size_t>(nl) + static_cast<size_t>(g);
       hipStream_t st = lane_stream(g);
       for (size_t j : rounds[r])
-        if (lane_of(j) == g && jrefs[j].ov && !golden_copy(jrefs[j], st)) return fail(EBIV_ERR_DEVICE, "golden copy failed");
+        if (lane_of(S.jrefs[j].si) == g && S.jrefs[j].ov && !golden_copy(S.jrefs[j], st))
+          return fail(EBIV_ERR_DEVICE, "golden copy failed");
       if (intra_off[q] > inter_off[q]) {
         const int m = mark_begin(1, inter_mbs_of[q], st);
-        launch_recon_inter(st, static_cast<int>(inter_groups_of[q]), d_jobs.d, d_items.d + inter_off[q],
+        launch_recon_inter(st, static_cast<int>(inter_groups_of[q]), S.d_jobs.d, S.d_items.d + inter_off[q],
                            static_cast<int>(intra_off[q] - inter_off[q]));
         mark_end(m, st);
       }
       if (key_end[q] > intra_off[q]) {
         const int m = mark_begin(2, 0, st);
-        launch_recon_intra(st, static_cast<int>(key_end[q] - intra_off[q]), d_jobs.d, d_items.d + intra_off[q],
+        launch_recon_intra(st, static_cast<int>(key_end[q] - intra_off[q]), S.d_jobs.d, S.d_items.d + intra_off[q],
                            sched.intra_waves_key, true);
         mark_end(m, st);
       }
       if (round_end[q] > key_end[q]) {
         const size_t n = round_end[q] - key_end[q];
         const int m = mark_begin(2, 0, st);
-        launch_recon_intra(st, static_cast<int>(n), d_jobs.d, d_items.d + key_end[q], sched.inter_intra_waves(n),
+        launch_recon_intra(st, static_cast<int>(n), S.d_jobs.d, S.d_items.d + key_end[q], sched.inter_intra_waves(n),
                            false);
         mark_end(m, st);
       }
@@ -803,82 +951,20 @@ int Engine::run_step(std::vector<Session*>& ss, std::vector<std::vector<StepFram
     if (hipEventRecord(lane_join[g - 1], lanes[g - 1]) != hipSuccess ||
         hipStreamWaitEvent(stream, lane_join[g - 1], 0) != hipSuccess)
       return fail(EBIV_ERR_DEVICE, "lane join failed");
-  if (nparse && hipMemcpyAsync(h_step_status.h, d_step_status.d, sizeof(TileStatus) * nparse, hipMemcpyDeviceToHost,
-                                stream) != hipSuccess)
-    return fail(EBIV_ERR_DEVICE, "status download failed");
   if (hipGetLastError() != hipSuccess) return fail(EBIV_ERR_DEVICE, "kernel launch failed");
-  if (hipStreamSynchronize(stream) != hipSuccess) return fail(EBIV_ERR_DEVICE, "decode failed on the device");
-  collect_marks();
-  for (const JobRef& jr : jrefs) {
-    PayloadSlot* ps = payload_of(ss, frames, jr);
-    std::memcpy(ps->h_status.h, h_step_status.h + ps->status_off, sizeof(TileStatus) * ps->ntiles);
-  }
+  return EBIV_OK;
+}
 
-  // ---- outcomes; where a payload failed, the session's later payloads are
-  // reconstructed again against the buffers Go would have used
-  std::vector<std::vector<char>> okm(ss.size()), oko(ss.size());
-  for (size_t i = 0; i < ss.size(); i++) {
-    okm[i].assign(frames[i].size(), 1);
-    oko[i].assign(frames[i].size(), 1);
-  }
-  std::vector<char> any_fail(ss.size(), 0);
-  for (const JobRef& jr : jrefs) {
-    Session& s = *ss[jr.si];
-    StepFrame& f = frames[jr.si][jr.fi];
-    if (s.tile_result(payload_of(ss, frames, jr)) == EBIV_OK) continue;
-    any_fail[jr.si] = 1;
-    if (jr.ov) {
-      oko[jr.si][jr.fi] = 0;
-      f.ov_msg = s.msg;
-    } else {
-      okm[jr.si][jr.fi] = 0;
-      f.msg = s.msg;
-    }
-  }
-  for (size_t i = 0; i < ss.size(); i++) {
-    if (!any_fail[i]) continue;
-    std::vector<std::pair<int, int>> before;
-    for (auto& f : frames[i]) before.push_back({f.ref_in, f.gold_in});
-    if (!assign_buffers(*ss[i], frames[i], okm[i], oko[i], taken[i])) return fail(EBIV_ERR_DEVICE, "frame pool exhausted");
-    bool dirty = false;
-    for (size_t k = 0; k < frames[i].size(); k++) {
-      StepFrame& f = frames[i][k];
-      if (f.rc || !f.device_work || f.raw) continue;
-      dirty = dirty || before[k].first != f.ref_in || before[k].second != f.gold_in;
-      if (!dirty) continue;
-      for (bool ov : {false, true}) {
-        if (ov && f.ov_slot < 0) continue;
-        const JobRef jr{i, k, ov};
-        const PayloadSlot* ps = payload_of(ss, frames, jr);
-        const Session& s = *ss[i];
-        fill_job(h_jobs.h[0], jr);
-        size_t n = 0;
-        h_items.h[n++] = WorkItem{0, 0};
-        const size_t ninter = n;
-        const uint32_t groups = (static_cast<uint32_t>(s.mbc * s.mbr) + kInterGroupMbs - 1) / kInterGroupMbs;
-        for (int t = 0; t < ps->ntiles; t++) h_items.h[n++] = WorkItem{0, static_cast<uint32_t>(t)};
-        if (ov && !golden_copy(jr, stream)) return fail(EBIV_ERR_DEVICE, "golden copy failed");
-        if (hipMemcpyAsync(d_jobs.d, h_jobs.h, sizeof(FrameJob), hipMemcpyHostToDevice, stream) != hipSuccess ||
-            hipMemcpyAsync(d_items.d, h_items.h, sizeof(WorkItem) * n, hipMemcpyHostToDevice, stream) != hipSuccess)
-          return fail(EBIV_ERR_DEVICE, "job table upload failed");
-        if (ninter) launch_recon_inter(stream, static_cast<int>(groups), d_jobs.d, d_items.d, 1);
-        launch_recon_intra(stream, ps->ntiles, d_jobs.d, d_items.d + ninter,
-                           ps->inter ? sched.inter_intra_waves(static_cast<size_t>(ps->ntiles)) : sched.intra_waves_key,
-                           !ps->inter);
-        if (hipGetLastError() != hipSuccess) return fail(EBIV_ERR_DEVICE, "kernel launch failed");
-        // the pinned tables are rewritten by the next payload
-        if (hipStreamSynchronize(stream) != hipSuccess) return fail(EBIV_ERR_DEVICE, "decode failed on the device");
-      }
-    }
-  }
-
-  // ---- Go's state transitions in stream order (codec.go:84-110, 174-190)
-  for (size_t i = 0; i < ss.size(); i++) {
-    Session& s = *ss[i];
-    for (size_t k = 0; k < frames[i].size(); k++) {
-      StepFrame& f = frames[i][k];
+// Go's state transitions in stream order (codec.go:84-110, 174-190), from
+// the payloads' outcomes.
+void Engine::commit(StepBuf& S) {
+  for (size_t i = 0; i < S.ss.size(); i++) {
+    if (!S.ss[i]) continue;
+    Session& s = *S.ss[i];
+    for (size_t k = 0; k < S.frames[i].size(); k++) {
+      StepFrame& f = S.frames[i][k];
       if (f.rc) continue;  // failed on the host: nothing ran on the device
-      const int st = s.resolve(f, okm[i][k] != 0, oko[i][k] != 0);
+      const int st = s.resolve(f, S.okm[i][k] != 0, S.oko[i][k] != 0);
       if (f.golden_written) s.golden = f.has_ov ? f.gtarget : f.target;
       if (st) {
         f.rc = st;
@@ -890,6 +976,27 @@ int Engine::run_step(std::vector<Session*>& ss, std::vector<std::vector<StepFram
       s.stats.frames++;
     }
   }
+}
+
+int Engine::wait_recon() {
+  if (hipStreamSynchronize(stream) != hipSuccess) return fail(EBIV_ERR_DEVICE, "decode failed on the device");
+  collect_marks();
+  return EBIV_OK;
+}
+
+// A one-frame step of the plain codec: entropy and reconstruction back to
+// back on one stream (buffers assigned as if the frame decodes: exact for a
+// lone frame, whose failure discards its output, and whose failing golden
+// override keeps the partial rewrite the reconstruction made), then one
+// synchronisation.
+int Engine::run_plain(StepBuf& S) {
+  S.okm.clear();
+  if (int rc = launch_entropy(S, stream, true)) return rc;
+  if (int rc = launch_recon(S)) return rc;
+  if (int rc = wait_recon()) return rc;
+  if (int rc = finish_entropy(S)) return rc;
+  commit(S);
+  S.live = false;
   return EBIV_OK;
 }
 
@@ -949,6 +1056,10 @@ int engine_init(Engine& e, int device) {
     e.stream = nullptr;
     return EBIV_ERR_DEVICE;
   }
+  if (hipStreamCreateWithFlags(&e.pstream, hipStreamNonBlocking) != hipSuccess) {
+    e.pstream = nullptr;
+    return EBIV_ERR_DEVICE;
+  }
   return EBIV_OK;
 }
 
@@ -958,10 +1069,12 @@ int decode_one(ebiv_codec* c, const uint8_t* data, const uint8_t* dev, size_t le
   Engine& e = c->eng;
   if (!data && len) return s.fail(EBIV_ERR_ARGUMENT, "null packet");
   if (hipSetDevice(e.device) != hipSuccess) return s.fail(EBIV_ERR_DEVICE, "hipSetDevice failed");
-  std::vector<Session*> ss{&s};
-  std::vector<std::vector<StepFrame>> frames(1);
-  frames[0].emplace_back();
-  StepFrame& f = frames[0][0];
+  StepBuf& S = e.steps[0];
+  S.clear();
+  S.ss.push_back(&s);
+  S.frames.resize(1);
+  S.frames[0].emplace_back();
+  StepFrame& f = S.frames[0][0];
   f.host = data;
   f.dev = dev;
   f.len = len;
@@ -969,7 +1082,9 @@ int decode_one(ebiv_codec* c, const uint8_t* data, const uint8_t* dev, size_t le
   int next_slot = 0;
   s.prepare_frame(f, &next_slot);
   if (f.rc) return s.fail(f.rc, f.msg);
-  if (int rc = e.run_step(ss, frames)) return s.fail(rc, e.msg);
+  const int rc = e.run_plain(S);
+  S.live = false;
+  if (rc) return s.fail(rc, e.msg);
   if (f.rc) return s.fail(f.rc, f.msg);
   if (!dev) {
     if (int rc = download(e, s, f.out_buf, f.ring_slot)) return rc;
@@ -1156,11 +1271,21 @@ int ebiv_batch_open(ebiv_batch* b, int ring_size, int window, int* session) {
 
 int ebiv_batch_close(ebiv_batch* b, int session) {
   if (!b) return EBIV_ERR_ARGUMENT;
-  auto& v = b->eng.sessions;
+  Engine& e = b->eng;
+  auto& v = e.sessions;
   for (size_t i = 0; i < v.size(); i++)
     if (v[i]->id == session) {
-      (void)hipSetDevice(b->eng.device);
-      (void)hipStreamSynchronize(b->eng.stream);
+      (void)hipSetDevice(e.device);
+      (void)hipStreamSynchronize(e.pstream);
+      (void)hipStreamSynchronize(e.stream);
+      // A step in flight (its entropy launch queued by the last call) drops
+      // the session's frames: no completions for a closed session.
+      for (auto& S : e.steps)
+        for (auto& p : S.ss)
+          if (p == v[i].get()) p = nullptr;
+      e.carry.erase(std::remove_if(e.carry.begin(), e.carry.end(),
+                                   [&](const ebiv_completion& c) { return c.session == session; }),
+                    e.carry.end());
       v.erase(v.begin() + static_cast<long>(i));
       return EBIV_OK;
     }
@@ -1205,50 +1330,53 @@ int ebiv_batch_flush(ebiv_batch* b, int session) {
 
 int ebiv_batch_pending(const ebiv_batch* b, int session) {
   if (!b) return 0;
+  const Engine& e = b->eng;
   int n = 0;
-  for (auto& s : b->eng.sessions)
+  for (auto& s : e.sessions)
     if (session == 0 || s->id == session) n += static_cast<int>(s->queue.size());
+  for (const auto& S : e.steps)  // taken by a step in flight
+    if (S.live) n += static_cast<int>(S.frames_of(session));
+  for (const auto& c : e.carry)  // decoded, not yet returned
+    if (session == 0 || c.session == session) n++;
   return n;
 }
 
-int ebiv_batch_step(ebiv_batch* b, ebiv_completion* out, int max_out, int* n_out) {
-  if (!b || !n_out || max_out < 0 || (max_out > 0 && !out)) return EBIV_ERR_ARGUMENT;
-  *n_out = 0;
-  Engine& e = b->eng;
-  if (hipSetDevice(e.device) != hipSuccess) return e.fail(EBIV_ERR_DEVICE, "hipSetDevice failed");
-  std::vector<Session*> ss;
-  std::vector<std::vector<StepFrame>> frames;
-  std::vector<std::vector<Session::Queued>> held;
-  int budget = max_out;
-  // Take up to a window of packets from every session, starting one session
-  // further on each step, so a max_out below the queued work is shared ...
+namespace {
+
+constexpr int kSlotsPerStep = 128;  // payload slots of a session per step buffer (window + an override)
+
+// Takes up to a window of queued packets from every session into S (at most
+// `budget` in all), starting one session further on each step so that a
+// budget below the queued work is shared, and parses their headers and
+// tables, sessions in parallel, each in stream order.  A key frame with a
+// golden override ends its session's window: the packets after it go back
+// to the front of the queue.
+void take_step(Engine& e, StepBuf& S, int budget, int slot_base) {
+  S.clear();
   const size_t ns = e.sessions.size();
   const size_t first = ns ? e.rotate++ % ns : 0;
   for (size_t n = 0; n < ns; n++) {
     Session& s = *e.sessions[(first + n) % ns];
     if (s.queue.empty() || budget <= 0) continue;
-    ss.push_back(&s);
-    frames.emplace_back();
-    held.emplace_back();
+    S.ss.push_back(&s);
+    S.frames.emplace_back();
+    S.held.emplace_back();
     const int window = std::min(s.window, budget);
-    frames.back().reserve(static_cast<size_t>(window));
-    held.back().reserve(static_cast<size_t>(window));
-    while (!s.queue.empty() && static_cast<int>(held.back().size()) < window) {
-      held.back().push_back(std::move(s.queue.front()));
+    S.frames.back().reserve(static_cast<size_t>(window));
+    S.held.back().reserve(static_cast<size_t>(window));
+    while (!s.queue.empty() && static_cast<int>(S.held.back().size()) < window) {
+      S.held.back().push_back(std::move(s.queue.front()));
       s.queue.pop_front();
       budget--;
     }
   }
-  if (ss.empty()) return EBIV_OK;
-  // ... and parse them, sessions in parallel, each in stream order.  A key
-  // frame with a golden override ends its session's window: the packets
-  // after it go back to the front of the queue.
+  if (S.ss.empty()) return;
   if (!e.pool) e.pool.reset(new HostPool(e.device));
-  e.pool->parallel_for(static_cast<int>(ss.size()), [&](int i) {
-    Session& s = *ss[static_cast<size_t>(i)];
-    std::vector<StepFrame>& fr = frames[static_cast<size_t>(i)];
-    std::vector<Session::Queued>& h = held[static_cast<size_t>(i)];
-    int next_slot = 0;
+  e.pool->parallel_for(static_cast<int>(S.ss.size()), [&](int i) {
+    Session& s = *S.ss[static_cast<size_t>(i)];
+    std::vector<StepFrame>& fr = S.frames[static_cast<size_t>(i)];
+    std::vector<Session::Queued>& h = S.held[static_cast<size_t>(i)];
+    int next_slot = slot_base;
     size_t k = 0;
     while (k < h.size()) {
       const Session::Queued& q = h[k++];
@@ -1264,23 +1392,67 @@ int ebiv_batch_step(ebiv_batch* b, ebiv_completion* out, int max_out, int* n_out
     for (size_t j = h.size(); j > k; j--) s.queue.push_front(std::move(h[j - 1]));
     h.resize(k);
   });
-  if (int rc = e.run_step(ss, frames)) return rc;
-  int n = 0;
+}
+
+}  // namespace
+
+// A step: the entropy launch of the current step buffer (already in flight
+// when the previous call launched it), its tile statuses, reconstruction
+// from the real outcomes and Go's state transitions; then, with the
+// pipeline on, the next step's packets are taken and its entropy launch
+// queued on the entropy stream, where it runs beside this step's
+// reconstruction; then this step's completions.
+int ebiv_batch_step(ebiv_batch* b, ebiv_completion* out, int max_out, int* n_out) {
+  if (!b || !n_out || max_out < 0 || (max_out > 0 && !out)) return EBIV_ERR_ARGUMENT;
+  *n_out = 0;
+  Engine& e = b->eng;
+  if (hipSetDevice(e.device) != hipSuccess) return e.fail(EBIV_ERR_DEVICE, "hipSetDevice failed");
+  if (!e.carry.empty()) {  // completions an earlier call had no room for
+    const size_t give = std::min(e.carry.size(), static_cast<size_t>(max_out));
+    std::copy(e.carry.begin(), e.carry.begin() + static_cast<long>(give), out);
+    e.carry.erase(e.carry.begin(), e.carry.begin() + static_cast<long>(give));
+    *n_out = static_cast<int>(give);
+    return EBIV_OK;
+  }
+  const bool pipe = e.sched.pipeline;
+  hipStream_t ps = pipe ? e.pstream : e.stream;
+  StepBuf& S = e.steps[e.cur];
+  if (!S.live) {
+    take_step(e, S, max_out, e.cur * kSlotsPerStep);
+    if (S.ss.empty()) return EBIV_OK;
+    if (int rc = e.launch_entropy(S, ps, false)) return rc;
+  }
+  if (int rc = e.finish_entropy(S)) return rc;
+  if (int rc = e.launch_recon(S)) return rc;
+  e.commit(S);
+  if (pipe) {
+    const int nx = e.cur ^ 1;
+    StepBuf& N = e.steps[nx];
+    take_step(e, N, max_out, nx * kSlotsPerStep);
+    if (!N.ss.empty())
+      if (int rc = e.launch_entropy(N, ps, false)) return rc;
+    e.cur = nx;
+  }
+  if (int rc = e.wait_recon()) return rc;
   size_t npresent = 0, nframes = 0;
   uint32_t max_px8 = 0;
-  for (auto& fr : frames) nframes += fr.size();
+  for (auto& fr : S.frames) nframes += fr.size();
   // (reserve does not keep contents: once, before the table is filled)
   if (!e.h_present.reserve(nframes + 1)) return e.fail(EBIV_ERR_DEVICE, "presentation table allocation failed");
-  for (size_t i = 0; i < ss.size(); i++)
-    for (auto& f : frames[i]) {
-      ebiv_completion& c = out[n++];
+  std::vector<ebiv_completion> done;
+  done.reserve(nframes);
+  for (size_t i = 0; i < S.ss.size(); i++) {
+    if (!S.ss[i]) continue;  // closed while in flight
+    for (auto& f : S.frames[i]) {
+      done.emplace_back();
+      ebiv_completion& c = done.back();
       std::memset(&c, 0, sizeof c);
-      c.session = ss[i]->id;
+      c.session = S.ss[i]->id;
       c.status = f.rc;
       c.tag = f.tag;
       c.surface = -1;
       if (f.rc == EBIV_OK) {
-        Session& s = *ss[i];
+        Session& s = *S.ss[i];
         fill_frame(s, f.out_buf, f.ring_slot, f.ts, true, &c.frame);
         if (!s.surfaces.empty() && s.surface_pitch >= 4 * static_cast<size_t>(s.W)) {
           c.surface = s.next_surface;
@@ -1292,9 +1464,10 @@ int ebiv_batch_step(ebiv_batch* b, ebiv_completion* out, int max_out, int* n_out
         }
       } else {
         std::strncpy(c.message, f.msg.c_str(), sizeof c.message - 1);
-        ss[i]->msg = f.msg;
+        S.ss[i]->msg = f.msg;
       }
     }
+  }
   // ---- presentation: every presented frame of the step in one launch
   if (npresent) {
     if (!e.d_present.reserve(npresent)) return e.fail(EBIV_ERR_DEVICE, "presentation table allocation failed");
@@ -1308,9 +1481,14 @@ int ebiv_batch_step(ebiv_batch* b, ebiv_completion* out, int max_out, int* n_out
       return e.fail(EBIV_ERR_DEVICE, "presentation failed on the device");
     e.collect_marks();
   }
-  *n_out = n;
+  S.clear();
+  const size_t give = std::min(done.size(), static_cast<size_t>(max_out));
+  std::copy(done.begin(), done.begin() + static_cast<long>(give), out);
+  e.carry.insert(e.carry.end(), done.begin() + static_cast<long>(give), done.end());
+  *n_out = static_cast<int>(give);
   return EBIV_OK;
 }
+
 
 const char* ebiv_batch_last_error(const ebiv_batch* b) { return b ? b->eng.msg.c_str() : "null batch"; }
 
@@ -1362,13 +1540,18 @@ int ebiv_batch_set_schedule(ebiv_batch* b, const ebiv_schedule* sc) {
   waves(sc->intra_waves_many, d.intra_waves_many);
   if (sc->many_tiles > 0) d.many_tiles = sc->many_tiles;
   if (sc->recon_lanes > 0) d.recon_lanes = sc->recon_lanes > Engine::kMaxLanes ? Engine::kMaxLanes : sc->recon_lanes;
+  // (a step in flight is finished by the next call either way: it waits for
+  // its entropy launch's event, on whichever stream that ran)
+  if (sc->pipeline > 2) return EBIV_ERR_ARGUMENT;
+  if (sc->pipeline > 0) d.pipeline = sc->pipeline == 1;
   return EBIV_OK;
 }
 
 int ebiv_batch_get_schedule(const ebiv_batch* b, ebiv_schedule* out) {
   if (!b || !out) return EBIV_ERR_ARGUMENT;
   const Schedule& d = b->eng.sched;
-  *out = ebiv_schedule{d.intra_waves_key, d.intra_waves_few, d.intra_waves_many, d.many_tiles, d.recon_lanes};
+  *out = ebiv_schedule{d.intra_waves_key, d.intra_waves_few, d.intra_waves_many, d.many_tiles, d.recon_lanes,
+                       d.pipeline ? 1 : 2};
   return EBIV_OK;
 }
 
@@ -1383,8 +1566,10 @@ size_t ebiv_session_bytes(int width, int height, int ring_size, int window) {
   const size_t ysz = mbc * kMB * mbr * kMB, csz = align256(mbc * kCMB * mbr * kCMB);
   const size_t rs = static_cast<size_t>(std::max(ring_size >= 2 ? ring_size : 3, window));
   const size_t pool = (rs + 2 + static_cast<size_t>(window) + 1) * (ysz + 2 * csz);
-  // payload slots: a window's payloads plus an override
-  const size_t payload = (static_cast<size_t>(window) + 1) * mbc * mbr * (sizeof(MBRec) + kCoefPerMB * 4 + kResPerMB * 2);
+  // payload slots: a window's payloads plus an override, in each of the
+  // engine's two step buffers (one reconstructing while the next parses)
+  const size_t payload =
+      2 * (static_cast<size_t>(window) + 1) * mbc * mbr * (sizeof(MBRec) + kCoefPerMB * 4 + kResPerMB * 2);
   return pool + payload;
 }
 

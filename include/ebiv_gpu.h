@@ -173,15 +173,23 @@ int ebiv_batch_submit_many(ebiv_batch *b, int session, int n, const uint8_t *con
                            const uint64_t *tags);
 
 /* Codec.Flush for a session (codec.go:115-117), dropping its queued packets:
- * the seek path (source.go:45-50 and 226-234 seek the demuxer, then flush). */
+ * the seek path (source.go:45-50 and 226-234 seek the demuxer, then flush).
+ * Packets a step in flight has taken were decoded before the flush, in
+ * submission order, and complete as usual. */
 int ebiv_batch_flush(ebiv_batch *b, int session);
 
-/* Queued packets of one session (or of all when session is 0). */
+/* Packets of one session (or of all when session is 0) not yet returned:
+ * queued, taken by a step in flight, or decoded and held for the next call. */
 int ebiv_batch_pending(const ebiv_batch *b, int session);
 
-/* Runs one step; writes at most max_out completions.  When max_out is less
- * than the queued work, the sessions served first rotate from step to step.
- * Returns a runtime status (decode errors are reported per completion). */
+/* Runs one step; writes at most max_out completions (a step's completions
+ * beyond it come first from the next call).  When max_out is less than the
+ * queued work, the sessions served first rotate from step to step.  With the
+ * pipeline on (ebiv_schedule), a call also takes the next step's packets from
+ * the queues and starts their entropy decoding, which the next call
+ * finishes: packets submitted after a call join the step after the one in
+ * flight.  Returns a runtime status (decode errors are reported per
+ * completion). */
 int ebiv_batch_step(ebiv_batch *b, ebiv_completion *out, int max_out, int *n_out);
 
 /* Zero-copy presentation.  govid's Ebitengine bridge uploads each frame's
@@ -210,10 +218,13 @@ int ebiv_batch_frame_rgba(ebiv_batch *b, const ebiv_frame *f, uint8_t *dst, int 
  * intra_waves_many: per inter-frame tile, for launches of fewer than / at
  * least many_tiles tiles (1-16).  recon_lanes: HIP streams (1-4) the
  * sessions' reconstruction launches are dealt to after the step's entropy
- * launch, so that one group's launches overlap another's.  A field <= 0 keeps
- * its value.  Results do not depend on the schedule. */
+ * launch, so that one group's launches overlap another's.  pipeline: 1
+ * (default) queues the next step's entropy launch on a second stream, where
+ * it runs beside this step's reconstruction; 2 runs one step at a time.  A
+ * field <= 0 keeps its value.  Results do not depend on the schedule. */
 typedef struct {
     int32_t intra_waves_key, intra_waves_few, intra_waves_many, many_tiles, recon_lanes;
+    int32_t pipeline;
 } ebiv_schedule;
 int ebiv_batch_set_schedule(ebiv_batch *b, const ebiv_schedule *s);
 int ebiv_batch_get_schedule(const ebiv_batch *b, ebiv_schedule *out);

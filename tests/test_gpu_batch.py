@@ -94,11 +94,14 @@ def test_sessions_of_mixed_geometry_and_windows():
     dict(intra_waves_key=16, intra_waves_few=16, intra_waves_many=16, many_tiles=1000),
     dict(recon_lanes=2),
     dict(recon_lanes=4, intra_waves_few=3),
+    dict(pipeline=2),
+    dict(pipeline=2, recon_lanes=2),
 ])
 def test_launch_schedules_do_not_change_results(schedule):
     """Intra reconstruction with 1 to 16 wavefronts per tile (key frames from
     the LDS frontier, inter frames from the frame), reconstruction split over
-    2 or 4 streams, bit for bit."""
+    2 or 4 streams, and steps run one at a time instead of the next step's
+    entropy launch beside this step's reconstruction, bit for bit."""
     streams = [
         (_clip("rects", 96, 64, 8, qp=22, gop=4, tile_cols=2, tile_rows=2), 3, 4),
         (_clip("parallax", 64, 48, 7, qp=28, gop=7, altref=3), 4, 4),
@@ -237,4 +240,39 @@ def test_bench_clip_two_sessions_window_8():
                 n2 += 1
         print(f"  {n1} + {n2} frames checked", flush=True)
     assert (n1, n2) == (40, 24)
+    b.close()
+
+
+def test_closing_a_session_with_a_step_in_flight():
+    """A step() call also starts the next step's entropy decoding; a session
+    closed before the next call gets no completions from it, and the other
+    sessions' frames are unaffected."""
+    from govid_amd.codec import Batch
+
+    pa = list(_clip("synthpan", 64, 48, 9, qp=22, gop=4))
+    pb = list(_clip("rects", 96, 64, 10, qp=22, gop=5, tile_cols=2, tile_rows=2))
+    b = Batch()
+    sa, sb = b.open(ring=3, window=3), b.open(ring=3, window=3)
+    b.submit_many(sa, pa, tags=list(range(len(pa))))
+    b.submit_many(sb, pb, tags=list(range(len(pb))))
+    first = b.step()
+    assert {c.session for c in first} == {sa, sb}
+    assert b.pending(sa) == len(pa) - 3  # three in flight, three queued
+    b.close_session(sa)
+    ref = _oracle_outcomes(pb, 3)
+    seen = []
+
+    def check(done):  # at once: a frame stays valid for `ring` more frames
+        for c in done:
+            assert c.session == sb
+            st, planes = ref[c.tag]
+            assert c.status == st
+            if st == 0:
+                assert_planes_equal(c.frame.planes(), planes, f"frame {c.tag}")
+            seen.append(c.tag)
+
+    check([c for c in first if c.session == sb])
+    while b.pending():
+        check(b.step())
+    assert seen == list(range(len(pb)))
     b.close()
