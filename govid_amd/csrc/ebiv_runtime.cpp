@@ -836,15 +836,42 @@ int Engine::launch_recon(StepBuf& S) {
   const int nl = std::max(1, std::min({sched.recon_lanes, static_cast<int>(kMaxLanes), static_cast<int>(ss.size())}));
   if (nl > 1 && !ensure_lanes(nl)) return fail(EBIV_ERR_DEVICE, "stream creation failed");
   auto lane_of = [&](size_t si) { return static_cast<int>(si % static_cast<size_t>(nl)); };
-  // rounds: the reconstructed jobs of each session in order
+  // Rounds by dependency depth: a payload's round is one after the latest
+  // round of the payloads of this step whose output it reads (its ref and
+  // golden; an override, its key and the golden it copies), and 0 when it
+  // reads none.  Intra-coded payloads (key frames) read no reference, so
+  // every key frame of the step reconstructs in round 0, beside the others
+  // there, instead of each holding up the round of its stream position.
+  // Targets are fresh buffers, so no payload writes what another still reads.
   std::vector<std::vector<size_t>> rounds;
   {
-    std::vector<size_t> u(ss.size(), 0);
+    std::vector<int> level(S.jrefs.size(), -1);
+    std::vector<int> writer;  // pool buffer -> job writing it in this step
+    size_t cur_si = SIZE_MAX;
     for (size_t j = 0; j < S.jrefs.size(); j++) {
       const JobRef& r = S.jrefs[j];
       if (!ss[r.si] || (exact && !reconstructs(S, r))) continue;
-      if (rounds.size() <= u[r.si]) rounds.emplace_back();
-      rounds[u[r.si]++].push_back(j);
+      if (r.si != cur_si) {  // (a session's jobs are contiguous, in stream order)
+        cur_si = r.si;
+        writer.assign(ss[r.si]->pool.size(), -1);
+      }
+      const StepFrame& f = frames[r.si][r.fi];
+      int lv = 0;
+      auto dep = [&](int buf) {
+        if (buf >= 0 && writer[static_cast<size_t>(buf)] >= 0)
+          lv = std::max(lv, level[static_cast<size_t>(writer[static_cast<size_t>(buf)])] + 1);
+      };
+      if (r.ov) {
+        dep(f.target);
+        dep(f.gold_in);
+      } else if (S.payload(r)->inter) {
+        dep(f.ref_in);
+        dep(f.gold_in);
+      }
+      level[j] = lv;
+      writer[static_cast<size_t>(r.ov ? f.gtarget : f.target)] = static_cast<int>(j);
+      if (rounds.size() <= static_cast<size_t>(lv)) rounds.resize(static_cast<size_t>(lv) + 1);
+      rounds[static_cast<size_t>(lv)].push_back(j);
     }
   }
   auto inter_groups = [&](size_t j) -> uint32_t {
