@@ -553,6 +553,7 @@ def main():
     ap.add_argument("--serve-window", type=int, default=1,
                     help="frames per session per engine step when serving (1 keeps frame latency lowest)")
     ap.add_argument("--no-pipeline", action="store_true", help="batch steps one at a time (A/B of the pipeline)")
+    ap.add_argument("--schedule", default="", help="engine schedule fields, e.g. intra_waves_key=4,recon_lanes=2")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 2 GiB device-to-device copy (scale for HBM byte counters)")
     args = ap.parse_args()
@@ -578,6 +579,9 @@ def main():
 
     if args.no_pipeline:
         SCHEDULE["pipeline"] = 2
+    for kv in filter(None, args.schedule.split(",")):
+        k, v = kv.split("=")
+        SCHEDULE[k.strip()] = int(v)
     serve_budget = int(args.serve_budget_gb * (1 << 30))
     if args.serve_only:
         sv = run_serving(args.serve_seconds, args.serve_rate, serve_budget, args.serve_window, check_every=17)

@@ -314,7 +314,8 @@ struct Session : HostCodec {
 
 // Launch shapes of the reconstruction kernels (see ebiv_batch_set_schedule).
 struct Schedule {
-  int intra_waves_key = 16;   // wavefronts per key-frame tile
+  int intra_waves_key = 16;   // wavefronts per key-frame tile, launches of few tiles
+  int intra_waves_key_many = 4;  // ... launches of at least many_tiles tiles
   int intra_waves_few = 16;   // wavefronts per inter-frame tile, launches of few tiles
   int intra_waves_many = 4;   // ... launches of at least many_tiles tiles
   int many_tiles = 256;
@@ -322,6 +323,11 @@ struct Schedule {
   bool pipeline = true;       // next step's entropy launch beside this step's reconstruction
   int inter_intra_waves(size_t ntiles) const {
     return ntiles >= static_cast<size_t>(many_tiles) ? intra_waves_many : intra_waves_few;
+  }
+  // (a launch of many key-frame tiles keeps more of them resident at once;
+  // a few tiles, as in the plain codec, finish sooner with more wavefronts)
+  int key_intra_waves(size_t ntiles) const {
+    return ntiles >= static_cast<size_t>(many_tiles) ? intra_waves_key_many : intra_waves_key;
   }
 };
 
@@ -961,8 +967,9 @@ int Engine::launch_recon(StepBuf& S) {
       }
       if (key_end[q] > intra_off[q]) {
         const int m = mark_begin(2, 0, st);
-        launch_recon_intra(st, static_cast<int>(key_end[q] - intra_off[q]), S.d_jobs.d, S.d_items.d + intra_off[q],
-                           sched.intra_waves_key, true);
+        const size_t n = key_end[q] - intra_off[q];
+        launch_recon_intra(st, static_cast<int>(n), S.d_jobs.d, S.d_items.d + intra_off[q], sched.key_intra_waves(n),
+                           true);
         mark_end(m, st);
       }
       if (round_end[q] > key_end[q]) {
@@ -1563,6 +1570,7 @@ int ebiv_batch_set_schedule(ebiv_batch* b, const ebiv_schedule* sc) {
   Schedule& d = b->eng.sched;
   auto waves = [](int32_t v, int& dst) { if (v > 0) dst = v > 16 ? 16 : v; };
   waves(sc->intra_waves_key, d.intra_waves_key);
+  waves(sc->intra_waves_key_many, d.intra_waves_key_many);
   waves(sc->intra_waves_few, d.intra_waves_few);
   waves(sc->intra_waves_many, d.intra_waves_many);
   if (sc->many_tiles > 0) d.many_tiles = sc->many_tiles;
@@ -1578,7 +1586,7 @@ int ebiv_batch_get_schedule(const ebiv_batch* b, ebiv_schedule* out) {
   if (!b || !out) return EBIV_ERR_ARGUMENT;
   const Schedule& d = b->eng.sched;
   *out = ebiv_schedule{d.intra_waves_key, d.intra_waves_few, d.intra_waves_many, d.many_tiles, d.recon_lanes,
-                       d.pipeline ? 1 : 2};
+                       d.pipeline ? 1 : 2, d.intra_waves_key_many};
   return EBIV_OK;
 }
 

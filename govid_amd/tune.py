@@ -6,7 +6,9 @@ hold every macroblock of the tile as intra work, inter-frame tiles only a
 few, and a launch of many tiles fills the chip with small workgroups better
 than with wide ones.  The best shapes depend on the caller's mix (tile size,
 key-frame interval, sessions per step), so this module measures candidate
-schedules on the caller's own stream and reports the fastest.  Every
+schedules on the caller's own stream and reports the fastest by wall time
+per frame (the engine overlaps launches on several streams, so summed kernel
+time is not the measure).  Every
 candidate's decoded frames are compared with the first candidate's (frame
 CRCs per packet index): schedules must not change results.
 
@@ -14,7 +16,7 @@ The window (frames per session per engine step) is searched first when
 several are given: a wider one puts more independent tile chains into each
 entropy launch and spreads each step's fixed costs over more frames, at one
 step's latency per frame and (ring + window) frame buffers per session.  It
-is picked by wall time per frame, the schedule then by kernel time.
+is picked by wall time per frame, and so is the schedule.
 
     python -m govid_amd.tune STREAM.ebiv [--sessions 64] [--window 4 | --windows 8,16,32] [--passes 2]
 
@@ -34,10 +36,12 @@ from .container import Reader
 
 # Candidate values; the engine accepts 1-16 wavefronts per tile.
 KEY_WAVES = (8, 16)
+KEY_MANY_WAVES = (2, 4, 8)
 FEW_WAVES = (4, 8, 16)
 MANY_WAVES = (2, 4, 8)
 MANY_TILES = (128, 256, 512)
 RECON_LANES = (1, 2, 4)
+PIPELINE = (1, 2)  # next step's entropy launch beside this step's reconstruction, or not
 
 
 def _crc(planes) -> int:
@@ -110,16 +114,18 @@ def tune(source, sessions: int = 64, window: int = 4, passes: int = 2, exhaustiv
             window_results.append(dict(window=w, **r))
         window = min(window_results, key=lambda r: r["wall_ms_per_frame"])["window"]
     if exhaustive:
-        cands = [dict(intra_waves_key=a, intra_waves_few=b, intra_waves_many=c, many_tiles=d, recon_lanes=e)
-                 for a, b, c, d, e in itertools.product(KEY_WAVES, FEW_WAVES, MANY_WAVES, MANY_TILES, RECON_LANES)]
+        cands = [dict(intra_waves_key=a, intra_waves_key_many=k, intra_waves_few=b, intra_waves_many=c, many_tiles=d,
+                      recon_lanes=e, pipeline=p)
+                 for a, k, b, c, d, e, p in itertools.product(KEY_WAVES, KEY_MANY_WAVES, FEW_WAVES, MANY_WAVES,
+                                                              MANY_TILES, RECON_LANES, PIPELINE)]
     else:  # one field at a time around the engine's defaults
         base = Batch()
         d0 = base.schedule()
         base.close()
         cands = [dict(d0)]
-        for field, vals in (("intra_waves_key", KEY_WAVES), ("intra_waves_few", FEW_WAVES),
-                            ("intra_waves_many", MANY_WAVES), ("many_tiles", MANY_TILES),
-                            ("recon_lanes", RECON_LANES)):
+        for field, vals in (("intra_waves_key", KEY_WAVES), ("intra_waves_key_many", KEY_MANY_WAVES),
+                            ("intra_waves_few", FEW_WAVES), ("intra_waves_many", MANY_WAVES),
+                            ("many_tiles", MANY_TILES), ("recon_lanes", RECON_LANES), ("pipeline", PIPELINE)):
             for v in vals:
                 c = dict(d0, **{field: v})
                 if c not in cands:
@@ -134,7 +140,7 @@ def tune(source, sessions: int = 64, window: int = 4, passes: int = 2, exhaustiv
         results.append(dict(schedule=c, **r))
     for d in dev:
         d.close()
-    best = min(results, key=lambda r: r["kernel_ms_per_frame"])
+    best = min(results, key=lambda r: r["wall_ms_per_frame"])
     name = source if isinstance(source, str) else f"{len(packets)} packets"
     return {"stream": name, "sessions": sessions, "window": window, "windows_tried": window_results,
             "tried": results, "best": best["schedule"]}

@@ -214,9 +214,9 @@ int ebiv_batch_frame_rgba(ebiv_batch *b, const ebiv_frame *f, uint8_t *dst, int 
 /* Launch shapes of the batch engine.  No counterpart in the reference (its
  * tile pool, ebiv/pool.go:35-72, has no launch shape); for callers that tune
  * the engine to their mix (govid_amd/tune.py).
- * intra_waves_key: wavefronts per key-frame tile; intra_waves_few and
- * intra_waves_many: per inter-frame tile, for launches of fewer than / at
- * least many_tiles tiles (1-16).  recon_lanes: HIP streams (1-4) the
+ * intra_waves_key and intra_waves_key_many: wavefronts per key-frame tile,
+ * intra_waves_few and intra_waves_many: per inter-frame tile, each pair for
+ * launches of fewer than / at least many_tiles tiles (1-16).  recon_lanes: HIP streams (1-4) the
  * sessions' reconstruction launches are dealt to after the step's entropy
  * launch, so that one group's launches overlap another's.  pipeline: 1
  * (default) queues the next step's entropy launch on a second stream, where
@@ -225,6 +225,7 @@ int ebiv_batch_frame_rgba(ebiv_batch *b, const ebiv_frame *f, uint8_t *dst, int 
 typedef struct {
     int32_t intra_waves_key, intra_waves_few, intra_waves_many, many_tiles, recon_lanes;
     int32_t pipeline;
+    int32_t intra_waves_key_many;
 } ebiv_schedule;
 int ebiv_batch_set_schedule(ebiv_batch *b, const ebiv_schedule *s);
 int ebiv_batch_get_schedule(const ebiv_batch *b, ebiv_schedule *out);

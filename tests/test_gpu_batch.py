@@ -90,7 +90,7 @@ def test_sessions_of_mixed_geometry_and_windows():
 
 @pytest.mark.parametrize("schedule", [
     dict(intra_waves_key=1, intra_waves_few=1, intra_waves_many=1),
-    dict(intra_waves_key=5, intra_waves_few=3, intra_waves_many=7, many_tiles=1),
+    dict(intra_waves_key=5, intra_waves_few=3, intra_waves_many=7, intra_waves_key_many=2, many_tiles=1),
     dict(intra_waves_key=16, intra_waves_few=16, intra_waves_many=16, many_tiles=1000),
     dict(recon_lanes=2),
     dict(recon_lanes=4, intra_waves_few=3),
