@@ -45,7 +45,7 @@ class IndexEntry(C.Structure):
 class Schedule(C.Structure):
     _fields_ = [("intra_waves_key", C.c_int32), ("intra_waves_few", C.c_int32),
                 ("intra_waves_many", C.c_int32), ("many_tiles", C.c_int32), ("recon_lanes", C.c_int32),
-                ("pipeline", C.c_int32), ("intra_waves_key_many", C.c_int32)]
+                ("pipeline", C.c_int32), ("intra_waves_key_many", C.c_int32), ("intra_cells", C.c_int32)]
 
 
 class Frame(C.Structure):
@@ -68,7 +68,8 @@ class KernelTimes(C.Structure):
 
 
 class Stats(C.Structure):
-    _fields_ = [("frames", C.c_uint64), ("symbols", C.c_uint64), ("tiles", C.c_uint64)]
+    _fields_ = [("frames", C.c_uint64), ("symbols", C.c_uint64), ("tiles", C.c_uint64),
+                ("tiles_generic", C.c_uint64)]
 
 
 def bind(L):

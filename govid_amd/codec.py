@@ -197,7 +197,7 @@ class Codec:
     def stats(self) -> dict:
         s = capi.Stats()
         self._L.ebiv_codec_stats(self._h, C.byref(s))
-        return {"frames": s.frames, "symbols": s.symbols, "tiles": s.tiles}
+        return {f: getattr(s, f) for f, _ in capi.Stats._fields_}
 
     def _rgba(self, raw, dst, on_device):
         return self._L.ebiv_frame_rgba(self._h, C.byref(raw), dst, on_device)
@@ -365,7 +365,7 @@ class Batch:
     def stats(self, session: int = 0) -> dict:
         s = capi.Stats()
         self._L.ebiv_batch_stats(self._h, session, C.byref(s))
-        return {"frames": s.frames, "symbols": s.symbols, "tiles": s.tiles}
+        return {f: getattr(s, f) for f, _ in capi.Stats._fields_}
 
     def set_timing(self, enabled: bool) -> None:
         self._L.ebiv_batch_set_timing(self._h, int(enabled))

@@ -31,6 +31,9 @@ static_assert(sizeof(TileDesc) == 24, "TileDesc layout");
 // The bypass context is fixed at {2048, 2048} (rans.go:312-322) and has no entry.
 constexpr int kNumTableVecs = 15;
 constexpr uint32_t kNoSym = 0xFFFFFFFFu;
+// Where the sign context's table starts in the table vectors (vector 0, the
+// first lane group): a frame without one takes the generic grammar decoder.
+constexpr int kSignVec = 0, kSignLane = 0;
 // Vector and first lane of a context's entries; false for the bypass context.
 constexpr bool table_place(int ctx, int* vec, int* lane0) {
   // contexts 0-11 in enum order: tx, luma mode, chroma mode, MB mode, ref,

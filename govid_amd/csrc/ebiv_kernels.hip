@@ -338,6 +338,7 @@ struct Tables {
 constexpr uint32_t kGSign = 0, kGMBMode = 16, kGTx = 32, kGCBP = 48;       // misc0
 constexpr uint32_t kGLumaMode = 0, kGChromaMode = 16, kGRef = 32, kGPart = 48;  // misc1
 constexpr uint32_t kGEsc = 0, kGMVX = 16, kGMVY = 32;                       // misc2
+static_assert(kSignVec == 0 && kGSign == kSignLane, "sign table position");
 
 // Coefficient list output: lane k stages entry base+k until 64 are queued.
 struct CoefOut {
@@ -2474,7 +2475,7 @@ __device__ __forceinline__ void parse_item(const FrameJob* __restrict__ jobs, co
   else if (prio == 1)
     __builtin_amdgcn_s_setprio(1);
   const int t = static_cast<int>(it.idx & 0x3FFFFFFFu);
-  if (hdr->vt[0][kGSign] != kNoSym)
+  if (hdr->vt[kSignVec][kSignLane] != kNoSym)
     parse_tile_asm(job.pkt, job.desc, job.recs, job.coefs, job.status, t);
   else
     parse_tile(job.pkt, job.desc, job.recs, job.coefs, job.status, t);
@@ -3073,7 +3074,7 @@ struct Frontier {
 };
 
 // res: the macroblock's residual as the inter launch formed it.
-__device__ void intra_mb(IntraScratch& S, const int16_t* __restrict__ res, const MBRec& r, const TileDesc& td,
+__device__ __forceinline__ void intra_mb(IntraScratch& S, const int16_t* __restrict__ res, const MBRec& r, const TileDesc& td,
                          Planes rec, int mbx, int mby, Frontier* fr, int lane) {
   const int n = 4 << (r.tx > 2 ? 2 : r.tx);
   WaveScratch& ws = S.ws;
@@ -3217,6 +3218,402 @@ __device__ void intra_mb(IntraScratch& S, const int16_t* __restrict__ res, const
   }
 }
 
+// ---------------------------------------------------------------------------
+// Intra reconstruction by block levels: key-frame tiles and tiles dense with
+// intra macroblocks.
+//
+// Go reconstructs a tile's macroblocks in raster order and each macroblock's
+// blocks in raster order (decode_intra.go:121-187); a block reads the
+// reconstructed samples to its left, above, above-right (when
+// aboveRightAvailable, predict.go:50-58) and above-left.  Give macroblock
+// (x, y) of the tile the rank t = x + 2y and each of its units (a luma
+// block, or the chroma pair) the level 6t + local, with local
+//   4x4 block (bx, by): bx + 2by   8x8 block: 0, 2, 4, 5 in raster order
+//   16x16: 4                       chroma (Cb and Cr, one mode): 0.
+// Every sample a unit reads belongs to a unit of a lower level: within a
+// macroblock the raster dependencies climb the local order; the left and
+// above-right macroblocks have rank t - 1 (levels up to 6t + 3, below every
+// local of the right half of a macroblock's first unit row that reads
+// them), the above macroblock t - 2 and the above-left t - 3; and no unit
+// reads a sample of the next macroblock, whose levels start at 6t + 6.  So
+// the units of one level are independent and run in one parallel round,
+// and a 30 x 23-macroblock tile needs at most 454 rounds where the
+// macroblock wavefront needs 75 steps of up to 16 serial blocks.
+//
+// Wavefront w of W owns the tile's macroblock rows w, w + W, w + 2W, ...
+// and sweeps them: in levels 6t .. 6t + 5 its row y with 2y <= t <= 2y + tw
+// works on macroblock x = t - 2y at locals 0-5 and on its left neighbour at
+// locals 6-9 (the last 4x4 blocks).  It has at most one such row when
+// th <= W or tw < 2W.  Its lanes have fixed roles: 0-31 the chroma pair,
+// 32-47 the macroblock's two 4x4 blocks (4 lanes each) or its 8x8 block,
+// 48-55 the left neighbour's two 4x4 blocks, all 64 a 16x16 block: one lane
+// per 4-sample row segment.  A unit's lanes first read its reference array
+// (predict.go:63-90) into the wavefront's LDS, then predict from it and
+// write the frame and the frontier.
+//
+// The context lives in LDS as a frontier: per luma column the bottom row of
+// the latest unit reconstructed there, per luma row the right column of the
+// latest one, per 4x4 cell its bottom-right sample (the corner of the cells
+// below-right of it), and the same per macroblock for chroma.  A unit's
+// above row is the bottom row of the unit directly above it (no unit below
+// it in those columns is reconstructed yet: each depends on it or on its
+// right neighbour), its left column likewise; units of one level write
+// frontier entries no other unit of that level reads.  Samples of inter
+// macroblocks, final before this launch, are read from the frame.
+//
+// A macroblock's residual (768 bytes) is loaded during the rank before its
+// own into a 3-slot LDS ring of the wavefront (slot: its rank mod 3) and
+// parked there before that rank's last frame stores: the compiler waits for
+// every outstanding vector memory operation, stores included, when it
+// waits for a load while stores are pending, so the wait falls on stores
+// at least one level old.
+constexpr int kCellMaxMbs = 1024;   // tile macroblocks the cell path takes
+constexpr int kCellMaxDim = 64;     // tile width and height in macroblocks
+constexpr int kCellStride = 6;      // levels per macroblock rank
+constexpr int kCellRankWords = (3 * kCellMaxDim) / 32 + 1;
+constexpr int kCellRefPerLane = 5;  // reference bytes per lane of a unit (4x4: 17 over 4 lanes)
+
+struct CellFront {
+  uint8_t A[kCellMaxDim * 16];      // luma: bottom row of the latest unit per column
+  uint8_t Lc[kCellMaxDim * 16];     // luma: right column of the latest unit per row
+  uint8_t BR[kCellMaxMbs * 16];     // luma: bottom-right sample of each 4x4 cell
+  uint8_t Ac[2][kCellMaxDim * 8];   // chroma: the same per plane
+  uint8_t Lcc[2][kCellMaxDim * 8];
+  uint8_t BRc[2][kCellMaxMbs];      // chroma: bottom-right sample of each macroblock
+  uint2 lmode[kCellMaxMbs];         // luma modes (MBRec::lmode)
+  uint32_t info[kCellMaxMbs];       // wmask | transform index << 18 | residual << 20 | chroma mode << 24
+  uint32_t intra[kCellMaxMbs / 32]; // intra macroblocks (tile raster index)
+  uint32_t ranks[kCellRankWords];   // ranks holding an intra macroblock
+};
+struct CellWave {
+  uint32_t res[3][kResPerMB / 2];    // residual ring, the macroblock of rank t in slot t % 3
+  uint8_t ref[64 * kCellRefPerLane];  // reference arrays of this level's units
+};
+__host__ __device__ constexpr size_t cell_lds_bytes(int waves) {
+  return sizeof(CellFront) + static_cast<size_t>(waves) * sizeof(CellWave);
+}
+// Tiles the cell path takes with `waves` wavefronts.
+__host__ __device__ constexpr bool cell_tile_fits(int tw, int th, int waves) {
+  return tw <= kCellMaxDim && th <= kCellMaxDim && tw * th <= kCellMaxMbs && (th <= waves || tw < 2 * waves);
+}
+
+// Luma unit `slot` (0 or 1) of local k in a macroblock with transform index
+// tx, or -1: returns type << 4 | by << 2 | bx.
+__device__ __forceinline__ int cell_luma(int tx, int k, int slot) {
+  if (tx == 0) {  // 4x4: bx + 2by = k
+    const int by = (k >> 1) - slot, bx = (k & 1) + 2 * slot;
+    return (by >= 0 && by <= 3) ? (by << 2 | bx) : -1;
+  }
+  if (slot == 1) return -1;
+  if (tx == 1) {  // 8x8
+    switch (k) {
+      case 0: return 1 << 4;
+      case 2: return 1 << 4 | 1;
+      case 4: return 1 << 4 | 1 << 2;
+      case 5: return 1 << 4 | 1 << 2 | 1;
+      default: return -1;
+    }
+  }
+  return k == 4 ? 2 << 4 : -1;  // 16x16
+}
+// Synchronisation of the block levels orders LDS only: no sample written to
+// the frame in this launch is read back in it (inter samples are final
+// before it, intra ones travel through the frontier), so the frame stores
+// and the residual loads stay in flight across levels where a fence or
+// __syncthreads() would wait for them.
+__device__ __forceinline__ void cell_wave_sync() {
+  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  __builtin_amdgcn_wave_barrier();
+}
+__device__ __forceinline__ void cell_block_sync() {
+  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+// kAngModes in arithmetic form (a lookup by a per-lane mode is a memory
+// load): D45..D113 repeat angle 32, 13, -13 (the negative one with inverse
+// angle -630), the first three vertical.
+__host__ __device__ constexpr int ang_angle(int m) {
+  return m < kModeD45 ? 0 : ((m - kModeD45) % 3 == 0 ? 32 : ((m - kModeD45) % 3 == 1 ? 13 : -13));
+}
+__host__ __device__ constexpr int ang_inv(int m) { return m >= kModeD45 && (m - kModeD45) % 3 == 2 ? -630 : 0; }
+__host__ __device__ constexpr bool ang_vert(int m) { return m >= kModeD45 && m < kModeD45 + 3; }
+constexpr bool ang_form_matches(int m) {
+  return m == kNumIntraModes || (ang_angle(m) == kAngModes[m].angle && ang_inv(m) == kAngModes[m].inv_angle &&
+                                 ang_vert(m) == kAngModes[m].vertical && ang_form_matches(m + 1));
+}
+static_assert(ang_form_matches(kModeD45), "angular parameters");
+__device__ void intra_cells(uint8_t* __restrict__ lds, const MBRec* __restrict__ recs, const int16_t* __restrict__ res,
+                            const TileDesc& td, const int mbc, const Planes rec, const uint32_t mbs_done,
+                            const int lane, const int wave, const int nwaves) {
+  CellFront& F = *reinterpret_cast<CellFront*>(lds);
+  CellWave& CW = reinterpret_cast<CellWave*>(lds + sizeof(CellFront))[wave];
+  const int tw = td.x1 - td.x0, th = td.y1 - td.y0;
+  const int tpx = tw * 16;  // tile width in luma samples
+  const int tid = static_cast<int>(threadIdx.x), nthr = static_cast<int>(blockDim.x);
+  for (int i = tid; i < kCellMaxMbs / 32; i += nthr) F.intra[i] = 0;
+  for (int i = tid; i < kCellRankWords; i += nthr) F.ranks[i] = 0;
+  __syncthreads();
+  const int nmb = tw * th;
+  for (int i = tid; i < nmb; i += nthr) {
+    const int ry = i / tw, rx = i - ry * tw;
+    const MBRec& r = recs[(td.y0 + ry) * mbc + td.x0 + rx];
+    const uint4 w = *reinterpret_cast<const uint4*>(&r);  // kind..tx, refsel..pad1, wmask, coef_off
+    const uint32_t ncoef = reinterpret_cast<const uint32_t*>(&r)[4];
+    F.lmode[i] = make_uint2(r.lmode[0], r.lmode[1]);
+    F.info[i] = (w.z & 0x3FFFFu) | min((w.x >> 24) & 0xFFu, 2u) << 18 | (ncoef > 0 ? 1u : 0u) << 20 |
+                ((w.y >> 8) & 0xFFu) << 24;
+    if (static_cast<uint32_t>(i) < mbs_done && (w.x & 0xFFu) == 2) {
+      atomicOr(&F.intra[i >> 5], 1u << (i & 31));
+      const int t = rx + 2 * ry;
+      atomicOr(&F.ranks[t >> 5], 1u << (t & 31));
+    }
+  }
+  __syncthreads();
+  auto is_intra = [&](int li) { return (F.intra[li >> 5] >> (li & 31)) & 1u; };
+  auto rank_has = [&](int t) { return t >= 0 && ((F.ranks[t >> 5] >> (t & 31)) & 1u); };
+  const int gxl = td.x0 * 16, gyl = td.y0 * 16, gxc = td.x0 * 8, gyc = td.y0 * 8;
+  // This wavefront's row at rank t (-1: none): the last of its rows (those
+  // below th) with 2y <= t, if t - 2y <= tw.
+  const int y_last = wave < th ? wave + nwaves * ((th - 1 - wave) / nwaves) : -1;
+  auto row_at = [&](int t) -> int {
+    if (y_last < 0 || t < 2 * wave) return -1;
+    const int y = min(wave + nwaves * ((t / 2 - wave) / nwaves), y_last);
+    return t - 2 * y <= tw ? y : -1;
+  };
+  // Its macroblock of rank t (tile raster index) if that has a residual, else -1.
+  auto res_mb = [&](int t) -> int {
+    const int y = row_at(t);
+    if (y < 0 || t - 2 * y >= tw) return -1;
+    const int li = y * tw + t - 2 * y;
+    return (is_intra(li) && ((F.info[li] >> 20) & 1u)) ? li : -1;
+  };
+  auto res_src = [&](int li) {
+    const int ry = li / tw, rx = li - ry * tw;
+    return reinterpret_cast<const uint3*>(res + static_cast<size_t>((td.y0 + ry) * mbc + td.x0 + rx) * kResPerMB) + lane;
+  };
+  {
+    const int li = res_mb(0);
+    if (li >= 0) reinterpret_cast<uint3*>(CW.res[0])[lane] = *res_src(li);
+  }
+  const int tmax = (tw - 1) + 2 * (th - 1);
+  for (int t1 = 0; t1 <= tmax + 1; t1++) {
+    // the next rank's residual, loaded now and parked before this rank's last stores
+    const int li_next = res_mb(t1 + 1);
+    uint32_t* const park = CW.res[(t1 + 1) % 3];
+    if (!rank_has(t1) && !rank_has(t1 - 1)) {  // (uniform) no unit in these 6 levels
+      if (li_next >= 0) reinterpret_cast<uint3*>(park)[lane] = *res_src(li_next);
+      continue;
+    }
+    const uint3 nx = li_next >= 0 ? *res_src(li_next) : make_uint3(0u, 0u, 0u);
+    // this wavefront's macroblocks: x (rank t1) and its left neighbour (t1 - 1)
+    const int y = row_at(t1);
+    const int x = y >= 0 ? t1 - 2 * y : tw;
+    const int lic = y >= 0 ? y * tw + min(x, tw - 1) : 0;
+    const int lip = y >= 0 ? y * tw + max(x - 1, 0) : 0;
+    const bool has_cur = y >= 0 && x < tw && is_intra(lic);
+    const bool has_prev = y >= 0 && x >= 1 && is_intra(lip);
+    const uint32_t info_c = F.info[lic], info_p = F.info[lip];
+    const uint2 lm_c = F.lmode[lic], lm_p = F.lmode[lip];
+    const int txc = static_cast<int>((info_c >> 18) & 3u), txp = static_cast<int>((info_p >> 18) & 3u);
+    for (int k1 = 0; k1 < kCellStride; k1++) {
+      // ---- this lane's unit
+      int u = -1, qi = 0, nq = 4;
+      bool prev = false;
+      if (has_cur) {
+        if (txc == 2 && k1 == 4) {
+          u = 2 << 4; qi = lane; nq = 64;
+        } else if (lane < 32) {
+          if (k1 == 0) { u = 3 << 4; qi = lane; nq = 32; }
+        } else if (lane < 48) {
+          if (txc == 1) { u = cell_luma(1, k1, 0); qi = lane - 32; nq = 16; }
+          else if (txc == 0 && lane < 40) { u = cell_luma(0, k1, (lane - 32) >> 2); qi = (lane - 32) & 3; }
+        }
+      }
+      if (has_prev && txp == 0 && k1 <= 3 && lane >= 48 && lane < 56) {
+        u = cell_luma(0, k1 + kCellStride, (lane - 48) >> 2); qi = (lane - 48) & 3; prev = true;
+      }
+      const bool active = u >= 0;
+      const int li = prev ? lip : lic;
+      const int ry = li / tw, rx = li - ry * tw;
+      const uint32_t info = prev ? info_p : info_c;
+      const uint2 lm = prev ? lm_p : lm_c;
+      const int type = active ? u >> 4 : 0, bxu = u & 3, byu = (u >> 2) & 3;
+      const bool chroma = type == 3;
+      const int n = chroma ? 8 : (4 << type);
+      const int sh = chroma ? 3 : 4;
+      const int X0 = chroma ? rx * 8 : rx * 16 + bxu * n, Y0 = chroma ? ry * 8 : ry * 16 + byu * n;
+      const bool av_above = Y0 > 0, av_left = X0 > 0;
+      bool av_ar = false;  // aboveRightAvailable, predict.go:50-58 (never for chroma)
+      if (!chroma && Y0 > 0 && X0 + 2 * n <= tpx) av_ar = Y0 == ry * 16 || X0 + 2 * n <= rx * 16 + 16;
+      // this lane's row segment: plane, row, first column
+      const int pl = chroma ? qi >> 4 : 0;
+      const int rr = chroma ? (qi & 15) >> 1 : qi / (n >> 2);
+      const int c0 = chroma ? (qi & 1) * 4 : (qi - rr * (n >> 2)) * 4;
+      const int blk = chroma ? 16 + pl : byu * (16 / n) + bxu;
+      // its residual, from the ring
+      const uint32_t* rsl = CW.res[(prev ? t1 + 2 : t1) % 3];  // (t1 - 1) % 3 for the neighbour
+      const uint2 rv = active && ((info >> 20) & 1u)
+                           ? *reinterpret_cast<const uint2*>(rsl + (chroma ? 128 + pl * 32 + rr * 4 + (c0 >> 1)
+                                                                          : (blk * n * n + rr * n + c0) >> 1))
+                           : make_uint2(0u, 0u);
+      // ---- the reference array: entry e of R = 4n + 1 (per chroma plane),
+      // with the substitutions of predict.go:134-156 in closed form
+      uint8_t* ref = CW.ref + (lane - qi) * kCellRefPerLane;
+      const int R = 4 * n + 1, Rt = chroma ? 2 * R : R;
+      // (branch-free, so every entry's LDS reads issue together; an entry not
+      // needed reads the unit's own first sample position, in bounds)
+      const bool any_av = av_left || av_above;
+      const uint8_t* fl = reinterpret_cast<const uint8_t*>(&F);
+      int32_t got[kCellRefPerLane];
+      uint32_t iw[kCellRefPerLane];
+      int lsh[kCellRefPerLane];
+      bool need[kCellRefPerLane];
+      const uint8_t* gp[kCellRefPerLane];
+#pragma unroll
+      for (int k = 0; k < kCellRefPerLane; k++) {
+        const int e = qi + k * nq;
+        need[k] = active && e < Rt && any_av;
+        const int epl = e >= R ? 1 : 0, i = e - epl * R;
+        int dx = 0, dy = 0;
+        if (need[k]) {
+          if (i < 2 * n) {
+            dx = av_left ? -1 : 0;
+            dy = av_left ? min(2 * n - 1 - i, n - 1) : -1;
+          } else if (i == 2 * n) {
+            dx = av_left ? -1 : 0;
+            dy = av_left ? (av_above ? -1 : 0) : -1;
+          } else if (!av_above) {
+            dx = -1;
+          } else {
+            const int j = i - 2 * n - 1;
+            dx = (j >= n && !av_ar) ? n - 1 : j;
+            dy = -1;
+          }
+        }
+        const int x2 = X0 + dx, y2 = Y0 + dy;
+        const int mli = (y2 >> sh) * tw + (x2 >> sh);
+        iw[k] = F.intra[mli >> 5];
+        lsh[k] = mli & 31;
+        const bool corner = dx < 0 && dy < 0;
+        size_t lo;
+        if (chroma) {
+          lo = corner ? offsetof(CellFront, BRc) + epl * kCellMaxMbs + mli
+                      : (dx < 0 ? offsetof(CellFront, Lcc) + epl * (kCellMaxDim * 8) + y2
+                                : offsetof(CellFront, Ac) + epl * (kCellMaxDim * 8) + x2);
+        } else {
+          lo = corner ? offsetof(CellFront, BR) + (y2 >> 2) * (tw * 4) + (x2 >> 2)
+                      : (dx < 0 ? offsetof(CellFront, Lc) + y2 : offsetof(CellFront, A) + x2);
+        }
+        got[k] = fl[lo];
+        gp[k] = chroma ? (epl ? rec.cr : rec.cb) + static_cast<size_t>(gyc + y2) * static_cast<size_t>(rec.cw) + (gxc + x2)
+                       : rec.y + static_cast<size_t>(gyl + y2) * static_cast<size_t>(rec.pw) + (gxl + x2);
+      }
+      // Samples of inter macroblocks from the frame: loaded for every entry
+      // when any lane of the wavefront needs one (never in a key frame), so
+      // the loads issue together.
+      bool from_frame[kCellRefPerLane];
+      bool any_frame = false;
+#pragma unroll
+      for (int k = 0; k < kCellRefPerLane; k++) {
+        from_frame[k] = need[k] && !((iw[k] >> lsh[k]) & 1u);
+        any_frame |= from_frame[k];
+      }
+      if (__ballot(any_frame) != 0) {
+        int32_t fv[kCellRefPerLane];
+#pragma unroll
+        for (int k = 0; k < kCellRefPerLane; k++) fv[k] = *gp[k];
+#pragma unroll
+        for (int k = 0; k < kCellRefPerLane; k++) got[k] = from_frame[k] ? fv[k] : got[k];
+      }
+#pragma unroll
+      for (int k = 0; k < kCellRefPerLane; k++) {
+        const int e = qi + k * nq;
+        if (active && e < Rt) ref[e] = static_cast<uint8_t>(any_av ? got[k] : kDCPred);
+      }
+      cell_wave_sync();
+      const bool writes = active && ((info >> blk) & 1u);  // reached by its tile stream
+      uint32_t word = 0;
+      if (writes) {
+        const uint8_t* rp = ref + pl * R;
+        const int mode = min(chroma ? static_cast<int>(info >> 24)
+                                    : static_cast<int>(((blk < 8 ? lm.x >> (4 * blk) : lm.y >> (4 * (blk - 8)))) & 15u),
+                             kNumIntraModes - 1);  // (a decoded block's mode is always in range)
+        // Every mode as (w0 ref[i0] + w1 ref[i1] + w2 ref[i2] + 16) >> 5,
+        // clamped (a no-op except for TrueMotion): DC from the mean, V/H
+        // w0 = 32, TM 32, 32, -32 (top + left - corner), angular the two
+        // taps of predict.go:162-203.
+        int32_t dc = 0;
+        if (mode == kModeDC) {  // the n left samples nearest the corner and the first n above
+#pragma unroll
+          for (int i = 0; i < 16; i++) {
+            const int32_t two = rp[min(n + i, 2 * n - 1)] + rp[min(2 * n + 1 + i, 3 * n)];
+            dc += i < n ? two : 0;
+          }
+          dc = (dc + n) / (2 * n);
+        }
+        const int angle = ang_angle(mode), inv = ang_inv(mode);
+        const bool vert = ang_vert(mode);
+        // main line ext[kk] of the angular modes as a reference index (kk in
+        // [0, 2n]), its projected side for kk < 0; -1 past 2n (weight 0)
+        auto ext_index = [&](int kk) -> int {
+          if (kk > 2 * n) return -1;
+          if (kk >= 0) return vert ? 2 * n + kk : 2 * n - kk;
+          const int idx = (kk * inv + 128) >> 8;
+          return vert ? 2 * n - idx : 2 * n + idx;
+        };
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          int i0 = 0, i1 = 0, i2 = 0, w0 = 32, w1 = 0, w2 = 0;
+          if (mode == kModeV) {
+            i0 = 2 * n + 1 + c0 + j;
+          } else if (mode == kModeH) {
+            i0 = 2 * n - 1 - rr;
+          } else if (mode == kModeTM) {
+            i0 = 2 * n + 1 + c0 + j;
+            i1 = 2 * n - 1 - rr;
+            i2 = 2 * n;
+            w1 = 32;
+            w2 = -32;
+          } else if (mode != kModeDC) {
+            const int a2 = vert ? rr : c0 + j, b2 = vert ? c0 + j : rr;
+            const int pos = (a2 + 1) * angle, idx = pos >> 5, f = pos & 31;
+            i0 = ext_index(b2 + idx + 1);
+            i1 = ext_index(b2 + idx + 2);
+            w0 = i0 < 0 ? 0 : 32 - f;
+            w1 = i1 < 0 ? 0 : f;
+            i0 = max(i0, 0);
+            i1 = max(i1, 0);
+          }
+          const int32_t acc = mode == kModeDC ? 32 * dc + 16 : w0 * rp[i0] + w1 * rp[i1] + w2 * rp[i2] + 16;
+          const uint32_t rw16 = (j < 2 ? rv.x : rv.y) >> (16 * (j & 1));
+          const int32_t val = clamp255(clamp255(acc >> 5) + static_cast<int16_t>(rw16 & 0xFFFFu));
+          word |= static_cast<uint32_t>(val) << (8 * j);
+        }
+      }
+      if (k1 == kCellStride - 1) reinterpret_cast<uint3*>(park)[lane] = nx;  // before this rank's last stores
+      if (writes) {
+        const uint8_t last = static_cast<uint8_t>(word >> 24);
+        if (chroma) {
+          uint8_t* dp = pl ? rec.cr : rec.cb;
+          *reinterpret_cast<uint32_t*>(dp + static_cast<size_t>(gyc + Y0 + rr) * static_cast<size_t>(rec.cw) +
+                                       static_cast<size_t>(gxc + X0 + c0)) = word;
+          if (rr == 7) *reinterpret_cast<uint32_t*>(&F.Ac[pl][X0 + c0]) = word;
+          if (c0 == 4) F.Lcc[pl][Y0 + rr] = last;
+          if (rr == 7 && c0 == 4) F.BRc[pl][li] = last;
+        } else {
+          *reinterpret_cast<uint32_t*>(rec.y + static_cast<size_t>(gyl + Y0 + rr) * static_cast<size_t>(rec.pw) +
+                                       static_cast<size_t>(gxl + X0 + c0)) = word;
+          if (rr == n - 1) *reinterpret_cast<uint32_t*>(&F.A[X0 + c0]) = word;
+          if (c0 + 4 == n) F.Lc[Y0 + rr] = last;
+          if (((Y0 + rr) & 3) == 3) F.BR[((Y0 + rr) >> 2) * (tw * 4) + ((X0 + c0) >> 2)] = last;
+        }
+      }
+      cell_block_sync();  // the next level reads this one's frontier entries (and the ref space is reused)
+    }
+  }
+}
+
 // One workgroup per tile.  Macroblock (x, y) depends on (x-1, y), (x-1, y-1),
 // (x, y-1) and (x+1, y-1), so step s = x + 2y (tile-relative) may run all its
 // macroblocks at once.  Only intra macroblocks are reconstructed here (the
@@ -3235,7 +3632,7 @@ constexpr int kIntraBitWords = kFrontMax * kFrontMax / 32 + 1;
 __device__ __forceinline__ void recon_intra_tile(const uint8_t* __restrict__ desc, const MBRec* __restrict__ recs,
                                                  const int16_t* __restrict__ res,
                                                  const TileStatus* __restrict__ status, const Planes rec, const int t,
-                                                 const bool with_front) {
+                                                 const bool with_front, const bool with_cells) {
   extern __shared__ __attribute__((aligned(16))) uint8_t intra_dyn[];
   __shared__ uint32_t step_mask[kMaxStepWords];
   __shared__ uint32_t s_bits[kIntraBitWords];  // intra macroblocks of the tile (local index)
@@ -3252,6 +3649,12 @@ __device__ __forceinline__ void recon_intra_tile(const uint8_t* __restrict__ des
   const int tw = td.x1 - td.x0, th = td.y1 - td.y0;
   const int steps = (tw - 1) + 2 * (th - 1) + 1;
   const int mbc = hdr->mbc;
+  // Key-frame tiles and tiles dense with intra macroblocks: by block levels.
+  if (with_cells && cell_tile_fits(tw, th, nwaves) &&
+      (hdr->inter == 0 || st.n_intra > static_cast<uint32_t>(kMaxIntraList))) {
+    intra_cells(intra_dyn, recs, res, td, mbc, rec, st.mbs_done, lane, wave, nwaves);
+    return;
+  }
   // A key frame's tile (every macroblock intra) keeps its context in LDS.
   const bool keyf = with_front && hdr->inter == 0 && tw <= kFrontMax && th <= kFrontMax;
   // Otherwise, a tile with few intra macroblocks runs them by dependency
@@ -3383,11 +3786,11 @@ __device__ __forceinline__ void recon_intra_tile(const uint8_t* __restrict__ des
 
 __global__ __launch_bounds__(64 * kIntraMaxWaves) void ebiv_recon_intra(const FrameJob* __restrict__ jobs,
                                                                           const WorkItem* __restrict__ items,
-                                                                          int with_front) {
+                                                                          int flags) {
   const WorkItem it = item_of(items);
   ConstJob& job = job_of(jobs, it.job);
   recon_intra_tile(job.desc, job.recs, job.res, job.status, planes_of(job.rec), static_cast<int>(it.idx),
-                   with_front != 0);
+                   (flags & 1) != 0, (flags & 2) != 0);
 }
 
 // ============================================================================
@@ -3474,10 +3877,15 @@ void launch_recon_inter(hipStream_t s, int ngroups, const FrameJob* jobs, const 
 
 // waves: wavefronts per tile (1-16); key: the items include key-frame tiles,
 // which then use the LDS frontier.
-void launch_recon_intra(hipStream_t s, int nitems, const FrameJob* jobs, const WorkItem* items, int waves, bool key) {
+// cells: key-frame tiles and tiles dense with intra macroblocks take the
+// block-level path (its LDS overlays the other paths').
+void launch_recon_intra(hipStream_t s, int nitems, const FrameJob* jobs, const WorkItem* items, int waves, bool key,
+                        bool cells) {
   waves = waves < 1 ? 1 : (waves > kIntraMaxWaves ? kIntraMaxWaves : waves);
-  const size_t lds = static_cast<size_t>(waves) * sizeof(IntraScratch) + (key ? sizeof(Frontier) : 0);
-  hipLaunchKernelGGL(ebiv_recon_intra, dim3(nitems), dim3(64 * waves), lds, s, jobs, items, key ? 1 : 0);
+  size_t lds = static_cast<size_t>(waves) * sizeof(IntraScratch) + (key ? sizeof(Frontier) : 0);
+  if (cells && cell_lds_bytes(waves) > lds) lds = cell_lds_bytes(waves);
+  hipLaunchKernelGGL(ebiv_recon_intra, dim3(nitems), dim3(64 * waves), lds, s, jobs, items,
+                     (key ? 1 : 0) | (cells ? 2 : 0));
 }
 
 void launch_rgba(hipStream_t s, const RgbaJob& j) {

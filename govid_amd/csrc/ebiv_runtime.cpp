@@ -51,7 +51,8 @@ namespace ebiv {
 hipError_t upload_tables(const int32_t dct[3][256], const uint8_t zz[3][256]);
 void launch_parse(hipStream_t s, int nitems, int max_tile_w, const FrameJob* jobs, const WorkItem* items);
 void launch_recon_inter(hipStream_t s, int ngroups, const FrameJob* jobs, const WorkItem* items, int njobs);
-void launch_recon_intra(hipStream_t s, int nitems, const FrameJob* jobs, const WorkItem* items, int waves, bool key);
+void launch_recon_intra(hipStream_t s, int nitems, const FrameJob* jobs, const WorkItem* items, int waves, bool key,
+                        bool cells);
 void launch_rgba(hipStream_t s, const RgbaJob& j);
 void launch_rgba_frames(hipStream_t s, const RgbaJob* jobs, int njobs, uint32_t max_px8);
 
@@ -169,6 +170,7 @@ struct PayloadSlot {
   int ntiles = 0;
   int max_tile_w = 0;
   bool inter = false;
+  bool generic = false;  // no sign table: the entropy kernel runs its generic decoder
   DevArray<MBRec> recs;
   DevArray<uint32_t> coefs;
   DevArray<int16_t> res;
@@ -200,7 +202,7 @@ struct Session : HostCodec {
   int ref = -1, golden = -1;
   std::vector<uint8_t*> host_ring;
   std::vector<std::unique_ptr<PayloadSlot>> slots;
-  ebiv_stats stats{0, 0, 0};
+  ebiv_stats stats{0, 0, 0, 0};
   // presentation surfaces (ebiv_batch_set_surfaces)
   std::vector<uint8_t*> surfaces;
   size_t surface_pitch = 0;
@@ -284,6 +286,8 @@ struct Session : HostCodec {
     ps->ntiles = hp.ntiles;
     ps->max_tile_w = hp.max_tile_w;
     ps->inter = hp.inter;
+    // (the entropy kernel's own choice, parse_item)
+    ps->generic = reinterpret_cast<const FrameHdr*>(hp.desc.data())->vt[kSignVec][kSignLane] == kNoSym;
     const size_t nmb = static_cast<size_t>(mbc) * mbr;
     if (!ps->recs.reserve(nmb) || !ps->coefs.reserve(nmb * kCoefPerMB) || !ps->res.reserve(nmb * kResPerMB) ||
         !ps->h_status.reserve(static_cast<size_t>(hp.ntiles)))
@@ -295,6 +299,7 @@ struct Session : HostCodec {
     for (int i = 0; i < ps->ntiles; i++) {
       stats.symbols += ps->h_status.h[i].nsym;
       stats.tiles++;
+      if (ps->generic) stats.tiles_generic++;
     }
     for (int i = 0; i < ps->ntiles; i++) {
       const TileStatus& s = ps->h_status.h[i];
@@ -321,6 +326,7 @@ struct Schedule {
   int many_tiles = 256;
   int recon_lanes = 1;        // HIP streams the sessions' reconstruction is split over
   bool pipeline = true;       // next step's entropy launch beside this step's reconstruction
+  bool intra_cells = false;   // key-frame and intra-dense tiles by block levels (measured slower)
   int inter_intra_waves(size_t ntiles) const {
     return ntiles >= static_cast<size_t>(many_tiles) ? intra_waves_many : intra_waves_few;
   }
@@ -969,14 +975,14 @@ int Engine::launch_recon(StepBuf& S) {
         const int m = mark_begin(2, 0, st);
         const size_t n = key_end[q] - intra_off[q];
         launch_recon_intra(st, static_cast<int>(n), S.d_jobs.d, S.d_items.d + intra_off[q], sched.key_intra_waves(n),
-                           true);
+                           true, sched.intra_cells);
         mark_end(m, st);
       }
       if (round_end[q] > key_end[q]) {
         const size_t n = round_end[q] - key_end[q];
         const int m = mark_begin(2, 0, st);
         launch_recon_intra(st, static_cast<int>(n), S.d_jobs.d, S.d_items.d + key_end[q], sched.inter_intra_waves(n),
-                           false);
+                           false, sched.intra_cells);
         mark_end(m, st);
       }
     }
@@ -1541,12 +1547,13 @@ int ebiv_batch_kernel_times(const ebiv_batch* b, ebiv_kernel_times* out) {
 
 int ebiv_batch_stats(const ebiv_batch* b, int session, ebiv_stats* out) {
   if (!b || !out) return EBIV_ERR_ARGUMENT;
-  ebiv_stats t{0, 0, 0};
+  ebiv_stats t{0, 0, 0, 0};
   for (auto& s : b->eng.sessions)
     if (session == 0 || s->id == session) {
       t.frames += s->stats.frames;
       t.symbols += s->stats.symbols;
       t.tiles += s->stats.tiles;
+      t.tiles_generic += s->stats.tiles_generic;
     }
   *out = t;
   return EBIV_OK;
@@ -1579,6 +1586,8 @@ int ebiv_batch_set_schedule(ebiv_batch* b, const ebiv_schedule* sc) {
   // its entropy launch's event, on whichever stream that ran)
   if (sc->pipeline > 2) return EBIV_ERR_ARGUMENT;
   if (sc->pipeline > 0) d.pipeline = sc->pipeline == 1;
+  if (sc->intra_cells > 2) return EBIV_ERR_ARGUMENT;
+  if (sc->intra_cells > 0) d.intra_cells = sc->intra_cells == 1;
   return EBIV_OK;
 }
 
@@ -1586,7 +1595,7 @@ int ebiv_batch_get_schedule(const ebiv_batch* b, ebiv_schedule* out) {
   if (!b || !out) return EBIV_ERR_ARGUMENT;
   const Schedule& d = b->eng.sched;
   *out = ebiv_schedule{d.intra_waves_key, d.intra_waves_few, d.intra_waves_many, d.many_tiles, d.recon_lanes,
-                       d.pipeline ? 1 : 2, d.intra_waves_key_many};
+                       d.pipeline ? 1 : 2, d.intra_waves_key_many, d.intra_cells ? 1 : 2};
   return EBIV_OK;
 }
 

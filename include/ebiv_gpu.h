@@ -57,6 +57,9 @@ typedef struct {
     uint64_t frames;       /* successful decodes */
     uint64_t symbols;      /* rANS decode calls, bypass bits included */
     uint64_t tiles;        /* entropy tiles parsed */
+    uint64_t tiles_generic; /* of those, parsed by the generic grammar decoder
+                              (a frame with no sign table: the hand-scheduled
+                              token loop needs one) */
 } ebiv_stats;
 
 /* ebiv.NewCodec(WithFrameRing(ring_size)) — ebiv/codec.go:45-64.  ring_size
@@ -220,12 +223,17 @@ int ebiv_batch_frame_rgba(ebiv_batch *b, const ebiv_frame *f, uint8_t *dst, int 
  * sessions' reconstruction launches are dealt to after the step's entropy
  * launch, so that one group's launches overlap another's.  pipeline: 1
  * (default) queues the next step's entropy launch on a second stream, where
- * it runs beside this step's reconstruction; 2 runs one step at a time.  A
+ * it runs beside this step's reconstruction; 2 runs one step at a time.
+ * intra_cells: 1 reconstructs key-frame tiles and tiles dense with intra
+ * macroblocks by levels of blocks (tiles that fit: at most 64 x 64 and
+ * 1024 macroblocks, and at most as many rows as wavefronts or fewer than
+ * twice as many columns), 2 (default) by macroblock wavefronts.  A
  * field <= 0 keeps its value.  Results do not depend on the schedule. */
 typedef struct {
     int32_t intra_waves_key, intra_waves_few, intra_waves_many, many_tiles, recon_lanes;
     int32_t pipeline;
     int32_t intra_waves_key_many;
+    int32_t intra_cells;
 } ebiv_schedule;
 int ebiv_batch_set_schedule(ebiv_batch *b, const ebiv_schedule *s);
 int ebiv_batch_get_schedule(const ebiv_batch *b, ebiv_schedule *out);

@@ -96,12 +96,17 @@ def test_sessions_of_mixed_geometry_and_windows():
     dict(recon_lanes=4, intra_waves_few=3),
     dict(pipeline=2),
     dict(pipeline=2, recon_lanes=2),
+    dict(intra_cells=1),
+    dict(intra_cells=1, intra_waves_key=16, intra_waves_key_many=16, intra_waves_few=16, intra_waves_many=16),
+    dict(intra_cells=1, intra_waves_key=2, intra_waves_key_many=2, intra_waves_few=2, intra_waves_many=2,
+         many_tiles=1),
 ])
 def test_launch_schedules_do_not_change_results(schedule):
     """Intra reconstruction with 1 to 16 wavefronts per tile (key frames from
-    the LDS frontier, inter frames from the frame), reconstruction split over
-    2 or 4 streams, and steps run one at a time instead of the next step's
-    entropy launch beside this step's reconstruction, bit for bit."""
+    the LDS frontier, inter frames from the frame), by block levels or by
+    macroblock wavefronts, reconstruction split over 2 or 4 streams, and
+    steps run one at a time instead of the next step's entropy launch beside
+    this step's reconstruction, bit for bit."""
     streams = [
         (_clip("rects", 96, 64, 8, qp=22, gop=4, tile_cols=2, tile_rows=2), 3, 4),
         (_clip("parallax", 64, 48, 7, qp=28, gop=7, altref=3), 4, 4),
@@ -109,6 +114,38 @@ def test_launch_schedules_do_not_change_results(schedule):
     ]
     print(f"schedule {schedule}", flush=True)
     _run_batch(streams, schedule=schedule)
+
+
+CELLS_16 = dict(intra_cells=1, intra_waves_key=16, intra_waves_key_many=16, intra_waves_few=16, intra_waves_many=16)
+
+
+@pytest.mark.parametrize("case", [
+    dict(w=1024, h=256, tiles=(1, 1)),   # 64 x 16 macroblocks: one row per wavefront, the widest tile
+    dict(w=992, h=1024, tiles=(2, 2)),   # 31 x 32 tiles: two rows per wavefront, 992 macroblocks
+    dict(w=1024, h=1024, tiles=(2, 2)),  # 32 x 32 tiles: by macroblock wavefronts (rows would overlap)
+    dict(w=1040, h=64, tiles=(1, 1)),    # 65 macroblocks wide: by macroblock wavefronts
+    dict(w=528, h=512, tiles=(1, 1)),    # 33 x 32 = 1056 macroblocks: by macroblock wavefronts
+])
+def test_intra_block_levels_at_their_tile_limits(case):
+    """The block-level intra reconstruction (schedule intra_cells=1, 16
+    wavefronts) on key frames and inter frames in tiles at and just past the
+    sizes it takes, against the oracle."""
+    w, h, (tc, tr) = case["w"], case["h"], case["tiles"]
+    print(f"case {w}x{h} tiles {tc}x{tr}", flush=True)
+    pk = clip_kw("parallax", w, h, 4, qp=22, gop=2, tile_cols=tc, tile_rows=tr, threads=8).packets
+    _run_batch([(pk, 3, 1)], schedule=CELLS_16)
+
+
+def test_intra_block_levels_on_the_bench_clip():
+    """The 1080p clip's key and flash frames (30 x 23-macroblock tiles, two
+    rows per wavefront) by block levels, every frame against the oracle."""
+    import os
+
+    from govid_amd.container import Reader
+
+    path = os.path.join(os.path.dirname(os.path.dirname(__file__)), "bench_data", "parallax_1080p.ebiv")
+    pk = Reader.open(path).packets()[:40]
+    _run_batch([(pk, 3, 4)], schedule=CELLS_16)
 
 
 def test_tuner_checks_every_schedule_and_picks_one():

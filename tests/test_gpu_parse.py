@@ -110,3 +110,48 @@ def test_parse_bench_clip_first_frames():
         width = rd.header.width
 
     run_clip(C, label="parallax_1080p")
+
+
+def test_generic_grammar_decoder_without_a_sign_table():
+    """A frame with no sign table takes the entropy kernel's generic grammar
+    decoder (parse_tile) rather than the hand-scheduled token loop, and the
+    stats count those tiles.  The encoder leaves the table out when a frame
+    codes no coefficient (flat content: key-frame modes, then skips); a coded
+    key frame whose sign table was removed fails at its first sign symbol
+    (unused context, rans.go:275-278) after the generic decoder has read its
+    first blocks.  Records, planes and statuses match the oracle."""
+    from govid_amd import codec as K
+    from govid_amd import encoder
+    from govid_amd.codec import Codec
+    from helpers import assert_planes_equal, edit_tables
+
+    w, h = 96, 64
+    flat = [(np.full((h, w), 128, np.uint8), np.full((h // 2, w // 2), 128, np.uint8),
+             np.full((h // 2, w // 2), 128, np.uint8)) for _ in range(4)]
+    clip = encoder.encode_frames(flat, w, h, qp=22, gop=2, tile_cols=2, tile_rows=2)
+    print("flat clip", flush=True)
+    run_clip(clip, label="flat")
+    gpu, ref = Codec(ring=3), O.OracleCodec(ring=3)
+    for i, p in enumerate(clip.packets):
+        assert_planes_equal(gpu.decode(p).planes(), ref.decode(p), f"flat frame {i}")
+    st = gpu.stats()
+    assert st["tiles"] == 4 * len(clip.packets) and st["tiles_generic"] == st["tiles"], st
+
+    coded = list(clip_kw("rects", w, h, 4, qp=22, gop=2, tile_cols=2, tile_rows=2).packets)
+    CTX_SIGN = 7  # the sign context among the table block's 72
+    coded[2] = edit_tables(coded[2], lambda e: [(0, []) if c == CTX_SIGN else ev for c, ev in enumerate(e)])
+    gpu2, ref2 = Codec(ring=3), O.OracleCodec(ring=3)
+    for i, p in enumerate(coded):
+        print(f"  coded frame {i}", flush=True)
+        try:
+            want = ref2.decode(p)
+        except O.DecodeError as e:
+            with pytest.raises(K.EbivError) as got:
+                gpu2.decode(p)
+            status = {K.ErrCorrupt: 1, K.ErrUnsupported: 2, K.ErrNoKeyframe: 3, K.ErrDimensions: 4}[type(got.value)]
+            assert status == e.status, (i, got.value, e)
+            continue
+        assert_planes_equal(gpu2.decode(p).planes(), want, f"coded frame {i}")
+    st2 = gpu2.stats()
+    # (the frame after it fails on the host: its tables delta-code the removed one)
+    assert st2["tiles_generic"] == 4 and st2["tiles"] == 4 * 3, st2
