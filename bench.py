@@ -395,7 +395,7 @@ def run_sessions(plan, window, steps, warmup, present=False):
     for x in surfaces:
         x.close()
     return {"elapsed": elapsed, "frames": frames, "lat": lat, "kt": kt, "bad": bad,
-            "symbols": s1["symbols"] - s0["symbols"]}
+            "symbols": s1["symbols"] - s0["symbols"], "symbols_all": s1["symbols"]}
 
 
 def batched_summary(bt, steps, recon_b=None, workload="", traffic_wl=None, **extra):
@@ -472,6 +472,7 @@ def run_entropy(sessions, window, steps, warmup):
                           stream=f"1920x1080 noise q12 gop15 4x3 tiles, {len(pk)} frames, "
                                  f"{sum(map(len, pk)) // len(pk)} B per frame")
     out["symbols_per_frame"] = round(bt["symbols"] / max(bt["frames"], 1))
+    out["symbols_all"] = bt["symbols_all"]
     return out
 
 
@@ -546,6 +547,7 @@ def main():
     ap.add_argument("--no-entropy", action="store_true")
     ap.add_argument("--no-host", action="store_true")
     ap.add_argument("--entropy-sessions", type=int, default=32)
+    ap.add_argument("--entropy-only", action="store_true", help="run only the entropy-bound batch (for profiling)")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving mix")
     ap.add_argument("--serve-seconds", type=float, default=4.0, help="length of the Poisson arrival window")
     ap.add_argument("--serve-rate", type=float, default=200.0)
@@ -587,6 +589,10 @@ def main():
         sv = run_serving(args.serve_seconds, args.serve_rate, serve_budget, args.serve_window, check_every=17)
         print(json.dumps({"mode": "serving", **{k: (round(v, 3) if isinstance(v, float) else v) for k, v in sv.items()}}))
         return
+    if args.entropy_only:
+        print(json.dumps({"mode": "entropy", **run_entropy(args.entropy_sessions, args.window, args.steps,
+                                                            args.warmup)}))
+        return
     rd, want = load_stream()
     packets = rd.packets()
     hdr = rd.header
@@ -596,8 +602,11 @@ def main():
     dev = [DeviceBuffer(p) for p in packets]
     if args.batched_only:
         bt = run_batched(packets, dev, want, args.sessions, args.window, args.steps, args.warmup)
+        # symbols_all: every symbol the process decoded (warm-up included), the
+        # divisor of a counter run's entropy-kernel totals
         print(json.dumps({"mode": "batched", "frames_per_s": round(bt["frames"] / bt["elapsed"], 2),
-                          "frames_differing": bt["bad"], "sessions": args.sessions, "window": args.window}))
+                          "frames_differing": bt["bad"], "sessions": args.sessions, "window": args.window,
+                          "symbols_all": bt["symbols_all"]}))
         return
     bad = 0 if args.no_verify else verify_plain(packets, want)
     log(f"bench: plain verify -> {bad} frames differ from the expected CRCs")

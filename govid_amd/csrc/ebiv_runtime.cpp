@@ -1469,9 +1469,13 @@ int ebiv_batch_step(ebiv_batch* b, ebiv_completion* out, int max_out, int* n_out
     const int nx = e.cur ^ 1;
     StepBuf& N = e.steps[nx];
     take_step(e, N, max_out, nx * kSlotsPerStep);
-    if (!N.ss.empty())
+    // Only a step that launched moves the engine to the other buffer set: a
+    // lone step (every queued frame fits the window) keeps reusing one set,
+    // whose device buffers are already sized, instead of allocating both.
+    if (!N.ss.empty()) {
       if (int rc = e.launch_entropy(N, ps, false)) return rc;
-    e.cur = nx;
+      e.cur = nx;
+    }
   }
   if (int rc = e.wait_recon()) return rc;
   size_t npresent = 0, nframes = 0;
