@@ -424,12 +424,26 @@ struct StepBuf {
   std::vector<JobRef> jrefs;
   size_t nparse = 0;
   int max_tw = 1;
-  PinnedArray<uint8_t> h_stage;  // descriptors and host packets
+  // One staging blob, host and device, uploaded in as few copies as the step
+  // allows: host packets and descriptors, then the job table (hj/dj), then
+  // the work items (hi/di: entropy items, then reconstruction items).
+  PinnedArray<uint8_t> h_stage;
   DevArray<uint8_t> d_stage;
-  PinnedArray<FrameJob> h_jobs;
-  DevArray<FrameJob> d_jobs;
-  PinnedArray<WorkItem> h_items;  // entropy items, then reconstruction items
-  DevArray<WorkItem> d_items;
+  size_t job_off = 0, item_off = 0;
+  FrameJob* hj = nullptr;
+  FrameJob* dj = nullptr;
+  WorkItem* hi = nullptr;
+  WorkItem* di = nullptr;
+  // The reconstruction plan (plan_recon): rounds of jobs, and per round and
+  // lane q the item ranges of its inter launch, key-frame intra launch and
+  // inter-frame intra launch.
+  bool planned = false;
+  int lanes = 1;
+  size_t items_end = 0;
+  std::vector<std::vector<size_t>> rounds;
+  std::vector<size_t> inter_off, intra_off, key_end, round_end;
+  std::vector<uint32_t> inter_groups_of;
+  std::vector<uint64_t> inter_mbs_of;
   DevArray<TileStatus> d_status;  // every payload's tiles, contiguous: one download
   PinnedArray<TileStatus> h_status;
   hipEvent_t parsed = nullptr;  // the statuses are on the host
@@ -453,6 +467,8 @@ struct StepBuf {
     nparse = 0;
     max_tw = 1;
     live = false;
+    planned = false;
+    items_end = 0;
   }
   size_t frames_of(int session) const {
     size_t n = 0;
@@ -586,7 +602,9 @@ struct Engine {
 
   // Phases of a step (see run_plain and the batch step).
   int launch_entropy(StepBuf& S, hipStream_t ps, bool speculative);
+  int download_status(StepBuf& S, hipStream_t st);
   int finish_entropy(StepBuf& S);
+  int plan_recon(StepBuf& S);
   int launch_recon(StepBuf& S);
   void commit(StepBuf& S);
   int wait_recon();
@@ -644,14 +662,18 @@ bool reconstructs(const StepBuf& S, const JobRef& r) { return S.okm[r.si][r.fi] 
 
 }  // namespace
 
-// Stages the step's packets and descriptors, builds its job table (with the
-// reconstruction's frame buffers when `speculative`: assigned as if every
-// payload decodes, which is exact for a one-frame step), and launches the
-// entropy kernel over every tile of every payload on `ps`, then the status
-// download, then records S.parsed.
+// Stages the step's packets and descriptors, builds its job table and
+// entropy items and launches the entropy kernel over every tile of every
+// payload on `ps`.  `speculative` (a one-frame step): the reconstruction's
+// frame buffers are assigned as if every payload decodes, which is exact for
+// a lone frame, and the reconstruction is planned here too, so that one copy
+// uploads everything and the caller downloads the statuses after the
+// reconstruction; otherwise the statuses follow the entropy launch and the
+// reconstruction is planned from them.
 int Engine::launch_entropy(StepBuf& S, hipStream_t ps, bool speculative) {
   auto& ss = S.ss;
   auto& frames = S.frames;
+  // ---- the blob's layout: staged packets and descriptors, jobs, items
   size_t stage = 0;
   for (size_t i = 0; i < ss.size(); i++)
     for (auto& f : frames[i]) {
@@ -668,31 +690,6 @@ int Engine::launch_entropy(StepBuf& S, hipStream_t ps, bool speculative) {
         stage = align256(stage + p->desc_bytes);
       }
     }
-  if (stage) {
-    if (!S.h_stage.reserve(stage) || !S.d_stage.reserve(stage)) return fail(EBIV_ERR_DEVICE, "staging allocation failed");
-    for (size_t i = 0; i < ss.size(); i++)
-      for (auto& f : frames[i]) {
-        if (f.rc || !f.device_work) continue;
-        if (f.staged) std::memcpy(S.h_stage.h + f.staged_off, f.host, f.len);
-        for (int sl : {f.main_slot, f.ov_slot}) {
-          if (sl < 0) continue;
-          PayloadSlot* p = ss[i]->slot(sl);
-          std::memcpy(S.h_stage.h + p->desc_off, p->hdesc, p->desc_bytes);
-        }
-      }
-    if (hipMemcpyAsync(S.d_stage.d, S.h_stage.h, stage, hipMemcpyHostToDevice, ps) != hipSuccess)
-      return fail(EBIV_ERR_DEVICE, "staging upload failed");
-    for (size_t i = 0; i < ss.size(); i++)
-      for (auto& f : frames[i])
-        if (f.staged) f.dev = S.d_stage.d + f.staged_off;
-  }
-  if (speculative)
-    for (size_t i = 0; i < ss.size(); i++) {
-      const std::vector<char> ok(frames[i].size(), 1);
-      if (!assign_buffers(*ss[i], frames[i], ok, ok)) return fail(EBIV_ERR_DEVICE, "frame pool exhausted");
-    }
-
-  // ---- jobs: one per coded payload, a session's in stream order
   S.jrefs.clear();
   for (size_t i = 0; i < ss.size(); i++)
     for (size_t k = 0; k < frames[i].size(); k++) {
@@ -701,6 +698,7 @@ int Engine::launch_entropy(StepBuf& S, hipStream_t ps, bool speculative) {
       S.jrefs.push_back({i, k, false});
       if (f.ov_slot >= 0) S.jrefs.push_back({i, k, true});
     }
+  const size_t njobs = S.jrefs.size();
   size_t nitems = 0, nparse = 0;
   int max_tw = 1;
   uint32_t max_tile_bytes = 1;
@@ -709,22 +707,49 @@ int Engine::launch_entropy(StepBuf& S, hipStream_t ps, bool speculative) {
     PayloadSlot* p = S.payload(jr);
     p->status_off = nparse;
     nparse += static_cast<size_t>(p->ntiles);
-    nitems += 2 * static_cast<size_t>(p->ntiles) + 1;
+    nitems += 2 * static_cast<size_t>(p->ntiles) + 1;  // entropy items, then reconstruction items
     max_tw = std::max(max_tw, p->max_tile_w);
     for (int t = 0; t < p->ntiles; t++) max_tile_bytes = std::max(max_tile_bytes, tiles_of(p)[t].byte_len);
   }
   S.nparse = nparse;
   S.max_tw = max_tw;
-  const size_t njobs = S.jrefs.size();
-  if (!S.h_jobs.reserve(njobs + 1) || !S.d_jobs.reserve(njobs + 1) || !S.h_items.reserve(nitems + 1) ||
-      !S.d_items.reserve(nitems + 1) || !S.d_status.reserve(nparse + 1) || !S.h_status.reserve(nparse + 1))
-    return fail(EBIV_ERR_DEVICE, "job table allocation failed");
+  S.job_off = align256(stage);
+  S.item_off = align256(S.job_off + sizeof(FrameJob) * (njobs + 1));
+  const size_t blob = S.item_off + sizeof(WorkItem) * (nitems + 1);
+  // (reserve keeps no contents: before anything is written)
+  if (!S.h_stage.reserve(blob) || !S.d_stage.reserve(blob) || !S.d_status.reserve(nparse + 1) ||
+      !S.h_status.reserve(nparse + 1))
+    return fail(EBIV_ERR_DEVICE, "staging allocation failed");
+  S.hj = reinterpret_cast<FrameJob*>(S.h_stage.h + S.job_off);
+  S.dj = reinterpret_cast<FrameJob*>(S.d_stage.d + S.job_off);
+  S.hi = reinterpret_cast<WorkItem*>(S.h_stage.h + S.item_off);
+  S.di = reinterpret_cast<WorkItem*>(S.d_stage.d + S.item_off);
+  for (size_t i = 0; i < ss.size(); i++)
+    for (auto& f : frames[i]) {
+      if (f.rc || !f.device_work) continue;
+      if (f.staged) {
+        std::memcpy(S.h_stage.h + f.staged_off, f.host, f.len);
+        f.dev = S.d_stage.d + f.staged_off;
+      }
+      for (int sl : {f.main_slot, f.ov_slot}) {
+        if (sl < 0) continue;
+        PayloadSlot* p = ss[i]->slot(sl);
+        std::memcpy(S.h_stage.h + p->desc_off, p->hdesc, p->desc_bytes);
+      }
+    }
+  if (speculative)
+    for (size_t i = 0; i < ss.size(); i++) {
+      const std::vector<char> ok(frames[i].size(), 1);
+      if (!assign_buffers(*ss[i], frames[i], ok, ok)) return fail(EBIV_ERR_DEVICE, "frame pool exhausted");
+    }
+
+  // ---- jobs: one per coded payload, a session's in stream order
   for (size_t j = 0; j < njobs; j++) {
     const JobRef& r = S.jrefs[j];
     const Session& s = *ss[r.si];
     const StepFrame& f = frames[r.si][r.fi];
     PayloadSlot* p = S.payload(r);
-    FrameJob& jb = S.h_jobs.h[j];
+    FrameJob& jb = S.hj[j];
     std::memset(&jb, 0, sizeof jb);
     jb.pkt = f.dev;
     jb.desc = S.d_stage.d + p->desc_off;
@@ -755,36 +780,45 @@ int Engine::launch_entropy(StepBuf& S, hipStream_t ps, bool speculative) {
     }
   }
   std::stable_sort(order.begin(), order.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
-  for (size_t w = 0; w < order.size(); w++) S.h_items.h[w] = order[w].second;
-  if (speculative)
+  for (size_t w = 0; w < order.size(); w++) S.hi[w] = order[w].second;
+  if (speculative) {  // the frame buffers assigned above, then the reconstruction's items
     for (size_t j = 0; j < njobs; j++) {
       const JobRef& r = S.jrefs[j];
       const Session& s = *ss[r.si];
       const StepFrame& f = frames[r.si][r.fi];
-      FrameJob& jb = S.h_jobs.h[j];
+      FrameJob& jb = S.hj[j];
       jb.rec = s.pool[static_cast<size_t>(r.ov ? f.gtarget : f.target)].p;
       jb.ref = s.pool[static_cast<size_t>(r.ov ? f.target : f.ref_in)].p;
       jb.gold = s.pool[static_cast<size_t>(r.ov ? f.target : f.gold_in)].p;
     }
-  if (njobs &&
-      (hipMemcpyAsync(S.d_jobs.d, S.h_jobs.h, sizeof(FrameJob) * njobs, hipMemcpyHostToDevice, ps) != hipSuccess ||
-       hipMemcpyAsync(S.d_items.d, S.h_items.h, sizeof(WorkItem) * nparse, hipMemcpyHostToDevice, ps) != hipSuccess))
-    return fail(EBIV_ERR_DEVICE, "job table upload failed");
+    if (int rc = plan_recon(S)) return rc;
+  }
+  const size_t used = S.item_off + sizeof(WorkItem) * (speculative ? S.items_end : nparse);
+  if (hipMemcpyAsync(S.d_stage.d, S.h_stage.h, used, hipMemcpyHostToDevice, ps) != hipSuccess)
+    return fail(EBIV_ERR_DEVICE, "staging upload failed");
   if (nparse) {
     const int m = mark_begin(0, 0, ps);
-    launch_parse(ps, static_cast<int>(nparse), max_tw, S.d_jobs.d, S.d_items.d);
+    launch_parse(ps, static_cast<int>(nparse), max_tw, S.dj, S.di);
     mark_end(m, ps);
-    if (hipMemcpyAsync(S.h_status.h, S.d_status.d, sizeof(TileStatus) * nparse, hipMemcpyDeviceToHost, ps) !=
-        hipSuccess)
-      return fail(EBIV_ERR_DEVICE, "status download failed");
   }
   if (hipGetLastError() != hipSuccess) return fail(EBIV_ERR_DEVICE, "kernel launch failed");
+  if (!speculative)
+    if (int rc = download_status(S, ps)) return rc;
+  S.live = true;
+  return EBIV_OK;
+}
+
+// The step's tile statuses to the host on `st`, then the event
+// finish_entropy waits for.
+int Engine::download_status(StepBuf& S, hipStream_t st) {
+  if (S.nparse && hipMemcpyAsync(S.h_status.h, S.d_status.d, sizeof(TileStatus) * S.nparse, hipMemcpyDeviceToHost,
+                                 st) != hipSuccess)
+    return fail(EBIV_ERR_DEVICE, "status download failed");
   if (!S.parsed && hipEventCreateWithFlags(&S.parsed, hipEventDisableTiming) != hipSuccess) {
     S.parsed = nullptr;
     return fail(EBIV_ERR_DEVICE, "event creation failed");
   }
-  if (hipEventRecord(S.parsed, ps) != hipSuccess) return fail(EBIV_ERR_DEVICE, "event record failed");
-  S.live = true;
+  if (hipEventRecord(S.parsed, st) != hipSuccess) return fail(EBIV_ERR_DEVICE, "event record failed");
   return EBIV_OK;
 }
 
@@ -827,7 +861,7 @@ int Engine::finish_entropy(StepBuf& S) {
 // tile, for throughput).  Sessions are dealt to lanes round robin; a lane's
 // launches run in order on its own stream, and the lanes overlap.  Sessions
 // share nothing, so the split leaves every result unchanged.
-int Engine::launch_recon(StepBuf& S) {
+int Engine::plan_recon(StepBuf& S) {
   auto& ss = S.ss;
   auto& frames = S.frames;
   const bool exact = !S.okm.empty();  // statuses known: buffers from the real outcomes
@@ -839,14 +873,14 @@ int Engine::launch_recon(StepBuf& S) {
       if (!ss[r.si]) continue;
       const Session& s = *ss[r.si];
       const StepFrame& f = frames[r.si][r.fi];
-      FrameJob& jb = S.h_jobs.h[j];
+      FrameJob& jb = S.hj[j];
       jb.rec = s.pool[static_cast<size_t>(r.ov ? f.gtarget : f.target)].p;
       jb.ref = s.pool[static_cast<size_t>(r.ov ? f.target : f.ref_in)].p;
       jb.gold = s.pool[static_cast<size_t>(r.ov ? f.target : f.gold_in)].p;
     }
   }
   const int nl = std::max(1, std::min({sched.recon_lanes, static_cast<int>(kMaxLanes), static_cast<int>(ss.size())}));
-  if (nl > 1 && !ensure_lanes(nl)) return fail(EBIV_ERR_DEVICE, "stream creation failed");
+  S.lanes = nl;
   auto lane_of = [&](size_t si) { return static_cast<int>(si % static_cast<size_t>(nl)); };
   // Rounds by dependency depth: a payload's round is one after the latest
   // round of the payloads of this step whose output it reads (its ref and
@@ -855,7 +889,8 @@ int Engine::launch_recon(StepBuf& S) {
   // every key frame of the step reconstructs in round 0, beside the others
   // there, instead of each holding up the round of its stream position.
   // Targets are fresh buffers, so no payload writes what another still reads.
-  std::vector<std::vector<size_t>> rounds;
+  auto& rounds = S.rounds;
+  rounds.clear();
   {
     std::vector<int> level(S.jrefs.size(), -1);
     std::vector<int> writer;  // pool buffer -> job writing it in this step
@@ -893,9 +928,18 @@ int Engine::launch_recon(StepBuf& S) {
   auto is_key = [&](size_t j) { return !S.payload(S.jrefs[j])->inter; };
   size_t w = S.nparse;
   const size_t nrl = rounds.size() * static_cast<size_t>(nl);
-  std::vector<size_t> inter_off(nrl), intra_off(nrl), key_end(nrl), round_end(nrl);
-  std::vector<uint32_t> inter_groups_of(nrl);
-  std::vector<uint64_t> inter_mbs_of(nrl);
+  auto& inter_off = S.inter_off;
+  auto& intra_off = S.intra_off;
+  auto& key_end = S.key_end;
+  auto& round_end = S.round_end;
+  auto& inter_groups_of = S.inter_groups_of;
+  auto& inter_mbs_of = S.inter_mbs_of;
+  inter_off.assign(nrl, 0);
+  intra_off.assign(nrl, 0);
+  key_end.assign(nrl, 0);
+  round_end.assign(nrl, 0);
+  inter_groups_of.assign(nrl, 0);
+  inter_mbs_of.assign(nrl, 0);
   for (size_t r = 0; r < rounds.size(); r++)
     for (int g = 0; g < nl; g++) {
       const size_t q = r * static_cast<size_t>(nl) + static_cast<size_t>(g);
@@ -905,7 +949,7 @@ int Engine::launch_recon(StepBuf& S) {
       uint32_t prefix = 0;
       for (size_t j : rounds[r])
         if (lane_of(S.jrefs[j].si) == g) {
-          S.h_items.h[w++] = WorkItem{static_cast<uint32_t>(j), prefix};
+          S.hi[w++] = WorkItem{static_cast<uint32_t>(j), prefix};
           prefix += inter_groups(j);
           if (!is_key(j)) inter_mbs_of[q] += static_cast<uint64_t>(ss[S.jrefs[j].si]->mbc) * ss[S.jrefs[j].si]->mbr;
         }
@@ -915,19 +959,39 @@ int Engine::launch_recon(StepBuf& S) {
         for (size_t j : rounds[r])
           if (lane_of(S.jrefs[j].si) == g && is_key(j) == (pass == 0))
             for (int t = 0; t < S.payload(S.jrefs[j])->ntiles; t++)
-              S.h_items.h[w++] = WorkItem{static_cast<uint32_t>(j), static_cast<uint32_t>(t)};
+              S.hi[w++] = WorkItem{static_cast<uint32_t>(j), static_cast<uint32_t>(t)};
         if (pass == 0) key_end[q] = w;
       }
       round_end[q] = w;
     }
-  if (exact && !S.jrefs.empty() &&
-      hipMemcpyAsync(S.d_jobs.d, S.h_jobs.h, sizeof(FrameJob) * S.jrefs.size(), hipMemcpyHostToDevice, stream) !=
-          hipSuccess)
-    return fail(EBIV_ERR_DEVICE, "job table upload failed");
-  if (w > S.nparse &&
-      hipMemcpyAsync(S.d_items.d + S.nparse, S.h_items.h + S.nparse, sizeof(WorkItem) * (w - S.nparse),
-                     hipMemcpyHostToDevice, stream) != hipSuccess)
-    return fail(EBIV_ERR_DEVICE, "job table upload failed");
+  S.items_end = w;
+  S.planned = true;
+  return EBIV_OK;
+}
+
+// Reconstructs the step: plans it (unless launch_entropy did), uploads the
+// job table and items the plan changed (one copy, from the job table to the
+// last item), and launches the rounds.
+int Engine::launch_recon(StepBuf& S) {
+  auto& ss = S.ss;
+  auto& frames = S.frames;
+  if (!S.planned) {
+    if (int rc = plan_recon(S)) return rc;
+    const size_t from = S.job_off, to = S.item_off + sizeof(WorkItem) * S.items_end;
+    if (!S.jrefs.empty() &&
+        hipMemcpyAsync(S.d_stage.d + from, S.h_stage.h + from, to - from, hipMemcpyHostToDevice, stream) != hipSuccess)
+      return fail(EBIV_ERR_DEVICE, "job table upload failed");
+  }
+  const int nl = S.lanes;
+  if (nl > 1 && !ensure_lanes(nl)) return fail(EBIV_ERR_DEVICE, "stream creation failed");
+  auto lane_of = [&](size_t si) { return static_cast<int>(si % static_cast<size_t>(nl)); };
+  const auto& rounds = S.rounds;
+  const auto& inter_off = S.inter_off;
+  const auto& intra_off = S.intra_off;
+  const auto& key_end = S.key_end;
+  const auto& round_end = S.round_end;
+  const auto& inter_groups_of = S.inter_groups_of;
+  const auto& inter_mbs_of = S.inter_mbs_of;
 
   // ---- raw frames: packed planes into their buffers (codec.go:195-207)
   for (size_t i = 0; i < ss.size(); i++) {
@@ -967,21 +1031,21 @@ int Engine::launch_recon(StepBuf& S) {
           return fail(EBIV_ERR_DEVICE, "golden copy failed");
       if (intra_off[q] > inter_off[q]) {
         const int m = mark_begin(1, inter_mbs_of[q], st);
-        launch_recon_inter(st, static_cast<int>(inter_groups_of[q]), S.d_jobs.d, S.d_items.d + inter_off[q],
+        launch_recon_inter(st, static_cast<int>(inter_groups_of[q]), S.dj, S.di + inter_off[q],
                            static_cast<int>(intra_off[q] - inter_off[q]));
         mark_end(m, st);
       }
       if (key_end[q] > intra_off[q]) {
         const int m = mark_begin(2, 0, st);
         const size_t n = key_end[q] - intra_off[q];
-        launch_recon_intra(st, static_cast<int>(n), S.d_jobs.d, S.d_items.d + intra_off[q], sched.key_intra_waves(n),
+        launch_recon_intra(st, static_cast<int>(n), S.dj, S.di + intra_off[q], sched.key_intra_waves(n),
                            true, sched.intra_cells);
         mark_end(m, st);
       }
       if (round_end[q] > key_end[q]) {
         const size_t n = round_end[q] - key_end[q];
         const int m = mark_begin(2, 0, st);
-        launch_recon_intra(st, static_cast<int>(n), S.d_jobs.d, S.d_items.d + key_end[q], sched.inter_intra_waves(n),
+        launch_recon_intra(st, static_cast<int>(n), S.dj, S.di + key_end[q], sched.inter_intra_waves(n),
                            false, sched.intra_cells);
         mark_end(m, st);
       }
@@ -1033,6 +1097,7 @@ int Engine::run_plain(StepBuf& S) {
   S.okm.clear();
   if (int rc = launch_entropy(S, stream, true)) return rc;
   if (int rc = launch_recon(S)) return rc;
+  if (int rc = download_status(S, stream)) return rc;
   if (int rc = wait_recon()) return rc;
   if (int rc = finish_entropy(S)) return rc;
   commit(S);
