@@ -2874,7 +2874,7 @@ __device__ __forceinline__ void mc_part(const Planes& src, const Planes& rec, co
 // only the blocks of the write mask are stored.
 __device__ __forceinline__ void recon_inter_mb(const uint8_t* __restrict__ desc, const MBRec r,
                                                const uint32_t* __restrict__ coefs, ConstJob& job, const int mb,
-                                               InterScratch& S, const int lane) {
+                                               const int mbx, const int mby, InterScratch& S, const int lane) {
   // Everything per macroblock is wave-uniform: scalar registers and loads.
   // Macroblocks a failed tile stream never reached carry kind kKindUnreached.
   const int mbc = job.mbc;
@@ -2895,7 +2895,6 @@ __device__ __forceinline__ void recon_inter_mb(const uint8_t* __restrict__ desc,
   }
   if (r.kind > 2) return;
   const bool has_res = r.kind == 1 && r.coef_count > 0;
-  const int mby = static_cast<int>(sgpr(static_cast<uint32_t>(mb / mbc))), mbx = mb - mby * mbc;
   // (the reference is chosen by address in constant memory: selecting
   // between two copies of the plane table would put them in scratch)
   const bool golden = sgpr(static_cast<uint32_t>(r.kind == 1 && r.refsel == kRefGolden)) != 0;
@@ -2956,8 +2955,19 @@ __global__ __launch_bounds__(256, 8) void ebiv_recon_inter(const FrameJob* __res
   if (mb0 >= nmb) return;
   const MBRec r0 = recs[mb0];
   const MBRec r1 = recs[mb1 < nmb ? mb1 : mb0];
-  recon_inter_mb(job.desc, r0, job.coefs, job, mb0, scratch[wave], lane);
-  if (mb1 < nmb) recon_inter_mb(job.desc, r1, job.coefs, job, mb1, scratch[wave], lane);
+  // their grid positions: one division per wavefront (mb1 is 4 columns on)
+  const int mbc = job.mbc;
+  const int y0 = static_cast<int>(sgpr(static_cast<uint32_t>(mb0 / mbc))), x0 = mb0 - y0 * mbc;
+  int x1 = x0 + 4, y1 = y0;
+  while (x1 >= mbc) {  // (at most 4 rounds, when the frame is narrower than 4 macroblocks)
+    x1 -= mbc;
+    y1++;
+  }
+  const uint32_t pos1 = sgpr(static_cast<uint32_t>(x1 | y1 << 16));  // one register across the first
+  recon_inter_mb(job.desc, r0, job.coefs, job, mb0, x0, y0, scratch[wave], lane);
+  if (mb1 < nmb)
+    recon_inter_mb(job.desc, r1, job.coefs, job, mb1, static_cast<int>(pos1 & 0xFFFF), static_cast<int>(pos1 >> 16),
+                   scratch[wave], lane);
 }
 
 // ============================================================================
