@@ -100,6 +100,7 @@ inline bool operator==(MV a, MV b) { return a.x == b.x && a.y == b.y; }
 inline int16_t clamp_mv(int v) { return static_cast<int16_t>(std::clamp(v, -kMVClamp, kMVClamp)); }
 
 struct Tok { uint16_t ctx, sym; };
+static_assert(sizeof(Tok) == 4, "token log exports (ctx, sym) as u16 pairs");
 struct TileStream {
   std::vector<Tok> toks;
   void put(int ctx, int sym) { toks.push_back({static_cast<uint16_t>(ctx), static_cast<uint16_t>(sym)}); }
@@ -1086,6 +1087,16 @@ void put_le(std::vector<uint8_t>& o, uint64_t v, int bytes) {
 
 struct IndexEntry { uint64_t off; uint32_t size; bool key; };
 
+// One coded payload's token streams as the rANS coder consumed them, with the
+// final frequency vectors (for the grammar walk in the tests).
+struct PayloadLog {
+  int packet;  // index of the record the payload went into
+  bool inter;
+  int grid_cols, grid_rows, mbc, mbr;
+  std::vector<std::vector<Tok>> tiles;
+  Freqs freqs;
+};
+
 // ---- encoder ------------------------------------------------------------------------------
 struct Encoder {
   int W, H, CW, CH;
@@ -1104,6 +1115,8 @@ struct Encoder {
   std::vector<std::vector<uint8_t>> packets;  // frame payloads in stream order
   CostModel cost_store;
   int64_t accepted = 0;
+  bool log_tokens = false;
+  std::vector<PayloadLog> logs;
 
   void write_record(const std::vector<uint8_t>& payload, bool key) {  // writer.go:56-88
     index.push_back({out.size(), static_cast<uint32_t>(payload.size()), key});
@@ -1195,6 +1208,12 @@ struct Encoder {
     fe.encode(r != nullptr, !fast, cost_store);
     auto payload = fe.finish(qp, pv, freqs);
     rec_out = fe.rec;
+    if (log_tokens) {
+      PayloadLog l{static_cast<int>(packets.size()), r != nullptr, fe.grid_cols, fe.grid_rows, fe.src.mbc, fe.src.mbr,
+                   {}, freqs};
+      for (auto& st : fe.streams) l.tiles.push_back(std::move(st.toks));
+      logs.push_back(std::move(l));
+    }
     return payload;
   }
   void flush_gop() {  // encoder.go:250-306
@@ -1334,6 +1353,37 @@ const uint8_t* ebivenc_packet(void* h, size_t i, size_t* len) {
 size_t ebivenc_recon_count(void* h) { return static_cast<Encoder*>(h)->recons.size(); }
 const uint8_t* ebivenc_recon(void* h, size_t i) { return static_cast<Encoder*>(h)->recons[i].data(); }
 void ebivenc_free(void* h) { delete static_cast<Encoder*>(h); }
+
+// Token log of the coded payloads (set before the first write): per payload
+// its record index, inter flag, tile grid, macroblock grid, the (ctx, sym)
+// tokens of each tile in stream order and the frequency vector of every
+// context (zeros where the context shipped none).
+void ebivenc_log_tokens(void* h, int on) { static_cast<Encoder*>(h)->log_tokens = on != 0; }
+size_t ebivenc_payload_count(void* h) { return static_cast<Encoder*>(h)->logs.size(); }
+int ebivenc_payload_info(void* h, size_t i, int32_t* info) {
+  auto* e = static_cast<Encoder*>(h);
+  if (i >= e->logs.size()) return -1;
+  const auto& l = e->logs[i];
+  info[0] = l.packet; info[1] = l.inter; info[2] = l.grid_cols; info[3] = l.grid_rows; info[4] = l.mbc; info[5] = l.mbr;
+  return 0;
+}
+const uint16_t* ebivenc_payload_tokens(void* h, size_t i, int tile, size_t* count) {
+  auto* e = static_cast<Encoder*>(h);
+  *count = 0;
+  if (i >= e->logs.size() || tile < 0 || tile >= static_cast<int>(e->logs[i].tiles.size())) return nullptr;
+  const auto& t = e->logs[i].tiles[tile];
+  *count = t.size();
+  return reinterpret_cast<const uint16_t*>(t.data());
+}
+int ebivenc_payload_freqs(void* h, size_t i, uint32_t* out) {  // kNumContexts x kMaxAlphabet
+  auto* e = static_cast<Encoder*>(h);
+  if (i >= e->logs.size()) return -1;
+  const auto& f = e->logs[i].freqs;
+  for (int c = 0; c < ebiv::kNumContexts; c++)
+    for (int k = 0; k < ebiv::kMaxAlphabet; k++)
+      out[c * ebiv::kMaxAlphabet + k] = c < static_cast<int>(f.size()) && k < static_cast<int>(f[c].size()) ? f[c][k] : 0;
+  return 0;
+}
 
 // Test hooks over the encoder's own block grammar (residual.go:30-75,
 // rans.go:199-214) so the decoder can be checked on hand-written level

@@ -8,7 +8,7 @@ encoder's default (encoder.go:134-143).
 from __future__ import annotations
 
 import ctypes as C
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 
 import numpy as np
 
@@ -22,12 +22,26 @@ class EncodedClip:
     recons: list              # encoder reconstruction per frame: (Y, Cb, Cr) packed, or []
     width: int
     height: int
+    # token log (log_tokens=True): one Payload per coded payload in stream order
+    payloads: list = field(default_factory=list)
+
+
+@dataclass
+class Payload:
+    """A coded payload's token streams as the encoder produced them."""
+    packet: int               # index of the packet the payload went into (a key frame's override shares it)
+    inter: bool
+    grid: tuple               # (tile_cols, tile_rows) after newTileGrid's clamp
+    mbs: tuple                # (mbc, mbr)
+    tiles: list               # per tile: uint16 array (tokens, 2) of (ctx, sym)
+    freqs: np.ndarray         # (contexts, 16) frequency vectors, zero where none shipped
 
 
 class Encoder:
     def __init__(self, width: int, height: int, *, qp: int = 20, gop: int = 1, tile_cols: int = 1,
                  tile_rows: int = 1, altref: int = 0, fast: bool = True, coding: str = "coded",
-                 fps_num: int = 30, fps_den: int = 1, threads: int = 8, keep_recon: bool = False):
+                 fps_num: int = 30, fps_den: int = 1, threads: int = 8, keep_recon: bool = False,
+                 log_tokens: bool = False):
         self._L = native.encoder_lib()
         self.width, self.height = width, height
         self.cw, self.ch = (width + 1) // 2, (height + 1) // 2
@@ -37,6 +51,8 @@ class Encoder:
                                       cod, int(fast), threads, int(keep_recon))
         if not self._h:
             raise ValueError("invalid encoder configuration")
+        if log_tokens:
+            self._L.ebivenc_log_tokens(self._h, 1)
 
     def write(self, y: np.ndarray, cb: np.ndarray, cr: np.ndarray) -> None:
         y = np.ascontiguousarray(y, np.uint8)
@@ -61,9 +77,26 @@ class Encoder:
             recons.append((raw[:ys].reshape(self.height, self.width),
                            raw[ys:ys + cs].reshape(self.ch, self.cw),
                            raw[ys + cs:].reshape(self.ch, self.cw)))
+        payloads = []
+        info = (C.c_int32 * 6)()
+        freqs = np.zeros((NUM_CONTEXTS, 16), np.uint32)
+        for i in range(L.ebivenc_payload_count(self._h)):
+            assert L.ebivenc_payload_info(self._h, i, info) == 0
+            tiles = []
+            for t in range(info[2] * info[3]):
+                n = C.c_size_t()
+                p = L.ebivenc_payload_tokens(self._h, i, t, C.byref(n))
+                tiles.append(np.frombuffer(C.string_at(p, 4 * n.value), np.uint16).reshape(n.value, 2).copy()
+                             if n.value else np.zeros((0, 2), np.uint16))
+            assert L.ebivenc_payload_freqs(self._h, i, freqs.ctypes.data) == 0
+            payloads.append(Payload(info[0], bool(info[1]), (info[2], info[3]), (info[4], info[5]), tiles,
+                                    freqs.copy()))
         L.ebivenc_free(self._h)
         self._h = None
-        return EncodedClip(data, packets, recons, self.width, self.height)
+        return EncodedClip(data, packets, recons, self.width, self.height, payloads)
+
+
+NUM_CONTEXTS = 72  # ctxTokenBase (12) + 45 luma + 15 chroma token contexts (model.go:124-149)
 
 
 def encode_frames(frames, width, height, **kw) -> EncodedClip:

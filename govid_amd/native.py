@@ -53,6 +53,15 @@ def encoder_lib():
         L.ebivenc_recon.restype = vp
         L.ebivenc_recon.argtypes = [vp, sz]
         L.ebivenc_free.argtypes = [vp]
+        L.ebivenc_log_tokens.argtypes = [vp, C.c_int]
+        L.ebivenc_payload_count.restype = sz
+        L.ebivenc_payload_count.argtypes = [vp]
+        L.ebivenc_payload_info.restype = C.c_int
+        L.ebivenc_payload_info.argtypes = [vp, sz, vp]
+        L.ebivenc_payload_tokens.restype = vp
+        L.ebivenc_payload_tokens.argtypes = [vp, sz, C.c_int, C.POINTER(sz)]
+        L.ebivenc_payload_freqs.restype = C.c_int
+        L.ebivenc_payload_freqs.argtypes = [vp, sz, vp]
         L.ebivenc_blocks.restype = C.c_int
         L.ebivenc_blocks.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, vp, C.c_int, vp, C.POINTER(sz), vp,
                                      C.POINTER(sz)]

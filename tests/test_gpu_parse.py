@@ -155,3 +155,41 @@ def test_generic_grammar_decoder_without_a_sign_table():
     st2 = gpu2.stats()
     # (the frame after it fails on the host: its tables delta-code the removed one)
     assert st2["tiles_generic"] == 4 and st2["tiles"] == 4 * 3, st2
+
+
+def test_parse_matches_the_grammar_walk():
+    """The entropy kernel against the independent grammar walk of the
+    encoder's own token streams (ebiv/bitaudit_test.go:306-441), not against
+    the oracle: per tile the GPU's rANS decode count equals the walker's
+    token count, per packet (override payloads included) the codec's symbol
+    count does, and per macroblock the records carry the walker's mode,
+    transform size, intra modes, reference, partition, coded-block pattern
+    and nonzero count."""
+    from govid_amd.codec import Codec
+    from grammar_walk import MB_INTER, MB_INTRA, walk_payload
+    from test_grammar_walk import CASES, _by_packet, _clip
+
+    for case in CASES:
+        print(f"case {case}", flush=True)
+        clip = _clip(case)
+        by_packet = _by_packet(clip)
+        gpu = Codec()
+        for i, pkt in enumerate(clip.packets):
+            before = gpu.stats()["symbols"]
+            gpu.decode(pkt)
+            walks = [walk_payload(p) for p in by_packet[i]]
+            assert gpu.stats()["symbols"] - before == sum(w["tokens"] for w in walks), f"packet {i}"
+            recs, _, status = gpu.debug_dump(0)
+            tiles = by_packet[i][0].tiles
+            assert [int(s["nsym"]) for s in status[:len(tiles)]] == [len(t) for t in tiles], f"packet {i}"
+            assert all(int(s["err"]) == 0 for s in status[:len(tiles)])
+            for m, r in enumerate(walks[0]["recs"]):
+                g, where = recs[m], f"{case['workload']} packet {i} mb {m}"
+                assert int(g["kind"]) == r["kind"] and int(g["coef_count"]) == r["ncoef"], where
+                if r["kind"] == MB_INTRA:
+                    nb = len(r["lmode"])
+                    lm = [(int(g["lmode"][b // 8]) >> (4 * (b % 8))) & 15 for b in range(nb)]
+                    assert (int(g["tx"]), lm, int(g["cmode"])) == (r["tx"], r["lmode"], r["cmode"]), where
+                elif r["kind"] == MB_INTER:
+                    assert (int(g["refsel"]), int(g["part"]), int(g["cbp"])) == (r["ref"], r["part"], r["cbp"]), where
+        gpu.close()
