@@ -21,7 +21,7 @@ static_assert(sizeof(TileDesc) == 24, "TileDesc layout");
 
 // The frame's 72 static rANS tables (rans.go:120-148) as the entropy kernel
 // holds them in registers: 15 vectors of 64 lanes, one packed entry per lane,
-//   entry = start << 13 | freq   (start <= 4095, freq <= 4096),
+//   entry = start << 16 | freq   (start <= 4096, freq <= 4096),
 // kNoSym past a context's alphabet and on every lane of an unused context.
 //   vector 0: sign (lanes 0-15), MB mode (16-31), transform size (32-47), CBP (48-63)
 //   vector 1: luma mode, chroma mode, reference, partition (16 lanes each)

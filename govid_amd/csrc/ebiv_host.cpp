@@ -58,7 +58,7 @@ int HostCodec::parse_tables(const uint8_t* b, size_t len, bool key, FrameHdr& hd
     if (table_place(c, &vec, &lane0)) {
       uint32_t start = 0;
       for (int s = 0; s < n; s++) {
-        hdr.vt[vec][lane0 + s] = start << 13 | f[s];
+        hdr.vt[vec][lane0 + s] = start << 16 | f[s];
         start += f[s];
       }
     }

@@ -159,14 +159,15 @@ __device__ __forceinline__ uint32_t mv_set(uint32_t v, int comp, int16_t c) {
 // and the coefficient staging buffer.
 //
 // Symbol search.  The frame's tables sit in 15 VGPRs (FrameHdr::vt): lane l
-// of a vector holds one packed entry start << 13 | freq.  For a slot the
-// entries of the context's lane group with start <= slot are exactly those
-// with entry <= (slot << 13 | 0x1FFF); their count minus one is the symbol
-// (the last symbol whose range starts at or below the slot, rans.go:120-148
-// slot2sym), and a count of zero means an unused context (every entry is
-// kNoSym).  One compare over the vector, one ballot, one popcount and one
-// readlane (start and frequency together) decode a symbol; no memory access
-// is on the dependency chain from one symbol to the next.
+// of a vector holds one packed entry start << 16 | freq.  For a slot the
+// lanes of the context's group whose start (the upper half, compared in
+// place with an SDWA word select) is at most the slot are counted; the count
+// minus one is the symbol (the last symbol whose range starts at or below
+// the slot, rans.go:120-148 slot2sym), and a count of zero means an unused
+// context (every entry is kNoSym, whose start 0xFFFF exceeds every slot).
+// One compare over the vector, one ballot, one popcount and one readlane
+// (start and frequency together) decode a symbol; no memory access is on the
+// dependency chain from one symbol to the next.
 // ----------------------------------------------------------------------------
 // Common tail of the single-symbol decoders: xn is the updated state.
 // Renormalise from the lane window while pos > lim (lim = max(window base, 0)),
@@ -243,9 +244,7 @@ struct Rans {
     uint64_t b;
     asm volatile(
         "s_and_b32 %[slot], %[x0], 0xfff\n"
-        "s_lshl_b32 %[t1], %[slot], 13\n"
-        "s_or_b32 %[t1], %[t1], 0x1fff\n"
-        "v_cmp_ge_u32_e64 %[b], %[t1], %[V]\n"
+        "v_cmp_ge_u32_sdwa %[b], %[slot], %[V] src0_sel:DWORD src1_sel:WORD_1\n"
         "s_lshr_b32 %[q], %[x0], 12\n"
         "s_bfe_u64 %[b], %[b], %[bfe]\n"
         "s_bcnt1_i32_b64 %[sym], %[b]\n"
@@ -254,8 +253,8 @@ struct Rans {
         "s_add_u32 %[t1], %[sym], %[gm1]\n"
         "v_readlane_b32 %[p], %[V], %[t1]\n"
         "s_add_u32 %[sym], %[sym], -1\n"
-        "s_and_b32 %[t2], %[p], 0x1fff\n"
-        "s_lshr_b32 %[p], %[p], 13\n"
+        "s_and_b32 %[t2], %[p], 0xffff\n"
+        "s_lshr_b32 %[p], %[p], 16\n"
         "s_mul_i32 %[xn], %[t2], %[q]\n"
         "s_sub_u32 %[t2], %[slot], %[p]\n"
         "s_add_u32 %[xn], %[xn], %[t2]\n"
@@ -280,14 +279,14 @@ struct Rans {
   __device__ __forceinline__ int decode_rt(uint32_t V, uint32_t goff, uint32_t gmask, int lane) {
     const uint32_t x = x0;
     const uint32_t slot = x & (kRansM - 1);
-    const uint64_t b = __ballot(V <= ((slot << 13) | 0x1FFFu));
+    const uint64_t b = __ballot((V >> 16) <= slot);
     const uint32_t cnt = static_cast<uint32_t>(__builtin_popcount(static_cast<uint32_t>(b >> goff) & gmask));
     if (__builtin_expect(cnt == 0, 0)) {
       err = kErrCorrupt | (kTileUnused << 8);
       return -1;
     }
     const uint32_t p = rd_lane(V, static_cast<int>(goff + cnt - 1));
-    if (!advance((p & 0x1FFFu) * (x >> kRansScaleBits) + slot - (p >> 13), lane)) return -1;
+    if (!advance((p & 0xFFFFu) * (x >> kRansScaleBits) + slot - (p >> 16), lane)) return -1;
     return static_cast<int>(cnt - 1);
   }
   // The bypass context, fixed at {2048, 2048} (rans.go:312-322).
@@ -415,302 +414,6 @@ __device__ __forceinline__ bool parse_block(Rans& R, CoefOut& out, const Tables&
   return true;
 }
 
-// Hand-scheduled token loop of decodeBlock (residual.go:86-115) for the common
-// case: every symbol of a (zero, one .. four) token and its sign.  It runs
-// until the block ends (EOB or the last scan position), an escape token has
-// been decoded (the caller decodes its magnitude and resumes), the byte window
-// needs to move, or a decode error.  All state is uniform and lives in SGPRs;
-// the only vector work is the table compare, lane reads of the table, scan and
-// byte window, and staging one coefficient entry per lane.
-//
-// Preconditions (checked by the caller): every rANS state is normalised
-// (>= 2^23), so a symbol needs at most two renormalisation bytes; the sign
-// context is in use (its two frequencies are f0 and f1); and the byte window
-// holds every byte the next iteration can consume: at most 4 bytes (a token
-// and a sign), which the loop re-checks against lim at each scan position.
-enum TokEvent : uint32_t { kTokDone = 0, kTokRefill = 1, kTokEscape = 2, kTokUnused = 3, kTokUnderrun = 4 };
-
-struct TokState {
-  uint32_t x0, x1, x2, x3, pos, nsym;
-  uint32_t si, stop, band, boff, prev;
-  uint32_t sumabs, firstsi, firstpm, last;
-  uint32_t cn, cbase;
-  uint32_t vcur, vbuf;  // per lane
-};
-
-// Renormalisation bytes of one symbol (rans.go:286-303): the byte below pos
-// from the lane window; underrun is an error.
-#define EBIV_RENORM(tag)                                  \
-  "L%=_" tag ":\n"                                    \
-      "s_cmp_eq_u32 %[pos], 0\n"                          \
-      "s_cbranch_scc1 L%=_underrun\n"                     \
-      "s_sub_u32 %[pos], %[pos], 1\n"                     \
-      "s_sub_u32 %[t1], %[pos], %[wbase]\n"               \
-      "s_lshr_b32 %[t2], %[t1], 2\n"                      \
-      "v_readlane_b32 %[t2], %[vwin], %[t2]\n"            \
-      "s_and_b32 %[t1], %[t1], 3\n"                       \
-      "s_lshl_b32 %[t1], %[t1], 3\n"                      \
-      "s_lshr_b32 %[t2], %[t2], %[t1]\n"                  \
-      "s_and_b32 %[t2], %[t2], 0xff\n"                    \
-      "s_lshl_b32 %[xn], %[xn], 8\n"                      \
-      "s_or_b32 %[xn], %[xn], %[t2]\n"                    \
-      "s_cmp_lt_u32 %[xn], 0x800000\n"                    \
-      "s_cbranch_scc1 L%=_" tag "\n"                      \
-      "s_branch L%=_" tag "done\n"
-
-__device__ __forceinline__ uint32_t token_loop(TokState& S, uint32_t resume, uint32_t lim, uint32_t wbase,
-                                               uint32_t total, uint32_t blk8, uint32_t f0, uint32_t f1,
-                                               uint32_t* dst, uint32_t vb, uint32_t vc, uint32_t zzv, uint32_t vwin,
-                                               uint32_t vlane4) {
-  uint32_t ev, slot, cnt, p, xn, t1, t2, mag, neg, vtmp, m0s;
-  uint64_t b;
-  asm volatile(
-      "s_nop 4\n"
-      "s_mov_b32 %[m0s], m0\n"
-      "s_cmp_lg_u32 %[resume], 0\n"
-      "s_cbranch_scc0 L%=_top\n"
-      "s_mov_b32 %[mag], %[resume]\n"
-      "s_mov_b32 %[prev], 16\n"
-      "s_branch L%=_nzcommon\n"
-
-      // ---- scan position si: the token under context (band, prev)
-      "L%=_top:\n"
-      "s_cmp_lt_i32 %[pos], %[lim]\n"
-      "s_cbranch_scc1 L%=_refill\n"
-      "s_and_b32 %[slot], %[x0], 0xfff\n"
-      "s_lshl_b32 %[t1], %[slot], 13\n"
-      "s_or_b32 %[t1], %[t1], 0x1fff\n"
-      "v_cmp_ge_u32_e64 %[b], %[t1], %[vcur]\n"
-      "s_add_u32 %[t2], %[boff], %[prev]\n"
-      "s_or_b32 %[t1], %[t2], 0x80000\n"
-      "s_lshr_b32 %[xn], %[x0], 12\n"
-      "s_bfe_u64 %[b], %[b], %[t1]\n"
-      "s_bcnt1_i32_b64 %[cnt], %[b]\n"
-      "s_cmp_eq_u32 %[cnt], 1\n"
-      "s_cbranch_scc0 L%=_nz\n"
-      // zero token: symbol 0 starts at 0
-      "v_readlane_b32 %[p], %[vcur], %[t2]\n"
-      "s_mov_b32 %[prev], 0\n"
-      "s_and_b32 %[p], %[p], 0x1fff\n"
-      "s_mul_i32 %[xn], %[p], %[xn]\n"
-      "s_add_u32 %[xn], %[xn], %[slot]\n"
-      "s_cmp_lt_u32 %[xn], 0x800000\n"
-      "s_cbranch_scc1 L%=_rz\n"
-      "L%=_rzdone:\n"
-      "s_mov_b32 %[x0], %[x1]\n"
-      "s_mov_b32 %[x1], %[x2]\n"
-      "s_mov_b32 %[x2], %[x3]\n"
-      "s_mov_b32 %[x3], %[xn]\n"
-      "s_add_u32 %[nsym], %[nsym], 1\n"
-      "s_add_u32 %[si], %[si], 1\n"
-      "s_cmp_lg_u32 %[si], %[stop]\n"
-      "s_cbranch_scc1 L%=_top\n"
-      "s_branch L%=_stop\n"
-
-      // ---- any other symbol of the token context
-      "L%=_nz:\n"
-      "s_cmp_eq_u32 %[cnt], 0\n"
-      "s_cbranch_scc1 L%=_unused\n"
-      "s_add_u32 %[t1], %[t2], %[cnt]\n"
-      "s_add_u32 %[t1], %[t1], -1\n"
-      "v_readlane_b32 %[p], %[vcur], %[t1]\n"
-      "s_add_u32 %[mag], %[cnt], -1\n"
-      "s_and_b32 %[t2], %[p], 0x1fff\n"
-      "s_lshr_b32 %[p], %[p], 13\n"
-      "s_mul_i32 %[xn], %[t2], %[xn]\n"
-      "s_sub_u32 %[t2], %[slot], %[p]\n"
-      "s_add_u32 %[xn], %[xn], %[t2]\n"
-      "s_cmp_lt_u32 %[xn], 0x800000\n"
-      "s_cbranch_scc1 L%=_rn\n"
-      "L%=_rndone:\n"
-      "s_mov_b32 %[x0], %[x1]\n"
-      "s_mov_b32 %[x1], %[x2]\n"
-      "s_mov_b32 %[x2], %[x3]\n"
-      "s_mov_b32 %[x3], %[xn]\n"
-      "s_add_u32 %[nsym], %[nsym], 1\n"
-      "s_cmp_ge_u32 %[cnt], 6\n"
-      "s_cbranch_scc1 L%=_escob\n"
-      // classForMag: 1 -> row 1 (8 lanes), 2..4 -> row 2
-      "s_cmp_eq_u32 %[mag], 1\n"
-      "s_cselect_b32 %[prev], 8, 16\n"
-
-      // ---- a nonzero level of magnitude mag at si
-      "L%=_nzcommon:\n"
-      "s_add_u32 %[sumabs], %[sumabs], %[mag]\n"
-      "s_cmp_eq_u32 %[firstsi], 0\n"
-      "s_cbranch_scc1 L%=_first\n"
-      // explicit sign: the two-symbol sign context from f0, f1
-      "s_sub_u32 %[neg], 0, %[mag]\n"
-      "s_and_b32 %[slot], %[x0], 0xfff\n"
-      "s_lshr_b32 %[xn], %[x0], 12\n"
-      "s_cmp_ge_u32 %[slot], %[f0]\n"
-      "s_cselect_b32 %[p], %[f0], 0\n"
-      "s_cselect_b32 %[t2], %[f1], %[f0]\n"
-      "s_cselect_b32 %[mag], %[neg], %[mag]\n"
-      "s_mul_i32 %[xn], %[t2], %[xn]\n"
-      "s_sub_u32 %[t2], %[slot], %[p]\n"
-      "s_add_u32 %[xn], %[xn], %[t2]\n"
-      "s_cmp_lt_u32 %[xn], 0x800000\n"
-      "s_cbranch_scc1 L%=_rs\n"
-      "L%=_rsdone:\n"
-      "s_mov_b32 %[x0], %[x1]\n"
-      "s_mov_b32 %[x1], %[x2]\n"
-      "s_mov_b32 %[x2], %[x3]\n"
-      "s_mov_b32 %[x3], %[xn]\n"
-      "s_add_u32 %[nsym], %[nsym], 1\n"
-      // coefficient entry: scan index | block << 8 | level << 13
-      "s_lshl_b32 %[t1], %[mag], 13\n"
-      "s_or_b32 %[t1], %[t1], %[si]\n"
-      "s_or_b32 %[t1], %[t1], %[blk8]\n"
-      "s_mov_b32 m0, %[cn]\n"
-      "v_writelane_b32 %[vbuf], %[t1], m0\n"
-      "s_add_u32 %[cn], %[cn], 1\n"
-      "s_cmp_eq_u32 %[cn], 64\n"
-      "s_cbranch_scc1 L%=_flush\n"
-      "L%=_flushdone:\n"
-      "s_mov_b32 %[last], %[si]\n"
-      "s_add_u32 %[si], %[si], 1\n"
-      "s_cmp_lg_u32 %[si], %[stop]\n"
-      "s_cbranch_scc1 L%=_top\n"
-      "s_branch L%=_stop\n"
-
-      // first nonzero of the block: its sign resolves at the end
-      "L%=_first:\n"
-      "s_lshl_b32 %[t1], %[mag], 8\n"
-      "s_or_b32 %[firstpm], %[t1], %[si]\n"
-      "s_add_u32 %[firstsi], %[si], 1\n"
-      "s_mov_b32 %[last], %[si]\n"
-      "s_add_u32 %[si], %[si], 1\n"
-      "s_cmp_lg_u32 %[si], %[stop]\n"
-      "s_cbranch_scc1 L%=_top\n"
-
-      // ---- si reached stop: the end of the block or a band boundary
-      // (bandOf thresholds 1, 3, 9, 21, model.go:165-180)
-      "L%=_stop:\n"
-      "s_cmp_eq_u32 %[si], %[total]\n"
-      "s_cbranch_scc1 L%=_done\n"
-      "s_add_u32 %[band], %[band], 1\n"
-      "s_xor_b32 %[boff], %[boff], 24\n"
-      "s_cmp_eq_u32 %[band], 2\n"
-      "s_cbranch_scc0 L%=_nb2\n"
-      "v_mov_b32 %[vcur], %[vb]\n"
-      "L%=_nb2:\n"
-      "s_cmp_eq_u32 %[band], 4\n"
-      "s_cbranch_scc0 L%=_nb4\n"
-      "v_mov_b32 %[vcur], %[vc]\n"
-      "s_mov_b32 %[stop], %[total]\n"
-      "s_branch L%=_top\n"
-      "L%=_nb4:\n"
-      "s_lshl_b32 %[t1], %[band], 3\n"
-      "s_lshr_b32 %[t1], 0x15090301, %[t1]\n"
-      "s_and_b32 %[t1], %[t1], 0xff\n"
-      "s_min_u32 %[stop], %[t1], %[total]\n"
-      "s_branch L%=_top\n"
-
-      // ---- escape or end of block (their symbol is consumed)
-      "L%=_escob:\n"
-      "s_cmp_eq_u32 %[cnt], 7\n"
-      "s_cbranch_scc1 L%=_done\n"
-      "s_mov_b32 %[ev], 2\n"
-      "s_branch L%=_exit\n"
-
-      // ---- 64 staged entries: one coalesced store
-      "L%=_flush:\n"
-      "s_lshl_b32 %[t1], %[cbase], 2\n"
-      "v_add_u32 %[vtmp], %[t1], %[vlane4]\n"
-      "global_store_dword %[vtmp], %[vbuf], %[dst]\n"
-      "s_add_u32 %[cbase], %[cbase], 64\n"
-      "s_mov_b32 %[cn], 0\n"
-      "s_branch L%=_flushdone\n"
-
-      // ---- renormalisation bytes (at most two; the loop checks underrun)
-      EBIV_RENORM("rz")
-      EBIV_RENORM("rn")
-      EBIV_RENORM("rs")
-
-      "L%=_refill:\n"
-      "s_mov_b32 %[ev], 1\n"
-      "s_branch L%=_exit\n"
-      "L%=_unused:\n"
-      "s_mov_b32 %[ev], 3\n"
-      "s_branch L%=_exit\n"
-      "L%=_underrun:\n"
-      "s_mov_b32 %[ev], 4\n"
-      "s_branch L%=_exit\n"
-      "L%=_done:\n"
-      "s_mov_b32 %[ev], 0\n"
-      "L%=_exit:\n"
-      "s_mov_b32 m0, %[m0s]\n"
-      "s_nop 4\n"
-      : [x0] "+&s"(S.x0), [x1] "+&s"(S.x1), [x2] "+&s"(S.x2), [x3] "+&s"(S.x3), [pos] "+&s"(S.pos), [nsym] "+&s"(S.nsym),
-        [si] "+&s"(S.si), [stop] "+&s"(S.stop), [band] "+&s"(S.band), [boff] "+&s"(S.boff), [prev] "+&s"(S.prev),
-        [sumabs] "+&s"(S.sumabs), [firstsi] "+&s"(S.firstsi), [firstpm] "+&s"(S.firstpm), [last] "+&s"(S.last),
-        [cn] "+&s"(S.cn), [cbase] "+&s"(S.cbase), [vcur] "+&v"(S.vcur), [vbuf] "+&v"(S.vbuf), [ev] "=&s"(ev),
-        [slot] "=&s"(slot), [cnt] "=&s"(cnt), [p] "=&s"(p), [xn] "=&s"(xn), [t1] "=&s"(t1), [t2] "=&s"(t2),
-        [mag] "=&s"(mag), [neg] "=&s"(neg), [b] "=&s"(b), [vtmp] "=&v"(vtmp), [m0s] "=&s"(m0s)
-      : [resume] "s"(sgpr(resume)), [lim] "s"(sgpr(lim)), [wbase] "s"(sgpr(wbase)), [total] "s"(sgpr(total)),
-        [blk8] "s"(sgpr(blk8)), [f0] "s"(sgpr(f0)), [f1] "s"(sgpr(f1)), [dst] "s"(sgpr(dst)), [vb] "v"(vb),
-        [vc] "v"(vc), [zzv] "v"(zzv), [vwin] "v"(vwin),
-        [vlane4] "v"(vlane4)
-      : "scc", "memory");
-  return ev;
-}
-
-// Per-tile inputs of the hand-scheduled token loop.
-struct FastTok {
-  bool sign_used;   // the sign context has a table
-  uint32_t f0, f1;  // its two frequencies
-  uint32_t vlane4;  // 4 * lane
-};
-
-// decodeBlock through token_loop: the block's state moves between the loop
-// and the generic decoder for escapes, window moves and errors.
-__device__ __forceinline__ bool parse_block_fast(Rans& R, CoefOut& out, const Tables& T, const FastTok& F,
-                                                 uint32_t va, uint32_t vb, uint32_t vc, uint32_t zzv, int total,
-                                                 int blk, int lane) {
-  TokState S;
-  S.x0 = R.x0; S.x1 = R.x1; S.x2 = R.x2; S.x3 = R.x3; S.pos = R.pos; S.nsym = R.nsym;
-  S.si = 0; S.stop = 1; S.band = 0; S.boff = 0; S.prev = 0;
-  S.sumabs = 0; S.firstsi = 0; S.firstpm = 0; S.last = 0;
-  S.cn = out.cn; S.cbase = out.base; S.vcur = va; S.vbuf = out.buf;
-  uint32_t resume = 0, ev;
-  for (;;) {
-    // The next scan position may consume 4 bytes below pos.
-    while (R.win.wbase > 0 && static_cast<int32_t>(S.pos) < R.win.wbase + 4) {
-      R.win.wbase -= 128;
-      R.win.load(lane);
-    }
-    const uint32_t lim = R.win.wbase > 0 ? static_cast<uint32_t>(R.win.wbase + 4) : 0x80000000u;
-    ev = token_loop(S, resume, lim, static_cast<uint32_t>(R.win.wbase), static_cast<uint32_t>(total),
-                    __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(blk) << 8), F.f0, F.f1, out.dst, vb, vc,
-                    zzv, R.win.cur, F.vlane4);
-    resume = 0;
-    if (ev == kTokRefill) continue;
-    R.x0 = S.x0; R.x1 = S.x1; R.x2 = S.x2; R.x3 = S.x3; R.pos = S.pos; R.nsym = S.nsym;
-    if (ev != kTokEscape) break;
-    const int e = R.class_value<kGEsc>(T.misc2, lane);  // getEscapeValue (mv.go:124-128)
-    if (e < 0) break;
-    resume = 5 + static_cast<uint32_t>(e);
-    S.x0 = R.x0; S.x1 = R.x1; S.x2 = R.x2; S.x3 = R.x3; S.pos = R.pos; S.nsym = R.nsym;
-  }
-  out.cn = S.cn; out.base = S.cbase; out.buf = S.vbuf;
-  if (ev == kTokEscape) return false;  // the escape value failed (R.err is set)
-  if (ev == kTokUnused) { R.err = kErrCorrupt | (kTileUnused << 8); return false; }
-  if (ev == kTokUnderrun) { R.err = kErrCorrupt | (kTileUnderrun << 8); return false; }
-  if (S.firstsi) {
-    const int first = static_cast<int>(S.firstsi) - 1;
-    const int32_t first_mag = static_cast<int32_t>(S.firstpm >> 8);
-    bool neg = (S.sumabs & 1u) != 0;
-    if (static_cast<int>(S.last) - first < kSDHMinSpan) {
-      const int sg = R.decode<kGSign, 16>(T.misc0, lane);
-      if (sg < 0) return false;
-      neg = sg != 0;
-    }
-    out.emit(coef_entry(static_cast<int>(S.firstpm & 0xFFu), blk, neg ? -first_mag : first_mag), lane);
-  }
-  return true;
-}
-
 // Grammar of one macroblock as decoded so far: what the record stores.
 struct MBState {
   uint32_t kind, part, cbp, tx, refsel, cmode, wmask, lmode0, lmode1;
@@ -731,8 +434,7 @@ struct MBState {
 // known; uncoded chroma planes once luma is complete (even when an earlier
 // coded chroma plane then fails); nothing when the failure comes before the
 // pattern.
-__device__ __forceinline__ bool parse_mb(Rans& R, CoefOut& out, MBState& m, const Tables& T, const FastTok& F,
-                                         uint32_t zz4, uint32_t zz8, uint32_t zz16, bool intra, uint32_t pred_last,
+__device__ __forceinline__ bool parse_mb(Rans& R, CoefOut& out, MBState& m, const Tables& T, uint32_t zz4, uint32_t zz8, uint32_t zz16, bool intra, uint32_t pred_last,
                                          int lane) {
   bool luma_coded = true, cb_coded = true, cr_coded = true;
   if (!intra) {
@@ -812,12 +514,7 @@ __device__ __forceinline__ bool parse_mb(Rans& R, CoefOut& out, MBState& m, cons
     const uint32_t va = chroma ? ca : la, vb = chroma ? cb : lb, vc = chroma ? cc : lc;
     const uint32_t zb = chroma ? zz8 : zzv;
     const int tb = chroma ? 64 : total;
-    // The token loop needs normalised states (a symbol then takes at most
-    // two bytes) and a sign table; otherwise the generic decoder runs.
-    const bool fast = F.sign_used && min(min(R.x0, R.x1), min(R.x2, R.x3)) >= kRansLower;
-    if (!(fast ? parse_block_fast(R, out, T, F, va, vb, vc, zb, tb, blk, lane)
-               : parse_block(R, out, T, va, vb, vc, zb, tb, blk, lane)))
-      return false;
+    if (!parse_block(R, out, T, va, vb, vc, zb, tb, blk, lane)) return false;
     m.wmask |= 1u << blk;
   }
   return true;
@@ -845,12 +542,21 @@ enum MbEvent : uint32_t { kMbDone = 0, kMbUnused = 3, kMbUnderrun = 4 };
 struct MbLoop {
   // rANS, window, outputs (in/out)
   uint32_t x0, x1, x2, x3, pos, nsym, wbase, lim, cn, cbase;
-  uint32_t mbx, mby, done, nintra, nst, firstmb, leftmv;
-  uint32_t vwin, vbuf, vstg, vprow, vcrow;  // per lane
-  uint32_t prowa, crowa;                    // LDS MV rows (wide tiles)
+  uint32_t mbx, mby, done, nintra, leftmv;
+  uint32_t vwin, vbuf;    // per lane
+  uint32_t prowa, crowa;  // LDS MV rows: the row above, the current row
   // the failing macroblock (outputs)
   uint32_t r0, r1, wmask, cstart, mv0, mv1, mv2, mv3, mvcur, lm0, lm1;
 };
+
+// Cost model of one wavefront alone on a SIMD (gfx950, measured): an SALU
+// instruction issues every 4 cycles; after a VALU instruction that writes an
+// SGPR (compare to a mask, readlane) the next SALU instruction waits about
+// 20 cycles, which VALU instructions can fill but SALU ones cannot; a
+// conditional branch costs about 10 cycles not taken and 25 taken.  The loop
+// is laid out so that the common paths fall through, and the VALU work of a
+// step (symbol search, the speculative table read of symbol 0, the MV
+// predictor) is issued together ahead of one wait.
 
 // Register roles that share one operand (never live together).
 #define MB_MAG "%[val]"
@@ -905,12 +611,11 @@ struct MbLoop {
   "v_cndmask_b32_e32 %[vwin], 0, %[vwin], vcc\n" \
   "s_branch L%=_" tag "_by\n"
 
-// Symbol of the context at lanes [goff, goff + 16) of vec into dst.
+// Symbol of the context at lanes [goff, goff + 16) of vec into dst.  The
+// compare takes the entries' start (upper half) against the slot.
 #define MB_DEC(vec, goff, gm1, bfe, dst, tag, unused) \
   "s_and_b32 %[slot], %[x0], 0xfff\n"            \
-  "s_lshl_b32 %[t1], %[slot], 13\n"              \
-  "s_or_b32 %[t1], %[t1], 0x1fff\n"              \
-  "v_cmp_ge_u32_e64 %[b], %[t1], " vec "\n"      \
+  "v_cmp_ge_u32_sdwa %[b], %[slot], " vec " src0_sel:DWORD src1_sel:WORD_1\n" \
   "s_lshr_b32 %[xn], %[x0], 12\n"                \
   "s_bfe_u64 %[b], %[b], " bfe "\n"              \
   "s_bcnt1_i32_b64 " dst ", %[b]\n"              \
@@ -919,8 +624,8 @@ struct MbLoop {
   "s_add_u32 %[t1], " dst ", " gm1 "\n"          \
   "v_readlane_b32 %[p], " vec ", %[t1]\n"        \
   "s_add_u32 " dst ", " dst ", -1\n"             \
-  "s_and_b32 %[t2], %[p], 0x1fff\n"              \
-  "s_lshr_b32 %[p], %[p], 13\n"                  \
+  "s_and_b32 %[t2], %[p], 0xffff\n"              \
+  "s_lshr_b32 %[p], %[p], 16\n"                  \
   "s_mul_i32 %[xn], %[t2], %[xn]\n"              \
   "s_sub_u32 %[t2], %[slot], %[p]\n"             \
   "s_add_u32 %[xn], %[xn], %[t2]\n"              \
@@ -992,9 +697,14 @@ struct MbLoop {
 // ---- token loop specialised by rANS phase ----------------------------------
 // Four copies of the hot token loop, one per interleave phase P: the next
 // symbol uses state register X[P] (x0..x3), which the decode updates in
-// place, so a symbol costs no register rotation.  Leaving the loop (escape,
-// end of block) rotates the registers back to phase 0, where the rest of
-// the loop keeps x0 as the next symbol's state.
+// place, so a symbol costs no register rotation.  A zero token (the common
+// case) falls through into the next phase's copy; other tokens leave the
+// straight line.  Leaving the loop (escape, end of block) rotates the
+// registers back to phase 0, where the rest of the loop keeps x0 as the next
+// symbol's state.
+//   off  = lane of the token context's symbol 0 (band group + previous
+//          class: 0, 8, 16); offz the same with previous class 0;
+//   bf   = the field of the context in the ballot (width 8 at off), and bfz.
 #define MBP_RN(tag, R)                           \
   "s_cmp_lt_u32 " R ", 0x800000\n"               \
   "s_cbranch_scc1 L%=_" tag "_rn\n"              \
@@ -1052,50 +762,51 @@ struct MbLoop {
   "s_sub_u32 %[t2], %[slot], %[p]\n"             \
   "s_add_u32 " R ", " R ", %[t2]\n"              \
   MBP_RN(tag, R)
-// The token loop at phase P (state RA) in state F: "a" before the block's
-// first level (whose sign resolves at the end of the block), "b" after it.
-// RB is the next phase's state (the sign's); NP1, NP2: the phases after one
-// and two symbols.  A zero token is symbol 0, whose table entry is its
-// frequency alone (start 0).
-#define MBP_TOK(P, NP1, NP2, RA, RB, F)          \
-  "L%=_tok" P #F ":\n"                            \
+// The token at phase P (state RA) in state F: "a" before the block's first
+// level (whose sign resolves at the end of the block), "b" after it.  The
+// straight line is the zero token: the compare and the table entry of
+// symbol 0 (start 0, so the entry is its frequency) are read together; it
+// falls through into phase NP1.
+#define MBP_TOK(P, NP1, RA, F)                   \
+  "L%=_tok" P #F ":\n"                           \
   "s_and_b32 %[slot], " RA ", 0xfff\n"           \
-  "s_lshl_b32 %[t1], %[slot], 13\n"              \
-  "s_or_b32 %[t1], %[t1], 0x1fff\n"              \
-  "v_cmp_ge_u32_e64 %[b], %[t1], %[vcur]\n"      \
-  "s_add_u32 %[t2], %[boff], %[prev]\n"          \
-  "s_or_b32 %[t1], %[t2], 0x80000\n"             \
+  "v_cmp_ge_u32_sdwa %[b], %[slot], %[vcur] src0_sel:DWORD src1_sel:WORD_1\n" \
+  "v_readlane_b32 %[p], %[vcur], %[off]\n"       \
   "s_lshr_b32 " RA ", " RA ", 12\n"              \
-  "s_bfe_u64 %[b], %[b], %[t1]\n"                \
-  "s_bcnt1_i32_b64 %[cnt], %[b]\n"               \
-  "s_cmp_eq_u32 %[cnt], 1\n"                     \
-  "s_cbranch_scc0 L%=_tnz" P #F "\n"              \
-  "v_readlane_b32 %[p], %[vcur], %[t2]\n"        \
-  "s_mov_b32 %[prev], 0\n"                       \
+  "s_bfe_u64 %[b], %[b], %[bf]\n"                \
+  "s_cmp_eq_u64 %[b], 1\n"                       \
+  "s_cbranch_scc0 L%=_tnz" P #F "\n"             \
   "s_mul_i32 " RA ", %[p], " RA "\n"             \
   "s_add_u32 " RA ", " RA ", %[slot]\n"          \
-  MBP_RN_NC("tz" P #F, RA)                        \
+  MBP_RN_NC("tz" P #F, RA)                       \
+  "s_mov_b32 %[off], %[offz]\n"                  \
+  "s_mov_b32 %[bf], %[bfz]\n"                    \
   "s_add_u32 %[si], %[si], 1\n"                  \
-  "s_cmp_lg_u32 %[si], %[stop]\n"                \
-  "s_cbranch_scc1 L%=_tok" NP1 #F "\n"            \
-  "s_branch L%=_tstop" NP1 #F "\n"                \
-  "L%=_tnz" P #F ":\n"                            \
+  "s_cmp_eq_u32 %[si], %[stop]\n"                \
+  "s_cbranch_scc1 L%=_tstop" NP1 #F "\n"
+// Any other token at phase P (out of line).  RB is the next phase's state
+// (the sign's); NP1, NP2: the phases after one and two symbols.
+#define MBP_NZ(P, NP1, NP2, RA, RB, F)           \
+  "L%=_tnz" P #F ":\n"                           \
+  "s_bcnt1_i32_b64 %[cnt], %[b]\n"               \
   "s_cmp_eq_u32 %[cnt], 0\n"                     \
   "s_cbranch_scc1 L%=_unused\n"                  \
-  "s_add_u32 %[t1], %[t2], %[cnt]\n"             \
+  "s_add_u32 %[t1], %[off], %[cnt]\n"            \
   "s_add_u32 %[t1], %[t1], -1\n"                 \
   "v_readlane_b32 %[p], %[vcur], %[t1]\n"        \
   "s_add_u32 " MB_MAG ", %[cnt], -1\n"           \
-  "s_and_b32 %[t2], %[p], 0x1fff\n"              \
-  "s_lshr_b32 %[p], %[p], 13\n"                  \
+  "s_and_b32 %[t2], %[p], 0xffff\n"              \
+  "s_lshr_b32 %[p], %[p], 16\n"                  \
   "s_mul_i32 " RA ", %[t2], " RA "\n"            \
   "s_sub_u32 %[t2], %[slot], %[p]\n"             \
   "s_add_u32 " RA ", " RA ", %[t2]\n"            \
-  MBP_RN_NC("tn" P #F, RA)                        \
+  MBP_RN_NC("tn" P #F, RA)                       \
   "s_cmp_ge_u32 %[cnt], 6\n"                     \
   "s_cbranch_scc1 L%=_nesc" NP1 "\n"             \
   "s_cmp_eq_u32 " MB_MAG ", 1\n"                 \
-  "s_cselect_b32 %[prev], 8, 16\n"               \
+  "s_cselect_b32 %[t1], 8, 16\n"                 \
+  "s_add_u32 %[off], %[offz], %[t1]\n"           \
+  "s_add_u32 %[bf], %[bfz], %[t1]\n"             \
   "s_add_u32 %[sumabs], %[sumabs], " MB_MAG "\n" \
   MBP_LEVEL_##F(P, NP1, NP2, RB)
 // A level in state a: the block's first, its sign deferred; state b follows.
@@ -1133,12 +844,16 @@ struct MbLoop {
   MBP_FIRST("0")
 // si reached stop at phase P in state F: the end of the block (back to phase
 // 0, then the end-of-block code) or a band boundary (bandOf thresholds 1, 3,
-// 9, 21, model.go:165-180).
+// 9, 21, model.go:165-180): the band's lane group alternates between offsets
+// 0 and 24 and its vector advances every second band; the previous class
+// carries over.
 #define MBP_TSTOP(P, F)                          \
   "L%=_tstop" P F ":\n"                          \
   "s_cmp_eq_u32 %[si], %[total]\n"               \
   "s_cbranch_scc1 L%=_nend" P "\n"               \
-  "s_xor_b32 %[boff], %[boff], 24\n"             \
+  "s_sub_u32 %[t1], %[off], %[offz]\n"           \
+  "s_xor_b32 %[offz], %[offz], 24\n"             \
+  "s_xor_b32 %[bfz], %[bfz], 24\n"               \
   "s_cmp_eq_u32 %[si], 1\n"                      \
   "s_cselect_b32 %[stop], 3, %[stop]\n"          \
   "s_cmp_eq_u32 %[si], 9\n"                      \
@@ -1154,6 +869,8 @@ struct MbLoop {
   "s_mov_b32 %[stop], %[total]\n"                \
   "L%=_tb4" P F ":\n"                            \
   "s_min_u32 %[stop], %[stop], %[total]\n"       \
+  "s_add_u32 %[off], %[offz], %[t1]\n"           \
+  "s_add_u32 %[bf], %[bfz], %[t1]\n"             \
   "s_branch L%=_tok" P F "\n"
 // Back to phase 0 from phase 1, 2, 3 (t1 as the spare register).
 #define MBP_NORM1                                \
@@ -1176,14 +893,24 @@ struct MbLoop {
   "s_mov_b32 %[x1], %[x0]\n"                     \
   "s_mov_b32 %[x0], %[t1]\n"
 #define MBP_TOKLOOP                              \
-  MBP_TOK("0", "1", "2", "%[x0]", "%[x1]", a)    \
-  MBP_TOK("1", "2", "3", "%[x1]", "%[x2]", a)    \
-  MBP_TOK("2", "3", "0", "%[x2]", "%[x3]", a)    \
-  MBP_TOK("3", "0", "1", "%[x3]", "%[x0]", a)    \
-  MBP_TOK("0", "1", "2", "%[x0]", "%[x1]", b)    \
-  MBP_TOK("1", "2", "3", "%[x1]", "%[x2]", b)    \
-  MBP_TOK("2", "3", "0", "%[x2]", "%[x3]", b)    \
-  MBP_TOK("3", "0", "1", "%[x3]", "%[x0]", b)    \
+  MBP_TOK("0", "1", "%[x0]", a)                  \
+  MBP_TOK("1", "2", "%[x1]", a)                  \
+  MBP_TOK("2", "3", "%[x2]", a)                  \
+  MBP_TOK("3", "0", "%[x3]", a)                  \
+  "s_branch L%=_tok0a\n"                         \
+  MBP_TOK("0", "1", "%[x0]", b)                  \
+  MBP_TOK("1", "2", "%[x1]", b)                  \
+  MBP_TOK("2", "3", "%[x2]", b)                  \
+  MBP_TOK("3", "0", "%[x3]", b)                  \
+  "s_branch L%=_tok0b\n"                         \
+  MBP_NZ("0", "1", "2", "%[x0]", "%[x1]", a)     \
+  MBP_NZ("1", "2", "3", "%[x1]", "%[x2]", a)     \
+  MBP_NZ("2", "3", "0", "%[x2]", "%[x3]", a)     \
+  MBP_NZ("3", "0", "1", "%[x3]", "%[x0]", a)     \
+  MBP_NZ("0", "1", "2", "%[x0]", "%[x1]", b)     \
+  MBP_NZ("1", "2", "3", "%[x1]", "%[x2]", b)     \
+  MBP_NZ("2", "3", "0", "%[x2]", "%[x3]", b)     \
+  MBP_NZ("3", "0", "1", "%[x3]", "%[x0]", b)     \
   MBP_TSTOP("0", "a") MBP_TSTOP("1", "a") MBP_TSTOP("2", "a") MBP_TSTOP("3", "a") \
   MBP_TSTOP("0", "b") MBP_TSTOP("1", "b") MBP_TSTOP("2", "b") MBP_TSTOP("3", "b") \
   MBP_ESCLEVEL                                   \
@@ -1210,87 +937,234 @@ struct MbLoop {
   MBP_RN_OOL("ts3", "underrun", "%[x0]") MB_EMIT_OOL("e13")                       \
   MBP_RN_OOL("tsg", "underrun", "%[x0]") MB_EMIT_OOL("e1g")
 
-// Record dword k (value v) of the staged macroblock nst.
-#define MB_REC(k, v)                             \
-  "s_add_u32 m0, %[t2], " #k "\n"                \
-  "v_writelane_b32 %[vstg], " v ", m0\n"
+// A coded macroblock's fields before its grammar: intra kind, nothing
+// written, zero vectors and modes, every plane coded.
+#define MB_INIT                                  \
+  "s_mov_b32 %[r0], 2\n"                         \
+  "s_mov_b32 %[r1], 0\n"                         \
+  "s_mov_b32 %[wmask], 0\n"                      \
+  "s_mov_b32 %[mv0], 0\n"                        \
+  "s_mov_b32 %[mv1], 0\n"                        \
+  "s_mov_b32 %[mv2], 0\n"                        \
+  "s_mov_b32 %[mv3], 0\n"                        \
+  "s_mov_b32 %[mvcur], 0\n"                      \
+  "s_mov_b32 %[lm0], 0\n"                        \
+  "s_mov_b32 %[lm1], 0\n"                        \
+  "s_mov_b32 %[cbpf], 7\n"
 
-// MV rows of the tile (mv.go:21-52 reads left, above, above-right or
-// above-left): for tiles at most 64 macroblocks wide in two VGPRs (lane =
-// tile column), otherwise in LDS at byte addresses prowa / crowa.
-#define MB_ROW_RD_VGPR(dst, idx) "v_readlane_b32 " dst ", %[vprow], " idx "\n"
-#define MB_ROW_WR_VGPR                           \
-  "s_mov_b32 m0, %[t1]\n"                        \
-  "v_writelane_b32 %[vcrow], %[mvcur], m0\n"
-#define MB_ROW_SWAP_VGPR "v_mov_b32 %[vprow], %[vcrow]\n"
-#define MB_ROW_RD_LDS(dst, idx)                  \
-  "s_lshl_b32 %[k2], " idx ", 2\n"               \
-  "s_add_u32 %[k2], %[k2], %[prowa]\n"           \
-  "v_mov_b32 %[vt], %[k2]\n"                     \
-  "ds_read_b32 %[vt], %[vt]\n"                   \
-  "s_waitcnt lgkmcnt(0)\n"                       \
-  "v_readfirstlane_b32 " dst ", %[vt]\n"
-#define MB_ROW_WR_LDS                            \
-  "s_lshl_b32 %[t1], %[t1], 2\n"                 \
-  "s_add_u32 %[t1], %[t1], %[crowa]\n"           \
-  "v_mov_b32 %[vaddr], %[t1]\n"                  \
-  "v_mov_b32 %[vt], %[mvcur]\n"                  \
-  "ds_write_b32 %[vaddr], %[vt]\n"
-#define MB_ROW_SWAP_LDS                          \
-  "s_mov_b32 %[t1], %[prowa]\n"                  \
-  "s_mov_b32 %[prowa], %[crowa]\n"               \
-  "s_mov_b32 %[crowa], %[t1]\n"
+// The records of a page of macroblocks are staged transposed: dword k of the
+// macroblock in lane l sits in lane l of vector vr<k>.  A page starts as the
+// record of a skip macroblock (kind 0, every block written, no coefficients,
+// zero vectors and modes), so a skip macroblock writes only its coefficient
+// offset and vector.
+#define MB_REC_TEMPLATE                          \
+  "v_mov_b32 %[vr0], 0\n"                        \
+  "v_mov_b32 %[vr1], 0\n"                        \
+  "v_mov_b32 %[vr2], 0x3ffff\n"                  \
+  "v_mov_b32 %[vr4], 0\n"                        \
+  "v_mov_b32 %[vr6], 0\n"                        \
+  "v_mov_b32 %[vr7], 0\n"                        \
+  "v_mov_b32 %[vr8], 0\n"                        \
+  "v_mov_b32 %[vr9], 0\n"                        \
+  "v_mov_b32 %[vr10], 0\n"                       \
+  "v_mov_b32 %[vr11], 0\n"                       \
+  "v_mov_b32 %[vmv], 0\n"
 
-template <bool kWide>
-__device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t zz4, uint32_t zz8, uint32_t zz16,
-                                            uint32_t vlane4, uint32_t x0t, uint32_t x1t, uint32_t y0t, uint32_t y1t,
-                                            uint32_t mbc, uint32_t interf, uint32_t tcb, uint32_t f0, uint32_t f1,
-                                            uint32_t hi2, uint32_t wlo, const uint8_t* tb, uint32_t* dst,
-                                            MBRec* recs) {
+// decodeTile (decode_intra.go:121-135) as one asm statement: macroblock
+// modes, MV prediction (mv.go:21-52), the inter header (decode_inter.go:18-62:
+// reference, partition, MV class values with raw bits, coded-block pattern),
+// transform size and intra modes (decode_intra.go:137-167), every token of
+// every block with escapes, explicit and hidden signs (residual.go:79-126),
+// coefficient entries, macroblock records and the MV rows.  All decoder state
+// is uniform and lives in SGPRs; vector registers hold the symbol tables, the
+// byte window, the coefficient staging register (64 entries), the staged
+// records of a page and the MV rows.
+//
+// Macroblocks are walked in pages: up to 64 consecutive columns of a row, one
+// lane each.  Per page the MVs of the row above (LDS) are loaded into two
+// vectors, B (above) and C (above-right, or above-left at the tile's right
+// edge, or above itself in a one-column tile); in the tile's first row B and
+// C are the int16 maximum and minimum.  predictMV is then the packed int16
+// median of (left, B, C) in lane `column - page base` for every macroblock:
+// zero for a missing left, the left vector alone in the first row, the above
+// vector alone in a one-column tile (mv.go:21-52 case by case).  A page's
+// records and MVs go out when it ends.
+//
+// It returns at the end of the tile (kMbDone) or at a decode error
+// (kMbUnused, kMbUnderrun) with the failing macroblock's fields decoded so
+// far, which the caller turns into its record exactly as the generic path
+// does; the records of the page before it are already stored.
+// Precondition: the sign context has a table (signs from two SGPRs).
+__device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t vlane4, uint32_t x0t, uint32_t x1t,
+                                            uint32_t y0t, uint32_t y1t, uint32_t mbc, uint32_t interf, uint32_t tcb,
+                                            uint32_t f0, uint32_t f1, uint32_t hi2, uint32_t wlo, const uint8_t* tb,
+                                            uint32_t* dst, MBRec* recs) {
   uint32_t ev, slot, t1, t2, t3, p, xn, cnt, sym, val, k2;
-  uint32_t k, nb, total, blk8, cbpf, si, stop, boff, prev, sumabs, firstsi, firstpm, last, m0s;
+  uint32_t k, nb, total, blk8, cbpf, si, stop, off, offz, bf, bfz, sumabs, firstsi, firstpm, last, m0s, ln, pend;
   uint64_t b;
   uint32_t vcur, va, vb, vc, vaddr, vt;
-  if constexpr (!kWide) {
-#define MB_ROW_RD MB_ROW_RD_VGPR
-#define MB_ROW_WR MB_ROW_WR_VGPR
-#define MB_ROW_SWAP MB_ROW_SWAP_VGPR
+  uint32_t vr0, vr1, vr2, vr3, vr4, vr5, vr6, vr7, vr8, vr9, vr10, vr11, vmv, vB, vC;
   asm volatile(
       "s_nop 4\n"
       "s_mov_b32 %[m0s], m0\n"
+      // the tile's first row: no row above
+      "v_mov_b32 %[vB], 0x7fff7fff\n"
+      "v_mov_b32 %[vC], 0x80008000\n"
+      "s_mov_b32 %[leftmv], 0\n"
+
+      // ------------------------------------------------------------- page
+      "L%=_page:\n"
+      "s_add_u32 %[pend], %[mbx], 64\n"
+      "s_min_u32 %[pend], %[pend], %[x1t]\n"
+      "s_mov_b32 %[ln], 0\n"
+      MB_REC_TEMPLATE
 
       // ---------------------------------------------------------------- MB
       "L%=_mb:\n"
       "s_add_u32 %[cstart], %[cbase], %[cn]\n"
-      "s_mov_b32 %[r0], 2\n"
-      "s_mov_b32 %[r1], 0\n"
-      "s_mov_b32 %[wmask], 0\n"
-      "s_mov_b32 %[mv0], 0\n"
-      "s_mov_b32 %[mv1], 0\n"
-      "s_mov_b32 %[mv2], 0\n"
-      "s_mov_b32 %[mv3], 0\n"
-      "s_mov_b32 %[mvcur], 0\n"
-      "s_mov_b32 %[lm0], 0\n"
-      "s_mov_b32 %[lm1], 0\n"
-      "s_mov_b32 %[cbpf], 7\n"
       "s_cmp_eq_u32 %[interf], 0\n"
-      "s_cbranch_scc1 L%=_tx\n"
-      // macroblock mode (decode_inter.go:9-16)
-      MB_DEC("%[vm0]", "16", "15", "0x100010", "%[sym]", "md", "mode_unused")
+      "s_cbranch_scc1 L%=_kmb\n"
+      // inter frame: the mode (decode_inter.go:9-16) and the MV predictor in
+      // one VALU group; symbol 0 (skip) is the straight line
+      "s_and_b32 %[slot], %[x0], 0xfff\n"
+      "v_cmp_ge_u32_sdwa %[b], %[slot], %[vm0] src0_sel:DWORD src1_sel:WORD_1\n"
+      "v_readlane_b32 %[p], %[vm0], 16\n"
+      "v_pk_min_i16 %[vaddr], %[leftmv], %[vB]\n"
+      "v_pk_max_i16 %[vt], %[leftmv], %[vB]\n"
+      "v_pk_min_i16 %[vt], %[vt], %[vC]\n"
+      "v_pk_max_i16 %[vaddr], %[vaddr], %[vt]\n"
+      "v_readlane_b32 " MB_PRED ", %[vaddr], %[ln]\n"
+      "s_lshr_b32 %[xn], %[x0], 12\n"
+      "s_bfe_u64 %[b], %[b], 0x100010\n"
+      "s_cmp_eq_u64 %[b], 1\n"
+      "s_cbranch_scc0 L%=_mdnz\n"
+      "s_mul_i32 %[xn], %[p], %[xn]\n"
+      "s_add_u32 %[xn], %[xn], %[slot]\n"
+      MB_RN("md")
+      // skip macroblock: the predicted vector, every block written
+      "s_mov_b32 m0, %[ln]\n"
+      "s_add_u32 %[t1], %[tcb], %[cstart]\n"
+      "v_writelane_b32 %[vr3], %[t1], m0\n"
+      "v_writelane_b32 %[vr5], " MB_PRED ", m0\n"
+      "v_writelane_b32 %[vmv], " MB_PRED ", m0\n"
+      "s_mov_b32 %[leftmv], " MB_PRED "\n"
+      "L%=_mbnext:\n"
+      "s_add_u32 %[done], %[done], 1\n"
+      "s_add_u32 %[mbx], %[mbx], 1\n"
+      "s_add_u32 %[ln], %[ln], 1\n"
+      "s_cmp_lt_u32 %[mbx], %[pend]\n"
+      "s_cbranch_scc1 L%=_mb\n"
+      // end of a page: its records and MVs out
+      "s_mov_b32 %[t3], 0\n"
+      "s_branch L%=_flush\n"
+      "L%=_pagedone:\n"
+      "s_cmp_lt_u32 %[mbx], %[x1t]\n"
+      "s_cbranch_scc1 L%=_abv\n"
+      // end of a row: the rows swap
+      "s_mov_b32 %[t1], %[prowa]\n"
+      "s_mov_b32 %[prowa], %[crowa]\n"
+      "s_mov_b32 %[crowa], %[t1]\n"
+      "s_mov_b32 %[mbx], %[x0t]\n"
+      "s_mov_b32 %[leftmv], 0\n"
+      "s_add_u32 %[mby], %[mby], 1\n"
+      "s_cmp_lt_u32 %[mby], %[y1t]\n"
+      "s_cbranch_scc0 L%=_tiledone\n"
+      // the row above at the page's columns: B = above, C = above-right, or
+      // above-left past the tile's last column (above itself when the tile
+      // is one column wide)
+      "L%=_abv:\n"
+      "s_cmp_eq_u32 %[mby], %[y0t]\n"
+      "s_cbranch_scc1 L%=_page\n"
+      "s_sub_u32 %[t1], %[mbx], %[x0t]\n"
+      "s_lshl_b32 %[t1], %[t1], 2\n"
+      "s_add_u32 %[t1], %[t1], %[prowa]\n"
+      "s_sub_u32 %[t2], %[x1t], %[mbx]\n"
+      "s_lshl_b32 %[t2], %[t2], 2\n"
+      "s_add_u32 %[t2], %[t2], -4\n"
+      "v_min_u32 %[vaddr], %[t2], %[vlane4]\n"
+      "v_add_u32 %[vt], 4, %[vaddr]\n"
+      "v_cmp_ge_u32_e32 vcc, %[t2], %[vt]\n"
+      "v_add_u32 %[vaddr], %[t1], %[vaddr]\n"
+      "v_add_u32 %[vt], %[t1], %[vt]\n"
+      "v_add_u32 %[vr0], -4, %[vaddr]\n"
+      "v_max_u32 %[vr0], %[prowa], %[vr0]\n"
+      "v_cndmask_b32_e32 %[vt], %[vr0], %[vt], vcc\n"
+      "ds_read_b32 %[vB], %[vaddr]\n"
+      "ds_read_b32 %[vC], %[vt]\n"
+      "s_waitcnt lgkmcnt(0)\n"
+      "s_branch L%=_page\n"
+
+      // ---- the staged records of the page (lanes [0, ln)) to recs; then
+      // (t3 = 0) the page's MVs into the current row, or (t3 = 1) the exit
+      "L%=_flush:\n"
+      "s_cmp_eq_u32 %[ln], 0\n"
+      "s_cbranch_scc1 L%=_flushed\n"
+      "s_bfm_b64 %[b], %[ln], 0\n"
+      "s_cmp_eq_u32 %[ln], 64\n"
+      "s_cselect_b64 %[b], -1, %[b]\n"
+      "s_sub_u32 %[t1], %[mbx], %[ln]\n"
+      "s_mul_i32 %[t2], %[mby], %[mbc]\n"
+      "s_add_u32 %[t2], %[t2], %[t1]\n"
+      "s_mul_i32 %[t2], %[t2], 48\n"
+      "v_mul_u32_u24 %[vaddr], 12, %[vlane4]\n"
+      "v_add_u32 %[vaddr], %[t2], %[vaddr]\n"
+      "s_mov_b64 exec, %[b]\n"
+      "global_store_dword %[vaddr], %[vr0], %[recs]\n"
+      "global_store_dword %[vaddr], %[vr1], %[recs] offset:4\n"
+      "global_store_dword %[vaddr], %[vr2], %[recs] offset:8\n"
+      "global_store_dword %[vaddr], %[vr3], %[recs] offset:12\n"
+      "global_store_dword %[vaddr], %[vr4], %[recs] offset:16\n"
+      "global_store_dword %[vaddr], %[vr5], %[recs] offset:20\n"
+      "global_store_dword %[vaddr], %[vr6], %[recs] offset:24\n"
+      "global_store_dword %[vaddr], %[vr7], %[recs] offset:28\n"
+      "global_store_dword %[vaddr], %[vr8], %[recs] offset:32\n"
+      "global_store_dword %[vaddr], %[vr9], %[recs] offset:36\n"
+      "global_store_dword %[vaddr], %[vr10], %[recs] offset:40\n"
+      "global_store_dword %[vaddr], %[vr11], %[recs] offset:44\n"
+      "s_cmp_eq_u32 %[t3], 0\n"
+      "s_cbranch_scc0 L%=_flushx\n"
+      "s_sub_u32 %[t1], %[t1], %[x0t]\n"
+      "s_lshl_b32 %[t1], %[t1], 2\n"
+      "s_add_u32 %[t1], %[t1], %[crowa]\n"
+      "v_add_u32 %[vt], %[t1], %[vlane4]\n"
+      "ds_write_b32 %[vt], %[vmv]\n"
+      "s_mov_b64 exec, -1\n"
+      "s_branch L%=_pagedone\n"
+      "L%=_flushx:\n"
+      "s_mov_b64 exec, -1\n"
+      "s_branch L%=_exit\n"
+      "L%=_flushed:\n"
+      "s_cmp_eq_u32 %[t3], 0\n"
+      "s_cbranch_scc1 L%=_pagedone\n"
+      "s_branch L%=_exit\n"
+      "L%=_tiledone:\n"
+      "s_mov_b32 %[ev], 0\n"
+      "s_branch L%=_exit\n"
+
+      // ---- a coded macroblock of an inter frame: the mode's other symbols
+      "L%=_mdnz:\n"
+      "s_bcnt1_i32_b64 %[sym], %[b]\n"
       "s_cmp_eq_u32 %[sym], 0\n"
-      "s_cbranch_scc1 L%=_skip\n"
+      "s_cbranch_scc1 L%=_mode_unused\n"
+      "s_add_u32 %[t1], %[sym], 15\n"
+      "v_readlane_b32 %[p], %[vm0], %[t1]\n"
+      "s_add_u32 %[sym], %[sym], -1\n"
+      "s_and_b32 %[t2], %[p], 0xffff\n"
+      "s_lshr_b32 %[p], %[p], 16\n"
+      "s_mul_i32 %[xn], %[t2], %[xn]\n"
+      "s_sub_u32 %[t2], %[slot], %[p]\n"
+      "s_add_u32 %[xn], %[xn], %[t2]\n"
+      MB_RN("md2")
+      MB_INIT
       "s_cmp_eq_u32 %[sym], 2\n"
       "s_cbranch_scc1 L%=_tx\n"
       // ---- inter macroblock: reference, partition, vectors, pattern
       "s_mov_b32 %[r0], 1\n"
       MB_DEC("%[vm1]", "32", "31", "0x100020", "%[sym]", "rf", "unused")
       "s_mov_b32 %[r1], %[sym]\n"
-      "s_mov_b32 " MB_PRED ", 0\n"
+      // golden: the vectors are coded against zero
       "s_cmp_eq_u32 %[sym], 1\n"
-      "s_cbranch_scc1 L%=_nopred\n"
-      "s_mov_b32 %[t3], 1\n"
-      "s_branch L%=_mvp\n"
-      "L%=_nopred:\n"
+      "s_cselect_b32 " MB_PRED ", 0, " MB_PRED "\n"
       MB_DEC("%[vm1]", "48", "47", "0x100030", "%[sym]", "pt", "unused")
       "s_lshl_b32 %[t1], %[sym], 8\n"
       "s_or_b32 %[r0], %[r0], %[t1]\n"
@@ -1345,434 +1219,9 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       "s_mov_b32 %[wmask], 0xffff\n"
       "s_branch L%=_blocks\n"
 
-      // ---- transform size; the luma blocks' vectors and scan
-      "L%=_tx:\n"
-      MB_DEC("%[vm0]", "32", "31", "0x100020", "%[sym]", "tx", "unused")
-      "s_lshl_b32 %[t1], %[sym], 24\n"
-      "s_or_b32 %[r0], %[r0], %[t1]\n"
-      "s_cmp_eq_u32 %[sym], 0\n"
-      "s_cbranch_scc0 L%=_tx1\n"
-      "s_mov_b32 %[nb], 16\n"
-      "s_mov_b32 %[total], 16\n"
-      "v_mov_b32 %[va], %[t00]\n"
-      "v_mov_b32 %[vb], %[t01]\n"
-      "v_mov_b32 %[vc], %[t02]\n"
-      "s_branch L%=_blocks\n"
-      "L%=_tx1:\n"
-      "s_cmp_eq_u32 %[sym], 1\n"
-      "s_cbranch_scc0 L%=_tx2\n"
-      "s_mov_b32 %[nb], 4\n"
-      "s_mov_b32 %[total], 64\n"
-      "v_mov_b32 %[va], %[t10]\n"
-      "v_mov_b32 %[vb], %[t11]\n"
-      "v_mov_b32 %[vc], %[t12]\n"
-      "s_branch L%=_blocks\n"
-      "L%=_tx2:\n"
-      "s_mov_b32 %[nb], 1\n"
-      "s_movk_i32 %[total], 0x100\n"
-      "v_mov_b32 %[va], %[t20]\n"
-      "v_mov_b32 %[vb], %[t21]\n"
-      "v_mov_b32 %[vc], %[t22]\n"
-
-      // ---- blocks: luma 0..nb-1, then Cb, Cr (when coded)
-      "L%=_blocks:\n"
-      "s_mov_b32 %[k], 0\n"
-      "L%=_bk:\n"
-      "s_cmp_eq_u32 %[k], %[nb]\n"
-      "s_cbranch_scc0 L%=_bk_nl\n"
-      // luma complete: the chroma mode (intra) or the uncoded chroma planes (inter)
-      "s_and_b32 %[t3], %[r0], 0xff\n"
-      "s_cmp_eq_u32 %[t3], 2\n"
-      "s_cbranch_scc0 L%=_bk_ic\n"
-      MB_DEC("%[vm1]", "16", "15", "0x100010", "%[sym]", "cm", "unused")
-      "s_lshl_b32 %[t1], %[sym], 8\n"
-      "s_or_b32 %[r1], %[r1], %[t1]\n"
-      "s_branch L%=_bk_nl\n"
-      "L%=_bk_ic:\n"
-      "s_bitcmp1_b32 %[cbpf], 1\n"
-      "s_cbranch_scc1 L%=_bk_icr\n"
-      "s_bitset1_b32 %[wmask], 16\n"
-      "L%=_bk_icr:\n"
-      "s_bitcmp1_b32 %[cbpf], 2\n"
-      "s_cbranch_scc1 L%=_bk_nl\n"
-      "s_bitset1_b32 %[wmask], 17\n"
-      "L%=_bk_nl:\n"
-      "s_cmp_lt_u32 %[k], %[nb]\n"
-      "s_cbranch_scc0 L%=_bk_ch\n"
-      "s_lshl_b32 %[blk8], %[k], 8\n"
-      "s_and_b32 %[t3], %[r0], 0xff\n"
-      "s_cmp_eq_u32 %[t3], 2\n"
-      "s_cbranch_scc0 L%=_tb\n"
-      // intra luma mode of block k (4 bits in decode order)
-      MB_DEC("%[vm1]", "0", "-1", "0x100000", "%[sym]", "lm", "unused")
-      "s_and_b32 %[t1], %[k], 7\n"
-      "s_lshl_b32 %[t1], %[t1], 2\n"
-      "s_lshl_b32 %[t1], %[sym], %[t1]\n"
-      "s_cmp_lt_u32 %[k], 8\n"
-      "s_cbranch_scc0 L%=_lmh\n"
-      "s_or_b32 %[lm0], %[lm0], %[t1]\n"
-      "s_branch L%=_tb\n"
-      "L%=_lmh:\n"
-      "s_or_b32 %[lm1], %[lm1], %[t1]\n"
-      "s_branch L%=_tb\n"
-      "L%=_bk_ch:\n"
-      "s_sub_u32 %[t1], %[k], %[nb]\n"
-      "s_add_u32 %[t3], %[t1], 1\n"
-      "s_lshr_b32 %[t3], %[cbpf], %[t3]\n"
-      "s_bitcmp1_b32 %[t3], 0\n"
-      "s_cbranch_scc0 L%=_bk_next\n"
-      "s_add_u32 %[t1], %[t1], 16\n"
-      "s_lshl_b32 %[blk8], %[t1], 8\n"
-      "s_mov_b32 %[total], 64\n"
-      "v_mov_b32 %[va], %[t30]\n"
-      "v_mov_b32 %[vb], %[t31]\n"
-      "v_mov_b32 %[vc], %[t32]\n"
-
-      // ------------------------------------------------------ token block
-      "L%=_tb:\n"
-      "s_mov_b32 %[si], 0\n"
-      "s_mov_b32 %[stop], 1\n"
-      "s_mov_b32 %[boff], 0\n"
-      "s_mov_b32 %[prev], 0\n"
-      "s_mov_b32 %[sumabs], 0\n"
-      "s_mov_b32 %[firstsi], 0\n"
-      "v_mov_b32 %[vcur], %[va]\n"
-      MBP_TOKLOOP
-      // escape (magnitude 5 + getEscapeValue, mv.go:124-128) or end of block
-      "L%=_tescob:\n"
-      "s_cmp_eq_u32 %[cnt], 7\n"
-      "s_cbranch_scc1 L%=_teob\n"
-      MB_CLASS("%[vm2]", "0", "-1", "0x100000", "es")
-      "s_add_u32 " MB_MAG ", %[sym], 5\n"
-      "s_mov_b32 %[prev], 16\n"
-      "s_branch L%=_tlev0\n"
-      // end of block: the first level's sign, hidden in the level-sum parity
-      // when last - first >= 4, else explicit
-      "L%=_teob:\n"
-      "s_add_u32 %[nsym], %[nsym], 1\n"
-      "L%=_tend:\n"
-      "s_add_u32 %[nsym], %[nsym], %[si]\n"
-      "s_cmp_eq_u32 %[firstsi], 0\n"
-      "s_cbranch_scc1 L%=_tbdone\n"
-      "s_lshr_b32 " MB_MAG ", %[firstpm], 8\n"
-      "s_add_u32 %[t1], %[firstsi], -1\n"
-      "s_sub_u32 %[t1], %[last], %[t1]\n"
-      "s_cmp_lt_i32 %[t1], 4\n"
-      "s_cbranch_scc0 L%=_tpar\n"
-      MB_SIGN("sd")
-      "s_branch L%=_temit\n"
-      "L%=_tpar:\n"
-      "s_bitcmp1_b32 %[sumabs], 0\n"
-      "s_cbranch_scc0 L%=_temit\n"
-      "s_sub_u32 " MB_MAG ", 0, " MB_MAG "\n"
-      "L%=_temit:\n"
-      "s_and_b32 %[t2], %[firstpm], 0xff\n"
-      MB_EMIT("e2")
-      "L%=_tbdone:\n"
-      "s_lshr_b32 %[t1], %[blk8], 8\n"
-      "s_lshl_b32 %[t1], 1, %[t1]\n"
-      "s_or_b32 %[wmask], %[wmask], %[t1]\n"
-      "L%=_bk_next:\n"
-      "s_add_u32 %[k], %[k], 1\n"
-      "s_add_u32 %[t1], %[nb], 2\n"
-      "s_cmp_lt_u32 %[k], %[t1]\n"
-      "s_cbranch_scc1 L%=_bk\n"
-      "s_branch L%=_mbend\n"
-
-      // ---- skip macroblock: the predicted vector, every block written
-      "L%=_skip:\n"
-      "s_mov_b32 %[t3], 0\n"
-      "s_branch L%=_mvp\n"
-      "L%=_skipmv:\n"
-      "s_mov_b32 %[r0], 0\n"
-      "s_mov_b32 %[mv0], " MB_PRED "\n"
-      "s_mov_b32 %[mvcur], " MB_PRED "\n"
-      "s_mov_b32 %[wmask], 0x3ffff\n"
-      "s_branch L%=_mbend\n"
-
-      // ---- predictMV (mv.go:21-52) into pred; t3 = 0 skip, 1 inter, 2 error
-      "L%=_mvp:\n"
-      "s_sub_u32 %[t1], %[mbx], %[x0t]\n"
-      "s_cmp_gt_u32 %[mby], %[y0t]\n"
-      "s_cbranch_scc0 L%=_mvp_nob\n"
-      MB_ROW_RD("%[t2]", "%[t1]")
-      "s_add_u32 %[p], %[t1], 1\n"
-      "s_sub_u32 %[xn], %[x1t], %[x0t]\n"
-      "s_cmp_lt_u32 %[p], %[xn]\n"
-      "s_cbranch_scc0 L%=_mvp_cl\n"
-      MB_ROW_RD("%[xn]", "%[p]")
-      "s_branch L%=_mvp_med\n"
-      "L%=_mvp_cl:\n"
-      "s_cmp_eq_u32 %[t1], 0\n"
-      "s_cbranch_scc1 L%=_mvp_b\n"
-      "s_add_u32 %[p], %[t1], -1\n"
-      MB_ROW_RD("%[xn]", "%[p]")
-      "L%=_mvp_med:\n"
-      "s_cmp_eq_u32 %[t1], 0\n"
-      "s_cselect_b32 %[p], 0, %[leftmv]\n"
-      // component-wise median of a (p), b (t2), c (xn)
-      "s_sext_i32_i16 %[slot], %[p]\n"
-      "s_sext_i32_i16 %[cnt], %[t2]\n"
-      "s_sext_i32_i16 %[k2], %[xn]\n"
-      "s_min_i32 %[t1], %[slot], %[cnt]\n"
-      "s_max_i32 %[slot], %[slot], %[cnt]\n"
-      "s_min_i32 %[slot], %[slot], %[k2]\n"
-      "s_max_i32 %[t1], %[t1], %[slot]\n"
-      "s_ashr_i32 %[slot], %[p], 16\n"
-      "s_ashr_i32 %[cnt], %[t2], 16\n"
-      "s_ashr_i32 %[k2], %[xn], 16\n"
-      "s_min_i32 %[p], %[slot], %[cnt]\n"
-      "s_max_i32 %[slot], %[slot], %[cnt]\n"
-      "s_min_i32 %[slot], %[slot], %[k2]\n"
-      "s_max_i32 %[p], %[p], %[slot]\n"
-      "s_and_b32 %[t1], %[t1], 0xffff\n"
-      "s_lshl_b32 %[p], %[p], 16\n"
-      "s_or_b32 " MB_PRED ", %[t1], %[p]\n"
-      "s_branch L%=_mvp_end\n"
-      "L%=_mvp_b:\n"
-      "s_mov_b32 " MB_PRED ", %[t2]\n"
-      "s_branch L%=_mvp_end\n"
-      "L%=_mvp_nob:\n"
-      "s_cmp_eq_u32 %[t1], 0\n"
-      "s_cselect_b32 " MB_PRED ", 0, %[leftmv]\n"
-      "L%=_mvp_end:\n"
-      "s_cmp_eq_u32 %[t3], 0\n"
-      "s_cbranch_scc1 L%=_skipmv\n"
-      "s_cmp_eq_u32 %[t3], 1\n"
-      "s_cbranch_scc1 L%=_nopred\n"
-      "s_branch L%=_modeerr_mv\n"
-
-      // ---- end of a macroblock: MV row, record, next macroblock
-      "L%=_mbend:\n"
-      "s_and_b32 %[t3], %[r0], 0xff\n"
-      "s_cmp_eq_u32 %[t3], 2\n"
-      "s_cselect_b32 %[t3], 1, 0\n"
-      "s_add_u32 %[nintra], %[nintra], %[t3]\n"
-      "s_sub_u32 %[t1], %[mbx], %[x0t]\n"
-      MB_ROW_WR
-      "s_mov_b32 %[leftmv], %[mvcur]\n"
-      "s_cmp_eq_u32 %[nst], 0\n"
-      "s_cbranch_scc0 L%=_rec\n"
-      "s_mul_i32 %[firstmb], %[mby], %[mbc]\n"
-      "s_add_u32 %[firstmb], %[firstmb], %[mbx]\n"
-      "L%=_rec:\n"
-      "s_mul_i32 %[t2], %[nst], 12\n"
-      "s_add_u32 %[t1], %[tcb], %[cstart]\n"
-      "s_add_u32 %[t3], %[cbase], %[cn]\n"
-      "s_sub_u32 %[t3], %[t3], %[cstart]\n"
-      MB_REC(0, "%[r0]")
-      MB_REC(1, "%[r1]")
-      MB_REC(2, "%[wmask]")
-      MB_REC(3, "%[t1]")
-      MB_REC(4, "%[t3]")
-      MB_REC(5, "%[mv0]")
-      MB_REC(6, "%[mv1]")
-      MB_REC(7, "%[mv2]")
-      MB_REC(8, "%[mv3]")
-      MB_REC(9, "%[lm0]")
-      MB_REC(10, "%[lm1]")
-      MB_REC(11, "0")
-      "s_add_u32 %[done], %[done], 1\n"
-      "s_add_u32 %[nst], %[nst], 1\n"
-      "s_cmp_eq_u32 %[nst], 5\n"
-      "s_cbranch_scc0 L%=_next\n"
-      "s_mov_b32 %[t3], 0\n"
-      "s_branch L%=_rflush\n"
-      "L%=_next:\n"
-      "s_add_u32 %[mbx], %[mbx], 1\n"
-      "s_cmp_lt_u32 %[mbx], %[x1t]\n"
-      "s_cbranch_scc1 L%=_mb\n"
-      // end of a row: its records out, the rows swap
-      "s_mov_b32 %[t3], 1\n"
-      "s_cmp_eq_u32 %[nst], 0\n"
-      "s_cbranch_scc0 L%=_rflush\n"
-      "L%=_rowend:\n"
-      MB_ROW_SWAP
-      "s_mov_b32 %[mbx], %[x0t]\n"
-      "s_add_u32 %[mby], %[mby], 1\n"
-      "s_cmp_lt_u32 %[mby], %[y1t]\n"
-      "s_cbranch_scc1 L%=_mb\n"
-      "s_mov_b32 %[ev], 0\n"
-      "s_branch L%=_exit\n"
-      // staged records out: lanes [0, 12 nst) to recs[firstmb ..]
-      // (t3: 0 continue with the next macroblock, 1 end of row)
-      "L%=_rflush:\n"
-      "s_mul_i32 %[t1], %[nst], 12\n"
-      "s_bfm_b64 %[b], %[t1], 0\n"
-      "s_mul_i32 %[t2], %[firstmb], 48\n"
-      "v_add_u32 %[vaddr], %[t2], %[vlane4]\n"
-      "s_mov_b64 exec, %[b]\n"
-      "global_store_dword %[vaddr], %[vstg], %[recs]\n"
-      "s_mov_b64 exec, -1\n"
-      "s_mov_b32 %[nst], 0\n"
-      "s_cmp_eq_u32 %[t3], 0\n"
-      "s_cbranch_scc1 L%=_next\n"
-      "s_branch L%=_rowend\n"
-
-      // ---- decode errors: the failing macroblock's fields stay as decoded
-      "L%=_mode_unused:\n"
-      "s_mov_b32 %[ev], 3\n"
-      "s_branch L%=_modeerr\n"
-      "L%=_mode_underrun:\n"
-      "s_mov_b32 %[ev], 4\n"
-      // a failed mode reads as a skip macroblock (Go reconstructs it)
-      "L%=_modeerr:\n"
-      "s_mov_b32 %[t3], 2\n"
-      "s_branch L%=_mvp\n"
-      "L%=_modeerr_mv:\n"
-      "s_mov_b32 %[r0], 0\n"
-      "s_mov_b32 %[mv0], " MB_PRED "\n"
-      "s_mov_b32 %[mvcur], " MB_PRED "\n"
-      "s_mov_b32 %[wmask], 0x3ffff\n"
-      "s_branch L%=_exit\n"
-      "L%=_unused:\n"
-      "s_mov_b32 %[ev], 3\n"
-      "s_branch L%=_exit\n"
-      "L%=_underrun:\n"
-      "s_mov_b32 %[ev], 4\n"
-      "s_branch L%=_exit\n"
-
-      // ---- out-of-line renormalisation and coefficient stores
-      MB_RN_OOL("md", "mode_underrun")
-      MB_RN_OOL("rf", "underrun")
-      MB_RN_OOL("pt", "underrun")
-      MB_RN_OOL("cxd", "underrun")
-      MB_RN_OOL("cxb", "underrun")
-      MB_RN_OOL("cyd", "underrun")
-      MB_RN_OOL("cyb", "underrun")
-      MB_RN_OOL("cb", "underrun")
-      MB_RN_OOL("tx", "underrun")
-      MB_RN_OOL("cm", "underrun")
-      MB_RN_OOL("lm", "underrun")
-      MB_RN_OOL("esd", "underrun")
-      MB_RN_OOL("esb", "underrun")
-      MB_RN_OOL("sd", "underrun")
-      MBP_TOKLOOP_OOL
-      MB_EMIT_OOL("e2")
-
-      "L%=_exit:\n"
-      "s_mov_b32 m0, %[m0s]\n"
-      "s_nop 4\n"
-      : [x0] "+&s"(S.x0), [x1] "+&s"(S.x1), [x2] "+&s"(S.x2), [x3] "+&s"(S.x3), [pos] "+&s"(S.pos),
-        [nsym] "+&s"(S.nsym), [wbase] "+&s"(S.wbase), [lim] "+&s"(S.lim), [cn] "+&s"(S.cn), [cbase] "+&s"(S.cbase),
-        [mbx] "+&s"(S.mbx), [mby] "+&s"(S.mby), [done] "+&s"(S.done), [nintra] "+&s"(S.nintra), [nst] "+&s"(S.nst),
-        [firstmb] "+&s"(S.firstmb), [leftmv] "+&s"(S.leftmv), [vwin] "+&v"(S.vwin), [vbuf] "+&v"(S.vbuf),
-        [vstg] "+&v"(S.vstg), [vprow] "+&v"(S.vprow), [vcrow] "+&v"(S.vcrow),
-        [prowa] "+&s"(S.prowa), [crowa] "+&s"(S.crowa), [r0] "=&s"(S.r0), [r1] "=&s"(S.r1),
-        [wmask] "=&s"(S.wmask), [cstart] "=&s"(S.cstart), [mv0] "=&s"(S.mv0), [mv1] "=&s"(S.mv1),
-        [mv2] "=&s"(S.mv2), [mv3] "=&s"(S.mv3), [mvcur] "=&s"(S.mvcur), [lm0] "=&s"(S.lm0), [lm1] "=&s"(S.lm1),
-        [ev] "=&s"(ev), [slot] "=&s"(slot), [t1] "=&s"(t1), [t2] "=&s"(t2), [t3] "=&s"(t3), [p] "=&s"(p),
-        [xn] "=&s"(xn), [cnt] "=&s"(cnt), [sym] "=&s"(sym), [val] "=&s"(val), [k2] "=&s"(k2), [k] "=&s"(k),
-        [nb] "=&s"(nb), [total] "=&s"(total), [blk8] "=&s"(blk8), [cbpf] "=&s"(cbpf), [si] "=&s"(si),
-        [stop] "=&s"(stop), [boff] "=&s"(boff), [prev] "=&s"(prev), [sumabs] "=&s"(sumabs),
-        [firstsi] "=&s"(firstsi), [firstpm] "=&s"(firstpm), [last] "=&s"(last), [m0s] "=&s"(m0s),
-        [b] "=&s"(b), [vcur] "=&v"(vcur), [va] "=&v"(va), [vb] "=&v"(vb), [vc] "=&v"(vc),
-        [vaddr] "=&v"(vaddr), [vt] "=&v"(vt)
-      : [vm0] "v"(T.misc0), [vm1] "v"(T.misc1), [vm2] "v"(T.misc2), [t00] "v"(T.t00), [t01] "v"(T.t01),
-        [t02] "v"(T.t02), [t10] "v"(T.t10), [t11] "v"(T.t11), [t12] "v"(T.t12), [t20] "v"(T.t20),
-        [t21] "v"(T.t21), [t22] "v"(T.t22), [t30] "v"(T.t30), [t31] "v"(T.t31), [t32] "v"(T.t32),
-        [vlane4] "v"(vlane4), [x0t] "s"(sgpr(x0t)),
-        [x1t] "s"(sgpr(x1t)), [y0t] "s"(sgpr(y0t)), [y1t] "s"(sgpr(y1t)), [mbc] "s"(sgpr(mbc)),
-        [interf] "s"(sgpr(interf)), [tcb] "s"(sgpr(tcb)), [f0] "s"(sgpr(f0)), [f1] "s"(sgpr(f1)),
-        [hi2] "s"(sgpr(hi2)), [wlo] "s"(sgpr(wlo)), [tb] "s"(sgpr(tb)), [dst] "s"(sgpr(dst)),
-        [recs] "s"(sgpr(recs))
-      : "scc", "vcc", "memory");
-#undef MB_ROW_RD
-#undef MB_ROW_WR
-#undef MB_ROW_SWAP
-  } else {
-#define MB_ROW_RD MB_ROW_RD_LDS
-#define MB_ROW_WR MB_ROW_WR_LDS
-#define MB_ROW_SWAP MB_ROW_SWAP_LDS
-  asm volatile(
-      "s_nop 4\n"
-      "s_mov_b32 %[m0s], m0\n"
-
-      // ---------------------------------------------------------------- MB
-      "L%=_mb:\n"
-      "s_add_u32 %[cstart], %[cbase], %[cn]\n"
-      "s_mov_b32 %[r0], 2\n"
-      "s_mov_b32 %[r1], 0\n"
-      "s_mov_b32 %[wmask], 0\n"
-      "s_mov_b32 %[mv0], 0\n"
-      "s_mov_b32 %[mv1], 0\n"
-      "s_mov_b32 %[mv2], 0\n"
-      "s_mov_b32 %[mv3], 0\n"
-      "s_mov_b32 %[mvcur], 0\n"
-      "s_mov_b32 %[lm0], 0\n"
-      "s_mov_b32 %[lm1], 0\n"
-      "s_mov_b32 %[cbpf], 7\n"
-      "s_cmp_eq_u32 %[interf], 0\n"
-      "s_cbranch_scc1 L%=_tx\n"
-      // macroblock mode (decode_inter.go:9-16)
-      MB_DEC("%[vm0]", "16", "15", "0x100010", "%[sym]", "md", "mode_unused")
-      "s_cmp_eq_u32 %[sym], 0\n"
-      "s_cbranch_scc1 L%=_skip\n"
-      "s_cmp_eq_u32 %[sym], 2\n"
-      "s_cbranch_scc1 L%=_tx\n"
-      // ---- inter macroblock: reference, partition, vectors, pattern
-      "s_mov_b32 %[r0], 1\n"
-      MB_DEC("%[vm1]", "32", "31", "0x100020", "%[sym]", "rf", "unused")
-      "s_mov_b32 %[r1], %[sym]\n"
-      "s_mov_b32 " MB_PRED ", 0\n"
-      "s_cmp_eq_u32 %[sym], 1\n"
-      "s_cbranch_scc1 L%=_nopred\n"
-      "s_mov_b32 %[t3], 1\n"
-      "s_branch L%=_mvp\n"
-      "L%=_nopred:\n"
-      MB_DEC("%[vm1]", "48", "47", "0x100030", "%[sym]", "pt", "unused")
-      "s_lshl_b32 %[t1], %[sym], 8\n"
-      "s_or_b32 %[r0], %[r0], %[t1]\n"
-      "s_lshl_b32 %[t1], %[sym], 2\n"
-      "s_lshr_b32 " MB_NPART ", 0x4221, %[t1]\n"
-      "s_and_b32 " MB_NPART ", " MB_NPART ", 15\n"
-      "s_mov_b32 " MB_PI ", 0\n"
-      "L%=_mvl:\n"
-      "s_cmp_eq_u32 " MB_PI ", 0\n"
-      "s_cselect_b32 %[cnt], " MB_PRED ", %[mv0]\n"
-      // x component: base.x + zigzag(class value), clamped to +-4096
-      MB_CLASS("%[vm2]", "16", "15", "0x100010", "cx")
-      "s_sext_i32_i16 %[t1], %[cnt]\n"
-      "s_lshr_b32 %[t2], %[sym], 1\n"
-      "s_and_b32 %[sym], %[sym], 1\n"
-      "s_sub_u32 %[sym], 0, %[sym]\n"
-      "s_xor_b32 %[t2], %[t2], %[sym]\n"
-      "s_add_u32 %[t1], %[t1], %[t2]\n"
-      "s_max_i32 %[t1], %[t1], -4096\n"
-      "s_min_i32 %[t1], %[t1], 4096\n"
-      "s_and_b32 " MB_MVT ", %[t1], 0xffff\n"
-      MB_CLASS("%[vm2]", "32", "31", "0x100020", "cy")
-      "s_ashr_i32 %[t1], %[cnt], 16\n"
-      "s_lshr_b32 %[t2], %[sym], 1\n"
-      "s_and_b32 %[sym], %[sym], 1\n"
-      "s_sub_u32 %[sym], 0, %[sym]\n"
-      "s_xor_b32 %[t2], %[t2], %[sym]\n"
-      "s_add_u32 %[t1], %[t1], %[t2]\n"
-      "s_max_i32 %[t1], %[t1], -4096\n"
-      "s_min_i32 %[t1], %[t1], 4096\n"
-      "s_lshl_b32 %[t1], %[t1], 16\n"
-      "s_or_b32 " MB_MVT ", " MB_MVT ", %[t1]\n"
-      "s_cmp_eq_u32 " MB_PI ", 0\n"
-      "s_cselect_b32 %[mv0], " MB_MVT ", %[mv0]\n"
-      "s_cmp_eq_u32 " MB_PI ", 1\n"
-      "s_cselect_b32 %[mv1], " MB_MVT ", %[mv1]\n"
-      "s_cmp_eq_u32 " MB_PI ", 2\n"
-      "s_cselect_b32 %[mv2], " MB_MVT ", %[mv2]\n"
-      "s_cmp_eq_u32 " MB_PI ", 3\n"
-      "s_cselect_b32 %[mv3], " MB_MVT ", %[mv3]\n"
-      "s_add_u32 " MB_PI ", " MB_PI ", 1\n"
-      "s_cmp_lt_u32 " MB_PI ", " MB_NPART "\n"
-      "s_cbranch_scc1 L%=_mvl\n"
-      "s_cmp_eq_u32 %[r1], 1\n"
-      "s_cselect_b32 %[mvcur], 0, %[mv0]\n"
-      MB_DEC("%[vm0]", "48", "47", "0x100030", "%[cbpf]", "cb", "unused")
-      "s_lshl_b32 %[t1], %[cbpf], 16\n"
-      "s_or_b32 %[r0], %[r0], %[t1]\n"
-      "s_mov_b32 %[nb], 0\n"
-      "s_bitcmp1_b32 %[cbpf], 0\n"
-      "s_cbranch_scc1 L%=_tx\n"
-      "s_mov_b32 %[wmask], 0xffff\n"
-      "s_branch L%=_blocks\n"
+      // ---- key frame macroblock (always intra)
+      "L%=_kmb:\n"
+      MB_INIT
 
       // ---- transform size; the luma blocks' vectors and scan
       "L%=_tx:\n"
@@ -1803,34 +1252,15 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       "v_mov_b32 %[vb], %[t21]\n"
       "v_mov_b32 %[vc], %[t22]\n"
 
-      // ---- blocks: luma 0..nb-1, then Cb, Cr (when coded)
+      // ---- blocks: luma 0..nb-1 (intra: each with its mode), then Cb, Cr
+      // when coded
       "L%=_blocks:\n"
       "s_mov_b32 %[k], 0\n"
-      "L%=_bk:\n"
-      "s_cmp_eq_u32 %[k], %[nb]\n"
-      "s_cbranch_scc0 L%=_bk_nl\n"
-      // luma complete: the chroma mode (intra) or the uncoded chroma planes (inter)
-      "s_and_b32 %[t3], %[r0], 0xff\n"
-      "s_cmp_eq_u32 %[t3], 2\n"
-      "s_cbranch_scc0 L%=_bk_ic\n"
-      MB_DEC("%[vm1]", "16", "15", "0x100010", "%[sym]", "cm", "unused")
-      "s_lshl_b32 %[t1], %[sym], 8\n"
-      "s_or_b32 %[r1], %[r1], %[t1]\n"
-      "s_branch L%=_bk_nl\n"
-      "L%=_bk_ic:\n"
-      "s_bitcmp1_b32 %[cbpf], 1\n"
-      "s_cbranch_scc1 L%=_bk_icr\n"
-      "s_bitset1_b32 %[wmask], 16\n"
-      "L%=_bk_icr:\n"
-      "s_bitcmp1_b32 %[cbpf], 2\n"
-      "s_cbranch_scc1 L%=_bk_nl\n"
-      "s_bitset1_b32 %[wmask], 17\n"
-      "L%=_bk_nl:\n"
-      "s_cmp_lt_u32 %[k], %[nb]\n"
-      "s_cbranch_scc0 L%=_bk_ch\n"
+      "s_cmp_eq_u32 %[nb], 0\n"
+      "s_cbranch_scc1 L%=_lumadone\n"
+      "L%=_lblk:\n"
       "s_lshl_b32 %[blk8], %[k], 8\n"
-      "s_and_b32 %[t3], %[r0], 0xff\n"
-      "s_cmp_eq_u32 %[t3], 2\n"
+      "s_bitcmp1_b32 %[r0], 1\n"
       "s_cbranch_scc0 L%=_tb\n"
       // intra luma mode of block k (4 bits in decode order)
       MB_DEC("%[vm1]", "0", "-1", "0x100000", "%[sym]", "lm", "unused")
@@ -1838,31 +1268,19 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       "s_lshl_b32 %[t1], %[t1], 2\n"
       "s_lshl_b32 %[t1], %[sym], %[t1]\n"
       "s_cmp_lt_u32 %[k], 8\n"
-      "s_cbranch_scc0 L%=_lmh\n"
-      "s_or_b32 %[lm0], %[lm0], %[t1]\n"
-      "s_branch L%=_tb\n"
-      "L%=_lmh:\n"
-      "s_or_b32 %[lm1], %[lm1], %[t1]\n"
-      "s_branch L%=_tb\n"
-      "L%=_bk_ch:\n"
-      "s_sub_u32 %[t1], %[k], %[nb]\n"
-      "s_add_u32 %[t3], %[t1], 1\n"
-      "s_lshr_b32 %[t3], %[cbpf], %[t3]\n"
-      "s_bitcmp1_b32 %[t3], 0\n"
-      "s_cbranch_scc0 L%=_bk_next\n"
-      "s_add_u32 %[t1], %[t1], 16\n"
-      "s_lshl_b32 %[blk8], %[t1], 8\n"
-      "s_mov_b32 %[total], 64\n"
-      "v_mov_b32 %[va], %[t30]\n"
-      "v_mov_b32 %[vb], %[t31]\n"
-      "v_mov_b32 %[vc], %[t32]\n"
+      "s_cselect_b32 %[t2], %[t1], 0\n"
+      "s_or_b32 %[lm0], %[lm0], %[t2]\n"
+      "s_xor_b32 %[t2], %[t2], %[t1]\n"
+      "s_or_b32 %[lm1], %[lm1], %[t2]\n"
 
       // ------------------------------------------------------ token block
       "L%=_tb:\n"
       "s_mov_b32 %[si], 0\n"
       "s_mov_b32 %[stop], 1\n"
-      "s_mov_b32 %[boff], 0\n"
-      "s_mov_b32 %[prev], 0\n"
+      "s_mov_b32 %[off], 0\n"
+      "s_mov_b32 %[offz], 0\n"
+      "s_mov_b32 %[bf], 0x80000\n"
+      "s_mov_b32 %[bfz], 0x80000\n"
       "s_mov_b32 %[sumabs], 0\n"
       "s_mov_b32 %[firstsi], 0\n"
       "v_mov_b32 %[vcur], %[va]\n"
@@ -1873,7 +1291,8 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       "s_cbranch_scc1 L%=_teob\n"
       MB_CLASS("%[vm2]", "0", "-1", "0x100000", "es")
       "s_add_u32 " MB_MAG ", %[sym], 5\n"
-      "s_mov_b32 %[prev], 16\n"
+      "s_add_u32 %[off], %[offz], 16\n"
+      "s_add_u32 %[bf], %[bfz], 16\n"
       "s_branch L%=_tlev0\n"
       // end of block: the first level's sign, hidden in the level-sum parity
       // when last - first >= 4, else explicit
@@ -1901,142 +1320,72 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       "s_lshr_b32 %[t1], %[blk8], 8\n"
       "s_lshl_b32 %[t1], 1, %[t1]\n"
       "s_or_b32 %[wmask], %[wmask], %[t1]\n"
-      "L%=_bk_next:\n"
       "s_add_u32 %[k], %[k], 1\n"
-      "s_add_u32 %[t1], %[nb], 2\n"
-      "s_cmp_lt_u32 %[k], %[t1]\n"
-      "s_cbranch_scc1 L%=_bk\n"
-      "s_branch L%=_mbend\n"
+      "s_cmp_lt_u32 %[k], %[nb]\n"
+      "s_cbranch_scc1 L%=_lblk\n"
+      "s_cmp_eq_u32 %[k], %[nb]\n"
+      "s_cbranch_scc0 L%=_chnext\n"
+      // luma complete: the chroma mode (intra) or the uncoded chroma planes
+      // (inter)
+      "L%=_lumadone:\n"
+      "s_bitcmp1_b32 %[r0], 1\n"
+      "s_cbranch_scc0 L%=_ic\n"
+      MB_DEC("%[vm1]", "16", "15", "0x100010", "%[sym]", "cm", "unused")
+      "s_lshl_b32 %[t1], %[sym], 8\n"
+      "s_or_b32 %[r1], %[r1], %[t1]\n"
+      "s_branch L%=_chnext\n"
+      "L%=_ic:\n"
+      "s_bitcmp1_b32 %[cbpf], 1\n"
+      "s_cbranch_scc1 L%=_icr\n"
+      "s_bitset1_b32 %[wmask], 16\n"
+      "L%=_icr:\n"
+      "s_bitcmp1_b32 %[cbpf], 2\n"
+      "s_cbranch_scc1 L%=_chnext\n"
+      "s_bitset1_b32 %[wmask], 17\n"
+      // chroma block k - nb (0 Cb, 1 Cr, 2 done)
+      "L%=_chnext:\n"
+      "s_sub_u32 %[t1], %[k], %[nb]\n"
+      "s_cmp_eq_u32 %[t1], 2\n"
+      "s_cbranch_scc1 L%=_mbend\n"
+      "s_add_u32 %[t3], %[t1], 1\n"
+      "s_lshr_b32 %[t3], %[cbpf], %[t3]\n"
+      "s_bitcmp1_b32 %[t3], 0\n"
+      "s_cbranch_scc0 L%=_chskip\n"
+      "s_add_u32 %[t1], %[t1], 16\n"
+      "s_lshl_b32 %[blk8], %[t1], 8\n"
+      "s_mov_b32 %[total], 64\n"
+      "v_mov_b32 %[va], %[t30]\n"
+      "v_mov_b32 %[vb], %[t31]\n"
+      "v_mov_b32 %[vc], %[t32]\n"
+      "s_branch L%=_tb\n"
+      "L%=_chskip:\n"
+      "s_add_u32 %[k], %[k], 1\n"
+      "s_branch L%=_chnext\n"
 
-      // ---- skip macroblock: the predicted vector, every block written
-      "L%=_skip:\n"
-      "s_mov_b32 %[t3], 0\n"
-      "s_branch L%=_mvp\n"
-      "L%=_skipmv:\n"
-      "s_mov_b32 %[r0], 0\n"
-      "s_mov_b32 %[mv0], " MB_PRED "\n"
-      "s_mov_b32 %[mvcur], " MB_PRED "\n"
-      "s_mov_b32 %[wmask], 0x3ffff\n"
-      "s_branch L%=_mbend\n"
-
-      // ---- predictMV (mv.go:21-52) into pred; t3 = 0 skip, 1 inter, 2 error
-      "L%=_mvp:\n"
-      "s_sub_u32 %[t1], %[mbx], %[x0t]\n"
-      "s_cmp_gt_u32 %[mby], %[y0t]\n"
-      "s_cbranch_scc0 L%=_mvp_nob\n"
-      MB_ROW_RD("%[t2]", "%[t1]")
-      "s_add_u32 %[p], %[t1], 1\n"
-      "s_sub_u32 %[xn], %[x1t], %[x0t]\n"
-      "s_cmp_lt_u32 %[p], %[xn]\n"
-      "s_cbranch_scc0 L%=_mvp_cl\n"
-      MB_ROW_RD("%[xn]", "%[p]")
-      "s_branch L%=_mvp_med\n"
-      "L%=_mvp_cl:\n"
-      "s_cmp_eq_u32 %[t1], 0\n"
-      "s_cbranch_scc1 L%=_mvp_b\n"
-      "s_add_u32 %[p], %[t1], -1\n"
-      MB_ROW_RD("%[xn]", "%[p]")
-      "L%=_mvp_med:\n"
-      "s_cmp_eq_u32 %[t1], 0\n"
-      "s_cselect_b32 %[p], 0, %[leftmv]\n"
-      // component-wise median of a (p), b (t2), c (xn)
-      "s_sext_i32_i16 %[slot], %[p]\n"
-      "s_sext_i32_i16 %[cnt], %[t2]\n"
-      "s_sext_i32_i16 %[k2], %[xn]\n"
-      "s_min_i32 %[t1], %[slot], %[cnt]\n"
-      "s_max_i32 %[slot], %[slot], %[cnt]\n"
-      "s_min_i32 %[slot], %[slot], %[k2]\n"
-      "s_max_i32 %[t1], %[t1], %[slot]\n"
-      "s_ashr_i32 %[slot], %[p], 16\n"
-      "s_ashr_i32 %[cnt], %[t2], 16\n"
-      "s_ashr_i32 %[k2], %[xn], 16\n"
-      "s_min_i32 %[p], %[slot], %[cnt]\n"
-      "s_max_i32 %[slot], %[slot], %[cnt]\n"
-      "s_min_i32 %[slot], %[slot], %[k2]\n"
-      "s_max_i32 %[p], %[p], %[slot]\n"
-      "s_and_b32 %[t1], %[t1], 0xffff\n"
-      "s_lshl_b32 %[p], %[p], 16\n"
-      "s_or_b32 " MB_PRED ", %[t1], %[p]\n"
-      "s_branch L%=_mvp_end\n"
-      "L%=_mvp_b:\n"
-      "s_mov_b32 " MB_PRED ", %[t2]\n"
-      "s_branch L%=_mvp_end\n"
-      "L%=_mvp_nob:\n"
-      "s_cmp_eq_u32 %[t1], 0\n"
-      "s_cselect_b32 " MB_PRED ", 0, %[leftmv]\n"
-      "L%=_mvp_end:\n"
-      "s_cmp_eq_u32 %[t3], 0\n"
-      "s_cbranch_scc1 L%=_skipmv\n"
-      "s_cmp_eq_u32 %[t3], 1\n"
-      "s_cbranch_scc1 L%=_nopred\n"
-      "s_branch L%=_modeerr_mv\n"
-
-      // ---- end of a macroblock: MV row, record, next macroblock
+      // ---- end of a coded macroblock: its record and MV
       "L%=_mbend:\n"
       "s_and_b32 %[t3], %[r0], 0xff\n"
       "s_cmp_eq_u32 %[t3], 2\n"
       "s_cselect_b32 %[t3], 1, 0\n"
       "s_add_u32 %[nintra], %[nintra], %[t3]\n"
-      "s_sub_u32 %[t1], %[mbx], %[x0t]\n"
-      MB_ROW_WR
-      "s_mov_b32 %[leftmv], %[mvcur]\n"
-      "s_cmp_eq_u32 %[nst], 0\n"
-      "s_cbranch_scc0 L%=_rec\n"
-      "s_mul_i32 %[firstmb], %[mby], %[mbc]\n"
-      "s_add_u32 %[firstmb], %[firstmb], %[mbx]\n"
-      "L%=_rec:\n"
-      "s_mul_i32 %[t2], %[nst], 12\n"
+      "s_mov_b32 m0, %[ln]\n"
       "s_add_u32 %[t1], %[tcb], %[cstart]\n"
       "s_add_u32 %[t3], %[cbase], %[cn]\n"
       "s_sub_u32 %[t3], %[t3], %[cstart]\n"
-      MB_REC(0, "%[r0]")
-      MB_REC(1, "%[r1]")
-      MB_REC(2, "%[wmask]")
-      MB_REC(3, "%[t1]")
-      MB_REC(4, "%[t3]")
-      MB_REC(5, "%[mv0]")
-      MB_REC(6, "%[mv1]")
-      MB_REC(7, "%[mv2]")
-      MB_REC(8, "%[mv3]")
-      MB_REC(9, "%[lm0]")
-      MB_REC(10, "%[lm1]")
-      MB_REC(11, "0")
-      "s_add_u32 %[done], %[done], 1\n"
-      "s_add_u32 %[nst], %[nst], 1\n"
-      "s_cmp_eq_u32 %[nst], 5\n"
-      "s_cbranch_scc0 L%=_next\n"
-      "s_mov_b32 %[t3], 0\n"
-      "s_branch L%=_rflush\n"
-      "L%=_next:\n"
-      "s_add_u32 %[mbx], %[mbx], 1\n"
-      "s_cmp_lt_u32 %[mbx], %[x1t]\n"
-      "s_cbranch_scc1 L%=_mb\n"
-      // end of a row: its records out, the rows swap
-      "s_mov_b32 %[t3], 1\n"
-      "s_cmp_eq_u32 %[nst], 0\n"
-      "s_cbranch_scc0 L%=_rflush\n"
-      "L%=_rowend:\n"
-      MB_ROW_SWAP
-      "s_mov_b32 %[mbx], %[x0t]\n"
-      "s_add_u32 %[mby], %[mby], 1\n"
-      "s_cmp_lt_u32 %[mby], %[y1t]\n"
-      "s_cbranch_scc1 L%=_mb\n"
-      "s_mov_b32 %[ev], 0\n"
-      "s_branch L%=_exit\n"
-      // staged records out: lanes [0, 12 nst) to recs[firstmb ..]
-      // (t3: 0 continue with the next macroblock, 1 end of row)
-      "L%=_rflush:\n"
-      "s_mul_i32 %[t1], %[nst], 12\n"
-      "s_bfm_b64 %[b], %[t1], 0\n"
-      "s_mul_i32 %[t2], %[firstmb], 48\n"
-      "v_add_u32 %[vaddr], %[t2], %[vlane4]\n"
-      "s_mov_b64 exec, %[b]\n"
-      "global_store_dword %[vaddr], %[vstg], %[recs]\n"
-      "s_mov_b64 exec, -1\n"
-      "s_mov_b32 %[nst], 0\n"
-      "s_cmp_eq_u32 %[t3], 0\n"
-      "s_cbranch_scc1 L%=_next\n"
-      "s_branch L%=_rowend\n"
+      "v_writelane_b32 %[vr0], %[r0], m0\n"
+      "v_writelane_b32 %[vr1], %[r1], m0\n"
+      "v_writelane_b32 %[vr2], %[wmask], m0\n"
+      "v_writelane_b32 %[vr3], %[t1], m0\n"
+      "v_writelane_b32 %[vr4], %[t3], m0\n"
+      "v_writelane_b32 %[vr5], %[mv0], m0\n"
+      "v_writelane_b32 %[vr6], %[mv1], m0\n"
+      "v_writelane_b32 %[vr7], %[mv2], m0\n"
+      "v_writelane_b32 %[vr8], %[mv3], m0\n"
+      "v_writelane_b32 %[vr9], %[lm0], m0\n"
+      "v_writelane_b32 %[vr10], %[lm1], m0\n"
+      "v_writelane_b32 %[vmv], %[mvcur], m0\n"
+      "s_mov_b32 %[leftmv], %[mvcur]\n"
+      "s_branch L%=_mbnext\n"
 
       // ---- decode errors: the failing macroblock's fields stay as decoded
       "L%=_mode_unused:\n"
@@ -2046,23 +1395,24 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       "s_mov_b32 %[ev], 4\n"
       // a failed mode reads as a skip macroblock (Go reconstructs it)
       "L%=_modeerr:\n"
-      "s_mov_b32 %[t3], 2\n"
-      "s_branch L%=_mvp\n"
-      "L%=_modeerr_mv:\n"
+      MB_INIT
       "s_mov_b32 %[r0], 0\n"
       "s_mov_b32 %[mv0], " MB_PRED "\n"
       "s_mov_b32 %[mvcur], " MB_PRED "\n"
       "s_mov_b32 %[wmask], 0x3ffff\n"
-      "s_branch L%=_exit\n"
+      "s_branch L%=_errx\n"
       "L%=_unused:\n"
       "s_mov_b32 %[ev], 3\n"
-      "s_branch L%=_exit\n"
+      "s_branch L%=_errx\n"
       "L%=_underrun:\n"
       "s_mov_b32 %[ev], 4\n"
-      "s_branch L%=_exit\n"
+      "L%=_errx:\n"
+      "s_mov_b32 %[t3], 1\n"
+      "s_branch L%=_flush\n"
 
       // ---- out-of-line renormalisation and coefficient stores
       MB_RN_OOL("md", "mode_underrun")
+      MB_RN_OOL("md2", "mode_underrun")
       MB_RN_OOL("rf", "underrun")
       MB_RN_OOL("pt", "underrun")
       MB_RN_OOL("cxd", "underrun")
@@ -2084,32 +1434,29 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       "s_nop 4\n"
       : [x0] "+&s"(S.x0), [x1] "+&s"(S.x1), [x2] "+&s"(S.x2), [x3] "+&s"(S.x3), [pos] "+&s"(S.pos),
         [nsym] "+&s"(S.nsym), [wbase] "+&s"(S.wbase), [lim] "+&s"(S.lim), [cn] "+&s"(S.cn), [cbase] "+&s"(S.cbase),
-        [mbx] "+&s"(S.mbx), [mby] "+&s"(S.mby), [done] "+&s"(S.done), [nintra] "+&s"(S.nintra), [nst] "+&s"(S.nst),
-        [firstmb] "+&s"(S.firstmb), [leftmv] "+&s"(S.leftmv), [vwin] "+&v"(S.vwin), [vbuf] "+&v"(S.vbuf),
-        [vstg] "+&v"(S.vstg), [vprow] "+&v"(S.vprow), [vcrow] "+&v"(S.vcrow),
+        [mbx] "+&s"(S.mbx), [mby] "+&s"(S.mby), [done] "+&s"(S.done), [nintra] "+&s"(S.nintra),
+        [leftmv] "+&s"(S.leftmv), [vwin] "+&v"(S.vwin), [vbuf] "+&v"(S.vbuf),
         [prowa] "+&s"(S.prowa), [crowa] "+&s"(S.crowa), [r0] "=&s"(S.r0), [r1] "=&s"(S.r1),
         [wmask] "=&s"(S.wmask), [cstart] "=&s"(S.cstart), [mv0] "=&s"(S.mv0), [mv1] "=&s"(S.mv1),
         [mv2] "=&s"(S.mv2), [mv3] "=&s"(S.mv3), [mvcur] "=&s"(S.mvcur), [lm0] "=&s"(S.lm0), [lm1] "=&s"(S.lm1),
         [ev] "=&s"(ev), [slot] "=&s"(slot), [t1] "=&s"(t1), [t2] "=&s"(t2), [t3] "=&s"(t3), [p] "=&s"(p),
         [xn] "=&s"(xn), [cnt] "=&s"(cnt), [sym] "=&s"(sym), [val] "=&s"(val), [k2] "=&s"(k2), [k] "=&s"(k),
         [nb] "=&s"(nb), [total] "=&s"(total), [blk8] "=&s"(blk8), [cbpf] "=&s"(cbpf), [si] "=&s"(si),
-        [stop] "=&s"(stop), [boff] "=&s"(boff), [prev] "=&s"(prev), [sumabs] "=&s"(sumabs),
-        [firstsi] "=&s"(firstsi), [firstpm] "=&s"(firstpm), [last] "=&s"(last), [m0s] "=&s"(m0s),
-        [b] "=&s"(b), [vcur] "=&v"(vcur), [va] "=&v"(va), [vb] "=&v"(vb), [vc] "=&v"(vc),
-        [vaddr] "=&v"(vaddr), [vt] "=&v"(vt)
+        [stop] "=&s"(stop), [off] "=&s"(off), [offz] "=&s"(offz), [bf] "=&s"(bf), [bfz] "=&s"(bfz),
+        [sumabs] "=&s"(sumabs), [firstsi] "=&s"(firstsi), [firstpm] "=&s"(firstpm), [last] "=&s"(last),
+        [m0s] "=&s"(m0s), [ln] "=&s"(ln), [pend] "=&s"(pend), [b] "=&s"(b), [vcur] "=&v"(vcur), [va] "=&v"(va),
+        [vb] "=&v"(vb), [vc] "=&v"(vc), [vaddr] "=&v"(vaddr), [vt] "=&v"(vt), [vr0] "=&v"(vr0),
+        [vr1] "=&v"(vr1), [vr2] "=&v"(vr2), [vr3] "=&v"(vr3), [vr4] "=&v"(vr4), [vr5] "=&v"(vr5),
+        [vr6] "=&v"(vr6), [vr7] "=&v"(vr7), [vr8] "=&v"(vr8), [vr9] "=&v"(vr9), [vr10] "=&v"(vr10),
+        [vr11] "=&v"(vr11), [vmv] "=&v"(vmv), [vB] "=&v"(vB), [vC] "=&v"(vC)
       : [vm0] "v"(T.misc0), [vm1] "v"(T.misc1), [vm2] "v"(T.misc2), [t00] "v"(T.t00), [t01] "v"(T.t01),
         [t02] "v"(T.t02), [t10] "v"(T.t10), [t11] "v"(T.t11), [t12] "v"(T.t12), [t20] "v"(T.t20),
         [t21] "v"(T.t21), [t22] "v"(T.t22), [t30] "v"(T.t30), [t31] "v"(T.t31), [t32] "v"(T.t32),
-        [vlane4] "v"(vlane4), [x0t] "s"(sgpr(x0t)),
-        [x1t] "s"(sgpr(x1t)), [y0t] "s"(sgpr(y0t)), [y1t] "s"(sgpr(y1t)), [mbc] "s"(sgpr(mbc)),
-        [interf] "s"(sgpr(interf)), [tcb] "s"(sgpr(tcb)), [f0] "s"(sgpr(f0)), [f1] "s"(sgpr(f1)),
-        [hi2] "s"(sgpr(hi2)), [wlo] "s"(sgpr(wlo)), [tb] "s"(sgpr(tb)), [dst] "s"(sgpr(dst)),
-        [recs] "s"(sgpr(recs))
+        [vlane4] "v"(vlane4), [x0t] "s"(sgpr(x0t)), [x1t] "s"(sgpr(x1t)), [y0t] "s"(sgpr(y0t)),
+        [y1t] "s"(sgpr(y1t)), [mbc] "s"(sgpr(mbc)), [interf] "s"(sgpr(interf)), [tcb] "s"(sgpr(tcb)),
+        [f0] "s"(sgpr(f0)), [f1] "s"(sgpr(f1)), [hi2] "s"(sgpr(hi2)), [wlo] "s"(sgpr(wlo)), [tb] "s"(sgpr(tb)),
+        [dst] "s"(sgpr(dst)), [recs] "s"(sgpr(recs))
       : "scc", "vcc", "memory");
-#undef MB_ROW_RD
-#undef MB_ROW_WR
-#undef MB_ROW_SWAP
-  }
   return ev;
 }
 
@@ -2194,14 +1541,6 @@ __device__ __forceinline__ void parse_tile(const uint8_t* __restrict__ pkt, cons
   T.t10 = vt[64 * 6]; T.t11 = vt[64 * 7]; T.t12 = vt[64 * 8];
   T.t20 = vt[64 * 9]; T.t21 = vt[64 * 10]; T.t22 = vt[64 * 11];
   T.t30 = vt[64 * 12]; T.t31 = vt[64 * 13]; T.t32 = vt[64 * 14];
-  FastTok F;
-  {
-    const uint32_t e0 = rd_lane(T.misc0, static_cast<int>(kGSign)), e1 = rd_lane(T.misc0, static_cast<int>(kGSign) + 1);
-    F.sign_used = e0 != kNoSym;
-    F.f0 = e0 & 0x1FFFu;
-    F.f1 = e1 & 0x1FFFu;
-    F.vlane4 = 4u * static_cast<uint32_t>(lane);
-  }
   // LDS: the macroblock records being staged, then two MV rows of the tile
   // width.  Records go to memory kRecStage at a time, every lane storing
   // dwords: a store per macroblock from one lane would have the next
@@ -2282,7 +1621,7 @@ __device__ __forceinline__ void parse_tile(const uint8_t* __restrict__ pkt, cons
       if (coded) {
         const bool intra = m.kind == 2;
         const uint32_t pred = intra ? 0u : predict_mv(prow, crow, mbx - td.x0, tw, mby == td.y0);
-        ok = parse_mb(R, out, m, T, F, zz4, zz8, zz16, intra, pred, lane);
+        ok = parse_mb(R, out, m, T, zz4, zz8, zz16, intra, pred, lane);
       }
       if (m.kind == 2) nintra++;
       if (lane == 0) {
@@ -2347,33 +1686,14 @@ __device__ __forceinline__ void parse_tile_asm(const uint8_t* __restrict__ pkt, 
   T.t10 = vt[64 * 6]; T.t11 = vt[64 * 7]; T.t12 = vt[64 * 8];
   T.t20 = vt[64 * 9]; T.t21 = vt[64 * 10]; T.t22 = vt[64 * 11];
   T.t30 = vt[64 * 12]; T.t31 = vt[64 * 13]; T.t32 = vt[64 * 14];
-  FastTok F;
-  {
-    const uint32_t e0 = rd_lane(T.misc0, static_cast<int>(kGSign)), e1 = rd_lane(T.misc0, static_cast<int>(kGSign) + 1);
-    F.sign_used = e0 != kNoSym;
-    F.f0 = e0 & 0x1FFFu;
-    F.f1 = e1 & 0x1FFFu;
-    F.vlane4 = 4u * static_cast<uint32_t>(lane);
-  }
-  // LDS: the macroblock records being staged, then two MV rows of the tile
-  // width.  Records go to memory kRecStage at a time, every lane storing
-  // dwords: a store per macroblock from one lane would have the next
-  // macroblock's record registers wait for it.
-  LdsU32* stage = (LdsU32*)smem;
+  // The sign context's two frequencies (symbol 0 starts at 0).
+  const uint32_t f0 = rd_lane(T.misc0, static_cast<int>(kGSign)) & 0xFFFFu;
+  const uint32_t f1 = rd_lane(T.misc0, static_cast<int>(kGSign) + 1) & 0xFFFFu;
+  const uint32_t vlane4 = 4u * static_cast<uint32_t>(lane);
+  // LDS: two MV rows of the widest tile (the row above, the current row),
+  // after the area the generic decoder stages records in.
   LdsU32* prow = (LdsU32*)(smem + kRecStage * sizeof(MBRec) / 4);
   LdsU32* crow = prow + hdr->max_tile_w;
-  uint32_t nst = 0, first_mb = 0;
-  auto flush = [&]() {
-    const LdsU32* src = stage;
-    uint32_t* dst = reinterpret_cast<uint32_t*>(recs + first_mb);
-    const uint32_t nd = nst * (sizeof(MBRec) / 4);
-    for (uint32_t i0 = 0; i0 < nd; i0 += 64) {
-      const uint32_t i = i0 + static_cast<uint32_t>(lane);
-      store_dword_if(dst + i0, 4u * static_cast<uint32_t>(lane), src[i < nd ? i : 0], i < nd);
-    }
-    nst = 0;
-  };
-  const int tw = td.x1 - td.x0;
 
   if (td.byte_len < 16) {  // ransDecoder.reset (rans.go:237-250)
     mark_unreached(recs, static_cast<uint32_t>(hdr->mbc), td, 0, lane);
@@ -2397,8 +1717,6 @@ __device__ __forceinline__ void parse_tile_asm(const uint8_t* __restrict__ pkt, 
     }
     R.x0 = sv[0]; R.x1 = sv[1]; R.x2 = sv[2]; R.x3 = sv[3];
   }
-  // The loop emits scan indices (reconstruction maps them to positions).
-  const uint32_t zz4 = 0, zz8 = 0, zz16 = 0;
   // Let every start-up load land here (an empty asm that takes the loaded
   // registers), so no load into a register the macroblock loop uses is
   // outstanding there: a wait in the loop would also wait for every store
@@ -2407,58 +1725,40 @@ __device__ __forceinline__ void parse_tile_asm(const uint8_t* __restrict__ pkt, 
   asm volatile("" : "+v"(T.t00), "+v"(T.t01), "+v"(T.t02), "+v"(T.t10), "+v"(T.t11), "+v"(T.t12));
   asm volatile("" : "+v"(T.t20), "+v"(T.t21), "+v"(T.t22), "+v"(T.t30), "+v"(T.t31), "+v"(T.t32));
 
-  CoefOut out;
-  out.dst = coefs + td.coef_base;
-  out.base = 0; out.cn = 0; out.buf = 0;
+  uint32_t* const cdst = coefs + td.coef_base;
   const int mbc = hdr->mbc;
   const bool inter_frame = hdr->inter != 0;
-  uint32_t done = 0, nintra = 0;
-  bool ok = true;
-  {
-    // The hand-scheduled loop over the whole tile.
-    MbLoop S;
-    S.x0 = R.x0; S.x1 = R.x1; S.x2 = R.x2; S.x3 = R.x3; S.pos = R.pos; S.nsym = R.nsym;
-    S.wbase = static_cast<uint32_t>(R.win.wbase); S.lim = static_cast<uint32_t>(R.win.lim);
-    S.cn = 0; S.cbase = 0;
-    S.mbx = td.x0; S.mby = td.y0; S.done = 0; S.nintra = 0; S.nst = 0; S.firstmb = 0; S.leftmv = 0;
-    S.vwin = R.win.cur; S.vbuf = 0; S.vstg = 0; S.vprow = 0; S.vcrow = 0;
-    S.prowa = sgpr(static_cast<uint32_t>(reinterpret_cast<uintptr_t>(prow)));
-    S.crowa = sgpr(static_cast<uint32_t>(reinterpret_cast<uintptr_t>(crow)));
-    S.x0 = sgpr(S.x0); S.x1 = sgpr(S.x1); S.x2 = sgpr(S.x2); S.x3 = sgpr(S.x3); S.pos = sgpr(S.pos);
-    S.nsym = sgpr(S.nsym); S.wbase = sgpr(S.wbase); S.lim = sgpr(S.lim); S.mbx = sgpr(S.mbx); S.mby = sgpr(S.mby);
-    // MV rows in two VGPRs up to 64 macroblocks of tile width, else in LDS
-    const uint32_t hi2 = static_cast<uint32_t>(R.win.hi - R.win.lo - 4), wlo = static_cast<uint32_t>(R.win.lo);
-    const uint32_t ev =
-        tw <= 64 ? mb_loop<false>(S, T, zz4, zz8, zz16, F.vlane4, td.x0, td.x1, td.y0, td.y1,
-                                  static_cast<uint32_t>(mbc), inter_frame ? 1u : 0u, td.coef_base, F.f0, F.f1, hi2,
-                                  wlo, R.win.tile + R.win.lo, out.dst, recs)
-                 : mb_loop<true>(S, T, zz4, zz8, zz16, F.vlane4, td.x0, td.x1, td.y0, td.y1,
-                                 static_cast<uint32_t>(mbc), inter_frame ? 1u : 0u, td.coef_base, F.f0, F.f1, hi2,
-                                 wlo, R.win.tile + R.win.lo, out.dst, recs);
-    R.x0 = S.x0; R.x1 = S.x1; R.x2 = S.x2; R.x3 = S.x3; R.pos = S.pos; R.nsym = S.nsym;
-    R.win.wbase = static_cast<int32_t>(S.wbase); R.win.lim = static_cast<int32_t>(S.lim); R.win.cur = S.vwin;
-    out.cn = S.cn; out.base = S.cbase; out.buf = S.vbuf;
-    done = S.done;
-    nintra = S.nintra;
-    if (ev != kMbDone) {
-      R.err = kErrCorrupt | ((ev == kMbUnused ? kTileUnused : kTileUnderrun) << 8);
-      // the records still staged, then the failing macroblock's as decoded
-      const uint32_t nd = S.nst * (sizeof(MBRec) / 4);
-      store_dword_if(reinterpret_cast<uint32_t*>(recs + S.firstmb), 4u * static_cast<uint32_t>(lane), S.vstg,
-                     static_cast<uint32_t>(lane) < nd);
-      if ((S.r0 & 0xFFu) == 2) nintra++;
-      if (lane == 0) {
-        uint32_t* rw = reinterpret_cast<uint32_t*>(recs + S.mby * static_cast<uint32_t>(mbc) + S.mbx);
-        rw[0] = S.r0; rw[1] = S.r1; rw[2] = S.wmask; rw[3] = td.coef_base + S.cstart;
-        rw[4] = S.cbase + S.cn - S.cstart;
-        rw[5] = S.mv0; rw[6] = S.mv1; rw[7] = S.mv2; rw[8] = S.mv3; rw[9] = S.lm0; rw[10] = S.lm1; rw[11] = 0;
-      }
-      done++;
+  MbLoop S;
+  S.x0 = sgpr(R.x0); S.x1 = sgpr(R.x1); S.x2 = sgpr(R.x2); S.x3 = sgpr(R.x3); S.pos = sgpr(R.pos);
+  S.nsym = sgpr(R.nsym); S.wbase = sgpr(static_cast<uint32_t>(R.win.wbase));
+  S.lim = sgpr(static_cast<uint32_t>(R.win.lim));
+  S.cn = 0; S.cbase = 0;
+  S.mbx = sgpr(static_cast<uint32_t>(td.x0)); S.mby = sgpr(static_cast<uint32_t>(td.y0));
+  S.done = 0; S.nintra = 0; S.leftmv = 0;
+  S.vwin = R.win.cur; S.vbuf = 0;
+  S.prowa = sgpr(static_cast<uint32_t>(reinterpret_cast<uintptr_t>(prow)));
+  S.crowa = sgpr(static_cast<uint32_t>(reinterpret_cast<uintptr_t>(crow)));
+  const uint32_t hi2 = static_cast<uint32_t>(R.win.hi - R.win.lo - 4), wlo = static_cast<uint32_t>(R.win.lo);
+  const uint32_t ev = mb_loop(S, T, vlane4, td.x0, td.x1, td.y0, td.y1, static_cast<uint32_t>(mbc),
+                              inter_frame ? 1u : 0u, td.coef_base, f0, f1, hi2, wlo, R.win.tile + R.win.lo, cdst,
+                              recs);
+  uint32_t done = S.done, nintra = S.nintra;
+  int32_t err = 0;
+  if (ev != kMbDone) {
+    err = static_cast<int32_t>(kErrCorrupt | ((ev == kMbUnused ? kTileUnused : kTileUnderrun) << 8));
+    // the failing macroblock's record as decoded (the ones before it are stored)
+    if ((S.r0 & 0xFFu) == 2) nintra++;
+    if (lane == 0) {
+      uint32_t* rw = reinterpret_cast<uint32_t*>(recs + S.mby * static_cast<uint32_t>(mbc) + S.mbx);
+      rw[0] = S.r0; rw[1] = S.r1; rw[2] = S.wmask; rw[3] = td.coef_base + S.cstart;
+      rw[4] = S.cbase + S.cn - S.cstart;
+      rw[5] = S.mv0; rw[6] = S.mv1; rw[7] = S.mv2; rw[8] = S.mv3; rw[9] = S.lm0; rw[10] = S.lm1; rw[11] = 0;
     }
+    done++;
   }
-  if (static_cast<uint32_t>(lane) < out.cn) out.dst[out.base + lane] = out.buf;
-  if (R.err) mark_unreached(recs, static_cast<uint32_t>(hdr->mbc), td, done, lane);
-  if (lane == 0) status[t] = TileStatus{R.err, done, R.nsym, nintra};
+  if (static_cast<uint32_t>(lane) < S.cn) cdst[S.cbase + lane] = S.vbuf;
+  if (err) mark_unreached(recs, static_cast<uint32_t>(hdr->mbc), td, done, lane);
+  if (lane == 0) status[t] = TileStatus{err, done, S.nsym, nintra};
 }
 
 // One tile: its job's packet and descriptor, the entropy outputs.
