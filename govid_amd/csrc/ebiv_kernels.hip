@@ -854,23 +854,21 @@ struct MbLoop {
   "s_sub_u32 %[t1], %[off], %[offz]\n"           \
   "s_xor_b32 %[offz], %[offz], 24\n"             \
   "s_xor_b32 %[bfz], %[bfz], 24\n"               \
+  "s_add_u32 %[off], %[offz], %[t1]\n"           \
+  "s_add_u32 %[bf], %[bfz], %[t1]\n"             \
   "s_cmp_eq_u32 %[si], 1\n"                      \
   "s_cselect_b32 %[stop], 3, %[stop]\n"          \
   "s_cmp_eq_u32 %[si], 9\n"                      \
   "s_cselect_b32 %[stop], 21, %[stop]\n"         \
   "s_cmp_eq_u32 %[si], 3\n"                      \
-  "s_cbranch_scc0 L%=_tb2" P F "\n"              \
-  "v_mov_b32 %[vcur], %[vb]\n"                   \
-  "s_mov_b32 %[stop], 9\n"                       \
-  "L%=_tb2" P F ":\n"                            \
+  "s_cselect_b32 %[stop], 9, %[stop]\n"          \
+  "s_cselect_b64 %[b], -1, 0\n"                  \
+  "v_cndmask_b32_e64 %[vcur], %[vcur], %[vb], %[b]\n" \
   "s_cmp_eq_u32 %[si], 21\n"                     \
-  "s_cbranch_scc0 L%=_tb4" P F "\n"              \
-  "v_mov_b32 %[vcur], %[vc]\n"                   \
-  "s_mov_b32 %[stop], %[total]\n"                \
-  "L%=_tb4" P F ":\n"                            \
+  "s_cselect_b32 %[stop], %[total], %[stop]\n"   \
+  "s_cselect_b64 %[b], -1, 0\n"                  \
+  "v_cndmask_b32_e64 %[vcur], %[vcur], %[vc], %[b]\n" \
   "s_min_u32 %[stop], %[stop], %[total]\n"       \
-  "s_add_u32 %[off], %[offz], %[t1]\n"           \
-  "s_add_u32 %[bf], %[bfz], %[t1]\n"             \
   "s_branch L%=_tok" P F "\n"
 // Back to phase 0 from phase 1, 2, 3 (t1 as the spare register).
 #define MBP_NORM1                                \
@@ -914,10 +912,9 @@ struct MbLoop {
   MBP_TSTOP("0", "a") MBP_TSTOP("1", "a") MBP_TSTOP("2", "a") MBP_TSTOP("3", "a") \
   MBP_TSTOP("0", "b") MBP_TSTOP("1", "b") MBP_TSTOP("2", "b") MBP_TSTOP("3", "b") \
   MBP_ESCLEVEL                                   \
-  "L%=_nesc0:\n" "s_branch L%=_tescob\n"         \
-  "L%=_nesc1:\n" MBP_NORM1 "s_branch L%=_tescob\n" \
-  "L%=_nesc2:\n" MBP_NORM2 "s_branch L%=_tescob\n" \
-  "L%=_nesc3:\n" MBP_NORM3 "s_branch L%=_tescob\n" \
+  "L%=_nesc1:\n" MBP_NORM1 "s_branch L%=_nesc0\n" \
+  "L%=_nesc2:\n" MBP_NORM2 "s_branch L%=_nesc0\n" \
+  "L%=_nesc3:\n" MBP_NORM3 "s_branch L%=_nesc0\n" \
   "L%=_nend0:\n" "s_branch L%=_tend\n"           \
   "L%=_nend1:\n" MBP_NORM1 "s_branch L%=_tend\n" \
   "L%=_nend2:\n" MBP_NORM2 "s_branch L%=_tend\n" \
@@ -1258,11 +1255,11 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       "s_mov_b32 %[k], 0\n"
       "s_cmp_eq_u32 %[nb], 0\n"
       "s_cbranch_scc1 L%=_lumadone\n"
-      "L%=_lblk:\n"
-      "s_lshl_b32 %[blk8], %[k], 8\n"
+      "s_mov_b32 %[blk8], 0\n"
       "s_bitcmp1_b32 %[r0], 1\n"
       "s_cbranch_scc0 L%=_tb\n"
       // intra luma mode of block k (4 bits in decode order)
+      "L%=_lmdec:\n"
       MB_DEC("%[vm1]", "0", "-1", "0x100000", "%[sym]", "lm", "unused")
       "s_and_b32 %[t1], %[k], 7\n"
       "s_lshl_b32 %[t1], %[t1], 2\n"
@@ -1285,8 +1282,9 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       "s_mov_b32 %[firstsi], 0\n"
       "v_mov_b32 %[vcur], %[va]\n"
       MBP_TOKLOOP
-      // escape (magnitude 5 + getEscapeValue, mv.go:124-128) or end of block
-      "L%=_tescob:\n"
+      // end of block or escape (magnitude 5 + getEscapeValue,
+      // mv.go:124-128), at phase 0
+      "L%=_nesc0:\n"
       "s_cmp_eq_u32 %[cnt], 7\n"
       "s_cbranch_scc1 L%=_teob\n"
       MB_CLASS("%[vm2]", "0", "-1", "0x100000", "es")
@@ -1318,11 +1316,16 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       MB_EMIT("e2")
       "L%=_tbdone:\n"
       "s_lshr_b32 %[t1], %[blk8], 8\n"
-      "s_lshl_b32 %[t1], 1, %[t1]\n"
-      "s_or_b32 %[wmask], %[wmask], %[t1]\n"
+      "s_bitset1_b32 %[wmask], %[t1]\n"
       "s_add_u32 %[k], %[k], 1\n"
       "s_cmp_lt_u32 %[k], %[nb]\n"
-      "s_cbranch_scc1 L%=_lblk\n"
+      "s_cbranch_scc0 L%=_tbluma\n"
+      // the next luma block: inter straight to its tokens, intra to its mode
+      "s_lshl_b32 %[blk8], %[k], 8\n"
+      "s_bitcmp1_b32 %[r0], 1\n"
+      "s_cbranch_scc0 L%=_tb\n"
+      "s_branch L%=_lmdec\n"
+      "L%=_tbluma:\n"
       "s_cmp_eq_u32 %[k], %[nb]\n"
       "s_cbranch_scc0 L%=_chnext\n"
       // luma complete: the chroma mode (intra) or the uncoded chroma planes
