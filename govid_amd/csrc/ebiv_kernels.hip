@@ -3060,6 +3060,49 @@ __device__ __forceinline__ void recon_intra_tile(const uint8_t* __restrict__ des
     if (n != 0xFFFFFFFFu) return;
     // (otherwise every anti-diagonal below, untracked)
   }
+  if (keyf) {
+    // A key frame's tile as a dataflow over its macroblock rows instead of a
+    // barrier per anti-diagonal: wavefront w owns rows w, w + W, ... and
+    // visits their macroblocks in anti-diagonal order (x + 2y); macroblock
+    // (x, y) waits only until row y - 1 has completed through column x + 1
+    // (its above and above-right neighbours; left and above-left are older in
+    // its own row and the row above), read from a per-row counter in LDS.
+    // A slow macroblock (16 serial 4x4 blocks) then delays its dependants
+    // only, not every macroblock of its anti-diagonal.  Every wait is for a
+    // macroblock of a lower anti-diagonal, and each wavefront goes through
+    // its macroblocks in that order, so every wait ends.  The frontier
+    // entries a macroblock reads were written by the macroblocks it waits
+    // for, and the ones it overwrites are read only by macroblocks that
+    // complete before it (its own row, or the row below's left part, which
+    // it waits for through row y - 1 when it is itself in row y + 1).
+    __shared__ int s_rowdone[kFrontMax];
+    for (int i = threadIdx.x; i < th; i += blockDim.x) s_rowdone[i] = 0;
+    __syncthreads();
+    volatile int* rowdone = s_rowdone;
+    for (int s = 0; s < steps; s++) {
+      for (int ry = wave; ry < th; ry += nwaves) {
+        const int rx = s - 2 * ry;
+        if (rx < 0 || rx >= tw) continue;
+        if (ry > 0) {
+          const int need = rx + 2 < tw ? rx + 2 : tw;
+          while (rowdone[ry - 1] < need) __builtin_amdgcn_s_sleep(1);
+          asm volatile("" ::: "memory");
+        }
+        const uint32_t local = static_cast<uint32_t>(ry * tw + rx);
+        if (local < st.mbs_done) {
+          const int mbx = td.x0 + rx, mby = td.y0 + ry;
+          const MBRec r = recs[mby * mbc + mbx];
+          if (r.kind == 2)
+            intra_mb(scratch[wave], res + static_cast<size_t>(mby * mbc + mbx) * kResPerMB, r, td, rec, mbx, mby,
+                     &front, lane);
+        }
+        // the frontier writes land before the row's count moves on
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        rowdone[ry] = rx + 1;
+      }
+    }
+    return;
+  }
   if (tracked) {
     for (uint32_t i = threadIdx.x; i < st.mbs_done; i += blockDim.x) {
       const int ry = static_cast<int>(i) / tw, rx = static_cast<int>(i) - ry * tw;
