@@ -277,6 +277,36 @@ def run_plain(packets, dev, steps, warmup):
             "symbols": s1["symbols"] - s0["symbols"]}
 
 
+def tile_report(packets, dev):
+    """Per frame of the clip (one warm-up pass first): the entropy kernel's
+    time (HIP events), each tile's rANS symbol count (its status) and the
+    macroblock kinds, so that the slowest tile's chain can be set against the
+    frame's parse time."""
+    import numpy as np
+    from govid_amd.codec import Codec
+
+    codec = Codec(ring=3)
+    for p, d in zip(packets, dev):
+        codec.decode_device(p, d.ptr)
+    codec.set_timing(True)
+    rows = []
+    for i, (p, d) in enumerate(zip(packets, dev)):
+        t0 = codec.kernel_times()
+        codec.decode_device(p, d.ptr)
+        t1 = codec.kernel_times()
+        recs, _, st = codec.debug_dump(0)
+        nt = int(np.count_nonzero(st["mbs_done"]))
+        nsym = [int(x) for x in st["nsym"][:nt]]
+        kinds = np.bincount(recs["kind"], minlength=4).tolist()
+        rows.append({"frame": i, "parse_us": round((t1["parse_ms"] - t0["parse_ms"]) * 1e3, 2),
+                     "intra_us": round((t1["intra_ms"] - t0["intra_ms"]) * 1e3, 2),
+                     "inter_us": round((t1["inter_ms"] - t0["inter_ms"]) * 1e3, 2),
+                     "max_tile_symbols": max(nsym) if nsym else 0, "symbols": sum(nsym),
+                     "skip": kinds[0], "inter": kinds[1], "intra": kinds[2]})
+    codec.close()
+    return rows
+
+
 def run_plain_host(packets, steps):
     """The clip through ebiv_codec_decode: host packets in, host planes out."""
     from govid_amd.codec import Codec
@@ -556,6 +586,8 @@ def main():
                     help="frames per session per engine step when serving (1 keeps frame latency lowest)")
     ap.add_argument("--no-pipeline", action="store_true", help="batch steps one at a time (A/B of the pipeline)")
     ap.add_argument("--schedule", default="", help="engine schedule fields, e.g. intra_waves_key=4,recon_lanes=2")
+    ap.add_argument("--tile-report", action="store_true",
+                    help="only report, per frame of the 1080p clip, the entropy kernel time and the tiles' symbol counts")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 2 GiB device-to-device copy (scale for HBM byte counters)")
     args = ap.parse_args()
@@ -600,6 +632,10 @@ def main():
     log(f"bench: {len(packets)} packets, {sum(map(len, packets))} bytes, {W}x{H}")
 
     dev = [DeviceBuffer(p) for p in packets]
+    if args.tile_report:
+        for row in tile_report(packets, dev):
+            print(json.dumps(row))
+        return
     if args.batched_only:
         bt = run_batched(packets, dev, want, args.sessions, args.window, args.steps, args.warmup)
         # symbols_all: every symbol the process decoded (warm-up included), the
