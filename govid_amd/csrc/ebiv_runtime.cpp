@@ -36,6 +36,7 @@
 #include <atomic>
 #include <condition_variable>
 #include <functional>
+#include <map>
 #include <mutex>
 #include <thread>
 #include <string>
@@ -124,6 +125,69 @@ int ensure_device(int dev) {
 
 inline size_t align256(size_t v) { return (v + 255) & ~static_cast<size_t>(255); }
 
+// Blocks of device memory (per device) and pinned host memory that sessions
+// released, kept by size for the next session of the same geometry.  A
+// serving loop opens and closes sessions all the time; each new session
+// would otherwise pay hipMalloc / hipHostMalloc for its frame pool and
+// payload buffers inside the step that decodes its first frame, and each
+// closed one hipFree, which waits for the device.  Released blocks are kept
+// up to a cap (beyond it they are freed) and held until the process ends.
+class BlockCache {
+ public:
+  void* get(bool pinned, size_t bytes) {
+    const int dev = pinned ? -1 : current_device();
+    {
+      std::lock_guard<std::mutex> g(m_);
+      auto it = blocks_.find(Key{dev, bytes});
+      if (it != blocks_.end()) {
+        void* p = it->second;
+        blocks_.erase(it);
+        (pinned ? pinned_bytes_ : device_bytes_) -= bytes;
+        return p;
+      }
+    }
+    void* p = nullptr;
+    const hipError_t rc = pinned ? hipHostMalloc(&p, bytes, hipHostMallocDefault) : hipMalloc(&p, bytes);
+    return rc == hipSuccess ? p : nullptr;
+  }
+  void put(bool pinned, void* p, size_t bytes) {
+    if (!p) return;
+    const int dev = pinned ? -1 : current_device();
+    {
+      std::lock_guard<std::mutex> g(m_);
+      size_t& held = pinned ? pinned_bytes_ : device_bytes_;
+      if (held + bytes <= (pinned ? kMaxPinned : kMaxDevice)) {
+        blocks_.emplace(Key{dev, bytes}, p);
+        held += bytes;
+        return;
+      }
+    }
+    (void)(pinned ? hipHostFree(p) : hipFree(p));
+  }
+
+ private:
+  static constexpr size_t kMaxDevice = size_t(8) << 30;  // per process, all devices
+  static constexpr size_t kMaxPinned = size_t(1) << 30;
+  struct Key {
+    int dev;
+    size_t bytes;
+    bool operator<(const Key& o) const { return dev != o.dev ? dev < o.dev : bytes < o.bytes; }
+  };
+  static int current_device() {
+    int d = 0;
+    (void)hipGetDevice(&d);
+    return d;
+  }
+  std::mutex m_;
+  std::multimap<Key, void*> blocks_;
+  size_t device_bytes_ = 0, pinned_bytes_ = 0;
+};
+
+BlockCache& block_cache() {
+  static BlockCache* c = new BlockCache();  // never destroyed: its blocks live until the process ends
+  return *c;
+}
+
 template <typename T>
 struct DevArray {
   T* d = nullptr;
@@ -131,11 +195,16 @@ struct DevArray {
   bool reserve(size_t n) {
     if (n <= cap) return true;
     release();
-    if (hipMalloc(&d, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) { d = nullptr; return false; }
+    d = static_cast<T*>(block_cache().get(false, std::max<size_t>(n, 1) * sizeof(T)));
+    if (!d) return false;
     cap = n;
     return true;
   }
-  void release() { if (d) (void)hipFree(d); d = nullptr; cap = 0; }
+  void release() {
+    if (d) block_cache().put(false, d, std::max<size_t>(cap, 1) * sizeof(T));
+    d = nullptr;
+    cap = 0;
+  }
   ~DevArray() { release(); }
 };
 
@@ -146,11 +215,16 @@ struct PinnedArray {
   bool reserve(size_t n) {
     if (n <= cap) return true;
     release();
-    if (hipHostMalloc(&h, std::max<size_t>(n, 1) * sizeof(T), hipHostMallocDefault) != hipSuccess) { h = nullptr; return false; }
+    h = static_cast<T*>(block_cache().get(true, std::max<size_t>(n, 1) * sizeof(T)));
+    if (!h) return false;
     cap = n;
     return true;
   }
-  void release() { if (h) (void)hipHostFree(h); h = nullptr; cap = 0; }
+  void release() {
+    if (h) block_cache().put(true, h, std::max<size_t>(cap, 1) * sizeof(T));
+    h = nullptr;
+    cap = 0;
+  }
   ~PinnedArray() { release(); }
 };
 
@@ -221,7 +295,7 @@ struct Session : HostCodec {
   }
 
   void release_pool() {
-    for (auto& b : pool) if (b.base) (void)hipFree(b.base);
+    for (auto& b : pool) if (b.base) block_cache().put(false, b.base, b.bytes);
     pool.clear();
   }
 
@@ -237,20 +311,18 @@ struct Session : HostCodec {
     pool.assign(npool, DevBuf{});
     for (auto& b : pool) {
       b.bytes = ysz + 2 * csz;
-      if (hipMalloc(&b.base, b.bytes) != hipSuccess) {
-        b.base = nullptr;
+      b.base = static_cast<uint8_t*>(block_cache().get(false, b.bytes));
+      if (!b.base) {
         release_pool();
         return fail(EBIV_ERR_DEVICE, "frame pool allocation failed");
       }
       b.p = Planes{b.base, b.base + ysz, b.base + ysz + csz, pw, ph, cw, ch};
+      // (the clear is ordered before every use of the pool: reconstruction,
+      // copies and presentation all run on this stream or join it)
       if (hipMemsetAsync(b.base, 0, b.bytes, stream) != hipSuccess) {
         release_pool();
         return fail(EBIV_ERR_DEVICE, "frame pool clear failed");
       }
-    }
-    if (hipStreamSynchronize(stream) != hipSuccess) {
-      release_pool();
-      return fail(EBIV_ERR_DEVICE, "frame pool clear failed");
     }
     ring.assign(static_cast<size_t>(ring_size), -1);
     ref = 0;

@@ -110,6 +110,8 @@ class Server:
         self.peak_bytes = 0
         self.waits: list = []        # admission wait per request, seconds
         self.first_frame: list = []  # arrival to first completed frame, seconds
+        self.step_times: list = []   # wall time of each engine step, seconds
+        self.feed_times: list = []   # admission and packet submission before each step, seconds
 
     class _Run:
         def __init__(self, req, sid, nbytes, order):
@@ -183,8 +185,11 @@ class Server:
                     time.sleep(max(0.0, min(0.01, pending[k].arrival - now)))
                 continue
             self._feed(now)
+            t_step = self.clock()
             out = self.engine.step()
             steps += 1
+            self.step_times.append(self.clock() - t_step)
+            self.feed_times.append(t_step - t0 - now)
             t_done = self.clock() - t0
             finished = []
             for c in out:
@@ -218,6 +223,8 @@ class Server:
                 "admission_wait_p99_ms": pct(self.waits, 0.99) * 1e3,
                 "first_frame_p50_ms": pct(self.first_frame, 0.5) * 1e3,
                 "first_frame_p99_ms": pct(self.first_frame, 0.99) * 1e3, "steps": steps,
+                "step_p50_ms": pct(self.step_times, 0.5) * 1e3, "step_p99_ms": pct(self.step_times, 0.99) * 1e3,
+                "admit_feed_p50_ms": pct(self.feed_times, 0.5) * 1e3,
                 "requests": len(pending), "max_active_sessions": self.max_active,
                 "peak_session_bytes": self.peak_bytes, "errors": sum(1 for f in self.frames if f.status)}
 
