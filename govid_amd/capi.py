@@ -45,7 +45,8 @@ class IndexEntry(C.Structure):
 class Schedule(C.Structure):
     _fields_ = [("intra_waves_key", C.c_int32), ("intra_waves_few", C.c_int32),
                 ("intra_waves_many", C.c_int32), ("many_tiles", C.c_int32), ("recon_lanes", C.c_int32),
-                ("pipeline", C.c_int32), ("intra_waves_key_many", C.c_int32), ("intra_cells", C.c_int32)]
+                ("pipeline", C.c_int32), ("intra_waves_key_many", C.c_int32), ("intra_cells", C.c_int32),
+                ("key_band_rows", C.c_int32)]
 
 
 class Frame(C.Structure):

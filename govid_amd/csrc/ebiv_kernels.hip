@@ -1765,6 +1765,11 @@ __device__ __forceinline__ void parse_tile_asm(const uint8_t* __restrict__ pkt, 
 }
 
 // One tile: its job's packet and descriptor, the entropy outputs.
+// The band counters after a payload's residuals (key_bands).
+__device__ __forceinline__ uint32_t* band_sync(int16_t* res, int mbc, int mbr) {
+  return reinterpret_cast<uint32_t*>(res + static_cast<size_t>(mbc) * mbr * kResPerMB);
+}
+
 __device__ __forceinline__ void parse_item(const FrameJob* __restrict__ jobs, const WorkItem it) {
   ConstJob& job = job_of(jobs, it.job);
   const FrameHdr* hdr = reinterpret_cast<const FrameHdr*>(job.desc);
@@ -1778,6 +1783,15 @@ __device__ __forceinline__ void parse_item(const FrameJob* __restrict__ jobs, co
   else if (prio == 1)
     __builtin_amdgcn_s_setprio(1);
   const int t = static_cast<int>(it.idx & 0x3FFFFFFFu);
+  if (hdr->inter == 0) {  // the tile's band counters for the intra launch
+    const TileDesc td = reinterpret_cast<const TileDesc*>(job.desc + desc_tiles_off())[t];
+    uint32_t* sync = band_sync(job.res, job.mbc, job.mbr);
+    const int th = td.y1 - td.y0;
+    for (int i0 = 0; i0 < th; i0 += 64) {  // (a uniform loop: the tile decoder's operands stay scalar)
+      const int i = i0 + lane_id();
+      if (i < th) sync[(td.y0 + i) * job.mbc + td.x0] = 0;
+    }
+  }
   if (hdr->vt[kSignVec][kSignLane] != kNoSym)
     parse_tile_asm(job.pkt, job.desc, job.recs, job.coefs, job.status, t);
   else
@@ -2379,35 +2393,102 @@ __device__ void prep_angular(const int32_t* ref, int32_t* ext, int32_t* dcout, i
 // neighbours (left, above-left, above, above-right) are exactly these
 // entries when it runs, and no macroblock of the same step writes an entry
 // another one of that step reads.
-constexpr int kFrontMax = 128;  // tile width and height (MBs) the LDS frontier covers
+// (kFrontMax: the tile width and height in MBs the LDS frontier covers)
 struct Frontier {
   uint8_t bot[kFrontMax][32];        // Y[16], Cb[8], Cr[8] of the bottom row
   uint8_t right[kFrontMax][32];      // Y[16], Cb[8], Cr[8] of the right column
   uint8_t corner[2][kFrontMax + 1][4];  // [row & 1][column]: Y, Cb, Cr
 };
 
-// res: the macroblock's residual as the inter launch formed it.
-__device__ __forceinline__ void intra_mb(IntraScratch& S, const int16_t* __restrict__ res, const MBRec& r, const TileDesc& td,
-                         Planes rec, int mbx, int mby, Frontier* fr, int lane) {
+// Device-coherent word access (relaxed, agent scope: the L2 of another XCD
+// holds no stale copy of what these read or write).
+__device__ __forceinline__ uint32_t coherent_load(const uint8_t* p) {
+  return __hip_atomic_load(reinterpret_cast<const uint32_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void coherent_store(uint8_t* p, uint32_t v) {
+  __hip_atomic_store(reinterpret_cast<uint32_t*>(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+constexpr int kEdgeIn = 1;   // intra_mb: the row above from the frame (a band's first row)
+constexpr int kEdgeOut = 2;  // intra_mb: the bottom rows also device-coherent (a band's last row)
+
+// A macroblock record gathered from lanes 0-11 of v (one word each), for
+// records loaded ahead by vector loads (which LDS waits do not hold up).
+__device__ __forceinline__ MBRec rec_of_lanes(uint32_t v) {
+  MBRec r;
+  const uint32_t a = __builtin_amdgcn_readlane(v, 0), b = __builtin_amdgcn_readlane(v, 1);
+  r.kind = static_cast<uint8_t>(a);
+  r.part = static_cast<uint8_t>(a >> 8);
+  r.cbp = static_cast<uint8_t>(a >> 16);
+  r.tx = static_cast<uint8_t>(a >> 24);
+  r.refsel = static_cast<uint8_t>(b);
+  r.cmode = static_cast<uint8_t>(b >> 8);
+  r.pad0 = static_cast<uint8_t>(b >> 16);
+  r.pad1 = static_cast<uint8_t>(b >> 24);
+  r.wmask = __builtin_amdgcn_readlane(v, 2);
+  r.coef_off = __builtin_amdgcn_readlane(v, 3);
+  r.coef_count = __builtin_amdgcn_readlane(v, 4);
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const uint32_t w = __builtin_amdgcn_readlane(v, 5 + k);
+    r.mv[k][0] = static_cast<int16_t>(w & 0xFFFFu);
+    r.mv[k][1] = static_cast<int16_t>(w >> 16);
+  }
+  r.lmode[0] = __builtin_amdgcn_readlane(v, 9);
+  r.lmode[1] = __builtin_amdgcn_readlane(v, 10);
+  r.pad2 = __builtin_amdgcn_readlane(v, 11);
+  return r;
+}
+
+// w0-w2: this lane's words of the macroblock's residual as the inter launch
+// formed it (words lane, lane + 64, lane + 128; zero without coefficients).
+// edge (key-frame bands, with fr): kEdgeIn, kEdgeOut.
+__device__ __forceinline__ void intra_mb_w(IntraScratch& S, const uint32_t w0, const uint32_t w1, const uint32_t w2,
+                                           const MBRec& r, const TileDesc& td, Planes rec, int mbx, int mby,
+                                           Frontier* fr, int lane, int edge = 0) {
   const int n = 4 << (r.tx > 2 ? 2 : r.tx);
   WaveScratch& ws = S.ws;
-  if (r.coef_count > 0) {
-    const uint32_t* src = reinterpret_cast<const uint32_t*>(res);
+  {
+    const uint32_t w[3] = {w0, w1, w2};
 #pragma unroll
     for (int k = 0; k < kResPerMB / 128; k++) {
-      const uint32_t w = src[lane + 64 * k];
       const int i = 2 * (lane + 64 * k);
-      ws.coef[i] = static_cast<int16_t>(w & 0xFFFFu);
-      ws.coef[i + 1] = static_cast<int16_t>(w >> 16);
+      ws.coef[i] = static_cast<int16_t>(w[k] & 0xFFFFu);
+      ws.coef[i + 1] = static_cast<int16_t>(w[k] >> 16);
     }
-  } else {
-    for (int i = lane; i < 384; i += 64) ws.coef[i] = 0;
   }
   // Load the context: luma row above (x from -1 to 31) and left column,
   // chroma likewise; from the frontier in a key frame, else from the frame
   // (its inter macroblocks are final, its intra ones older in this launch).
   const int X = mbx * 16, Y = mby * 16;
-  if (fr) {
+  if (fr && (edge & kEdgeIn)) {
+    // The row above as the band above published it: whole words around the
+    // bytes needed (luma x from X - 4, chroma from CX - 4; Y >= 16, so every
+    // word lies inside the plane); the left column from the frontier.
+    const int rx = mbx - td.x0, ry = mby - td.y0, tw = td.x1 - td.x0;
+    const int CX = mbx * 8, CY = mby * 8;
+    if (lane < 9) {
+      const uint32_t w = coherent_load(rec.y + static_cast<size_t>(Y - 1) * rec.pw + X - 4 + 4 * lane);
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int i = 4 * lane - 3 + j;  // S.L[0][i] is x = X - 1 + i
+        if (i >= 0) S.L[0][i] = (i >= 17 && rx + 1 >= tw) ? 0 : static_cast<uint8_t>(w >> (8 * j));
+      }
+    } else if (lane >= 33 && lane < 49) {
+      S.L[lane - 32][0] = fr->right[ry][lane - 33];
+    }
+    const int pl = lane >> 5, l = lane & 31;
+    if (l < 3) {
+      const uint8_t* cp = pl ? rec.cr : rec.cb;
+      const uint32_t w = coherent_load(cp + static_cast<size_t>(CY - 1) * rec.cw + CX - 4 + 4 * l);
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int i = 4 * l - 3 + j;
+        if (i >= 0) S.C[pl][0][i] = static_cast<uint8_t>(w >> (8 * j));
+      }
+    } else if (l >= 9 && l < 17) {
+      S.C[pl][l - 8][0] = fr->right[ry][16 + 8 * pl + l - 9];
+    }
+  } else if (fr) {
     const int rx = mbx - td.x0, ry = mby - td.y0, tw = td.x1 - td.x0;
     if (lane < 33) {
       uint8_t v = 0;
@@ -2514,6 +2595,27 @@ __device__ __forceinline__ void intra_mb(IntraScratch& S, const int16_t* __restr
     }
   }
   wave_sync();
+  if (edge & kEdgeOut) {
+    // the bottom rows once more, device-coherent, for the band below (the
+    // same words as above, under the same write mask)
+    if (lane < 4) {
+      const int blk = (15 / n) * per + (4 * lane) / n;
+      if ((r.wmask >> blk) & 1u) {
+        uint32_t w = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) w |= static_cast<uint32_t>(S.L[16][4 * lane + k + 1]) << (8 * k);
+        coherent_store(rec.y + static_cast<size_t>(Y + 15) * rec.pw + X + 4 * lane, w);
+      }
+    } else if (lane < 8) {
+      const int pl = (lane - 4) >> 1, k2 = lane & 1;
+      if ((r.wmask >> (16 + pl)) & 1u) {
+        uint32_t w = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) w |= static_cast<uint32_t>(S.C[pl][8][4 * k2 + k + 1]) << (8 * k);
+        coherent_store((pl ? rec.cr : rec.cb) + static_cast<size_t>(mby * 8 + 7) * rec.cw + mbx * 8 + 4 * k2, w);
+      }
+    }
+  }
   if (fr) {  // this macroblock's edges for its right and lower neighbours
     const int rx = mbx - td.x0, ry = mby - td.y0;
     if (lane < 16) {
@@ -2529,6 +2631,19 @@ __device__ __forceinline__ void intra_mb(IntraScratch& S, const int16_t* __restr
     }
     wave_sync();
   }
+}
+
+// res: the macroblock's residual as the inter launch formed it.
+__device__ __forceinline__ void intra_mb(IntraScratch& S, const int16_t* __restrict__ res, const MBRec& r,
+                                         const TileDesc& td, Planes rec, int mbx, int mby, Frontier* fr, int lane) {
+  uint32_t w0 = 0, w1 = 0, w2 = 0;
+  if (r.coef_count > 0) {
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(res);
+    w0 = src[lane];
+    w1 = src[lane + 64];
+    w2 = src[lane + 128];
+  }
+  intra_mb_w(S, w0, w1, w2, r, td, rec, mbx, mby, fr, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -2941,11 +3056,15 @@ constexpr int kMaxStepWords = 16;  // steps tracked individually (512)
 // (in tiles of at most kFrontMax x kFrontMax macroblocks) are listed with
 // their dependency level.
 constexpr int kMaxIntraList = 64;
+// Polls of a band's published count before its wavefront gives up (about
+// 0.1 s: a band waits at most for the bands above to fill their pipeline).
+constexpr uint32_t kBandSpinMax = 1u << 16;
 constexpr int kIntraBitWords = kFrontMax * kFrontMax / 32 + 1;
 __device__ __forceinline__ void recon_intra_tile(const uint8_t* __restrict__ desc, const MBRec* __restrict__ recs,
                                                  const int16_t* __restrict__ res,
                                                  const TileStatus* __restrict__ status, const Planes rec, const int t,
-                                                 const bool with_front, const bool with_cells) {
+                                                 const bool with_front, const bool with_cells, uint32_t* sync,
+                                                 const int band_rows) {
   extern __shared__ __attribute__((aligned(16))) uint8_t intra_dyn[];
   __shared__ uint32_t step_mask[kMaxStepWords];
   __shared__ uint32_t s_bits[kIntraBitWords];  // intra macroblocks of the tile (local index)
@@ -3075,31 +3194,106 @@ __device__ __forceinline__ void recon_intra_tile(const uint8_t* __restrict__ des
     // for, and the ones it overwrites are read only by macroblocks that
     // complete before it (its own row, or the row below's left part, which
     // it waits for through row y - 1 when it is itself in row y + 1).
+    //
+    // A tall tile (key_bands > 1) is split into bands of band_rows rows, one
+    // workgroup each; each workgroup claims the next band of its tile from
+    // the tile's counter, so a band's workgroup only ever waits for one that
+    // runs already (it claimed earlier).  A band's first row takes the row
+    // above from the frame, waiting for the band above to have published
+    // its last row through column x + 1: that row stores its bottom samples
+    // device-coherently, waits for them, then moves the published count.
+    // The wait is bounded: a count that never moved (which the zeroed
+    // counters and the item count exclude) would leave wrong samples in
+    // the band's rows rather than hold the launch.
     __shared__ int s_rowdone[kFrontMax];
+    __shared__ uint32_t s_band;
+    const int nb = key_bands(tw, th, band_rows);
+    int by0 = 0, by1 = th;
+    if (nb > 1) {
+      if (threadIdx.x == 0) s_band = atomicAdd(&sync[td.y0 * mbc + td.x0], 1u);
+      __syncthreads();
+      const int band = static_cast<int>(s_band);
+      if (band >= nb) return;
+      by0 = band * band_rows;
+      by1 = by0 + band_rows < th ? by0 + band_rows : th;
+    }
+    const uint32_t* prog_in = nb > 1 && by0 > 0 ? &sync[(td.y0 + by0) * mbc + td.x0] : nullptr;
+    uint32_t* prog_out = nb > 1 && by1 < th ? &sync[(td.y0 + by1) * mbc + td.x0] : nullptr;
     for (int i = threadIdx.x; i < th; i += blockDim.x) s_rowdone[i] = 0;
     __syncthreads();
     volatile int* rowdone = s_rowdone;
-    for (int s = 0; s < steps; s++) {
-      for (int ry = wave; ry < th; ry += nwaves) {
-        const int rx = s - 2 * ry;
-        if (rx < 0 || rx >= tw) continue;
-        if (ry > 0) {
-          const int need = rx + 2 < tw ? rx + 2 : tw;
-          while (rowdone[ry - 1] < need) __builtin_amdgcn_s_sleep(1);
-          asm volatile("" ::: "memory");
+    bool lost = false;  // this wavefront gave up on a published count
+    const int bsteps = (tw - 1) + 2 * (by1 - by0 - 1) + 1;
+    // The wavefront's macroblocks in the order above: step s, then its rows
+    // ry = by0 + wave + k * nwaves with rx = s - 2 * (ry - by0) in the tile.
+    // The next one's record and residual are loaded while this one runs.
+    auto advance = [&](int& s, int& ry) -> bool {
+      if (by0 + wave >= by1) return false;  // (a wavefront with no row in the band)
+      for (;;) {
+        ry += nwaves;
+        if (ry >= by1) {
+          ry = by0 + wave;
+          if (++s >= bsteps) return false;
         }
-        const uint32_t local = static_cast<uint32_t>(ry * tw + rx);
-        if (local < st.mbs_done) {
-          const int mbx = td.x0 + rx, mby = td.y0 + ry;
-          const MBRec r = recs[mby * mbc + mbx];
-          if (r.kind == 2)
-            intra_mb(scratch[wave], res + static_cast<size_t>(mby * mbc + mbx) * kResPerMB, r, td, rec, mbx, mby,
-                     &front, lane);
-        }
-        // the frontier writes land before the row's count moves on
-        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-        rowdone[ry] = rx + 1;
+        const int rx = s - 2 * (ry - by0);
+        if (rx >= 0 && rx < tw) return true;
       }
+    };
+    uint32_t vrec = 0, w0 = 0, w1 = 0, w2 = 0;
+    auto fetch = [&](int s, int ry) {
+      const int mb = (td.y0 + ry) * mbc + td.x0 + s - 2 * (ry - by0);
+      const uint32_t* rp = reinterpret_cast<const uint32_t*>(recs + mb);
+      vrec = lane < 12 ? rp[lane] : 0u;
+      const uint32_t* src = reinterpret_cast<const uint32_t*>(res + static_cast<size_t>(mb) * kResPerMB);
+      w0 = src[lane];
+      w1 = src[lane + 64];
+      w2 = src[lane + 128];
+    };
+    int s = 0, ry = by0 + wave - nwaves;
+    bool have = advance(s, ry);
+    if (have) fetch(s, ry);
+    while (have) {
+      const int rx = s - 2 * (ry - by0);
+      const uint32_t cr = vrec, c0 = w0, c1 = w1, c2 = w2;
+      int s2 = s, ry2 = ry;
+      const bool have2 = advance(s2, ry2);
+      if (have2) fetch(s2, ry2);
+      const int need = rx + 2 < tw ? rx + 2 : tw;
+      int edge = 0;
+      if (ry > by0) {
+        while (rowdone[ry - 1] < need) __builtin_amdgcn_s_sleep(1);
+        asm volatile("" ::: "memory");
+      } else if (prog_in) {
+        uint32_t spins = 0;
+        while (!lost && __hip_atomic_load(prog_in, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+                            static_cast<uint32_t>(need)) {
+          __builtin_amdgcn_s_sleep(2);
+          if (++spins >= kBandSpinMax) lost = true;
+        }
+        asm volatile("" ::: "memory");
+        edge = kEdgeIn;
+      }
+      if (ry == by1 - 1 && prog_out) edge |= kEdgeOut;
+      const uint32_t local = static_cast<uint32_t>(ry * tw + rx);
+      if (local < st.mbs_done) {
+        const MBRec r = rec_of_lanes(cr);
+        if (r.kind == 2) {
+          const bool z = r.coef_count == 0;  // (then the residual words were never written)
+          intra_mb_w(scratch[wave], z ? 0u : c0, z ? 0u : c1, z ? 0u : c2, r, td, rec, td.x0 + rx, td.y0 + ry,
+                     &front, lane, edge);
+        }
+      }
+      if (edge & kEdgeOut) {
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        if (lane == 0) __hip_atomic_store(prog_out, static_cast<uint32_t>(rx + 1), __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+      }
+      // the frontier writes land before the row's count moves on
+      asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+      rowdone[ry] = rx + 1;
+      s = s2;
+      ry = ry2;
+      have = have2;
     }
     return;
   }
@@ -3146,7 +3340,7 @@ __global__ __launch_bounds__(64 * kIntraMaxWaves) void ebiv_recon_intra(const Fr
   const WorkItem it = item_of(items);
   ConstJob& job = job_of(jobs, it.job);
   recon_intra_tile(job.desc, job.recs, job.res, job.status, planes_of(job.rec), static_cast<int>(it.idx),
-                   (flags & 1) != 0, (flags & 2) != 0);
+                   (flags & 1) != 0, (flags & 2) != 0, band_sync(job.res, job.mbc, job.mbr), flags >> 8);
 }
 
 // ============================================================================
@@ -3235,13 +3429,15 @@ void launch_recon_inter(hipStream_t s, int ngroups, const FrameJob* jobs, const 
 // which then use the LDS frontier.
 // cells: key-frame tiles and tiles dense with intra macroblocks take the
 // block-level path (its LDS overlays the other paths').
+// band_rows: key_bands of the key-frame tiles (the items hold one per band).
 void launch_recon_intra(hipStream_t s, int nitems, const FrameJob* jobs, const WorkItem* items, int waves, bool key,
-                        bool cells) {
+                        bool cells, int band_rows) {
   waves = waves < 1 ? 1 : (waves > kIntraMaxWaves ? kIntraMaxWaves : waves);
   size_t lds = static_cast<size_t>(waves) * sizeof(IntraScratch) + (key ? sizeof(Frontier) : 0);
   if (cells && cell_lds_bytes(waves) > lds) lds = cell_lds_bytes(waves);
+  band_rows = band_rows < 0 ? 0 : (band_rows > kFrontMax ? kFrontMax : band_rows);
   hipLaunchKernelGGL(ebiv_recon_intra, dim3(nitems), dim3(64 * waves), lds, s, jobs, items,
-                     (key ? 1 : 0) | (cells ? 2 : 0));
+                     (key ? 1 : 0) | (cells ? 2 : 0) | (band_rows << 8));
 }
 
 void launch_rgba(hipStream_t s, const RgbaJob& j) {

@@ -53,7 +53,7 @@ hipError_t upload_tables(const int32_t dct[3][256], const uint8_t zz[3][256]);
 void launch_parse(hipStream_t s, int nitems, int max_tile_w, const FrameJob* jobs, const WorkItem* items);
 void launch_recon_inter(hipStream_t s, int ngroups, const FrameJob* jobs, const WorkItem* items, int njobs);
 void launch_recon_intra(hipStream_t s, int nitems, const FrameJob* jobs, const WorkItem* items, int waves, bool key,
-                        bool cells);
+                        bool cells, int band_rows);
 void launch_rgba(hipStream_t s, const RgbaJob& j);
 void launch_rgba_frames(hipStream_t s, const RgbaJob* jobs, int njobs, uint32_t max_px8);
 
@@ -361,7 +361,8 @@ struct Session : HostCodec {
     // (the entropy kernel's own choice, parse_item)
     ps->generic = reinterpret_cast<const FrameHdr*>(hp.desc.data())->vt[kSignVec][kSignLane] == kNoSym;
     const size_t nmb = static_cast<size_t>(mbc) * mbr;
-    if (!ps->recs.reserve(nmb) || !ps->coefs.reserve(nmb * kCoefPerMB) || !ps->res.reserve(nmb * kResPerMB) ||
+    if (!ps->recs.reserve(nmb) || !ps->coefs.reserve(nmb * kCoefPerMB) ||
+        !ps->res.reserve(nmb * (kResPerMB + kResSyncPerMB)) ||
         !ps->h_status.reserve(static_cast<size_t>(hp.ntiles)))
       return fail(EBIV_ERR_DEVICE, "payload buffer allocation failed");
     return EBIV_OK;
@@ -399,6 +400,19 @@ struct Schedule {
   int recon_lanes = 1;        // HIP streams the sessions' reconstruction is split over
   bool pipeline = true;       // next step's entropy launch beside this step's reconstruction
   bool intra_cells = false;   // key-frame and intra-dense tiles by block levels (measured slower)
+  int key_band_rows = 4;      // macroblock rows per workgroup of a key-frame tile (key_bands)
+  // Band rows of a step's key-frame tiles: none on the block-level path
+  // (whole tiles), nor in a step of many_tiles key-frame tiles or more,
+  // which fill the chip by themselves (64 x 12 1080p key tiles: 132.7 ms of
+  // intra kernel time whole, 155.2 ms in bands of 4).
+  template <typename Step, typename PayloadOf>
+  int band_rows(const Step& S, PayloadOf payload_of) const {
+    if (intra_cells) return 0;
+    size_t key_tiles = 0;
+    for (const auto& jr : S.jrefs)
+      if (!payload_of(jr)->inter) key_tiles += static_cast<size_t>(payload_of(jr)->ntiles);
+    return key_tiles < static_cast<size_t>(many_tiles) ? key_band_rows : 0;
+  }
   int inter_intra_waves(size_t ntiles) const {
     return ntiles >= static_cast<size_t>(many_tiles) ? intra_waves_many : intra_waves_few;
   }
@@ -502,6 +516,8 @@ struct StepBuf {
   PinnedArray<uint8_t> h_stage;
   DevArray<uint8_t> d_stage;
   size_t job_off = 0, item_off = 0;
+  int band_rows = 0;    // key-frame band rows of this step's launches (Schedule::band_rows)
+  bool cells = false;   // this step's intra launches take the block-level path
   FrameJob* hj = nullptr;
   FrameJob* dj = nullptr;
   WorkItem* hi = nullptr;
@@ -775,13 +791,20 @@ int Engine::launch_entropy(StepBuf& S, hipStream_t ps, bool speculative) {
   int max_tw = 1;
   uint32_t max_tile_bytes = 1;
   auto tiles_of = [](const PayloadSlot* p) { return reinterpret_cast<const TileDesc*>(p->hdesc + desc_tiles_off()); };
+  // (fixed for the step: the band rows set the item count)
+  S.band_rows = sched.band_rows(S, [&](const JobRef& jr) { return S.payload(jr); });
+  S.cells = sched.intra_cells;
   for (const JobRef& jr : S.jrefs) {
     PayloadSlot* p = S.payload(jr);
     p->status_off = nparse;
     nparse += static_cast<size_t>(p->ntiles);
-    nitems += 2 * static_cast<size_t>(p->ntiles) + 1;  // entropy items, then reconstruction items
+    nitems += static_cast<size_t>(p->ntiles) + 1;  // entropy items, the inter item
     max_tw = std::max(max_tw, p->max_tile_w);
-    for (int t = 0; t < p->ntiles; t++) max_tile_bytes = std::max(max_tile_bytes, tiles_of(p)[t].byte_len);
+    for (int t = 0; t < p->ntiles; t++) {
+      const TileDesc& td = tiles_of(p)[t];
+      max_tile_bytes = std::max(max_tile_bytes, td.byte_len);
+      nitems += p->inter ? 1 : static_cast<size_t>(key_bands(td.x1 - td.x0, td.y1 - td.y0, S.band_rows));
+    }
   }
   S.nparse = nparse;
   S.max_tw = max_tw;
@@ -1030,8 +1053,14 @@ int Engine::plan_recon(StepBuf& S) {
       for (int pass = 0; pass < 2; pass++) {  // key-frame tiles, then inter-frame tiles
         for (size_t j : rounds[r])
           if (lane_of(S.jrefs[j].si) == g && is_key(j) == (pass == 0))
-            for (int t = 0; t < S.payload(S.jrefs[j])->ntiles; t++)
-              S.hi[w++] = WorkItem{static_cast<uint32_t>(j), static_cast<uint32_t>(t)};
+            for (int t = 0; t < S.payload(S.jrefs[j])->ntiles; t++) {
+              // a banded key-frame tile: one item per band, the same item
+              // (each workgroup claims its band on the device)
+              const uint8_t* hd = S.payload(S.jrefs[j])->hdesc;
+              const TileDesc& td = reinterpret_cast<const TileDesc*>(hd + desc_tiles_off())[t];
+              const int nb = pass == 0 ? key_bands(td.x1 - td.x0, td.y1 - td.y0, S.band_rows) : 1;
+              for (int b = 0; b < nb; b++) S.hi[w++] = WorkItem{static_cast<uint32_t>(j), static_cast<uint32_t>(t)};
+            }
         if (pass == 0) key_end[q] = w;
       }
       round_end[q] = w;
@@ -1110,15 +1139,16 @@ int Engine::launch_recon(StepBuf& S) {
       if (key_end[q] > intra_off[q]) {
         const int m = mark_begin(2, 0, st);
         const size_t n = key_end[q] - intra_off[q];
-        launch_recon_intra(st, static_cast<int>(n), S.dj, S.di + intra_off[q], sched.key_intra_waves(n),
-                           true, sched.intra_cells);
+        // (a banded tile's workgroup holds band_rows rows: no more waves)
+        const int kw = S.band_rows > 0 ? std::min(sched.key_intra_waves(n), S.band_rows) : sched.key_intra_waves(n);
+        launch_recon_intra(st, static_cast<int>(n), S.dj, S.di + intra_off[q], kw, true, S.cells, S.band_rows);
         mark_end(m, st);
       }
       if (round_end[q] > key_end[q]) {
         const size_t n = round_end[q] - key_end[q];
         const int m = mark_begin(2, 0, st);
         launch_recon_intra(st, static_cast<int>(n), S.dj, S.di + key_end[q], sched.inter_intra_waves(n),
-                           false, sched.intra_cells);
+                           false, S.cells, 0);
         mark_end(m, st);
       }
     }
@@ -1729,6 +1759,7 @@ int ebiv_batch_set_schedule(ebiv_batch* b, const ebiv_schedule* sc) {
   if (sc->pipeline > 0) d.pipeline = sc->pipeline == 1;
   if (sc->intra_cells > 2) return EBIV_ERR_ARGUMENT;
   if (sc->intra_cells > 0) d.intra_cells = sc->intra_cells == 1;
+  if (sc->key_band_rows > 0) d.key_band_rows = sc->key_band_rows > kFrontMax ? kFrontMax : sc->key_band_rows;
   return EBIV_OK;
 }
 
@@ -1736,7 +1767,8 @@ int ebiv_batch_get_schedule(const ebiv_batch* b, ebiv_schedule* out) {
   if (!b || !out) return EBIV_ERR_ARGUMENT;
   const Schedule& d = b->eng.sched;
   *out = ebiv_schedule{d.intra_waves_key, d.intra_waves_few, d.intra_waves_many, d.many_tiles, d.recon_lanes,
-                       d.pipeline ? 1 : 2, d.intra_waves_key_many, d.intra_cells ? 1 : 2};
+                       d.pipeline ? 1 : 2, d.intra_waves_key_many, d.intra_cells ? 1 : 2,
+                       d.key_band_rows};
   return EBIV_OK;
 }
 
@@ -1754,7 +1786,7 @@ size_t ebiv_session_bytes(int width, int height, int ring_size, int window) {
   // payload slots: a window's payloads plus an override, in each of the
   // engine's two step buffers (one reconstructing while the next parses)
   const size_t payload =
-      2 * (static_cast<size_t>(window) + 1) * mbc * mbr * (sizeof(MBRec) + kCoefPerMB * 4 + kResPerMB * 2);
+      2 * (static_cast<size_t>(window) + 1) * mbc * mbr * (sizeof(MBRec) + kCoefPerMB * 4 + (kResPerMB + kResSyncPerMB) * 2);
   return pool + payload;
 }
 

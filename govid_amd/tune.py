@@ -42,6 +42,7 @@ MANY_WAVES = (2, 4, 8)
 MANY_TILES = (128, 256, 512)
 RECON_LANES = (1, 2, 4)
 PIPELINE = (1, 2)  # next step's entropy launch beside this step's reconstruction, or not
+KEY_BAND_ROWS = (2, 4, 8, 128)  # rows per workgroup of a key-frame tile (128: whole tiles)
 
 
 def _crc(planes) -> int:
@@ -125,7 +126,8 @@ def tune(source, sessions: int = 64, window: int = 4, passes: int = 2, exhaustiv
         cands = [dict(d0)]
         for field, vals in (("intra_waves_key", KEY_WAVES), ("intra_waves_key_many", KEY_MANY_WAVES),
                             ("intra_waves_few", FEW_WAVES), ("intra_waves_many", MANY_WAVES),
-                            ("many_tiles", MANY_TILES), ("recon_lanes", RECON_LANES), ("pipeline", PIPELINE)):
+                            ("many_tiles", MANY_TILES), ("recon_lanes", RECON_LANES), ("pipeline", PIPELINE),
+                            ("key_band_rows", KEY_BAND_ROWS)):
             for v in vals:
                 c = dict(d0, **{field: v})
                 if c not in cands:

@@ -227,13 +227,20 @@ int ebiv_batch_frame_rgba(ebiv_batch *b, const ebiv_frame *f, uint8_t *dst, int 
  * intra_cells: 1 reconstructs key-frame tiles and tiles dense with intra
  * macroblocks by levels of blocks (tiles that fit: at most 64 x 64 and
  * 1024 macroblocks, and at most as many rows as wavefronts or fewer than
- * twice as many columns), 2 (default) by macroblock wavefronts.  A
+ * twice as many columns), 2 (default) by macroblock wavefronts.
+ * key_band_rows: macroblock rows per workgroup of a key-frame tile in a
+ * step of fewer than many_tiles key-frame tiles (default 4: a tall tile's
+ * rows spread over that many more CUs, each band waiting on the band above
+ * through device memory); 128 or more keeps one workgroup per tile.  A
+ * key-frame launch of banded tiles runs at most key_band_rows wavefronts
+ * per workgroup.  A
  * field <= 0 keeps its value.  Results do not depend on the schedule. */
 typedef struct {
     int32_t intra_waves_key, intra_waves_few, intra_waves_many, many_tiles, recon_lanes;
     int32_t pipeline;
     int32_t intra_waves_key_many;
     int32_t intra_cells;
+    int32_t key_band_rows;
 } ebiv_schedule;
 int ebiv_batch_set_schedule(ebiv_batch *b, const ebiv_schedule *s);
 int ebiv_batch_get_schedule(const ebiv_batch *b, ebiv_schedule *out);

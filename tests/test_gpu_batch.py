@@ -100,11 +100,17 @@ def test_sessions_of_mixed_geometry_and_windows():
     dict(intra_cells=1, intra_waves_key=16, intra_waves_key_many=16, intra_waves_few=16, intra_waves_many=16),
     dict(intra_cells=1, intra_waves_key=2, intra_waves_key_many=2, intra_waves_few=2, intra_waves_many=2,
          many_tiles=1),
+    dict(key_band_rows=1),
+    dict(key_band_rows=2, intra_waves_key=1, intra_waves_key_many=1),
+    dict(key_band_rows=2, recon_lanes=2, many_tiles=1000),
+    dict(key_band_rows=3, many_tiles=1),  # (a step of many key-frame tiles keeps them whole)
+    dict(key_band_rows=128),
 ])
 def test_launch_schedules_do_not_change_results(schedule):
     """Intra reconstruction with 1 to 16 wavefronts per tile (key frames from
     the LDS frontier, inter frames from the frame), by block levels or by
-    macroblock wavefronts, reconstruction split over 2 or 4 streams, and
+    macroblock wavefronts, key-frame tiles in bands of 1 or 2 rows or whole,
+    reconstruction split over 2 or 4 streams, and
     steps run one at a time instead of the next step's entropy launch beside
     this step's reconstruction, bit for bit."""
     streams = [
