@@ -107,13 +107,15 @@ constexpr int kCoefPerMB = 384;
 // [0, 255], so clamping the residual to [-32768, 32767] changes no sample).
 constexpr int kResPerMB = 384;
 
-// Key-frame tiles of at most kFrontMax x kFrontMax macroblocks reconstruct
-// as a dataflow over their macroblock rows with the context in LDS.  With
+// Tiles of at most kFrontMax x kFrontMax macroblocks whose macroblocks are
+// all intra (key frames, and inter frames coded all intra) reconstruct as a
+// dataflow over their macroblock rows with the context in LDS.  With
 // band_rows > 0 a taller tile is split into bands of band_rows rows, one
 // workgroup each, so that a big tile's rows spread over many CUs; a band's
 // first row then takes the row above from the frame, after the band above
 // has published it.  key_bands is the number of workgroups (intra launch
-// items) of such a tile; the host and the kernel both use it.
+// items) of such a tile; the host and the kernel both use it.  (An
+// inter-frame tile that turns out not to be all intra runs in the first.)
 constexpr int kFrontMax = 128;
 constexpr int key_bands(int tw, int th, int band_rows) {
   return band_rows > 0 && th > band_rows && tw <= kFrontMax && th <= kFrontMax ? (th + band_rows - 1) / band_rows
@@ -123,7 +125,7 @@ constexpr int key_bands(int tw, int th, int band_rows) {
 // residuals (kResPerMB each).  For a key-frame tile at columns [x0, x1),
 // rows [y0, y1): entry (y0, x0) counts the tile's claimed bands, entry
 // (y, x0) for y > y0 the macroblocks of row y - 1 published to the band
-// starting at row y.  The entropy kernel zeroes a key-frame tile's entries.
+// starting at row y.  The entropy kernel zeroes every tile's entries.
 constexpr int kResSyncPerMB = 2;  // int16 slots per macroblock that the counters take
 
 // Per-tile outcome of the entropy kernel.

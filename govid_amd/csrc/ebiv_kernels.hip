@@ -1783,7 +1783,7 @@ __device__ __forceinline__ void parse_item(const FrameJob* __restrict__ jobs, co
   else if (prio == 1)
     __builtin_amdgcn_s_setprio(1);
   const int t = static_cast<int>(it.idx & 0x3FFFFFFFu);
-  if (hdr->inter == 0) {  // the tile's band counters for the intra launch
+  {  // the tile's band counters for the intra launch
     const TileDesc td = reinterpret_cast<const TileDesc*>(job.desc + desc_tiles_off())[t];
     uint32_t* sync = band_sync(job.res, job.mbc, job.mbr);
     const int th = td.y1 - td.y0;
@@ -3055,6 +3055,32 @@ constexpr int kMaxStepWords = 16;  // steps tracked individually (512)
 // Inter frames: up to one wavefront's worth of intra macroblocks per tile
 // (in tiles of at most kFrontMax x kFrontMax macroblocks) are listed with
 // their dependency level.
+// The frontier entries of a macroblock that is already final in the frame
+// (an inter frame's skip or inter macroblock): its bottom row, right column
+// and bottom-right corner, as intra_mb leaves them for an intra one.
+__device__ __forceinline__ void front_from_frame(Frontier& fr, const Planes rec, const TileDesc& td, int mbx, int mby,
+                                                 int lane) {
+  const int rx = mbx - td.x0, ry = mby - td.y0, X = mbx * 16, Y = mby * 16;
+  const int g = lane >> 4, i = lane & 15;  // g 0: luma bottom, 1: luma right, 2: Cb, 3: Cr
+  if (g < 2) {
+    const uint8_t v = g == 0 ? rec.y[static_cast<size_t>(Y + 15) * rec.pw + X + i]
+                             : rec.y[static_cast<size_t>(Y + i) * rec.pw + X + 15];
+    if (g == 0) fr.bot[rx][i] = v;
+    else fr.right[ry][i] = v;
+    if (g == 0 && i == 15) fr.corner[(ry + 1) & 1][rx + 1][0] = v;
+  } else {
+    const int pl = g - 2, k = i & 7;
+    const uint8_t* cp = pl ? rec.cr : rec.cb;
+    const int CX = mbx * 8, CY = mby * 8;
+    const uint8_t v = i < 8 ? cp[static_cast<size_t>(CY + 7) * rec.cw + CX + k]
+                            : cp[static_cast<size_t>(CY + k) * rec.cw + CX + 7];
+    if (i < 8) fr.bot[rx][16 + 8 * pl + k] = v;
+    else fr.right[ry][16 + 8 * pl + k] = v;
+    if (i == 7) fr.corner[(ry + 1) & 1][rx + 1][1 + pl] = v;
+  }
+  wave_sync();
+}
+
 constexpr int kMaxIntraList = 64;
 // Polls of a band's published count before its wavefront gives up (about
 // 0.1 s: a band waits at most for the bands above to fill their pipeline).
@@ -3087,8 +3113,22 @@ __device__ __forceinline__ void recon_intra_tile(const uint8_t* __restrict__ des
     intra_cells(intra_dyn, recs, res, td, mbc, rec, st.mbs_done, lane, wave, nwaves);
     return;
   }
-  // A key frame's tile (every macroblock intra) keeps its context in LDS.
-  const bool keyf = with_front && hdr->inter == 0 && tw <= kFrontMax && th <= kFrontMax;
+  // A key frame's tile, and an inter frame's tile that is mostly intra,
+  // keeps its context in LDS (the inter macroblocks' edges come from the
+  // frame, where the inter launch left them final).
+  const bool keyf = with_front && (hdr->inter == 0 || 2 * st.n_intra > st.mbs_done) && tw <= kFrontMax &&
+                    th <= kFrontMax;
+  // A tile given several items (key_bands): each workgroup claims one of
+  // them; a tile that takes another path runs it in the first only.
+  const int nb = key_bands(tw, th, band_rows);
+  int band = 0;
+  if (nb > 1) {
+    __shared__ uint32_t s_band;
+    if (threadIdx.x == 0) s_band = atomicAdd(&sync[td.y0 * mbc + td.x0], 1u);
+    __syncthreads();
+    band = static_cast<int>(s_band);
+    if (band >= nb || (!keyf && band > 0)) return;
+  }
   // Otherwise, a tile with few intra macroblocks runs them by dependency
   // level: level 0 has no intra neighbour among left, above-left, above and
   // above-right, level k + 1 has one of level k at most.  Macroblocks of one
@@ -3206,14 +3246,8 @@ __device__ __forceinline__ void recon_intra_tile(const uint8_t* __restrict__ des
     // counters and the item count exclude) would leave wrong samples in
     // the band's rows rather than hold the launch.
     __shared__ int s_rowdone[kFrontMax];
-    __shared__ uint32_t s_band;
-    const int nb = key_bands(tw, th, band_rows);
     int by0 = 0, by1 = th;
     if (nb > 1) {
-      if (threadIdx.x == 0) s_band = atomicAdd(&sync[td.y0 * mbc + td.x0], 1u);
-      __syncthreads();
-      const int band = static_cast<int>(s_band);
-      if (band >= nb) return;
       by0 = band * band_rows;
       by1 = by0 + band_rows < th ? by0 + band_rows : th;
     }
@@ -3281,6 +3315,8 @@ __device__ __forceinline__ void recon_intra_tile(const uint8_t* __restrict__ des
           const bool z = r.coef_count == 0;  // (then the residual words were never written)
           intra_mb_w(scratch[wave], z ? 0u : c0, z ? 0u : c1, z ? 0u : c2, r, td, rec, td.x0 + rx, td.y0 + ry,
                      &front, lane, edge);
+        } else {
+          front_from_frame(front, rec, td, td.x0 + rx, td.y0 + ry, lane);
         }
       }
       if (edge & kEdgeOut) {

@@ -401,17 +401,13 @@ struct Schedule {
   bool pipeline = true;       // next step's entropy launch beside this step's reconstruction
   bool intra_cells = false;   // key-frame and intra-dense tiles by block levels (measured slower)
   int key_band_rows = 4;      // macroblock rows per workgroup of a key-frame tile (key_bands)
-  // Band rows of a step's key-frame tiles: none on the block-level path
-  // (whole tiles), nor in a step of many_tiles key-frame tiles or more,
+  // Band rows of a step's key-frame tiles (key_tiles of them), or of its
+  // inter-frame tiles (which band only when they turn out all intra): none
+  // on the block-level path (whole tiles), nor for many_tiles tiles or more,
   // which fill the chip by themselves (64 x 12 1080p key tiles: 132.7 ms of
   // intra kernel time whole, 155.2 ms in bands of 4).
-  template <typename Step, typename PayloadOf>
-  int band_rows(const Step& S, PayloadOf payload_of) const {
-    if (intra_cells) return 0;
-    size_t key_tiles = 0;
-    for (const auto& jr : S.jrefs)
-      if (!payload_of(jr)->inter) key_tiles += static_cast<size_t>(payload_of(jr)->ntiles);
-    return key_tiles < static_cast<size_t>(many_tiles) ? key_band_rows : 0;
+  int band_rows(size_t tiles) const {
+    return !intra_cells && tiles < static_cast<size_t>(many_tiles) ? key_band_rows : 0;
   }
   int inter_intra_waves(size_t ntiles) const {
     return ntiles >= static_cast<size_t>(many_tiles) ? intra_waves_many : intra_waves_few;
@@ -516,7 +512,8 @@ struct StepBuf {
   PinnedArray<uint8_t> h_stage;
   DevArray<uint8_t> d_stage;
   size_t job_off = 0, item_off = 0;
-  int band_rows = 0;    // key-frame band rows of this step's launches (Schedule::band_rows)
+  int band_rows = 0;        // key-frame band rows of this step's launches (Schedule::band_rows)
+  int band_rows_inter = 0;  // the same for its inter-frame tiles
   bool cells = false;   // this step's intra launches take the block-level path
   FrameJob* hj = nullptr;
   FrameJob* dj = nullptr;
@@ -792,7 +789,13 @@ int Engine::launch_entropy(StepBuf& S, hipStream_t ps, bool speculative) {
   uint32_t max_tile_bytes = 1;
   auto tiles_of = [](const PayloadSlot* p) { return reinterpret_cast<const TileDesc*>(p->hdesc + desc_tiles_off()); };
   // (fixed for the step: the band rows set the item count)
-  S.band_rows = sched.band_rows(S, [&](const JobRef& jr) { return S.payload(jr); });
+  {
+    size_t key_tiles = 0, inter_tiles = 0;
+    for (const JobRef& jr : S.jrefs)
+      (S.payload(jr)->inter ? inter_tiles : key_tiles) += static_cast<size_t>(S.payload(jr)->ntiles);
+    S.band_rows = sched.band_rows(key_tiles);
+    S.band_rows_inter = sched.band_rows(inter_tiles);
+  }
   S.cells = sched.intra_cells;
   for (const JobRef& jr : S.jrefs) {
     PayloadSlot* p = S.payload(jr);
@@ -803,7 +806,8 @@ int Engine::launch_entropy(StepBuf& S, hipStream_t ps, bool speculative) {
     for (int t = 0; t < p->ntiles; t++) {
       const TileDesc& td = tiles_of(p)[t];
       max_tile_bytes = std::max(max_tile_bytes, td.byte_len);
-      nitems += p->inter ? 1 : static_cast<size_t>(key_bands(td.x1 - td.x0, td.y1 - td.y0, S.band_rows));
+      nitems += static_cast<size_t>(key_bands(td.x1 - td.x0, td.y1 - td.y0,
+                                              p->inter ? S.band_rows_inter : S.band_rows));
     }
   }
   S.nparse = nparse;
@@ -1058,7 +1062,7 @@ int Engine::plan_recon(StepBuf& S) {
               // (each workgroup claims its band on the device)
               const uint8_t* hd = S.payload(S.jrefs[j])->hdesc;
               const TileDesc& td = reinterpret_cast<const TileDesc*>(hd + desc_tiles_off())[t];
-              const int nb = pass == 0 ? key_bands(td.x1 - td.x0, td.y1 - td.y0, S.band_rows) : 1;
+              const int nb = key_bands(td.x1 - td.x0, td.y1 - td.y0, pass == 0 ? S.band_rows : S.band_rows_inter);
               for (int b = 0; b < nb; b++) S.hi[w++] = WorkItem{static_cast<uint32_t>(j), static_cast<uint32_t>(t)};
             }
         if (pass == 0) key_end[q] = w;
@@ -1147,8 +1151,9 @@ int Engine::launch_recon(StepBuf& S) {
       if (round_end[q] > key_end[q]) {
         const size_t n = round_end[q] - key_end[q];
         const int m = mark_begin(2, 0, st);
+        // (with bands, the frontier for the tiles coded all intra)
         launch_recon_intra(st, static_cast<int>(n), S.dj, S.di + key_end[q], sched.inter_intra_waves(n),
-                           false, S.cells, 0);
+                           S.band_rows_inter > 0, S.cells, S.band_rows_inter);
         mark_end(m, st);
       }
     }
