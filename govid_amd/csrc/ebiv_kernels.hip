@@ -2297,9 +2297,9 @@ struct IntraScratch {
   WaveScratch ws;
   uint8_t L[17][33];     // [0][*] row above, [r][0] left column, [1..16][1..16] the MB
   uint8_t C[2][9][9];    // chroma: [0][*] corner+above, [r][0] left
-  int32_t ref[2][68];    // HEVC reference array per plane in flight
-  int32_t ext[2][52];
-  int32_t dc[2];
+  int32_t ref[3][68];    // HEVC reference array per plane in flight
+  int32_t ext[3][52];
+  int32_t dc[3];
 };
 
 // Builds ref[0..4n] for one block (predictIntra + substituteRefs,
@@ -2630,6 +2630,139 @@ __device__ __forceinline__ void intra_mb_w(IntraScratch& S, const uint32_t w0, c
       fr->corner[(ry + 1) & 1][rx + 1][c] = c == 0 ? S.L[16][16] : S.C[c - 1][8][8];
     }
     wave_sync();
+  }
+}
+
+// A whole 16x16 intra macroblock (transform size 2, every plane written) on
+// the LDS frontier, in three phases instead of intra_mb_w's block chain:
+//   A: the reference arrays of luma, Cb and Cr straight from the frontier;
+//   B: the angular lines and DC values of all three;
+//   C: every lane predicts its 4 luma and 2 chroma samples, adds the
+//      residual words it holds (w0: luma row lane / 8, columns 2 (lane % 8)
+//      and + 1; w1: the same 8 rows lower; w2: chroma plane lane / 32, row
+//      (lane % 32) / 4, columns 2 (lane % 4) and + 1), stores them and
+//      writes the frontier entries it holds.
+// Prediction and substitution are intra_mb_w's (build_ref_one,
+// prep_angular, intra_pixel), so the samples are the same.
+__device__ __forceinline__ void intra_mb16(IntraScratch& S, const uint32_t w0, const uint32_t w1, const uint32_t w2,
+                                           const MBRec& r, const TileDesc& td, Planes rec, int mbx, int mby,
+                                           Frontier& fr, int lane, int edge) {
+  const int rx = mbx - td.x0, ry = mby - td.y0, tw = td.x1 - td.x0;
+  const int X = mbx * 16, Y = mby * 16, CX = mbx * 8, CY = mby * 8;
+  if (edge & kEdgeIn) {
+    // the band above's last row into this row's frontier entries: above,
+    // above-right (luma) and the corner (no macroblock of this band writes
+    // them before this one reads them, and the ones after it overwrite them)
+    if (lane < 9) {
+      const uint32_t w = coherent_load(rec.y + static_cast<size_t>(Y - 1) * rec.pw + X - 4 + 4 * lane);
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int i = 4 * lane - 3 + j;  // x = X - 1 + i
+        const uint8_t v = static_cast<uint8_t>(w >> (8 * j));
+        if (i == 0) fr.corner[ry & 1][rx][0] = v;
+        else if (i >= 1 && i <= 16) fr.bot[rx][i - 1] = v;
+        else if (i >= 17 && rx + 1 < tw) fr.bot[rx + 1][i - 17] = v;
+      }
+    } else if (lane >= 16 && lane < 22) {
+      const int pl = (lane - 16) / 3, k = (lane - 16) % 3;
+      const uint8_t* cp = pl ? rec.cr : rec.cb;
+      const uint32_t w = coherent_load(cp + static_cast<size_t>(CY - 1) * rec.cw + CX - 4 + 4 * k);
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int i = 4 * k - 3 + j;
+        const uint8_t v = static_cast<uint8_t>(w >> (8 * j));
+        if (i == 0) fr.corner[ry & 1][rx][1 + pl] = v;
+        else if (i >= 1) fr.bot[rx][16 + 8 * pl + i - 1] = v;
+      }
+    }
+    wave_sync();
+  }
+  // ---- A: reference arrays.  Each entry is one frontier byte (or the
+  // no-neighbour value): build_ref_one's choice of sample written as an
+  // offset into the frontier, so that a lane issues all its reads at once
+  // instead of one per branch of the choice.
+  const int tl = td.x0 * 16, tt = td.y0 * 16, trr = td.x1 * 16;
+  const bool av_above = Y > tt, av_left = X > tl;
+  const bool av_ar = !(Y <= tt || X + 32 > trr);  // aboveRightAvailable, predict.go:50-58, at the MB origin
+  {
+    const uint8_t* fb = reinterpret_cast<const uint8_t*>(&fr);
+    constexpr int kBot = offsetof(Frontier, bot), kRight = offsetof(Frontier, right),
+                  kCorner = offsetof(Frontier, corner);
+    const int bot0 = kBot + rx * 32, right0 = kRight + ry * 32, corner0 = kCorner + ((ry & 1) * (kFrontMax + 1) + rx) * 4;
+    // plane p (0 luma, 1 Cb, 2 Cr) of n x n, entry i in [0, 4n]: the byte offset
+    auto src = [&](int p, int n, int i) -> int {
+      const int po = p == 0 ? 0 : 8 + 8 * p;  // the plane's bytes in a frontier entry
+      const int above0 = bot0 + po, left0 = right0 + po;
+      if (i < 2 * n) {
+        const int rr = 2 * n - 1 - i < n ? 2 * n - 1 - i : n - 1;
+        return av_left ? left0 + rr : above0;
+      }
+      if (i == 2 * n) return av_left ? (av_above ? corner0 + p : left0) : above0;
+      int j = i - 2 * n - 1;
+      if (j >= n && (p != 0 || !av_ar)) j = n - 1;
+      return av_above ? (j < n ? above0 + j : bot0 + 32 + (j - n)) : left0;
+    };
+    const bool none = !av_left && !av_above;
+    const int oy = src(0, 16, lane), ocb = src(1, 8, lane < 33 ? lane : 32), ocr = src(2, 8, lane < 33 ? lane : 32);
+    const int oy64 = src(0, 16, 64);
+    const int32_t vy = fb[oy], vcb = fb[ocb], vcr = fb[ocr], vy64 = fb[oy64];
+    S.ref[0][lane] = none ? kDCPred : vy;
+    if (lane == 0) S.ref[0][64] = none ? kDCPred : vy64;
+    if (lane < 33) {
+      S.ref[1][lane] = none ? kDCPred : vcb;
+      S.ref[2][lane] = none ? kDCPred : vcr;
+    }
+  }
+  wave_sync();
+  // ---- B: angular lines and DC values
+  const int lmode = r.lmode[0] & 15, cmode = r.cmode;
+  prep_angular(S.ref[0], S.ext[0], &S.dc[0], 16, lmode, lane);
+  {
+    const int pl = lane >> 5;
+    prep_angular(S.ref[1 + pl], S.ext[1 + pl], &S.dc[1 + pl], 8, cmode, lane & 31);
+  }
+  wave_sync();
+  // ---- C: samples, frame, frontier
+  // (sample pairs: rows q and q + 8 of luma, then chroma)
+  const int q = lane >> 3, c = 2 * (lane & 7);
+  const int pl = lane >> 5, cr = (lane & 31) >> 2, cc = 2 * (lane & 3);
+#pragma unroll
+  for (int h = 0; h < 3; h++) {
+    const bool luma = h < 2;
+    const uint32_t w = h == 0 ? w0 : (h == 1 ? w1 : w2);
+    const int rr = luma ? q + 8 * h : cr, c0 = luma ? c : cc, n = luma ? 16 : 8, mode = luma ? lmode : cmode;
+    const int pi = luma ? 0 : 1 + pl;
+    const int32_t a0 = clamp255(intra_pixel(S.ref[pi], S.ext[pi], S.dc[pi], n, mode, rr, c0) +
+                                static_cast<int16_t>(w & 0xFFFFu));
+    const int32_t a1 = clamp255(intra_pixel(S.ref[pi], S.ext[pi], S.dc[pi], n, mode, rr, c0 + 1) +
+                                static_cast<int16_t>(w >> 16));
+    uint8_t* dp = luma ? rec.y + static_cast<size_t>(Y + rr) * rec.pw + X
+                       : (pl ? rec.cr : rec.cb) + static_cast<size_t>(CY + rr) * rec.cw + CX;
+    *reinterpret_cast<uint16_t*>(dp + c0) = static_cast<uint16_t>(a0 | (a1 << 8));
+    const int fo = luma ? 0 : 16 + 8 * pl;  // the plane's offset in a frontier entry
+    if (rr == n - 1) {  // the bottom row
+      fr.bot[rx][fo + c0] = static_cast<uint8_t>(a0);
+      fr.bot[rx][fo + c0 + 1] = static_cast<uint8_t>(a1);
+    }
+    if (c0 + 1 == n - 1) {  // the right column (and the corner)
+      fr.right[ry][fo + rr] = static_cast<uint8_t>(a1);
+      if (rr == n - 1) fr.corner[(ry + 1) & 1][rx + 1][luma ? 0 : 1 + pl] = static_cast<uint8_t>(a1);
+    }
+  }
+  wave_sync();
+  if (edge & kEdgeOut) {
+    // the bottom rows once more, device-coherent, for the band below
+    if (lane < 8) {
+      uint32_t w = 0;
+#pragma unroll
+      for (int k = 0; k < 4; k++) w |= static_cast<uint32_t>(fr.bot[rx][4 * lane + k]) << (8 * k);
+      if (lane < 4) {
+        coherent_store(rec.y + static_cast<size_t>(Y + 15) * rec.pw + X + 4 * lane, w);
+      } else {
+        const int pl = (lane - 4) >> 1, k2 = lane & 1;
+        coherent_store((pl ? rec.cr : rec.cb) + static_cast<size_t>(CY + 7) * rec.cw + CX + 4 * k2, w);
+      }
+    }
   }
 }
 
@@ -3289,6 +3422,11 @@ __device__ __forceinline__ void recon_intra_tile(const uint8_t* __restrict__ des
     while (have) {
       const int rx = s - 2 * (ry - by0);
       const uint32_t cr = vrec, c0 = w0, c1 = w1, c2 = w2;
+      // (the lane index made opaque per macroblock: otherwise the compiler
+      // hoists every lane-dependent term of the macroblock code out of the
+      // loop and keeps them all live, past the registers the launch has)
+      int ln = lane;
+      asm volatile("" : "+v"(ln));
       int s2 = s, ry2 = ry;
       const bool have2 = advance(s2, ry2);
       if (have2) fetch(s2, ry2);
@@ -3313,10 +3451,14 @@ __device__ __forceinline__ void recon_intra_tile(const uint8_t* __restrict__ des
         const MBRec r = rec_of_lanes(cr);
         if (r.kind == 2) {
           const bool z = r.coef_count == 0;  // (then the residual words were never written)
-          intra_mb_w(scratch[wave], z ? 0u : c0, z ? 0u : c1, z ? 0u : c2, r, td, rec, td.x0 + rx, td.y0 + ry,
-                     &front, lane, edge);
+          if (r.tx == 2 && (r.wmask & 0x30001u) == 0x30001u)
+            intra_mb16(scratch[wave], z ? 0u : c0, z ? 0u : c1, z ? 0u : c2, r, td, rec, td.x0 + rx, td.y0 + ry,
+                       front, ln, edge);
+          else
+            intra_mb_w(scratch[wave], z ? 0u : c0, z ? 0u : c1, z ? 0u : c2, r, td, rec, td.x0 + rx, td.y0 + ry,
+                       &front, ln, edge);
         } else {
-          front_from_frame(front, rec, td, td.x0 + rx, td.y0 + ry, lane);
+          front_from_frame(front, rec, td, td.x0 + rx, td.y0 + ry, ln);
         }
       }
       if (edge & kEdgeOut) {
