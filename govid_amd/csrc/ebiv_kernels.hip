@@ -1225,24 +1225,10 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       MB_DEC("%[vm0]", "32", "31", "0x100020", "%[sym]", "tx", "unused")
       "s_lshl_b32 %[t1], %[sym], 24\n"
       "s_or_b32 %[r0], %[r0], %[t1]\n"
-      "s_cmp_eq_u32 %[sym], 0\n"
-      "s_cbranch_scc0 L%=_tx1\n"
-      "s_mov_b32 %[nb], 16\n"
-      "s_mov_b32 %[total], 16\n"
-      "v_mov_b32 %[va], %[t00]\n"
-      "v_mov_b32 %[vb], %[t01]\n"
-      "v_mov_b32 %[vc], %[t02]\n"
-      "s_branch L%=_blocks\n"
-      "L%=_tx1:\n"
-      "s_cmp_eq_u32 %[sym], 1\n"
-      "s_cbranch_scc0 L%=_tx2\n"
-      "s_mov_b32 %[nb], 4\n"
-      "s_mov_b32 %[total], 64\n"
-      "v_mov_b32 %[va], %[t10]\n"
-      "v_mov_b32 %[vb], %[t11]\n"
-      "v_mov_b32 %[vc], %[t12]\n"
-      "s_branch L%=_blocks\n"
-      "L%=_tx2:\n"
+      // 16x16 (the most common size) on the straight line, 4x4 and 8x8
+      // out of line
+      "s_cmp_lt_u32 %[sym], 2\n"
+      "s_cbranch_scc1 L%=_tx01\n"
       "s_mov_b32 %[nb], 1\n"
       "s_movk_i32 %[total], 0x100\n"
       "v_mov_b32 %[va], %[t20]\n"
@@ -1271,16 +1257,31 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       "s_or_b32 %[lm1], %[lm1], %[t2]\n"
 
       // ------------------------------------------------------ token block
+      // The block's first token (band 0, previous class 0, phase 0): the
+      // zero token and the end of block (symbol 6: the ballot's field is
+      // 0x7f) are tried together; an end of block here is a block with no
+      // level, which goes straight to the next block.
       "L%=_tb:\n"
-      "s_mov_b32 %[si], 0\n"
-      "s_mov_b32 %[stop], 1\n"
-      "s_mov_b32 %[off], 0\n"
-      "s_mov_b32 %[offz], 0\n"
-      "s_mov_b32 %[bf], 0x80000\n"
-      "s_mov_b32 %[bfz], 0x80000\n"
+      "s_and_b32 %[slot], %[x0], 0xfff\n"
+      "v_cmp_ge_u32_sdwa %[b], %[slot], %[va] src0_sel:DWORD src1_sel:WORD_1\n"
+      "v_readlane_b32 %[p], %[va], 0\n"
+      "v_readlane_b32 %[t3], %[va], 6\n"
+      "v_mov_b32 %[vcur], %[va]\n"
+      "s_lshr_b32 %[x0], %[x0], 12\n"
+      "s_bfe_u64 %[b], %[b], 0x80000\n"
       "s_mov_b32 %[sumabs], 0\n"
       "s_mov_b32 %[firstsi], 0\n"
-      "v_mov_b32 %[vcur], %[va]\n"
+      "s_cmp_eq_u64 %[b], 1\n"
+      "s_cbranch_scc1 L%=_tbz\n"
+      "s_cmp_eq_u64 %[b], 0x7f\n"
+      "s_cbranch_scc0 L%=_tbnz\n"
+      "s_and_b32 %[t2], %[t3], 0xffff\n"
+      "s_lshr_b32 %[t3], %[t3], 16\n"
+      "s_mul_i32 %[xn], %[t2], %[x0]\n"
+      "s_sub_u32 %[t2], %[slot], %[t3]\n"
+      "s_add_u32 %[xn], %[xn], %[t2]\n"
+      MB_RN("tbe")
+      "s_branch L%=_tbdone\n"
       MBP_TOKLOOP
       // end of block or escape (magnitude 5 + getEscapeValue,
       // mv.go:124-128), at phase 0
@@ -1413,6 +1414,47 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       "s_mov_b32 %[t3], 1\n"
       "s_branch L%=_flush\n"
 
+      // ---- transform sizes 4x4 and 8x8: the luma blocks' vectors and scan
+      "L%=_tx01:\n"
+      "s_cmp_eq_u32 %[sym], 0\n"
+      "s_cbranch_scc0 L%=_tx1\n"
+      "s_mov_b32 %[nb], 16\n"
+      "s_mov_b32 %[total], 16\n"
+      "v_mov_b32 %[va], %[t00]\n"
+      "v_mov_b32 %[vb], %[t01]\n"
+      "v_mov_b32 %[vc], %[t02]\n"
+      "s_branch L%=_blocks\n"
+      "L%=_tx1:\n"
+      "s_mov_b32 %[nb], 4\n"
+      "s_mov_b32 %[total], 64\n"
+      "v_mov_b32 %[va], %[t10]\n"
+      "v_mov_b32 %[vb], %[t11]\n"
+      "v_mov_b32 %[vc], %[t12]\n"
+      "s_branch L%=_blocks\n"
+
+      // ---- a block's first token when it is not the end of block: a zero
+      // token (band 0 to band 1, as _tstop1a would) or any other (as _tok0a
+      // leaves it for _tnz0a)
+      "L%=_tbz:\n"
+      "s_mul_i32 %[x0], %[p], %[x0]\n"
+      "s_add_u32 %[x0], %[x0], %[slot]\n"
+      MBP_RN_NC("tbz", "%[x0]")
+      "s_mov_b32 %[si], 1\n"
+      "s_mov_b32 %[stop], 3\n"
+      "s_movk_i32 %[off], 24\n"
+      "s_movk_i32 %[offz], 24\n"
+      "s_mov_b32 %[bf], 0x80018\n"
+      "s_mov_b32 %[bfz], 0x80018\n"
+      "s_branch L%=_tok1a\n"
+      "L%=_tbnz:\n"
+      "s_mov_b32 %[si], 0\n"
+      "s_mov_b32 %[stop], 1\n"
+      "s_mov_b32 %[off], 0\n"
+      "s_mov_b32 %[offz], 0\n"
+      "s_mov_b32 %[bf], 0x80000\n"
+      "s_mov_b32 %[bfz], 0x80000\n"
+      "s_branch L%=_tnz0a\n"
+
       // ---- out-of-line renormalisation and coefficient stores
       MB_RN_OOL("md", "mode_underrun")
       MB_RN_OOL("md2", "mode_underrun")
@@ -1429,6 +1471,8 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       MB_RN_OOL("esd", "underrun")
       MB_RN_OOL("esb", "underrun")
       MB_RN_OOL("sd", "underrun")
+      MB_RN_OOL("tbe", "underrun")
+      MBP_RN_OOL("tbz", "underrun", "%[x0]")
       MBP_TOKLOOP_OOL
       MB_EMIT_OOL("e2")
 
