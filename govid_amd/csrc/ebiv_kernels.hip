@@ -934,6 +934,33 @@ struct MbLoop {
   MBP_RN_OOL("ts3", "underrun", "%[x0]") MB_EMIT_OOL("e13")                       \
   MBP_RN_OOL("tsg", "underrun", "%[x0]") MB_EMIT_OOL("e1g")
 
+// A block's first token (band 0, previous class 0, phase 0; the block's
+// vectors in va, vb, vc): the zero token and the end of block (symbol 6: the
+// ballot's field is 0x7f) are tried together.  A zero token goes to _tbz,
+// any other token but the end of block to _tbnz; the end of block (a block
+// with no level) falls through, decoded and rotated as MB_DEC leaves a
+// symbol.
+#define MB_FIRST_TOKEN(tag)                      \
+  "s_and_b32 %[slot], %[x0], 0xfff\n"            \
+  "v_cmp_ge_u32_sdwa %[b], %[slot], %[va] src0_sel:DWORD src1_sel:WORD_1\n" \
+  "v_readlane_b32 %[p], %[va], 0\n"              \
+  "v_readlane_b32 %[t3], %[va], 6\n"             \
+  "v_mov_b32 %[vcur], %[va]\n"                   \
+  "s_lshr_b32 %[x0], %[x0], 12\n"                \
+  "s_bfe_u64 %[b], %[b], 0x80000\n"              \
+  "s_mov_b32 %[sumabs], 0\n"                     \
+  "s_mov_b32 %[firstsi], 0\n"                    \
+  "s_cmp_eq_u64 %[b], 1\n"                       \
+  "s_cbranch_scc1 L%=_tbz\n"                     \
+  "s_cmp_eq_u64 %[b], 0x7f\n"                    \
+  "s_cbranch_scc0 L%=_tbnz\n"                    \
+  "s_and_b32 %[t2], %[t3], 0xffff\n"             \
+  "s_lshr_b32 %[t3], %[t3], 16\n"                \
+  "s_mul_i32 %[xn], %[t2], %[x0]\n"              \
+  "s_sub_u32 %[t2], %[slot], %[t3]\n"            \
+  "s_add_u32 %[xn], %[xn], %[t2]\n"              \
+  MB_RN(tag)
+
 // A coded macroblock's fields before its grammar: intra kind, nothing
 // written, zero vectors and modes, every plane coded.
 #define MB_INIT                                  \
@@ -1052,6 +1079,7 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       "s_cmp_lt_u32 %[mbx], %[pend]\n"
       "s_cbranch_scc1 L%=_mb\n"
       // end of a page: its records and MVs out
+      "L%=_pageend:\n"
       "s_mov_b32 %[t3], 0\n"
       "s_branch L%=_flush\n"
       "L%=_pagedone:\n"
@@ -1257,30 +1285,10 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       "s_or_b32 %[lm1], %[lm1], %[t2]\n"
 
       // ------------------------------------------------------ token block
-      // The block's first token (band 0, previous class 0, phase 0): the
-      // zero token and the end of block (symbol 6: the ballot's field is
-      // 0x7f) are tried together; an end of block here is a block with no
+      // The block's first token; an end of block here is a block with no
       // level, which goes straight to the next block.
       "L%=_tb:\n"
-      "s_and_b32 %[slot], %[x0], 0xfff\n"
-      "v_cmp_ge_u32_sdwa %[b], %[slot], %[va] src0_sel:DWORD src1_sel:WORD_1\n"
-      "v_readlane_b32 %[p], %[va], 0\n"
-      "v_readlane_b32 %[t3], %[va], 6\n"
-      "v_mov_b32 %[vcur], %[va]\n"
-      "s_lshr_b32 %[x0], %[x0], 12\n"
-      "s_bfe_u64 %[b], %[b], 0x80000\n"
-      "s_mov_b32 %[sumabs], 0\n"
-      "s_mov_b32 %[firstsi], 0\n"
-      "s_cmp_eq_u64 %[b], 1\n"
-      "s_cbranch_scc1 L%=_tbz\n"
-      "s_cmp_eq_u64 %[b], 0x7f\n"
-      "s_cbranch_scc0 L%=_tbnz\n"
-      "s_and_b32 %[t2], %[t3], 0xffff\n"
-      "s_lshr_b32 %[t3], %[t3], 16\n"
-      "s_mul_i32 %[xn], %[t2], %[x0]\n"
-      "s_sub_u32 %[t2], %[slot], %[t3]\n"
-      "s_add_u32 %[xn], %[xn], %[t2]\n"
-      MB_RN("tbe")
+      MB_FIRST_TOKEN("tbe")
       "s_branch L%=_tbdone\n"
       MBP_TOKLOOP
       // end of block or escape (magnitude 5 + getEscapeValue,
@@ -1315,20 +1323,41 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       "L%=_temit:\n"
       "s_and_b32 %[t2], %[firstpm], 0xff\n"
       MB_EMIT("e2")
+      // After a block: the next luma block out of line; the chroma mode
+      // after the last luma block out of line; the chroma blocks on the
+      // straight line, each with its own copy of the first token, so that an
+      // empty chroma block costs one taken branch.
       "L%=_tbdone:\n"
       "s_lshr_b32 %[t1], %[blk8], 8\n"
       "s_bitset1_b32 %[wmask], %[t1]\n"
       "s_add_u32 %[k], %[k], 1\n"
       "s_cmp_lt_u32 %[k], %[nb]\n"
-      "s_cbranch_scc0 L%=_tbluma\n"
+      "s_cbranch_scc1 L%=_tbmore\n"
+      "s_cmp_eq_u32 %[k], %[nb]\n"
+      "s_cbranch_scc1 L%=_lumadone\n"
+      // chroma block k - nb (0 Cb, 1 Cr, 2 done)
+      "L%=_chnext:\n"
+      "s_sub_u32 %[t1], %[k], %[nb]\n"
+      "s_cmp_eq_u32 %[t1], 2\n"
+      "s_cbranch_scc1 L%=_mbend\n"
+      "s_add_u32 %[t3], %[t1], 1\n"
+      "s_lshr_b32 %[t3], %[cbpf], %[t3]\n"
+      "s_bitcmp1_b32 %[t3], 0\n"
+      "s_cbranch_scc0 L%=_chskip\n"
+      "s_add_u32 %[t1], %[t1], 16\n"
+      "s_lshl_b32 %[blk8], %[t1], 8\n"
+      "s_mov_b32 %[total], 64\n"
+      "v_mov_b32 %[va], %[t30]\n"
+      "v_mov_b32 %[vb], %[t31]\n"
+      "v_mov_b32 %[vc], %[t32]\n"
+      MB_FIRST_TOKEN("tbe2")
+      "s_branch L%=_tbdone\n"
       // the next luma block: inter straight to its tokens, intra to its mode
+      "L%=_tbmore:\n"
       "s_lshl_b32 %[blk8], %[k], 8\n"
       "s_bitcmp1_b32 %[r0], 1\n"
       "s_cbranch_scc0 L%=_tb\n"
       "s_branch L%=_lmdec\n"
-      "L%=_tbluma:\n"
-      "s_cmp_eq_u32 %[k], %[nb]\n"
-      "s_cbranch_scc0 L%=_chnext\n"
       // luma complete: the chroma mode (intra) or the uncoded chroma planes
       // (inter)
       "L%=_lumadone:\n"
@@ -1346,27 +1375,13 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       "s_bitcmp1_b32 %[cbpf], 2\n"
       "s_cbranch_scc1 L%=_chnext\n"
       "s_bitset1_b32 %[wmask], 17\n"
-      // chroma block k - nb (0 Cb, 1 Cr, 2 done)
-      "L%=_chnext:\n"
-      "s_sub_u32 %[t1], %[k], %[nb]\n"
-      "s_cmp_eq_u32 %[t1], 2\n"
-      "s_cbranch_scc1 L%=_mbend\n"
-      "s_add_u32 %[t3], %[t1], 1\n"
-      "s_lshr_b32 %[t3], %[cbpf], %[t3]\n"
-      "s_bitcmp1_b32 %[t3], 0\n"
-      "s_cbranch_scc0 L%=_chskip\n"
-      "s_add_u32 %[t1], %[t1], 16\n"
-      "s_lshl_b32 %[blk8], %[t1], 8\n"
-      "s_mov_b32 %[total], 64\n"
-      "v_mov_b32 %[va], %[t30]\n"
-      "v_mov_b32 %[vb], %[t31]\n"
-      "v_mov_b32 %[vc], %[t32]\n"
-      "s_branch L%=_tb\n"
+      "s_branch L%=_chnext\n"
       "L%=_chskip:\n"
       "s_add_u32 %[k], %[k], 1\n"
       "s_branch L%=_chnext\n"
 
-      // ---- end of a coded macroblock: its record and MV
+      // ---- end of a coded macroblock: its record and MV, then the next
+      // macroblock as at _mbnext
       "L%=_mbend:\n"
       "s_and_b32 %[t3], %[r0], 0xff\n"
       "s_cmp_eq_u32 %[t3], 2\n"
@@ -1389,7 +1404,12 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       "v_writelane_b32 %[vr10], %[lm1], m0\n"
       "v_writelane_b32 %[vmv], %[mvcur], m0\n"
       "s_mov_b32 %[leftmv], %[mvcur]\n"
-      "s_branch L%=_mbnext\n"
+      "s_add_u32 %[done], %[done], 1\n"
+      "s_add_u32 %[mbx], %[mbx], 1\n"
+      "s_add_u32 %[ln], %[ln], 1\n"
+      "s_cmp_lt_u32 %[mbx], %[pend]\n"
+      "s_cbranch_scc1 L%=_mb\n"
+      "s_branch L%=_pageend\n"
 
       // ---- decode errors: the failing macroblock's fields stay as decoded
       "L%=_mode_unused:\n"
@@ -1472,6 +1492,7 @@ __device__ __forceinline__ uint32_t mb_loop(MbLoop& S, const Tables& T, uint32_t
       MB_RN_OOL("esb", "underrun")
       MB_RN_OOL("sd", "underrun")
       MB_RN_OOL("tbe", "underrun")
+      MB_RN_OOL("tbe2", "underrun")
       MBP_RN_OOL("tbz", "underrun", "%[x0]")
       MBP_TOKLOOP_OOL
       MB_EMIT_OOL("e2")
