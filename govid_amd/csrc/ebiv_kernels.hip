@@ -1942,6 +1942,16 @@ using LdsI32 = __attribute__((address_space(3))) int32_t;
 // first pass is at most 5.56x the input and the second 45,488 x that), and
 // 32-bit accumulation gives the same integers.
 constexpr int32_t kIdct32Max = 8190;
+// One product of the transform.  In the 32-bit form every operand fits in 24
+// signed bits (matrix entries below 2^13, inputs at most kIdct32Max, first-pass
+// outputs at most 45,488 x 8190 / 2^13 + 1) and every product in 32, so the
+// full-rate 24-bit multiply gives the exact product (a 32-bit multiply issues
+// at a quarter of the rate).
+template <typename Acc>
+__device__ __forceinline__ Acc idct_mul(int32_t c, int32_t x) {
+  if constexpr (sizeof(Acc) == 4) return __mul24(c, x);
+  else return static_cast<Acc>(c) * x;
+}
 template <int N, typename Acc>
 __device__ __forceinline__ void idct_blocks(WaveScratch& ws, const LdsI32* C, int base, int nblk, uint32_t nzm,
                                             uint32_t acm, int lane) {
@@ -1955,7 +1965,7 @@ __device__ __forceinline__ void idct_blocks(WaveScratch& ws, const LdsI32* C, in
       const int32_t* src = ws.coef + base + b * NN + l;
       Acc acc = 0;
 #pragma unroll
-      for (int k = 0; k < N; k++) acc += static_cast<Acc>(C[k * N + i]) * src[k * N];
+      for (int k = 0; k < N; k++) acc += idct_mul<Acc>(C[k * N + i], src[k * N]);
       ws.tmp[base + e] = static_cast<int32_t>((acc + 4096) >> 13);
     } else if (((nzm >> b) & 1u) && local == 0) {
       ws.tmp[base + e] = static_cast<int32_t>((static_cast<Acc>(c0) * ws.coef[base + e] + 4096) >> 13);
@@ -1969,7 +1979,7 @@ __device__ __forceinline__ void idct_blocks(WaveScratch& ws, const LdsI32* C, in
       const int32_t* t = ws.tmp + base + b * NN + i * N;
       Acc acc = 0;
 #pragma unroll
-      for (int l = 0; l < N; l++) acc += static_cast<Acc>(C[l * N + m]) * t[l];
+      for (int l = 0; l < N; l++) acc += idct_mul<Acc>(C[l * N + m], t[l]);
       ws.coef[base + e] = static_cast<int32_t>((acc + 4096) >> 13);
     } else if ((nzm >> b) & 1u) {
       ws.coef[base + e] = static_cast<int32_t>((static_cast<Acc>(c0) * ws.tmp[base + b * NN] + 4096) >> 13);
@@ -2060,6 +2070,13 @@ __device__ __forceinline__ uint32_t clamp_pack4(const int32_t (&v)[4]) {
          (static_cast<uint32_t>(clamp255(c)) << 16) | (static_cast<uint32_t>(clamp255(d)) << 24);
 }
 
+// Two samples clamped to [0, 255] and packed into 16 bits, sample 0 in the
+// low byte (the fusion blocked as in clamp_pack4).
+__device__ __forceinline__ uint16_t clamp_pack2(int32_t a, int32_t b) {
+  asm("" : "+v"(a), "+v"(b));
+  return static_cast<uint16_t>(static_cast<uint32_t>(clamp255(a)) | (static_cast<uint32_t>(clamp255(b)) << 8));
+}
+
 // Bytes col .. col + 4K - 1 of a plane row of width w as K dwords, each
 // column clamped to [0, w) as clampCoord clamps it (motioncomp.go:37-45).
 // col is a multiple of 4, so a span inside the row is K aligned dword
@@ -2091,21 +2108,31 @@ __device__ __forceinline__ Span<K> load_span(const uint8_t* __restrict__ row, in
   return out;
 }
 
+// Signed 4 x int8 dot product of a with taps, accumulator 0.  Asked for with
+// saturation, it is v_dot4_i32_i8's three-operand form with the zero as an
+// inline constant; without, it is v_dot4c_i32_i8, whose accumulator is its
+// destination, and every product costs a v_mov of the zero first.  The sums
+// here are far inside int32 (at most 4 x 128 x 128), so saturation never acts.
+__device__ __forceinline__ int32_t sdot4z(uint32_t a, uint32_t taps) {
+  return __builtin_amdgcn_sdot4(static_cast<int32_t>(a), static_cast<int32_t>(taps), 0, true);
+}
+
 // Horizontal 4-tap sums of samples 0-3 from a row segment d0..d2 whose
 // first tap is byte sh, on samples biased by -128 (each sum is 128 * 64 low).
 __device__ __forceinline__ void luma_hsum4(const Span<3>& s, int sh, int32_t tx, int32_t (&h)[4]) {
   const uint32_t e0 = __builtin_amdgcn_alignbyte(s.d[1], s.d[0], sh) ^ 0x80808080u;
   const uint32_t e1 = __builtin_amdgcn_alignbyte(s.d[2], s.d[1], sh) ^ 0x80808080u;
-  h[0] = __builtin_amdgcn_sdot4(static_cast<int32_t>(e0), tx, 0, false);
-  h[1] = __builtin_amdgcn_sdot4(static_cast<int32_t>(__builtin_amdgcn_alignbyte(e1, e0, 1)), tx, 0, false);
-  h[2] = __builtin_amdgcn_sdot4(static_cast<int32_t>(__builtin_amdgcn_alignbyte(e1, e0, 2)), tx, 0, false);
-  h[3] = __builtin_amdgcn_sdot4(static_cast<int32_t>(__builtin_amdgcn_alignbyte(e1, e0, 3)), tx, 0, false);
+  const uint32_t t = static_cast<uint32_t>(tx);
+  h[0] = sdot4z(e0, t);
+  h[1] = sdot4z(__builtin_amdgcn_alignbyte(e1, e0, 1), t);
+  h[2] = sdot4z(__builtin_amdgcn_alignbyte(e1, e0, 2), t);
+  h[3] = sdot4z(__builtin_amdgcn_alignbyte(e1, e0, 3), t);
 }
 
 // Motion compensation of one partition, its residual added and the blocks
 // of the write mask stored (decode_inter.go:64-152, motioncomp.go:37-97).
-// Luma: lane = 4 samples of partition row lane / 4.  Chroma: lanes 0-15 Cb,
-// 16-31 Cr, 4 samples of one row each.  Every reference load of the
+// Luma: lane = 4 samples of partition row lane / 4.  Chroma: lanes 0-31 Cb,
+// 32-63 Cr, 2 samples of one row each.  Every reference load of the
 // partition is issued before any sample is computed.
 //  - Luma: the row segment of a lane's samples is read as aligned dwords
 //    and shifted once by the partition's byte phase (the same for every
@@ -2116,9 +2143,12 @@ __device__ __forceinline__ void luma_hsum4(const Span<3>& s, int sh, int32_t tx,
 //    vertical sum and (v + 2048) >> 12 follow; a {0, 64, 0, 0} tap row
 //    makes that equal the reference's 1-D forms, (v + 32) >> 6, and the
 //    full-pel case takes the bytes directly.
-//  - Chroma: the eighth-pel bilinear sum as two horizontal 2-tap sums (one
-//    v_dot4 each, weights 8 - fx and fx) and the vertical blend (8 - fy,
-//    fy); (sum + 32) >> 6 is the reference's rounding.
+//  - Chroma: the eighth-pel bilinear sum (8 - fy)((8 - fx) a0 + fx a1) +
+//    fy((8 - fx) b0 + fx b1) as two chained v_dot4, one per row, with the
+//    row weight folded into the byte taps (each product of weights is at
+//    most 64); the second sample's taps are the first's moved up one byte,
+//    so both read the same shifted dwords.  (sum + 32) >> 6 is the
+//    reference's rounding.
 // The reference rows of one lane (luma: up to four 12-byte spans; chroma:
 // one 8-byte span of each of two rows).
 struct McRows {
@@ -2138,9 +2168,9 @@ __device__ __forceinline__ void mc_load(const Planes& src, int mbx, int mby, int
   const int sft = copy ? sx0 : sx0 - 1;  // first byte read of the partition row
   const int sh = sft & 3;
   const int col = (sft & ~3) + lc;
-  auto lrow = [&](int yy) {
-    return src.y + static_cast<uint32_t>(clampc(yy, src.ph)) * static_cast<uint32_t>(src.pw);
-  };
+  // (row offsets by 24-bit multiplies: padded planes are far below 2^24
+  // rows and columns)
+  auto lrow = [&](int yy) { return src.y + __umul24(static_cast<uint32_t>(clampc(yy, src.ph)), src.pw); };
   if (fy == 0) {  // one row (the full-pel copy uses its first 8 bytes)
     m.q1 = load_span<3>(lrow(sy), col, src.pw);
     asm volatile("");  // keeps the compiler from merging this load with the 2-D path's last one
@@ -2150,16 +2180,15 @@ __device__ __forceinline__ void mc_load(const Planes& src, int mbx, int mby, int
     m.q2 = load_span<3>(lrow(sy + 1), col, src.pw);
     m.q3 = load_span<3>(lrow(sy + 2), col, src.pw);
   }
-  // ---- chroma loads (lanes 32-63 repeat lanes 0-31)
+  // ---- chroma loads: the dword-aligned 8 bytes holding the lane's first
+  // sample column and the two after it, in rows csy and csy + 1
   const int cpx = px >> 1, cpy = py >> 1;
-  const int pl = (lane >> 4) & 1, cr = (lane >> 1) & 7, cc = (lane & 1) * 4;
+  const int pl = lane >> 5, cr = (lane >> 2) & 7, cc = (lane & 3) * 2;
   const int csy = mby * 8 + cpy + cr + (mvy >> 3);
-  const int csx0 = mbx * 8 + cpx + (mvx >> 3);
-  const int ccol = (csx0 & ~3) + cc;
+  const int ccol = (mbx * 8 + cpx + cc + (mvx >> 3)) & ~3;
   const uint8_t* plane = pl ? src.cr : src.cb;
-  m.ca = load_span<2>(plane + static_cast<uint32_t>(clampc(csy, src.ch)) * static_cast<uint32_t>(src.cw), ccol, src.cw);
-  m.cb = load_span<2>(plane + static_cast<uint32_t>(clampc(csy + 1, src.ch)) * static_cast<uint32_t>(src.cw), ccol,
-                      src.cw);
+  m.ca = load_span<2>(plane + __umul24(static_cast<uint32_t>(clampc(csy, src.ch)), src.cw), ccol, src.cw);
+  m.cb = load_span<2>(plane + __umul24(static_cast<uint32_t>(clampc(csy + 1, src.ch)), src.cw), ccol, src.cw);
 }
 
 __device__ __forceinline__ void mc_finish(const Planes& rec, const WaveScratch& ws, const MBRec& r, bool has_res,
@@ -2172,13 +2201,18 @@ __device__ __forceinline__ void mc_finish(const Planes& rec, const WaveScratch& 
   const bool copy = (fx | fy) == 0;
   const int sh = (copy ? sx0 : sx0 - 1) & 3;
   const int cpx = px >> 1, cpy = py >> 1, cpw = pwd >> 1, cph = pht >> 1;
-  const int pl = (lane >> 4) & 1, cr = (lane >> 1) & 7, cc = (lane & 1) * 4;
+  const int pl = lane >> 5, cr = (lane >> 2) & 7, cc = (lane & 3) * 2;
   const int cy = cpy + cr, cx = cpx + cc;
-  const int csh = (mbx * 8 + cpx + (mvx >> 3)) & 3;
+  const int csh = (mbx * 8 + cpx + cc + (mvx >> 3)) & 3;
   // ---- luma samples
+  // (a full-pel partition without a luma residual stores the shifted
+  // reference dword as it is)
+  const bool lres = has_res && (r.cbp & kCBPLuma);
   int32_t v[4];
+  uint32_t direct = 0;
   if (copy) {
     const uint32_t w = __builtin_amdgcn_alignbyte(m.q1.d[1], m.q1.d[0], sh);
+    direct = w;
 #pragma unroll
     for (int j = 0; j < 4; j++) v[j] = static_cast<int32_t>((w >> (8 * j)) & 0xFFu);
   } else {
@@ -2204,41 +2238,38 @@ __device__ __forceinline__ void mc_finish(const Planes& rec, const WaveScratch& 
   }
   const int blk = ((y >> lgn) << (4 - lgn)) + (x >> lgn);
   if (lr < pht && lc < pwd && ((r.wmask >> blk) & 1u)) {
-    if (has_res && (r.cbp & kCBPLuma)) {
+    if (lres) {
       const int m = (1 << lgn) - 1;
       const int32_t* res = ws.coef + (blk << (2 * lgn)) + ((y & m) << lgn) + (x & m);
 #pragma unroll
       for (int j = 0; j < 4; j++) v[j] += res[j];
     }
-    *reinterpret_cast<uint32_t*>(rec.y + (static_cast<uint32_t>(mby * 16 + y) * static_cast<uint32_t>(rec.pw) +
-                                          static_cast<uint32_t>(mbx * 16 + x))) = clamp_pack4(v);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(rec.y + (__umul24(static_cast<uint32_t>(mby * 16 + y), rec.pw) +
+                                                         static_cast<uint32_t>(mbx * 16 + x)));
+    if (sgpr(static_cast<uint32_t>(copy && !lres)))
+      *dst = direct;
+    else
+      *dst = clamp_pack4(v);
   }
   // ---- chroma samples
-  if (lane < 32 && cr < cph && cc < cpw && ((r.wmask >> (16 + pl)) & 1u)) {
+  if (cr < cph && cc < cpw && ((r.wmask >> (16 + pl)) & 1u)) {
     const int cfx = mvx & 7, cfy = mvy & 7;
     const uint32_t ea = __builtin_amdgcn_alignbyte(m.ca.d[1], m.ca.d[0], csh) ^ 0x80808080u;
-    const uint32_t fa = __builtin_amdgcn_alignbyte(m.ca.d[1], m.ca.d[1], csh) ^ 0x80808080u;
     const uint32_t eb = __builtin_amdgcn_alignbyte(m.cb.d[1], m.cb.d[0], csh) ^ 0x80808080u;
-    const uint32_t fb = __builtin_amdgcn_alignbyte(m.cb.d[1], m.cb.d[1], csh) ^ 0x80808080u;
-    const int32_t tx = (8 - cfx) | (cfx << 8);
-    const int32_t wy0 = 8 - cfy, wy1 = cfy;
-    int32_t u[4];
-    u[0] = (wy0 * __builtin_amdgcn_sdot4(static_cast<int32_t>(ea), tx, 0, false) +
-            wy1 * __builtin_amdgcn_sdot4(static_cast<int32_t>(eb), tx, 0, false) + (8192 + 32)) >> 6;
-#pragma unroll
-    for (int j = 1; j < 4; j++) {
-      const uint32_t wa = __builtin_amdgcn_alignbyte(fa, ea, j), wb = __builtin_amdgcn_alignbyte(fb, eb, j);
-      u[j] = (wy0 * __builtin_amdgcn_sdot4(static_cast<int32_t>(wa), tx, 0, false) +
-              wy1 * __builtin_amdgcn_sdot4(static_cast<int32_t>(wb), tx, 0, false) + (8192 + 32)) >> 6;
-    }
+    const int32_t ta = (8 - cfy) * (8 - cfx) | ((8 - cfy) * cfx) << 8, tb = cfy * (8 - cfx) | (cfy * cfx) << 8;
+    int32_t u0 = __builtin_amdgcn_sdot4(static_cast<int32_t>(eb), tb,
+                                        __builtin_amdgcn_sdot4(static_cast<int32_t>(ea), ta, 8192 + 32, false), false) >> 6;
+    int32_t u1 = __builtin_amdgcn_sdot4(static_cast<int32_t>(eb), tb << 8,
+                                        __builtin_amdgcn_sdot4(static_cast<int32_t>(ea), ta << 8, 8192 + 32, false),
+                                        false) >> 6;
     if (has_res && (r.cbp & (pl ? kCBPCr : kCBPCb))) {
       const int32_t* res = ws.coef + 256 + pl * 64 + cy * 8 + cx;
-#pragma unroll
-      for (int j = 0; j < 4; j++) u[j] += res[j];
+      u0 += res[0];
+      u1 += res[1];
     }
     uint8_t* dp = pl ? rec.cr : rec.cb;
-    *reinterpret_cast<uint32_t*>(dp + (static_cast<uint32_t>(mby * 8 + cy) * static_cast<uint32_t>(rec.cw) +
-                                       static_cast<uint32_t>(mbx * 8 + cx))) = clamp_pack4(u);
+    *reinterpret_cast<uint16_t*>(dp + (__umul24(static_cast<uint32_t>(mby * 8 + cy), rec.cw) +
+                                       static_cast<uint32_t>(mbx * 8 + cx))) = clamp_pack2(u0, u1);
   }
 }
 
@@ -2421,9 +2452,11 @@ __device__ __forceinline__ int32_t intra_pixel(const int32_t* ref, const int32_t
   const bool vert = c_ang[mode][2] != 0;
   int a = vert ? r : c;
   int b = vert ? c : r;
-  int pos = (a + 1) * c_ang[mode][0], idx = pos >> 5, fact = pos & 31;
+  // (24-bit multiplies: every operand is a sample, a weight up to 32 or a
+  // position below 17 times an angle of at most 32 in magnitude)
+  int pos = __mul24(a + 1, c_ang[mode][0]), idx = pos >> 5, fact = pos & 31;
   int32_t s0 = ext[n + b + idx + 1], s1 = ext[n + b + idx + 2];
-  return ((32 - fact) * s0 + fact * s1 + 16) >> 5;
+  return (__mul24(32 - fact, s0) + __mul24(fact, s1) + 16) >> 5;
 }
 
 // Angular main line (predict.go:162-186) for the modes that need it, and the
