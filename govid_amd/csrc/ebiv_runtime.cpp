@@ -526,7 +526,7 @@ struct StepBuf {
   int lanes = 1;
   size_t items_end = 0;
   std::vector<std::vector<size_t>> rounds;
-  std::vector<size_t> inter_off, intra_off, key_end, round_end;
+  std::vector<size_t> inter_off, intra_off, key_end, dense_off, round_end;
   std::vector<uint32_t> inter_groups_of;
   std::vector<uint64_t> inter_mbs_of;
   DevArray<TileStatus> d_status;  // every payload's tiles, contiguous: one download
@@ -1030,13 +1030,26 @@ int Engine::plan_recon(StepBuf& S) {
   auto& inter_off = S.inter_off;
   auto& intra_off = S.intra_off;
   auto& key_end = S.key_end;
+  auto& dense_off = S.dense_off;
   auto& round_end = S.round_end;
   auto& inter_groups_of = S.inter_groups_of;
   auto& inter_mbs_of = S.inter_mbs_of;
   inter_off.assign(nrl, 0);
   intra_off.assign(nrl, 0);
   key_end.assign(nrl, 0);
+  dense_off.assign(nrl, 0);
   round_end.assign(nrl, 0);
+  // With the tiles' statuses known and inter-frame tiles unbanded (a step of
+  // many tiles), an inter-frame tile coded mostly intra (a scene cut or a
+  // flash coded as an inter frame) goes to a launch of its own with the LDS
+  // frontier, shaped as a key-frame launch; the kernel takes the frontier
+  // path by the same test.  In the launch of the other tiles (no frontier)
+  // it would run anti-diagonal rounds and hold up the whole round.
+  const bool split = exact && S.band_rows_inter == 0;
+  auto dense = [&](size_t j, int t) {
+    const TileStatus& st = reinterpret_cast<const TileStatus*>(S.payload(S.jrefs[j])->h_status.h)[t];
+    return 2 * st.n_intra > st.mbs_done;
+  };
   inter_groups_of.assign(nrl, 0);
   inter_mbs_of.assign(nrl, 0);
   for (size_t r = 0; r < rounds.size(); r++)
@@ -1054,10 +1067,13 @@ int Engine::plan_recon(StepBuf& S) {
         }
       inter_groups_of[q] = prefix;
       intra_off[q] = w;
-      for (int pass = 0; pass < 2; pass++) {  // key-frame tiles, then inter-frame tiles
+      // key-frame tiles, then inter-frame tiles, then (split) the inter-frame
+      // tiles coded mostly intra
+      for (int pass = 0; pass < 3; pass++) {
         for (size_t j : rounds[r])
           if (lane_of(S.jrefs[j].si) == g && is_key(j) == (pass == 0))
             for (int t = 0; t < S.payload(S.jrefs[j])->ntiles; t++) {
+              if (pass > 0 && (split ? dense(j, t) : false) != (pass == 2)) continue;
               // a banded key-frame tile: one item per band, the same item
               // (each workgroup claims its band on the device)
               const uint8_t* hd = S.payload(S.jrefs[j])->hdesc;
@@ -1066,6 +1082,7 @@ int Engine::plan_recon(StepBuf& S) {
               for (int b = 0; b < nb; b++) S.hi[w++] = WorkItem{static_cast<uint32_t>(j), static_cast<uint32_t>(t)};
             }
         if (pass == 0) key_end[q] = w;
+        if (pass == 1) dense_off[q] = w;
       }
       round_end[q] = w;
     }
@@ -1094,6 +1111,7 @@ int Engine::launch_recon(StepBuf& S) {
   const auto& inter_off = S.inter_off;
   const auto& intra_off = S.intra_off;
   const auto& key_end = S.key_end;
+  const auto& dense_off = S.dense_off;
   const auto& round_end = S.round_end;
   const auto& inter_groups_of = S.inter_groups_of;
   const auto& inter_mbs_of = S.inter_mbs_of;
@@ -1148,12 +1166,21 @@ int Engine::launch_recon(StepBuf& S) {
         launch_recon_intra(st, static_cast<int>(n), S.dj, S.di + intra_off[q], kw, true, S.cells, S.band_rows);
         mark_end(m, st);
       }
-      if (round_end[q] > key_end[q]) {
-        const size_t n = round_end[q] - key_end[q];
+      if (dense_off[q] > key_end[q]) {
+        const size_t n = dense_off[q] - key_end[q];
         const int m = mark_begin(2, 0, st);
         // (with bands, the frontier for the tiles coded all intra)
         launch_recon_intra(st, static_cast<int>(n), S.dj, S.di + key_end[q], sched.inter_intra_waves(n),
                            S.band_rows_inter > 0, S.cells, S.band_rows_inter);
+        mark_end(m, st);
+      }
+      if (round_end[q] > dense_off[q]) {  // inter-frame tiles coded mostly intra, split off
+        // (shaped as a key-frame launch: many wavefronts per tile when few
+        // tiles, few when many, so that the tiles' LDS keeps them resident)
+        const size_t n = round_end[q] - dense_off[q];
+        const int m = mark_begin(2, 0, st);
+        launch_recon_intra(st, static_cast<int>(n), S.dj, S.di + dense_off[q], sched.key_intra_waves(n), true, S.cells,
+                           0);
         mark_end(m, st);
       }
     }

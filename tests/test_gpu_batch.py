@@ -122,6 +122,26 @@ def test_launch_schedules_do_not_change_results(schedule):
     _run_batch(streams, schedule=schedule)
 
 
+@pytest.mark.parametrize("schedule", [
+    dict(many_tiles=1),
+    dict(many_tiles=1, intra_waves_key_many=1, intra_waves_many=2),
+    dict(many_tiles=1, recon_lanes=2, intra_waves_key_many=16),
+    dict(many_tiles=64, intra_waves_key=3),
+    dict(many_tiles=1000),
+])
+def test_inter_frames_coded_intra(schedule):
+    """The parallax clip's flash at frame 14 is an inter frame whose tiles are
+    all intra.  In a step of at least many_tiles tiles such tiles get a launch
+    of their own (LDS frontier, shaped as a key-frame launch) beside the launch
+    of the round's other inter-frame tiles; with fewer tiles they run banded.
+    Sessions start at different frames, so the flash falls in different
+    rounds of a step, bit for bit against the oracle."""
+    pk = _clip("parallax", 128, 96, 28, qp=22, gop=60, tile_cols=2, tile_rows=2)
+    streams = [(pk, 3, 16), (pk[12:], 3, 8), (pk[:17], 4, 16)]
+    print(f"flash clip, schedule {schedule}", flush=True)
+    _run_batch(streams, schedule=schedule)
+
+
 CELLS_16 = dict(intra_cells=1, intra_waves_key=16, intra_waves_key_many=16, intra_waves_few=16, intra_waves_many=16)
 
 
